@@ -1,0 +1,638 @@
+// Device-side building blocks of the MI355X transform-batch path (gfx950, wave64).
+//
+// Payload layouts (SoA columns in HBM), the keyed Philox stream, the compiled-in functor set
+// and the per-tile work functions shared by the plain per-stage kernels and the device-resident
+// step engine. A tile is a workgroup of 4 wave64s over T = 256*R consecutive items of one chunk;
+// wave w owns the contiguous item range [w*64R, (w+1)*64R) of the tile so that per-wave counters
+// give the canonical stable order (bin-major, then batch input order) without a tile-wide sort.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include <type_traits>
+
+#include "../../include/graphflow.h"
+
+namespace gfd {
+
+constexpr int kWaves = 4;
+constexpr int kThreads = 64 * kWaves;
+
+// ------------------------------------------------------------------------------------------
+// RNG: Philox4x32-10 (Salmon et al., SC'11) keyed by the stream key of
+// rng/ParticleRng.hpp:57-94 (SplitMix64 finalizer over the packed (id, event, purpose)).
+// ------------------------------------------------------------------------------------------
+__host__ __device__ inline uint64_t mix64(uint64_t v)
+{
+    v ^= v >> 30;
+    v *= 0xBF58476D1CE4E5B9ull;
+    v ^= v >> 27;
+    v *= 0x94D049BB133111EBull;
+    v ^= v >> 31;
+    return v;
+}
+
+__host__ __device__ inline uint64_t stream_key(uint64_t seed, uint64_t id, uint64_t event, uint32_t purpose)
+{
+    uint64_t packed = (id & 0xFFFFFFFFull) | ((event & 0xFFFFull) << 32) | ((uint64_t)(purpose & 0xFFFFu) << 48);
+    return mix64(packed ^ mix64(seed));
+}
+
+__device__ inline void philox10(uint32_t c[4], uint32_t k0, uint32_t k1)
+{
+#pragma unroll
+    for (int r = 0; r < 10; ++r) {
+        if (r > 0) {
+            k0 += 0x9E3779B9u;
+            k1 += 0xBB67AE85u;
+        }
+        uint32_t hi0 = __umulhi(0xD2511F53u, c[0]), lo0 = 0xD2511F53u * c[0];
+        uint32_t hi1 = __umulhi(0xCD9E8D57u, c[2]), lo1 = 0xCD9E8D57u * c[2];
+        uint32_t n0 = hi1 ^ c[1] ^ k0, n2 = hi0 ^ c[3] ^ k1;
+        c[0] = n0;
+        c[1] = lo1;
+        c[2] = n2;
+        c[3] = lo0;
+    }
+}
+
+__device__ inline void stream_words(uint64_t seed, uint64_t id, uint64_t event, uint32_t purpose, uint32_t out[4])
+{
+    uint64_t k = stream_key(seed, id, event, purpose);
+    out[0] = out[1] = out[2] = out[3] = 0u;
+    philox10(out, (uint32_t)k, (uint32_t)(k >> 32));
+}
+
+__host__ __device__ inline uint32_t fmix32(uint32_t h)
+{
+    h ^= h >> 16;
+    h *= 0x85EBCA6Bu;
+    h ^= h >> 13;
+    h *= 0xC2B2AE35u;
+    h ^= h >> 16;
+    return h;
+}
+
+__device__ inline float uniform_oc(uint32_t w) { return (float)((w >> 8) + 1u) * (1.0f / 16777216.0f); }
+__device__ inline float uniform_co(uint32_t w) { return (float)(w >> 8) * (1.0f / 16777216.0f); }
+__device__ inline double uniform_f64(uint32_t w0, uint32_t w1)
+{
+    return ((double)(w0 >> 5) * 67108864.0 + (double)(w1 >> 6)) * (1.0 / 9007199254740992.0);
+}
+
+// ------------------------------------------------------------------------------------------
+// Payload layouts. Column i of the pool is cols.c[i]; slot s lives at index s of every column.
+// ------------------------------------------------------------------------------------------
+struct Cols {
+    void* c[9];
+};
+
+struct Particle {
+    float w;
+    uint32_t id, gen;
+    __device__ static Particle load(const Cols& k, size_t s)
+    {
+        Particle p;
+        p.w = ((const float*)k.c[0])[s];
+        p.id = ((const uint32_t*)k.c[1])[s];
+        p.gen = ((const uint32_t*)k.c[2])[s];
+        return p;
+    }
+    __device__ void store(const Cols& k, size_t s) const
+    {
+        ((float*)k.c[0])[s] = w;
+        ((uint32_t*)k.c[1])[s] = id;
+        ((uint32_t*)k.c[2])[s] = gen;
+    }
+};
+
+struct Keyed {
+    uint32_t key;
+    uint64_t val;
+    __device__ static Keyed load(const Cols& k, size_t s)
+    {
+        Keyed p;
+        p.key = ((const uint32_t*)k.c[0])[s];
+        p.val = ((const uint64_t*)k.c[1])[s];
+        return p;
+    }
+    __device__ void store(const Cols& k, size_t s) const
+    {
+        ((uint32_t*)k.c[0])[s] = key;
+        ((uint64_t*)k.c[1])[s] = val;
+    }
+};
+
+struct Photon {
+    float x, y, z, ux, uy, uz, w;
+    uint32_t id, step;
+    __device__ static Photon load(const Cols& k, size_t s)
+    {
+        Photon p;
+        p.x = ((const float*)k.c[0])[s];
+        p.y = ((const float*)k.c[1])[s];
+        p.z = ((const float*)k.c[2])[s];
+        p.ux = ((const float*)k.c[3])[s];
+        p.uy = ((const float*)k.c[4])[s];
+        p.uz = ((const float*)k.c[5])[s];
+        p.w = ((const float*)k.c[6])[s];
+        p.id = ((const uint32_t*)k.c[7])[s];
+        p.step = ((const uint32_t*)k.c[8])[s];
+        return p;
+    }
+    __device__ void store(const Cols& k, size_t s) const
+    {
+        ((float*)k.c[0])[s] = x;
+        ((float*)k.c[1])[s] = y;
+        ((float*)k.c[2])[s] = z;
+        ((float*)k.c[3])[s] = ux;
+        ((float*)k.c[4])[s] = uy;
+        ((float*)k.c[5])[s] = uz;
+        ((float*)k.c[6])[s] = w;
+        ((uint32_t*)k.c[7])[s] = id;
+        ((uint32_t*)k.c[8])[s] = step;
+    }
+};
+
+// ------------------------------------------------------------------------------------------
+// Node, batch and run parameters as the kernels see them.
+// ------------------------------------------------------------------------------------------
+struct PhotonParams {
+    float L, mu_t, albedo, g, roulette_w, roulette_p;
+    uint32_t tally_bins;
+};
+
+struct RunParams {
+    uint64_t seed;
+    uint32_t C;
+    const double* zipf;
+    uint32_t zipf_values;
+    PhotonParams ph;
+};
+
+// One node's current batch. The plain stages and the step engine fill the same structure.
+struct Batch {
+    uint32_t active, n_in, n_tiles, n_out;
+    uint32_t particles, reserved, tile_items, purpose;
+    uint64_t first_id;
+    uint64_t digest_base;
+    gf_chunk_desc* in;     // cap_in
+    uint32_t* tile_prefix; // cap_in + 1: tiles before input chunk c
+    uint32_t* item_prefix; // cap_in + 1: items before input chunk c
+    gf_chunk_desc* out;    // cap_out
+    uint32_t* counts;      // [tile][bin] counts, after planning the flat output start of (tile, bin)
+    uint32_t* totals;      // [bins]
+};
+
+__device__ inline void set_status(uint32_t* status, uint32_t code) { atomicCAS(status, 0u, code); }
+
+__device__ inline uint32_t route_of(uint32_t dist, uint32_t ndest, uint32_t h)
+{
+    switch (dist) {
+    case GF_ROUTE_PARITY: return h & 1u;
+    case GF_ROUTE_SKEW_90_10: return (h % 10u == 0u) ? 1u : 0u;
+    case GF_ROUTE_ALL_ONE: return 0u;
+    default: return h % ndest;
+    }
+}
+
+// Classifier functors (pure functions of the payload, Concepts.hpp:134-145).
+__device__ inline uint32_t classify(const gf_node_desc& nd, const Particle& p)
+{
+    switch (nd.op) {
+    case GF_CLS_WEIGHT_ROUTE:
+        if (p.w < nd.f[0]) return 0u;
+        return 1u + route_of(nd.u[0], nd.bins - 1u, fmix32(p.id));
+    case GF_CLS_ID_MOD:
+        if (nd.u[0] == 1u) return 0u;
+        if (nd.u[0] == 2u) return (p.id % 8u == 0u) ? p.id % nd.bins : 0u;
+        return p.id % nd.bins;
+    case GF_CLS_LUT: {
+        uint32_t a = p.id < nd.u[0] ? p.id : nd.u[0] - 1u;
+        uint32_t g = p.gen < nd.u[1] ? p.gen : nd.u[1] - 1u;
+        return ((const uint32_t*)nd.ptr[0])[nd.u[3] + a * nd.u[1] + g];
+    }
+    default: return 0xFFFFFFFFu;
+    }
+}
+__device__ inline uint32_t classify(const gf_node_desc& nd, const Keyed& p)
+{
+    if (nd.op == GF_CLS_KEY_BUCKET) return fmix32(p.key) % nd.bins;
+    return 0xFFFFFFFFu;
+}
+__device__ inline uint32_t classify(const gf_node_desc& nd, const Photon& p)
+{
+    if (nd.op == GF_CLS_PHOTON_ALIVE) return p.w > 0.0f ? 0u : 1u;
+    return 0xFFFFFFFFu;
+}
+
+// Partition updaters (UpdateResult, Concepts.hpp:33-39): 0 success, GF_INVALID, GF_OVERFLOW.
+__device__ inline uint32_t update(const gf_node_desc& nd, uint32_t bin, Particle& p)
+{
+    if (nd.upd == GF_UPD_ADD_ON_BIN) {
+        if (bin == nd.u[2]) {
+            float w = p.w + nd.f[1];
+            if (isinf(w)) return GF_OVERFLOW;
+            p.w = w;
+            p.gen += 1u;
+        }
+    } else if (nd.upd == GF_UPD_GEN_TO_BIN) {
+        p.gen = bin;
+    }
+    return 0u;
+}
+__device__ inline uint32_t update(const gf_node_desc&, uint32_t, Keyed&) { return 0u; }
+__device__ inline uint32_t update(const gf_node_desc&, uint32_t, Photon&) { return 0u; }
+
+// One photon flight + interaction + roulette + Henyey-Greenstein scatter (fp32), tallying the
+// absorbed weight into depth bins.
+__device__ inline void photon_step(const RunParams& rp, uint32_t purpose, Photon& p, double* tally)
+{
+    if (!(p.w > 0.0f)) return;
+    uint32_t r[4];
+    stream_words(rp.seed, p.id, p.step, purpose, r);
+    const PhotonParams& c = rp.ph;
+    float s = -logf(uniform_oc(r[0])) / c.mu_t;
+    float nx = p.x + s * p.ux, ny = p.y + s * p.uy, nz = p.z + s * p.uz;
+    p.step += 1u;
+    if (nz < 0.0f || nz > c.L) {
+        p.w = 0.0f;
+        p.z = nz;
+        return;
+    }
+    p.x = nx;
+    p.y = ny;
+    p.z = nz;
+    uint32_t bin = (uint32_t)(nz / c.L * (float)c.tally_bins);
+    if (bin >= c.tally_bins) bin = c.tally_bins - 1u;
+    atomicAdd(&tally[bin], (double)(p.w * (1.0f - c.albedo)));
+    p.w = p.w * c.albedo;
+    if (p.w < c.roulette_w) {
+        if (uniform_co(r[3]) < c.roulette_p) {
+            p.w = p.w / c.roulette_p;
+        } else {
+            p.w = 0.0f;
+            return;
+        }
+    }
+    float g = c.g;
+    float xi = uniform_co(r[1]);
+    float tmp = (1.0f - g * g) / (1.0f - g + 2.0f * g * xi);
+    float ct = (1.0f + g * g - tmp * tmp) / (2.0f * g);
+    ct = fminf(1.0f, fmaxf(-1.0f, ct));
+    float st = sqrtf(1.0f - ct * ct);
+    float phi = 6.2831853071795864f * uniform_co(r[2]);
+    float cp = cosf(phi), sp = sinf(phi);
+    float ux = p.ux, uy = p.uy, uz = p.uz;
+    if (fabsf(uz) > 0.99999f) {
+        p.ux = st * cp;
+        p.uy = st * sp;
+        p.uz = uz > 0.0f ? ct : -ct;
+    } else {
+        float t = sqrtf(1.0f - uz * uz);
+        p.ux = st * (ux * uz * cp - uy * sp) / t + ux * ct;
+        p.uy = st * (uy * uz * cp + ux * sp) / t + uy * ct;
+        p.uz = -st * cp * t + uz * ct;
+    }
+}
+
+// Plain transform functors.
+__device__ inline void transform(const gf_node_desc& nd, const RunParams&, uint32_t, Particle& p)
+{
+    if (nd.op == GF_XF_SCALE_WEIGHT) {
+        p.w = p.w * nd.f[0];
+        p.gen += 1u;
+    }
+}
+__device__ inline void transform(const gf_node_desc&, const RunParams&, uint32_t, Keyed&) {}
+__device__ inline void transform(const gf_node_desc& nd, const RunParams& rp, uint32_t purpose, Photon& p)
+{
+    if (nd.op == GF_XF_PHOTON_STEP) photon_step(rp, purpose, p, (double*)nd.ptr[0]);
+}
+
+// Source functors.
+__device__ inline void source(const gf_node_desc&, const RunParams& rp, uint32_t purpose, uint64_t id, Particle& p)
+{
+    uint32_t r[4];
+    stream_words(rp.seed, id, 0, purpose, r);
+    p.w = uniform_oc(r[0]);
+    p.id = (uint32_t)id;
+    p.gen = 0u;
+}
+__device__ inline void source(const gf_node_desc&, const RunParams& rp, uint32_t purpose, uint64_t id, Keyed& p)
+{
+    uint32_t r[4];
+    stream_words(rp.seed, id, 0, purpose, r);
+    double target = uniform_f64(r[0], r[1]) * rp.zipf[rp.zipf_values - 1u];
+    uint32_t lo = 0, hi = rp.zipf_values - 1u;
+    while (lo < hi) {
+        uint32_t mid = lo + (hi - lo) / 2u;
+        if (rp.zipf[mid] > target) hi = mid;
+        else lo = mid + 1u;
+    }
+    p.key = lo;
+    p.val = id;
+}
+__device__ inline void source(const gf_node_desc&, const RunParams&, uint32_t, uint64_t id, Photon& p)
+{
+    p.x = p.y = p.z = 0.0f;
+    p.ux = p.uy = 0.0f;
+    p.uz = 1.0f;
+    p.w = 1.0f;
+    p.id = (uint32_t)id;
+    p.step = 0u;
+}
+
+__device__ inline uint64_t keyed_hash(const Keyed& p)
+{
+    return mix64(((uint64_t)p.key << 32) ^ mix64(p.val + 0x9E3779B97F4A7C15ull));
+}
+
+// ------------------------------------------------------------------------------------------
+// Wave helpers
+// ------------------------------------------------------------------------------------------
+__device__ inline uint32_t lane_id() { return threadIdx.x & 63u; }
+__device__ inline uint64_t lanemask_lt() { return (1ull << lane_id()) - 1ull; }
+
+// Lanes with the same bin (the "peers" of each lane); iterations = distinct bins in the wave.
+__device__ inline uint64_t peer_mask(bool valid, uint32_t bin)
+{
+    uint64_t active = __ballot(valid);
+    uint64_t peers = 0;
+    while (active) {
+        int leader = __ffsll((unsigned long long)active) - 1;
+        uint32_t lb = (uint32_t)__shfl((int)bin, leader);
+        uint64_t m = __ballot(valid && bin == lb);
+        if (valid && bin == lb) peers = m;
+        active &= ~m;
+    }
+    return peers;
+}
+
+// Chunk of tile t: largest c with prefix[c] <= t (prefix has n+1 entries).
+__device__ inline uint32_t find_chunk(const uint32_t* prefix, uint32_t n, uint32_t t)
+{
+    uint32_t lo = 0, hi = n;  // invariant: prefix[lo] <= t < prefix[hi]
+    while (hi - lo > 1u) {
+        uint32_t mid = (lo + hi) >> 1;
+        if (prefix[mid] <= t) lo = mid;
+        else hi = mid;
+    }
+    return lo;
+}
+
+struct TileSpan {
+    uint32_t chunk;  // position in the batch's input list
+    uint32_t offset; // first item of the tile inside the chunk
+    uint32_t count;
+    size_t base;     // first slot
+};
+
+__device__ inline TileSpan tile_span(const Batch& b, const gf_chunk_desc* list, uint32_t n, uint32_t t, uint32_t C)
+{
+    TileSpan s;
+    s.chunk = find_chunk(b.tile_prefix, n, t);
+    gf_chunk_desc d = list[s.chunk];
+    s.offset = (t - b.tile_prefix[s.chunk]) * b.tile_items;
+    uint32_t rem = d.fill - s.offset;
+    s.count = rem < b.tile_items ? rem : b.tile_items;
+    s.base = (size_t)d.index * C + s.offset;
+    return s;
+}
+
+// ------------------------------------------------------------------------------------------
+// Tile functions
+// ------------------------------------------------------------------------------------------
+
+// Partition classify of one tile: per-tile bin counts (Execution.hpp:869-954 semantics; the
+// tile-local histogram lives in LDS and is fed by wave-aggregated increments).
+template <class P, int R>
+__device__ void count_tile(const gf_node_desc& nd, const Batch& b, uint32_t t, const Cols& cols, uint32_t C,
+                           uint32_t* status, char* smem)
+{
+    const uint32_t B = nd.bins, tid = threadIdx.x, w = tid >> 6, lane = lane_id();
+    uint32_t* tcnt = (uint32_t*)smem;
+    for (uint32_t i = tid; i < B; i += kThreads) tcnt[i] = 0u;
+    __syncthreads();
+    TileSpan s = tile_span(b, b.in, b.n_in, t, C);
+#pragma unroll
+    for (int r = 0; r < R; ++r) {
+        uint32_t i = w * 64u * R + (uint32_t)r * 64u + lane;
+        bool v = i < s.count;
+        uint32_t bin = 0u;
+        if (v) {
+            P p = P::load(cols, s.base + i);
+            bin = classify(nd, p);
+            if (bin >= B) {
+                set_status(status, GF_INVALID);
+                v = false;
+            }
+        }
+        uint64_t peers = peer_mask(v, bin);
+        if (v && (peers & lanemask_lt()) == 0ull) atomicAdd(&tcnt[bin], (uint32_t)__popcll(peers));
+    }
+    __syncthreads();
+    uint32_t* dst = b.counts + (size_t)t * B;
+    for (uint32_t i = tid; i < B; i += kThreads) dst[i] = tcnt[i];
+    __syncthreads();
+}
+
+// LDS bytes of scatter_tile.
+__host__ __device__ inline size_t scatter_smem(uint32_t bins, uint32_t tile_items, uint32_t rec_bytes, bool stage)
+{
+    size_t s = (size_t)(kWaves + 2) * bins * 4u;
+    if (stage) s += (size_t)tile_items * (rec_bytes + 4u);
+    return s;
+}
+
+// Stable scatter of one tile (Execution.hpp:1096-1187 / :1327-1502 semantics with the canonical
+// order): the records are held in registers between the ranking and the write; with staging the
+// tile is reordered in LDS so consecutive lanes write consecutive output slots of one bin run.
+// b.counts[t*B + bin] holds the flat output position of the tile's first item of `bin`; flat
+// position f lives at slot out[f / C].index * C + f % C.
+template <class P, int R, bool Stage>
+__device__ void scatter_tile(const gf_node_desc& nd, const Batch& b, uint32_t t, const Cols& cols, uint32_t C,
+                             uint32_t* status, char* smem)
+{
+    const uint32_t B = nd.bins, tid = threadIdx.x, w = tid >> 6, lane = lane_id();
+    uint32_t* wcnt = (uint32_t*)smem;      // [kWaves][B] running, then exclusive wave prefix
+    uint32_t* ttot = wcnt + kWaves * B;     // [B] tile totals
+    uint32_t* tstart = ttot + B;            // [B] exclusive prefix of ttot over bins
+    P* stage = (P*)(tstart + B);            // [T]
+    uint32_t* flat = (uint32_t*)(stage + (Stage ? b.tile_items : 0u));
+    for (uint32_t i = tid; i < kWaves * B; i += kThreads) wcnt[i] = 0u;
+    __syncthreads();
+    TileSpan s = tile_span(b, b.in, b.n_in, t, C);
+    P rec[R];
+    uint32_t bin[R], rank[R];
+#pragma unroll
+    for (int r = 0; r < R; ++r) {
+        uint32_t i = w * 64u * R + (uint32_t)r * 64u + lane;
+        bool v = i < s.count;
+        bin[r] = 0xFFFFFFFFu;
+        if (v) {
+            rec[r] = P::load(cols, s.base + i);
+            bin[r] = classify(nd, rec[r]);
+            if (bin[r] >= B) {
+                set_status(status, GF_INVALID);
+                v = false;
+                bin[r] = 0xFFFFFFFFu;
+            }
+        }
+        uint64_t peers = peer_mask(v, bin[r]);
+        uint32_t cur = v ? wcnt[w * B + bin[r]] : 0u;
+        rank[r] = cur + (uint32_t)__popcll(peers & lanemask_lt());
+        // every lane of the wave has read `cur` before a leader stores the new running count
+        if (v && (peers & lanemask_lt()) == 0ull) wcnt[w * B + bin[r]] = cur + (uint32_t)__popcll(peers);
+    }
+    __syncthreads();
+    for (uint32_t k = tid; k < B; k += kThreads) {
+        uint32_t acc = 0;
+        for (int ww = 0; ww < kWaves; ++ww) {
+            uint32_t x = wcnt[ww * B + k];
+            wcnt[ww * B + k] = acc;
+            acc += x;
+        }
+        ttot[k] = acc;
+    }
+    __syncthreads();
+    const uint32_t* toff = b.counts + (size_t)t * B;
+    if constexpr (Stage) {
+        if (tid == 0) {
+            uint32_t acc = 0;
+            for (uint32_t k = 0; k < B; ++k) {
+                tstart[k] = acc;
+                acc += ttot[k];
+            }
+        }
+        __syncthreads();
+#pragma unroll
+        for (int r = 0; r < R; ++r) {
+            if (bin[r] == 0xFFFFFFFFu) continue;
+            uint32_t k = bin[r], within = wcnt[w * B + k] + rank[r];
+            uint32_t u = update(nd, k, rec[r]);
+            if (u) set_status(status, u);
+            uint32_t p = tstart[k] + within;
+            stage[p] = rec[r];
+            flat[p] = toff[k] + within;
+        }
+        __syncthreads();
+        for (uint32_t j = tid; j < s.count; j += kThreads) {
+            uint32_t f = flat[j];
+            size_t slot = (size_t)b.out[f / C].index * C + f % C;
+            stage[j].store(cols, slot);
+        }
+        __syncthreads();
+    } else {
+#pragma unroll
+        for (int r = 0; r < R; ++r) {
+            if (bin[r] == 0xFFFFFFFFu) continue;
+            uint32_t k = bin[r];
+            uint32_t f = toff[k] + wcnt[w * B + k] + rank[r];
+            P p = rec[r];
+            uint32_t u = update(nd, k, p);
+            if (u) set_status(status, u);
+            p.store(cols, (size_t)b.out[f / C].index * C + f % C);
+        }
+        __syncthreads();
+    }
+}
+
+// Out-of-place transform of one tile (TransformKernel, Execution.hpp:708-776): dense
+// concatenation, global position = items before the chunk + offset.
+template <class P, int R>
+__device__ void transform_tile(const gf_node_desc& nd, const Batch& b, uint32_t t, const Cols& cols, uint32_t C,
+                               const RunParams& rp)
+{
+    TileSpan s = tile_span(b, b.in, b.n_in, t, C);
+    const uint32_t pos0 = b.item_prefix[s.chunk] + s.offset;
+    for (uint32_t i = threadIdx.x; i < s.count; i += kThreads) {
+        P p = P::load(cols, s.base + i);
+        transform(nd, rp, b.purpose, p);
+        uint32_t g = pos0 + i;
+        p.store(cols, (size_t)b.out[g / C].index * C + g % C);
+    }
+}
+
+// Source of one tile of the admitted chunk (SourceKernel, Execution.hpp:664-692).
+template <class P>
+__device__ void source_tile(const gf_node_desc& nd, const Batch& b, uint32_t t, const Cols& cols, uint32_t C,
+                            const RunParams& rp)
+{
+    uint32_t off = t * b.tile_items;
+    uint32_t count = b.particles - off < b.tile_items ? b.particles - off : b.tile_items;
+    size_t base = (size_t)b.out[0].index * C + off;
+    for (uint32_t i = threadIdx.x; i < count; i += kThreads) {
+        P p;
+        source(nd, rp, b.purpose, b.first_id + off + i, p);
+        p.store(cols, base + i);
+    }
+}
+
+// Sink of one tile of one input chunk (SinkKernel, Execution.hpp:1508-1534).
+template <class P>
+__device__ void sink_tile(const gf_node_desc& nd, const Batch& b, uint32_t t, const Cols& cols, uint32_t C,
+                          uint32_t* status, char* smem)
+{
+    TileSpan s = tile_span(b, b.in, b.n_in, t, C);
+    if constexpr (std::is_same_v<P, Particle>) {
+        for (uint32_t i = threadIdx.x; i < s.count; i += kThreads) {
+            Particle p = Particle::load(cols, s.base + i);
+            if (nd.op == GF_SNK_HIST) {
+                uint32_t v = nd.u[0] == 0u ? __float_as_uint(p.w) : p.gen;
+                uint32_t rel = v - nd.u[1];
+                if (v < nd.u[1] || rel >= nd.u[2]) set_status(status, GF_INVALID);
+                else atomicAdd((unsigned long long*)nd.ptr[0] + rel, 1ull);
+            } else if (nd.op == GF_SNK_RECORD_WEIGHT) {
+                if ((uint64_t)p.id < nd.size[0]) ((uint32_t*)nd.ptr[0])[p.id] = __float_as_uint(p.w);
+                else set_status(status, GF_INVALID);
+            }
+        }
+    } else if constexpr (std::is_same_v<P, Keyed>) {
+        if (nd.op != GF_SNK_CHUNK_DIGEST) return;
+        const gf_chunk_desc d = b.in[s.chunk];
+        uint64_t acc = 0, msum = 0;
+        for (uint32_t i = threadIdx.x; i < s.count; i += kThreads) {
+            Keyed p = Keyed::load(cols, s.base + i);
+            uint64_t h = keyed_hash(p);
+            acc += mix64(h ^ ((uint64_t)(s.offset + i + 1u) * 0xD6E8FEB86659FD93ull));
+            msum += h;
+        }
+        // workgroup reduction of the two sums
+        uint64_t* red = (uint64_t*)smem;
+        for (int o = 32; o > 0; o >>= 1) {
+            acc += __shfl_down(acc, o);
+            msum += __shfl_down(msum, o);
+        }
+        if (lane_id() == 0) {
+            red[2 * (threadIdx.x >> 6)] = acc;
+            red[2 * (threadIdx.x >> 6) + 1] = msum;
+        }
+        __syncthreads();
+        if (threadIdx.x == 0) {
+            uint64_t a = 0, m = 0;
+            for (int ww = 0; ww < kWaves; ++ww) {
+                a += red[2 * ww];
+                m += red[2 * ww + 1];
+            }
+            uint64_t entry = b.digest_base + s.chunk;
+            if (nd.ptr[0] && entry < nd.size[0]) {
+                uint32_t* log = (uint32_t*)nd.ptr[0] + 4 * entry;
+                if (s.offset == 0) {
+                    log[0] = d.bin;
+                    log[1] = d.fill;
+                }
+                atomicAdd((unsigned long long*)(log + 2), (unsigned long long)a);
+            }
+            if (nd.ptr[1] && d.bin < nd.size[1]) {
+                unsigned long long* ms = (unsigned long long*)nd.ptr[1] + 2 * d.bin;
+                atomicAdd(ms, (unsigned long long)m);
+                atomicAdd(ms + 1, (unsigned long long)s.count);
+            }
+        }
+        __syncthreads();
+    }
+}
+
+}  // namespace gfd

@@ -1,0 +1,1730 @@
+// libgraphflow_amd: MI355X transform-batch path behind the graphflow.h C ABI.
+//
+// Plain form  (GF_SCHEDULE_HOST):   a host GraphRunner (sched/GraphRunner.hpp:716-1257 with every
+//   batch completed before the next loop iteration) drives per-node stages; a partition batch is
+//   classify kernel -> host plan (DenseBinTable.hpp:39-91) -> stable scatter kernel, like the
+//   reference's two host round trips per batch.
+// Batched form (GF_SCHEDULE_DEVICE): the device-resident step engine of gf_engine.hpp replayed
+//   from a hipGraph; the host never plans.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <chrono>
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <deque>
+#include <memory>
+#include <string>
+#include <type_traits>
+#include <vector>
+
+#include "gf_engine.hpp"
+
+using namespace gfd;
+
+namespace {
+
+struct HipError {
+    hipError_t e;
+    const char* what;
+};
+
+#define HIPX(call)                                                                                 \
+    do {                                                                                           \
+        hipError_t _e = (call);                                                                    \
+        if (_e != hipSuccess) throw HipError{_e, #call};                                           \
+    } while (0)
+
+struct GfError {
+    gf_status s;
+    std::string msg;
+};
+
+[[noreturn]] void fail(gf_status s, const std::string& m) { throw GfError{s, m}; }
+
+template <class F>
+gf_status guard(std::string* err, F&& f)
+{
+    try {
+        f();
+        return GF_OK;
+    } catch (const GfError& g) {
+        if (err) *err = g.msg;
+        return g.s;
+    } catch (const HipError& h) {
+        if (err) *err = std::string(h.what) + ": " + hipGetErrorString(h.e);
+        return GF_DEVICE_ERROR;
+    } catch (const std::bad_alloc&) {
+        if (err) *err = "host allocation failed";
+        return GF_BAD_ARGUMENT;
+    }
+}
+
+// ---- closed-form host math (Reservation.hpp:15-71, DenseBinTable.hpp:39-91) ----------------
+uint64_t ceil_div(uint64_t n, uint64_t d) { return n / d + (n % d != 0u ? 1u : 0u); }
+
+uint64_t output_chunk_bound(uint64_t n, uint64_t C, uint64_t B)
+{
+    if (n == 0u) return 0u;
+    if (C == 0u || B == 0u) fail(GF_BAD_ARGUMENT, "outputChunkBound: capacity and maxBins must be non-zero");
+    uint64_t per = ceil_div(n, C), bins = B < n ? B : n;
+    if (per > UINT64_MAX - bins + 1u) fail(GF_OVERFLOW, "outputChunkBound: chunk count overflow");
+    uint64_t r = per + bins - 1u;
+    return r < n ? r : n;
+}
+
+bool is_partition(const gf_node_desc& nd) { return nd.op >= GF_CLS_WEIGHT_ROUTE && nd.op < GF_XF_SCALE_WEIGHT; }
+
+uint32_t reserve_bound_of(const gf_node_desc& nd, uint32_t C, uint32_t Q)
+{
+    uint64_t b = output_chunk_bound((uint64_t)C * Q, C, is_partition(nd) ? nd.bins : 1u);
+    if (b > UINT32_MAX) fail(GF_OVERFLOW, "partition batch output-chunk bound overflow");
+    return (uint32_t)b;
+}
+
+uint32_t payload_bytes(uint32_t payload)
+{
+    return payload == GF_PAYLOAD_PARTICLE ? 12u : payload == GF_PAYLOAD_KEYED ? 12u : 36u;
+}
+
+std::vector<uint32_t> column_bytes(uint32_t payload)
+{
+    if (payload == GF_PAYLOAD_PARTICLE) return {4, 4, 4};
+    if (payload == GF_PAYLOAD_KEYED) return {4, 8};
+    return std::vector<uint32_t>(9, 4);
+}
+
+}  // namespace
+
+// ==============================================================================================
+// Kernels
+// ==============================================================================================
+template <class P, int R>
+__global__ __launch_bounds__(kThreads) void k_count(gf_node_desc nd, const Batch* bp, Cols cols, uint32_t C,
+                                                    uint32_t* status)
+{
+    extern __shared__ char smem[];
+    const Batch b = *bp;
+    count_tile<P, R>(nd, b, blockIdx.x, cols, C, status, smem);
+}
+
+template <class P, int R, bool Stage>
+__global__ __launch_bounds__(kThreads) void k_scatter(gf_node_desc nd, const Batch* bp, Cols cols, uint32_t C,
+                                                      uint32_t* status)
+{
+    extern __shared__ char smem[];
+    const Batch b = *bp;
+    scatter_tile<P, R, Stage>(nd, b, blockIdx.x, cols, C, status, smem);
+}
+
+template <class P, int R>
+__global__ __launch_bounds__(kThreads) void k_transform(gf_node_desc nd, const Batch* bp, Cols cols, uint32_t C,
+                                                        RunParams rp)
+{
+    const Batch b = *bp;
+    transform_tile<P, R>(nd, b, blockIdx.x, cols, C, rp);
+}
+
+template <class P>
+__global__ __launch_bounds__(kThreads) void k_source(gf_node_desc nd, const Batch* bp, Cols cols, uint32_t C,
+                                                     RunParams rp)
+{
+    const Batch b = *bp;
+    source_tile<P>(nd, b, blockIdx.x, cols, C, rp);
+}
+
+template <class P>
+__global__ __launch_bounds__(kThreads) void k_sink(gf_node_desc nd, const Batch* bp, Cols cols, uint32_t C,
+                                                   uint32_t* status)
+{
+    extern __shared__ char smem[];
+    const Batch b = *bp;
+    sink_tile<P>(nd, b, blockIdx.x, cols, C, status, smem);
+}
+
+// Tile descriptors of a batch as the device computes them (TileDescriptor list).
+__global__ void k_tiles(const Batch* bp, uint32_t C, gf_tile_desc* out)
+{
+    const Batch b = *bp;
+    for (uint32_t t = blockIdx.x * blockDim.x + threadIdx.x; t < b.n_tiles; t += gridDim.x * blockDim.x) {
+        TileSpan s = tile_span(b, b.in, b.n_in, t, C);
+        out[t] = gf_tile_desc{b.in[s.chunk].index, s.offset, s.count};
+    }
+}
+
+// ---- step engine ------------------------------------------------------------------------------
+__device__ inline bool engine_stopped(const Engine& e, uint32_t* flag)
+{
+    if (threadIdx.x == 0) *flag = (load_ctl(&e.ctl[CTL_DONE]) | load_ctl(&e.ctl[CTL_STATUS])) != 0u ? 1u : 0u;
+    __syncthreads();
+    return *flag != 0u;
+}
+
+template <class P, int R>
+__global__ __launch_bounds__(kThreads) void k_engine_ka(Engine e)
+{
+    extern __shared__ char smem[];
+    __shared__ uint32_t flag;
+    __shared__ PlanShared ps;
+    if (engine_stopped(e, &flag)) return;
+    const uint32_t total = e.ka_prefix[e.n_nodes];
+    for (uint32_t t = blockIdx.x; t < total; t += gridDim.x) {
+        uint32_t k = find_node(e.ka_prefix, e.n_nodes, t);
+        const Batch b = e.batches[k];
+        count_tile<P, R>(e.nodes[k], b, t - e.ka_prefix[k], e.cols, e.C, &e.ctl[CTL_STATUS], smem);
+    }
+    if (last_block(&e.ctl[CTL_TICKET_A], &flag)) {
+        engine_plan(e, ps);
+        if (threadIdx.x == 0) e.ctl[CTL_TICKET_A] = 0u;
+    }
+}
+
+template <class P, int R>
+__global__ __launch_bounds__(kThreads) void k_engine_kb(Engine e, uint32_t stage_mask)
+{
+    extern __shared__ char smem[];
+    __shared__ uint32_t flag;
+    __shared__ SelectShared ss;
+    if (engine_stopped(e, &flag)) return;
+    const uint32_t total = e.kb_prefix[e.n_nodes];
+    for (uint32_t t = blockIdx.x; t < total; t += gridDim.x) {
+        uint32_t k = find_node(e.kb_prefix, e.n_nodes, t);
+        const gf_node_desc& nd = e.nodes[k];
+        const Batch b = e.batches[k];
+        const uint32_t lt = t - e.kb_prefix[k];
+        if (nd.role == GF_ROLE_SOURCE) {
+            source_tile<P>(nd, b, lt, e.cols, e.C, e.rp);
+        } else if (nd.role == GF_ROLE_SINK) {
+            sink_tile<P>(nd, b, lt, e.cols, e.C, &e.ctl[CTL_STATUS], smem);
+        } else if (is_partition_op(nd.op)) {
+            if ((stage_mask >> (k & 31u)) & 1u)
+                scatter_tile<P, R, true>(nd, b, lt, e.cols, e.C, &e.ctl[CTL_STATUS], smem);
+            else
+                scatter_tile<P, R, false>(nd, b, lt, e.cols, e.C, &e.ctl[CTL_STATUS], smem);
+        } else {
+            transform_tile<P, R>(nd, b, lt, e.cols, e.C, e.rp);
+        }
+    }
+    if (last_block(&e.ctl[CTL_TICKET_B], &flag)) {
+        engine_retire(e);
+        engine_select(e, ss);
+        if (threadIdx.x == 0) e.ctl[CTL_TICKET_B] = 0u;
+    }
+}
+
+__global__ __launch_bounds__(kThreads) void k_engine_setup(Engine e)
+{
+    __shared__ SelectShared ss;
+    const uint32_t tid = threadIdx.x;
+    for (uint32_t i = tid; i < e.S; i += kThreads) {
+        e.free_stack[i] = e.S - 1u - i;  // index 0 is handed out first (ChunkLedger.hpp:43-46)
+        e.gens[i] = 0u;
+    }
+    for (uint32_t k = tid; k < e.n_nodes; k += kThreads) {
+        e.ring_head[k] = 0ull;
+        e.ring_tail[k] = 0ull;
+        e.batches[k].active = 0u;
+        e.batches[k].n_in = 0u;
+        e.batches[k].n_out = 0u;
+    }
+    for (uint32_t i = tid; i < CNT_WORDS; i += kThreads) e.ctr[i] = 0ull;
+    for (uint32_t i = tid; i < CTL_WORDS; i += kThreads) e.ctl[i] = 0u;
+    __syncthreads();
+    if (tid == 0) e.ctl[CTL_TOP] = e.S;
+    __syncthreads();
+    engine_select(e, ss);
+}
+
+// ---- dense histogram -> rows ------------------------------------------------------------------
+__global__ void k_nz_count(const unsigned long long* c, uint64_t n, uint64_t seg, uint32_t* block_nz)
+{
+    __shared__ uint32_t red[kWaves];
+    uint64_t lo = (uint64_t)blockIdx.x * seg, hi = lo + seg < n ? lo + seg : n;
+    uint32_t cnt = 0;
+    for (uint64_t i = lo + threadIdx.x; i < hi; i += kThreads) cnt += c[i] != 0ull;
+    uint32_t tot;
+    block_exscan(cnt, red, &tot);
+    if (threadIdx.x == 0) block_nz[blockIdx.x] = tot;
+}
+
+__global__ void k_nz_write(const unsigned long long* c, uint64_t n, uint64_t seg, const uint32_t* block_off,
+                           uint32_t lo_value, unsigned long long* rows)
+{
+    __shared__ uint32_t red[kWaves];
+    __shared__ uint32_t run;
+    uint64_t lo = (uint64_t)blockIdx.x * seg, hi = lo + seg < n ? lo + seg : n;
+    if (threadIdx.x == 0) run = block_off[blockIdx.x];
+    __syncthreads();
+    for (uint64_t i0 = lo; i0 < hi; i0 += kThreads) {
+        uint64_t i = i0 + threadIdx.x;
+        unsigned long long v = i < hi ? c[i] : 0ull;
+        uint32_t tot;
+        uint32_t pos = block_exscan(v != 0ull ? 1u : 0u, red, &tot);
+        if (v) {
+            rows[2ull * (run + pos)] = (unsigned long long)lo_value + i;
+            rows[2ull * (run + pos) + 1] = v;
+        }
+        __syncthreads();
+        if (threadIdx.x == 0) run += tot;
+        __syncthreads();
+    }
+}
+
+// ==============================================================================================
+// Host objects
+// ==============================================================================================
+struct gf_context {
+    int device = 0;
+    hipStream_t stream = nullptr;
+    uint32_t cus = 256;
+    std::string err;
+};
+
+struct gf_pool {
+    gf_context* ctx = nullptr;
+    uint32_t payload = 0, S = 0, C = 0;
+    std::vector<void*> cols;
+    std::vector<uint32_t> elem;
+    // host ledger (ChunkLedger.hpp:36-134)
+    std::vector<uint32_t> free_list, gen, lease;
+    uint32_t peak_live = 0, peak_leases = 0;
+
+    Cols dcols() const
+    {
+        Cols c{};
+        for (size_t i = 0; i < cols.size(); ++i) c.c[i] = cols[i];
+        return c;
+    }
+    gf_chunk_handle acquire()
+    {
+        if (free_list.empty()) fail(GF_POOL_EXHAUSTED, "graphFlow::ChunkPool: pool exhausted");
+        uint32_t idx = free_list.back();
+        free_list.pop_back();
+        peak_live = std::max<uint32_t>(peak_live, S - (uint32_t)free_list.size());
+        return gf_chunk_handle{idx, gen[idx]};
+    }
+    void validate(gf_chunk_handle h) const
+    {
+        if (h.index >= S) fail(GF_BAD_ARGUMENT, "graphFlow::ChunkPool: chunk index out of range");
+        if (h.generation != gen[h.index]) fail(GF_BAD_ARGUMENT, "graphFlow::ChunkPool: stale chunk handle");
+    }
+    void release(gf_chunk_handle h)
+    {
+        validate(h);
+        if (lease[h.index]) fail(GF_BAD_ARGUMENT, "graphFlow::ChunkPool: release while a reader lease is outstanding");
+        if (gen[h.index] == UINT32_MAX) fail(GF_OVERFLOW, "graphFlow::ChunkPool generation overflow");
+        gen[h.index] += 1u;
+        free_list.push_back(h.index);
+    }
+    void reset_ledger()
+    {
+        free_list.resize(S);
+        for (uint32_t i = 0; i < S; ++i) free_list[i] = S - 1u - i;
+        std::fill(gen.begin(), gen.end(), 0u);
+        std::fill(lease.begin(), lease.end(), 0u);
+        peak_live = peak_leases = 0;
+    }
+};
+
+namespace {
+
+template <class T>
+T* dalloc(size_t n)
+{
+    void* p = nullptr;
+    HIPX(hipMalloc(&p, std::max<size_t>(n, 1) * sizeof(T)));
+    return (T*)p;
+}
+
+template <class T>
+T* halloc(size_t n)
+{
+    void* p = nullptr;
+    HIPX(hipHostMalloc(&p, std::max<size_t>(n, 1) * sizeof(T), hipHostMallocDefault));
+    return (T*)p;
+}
+
+struct DevMem {
+    std::vector<void*> dev, host;
+    template <class T>
+    T* d(size_t n)
+    {
+        T* p = dalloc<T>(n);
+        dev.push_back(p);
+        return p;
+    }
+    template <class T>
+    T* h(size_t n)
+    {
+        T* p = halloc<T>(n);
+        host.push_back(p);
+        return p;
+    }
+    void clear()
+    {
+        for (void* p : dev) (void)hipFree(p);
+        for (void* p : host) (void)hipHostFree(p);
+        dev.clear();
+        host.clear();
+    }
+    ~DevMem() { clear(); }
+};
+
+uint32_t pick_tile_r(uint32_t tile_items)
+{
+    if (tile_items >= 4096u) return 16u;
+    if (tile_items >= 1024u) return 4u;
+    return 1u;
+}
+
+// Payload dispatch for host launchers.
+template <class F>
+void with_payload(uint32_t payload, uint32_t r, F&& f)
+{
+    auto go = [&](auto ptag) {
+        using P = typename decltype(ptag)::type;
+        if constexpr (std::is_same_v<P, Photon>) {
+            if (r >= 4u) f(ptag, std::integral_constant<int, 4>{});
+            else f(ptag, std::integral_constant<int, 1>{});
+        } else {
+            if (r >= 16u) f(ptag, std::integral_constant<int, 16>{});
+            else if (r >= 4u) f(ptag, std::integral_constant<int, 4>{});
+            else f(ptag, std::integral_constant<int, 1>{});
+        }
+    };
+    if (payload == GF_PAYLOAD_PARTICLE) go(std::type_identity<Particle>{});
+    else if (payload == GF_PAYLOAD_KEYED) go(std::type_identity<Keyed>{});
+    else go(std::type_identity<Photon>{});
+}
+
+uint32_t effective_r(uint32_t payload, uint32_t r)
+{
+    if (payload == GF_PAYLOAD_PHOTON) return r >= 4u ? 4u : 1u;
+    return r >= 16u ? 16u : r >= 4u ? 4u : 1u;
+}
+
+bool stage_fits(uint32_t bins, uint32_t tile_items, uint32_t rec_bytes)
+{
+    return bins <= 64u && scatter_smem(bins, tile_items, rec_bytes, true) <= 64u * 1024u;
+}
+
+std::vector<double> zipf_table(uint32_t values, double s)
+{
+    std::vector<double> t(values);
+    double acc = 0.0;
+    for (uint32_t r = 0; r < values; ++r) {
+        acc += std::pow((double)(r + 1u), -s);
+        t[r] = acc;
+    }
+    return t;
+}
+
+}  // namespace
+
+// ---- stage: one node's batch protocol (StageBase, Execution.hpp:1537-1561) ----------------------
+struct gf_stage {
+    gf_pool* pool = nullptr;
+    gf_node_desc nd{};
+    uint32_t key = 0, Q = 1, tile_items = 1024, r = 4, bins = 1, cap_in = 1, cap_out = 1, cap_tiles = 1;
+    uint32_t reserve_bound = 1;
+    uint64_t seed = 0;
+    RunParams rp{};
+    DevMem mem;
+    Batch* d_batch = nullptr;
+    Batch* h_batch = nullptr;
+    uint32_t* d_status = nullptr;
+    uint32_t* h_status = nullptr;
+    uint32_t* h_counts = nullptr;
+    gf_chunk_desc* h_in = nullptr;
+    gf_chunk_desc* h_out = nullptr;
+    uint32_t* h_prefix = nullptr;
+    double* d_zipf = nullptr;
+    hipEvent_t ev = nullptr;
+    std::vector<gf_chunk_desc> in, pending;
+    std::vector<uint32_t> totals;
+    uint32_t stage_no = 0, n_tiles = 0, particles = 0;
+    uint64_t first_id = 0;
+    bool failed = false, launched = false;
+    uint64_t launches = 0;
+    std::string err;
+
+    ~gf_stage()
+    {
+        if (ev) (void)hipEventDestroy(ev);
+    }
+
+    int kind() const
+    {
+        if (nd.role == GF_ROLE_SOURCE) return 0;
+        if (nd.role == GF_ROLE_SINK) return 3;
+        return is_partition(nd) ? 1 : 2;
+    }
+
+    void init(gf_pool* p, const gf_node_desc& d, uint32_t k, uint64_t sd, uint32_t q, uint32_t ti)
+    {
+        pool = p;
+        nd = d;
+        key = k;
+        seed = sd;
+        Q = q ? q : 1u;
+        r = effective_r(p->payload, pick_tile_r(ti ? ti : 1024u));
+        tile_items = 256u * r;
+        bins = nd.role == GF_ROLE_SINK ? 1u : std::max<uint32_t>(nd.bins, 1u);
+        if (kind() == 1 && nd.bins == 0) fail(GF_BAD_ARGUMENT, "graphFlow: a partition needs at least one bin");
+        cap_in = nd.role == GF_ROLE_SOURCE ? 1u : Q;
+        cap_out = nd.role == GF_ROLE_SOURCE ? 1u : nd.role == GF_ROLE_SINK ? 1u : reserve_bound_of(nd, p->C, Q);
+        reserve_bound = cap_out;
+        cap_tiles = (uint32_t)(ceil_div(p->C, tile_items) * cap_in);
+        d_batch = mem.d<Batch>(1);
+        h_batch = mem.h<Batch>(1);
+        std::memset(h_batch, 0, sizeof(Batch));
+        h_batch->in = mem.d<gf_chunk_desc>(cap_in);
+        h_batch->tile_prefix = mem.d<uint32_t>(cap_in + 1u);
+        h_batch->item_prefix = mem.d<uint32_t>(cap_in + 1u);
+        h_batch->out = mem.d<gf_chunk_desc>(cap_out);
+        h_batch->counts = mem.d<uint32_t>((size_t)cap_tiles * bins);
+        h_batch->totals = mem.d<uint32_t>(2u * bins);
+        h_batch->tile_items = tile_items;
+        h_batch->purpose = 2u * key;
+        d_status = mem.d<uint32_t>(1);
+        h_status = mem.h<uint32_t>(1);
+        h_counts = mem.h<uint32_t>((size_t)cap_tiles * bins);
+        h_in = mem.h<gf_chunk_desc>(cap_in);
+        h_out = mem.h<gf_chunk_desc>(cap_out);
+        h_prefix = mem.h<uint32_t>(2u * (cap_in + 1u));
+        HIPX(hipEventCreateWithFlags(&ev, hipEventDisableTiming));
+        rp.seed = seed;
+        rp.C = p->C;
+        if (nd.op == GF_SRC_KEYED_ZIPF) {
+            if (nd.u[0] == 0) fail(GF_BAD_ARGUMENT, "zipf source needs a value count");
+            auto t = zipf_table(nd.u[0], (double)nd.f[0]);
+            d_zipf = mem.d<double>(t.size());
+            HIPX(hipMemcpy(d_zipf, t.data(), t.size() * sizeof(double), hipMemcpyHostToDevice));
+            rp.zipf = d_zipf;
+            rp.zipf_values = nd.u[0];
+        }
+    }
+
+    void throw_if_failed() const
+    {
+        if (failed) fail(GF_STAGE_FAILED, "graphFlow: stage is single-use after a failed batch");
+    }
+
+    void upload_batch(hipStream_t st, uint32_t n_out)
+    {
+        HIPX(hipMemcpyAsync(h_batch->in, h_in, sizeof(gf_chunk_desc) * std::max<uint32_t>(h_batch->n_in, 1u),
+                            hipMemcpyHostToDevice, st));
+        HIPX(hipMemcpyAsync(h_batch->tile_prefix, h_prefix, sizeof(uint32_t) * (h_batch->n_in + 1u),
+                            hipMemcpyHostToDevice, st));
+        HIPX(hipMemcpyAsync(h_batch->item_prefix, h_prefix + cap_in + 1u, sizeof(uint32_t) * (h_batch->n_in + 1u),
+                            hipMemcpyHostToDevice, st));
+        if (n_out)
+            HIPX(hipMemcpyAsync(h_batch->out, h_out, sizeof(gf_chunk_desc) * n_out, hipMemcpyHostToDevice, st));
+        HIPX(hipMemcpyAsync(d_batch, h_batch, sizeof(Batch), hipMemcpyHostToDevice, st));
+    }
+
+    void reset_status(hipStream_t st) { HIPX(hipMemsetAsync(d_status, 0, sizeof(uint32_t), st)); }
+
+    void check_status(const char* what)
+    {
+        uint32_t s = *h_status;
+        if (s != 0u) {
+            failed = true;
+            const char* name = s == GF_INVALID ? "invalid" : s == GF_OVERFLOW ? "overflow" : "invalid";
+            fail(s == GF_OVERFLOW ? GF_OVERFLOW : GF_INVALID, std::string(what) + ": device status reports " + name);
+        }
+    }
+
+    void release_pending()
+    {
+        for (auto& d : pending) {
+            try {
+                pool->release(gf_chunk_handle{d.index, d.generation});
+            } catch (...) {
+            }
+        }
+        pending.clear();
+    }
+
+    // ---- launch (Execution.hpp:2363-2430 / 1805-1901 / 1615-1637 / 2809-2828)
+    void launch(const gf_chunk_desc* chunks, uint32_t n, uint64_t fid, hipStream_t st)
+    {
+        throw_if_failed();
+        if (n == 0) fail(GF_PLAN_MISMATCH, "graphFlow: empty batch");
+        if (n > cap_in) fail(GF_PLAN_MISMATCH, "graphFlow: batch exceeds batchChunks");
+        in.assign(chunks, chunks + n);
+        first_id = fid;
+        stage_no = 0;
+        pending.clear();
+        particles = 0;
+        n_tiles = 0;
+        for (uint32_t c = 0; c < n; ++c) {
+            h_in[c] = chunks[c];
+            h_prefix[c] = n_tiles;
+            h_prefix[cap_in + 1u + c] = particles;
+            n_tiles += (uint32_t)ceil_div(chunks[c].fill, tile_items);
+            particles += chunks[c].fill;
+        }
+        h_prefix[n] = n_tiles;
+        h_prefix[cap_in + 1u + n] = particles;
+        h_batch->n_in = n;
+        h_batch->n_tiles = n_tiles;
+        h_batch->particles = particles;
+        h_batch->first_id = fid;
+        h_batch->digest_base = 0;
+        Cols cols = pool->dcols();
+        const uint32_t C = pool->C;
+        reset_status(st);
+        const int kd = kind();
+        if (kd == 0) {
+            // the source fills the admitted chunk it is handed (SourceStage::launch)
+            h_out[0] = chunks[0];
+            h_batch->n_out = 1;
+            upload_batch(st, 1);
+            uint32_t tiles = (uint32_t)ceil_div(chunks[0].fill, tile_items);
+            if (tiles)
+                with_payload(pool->payload, r, [&](auto pt, auto) {
+                    using P = typename decltype(pt)::type;
+                    hipLaunchKernelGGL(k_source<P>, dim3(tiles), dim3(kThreads), 0, st, nd, d_batch, cols, C, rp);
+                });
+            pending.assign(chunks, chunks + 1);
+        } else if (kd == 3) {
+            h_batch->n_out = 0;
+            upload_batch(st, 0);
+            if (n_tiles)
+                with_payload(pool->payload, r, [&](auto pt, auto) {
+                    using P = typename decltype(pt)::type;
+                    hipLaunchKernelGGL(k_sink<P>, dim3(n_tiles), dim3(kThreads), 64, st, nd, d_batch, cols, C, d_status);
+                });
+        } else if (kd == 2) {
+            uint64_t R = ceil_div(particles, C);
+            if (particles == 0) fail(GF_PLAN_MISMATCH, "transform: empty batch");
+            if (R > reserve_bound) fail(GF_PLAN_MISMATCH, "transform: exact chunk count exceeds the reservation bound");
+            for (uint32_t k = 0; k < R; ++k) {
+                gf_chunk_handle h;
+                try {
+                    h = pool->acquire();
+                } catch (...) {
+                    release_pending();
+                    failed = true;
+                    throw;
+                }
+                uint64_t rem = particles - (uint64_t)k * C;
+                gf_chunk_desc d{h.index, h.generation, (uint32_t)std::min<uint64_t>(rem, C), 0u};
+                pending.push_back(d);
+                h_out[k] = d;
+            }
+            h_batch->n_out = (uint32_t)R;
+            upload_batch(st, (uint32_t)R);
+            with_payload(pool->payload, r, [&](auto pt, auto rc) {
+                using P = typename decltype(pt)::type;
+                hipLaunchKernelGGL((k_transform<P, decltype(rc)::value>), dim3(n_tiles), dim3(kThreads), 0, st, nd,
+                                   d_batch, cols, C, rp);
+            });
+        } else {
+            h_batch->n_out = 0;
+            upload_batch(st, 0);
+            with_payload(pool->payload, r, [&](auto pt, auto rc) {
+                using P = typename decltype(pt)::type;
+                hipLaunchKernelGGL((k_count<P, decltype(rc)::value>), dim3(n_tiles), dim3(kThreads),
+                                   std::max<uint32_t>(bins, 1u) * 4u, st, nd, d_batch, cols, C, d_status);
+            });
+            HIPX(hipMemcpyAsync(h_counts, h_batch->counts, sizeof(uint32_t) * (size_t)n_tiles * bins,
+                                hipMemcpyDeviceToHost, st));
+        }
+        HIPX(hipGetLastError());
+        HIPX(hipMemcpyAsync(h_status, d_status, sizeof(uint32_t), hipMemcpyDeviceToHost, st));
+        HIPX(hipEventRecord(ev, st));
+        launched = true;
+        ++launches;
+    }
+
+    // ---- advance (Execution.hpp:2432-2455, 2520-2642, 2715-2725)
+    bool advance(hipStream_t st, uint32_t& allocated)
+    {
+        throw_if_failed();
+        allocated = 0;
+        HIPX(hipEventSynchronize(ev));
+        const int kd = kind();
+        if (kd == 0) {
+            check_status("source");
+            return false;
+        }
+        if (kd == 3) {
+            check_status("sink");
+            return false;
+        }
+        if (kd == 2) {
+            try {
+                check_status("transform");
+            } catch (...) {
+                release_pending();
+                throw;
+            }
+            allocated = (uint32_t)pending.size();
+            return false;
+        }
+        if (stage_no == 0) {
+            try {
+                plan_and_scatter(st, allocated);
+            } catch (...) {
+                release_pending();
+                failed = true;
+                throw;
+            }
+            stage_no = 1;
+            return true;
+        }
+        try {
+            check_status("partition scatter");
+        } catch (...) {
+            release_pending();
+            throw;
+        }
+        stage_no = 0;
+        return false;
+    }
+
+    void plan_and_scatter(hipStream_t st, uint32_t& allocated)
+    {
+        check_status("partition classify");
+        const uint32_t B = bins, C = pool->C;
+        totals.assign(B, 0u);
+        uint64_t sum = 0;
+        for (uint32_t t = 0; t < n_tiles; ++t)
+            for (uint32_t b = 0; b < B; ++b) totals[b] += h_counts[(size_t)t * B + b];
+        for (uint32_t b = 0; b < B; ++b) sum += totals[b];
+        if (sum != particles) fail(GF_PLAN_MISMATCH, "partition classify: classified particle count does not match the batch");
+        // planDenseBins
+        std::vector<uint32_t> base(B);
+        uint64_t exact = 0;
+        for (uint32_t b = 0; b < B; ++b) {
+            base[b] = (uint32_t)exact;
+            exact += ceil_div(totals[b], C);
+        }
+        if (exact > output_chunk_bound(particles, C, B)) fail(GF_PLAN_MISMATCH, "planDenseBins: exact chunk count exceeds the reservation bound");
+        if (exact > reserve_bound) fail(GF_PLAN_MISMATCH, "partition plan: exact chunk count exceeds the reservation bound");
+        pending.clear();
+        for (uint32_t b = 0; b < B; ++b) {
+            uint32_t nch = (uint32_t)ceil_div(totals[b], C);
+            for (uint32_t k = 0; k < nch; ++k) {
+                gf_chunk_handle h = pool->acquire();
+                uint32_t rem = totals[b] - k * C;
+                pending.push_back(gf_chunk_desc{h.index, h.generation, std::min(rem, C), b});
+            }
+        }
+        for (size_t i = 0; i < pending.size(); ++i) h_out[i] = pending[i];
+        // flat output start of every (tile, bin): bin base + items of that bin in earlier tiles
+        std::vector<uint32_t> run(B, 0u);
+        for (uint32_t t = 0; t < n_tiles; ++t)
+            for (uint32_t b = 0; b < B; ++b) {
+                uint32_t x = h_counts[(size_t)t * B + b];
+                h_counts[(size_t)t * B + b] = base[b] * C + run[b];
+                run[b] += x;
+            }
+        h_batch->n_out = (uint32_t)pending.size();
+        HIPX(hipMemcpyAsync(h_batch->counts, h_counts, sizeof(uint32_t) * (size_t)n_tiles * B, hipMemcpyHostToDevice, st));
+        upload_batch(st, h_batch->n_out);
+        reset_status(st);
+        Cols cols = pool->dcols();
+        bool stage = stage_fits(B, tile_items, payload_bytes(pool->payload));
+        size_t lds = scatter_smem(B, tile_items, payload_bytes(pool->payload), stage);
+        with_payload(pool->payload, r, [&](auto pt, auto rc) {
+            using P = typename decltype(pt)::type;
+            constexpr int RR = decltype(rc)::value;
+            if (stage)
+                hipLaunchKernelGGL((k_scatter<P, RR, true>), dim3(n_tiles), dim3(kThreads), lds, st, nd, d_batch, cols, C,
+                                   d_status);
+            else
+                hipLaunchKernelGGL((k_scatter<P, RR, false>), dim3(n_tiles), dim3(kThreads), lds, st, nd, d_batch, cols,
+                                   C, d_status);
+        });
+        HIPX(hipGetLastError());
+        HIPX(hipMemcpyAsync(h_status, d_status, sizeof(uint32_t), hipMemcpyDeviceToHost, st));
+        HIPX(hipEventRecord(ev, st));
+        allocated = (uint32_t)pending.size();
+        ++launches;
+    }
+
+    void finish(std::vector<gf_chunk_desc>& outputs)
+    {
+        outputs.insert(outputs.end(), pending.begin(), pending.end());
+        pending.clear();
+    }
+
+    void abort()
+    {
+        if (kind() != 0) release_pending();
+        pending.clear();
+        failed = true;
+    }
+};
+
+// ---- flow ----------------------------------------------------------------------------------------
+struct NodeRec {
+    std::string name;
+    gf_node_desc nd{};
+    std::vector<int32_t> routes;
+};
+
+struct EngineBuffers {
+    DevMem mem;
+    std::string key;
+    Engine e{};
+    uint32_t stage_mask = 0, r = 4, grid = 256;
+    size_t lds_a = 0, lds_b = 0;
+    hipGraphExec_t exec = nullptr;
+    uint32_t graph_steps = 0;
+    ~EngineBuffers()
+    {
+        if (exec) (void)hipGraphExecDestroy(exec);
+    }
+};
+
+struct gf_flow {
+    gf_context* ctx = nullptr;
+    uint32_t payload = 0;
+    std::vector<NodeRec> nodes;
+    gf_tuning tuning{};
+    std::string err;
+    std::vector<uint32_t> trace;
+    std::unique_ptr<gf_pool> pool;
+    std::unique_ptr<EngineBuffers> eng;
+    double kt_ms[4] = {0, 0, 0, 0};
+    uint64_t kt_n[4] = {0, 0, 0, 0};
+    uint64_t part_items = 0, xf_items = 0;
+};
+
+namespace {
+
+void check_graph(const gf_flow& f, uint32_t source)
+{
+    if (source >= f.nodes.size() || f.nodes[source].nd.role != GF_ROLE_SOURCE)
+        fail(GF_BAD_ARGUMENT, "graphFlow: run() requires the registered source node");
+    uint32_t sources = 0;
+    for (const auto& n : f.nodes) {
+        if (n.nd.role == GF_ROLE_SOURCE) ++sources;
+        if (n.nd.role == GF_ROLE_SINK) continue;
+        if (n.nd.bins == 0) fail(GF_BAD_ARGUMENT, "graphFlow: a source or transform node needs at least one bin");
+        for (size_t b = 0; b < n.routes.size(); ++b) {
+            int32_t d = n.routes[b];
+            if (d < 0)
+                fail(GF_BAD_ARGUMENT, "graphFlow: missing route for node '" + n.name + "' bin " + std::to_string(b));
+            if ((size_t)d >= f.nodes.size()) fail(GF_BAD_ARGUMENT, "graphFlow: route destination out of range");
+        }
+    }
+    if (sources != 1) fail(GF_BAD_ARGUMENT, "graphFlow: a graph needs exactly one source node");
+}
+
+struct RunShape {
+    uint32_t C, P, S, Q, tile_items;
+};
+
+RunShape resolve(const gf_flow& f, uint64_t items)
+{
+    RunShape s{};
+    s.C = f.tuning.chunk_capacity ? f.tuning.chunk_capacity : 8192u;
+    s.Q = f.tuning.batch_chunks ? f.tuning.batch_chunks : 1u;
+    uint64_t P = f.tuning.live_particles ? f.tuning.live_particles : 8ull * s.C;
+    if (P > items) P = items;
+    if (P == 0) P = 1;
+    s.P = (uint32_t)P;
+    uint64_t reserve = 0;
+    uint32_t maxb = 1;
+    for (const auto& n : f.nodes) {
+        if (n.nd.role != GF_ROLE_TRANSFORM) continue;
+        reserve += output_chunk_bound(s.C, s.C, n.nd.bins);
+        maxb = std::max(maxb, n.nd.bins);
+    }
+    uint64_t S = items == 0 ? 1u : P + reserve;
+    if (f.tuning.num_chunks) S = f.tuning.num_chunks;
+    if (S > UINT32_MAX || S * (uint64_t)s.C > UINT32_MAX) fail(GF_OVERFLOW, "graphFlow: chunk pool would exceed 2^32 slots");
+    s.S = (uint32_t)S;
+    // autotuned tile: a workgroup tile must amortise its [tile][bin] metadata for wide partitions
+    uint32_t ti = f.tuning.tile_items;
+    if (!ti) ti = maxb >= 64u ? 4096u : 1024u;
+    s.tile_items = 256u * effective_r(f.payload, pick_tile_r(ti));
+    return s;
+}
+
+void ensure_pool(gf_flow& f, const RunShape& s)
+{
+    if (f.pool && f.pool->S == s.S && f.pool->C == s.C && f.pool->payload == f.payload) {
+        f.pool->reset_ledger();
+        return;
+    }
+    if (f.pool) {
+        for (void* p : f.pool->cols) (void)hipFree(p);
+        f.pool.reset();
+    }
+    auto p = std::make_unique<gf_pool>();
+    p->ctx = f.ctx;
+    p->payload = f.payload;
+    p->S = s.S;
+    p->C = s.C;
+    p->elem = column_bytes(f.payload);
+    for (uint32_t b : p->elem) p->cols.push_back(dalloc<char>((size_t)s.S * s.C * b));
+    p->gen.assign(s.S, 0u);
+    p->lease.assign(s.S, 0u);
+    p->reset_ledger();
+    f.pool = std::move(p);
+}
+
+void trace_host(std::vector<uint32_t>& tr, bool on, uint32_t node, uint64_t step, const std::vector<gf_chunk_desc>& in,
+                uint32_t nbins, const uint32_t* counts, const std::vector<gf_chunk_desc>& out)
+{
+    if (!on) return;
+    tr.push_back(0xB47C0000u | node);
+    tr.push_back((uint32_t)step);
+    tr.push_back((uint32_t)in.size());
+    for (auto& d : in) {
+        tr.push_back(d.index);
+        tr.push_back(d.generation);
+        tr.push_back(d.fill);
+    }
+    tr.push_back(nbins);
+    for (uint32_t i = 0; i < nbins; ++i) tr.push_back(counts[i]);
+    tr.push_back((uint32_t)out.size());
+    for (auto& d : out) {
+        tr.push_back(d.index);
+        tr.push_back(d.generation);
+        tr.push_back(d.fill);
+        tr.push_back(d.bin);
+    }
+}
+
+// Plain form: GraphRunner::run with every batch completed before the next loop iteration.
+void run_host(gf_flow& f, uint32_t src, uint64_t N, const RunShape& sh, gf_stats& st)
+{
+    gf_pool& pool = *f.pool;
+    const uint32_t n = (uint32_t)f.nodes.size(), C = sh.C, P = sh.P, Q = sh.Q;
+    std::vector<std::unique_ptr<gf_stage>> stages(n);
+    for (uint32_t k = 0; k < n; ++k) {
+        stages[k] = std::make_unique<gf_stage>();
+        stages[k]->init(&pool, f.nodes[k].nd, k, f.tuning.seed, Q, sh.tile_items);
+    }
+    hipStream_t s = f.ctx->stream;
+    struct Lane {
+        bool armed = false;
+        std::vector<gf_chunk_desc> batch;
+        uint32_t reserved = 0;
+        uint64_t particles = 0;
+    };
+    std::vector<Lane> lanes(n);
+    std::vector<std::deque<gf_chunk_desc>> ready(n);
+    uint64_t generated = 0, sunk = 0, iter = 0, batches = 0;
+    uint32_t reserved = 0, rr = 0;
+    const bool tr = f.tuning.trace != 0;
+    f.trace.clear();
+    f.part_items = f.xf_items = 0;
+    auto teardown = [&]() {
+        (void)hipStreamSynchronize(s);
+        for (uint32_t k = 0; k < n; ++k) {
+            stages[k]->abort();
+            if (lanes[k].armed && f.nodes[k].nd.role != GF_ROLE_SOURCE)
+                for (auto& d : lanes[k].batch) {
+                    try {
+                        pool.release(gf_chunk_handle{d.index, d.generation});
+                    } catch (...) {
+                    }
+                }
+            if (lanes[k].armed && f.nodes[k].nd.role == GF_ROLE_SOURCE)
+                for (auto& d : lanes[k].batch) {
+                    try {
+                        pool.release(gf_chunk_handle{d.index, d.generation});
+                    } catch (...) {
+                    }
+                }
+            for (auto& d : ready[k]) {
+                try {
+                    pool.release(gf_chunk_handle{d.index, d.generation});
+                } catch (...) {
+                }
+            }
+            ready[k].clear();
+            lanes[k].armed = false;
+        }
+    };
+    try {
+        for (;;) {
+            bool progress = false;
+            // 1) retire
+            for (uint32_t k = 0; k < n; ++k) {
+                Lane& L = lanes[k];
+                if (!L.armed) continue;
+                gf_stage& stg = *stages[k];
+                const gf_node_desc& nd = f.nodes[k].nd;
+                uint32_t allocated = 0;
+                bool more = stg.advance(s, allocated);
+                if (stg.kind() == 1 && more)
+                    trace_host(f.trace, tr, k, iter, L.batch, nd.bins, stg.totals.data(), stg.pending);
+                if (allocated) {
+                    if (allocated > L.reserved) fail(GF_BAD_ARGUMENT, "graphFlow: node acquired more chunks than reserved");
+                    reserved -= L.reserved;
+                    L.reserved = 0;
+                } else if (!more && L.reserved) {
+                    reserved -= L.reserved;
+                    L.reserved = 0;
+                }
+                if (!more) {
+                    std::vector<gf_chunk_desc> outs;
+                    stg.finish(outs);
+                    if (nd.role != GF_ROLE_SINK)
+                        for (auto& d : outs) {
+                            if (d.bin >= f.nodes[k].routes.size()) fail(GF_BAD_ARGUMENT, "graphFlow: node produced an unroutable bin");
+                            ready[(uint32_t)f.nodes[k].routes[d.bin]].push_back(d);
+                        }
+                    if (nd.role == GF_ROLE_SINK) sunk += L.particles;
+                    if (nd.role != GF_ROLE_SOURCE)
+                        for (auto& d : L.batch) pool.release(gf_chunk_handle{d.index, d.generation});
+                    if (nd.role == GF_ROLE_TRANSFORM) {
+                        ++batches;
+                        if (is_partition(nd)) f.part_items += L.particles;
+                        else f.xf_items += L.particles;
+                    }
+                    L.armed = false;
+                }
+                progress = true;
+            }
+            // 2) admission
+            {
+                uint64_t live = generated - sunk, remaining = N - generated;
+                uint64_t min_fill = std::min<uint64_t>(std::min<uint64_t>(C, P), remaining);
+                bool within = live < P, budget = live < P && (P - live) >= min_fill;
+                if (generated < N && !lanes[src].armed && within && budget && pool.free_list.size() > reserved) {
+                    uint64_t fill = std::min<uint64_t>(std::min<uint64_t>(remaining, P - live), C);
+                    gf_chunk_handle h = pool.acquire();
+                    gf_chunk_desc d{h.index, h.generation, (uint32_t)fill, 0u};
+                    Lane& L = lanes[src];
+                    L.batch = {d};
+                    L.particles = fill;
+                    L.armed = true;
+                    stages[src]->launch(&d, 1, generated, s);
+                    trace_host(f.trace, tr, src, iter, {}, 0, nullptr, L.batch);
+                    generated += fill;
+                    progress = true;
+                }
+            }
+            // 3) dispatch one batch: sinks first, then round robin
+            {
+                bool dispatched = false;
+                for (int pass = 0; pass < 2 && !dispatched; ++pass)
+                    for (uint32_t kk = 0; kk < n && !dispatched; ++kk) {
+                        uint32_t k = pass == 0 ? kk : (rr + kk) % n;
+                        const gf_node_desc& nd = f.nodes[k].nd;
+                        if (pass == 0 && nd.role != GF_ROLE_SINK) continue;
+                        Lane& L = lanes[k];
+                        if (ready[k].empty() || L.armed) continue;
+                        size_t wanted = nd.role == GF_ROLE_TRANSFORM ? std::min<size_t>(Q, ready[k].size()) : 1u;
+                        uint32_t reserve = 0;
+                        if (nd.role == GF_ROLE_TRANSFORM) {
+                            uint64_t parts = 0;
+                            for (size_t i = 0; i < wanted; ++i) parts += ready[k][i].fill;
+                            while (wanted) {
+                                uint64_t bound = output_chunk_bound(parts, C, nd.bins);
+                                if (pool.free_list.size() - reserved >= bound) {
+                                    reserve = (uint32_t)bound;
+                                    break;
+                                }
+                                --wanted;
+                                parts -= ready[k][wanted].fill;
+                            }
+                            if (!wanted) continue;
+                        }
+                        L.batch.assign(ready[k].begin(), ready[k].begin() + (long)wanted);
+                        ready[k].erase(ready[k].begin(), ready[k].begin() + (long)wanted);
+                        L.reserved = reserve;
+                        reserved += reserve;
+                        L.particles = 0;
+                        for (auto& d : L.batch) L.particles += d.fill;
+                        L.armed = true;
+                        stages[k]->launch(L.batch.data(), (uint32_t)L.batch.size(), 0, s);
+                        if (nd.role == GF_ROLE_SINK) trace_host(f.trace, tr, k, iter, L.batch, 0, nullptr, {});
+                        if (stages[k]->kind() == 2) {
+                            uint32_t tot = (uint32_t)L.particles;
+                            trace_host(f.trace, tr, k, iter, L.batch, 1, &tot, stages[k]->pending);
+                        }
+                        dispatched = true;
+                    }
+                if (dispatched) progress = true;
+                ++rr;
+            }
+            bool drained = true;
+            for (uint32_t k = 0; k < n; ++k)
+                if (lanes[k].armed || !ready[k].empty()) drained = false;
+            ++iter;
+            if (generated >= N && drained && reserved == 0) break;
+            if (!progress) fail(GF_STALL, "graphFlow: scheduler stall, no progress possible");
+        }
+    } catch (...) {
+        teardown();
+        throw;
+    }
+    HIPX(hipStreamSynchronize(s));
+    if (pool.free_list.size() != pool.S) fail(GF_BAD_ARGUMENT, "graphFlow: chunk leak after the run");
+    st.completed_batches = batches;
+    st.steps = iter;
+    st.generated = generated;
+    st.sunk = sunk;
+    st.peak_live_chunks = pool.peak_live;
+    uint64_t l = 0;
+    for (auto& sg : stages) l += sg->launches;
+    st.kernel_launches = l;
+}
+
+std::string shape_key(const gf_flow& f, const RunShape& s, uint64_t N)
+{
+    std::string k = std::to_string(f.payload) + "/" + std::to_string(s.C) + "/" + std::to_string(s.P) + "/" +
+                    std::to_string(s.S) + "/" + std::to_string(s.Q) + "/" + std::to_string(s.tile_items) + "/" +
+                    std::to_string(f.tuning.trace) + "/" + std::to_string(f.tuning.seed) + "/" + std::to_string(N) + "/" +
+                    std::to_string(f.tuning.steps_per_launch);
+    for (auto& n : f.nodes) {
+        k += "|" + std::to_string(n.nd.role) + "," + std::to_string(n.nd.op) + "," + std::to_string(n.nd.bins);
+        for (int32_t r : n.routes) k += "," + std::to_string(r);
+        const uint32_t* u = n.nd.u;
+        const float* fl = n.nd.f;
+        k += "," + std::to_string(u[0]) + "," + std::to_string(u[1]) + "," + std::to_string(u[2]) + "," + std::to_string(u[3]);
+        k += "," + std::to_string(fl[0]) + "," + std::to_string(fl[1]) + "," + std::to_string(n.nd.ptr[0]) + "," +
+             std::to_string(n.nd.ptr[1]) + "," + std::to_string(n.nd.size[0]);
+    }
+    return k;
+}
+
+void build_engine(gf_flow& f, uint64_t N, const RunShape& sh, uint32_t src)
+{
+    auto eb = std::make_unique<EngineBuffers>();
+    DevMem& m = eb->mem;
+    Engine& e = eb->e;
+    const uint32_t n = (uint32_t)f.nodes.size();
+    if (n > 64) fail(GF_BAD_ARGUMENT, "graphFlow: the device engine supports at most 64 nodes");
+    gf_pool& pool = *f.pool;
+    e.n_nodes = n;
+    e.C = sh.C;
+    e.P = sh.P;
+    e.Q = sh.Q;
+    e.S = sh.S;
+    e.ring_cap = sh.S;
+    e.N = N;
+    e.cols = pool.dcols();
+    e.rp.seed = f.tuning.seed;
+    e.rp.C = sh.C;
+    const gf_tuning& t = f.tuning;
+    e.rp.ph.L = (float)(t.photon_layers * t.photon_layer_mfp / (t.photon_mu_t > 0 ? t.photon_mu_t : 1.0));
+    e.rp.ph.mu_t = (float)t.photon_mu_t;
+    e.rp.ph.albedo = (float)t.photon_albedo;
+    e.rp.ph.g = (float)t.photon_hg_g;
+    e.rp.ph.roulette_w = (float)t.photon_roulette_w;
+    e.rp.ph.roulette_p = (float)t.photon_roulette_p;
+    e.rp.ph.tally_bins = t.photon_tally_bins;
+    std::vector<gf_node_desc> nds(n);
+    std::vector<int32_t> routes;
+    std::vector<uint32_t> rb(n, 0u);
+    uint32_t r = effective_r(f.payload, pick_tile_r(sh.tile_items));
+    const uint32_t rec = payload_bytes(f.payload);
+    size_t lds_a = 256, lds_b = 256;
+    uint32_t stage_mask = 0;
+    std::vector<Batch> hb(n);
+    uint32_t grid_tiles = 0;
+    for (uint32_t k = 0; k < n; ++k) {
+        nds[k] = f.nodes[k].nd;
+        nds[k].route_offset = (uint32_t)routes.size();
+        routes.insert(routes.end(), f.nodes[k].routes.begin(), f.nodes[k].routes.end());
+        const gf_node_desc& nd = nds[k];
+        const bool part = nd.role == GF_ROLE_TRANSFORM && is_partition(nd);
+        const uint32_t B = std::max<uint32_t>(nd.bins, 1u);
+        uint32_t cap_in = nd.role == GF_ROLE_SOURCE ? 1u : nd.role == GF_ROLE_SINK ? sh.S : sh.Q;
+        uint32_t cap_out = nd.role == GF_ROLE_SOURCE ? 1u : nd.role == GF_ROLE_SINK ? 1u : reserve_bound_of(nd, sh.C, sh.Q);
+        rb[k] = cap_out;
+        uint32_t cap_tiles = (uint32_t)(ceil_div(sh.C, sh.tile_items) * (nd.role == GF_ROLE_SINK ? 1u : cap_in));
+        Batch& b = hb[k];
+        std::memset(&b, 0, sizeof(b));
+        b.in = m.d<gf_chunk_desc>(cap_in);
+        b.tile_prefix = m.d<uint32_t>(cap_in + 1u);
+        b.item_prefix = m.d<uint32_t>(cap_in + 1u);
+        b.out = m.d<gf_chunk_desc>(cap_out);
+        b.counts = part ? m.d<uint32_t>((size_t)cap_tiles * B) : nullptr;
+        b.totals = m.d<uint32_t>(2u * B);
+        b.tile_items = sh.tile_items;
+        b.purpose = 2u * k;
+        if (part) {
+            lds_a = std::max(lds_a, (size_t)B * 4u);
+            bool stg = stage_fits(B, sh.tile_items, rec);
+            if (stg && k < 32) stage_mask |= 1u << k;
+            lds_b = std::max(lds_b, scatter_smem(B, sh.tile_items, rec, stg && k < 32));
+            grid_tiles += cap_tiles;
+        } else if (nd.role != GF_ROLE_SINK) {
+            grid_tiles += cap_tiles;
+        }
+        if (nd.op == GF_SRC_KEYED_ZIPF) {
+            auto z = zipf_table(nd.u[0], (double)nd.f[0]);
+            double* dz = m.d<double>(z.size());
+            HIPX(hipMemcpy(dz, z.data(), z.size() * sizeof(double), hipMemcpyHostToDevice));
+            e.rp.zipf = dz;
+            e.rp.zipf_values = nd.u[0];
+        }
+    }
+    if (lds_b > 64u * 1024u || lds_a > 64u * 1024u) fail(GF_BAD_ARGUMENT, "graphFlow: partition width exceeds the workgroup LDS budget");
+    gf_node_desc* dn = m.d<gf_node_desc>(n);
+    HIPX(hipMemcpy(dn, nds.data(), sizeof(gf_node_desc) * n, hipMemcpyHostToDevice));
+    int32_t* dr = m.d<int32_t>(std::max<size_t>(routes.size(), 1));
+    if (!routes.empty()) HIPX(hipMemcpy(dr, routes.data(), sizeof(int32_t) * routes.size(), hipMemcpyHostToDevice));
+    uint32_t* drb = m.d<uint32_t>(n);
+    HIPX(hipMemcpy(drb, rb.data(), sizeof(uint32_t) * n, hipMemcpyHostToDevice));
+    Batch* dbat = m.d<Batch>(n);
+    HIPX(hipMemcpy(dbat, hb.data(), sizeof(Batch) * n, hipMemcpyHostToDevice));
+    e.nodes = dn;
+    e.routes = dr;
+    e.reserve_bound = drb;
+    e.batches = dbat;
+    e.free_stack = m.d<uint32_t>(sh.S);
+    e.gens = m.d<uint32_t>(sh.S);
+    e.rings = m.d<gf_chunk_desc>((size_t)n * sh.S);
+    e.ring_head = m.d<unsigned long long>(n);
+    e.ring_tail = m.d<unsigned long long>(n);
+    e.ka_prefix = m.d<uint32_t>(n + 1u);
+    e.kb_prefix = m.d<uint32_t>(n + 1u);
+    e.ctl = m.d<uint32_t>(CTL_WORDS);
+    e.ctr = m.d<unsigned long long>(CNT_WORDS);
+    e.trace_cap = t.trace ? (unsigned long long)1 << 26 : 0ull;
+    e.trace = t.trace ? m.d<uint32_t>(e.trace_cap) : nullptr;
+    eb->grid = std::min<uint32_t>(std::max<uint32_t>(grid_tiles, 1u), 1024u);
+    e.grid = eb->grid;
+    eb->r = r;
+    eb->stage_mask = stage_mask;
+    eb->lds_a = lds_a;
+    eb->lds_b = lds_b;
+    eb->key = shape_key(f, sh, N);
+    (void)src;
+    f.eng = std::move(eb);
+}
+
+void launch_step(const EngineBuffers& eb, uint32_t payload, hipStream_t s)
+{
+    with_payload(payload, eb.r, [&](auto pt, auto rc) {
+        using P = typename decltype(pt)::type;
+        constexpr int RR = decltype(rc)::value;
+        hipLaunchKernelGGL((k_engine_ka<P, RR>), dim3(eb.grid), dim3(kThreads), eb.lds_a, s, eb.e);
+        hipLaunchKernelGGL((k_engine_kb<P, RR>), dim3(eb.grid), dim3(kThreads), eb.lds_b, s, eb.e, eb.stage_mask);
+    });
+}
+
+void run_device(gf_flow& f, uint32_t src, uint64_t N, const RunShape& sh, gf_stats& st, bool timed)
+{
+    const std::string key = shape_key(f, sh, N);
+    if (!f.eng || f.eng->key != key) build_engine(f, N, sh, src);
+    EngineBuffers& eb = *f.eng;
+    hipStream_t s = f.ctx->stream;
+    hipLaunchKernelGGL(k_engine_setup, dim3(1), dim3(kThreads), 0, s, eb.e);
+    HIPX(hipGetLastError());
+    const uint32_t K = f.tuning.steps_per_launch ? f.tuning.steps_per_launch : 64u;
+    const bool use_graph = f.tuning.use_graph != 0 && !timed;
+    if (use_graph && (!eb.exec || eb.graph_steps != K)) {
+        if (eb.exec) {
+            (void)hipGraphExecDestroy(eb.exec);
+            eb.exec = nullptr;
+        }
+        hipGraph_t g;
+        HIPX(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
+        for (uint32_t i = 0; i < K; ++i) launch_step(eb, f.payload, s);
+        HIPX(hipStreamEndCapture(s, &g));
+        HIPX(hipGraphInstantiate(&eb.exec, g, nullptr, nullptr, 0));
+        (void)hipGraphDestroy(g);
+        eb.graph_steps = K;
+    }
+    uint32_t ctl[CTL_WORDS];
+    uint64_t launches = 0;
+    hipEvent_t ev[4] = {nullptr, nullptr, nullptr, nullptr};
+    if (timed)
+        for (auto& x : ev) HIPX(hipEventCreate(&x));
+    for (int i = 0; i < 4; ++i) {
+        f.kt_ms[i] = 0.0;
+        f.kt_n[i] = 0;
+    }
+    // the stall bound: every step retires or admits something, so a run needs at most this many
+    const uint64_t max_launches = 4ull * (N + 16u) * 64u + 1024u;
+    for (;;) {
+        if (use_graph) {
+            HIPX(hipGraphLaunch(eb.exec, s));
+            launches += 2ull * K;
+        } else {
+            for (uint32_t i = 0; i < K; ++i) {
+                if (timed) {
+                    with_payload(f.payload, eb.r, [&](auto pt, auto rc) {
+                        using P = typename decltype(pt)::type;
+                        constexpr int RR = decltype(rc)::value;
+                        HIPX(hipEventRecord(ev[0], s));
+                        hipLaunchKernelGGL((k_engine_ka<P, RR>), dim3(eb.grid), dim3(kThreads), eb.lds_a, s, eb.e);
+                        HIPX(hipEventRecord(ev[1], s));
+                        hipLaunchKernelGGL((k_engine_kb<P, RR>), dim3(eb.grid), dim3(kThreads), eb.lds_b, s, eb.e,
+                                           eb.stage_mask);
+                        HIPX(hipEventRecord(ev[2], s));
+                    });
+                    HIPX(hipEventSynchronize(ev[2]));
+                    float a = 0, b = 0;
+                    HIPX(hipEventElapsedTime(&a, ev[0], ev[1]));
+                    HIPX(hipEventElapsedTime(&b, ev[1], ev[2]));
+                    f.kt_ms[0] += a;
+                    f.kt_ms[1] += b;
+                    f.kt_n[0]++;
+                    f.kt_n[1]++;
+                } else {
+                    launch_step(eb, f.payload, s);
+                }
+                launches += 2;
+            }
+        }
+        HIPX(hipGetLastError());
+        HIPX(hipMemcpyAsync(ctl, eb.e.ctl, sizeof(ctl), hipMemcpyDeviceToHost, s));
+        HIPX(hipStreamSynchronize(s));
+        if (ctl[CTL_STATUS] || ctl[CTL_DONE]) break;
+        if (launches > max_launches) fail(GF_STALL, "graphFlow: device engine exceeded its step bound");
+    }
+    if (timed)
+        for (auto& x : ev) (void)hipEventDestroy(x);
+    unsigned long long ctr[CNT_WORDS];
+    HIPX(hipMemcpy(ctr, eb.e.ctr, sizeof(ctr), hipMemcpyDeviceToHost));
+    if (f.tuning.trace) {
+        uint64_t len = std::min<uint64_t>(ctr[CNT_TRACE_LEN], eb.e.trace_cap);
+        f.trace.resize(len);
+        if (len) HIPX(hipMemcpy(f.trace.data(), eb.e.trace, len * sizeof(uint32_t), hipMemcpyDeviceToHost));
+    }
+    st.completed_batches = ctr[CNT_BATCHES];
+    st.steps = ctr[CNT_STEP];
+    st.generated = ctr[CNT_GENERATED];
+    st.sunk = ctr[CNT_SUNK];
+    st.peak_live_chunks = (uint32_t)ctr[CNT_PEAK_LIVE];
+    st.kernel_launches = launches;
+    f.part_items = ctr[CNT_PART_ITEMS];
+    f.xf_items = ctr[CNT_XF_ITEMS];
+    if (ctl[CTL_STATUS]) {
+        gf_status code = (gf_status)ctl[CTL_STATUS];
+        const char* what = code == GF_INVALID ? "partition: device status reports an invalid classifier or updater result"
+                         : code == GF_OVERFLOW ? "partition scatter: device status reports a checked updater overflow"
+                         : code == GF_POOL_EXHAUSTED ? "graphFlow::ChunkPool: pool exhausted"
+                         : code == GF_STALL ? "graphFlow: scheduler stall, no progress possible"
+                         : code == GF_BAD_ARGUMENT ? "graphFlow: chunk leak after the run"
+                                                   : "partition plan: exact chunk count exceeds the reservation bound";
+        fail(code, what);
+    }
+}
+
+}  // namespace
+
+// ==============================================================================================
+// C ABI
+// ==============================================================================================
+extern "C" {
+
+uint32_t gf_abi_version(void) { return 1u; }
+
+const char* gf_status_name(gf_status s)
+{
+    switch (s) {
+    case GF_OK: return "ok";
+    case GF_INVALID: return "invalid";
+    case GF_OVERFLOW: return "overflow";
+    case GF_STALL: return "stall";
+    case GF_POOL_EXHAUSTED: return "pool_exhausted";
+    case GF_BAD_ARGUMENT: return "bad_argument";
+    case GF_PLAN_MISMATCH: return "plan_mismatch";
+    case GF_DEVICE_ERROR: return "device_error";
+    case GF_STAGE_FAILED: return "stage_failed";
+    }
+    return "unknown";
+}
+
+gf_status gf_context_create(int device, gf_context** out)
+{
+    if (!out) return GF_BAD_ARGUMENT;
+    return guard(nullptr, [&] {
+        auto c = std::make_unique<gf_context>();
+        c->device = device;
+        HIPX(hipSetDevice(device));
+        HIPX(hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking));
+        hipDeviceProp_t prop;
+        HIPX(hipGetDeviceProperties(&prop, device));
+        c->cus = (uint32_t)prop.multiProcessorCount;
+        *out = c.release();
+    });
+}
+
+gf_status gf_context_destroy(gf_context* ctx)
+{
+    if (!ctx) return GF_BAD_ARGUMENT;
+    if (ctx->stream) (void)hipStreamDestroy(ctx->stream);
+    delete ctx;
+    return GF_OK;
+}
+
+gf_status gf_output_chunk_bound(uint64_t n, uint64_t capacity, uint64_t max_bins, uint64_t* out)
+{
+    if (!out) return GF_BAD_ARGUMENT;
+    return guard(nullptr, [&] { *out = output_chunk_bound(n, capacity, max_bins); });
+}
+
+gf_status gf_plan_dense_bins(const uint32_t* totals, uint32_t bins, uint32_t capacity, uint32_t* bin_base,
+                             uint32_t* fills, uint32_t fills_cap, uint32_t* num_output_chunks)
+{
+    if (!totals || !bin_base || !num_output_chunks || bins == 0 || capacity == 0) return GF_BAD_ARGUMENT;
+    return guard(nullptr, [&] {
+        uint64_t exact = 0, parts = 0;
+        for (uint32_t b = 0; b < bins; ++b) {
+            bin_base[b] = (uint32_t)exact;
+            exact += ceil_div(totals[b], capacity);
+            parts += totals[b];
+        }
+        if (exact > UINT32_MAX) fail(GF_OVERFLOW, "planDenseBins: output chunk count overflow");
+        if (exact > output_chunk_bound(parts, capacity, bins)) fail(GF_PLAN_MISMATCH, "planDenseBins: bound");
+        if (exact > fills_cap) fail(GF_BAD_ARGUMENT, "planDenseBins: fills buffer too small");
+        uint32_t k = 0;
+        for (uint32_t b = 0; b < bins; ++b)
+            for (uint32_t o = 0; o < totals[b]; o += capacity) fills[k++] = std::min(totals[b] - o, capacity);
+        *num_output_chunks = (uint32_t)exact;
+    });
+}
+
+gf_status gf_pool_create(gf_context* ctx, uint32_t payload, uint32_t num_chunks, uint32_t capacity, gf_pool** out)
+{
+    if (!ctx || !out || num_chunks == 0 || capacity == 0 || payload > GF_PAYLOAD_PHOTON) return GF_BAD_ARGUMENT;
+    if ((uint64_t)num_chunks * capacity > UINT32_MAX) return GF_OVERFLOW;
+    return guard(&ctx->err, [&] {
+        auto p = std::make_unique<gf_pool>();
+        p->ctx = ctx;
+        p->payload = payload;
+        p->S = num_chunks;
+        p->C = capacity;
+        p->elem = column_bytes(payload);
+        for (uint32_t b : p->elem) p->cols.push_back(dalloc<char>((size_t)num_chunks * capacity * b));
+        p->gen.assign(num_chunks, 0u);
+        p->lease.assign(num_chunks, 0u);
+        p->reset_ledger();
+        *out = p.release();
+    });
+}
+
+gf_status gf_pool_destroy(gf_pool* pool)
+{
+    if (!pool) return GF_BAD_ARGUMENT;
+    for (void* p : pool->cols) (void)hipFree(p);
+    delete pool;
+    return GF_OK;
+}
+
+gf_status gf_pool_acquire(gf_pool* pool, gf_chunk_handle* out)
+{
+    if (!pool || !out) return GF_BAD_ARGUMENT;
+    return guard(&pool->ctx->err, [&] { *out = pool->acquire(); });
+}
+
+gf_status gf_pool_release(gf_pool* pool, gf_chunk_handle h)
+{
+    if (!pool) return GF_BAD_ARGUMENT;
+    return guard(&pool->ctx->err, [&] { pool->release(h); });
+}
+
+gf_status gf_pool_acquire_lease(gf_pool* pool, gf_chunk_handle h)
+{
+    if (!pool) return GF_BAD_ARGUMENT;
+    return guard(&pool->ctx->err, [&] {
+        pool->validate(h);
+        if (pool->lease[h.index] == UINT32_MAX) fail(GF_OVERFLOW, "graphFlow::ChunkPool lease overflow");
+        pool->peak_leases = std::max(pool->peak_leases, ++pool->lease[h.index]);
+    });
+}
+
+gf_status gf_pool_release_lease(gf_pool* pool, gf_chunk_handle h)
+{
+    if (!pool) return GF_BAD_ARGUMENT;
+    return guard(&pool->ctx->err, [&] {
+        pool->validate(h);
+        if (pool->lease[h.index] == 0) fail(GF_BAD_ARGUMENT, "graphFlow::ChunkPool: lease release without a matching acquire");
+        --pool->lease[h.index];
+    });
+}
+
+gf_status gf_pool_counts(gf_pool* pool, uint32_t counts[6])
+{
+    if (!pool || !counts) return GF_BAD_ARGUMENT;
+    counts[0] = pool->S;
+    counts[1] = pool->C;
+    counts[2] = (uint32_t)pool->free_list.size();
+    counts[3] = pool->S - (uint32_t)pool->free_list.size();
+    counts[4] = pool->peak_live;
+    counts[5] = pool->peak_leases;
+    return GF_OK;
+}
+
+gf_status gf_pool_base(gf_pool* pool, gf_chunk_handle h, uint64_t* slot)
+{
+    if (!pool || !slot) return GF_BAD_ARGUMENT;
+    if (h.index >= pool->S) return GF_BAD_ARGUMENT;
+    *slot = (uint64_t)h.index * pool->C;
+    return GF_OK;
+}
+
+gf_status gf_pool_column(gf_pool* pool, uint32_t column, void** device_ptr, uint32_t* elem_bytes)
+{
+    if (!pool || column >= pool->cols.size()) return GF_BAD_ARGUMENT;
+    if (device_ptr) *device_ptr = pool->cols[column];
+    if (elem_bytes) *elem_bytes = pool->elem[column];
+    return GF_OK;
+}
+
+gf_status gf_pool_write(gf_pool* pool, uint32_t column, uint64_t first, const void* host, uint64_t count)
+{
+    if (!pool || column >= pool->cols.size() || first + count > (uint64_t)pool->S * pool->C) return GF_BAD_ARGUMENT;
+    return guard(&pool->ctx->err, [&] {
+        HIPX(hipMemcpy((char*)pool->cols[column] + first * pool->elem[column], host, count * pool->elem[column],
+                       hipMemcpyHostToDevice));
+    });
+}
+
+gf_status gf_pool_read(gf_pool* pool, uint32_t column, uint64_t first, void* host, uint64_t count)
+{
+    if (!pool || column >= pool->cols.size() || first + count > (uint64_t)pool->S * pool->C) return GF_BAD_ARGUMENT;
+    return guard(&pool->ctx->err, [&] {
+        HIPX(hipMemcpy(host, (char*)pool->cols[column] + first * pool->elem[column], count * pool->elem[column],
+                       hipMemcpyDeviceToHost));
+    });
+}
+
+gf_status gf_stage_create(gf_pool* pool, const gf_node_desc* node, uint32_t node_key, uint64_t seed,
+                          uint32_t batch_chunks, uint32_t tile_items, gf_stage** out)
+{
+    if (!pool || !node || !out) return GF_BAD_ARGUMENT;
+    return guard(&pool->ctx->err, [&] {
+        auto s = std::make_unique<gf_stage>();
+        s->init(pool, *node, node_key, seed, batch_chunks, tile_items);
+        *out = s.release();
+    });
+}
+
+gf_status gf_stage_destroy(gf_stage* stage)
+{
+    if (!stage) return GF_BAD_ARGUMENT;
+    delete stage;
+    return GF_OK;
+}
+
+gf_status gf_stage_launch(gf_stage* stage, const gf_chunk_desc* chunks, uint32_t n, uint64_t first_id, void* stream)
+{
+    if (!stage || !chunks) return GF_BAD_ARGUMENT;
+    return guard(&stage->err, [&] {
+        stage->launch(chunks, n, first_id, stream ? (hipStream_t)stream : stage->pool->ctx->stream);
+    });
+}
+
+gf_status gf_stage_advance(gf_stage* stage, void* stream, uint32_t* allocated, int* more_work)
+{
+    if (!stage || !allocated || !more_work) return GF_BAD_ARGUMENT;
+    return guard(&stage->err, [&] {
+        uint32_t a = 0;
+        *more_work = stage->advance(stream ? (hipStream_t)stream : stage->pool->ctx->stream, a) ? 1 : 0;
+        *allocated = a;
+    });
+}
+
+gf_status gf_stage_finish(gf_stage* stage, gf_chunk_desc* outputs, uint32_t cap, uint32_t* n_out)
+{
+    if (!stage || !n_out) return GF_BAD_ARGUMENT;
+    if (stage->pending.size() > cap) return GF_BAD_ARGUMENT;
+    std::vector<gf_chunk_desc> outs;
+    stage->finish(outs);
+    for (size_t i = 0; i < outs.size(); ++i) outputs[i] = outs[i];
+    *n_out = (uint32_t)outs.size();
+    return GF_OK;
+}
+
+gf_status gf_stage_abort(gf_stage* stage)
+{
+    if (!stage) return GF_BAD_ARGUMENT;
+    stage->abort();
+    return GF_OK;
+}
+
+gf_status gf_stage_counts(gf_stage* stage, uint32_t* counts, uint32_t bins)
+{
+    if (!stage || !counts || bins != stage->totals.size()) return GF_BAD_ARGUMENT;
+    std::copy(stage->totals.begin(), stage->totals.end(), counts);
+    return GF_OK;
+}
+
+gf_status gf_stage_tiles(gf_stage* stage, gf_tile_desc* tiles, uint32_t cap, uint32_t* n)
+{
+    if (!stage || !n) return GF_BAD_ARGUMENT;
+    return guard(&stage->err, [&] {
+        *n = stage->n_tiles;
+        if (stage->n_tiles > cap) fail(GF_BAD_ARGUMENT, "tile buffer too small");
+        if (!stage->n_tiles) return;
+        gf_tile_desc* d = dalloc<gf_tile_desc>(stage->n_tiles);
+        hipStream_t s = stage->pool->ctx->stream;
+        hipLaunchKernelGGL(k_tiles, dim3(std::min<uint32_t>((stage->n_tiles + 255u) / 256u, 1024u)), dim3(256), 0, s,
+                           stage->d_batch, stage->pool->C, d);
+        HIPX(hipGetLastError());
+        HIPX(hipMemcpyAsync(tiles, d, sizeof(gf_tile_desc) * stage->n_tiles, hipMemcpyDeviceToHost, s));
+        HIPX(hipStreamSynchronize(s));
+        (void)hipFree(d);
+    });
+}
+
+gf_status gf_flow_create(gf_context* ctx, uint32_t payload, gf_flow** out)
+{
+    if (!ctx || !out || payload > GF_PAYLOAD_PHOTON) return GF_BAD_ARGUMENT;
+    auto f = std::make_unique<gf_flow>();
+    f->ctx = ctx;
+    f->payload = payload;
+    f->tuning.use_graph = 1;
+    *out = f.release();
+    return GF_OK;
+}
+
+gf_status gf_flow_destroy(gf_flow* flow)
+{
+    if (!flow) return GF_BAD_ARGUMENT;
+    flow->eng.reset();
+    if (flow->pool)
+        for (void* p : flow->pool->cols) (void)hipFree(p);
+    delete flow;
+    return GF_OK;
+}
+
+gf_status gf_flow_add_node(gf_flow* flow, const char* name, const gf_node_desc* node, uint32_t* key)
+{
+    if (!flow || !node || !key) return GF_BAD_ARGUMENT;
+    return guard(&flow->err, [&] {
+        if (node->role > GF_ROLE_SINK) fail(GF_BAD_ARGUMENT, "graphFlow: unknown node role");
+        if (node->role == GF_ROLE_SOURCE)
+            for (auto& n : flow->nodes)
+                if (n.nd.role == GF_ROLE_SOURCE) fail(GF_BAD_ARGUMENT, "graphFlow: a Flow has exactly one source node");
+        if (node->role == GF_ROLE_SINK && node->bins != 0) fail(GF_BAD_ARGUMENT, "graphFlow: a sink node has no bins and no routes");
+        if (node->role != GF_ROLE_SINK && node->bins == 0)
+            fail(GF_BAD_ARGUMENT, "graphFlow: a source or transform node needs at least one bin");
+        if (node->bins > 65536u) fail(GF_BAD_ARGUMENT, "graphFlow: partition width exceeds maxSupportedBins");
+        if (flow->nodes.size() >= 32767u) fail(GF_OVERFLOW, "graphFlow: graph node count exceeds the RNG stream-purpose range");
+        NodeRec r;
+        r.name = name ? name : "";
+        r.nd = *node;
+        r.routes.assign(node->role == GF_ROLE_SINK ? 0u : node->bins, -1);
+        *key = (uint32_t)flow->nodes.size();
+        flow->nodes.push_back(std::move(r));
+    });
+}
+
+gf_status gf_flow_connect(gf_flow* flow, uint32_t from, uint32_t bin, uint32_t to)
+{
+    if (!flow) return GF_BAD_ARGUMENT;
+    return guard(&flow->err, [&] {
+        if (from >= flow->nodes.size() || to >= flow->nodes.size()) fail(GF_BAD_ARGUMENT, "graphFlow: malformed node handle");
+        NodeRec& p = flow->nodes[from];
+        if (p.nd.role == GF_ROLE_SINK) fail(GF_BAD_ARGUMENT, "graphFlow: a sink node has no output to connect");
+        if (flow->nodes[to].nd.role == GF_ROLE_SOURCE) fail(GF_BAD_ARGUMENT, "graphFlow: a route may not target the source node");
+        if (bin >= p.nd.bins) fail(GF_BAD_ARGUMENT, "graphFlow: output bin out of range");
+        if (p.routes[bin] != -1) fail(GF_BAD_ARGUMENT, "graphFlow: route for this output is already connected");
+        p.routes[bin] = (int32_t)to;
+    });
+}
+
+gf_status gf_flow_configure(gf_flow* flow, const gf_tuning* t)
+{
+    if (!flow || !t) return GF_BAD_ARGUMENT;
+    return guard(&flow->err, [&] {
+        if (t->schedule > GF_SCHEDULE_DEVICE) fail(GF_BAD_ARGUMENT, "graphFlow::Tuning: unknown schedule");
+        flow->tuning = *t;
+    });
+}
+
+gf_status gf_flow_run(gf_flow* flow, uint32_t source, uint64_t items, gf_stats* stats)
+{
+    if (!flow) return GF_BAD_ARGUMENT;
+    return guard(&flow->err, [&] {
+        check_graph(*flow, source);
+        if (items != 0 && items - 1u > 0xFFFFFFFFull) fail(GF_OVERFLOW, "graphFlow: item count exceeds the 32 bit RNG stream-id range");
+        RunShape sh = resolve(*flow, items);
+        ensure_pool(*flow, sh);
+        gf_stats st{};
+        st.pool_chunks = sh.S;
+        st.chunk_capacity = sh.C;
+        st.live_particles = sh.P;
+        auto t0 = std::chrono::steady_clock::now();
+        if (items == 0) {
+            // nothing to admit
+        } else if (flow->tuning.schedule == GF_SCHEDULE_DEVICE) {
+            run_device(*flow, source, items, sh, st, flow->tuning.use_graph == 2u);
+        } else {
+            run_host(*flow, source, items, sh, st);
+        }
+        st.wall_ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
+        if (stats) *stats = st;
+    });
+}
+
+gf_status gf_flow_last_error(gf_flow* flow, char* buffer, uint32_t size)
+{
+    if (!flow || !buffer || size == 0) return GF_BAD_ARGUMENT;
+    std::snprintf(buffer, size, "%s", flow->err.c_str());
+    return GF_OK;
+}
+
+gf_status gf_flow_trace(gf_flow* flow, uint32_t* host, uint64_t cap, uint64_t* len)
+{
+    if (!flow || !len) return GF_BAD_ARGUMENT;
+    *len = flow->trace.size();
+    if (host) std::memcpy(host, flow->trace.data(), sizeof(uint32_t) * std::min<uint64_t>(cap, flow->trace.size()));
+    return GF_OK;
+}
+
+gf_status gf_flow_kernel_times(gf_flow* flow, double* ms, uint64_t* launches, uint32_t kinds)
+{
+    if (!flow || !ms || !launches) return GF_BAD_ARGUMENT;
+    for (uint32_t i = 0; i < kinds && i < 4; ++i) {
+        ms[i] = flow->kt_ms[i];
+        launches[i] = flow->kt_n[i];
+    }
+    return GF_OK;
+}
+
+gf_status gf_flow_item_passes(gf_flow* flow, uint64_t* partition_items, uint64_t* transform_items)
+{
+    if (!flow || !partition_items || !transform_items) return GF_BAD_ARGUMENT;
+    *partition_items = flow->part_items;
+    *transform_items = flow->xf_items;
+    return GF_OK;
+}
+
+gf_status gf_hist_rows(gf_context* ctx, const uint64_t* counts, uint64_t n, uint32_t lo, uint64_t* host_rows,
+                       uint64_t cap, uint64_t* rows)
+{
+    if (!ctx || !counts || !rows) return GF_BAD_ARGUMENT;
+    return guard(&ctx->err, [&] {
+        const uint64_t seg = 1u << 16;
+        const uint32_t blocks = (uint32_t)std::max<uint64_t>(ceil_div(n, seg), 1u);
+        uint32_t* bnz = dalloc<uint32_t>(blocks);
+        hipStream_t s = ctx->stream;
+        hipLaunchKernelGGL(k_nz_count, dim3(blocks), dim3(kThreads), 0, s, (const unsigned long long*)counts, n, seg, bnz);
+        HIPX(hipGetLastError());
+        std::vector<uint32_t> h(blocks);
+        HIPX(hipMemcpyAsync(h.data(), bnz, sizeof(uint32_t) * blocks, hipMemcpyDeviceToHost, s));
+        HIPX(hipStreamSynchronize(s));
+        uint64_t total = 0;
+        for (auto& v : h) {
+            uint32_t x = v;
+            v = (uint32_t)total;
+            total += x;
+        }
+        *rows = total;
+        if (host_rows && total <= cap && total) {
+            HIPX(hipMemcpyAsync(bnz, h.data(), sizeof(uint32_t) * blocks, hipMemcpyHostToDevice, s));
+            unsigned long long* d = dalloc<unsigned long long>(2 * total);
+            hipLaunchKernelGGL(k_nz_write, dim3(blocks), dim3(kThreads), 0, s, (const unsigned long long*)counts, n, seg, bnz, lo,
+                               d);
+            HIPX(hipGetLastError());
+            HIPX(hipMemcpyAsync(host_rows, d, sizeof(uint64_t) * 2 * total, hipMemcpyDeviceToHost, s));
+            HIPX(hipStreamSynchronize(s));
+            (void)hipFree(d);
+        }
+        (void)hipFree(bnz);
+    });
+}
+
+}  // extern "C"

@@ -1,0 +1,189 @@
+/* graphflow.h — C ABI of the MI355X transform-batch library (libgraphflow_amd.so).
+ *
+ * This is the drop-in boundary for graphFlow's transform-batch path. Every entry point takes
+ * plain pointers and sizes, never throws, and returns a gf_status. Entry points are called from
+ * one host thread per context (the reference's single scheduler thread,
+ * docs/architecture.md:89-92); device work is stream ordered.
+ *
+ * Reference interfaces replaced (paths relative to the reference repository):
+ *   gf_pool_*        ChunkStorage concept      include/graphFlow/core/ChunkStorage.hpp:35-54,
+ *                    ChunkPool / ChunkLedger   include/graphFlow/core/ChunkPool.hpp:155-269,
+ *                                              include/graphFlow/core/ChunkLedger.hpp:83-188
+ *   gf_stage_*       StageBase hooks           include/graphFlow/flow/Execution.hpp:1537-1561
+ *                    (= GraphRunner::Hooks     include/graphFlow/sched/GraphRunner.hpp:451-461)
+ *   gf_flow_*        Flow<TPayload,TDevice,TStorage> registration/run
+ *                                              include/graphFlow/flow/Flow.hpp:85-412
+ *   gf_tuning        Tuning                    include/graphFlow/flow/Tuning.hpp:27-145
+ *   gf_stats         SchedulerStats            include/graphFlow/sched/GraphRunner.hpp:154-210
+ *   gf_status        StatusCode + BatchError / SchedulerStallError / invalid_argument
+ *                                              include/graphFlow/flow/Status.hpp:21-96,
+ *                                              include/graphFlow/sched/GraphRunner.hpp:62-66,355-378
+ *   gf_output_chunk_bound, gf_plan_dense_bins
+ *                                              include/graphFlow/core/Reservation.hpp:35-50,
+ *                                              include/graphFlow/graph/DenseBinTable.hpp:39-91
+ *
+ * Arbitrary C++ device functors cannot cross a C ABI, so nodes carry op descriptors: the
+ * functor set is compiled into the library (GF_OP_*).
+ */
+#ifndef GRAPHFLOW_H
+#define GRAPHFLOW_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* ---- status --------------------------------------------------------------------------- */
+typedef enum gf_status {
+    GF_OK = 0,
+    GF_INVALID = 1,        /* StatusCode::invalid: classifier/updater rejected an item      */
+    GF_OVERFLOW = 2,       /* StatusCode::overflow: checked counter / updater overflow      */
+    GF_STALL = 3,          /* SchedulerStallError                                            */
+    GF_POOL_EXHAUSTED = 4, /* ChunkPool "pool exhausted"                                     */
+    GF_BAD_ARGUMENT = 5,   /* std::invalid_argument / std::logic_error                      */
+    GF_PLAN_MISMATCH = 6,  /* BatchError from planning / finalise                            */
+    GF_DEVICE_ERROR = 7,   /* HIP runtime error                                              */
+    GF_STAGE_FAILED = 8    /* a stage is single-use after a failed batch                      */
+} gf_status;
+
+/* ---- chunk model (core/Chunk.hpp:19-37, graph/TileDescriptor.hpp:25-30) ----------------- */
+typedef struct gf_chunk_handle { uint32_t index, generation; } gf_chunk_handle;
+typedef struct gf_chunk_desc { uint32_t index, generation, fill, bin; } gf_chunk_desc;
+typedef struct gf_tile_desc { uint32_t chunk_index, tile_offset, count; } gf_tile_desc;
+
+/* ---- payloads --------------------------------------------------------------------------- */
+enum {
+    GF_PAYLOAD_PARTICLE = 0, /* weight f32, id u32, gen u32                      (12 B) */
+    GF_PAYLOAD_KEYED = 1,    /* key u32, value u64                               (12 B) */
+    GF_PAYLOAD_PHOTON = 2    /* x,y,z,ux,uy,uz,w f32, id u32, step u32           (36 B) */
+};
+
+/* ---- node roles and ops ------------------------------------------------------------------ */
+enum { GF_ROLE_SOURCE = 0, GF_ROLE_TRANSFORM = 1, GF_ROLE_SINK = 2 };
+
+enum {
+    GF_SRC_PARTICLE_UNIFORM = 1, /* weight = U(0,1] from Philox word 0, id = index, gen = 0       */
+    GF_SRC_KEYED_ZIPF = 2,       /* key = Zipf(f[0]) rank over u[0] values, value = index        */
+    GF_SRC_PHOTON = 3,           /* origin, +z direction, weight 1                                */
+    GF_CLS_WEIGHT_ROUTE = 16,    /* bin = w < f[0] ? 0 : 1 + route(fmix32(id)), u[0] distribution */
+    GF_CLS_KEY_BUCKET = 17,      /* bin = fmix32(key) % bins                                      */
+    GF_CLS_ID_MOD = 18,          /* u[0]: 0 balanced, 1 one-hot, 2 skewed                          */
+    GF_CLS_LUT = 19,             /* bin = lut[u[3] + min(id,u[0]-1)*u[1] + min(gen,u[1]-1)], ptr[0] */
+    GF_CLS_PHOTON_ALIVE = 20,    /* bin = weight > 0 ? 0 : 1                                       */
+    GF_XF_SCALE_WEIGHT = 32,     /* weight *= f[0]; gen += 1                                       */
+    GF_XF_PHOTON_STEP = 33,      /* one flight + interaction + roulette + HG scatter; tally ptr[0] */
+    GF_SNK_HIST = 48,            /* ptr[0] u64 counts of field u[0] (0 weight bits, 2 gen) - u[1], size u[2] */
+    GF_SNK_RECORD_WEIGHT = 49,   /* ptr[0] u32 record[id] = weight bits                            */
+    GF_SNK_CHUNK_DIGEST = 50,    /* ptr[0] u32x4 log per sunk chunk, size[0] entries; ptr[1] u64x2 per bin */
+    GF_SNK_PHOTON_TALLY = 51     /* no-op sink for finished photons                                */
+};
+enum { GF_UPD_NONE = 0, GF_UPD_ADD_ON_BIN = 1, GF_UPD_GEN_TO_BIN = 2 };
+enum { GF_ROUTE_PARITY = 0, GF_ROUTE_SKEW_90_10 = 1, GF_ROUTE_ALL_ONE = 2, GF_ROUTE_UNIFORM = 3 };
+
+typedef struct gf_node_desc {
+    uint32_t role, op, upd, bins;
+    uint32_t route_offset; /* filled by gf_flow_add_node */
+    uint32_t u[4];
+    float f[4];
+    uint64_t ptr[2];  /* device buffers of sinks / LUT classifiers */
+    uint64_t size[2]; /* element counts of ptr[] */
+} gf_node_desc;
+
+/* ---- tuning / config (flow/Tuning.hpp) ---------------------------------------------------- */
+enum { GF_SCHEDULE_HOST = 0 /* plain: host GraphRunner, host planning */,
+       GF_SCHEDULE_DEVICE = 1 /* batched: device-resident step engine  */ };
+
+typedef struct gf_tuning {
+    uint64_t seed;
+    uint32_t batch_chunks;   /* Q (0 -> 1)                              */
+    uint32_t chunk_capacity; /* C (0 -> 8192)                           */
+    uint32_t live_particles; /* P (0 -> min(N, 8*C))                    */
+    uint32_t num_chunks;     /* S (0 -> P + K*R)                        */
+    uint32_t tile_items;     /* items per workgroup tile (0 -> autotune) */
+    uint32_t schedule;       /* GF_SCHEDULE_*                           */
+    uint32_t steps_per_launch; /* device engine steps per graph launch (0 -> 64) */
+    uint32_t trace;          /* record a batch trace                    */
+    uint32_t use_graph;      /* capture the device step loop in a hipGraph (default 1) */
+    uint32_t photon_layers, photon_tally_bins;
+    double photon_layer_mfp, photon_mu_t, photon_albedo, photon_hg_g, photon_roulette_w, photon_roulette_p;
+} gf_tuning;
+
+typedef struct gf_stats {
+    uint64_t completed_batches, steps, generated, sunk;
+    uint32_t pool_chunks, peak_live_chunks, chunk_capacity, live_particles;
+    uint64_t kernel_launches;
+    double wall_ms;
+} gf_stats;
+
+typedef struct gf_context gf_context;
+typedef struct gf_pool gf_pool;
+typedef struct gf_stage gf_stage;
+typedef struct gf_flow gf_flow;
+
+/* ---- context ------------------------------------------------------------------------------ */
+gf_status gf_context_create(int device, gf_context** out);
+gf_status gf_context_destroy(gf_context* ctx);
+const char* gf_status_name(gf_status s);
+uint32_t gf_abi_version(void);
+
+/* ---- closed-form plan helpers (host) ------------------------------------------------------- */
+gf_status gf_output_chunk_bound(uint64_t n, uint64_t capacity, uint64_t max_bins, uint64_t* out);
+gf_status gf_plan_dense_bins(const uint32_t* totals, uint32_t bins, uint32_t capacity, uint32_t* bin_base,
+                             uint32_t* fills, uint32_t fills_cap, uint32_t* num_output_chunks);
+
+/* ---- pool: ChunkStorage contract (ChunkStorage.hpp:35-54) ----------------------------------- */
+gf_status gf_pool_create(gf_context* ctx, uint32_t payload, uint32_t num_chunks, uint32_t capacity, gf_pool** out);
+gf_status gf_pool_destroy(gf_pool* pool);
+gf_status gf_pool_acquire(gf_pool* pool, gf_chunk_handle* out);
+gf_status gf_pool_release(gf_pool* pool, gf_chunk_handle handle);
+gf_status gf_pool_acquire_lease(gf_pool* pool, gf_chunk_handle handle);
+gf_status gf_pool_release_lease(gf_pool* pool, gf_chunk_handle handle);
+/* counts[0..5] = numChunks, chunkCapacity, freeCount, liveCount, peakLive, peakLeases */
+gf_status gf_pool_counts(gf_pool* pool, uint32_t counts[6]);
+gf_status gf_pool_base(gf_pool* pool, gf_chunk_handle handle, uint64_t* slot);
+gf_status gf_pool_column(gf_pool* pool, uint32_t column, void** device_ptr, uint32_t* elem_bytes);
+gf_status gf_pool_write(gf_pool* pool, uint32_t column, uint64_t first_slot, const void* host, uint64_t count);
+gf_status gf_pool_read(gf_pool* pool, uint32_t column, uint64_t first_slot, void* host, uint64_t count);
+
+/* ---- stage: one node's batch protocol (Execution.hpp:1537-1561) -----------------------------
+ * launch:  enqueue the first device stage of a batch (classify for a partition, the whole
+ *          transform / source / sink kernel otherwise).
+ * advance: called after the previous device stage completed; *more_work = 1 when a further
+ *          device stage was enqueued; *allocated = output chunks acquired by this call.
+ * finish:  copy the published output descriptors (bin-major, then chunk order).
+ * abort:   release unpublished outputs; the stage is single-use afterwards. */
+gf_status gf_stage_create(gf_pool* pool, const gf_node_desc* node, uint32_t node_key, uint64_t seed,
+                          uint32_t batch_chunks, uint32_t tile_items, gf_stage** out);
+gf_status gf_stage_destroy(gf_stage* stage);
+gf_status gf_stage_launch(gf_stage* stage, const gf_chunk_desc* chunks, uint32_t n, uint64_t first_id, void* stream);
+gf_status gf_stage_advance(gf_stage* stage, void* stream, uint32_t* allocated, int* more_work);
+gf_status gf_stage_finish(gf_stage* stage, gf_chunk_desc* outputs, uint32_t cap, uint32_t* n_out);
+gf_status gf_stage_abort(gf_stage* stage);
+/* per-bin totals of the last classify (bins entries) */
+gf_status gf_stage_counts(gf_stage* stage, uint32_t* counts, uint32_t bins);
+/* tile descriptors of the last launch, flattened on the device (TileDescriptor list) */
+gf_status gf_stage_tiles(gf_stage* stage, gf_tile_desc* tiles, uint32_t cap, uint32_t* n);
+
+/* ---- flow: typed graph registration and run (Flow.hpp:85-412) ------------------------------- */
+gf_status gf_flow_create(gf_context* ctx, uint32_t payload, gf_flow** out);
+gf_status gf_flow_destroy(gf_flow* flow);
+gf_status gf_flow_add_node(gf_flow* flow, const char* name, const gf_node_desc* node, uint32_t* key);
+gf_status gf_flow_connect(gf_flow* flow, uint32_t from, uint32_t bin, uint32_t to);
+gf_status gf_flow_configure(gf_flow* flow, const gf_tuning* tuning);
+gf_status gf_flow_run(gf_flow* flow, uint32_t source, uint64_t items, gf_stats* stats);
+gf_status gf_flow_last_error(gf_flow* flow, char* buffer, uint32_t size);
+gf_status gf_flow_trace(gf_flow* flow, uint32_t* host, uint64_t cap, uint64_t* len);
+/* kernel timing of the last device-engine run: per kernel kind total ms and launches */
+gf_status gf_flow_kernel_times(gf_flow* flow, double* ms, uint64_t* launches, uint32_t kinds);
+/* algorithmic bytes moved by the last run's partition/transform/source/sink item passes */
+gf_status gf_flow_item_passes(gf_flow* flow, uint64_t* partition_items, uint64_t* transform_items);
+
+/* ---- device histogram rows: compact a dense u64 count array into ascending (value,count) -- */
+gf_status gf_hist_rows(gf_context* ctx, const uint64_t* device_counts, uint64_t n, uint32_t lo,
+                       uint64_t* host_rows /* 2 per row */, uint64_t cap, uint64_t* rows);
+
+#ifdef __cplusplus
+}
+#endif
+#endif
