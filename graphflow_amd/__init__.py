@@ -88,7 +88,8 @@ class TuningC(C.Structure):
     _fields_ = [("seed", C.c_uint64), ("batch_chunks", C.c_uint32), ("chunk_capacity", C.c_uint32),
                 ("live_particles", C.c_uint32), ("num_chunks", C.c_uint32), ("tile_items", C.c_uint32),
                 ("schedule", C.c_uint32), ("steps_per_launch", C.c_uint32), ("trace", C.c_uint32),
-                ("use_graph", C.c_uint32), ("photon_layers", C.c_uint32), ("photon_tally_bins", C.c_uint32),
+                ("use_graph", C.c_uint32), ("fuse_transforms", C.c_uint32), ("max_workgroups", C.c_uint32),
+                ("device_engine", C.c_uint32), ("photon_layers", C.c_uint32), ("photon_tally_bins", C.c_uint32),
                 ("photon_layer_mfp", C.c_double), ("photon_mu_t", C.c_double), ("photon_albedo", C.c_double),
                 ("photon_hg_g", C.c_double), ("photon_roulette_w", C.c_double), ("photon_roulette_p", C.c_double)]
 
@@ -142,6 +143,7 @@ SIGNATURES = {
     "gf_flow_trace": (C.c_int, [_V, _P(C.c_uint32), C.c_uint64, _P(C.c_uint64)]),
     "gf_flow_kernel_times": (C.c_int, [_V, _P(C.c_double), _P(C.c_uint64), C.c_uint32]),
     "gf_flow_item_passes": (C.c_int, [_V, _P(C.c_uint64), _P(C.c_uint64)]),
+    "gf_flow_phase_profile": (C.c_int, [_V, _P(C.c_uint64), C.c_uint32]),
     "gf_hist_rows": (C.c_int, [_V, _V, C.c_uint64, C.c_uint32, _P(C.c_uint64), C.c_uint64, _P(C.c_uint64)]),
 }
 
@@ -436,6 +438,9 @@ class Tuning:
     steps_per_launch: Optional[int] = None
     trace: bool = False
     use_graph: int = 1
+    fuse_transforms: bool = False
+    max_workgroups: Optional[int] = None
+    device_engine: int = 0  # 0 autotuned, 1 wide step kernels, 2 one persistent workgroup
     photon: Optional[dict] = None
 
     def validate(self):
@@ -450,7 +455,8 @@ class Tuning:
                     live_particles=self.live_particles or 0, num_chunks=self.num_chunks or 0,
                     tile_items=self.tile_items or 0, schedule=self.schedule,
                     steps_per_launch=self.steps_per_launch or 0, trace=1 if self.trace else 0,
-                    use_graph=self.use_graph)
+                    use_graph=self.use_graph, fuse_transforms=1 if self.fuse_transforms else 0,
+                    max_workgroups=self.max_workgroups or 0, device_engine=self.device_engine)
         ph = self.photon or {}
         t.photon_layers = ph.get("layers", 10)
         t.photon_tally_bins = ph.get("tally_bins", 1024)
@@ -549,6 +555,14 @@ class Flow:
         n = (C.c_uint64 * 4)()
         lib().gf_flow_kernel_times(self.handle, ms, n, 4)
         return list(ms), list(n)
+
+    def phase_profile(self):
+        """Micro engine per-phase wall-clock microseconds per step (run with use_graph=2)."""
+        a = (C.c_uint64 * 16)()
+        _raise(lib().gf_flow_phase_profile(self.handle, a, 16))
+        steps = max(int(a[8]), 1)
+        names = ["count", "plan", "source", "partition", "transform", "sink", "retire", "select"]
+        return {n: a[i] / 100.0 / steps for i, n in enumerate(names)}, int(a[8])
 
     def item_passes(self):
         a, b = C.c_uint64(), C.c_uint64()

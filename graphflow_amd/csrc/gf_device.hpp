@@ -353,6 +353,10 @@ __device__ inline uint64_t keyed_hash(const Keyed& p)
 // Wave helpers
 // ------------------------------------------------------------------------------------------
 __device__ inline uint32_t lane_id() { return threadIdx.x & 63u; }
+// Thread index inside the 256-thread tile group (a workgroup may hold several tile groups).
+__device__ inline uint32_t ltid() { return threadIdx.x & (uint32_t)(kThreads - 1); }
+// Tile index meaning "this group has no tile in this round": every __syncthreads is still executed.
+constexpr uint32_t kNoTile = 0xFFFFFFFFu;
 __device__ inline uint64_t lanemask_lt() { return (1ull << lane_id()) - 1ull; }
 
 // Lanes with the same bin (the "peers" of each lane); iterations = distinct bins in the wave.
@@ -392,6 +396,11 @@ struct TileSpan {
 __device__ inline TileSpan tile_span(const Batch& b, const gf_chunk_desc* list, uint32_t n, uint32_t t, uint32_t C)
 {
     TileSpan s;
+    if (t == kNoTile) {
+        s.chunk = s.offset = s.count = 0u;
+        s.base = 0;
+        return s;
+    }
     s.chunk = find_chunk(b.tile_prefix, n, t);
     gf_chunk_desc d = list[s.chunk];
     s.offset = (t - b.tile_prefix[s.chunk]) * b.tile_items;
@@ -411,7 +420,7 @@ template <class P, int R>
 __device__ void count_tile(const gf_node_desc& nd, const Batch& b, uint32_t t, const Cols& cols, uint32_t C,
                            uint32_t* status, char* smem)
 {
-    const uint32_t B = nd.bins, tid = threadIdx.x, w = tid >> 6, lane = lane_id();
+    const uint32_t B = nd.bins, tid = ltid(), w = tid >> 6, lane = lane_id();
     uint32_t* tcnt = (uint32_t*)smem;
     for (uint32_t i = tid; i < B; i += kThreads) tcnt[i] = 0u;
     __syncthreads();
@@ -433,8 +442,10 @@ __device__ void count_tile(const gf_node_desc& nd, const Batch& b, uint32_t t, c
         if (v && (peers & lanemask_lt()) == 0ull) atomicAdd(&tcnt[bin], (uint32_t)__popcll(peers));
     }
     __syncthreads();
-    uint32_t* dst = b.counts + (size_t)t * B;
-    for (uint32_t i = tid; i < B; i += kThreads) dst[i] = tcnt[i];
+    if (t != kNoTile) {
+        uint32_t* dst = b.counts + (size_t)t * B;
+        for (uint32_t i = tid; i < B; i += kThreads) dst[i] = tcnt[i];
+    }
     __syncthreads();
 }
 
@@ -453,9 +464,10 @@ __host__ __device__ inline size_t scatter_smem(uint32_t bins, uint32_t tile_item
 // position f lives at slot out[f / C].index * C + f % C.
 template <class P, int R, bool Stage>
 __device__ void scatter_tile(const gf_node_desc& nd, const Batch& b, uint32_t t, const Cols& cols, uint32_t C,
-                             uint32_t* status, char* smem)
+                             uint32_t* status, char* smem, const gf_node_desc* all = nullptr,
+                             const int32_t* fuse = nullptr, const RunParams* rp = nullptr)
 {
-    const uint32_t B = nd.bins, tid = threadIdx.x, w = tid >> 6, lane = lane_id();
+    const uint32_t B = nd.bins, tid = ltid(), w = tid >> 6, lane = lane_id();
     uint32_t* wcnt = (uint32_t*)smem;      // [kWaves][B] running, then exclusive wave prefix
     uint32_t* ttot = wcnt + kWaves * B;     // [B] tile totals
     uint32_t* tstart = ttot + B;            // [B] exclusive prefix of ttot over bins
@@ -497,7 +509,7 @@ __device__ void scatter_tile(const gf_node_desc& nd, const Batch& b, uint32_t t,
         ttot[k] = acc;
     }
     __syncthreads();
-    const uint32_t* toff = b.counts + (size_t)t * B;
+    const uint32_t* toff = b.counts + (size_t)(t == kNoTile ? 0u : t) * B;
     if constexpr (Stage) {
         if (tid == 0) {
             uint32_t acc = 0;
@@ -513,6 +525,7 @@ __device__ void scatter_tile(const gf_node_desc& nd, const Batch& b, uint32_t t,
             uint32_t k = bin[r], within = wcnt[w * B + k] + rank[r];
             uint32_t u = update(nd, k, rec[r]);
             if (u) set_status(status, u);
+            if (fuse && fuse[k] >= 0) transform(all[fuse[k]], *rp, 2u * (uint32_t)fuse[k], rec[r]);
             uint32_t p = tstart[k] + within;
             stage[p] = rec[r];
             flat[p] = toff[k] + within;
@@ -533,6 +546,7 @@ __device__ void scatter_tile(const gf_node_desc& nd, const Batch& b, uint32_t t,
             P p = rec[r];
             uint32_t u = update(nd, k, p);
             if (u) set_status(status, u);
+            if (fuse && fuse[k] >= 0) transform(all[fuse[k]], *rp, 2u * (uint32_t)fuse[k], p);
             p.store(cols, (size_t)b.out[f / C].index * C + f % C);
         }
         __syncthreads();
@@ -546,8 +560,8 @@ __device__ void transform_tile(const gf_node_desc& nd, const Batch& b, uint32_t 
                                const RunParams& rp)
 {
     TileSpan s = tile_span(b, b.in, b.n_in, t, C);
-    const uint32_t pos0 = b.item_prefix[s.chunk] + s.offset;
-    for (uint32_t i = threadIdx.x; i < s.count; i += kThreads) {
+    const uint32_t pos0 = s.count ? b.item_prefix[s.chunk] + s.offset : 0u;
+    for (uint32_t i = ltid(); i < s.count; i += kThreads) {
         P p = P::load(cols, s.base + i);
         transform(nd, rp, b.purpose, p);
         uint32_t g = pos0 + i;
@@ -560,10 +574,11 @@ template <class P>
 __device__ void source_tile(const gf_node_desc& nd, const Batch& b, uint32_t t, const Cols& cols, uint32_t C,
                             const RunParams& rp)
 {
+    if (t == kNoTile) return;
     uint32_t off = t * b.tile_items;
     uint32_t count = b.particles - off < b.tile_items ? b.particles - off : b.tile_items;
     size_t base = (size_t)b.out[0].index * C + off;
-    for (uint32_t i = threadIdx.x; i < count; i += kThreads) {
+    for (uint32_t i = ltid(); i < count; i += kThreads) {
         P p;
         source(nd, rp, b.purpose, b.first_id + off + i, p);
         p.store(cols, base + i);
@@ -577,7 +592,7 @@ __device__ void sink_tile(const gf_node_desc& nd, const Batch& b, uint32_t t, co
 {
     TileSpan s = tile_span(b, b.in, b.n_in, t, C);
     if constexpr (std::is_same_v<P, Particle>) {
-        for (uint32_t i = threadIdx.x; i < s.count; i += kThreads) {
+        for (uint32_t i = ltid(); i < s.count; i += kThreads) {
             Particle p = Particle::load(cols, s.base + i);
             if (nd.op == GF_SNK_HIST) {
                 uint32_t v = nd.u[0] == 0u ? __float_as_uint(p.w) : p.gen;
@@ -591,9 +606,9 @@ __device__ void sink_tile(const gf_node_desc& nd, const Batch& b, uint32_t t, co
         }
     } else if constexpr (std::is_same_v<P, Keyed>) {
         if (nd.op != GF_SNK_CHUNK_DIGEST) return;
-        const gf_chunk_desc d = b.in[s.chunk];
+        const gf_chunk_desc d = s.count ? b.in[s.chunk] : gf_chunk_desc{0, 0, 0, 0};
         uint64_t acc = 0, msum = 0;
-        for (uint32_t i = threadIdx.x; i < s.count; i += kThreads) {
+        for (uint32_t i = ltid(); i < s.count; i += kThreads) {
             Keyed p = Keyed::load(cols, s.base + i);
             uint64_t h = keyed_hash(p);
             acc += mix64(h ^ ((uint64_t)(s.offset + i + 1u) * 0xD6E8FEB86659FD93ull));
@@ -606,11 +621,11 @@ __device__ void sink_tile(const gf_node_desc& nd, const Batch& b, uint32_t t, co
             msum += __shfl_down(msum, o);
         }
         if (lane_id() == 0) {
-            red[2 * (threadIdx.x >> 6)] = acc;
-            red[2 * (threadIdx.x >> 6) + 1] = msum;
+            red[2 * (ltid() >> 6)] = acc;
+            red[2 * (ltid() >> 6) + 1] = msum;
         }
         __syncthreads();
-        if (threadIdx.x == 0) {
+        if (ltid() == 0 && s.count) {
             uint64_t a = 0, m = 0;
             for (int ww = 0; ww < kWaves; ++ww) {
                 a += red[2 * ww];

@@ -10,42 +10,76 @@
 //       credit-fitting prefix of <= Q ready chunks per transform node, all ready chunks per sink).
 // The scheduling rules are those of GraphRunner::run (sched/GraphRunner.hpp:902-1133) applied
 // once per node per step; the host only replays a hipGraph of K steps and polls a done flag.
+//
+// The last-workgroup phases are latency bound: they keep the whole control state in LDS and
+// touch global memory in a constant number of parallel round trips per step.
 #pragma once
 
 #include "gf_device.hpp"
 
 namespace gfd {
 
+constexpr uint32_t kMaxNodes = 64;
+constexpr uint32_t kListCap = 2048;   // chunk descriptors staged in LDS by the select phase
+constexpr uint32_t kPlanWords = 8192; // [tile][bin] words of one partition planned in LDS
+
+// Dynamic per-node state (64 bytes), rewritten by the last workgroups.
+struct NodeSt {
+    uint32_t active, n_in, n_tiles, particles;
+    uint32_t n_out, pad0, pad1, pad2;
+    unsigned long long first_id, digest_base, head, tail;
+};
+
+// Static per-node arrays and shape.
+struct NodeArr {
+    gf_chunk_desc* in;
+    uint32_t* tile_prefix;
+    uint32_t* item_prefix;
+    gf_chunk_desc* out;
+    uint32_t* counts;
+    uint32_t* totals;
+    uint32_t tile_items, purpose, cap_in, cap_out, reserve_bound, kind;  // kind: 0 src, 1 part, 2 xf, 3 sink
+};
+
+enum { KIND_SOURCE = 0, KIND_PARTITION = 1, KIND_TRANSFORM = 2, KIND_SINK = 3 };
+
 struct Engine {
-    uint32_t n_nodes, C, P, Q, S, ring_cap, grid, pad;
+    uint32_t n_nodes, C, P, Q, S, ring_cap, trace_on, pad;
     uint64_t N;
     RunParams rp;
     Cols cols;
     const gf_node_desc* nodes;
-    const int32_t* routes;
-    const uint32_t* reserve_bound; // per node PartitionStage/TransformStage reserve bound
-    Batch* batches;                // per node
-    uint32_t* free_stack;          // S
-    uint32_t* gens;                // S
-    gf_chunk_desc* rings;          // n_nodes * ring_cap
-    unsigned long long* ring_head; // n_nodes
-    unsigned long long* ring_tail; // n_nodes
-    uint32_t* ka_prefix;           // n_nodes + 1
-    uint32_t* kb_prefix;           // n_nodes + 1
-    uint32_t* ctl;                 // control words, see CTL_*
-    unsigned long long* ctr;       // counters, see CNT_*
+    const int32_t* routes;  // effective route per (node, bin), fused transforms bypassed
+    const int32_t* fuse;    // per (node, bin): transform node applied inside the scatter, or -1
+    const NodeArr* arr;
+    NodeSt* st;
+    unsigned long long* free_stack;  // entry = index | generation << 32
+    gf_chunk_desc* rings;            // n_nodes * ring_cap
+    uint32_t* ka_prefix;             // n_nodes + 1
+    uint32_t* kb_prefix;             // n_nodes + 1
+    uint32_t* ctl;                   // control words, CTL_*
+    unsigned long long* ctr;         // counters, CNT_*
     uint32_t* trace;
     unsigned long long trace_cap;
+    unsigned long long* prof;  // optional per-phase wall-clock totals (micro engine), PROF_*
 };
+
+enum { PROF_COUNT = 0, PROF_PLAN = 1, PROF_SOURCE = 2, PROF_PART = 3, PROF_XF = 4, PROF_SINK = 5, PROF_RETIRE = 6,
+       PROF_SELECT = 7, PROF_STEPS = 8, PROF_WORDS = 16 };
 
 enum { CTL_STATUS = 0, CTL_DONE = 1, CTL_TOP = 2, CTL_TICKET_A = 3, CTL_TICKET_B = 4, CTL_WORDS = 8 };
 enum {
     CNT_GENERATED = 0, CNT_SUNK = 1, CNT_STEP = 2, CNT_BATCHES = 3, CNT_PART_ITEMS = 4, CNT_XF_ITEMS = 5,
-    CNT_SRC_ITEMS = 6, CNT_SINK_ITEMS = 7, CNT_TRACE_LEN = 8, CNT_PEAK_LIVE = 9, CNT_DIGEST0 = 16,
-    CNT_WORDS = 16 + 64
+    CNT_SRC_ITEMS = 6, CNT_SINK_ITEMS = 7, CNT_TRACE_LEN = 8, CNT_PEAK_LIVE = 9, CNT_FUSED_ITEMS = 10,
+    CNT_DIGEST0 = 16, CNT_WORDS = 16 + kMaxNodes
 };
 
+constexpr uint32_t kMaxWaves = 16;  // last-workgroup phases run with 256 or 1024 threads
+__device__ inline uint32_t bsz() { return blockDim.x; }
+__device__ inline uint32_t bwaves() { return blockDim.x >> 6; }
+
 __device__ inline bool is_partition_op(uint32_t op) { return op >= GF_CLS_WEIGHT_ROUTE && op < GF_XF_SCALE_WEIGHT; }
+__device__ inline uint32_t load_ctl(uint32_t* p) { return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT); }
 
 // Block-wide exclusive scan of one value per thread; returns the exclusive prefix, *total gets the sum.
 __device__ inline uint32_t block_exscan(uint32_t v, uint32_t* sh, uint32_t* total)
@@ -59,8 +93,8 @@ __device__ inline uint32_t block_exscan(uint32_t v, uint32_t* sh, uint32_t* tota
     if (lane == 63) sh[w] = x;
     __syncthreads();
     uint32_t off = 0, tot = 0;
-    for (int k = 0; k < kWaves; ++k) {
-        if ((uint32_t)k < w) off += sh[k];
+    for (uint32_t k = 0; k < bwaves(); ++k) {
+        if (k < w) off += sh[k];
         tot += sh[k];
     }
     __syncthreads();
@@ -68,11 +102,11 @@ __device__ inline uint32_t block_exscan(uint32_t v, uint32_t* sh, uint32_t* tota
     return off + x - v;
 }
 
-// Exclusive scan of n values at a[0..n) in place (one workgroup), returns the total.
+// Exclusive scan of n values a[0..n) in place (one workgroup, any memory), returns the total.
 __device__ inline uint32_t block_exscan_array(uint32_t* a, uint32_t n, uint32_t* sh)
 {
     const uint32_t tid = threadIdx.x;
-    const uint32_t per = (n + kThreads - 1u) / kThreads;
+    const uint32_t per = (n + bsz() - 1u) / bsz();
     const uint32_t lo = tid * per, hi = lo + per < n ? lo + per : n;
     uint32_t s = 0;
     for (uint32_t i = lo; i < hi; ++i) s += a[i];
@@ -87,12 +121,61 @@ __device__ inline uint32_t block_exscan_array(uint32_t* a, uint32_t n, uint32_t*
     return total;
 }
 
+// Wave-wide inclusive scan.
+__device__ inline uint32_t wave_inscan(uint32_t x)
+{
+    const uint32_t lane = lane_id();
+    for (int o = 1; o < 64; o <<= 1) {
+        uint32_t y = (uint32_t)__shfl_up((int)x, o);
+        if (lane >= (uint32_t)o) x += y;
+    }
+    return x;
+}
+
+__device__ inline Batch make_batch(const Engine& e, uint32_t k, const NodeSt& s)
+{
+    const NodeArr& a = e.arr[k];
+    Batch b;
+    b.active = s.active;
+    b.n_in = s.n_in;
+    b.n_tiles = s.n_tiles;
+    b.n_out = s.n_out;
+    b.particles = s.particles;
+    b.reserved = 0;
+    b.tile_items = a.tile_items;
+    b.purpose = a.purpose;
+    b.first_id = s.first_id;
+    b.digest_base = s.digest_base;
+    b.in = a.in;
+    b.tile_prefix = a.tile_prefix;
+    b.item_prefix = a.item_prefix;
+    b.out = a.out;
+    b.counts = a.counts;
+    b.totals = a.totals;
+    return b;
+}
+
+__device__ inline void load_states(const Engine& e, NodeSt* sh)
+{
+    const uint32_t words = e.n_nodes * (uint32_t)(sizeof(NodeSt) / 4u);
+    const uint32_t* src = (const uint32_t*)e.st;
+    uint32_t* dst = (uint32_t*)sh;
+    for (uint32_t i = threadIdx.x; i < words; i += bsz()) dst[i] = src[i];
+}
+
+__device__ inline void store_states(const Engine& e, const NodeSt* sh)
+{
+    const uint32_t words = e.n_nodes * (uint32_t)(sizeof(NodeSt) / 4u);
+    uint32_t* dst = (uint32_t*)e.st;
+    const uint32_t* src = (const uint32_t*)sh;
+    for (uint32_t i = threadIdx.x; i < words; i += bsz()) dst[i] = src[i];
+}
+
 // Trace record (layout identical to the oracle): 0xB47C0000|node, step, n_in, (index,gen,fill)*n_in,
 // nbins, counts, R, (index,gen,fill,bin)*R.
 __device__ inline void trace_record(const Engine& e, uint32_t k, const Batch& b, uint32_t nbins,
-                                    const uint32_t* counts, unsigned long long* len_sh)
+                                    const uint32_t* counts, unsigned long long* len_sh, uint32_t step)
 {
-    if (!e.trace) return;
     const uint32_t tid = threadIdx.x;
     unsigned long long len = (unsigned long long)5u + 3ull * b.n_in + nbins + 4ull * b.n_out;
     __syncthreads();
@@ -107,19 +190,19 @@ __device__ inline void trace_record(const Engine& e, uint32_t k, const Batch& b,
     };
     if (tid == 0) {
         put(0, 0xB47C0000u | k);
-        put(1, (uint32_t)e.ctr[CNT_STEP]);
+        put(1, step);
         put(2, b.n_in);
         put(3 + 3ull * b.n_in, nbins);
         put(4 + 3ull * b.n_in + nbins, b.n_out);
     }
-    for (uint32_t i = tid; i < b.n_in; i += kThreads) {
+    for (uint32_t i = tid; i < b.n_in; i += bsz()) {
         put(3 + 3ull * i, b.in[i].index);
         put(4 + 3ull * i, b.in[i].generation);
         put(5 + 3ull * i, b.in[i].fill);
     }
-    for (uint32_t i = tid; i < nbins; i += kThreads) put(4 + 3ull * b.n_in + i, counts[i]);
+    for (uint32_t i = tid; i < nbins; i += bsz()) put(4 + 3ull * b.n_in + i, counts[i]);
     unsigned long long o0 = 5 + 3ull * b.n_in + nbins;
-    for (uint32_t i = tid; i < b.n_out; i += kThreads) {
+    for (uint32_t i = tid; i < b.n_out; i += bsz()) {
         put(o0 + 4ull * i, b.out[i].index);
         put(o0 + 4ull * i + 1, b.out[i].generation);
         put(o0 + 4ull * i + 2, b.out[i].fill);
@@ -128,207 +211,228 @@ __device__ inline void trace_record(const Engine& e, uint32_t k, const Batch& b,
     __syncthreads();
 }
 
-__device__ inline uint32_t load_ctl(uint32_t* p) { return atomicAdd(p, 0u); }
-
-struct PlanShared {
-    uint32_t scan[kWaves];
-    uint32_t top;
-    uint32_t ok;
-    uint32_t stop;
-    uint32_t R;
-    uint32_t sum;
+// ------------------------------------------------------------------------------------------
+// KA last workgroup: plan
+// ------------------------------------------------------------------------------------------
+struct PlanLds {
+    NodeSt st[kMaxNodes];
+    uint32_t R[kMaxNodes];
+    uint32_t off[kMaxNodes];
+    uint32_t tb_off[kMaxNodes];  // offset of node k's [tot(B), base(B)] in `words`, or ~0u when in global
+    uint32_t scan[kMaxWaves];
+    uint32_t top, status, fail, total_R, words_used;
     unsigned long long len;
+    unsigned long long digest[kMaxNodes];
+    uint32_t words[kPlanWords + 1024];
 };
 
-// Plan of one partition batch (Execution.hpp:2520-2642 on the device): per (tile, bin) flat
-// output start, per bin totals (b.totals[0..B)), bin bases (b.totals[B..2B)), exact chunk count R,
-// pops from the free stack in (bin, k) order, output descriptors.
-__device__ inline void plan_partition(const Engine& e, uint32_t k, Batch& b, PlanShared& ps)
+// Per-(tile, bin) exclusive prefix over tiles, totals and bin bases of one partition, in LDS when
+// the [tile][bin] matrix fits, else in global memory.
+__device__ inline void plan_partition_counts(const Engine& e, uint32_t k, PlanLds& L)
 {
     const gf_node_desc& nd = e.nodes[k];
-    const uint32_t B = nd.bins, nt = b.n_tiles, tid = threadIdx.x, C = e.C;
-    uint32_t* cnt = b.counts;
-    uint32_t* tot = b.totals;
-    uint32_t* base = b.totals + B;
-    if (B < 64u) {
-        for (uint32_t bin = 0; bin < B; ++bin) {
-            const uint32_t per = (nt + kThreads - 1u) / kThreads;
-            const uint32_t lo = tid * per, hi = lo + per < nt ? lo + per : nt;
-            uint32_t s = 0;
-            for (uint32_t t = lo; t < hi; ++t) s += cnt[(size_t)t * B + bin];
-            uint32_t total;
-            uint32_t run = block_exscan(s, ps.scan, &total);
-            for (uint32_t t = lo; t < hi; ++t) {
-                uint32_t x = cnt[(size_t)t * B + bin];
-                cnt[(size_t)t * B + bin] = run;
-                run += x;
+    const NodeArr& a = e.arr[k];
+    const uint32_t B = nd.bins, nt = L.st[k].n_tiles, tid = threadIdx.x, C = e.C;
+    const uint32_t w = tid >> 6, lane = lane_id();
+    const uint32_t need = nt * B;
+    uint32_t* tot;
+    uint32_t* base;
+    if (need <= kPlanWords && L.words_used + 2u * B <= 1024u) {
+        uint32_t* m = L.words;  // [tile][bin]
+        for (uint32_t i = tid; i < need; i += bsz()) m[i] = a.counts[i];
+        tot = L.words + kPlanWords + L.words_used;
+        base = tot + B;
+        __syncthreads();
+        // one wave per bin, 64 tiles per iteration with a carried prefix
+        for (uint32_t bin = w; bin < B; bin += bwaves()) {
+            uint32_t carry = 0;
+            for (uint32_t t0 = 0; t0 < nt; t0 += 64u) {
+                uint32_t t = t0 + lane;
+                uint32_t x = t < nt ? m[t * B + bin] : 0u;
+                uint32_t inc = wave_inscan(x);
+                if (t < nt) m[t * B + bin] = carry + inc - x;
+                carry += (uint32_t)__shfl((int)inc, 63);
             }
-            if (tid == 0) tot[bin] = total;
-            __syncthreads();
+            if (lane == 0) tot[bin] = carry;
+        }
+        __syncthreads();
+        if (tid == 0) {
+            L.tb_off[k] = kPlanWords + L.words_used;
+            L.words_used += 2u * B;
         }
     } else {
-        for (uint32_t bin = tid; bin < B; bin += kThreads) {
+        tot = a.totals;
+        base = a.totals + B;
+        for (uint32_t bin = tid; bin < B; bin += bsz()) {
             uint32_t acc = 0;
             for (uint32_t t = 0; t < nt; ++t) {
-                uint32_t x = cnt[(size_t)t * B + bin];
-                cnt[(size_t)t * B + bin] = acc;
+                uint32_t x = a.counts[(size_t)t * B + bin];
+                a.counts[(size_t)t * B + bin] = acc;
                 acc += x;
             }
             tot[bin] = acc;
         }
+        if (tid == 0) L.tb_off[k] = ~0u;
         __syncthreads();
     }
-    // checked totals and exact chunk counts
     uint32_t s = 0;
-    for (uint32_t bin = tid; bin < B; bin += kThreads) {
+    for (uint32_t bin = tid; bin < B; bin += bsz()) {
         s += tot[bin];
         base[bin] = (tot[bin] + C - 1u) / C;
     }
     uint32_t sum;
-    block_exscan(s, ps.scan, &sum);
-    uint32_t R = block_exscan_array(base, B, ps.scan);
+    block_exscan(s, L.scan, &sum);
+    uint32_t R = block_exscan_array(base, B, L.scan);
     if (tid == 0) {
-        ps.ok = 1u;
-        if (sum != b.particles) {
+        L.R[k] = R;
+        if (sum != L.st[k].particles || R > a.reserve_bound) {
             set_status(&e.ctl[CTL_STATUS], GF_PLAN_MISMATCH);
-            ps.ok = 0u;
-        } else if (R > e.reserve_bound[k]) {
-            set_status(&e.ctl[CTL_STATUS], GF_PLAN_MISMATCH);
-            ps.ok = 0u;
-        } else if (R > ps.top) {
-            set_status(&e.ctl[CTL_STATUS], GF_POOL_EXHAUSTED);
-            ps.ok = 0u;
-        }
-        ps.R = R;
-    }
-    __syncthreads();
-    if (!ps.ok) return;
-    const uint32_t top = ps.top;
-    for (uint32_t bin = tid; bin < B; bin += kThreads) {
-        uint32_t nch = (tot[bin] + C - 1u) / C;
-        for (uint32_t j = 0; j < nch; ++j) {
-            uint32_t r = base[bin] + j;
-            uint32_t idx = e.free_stack[top - 1u - r];
-            uint32_t rem = tot[bin] - j * C;
-            b.out[r] = gf_chunk_desc{idx, e.gens[idx], rem < C ? rem : C, bin};
+            L.fail = 1u;
         }
     }
-    const size_t ntb = (size_t)nt * B;
-    for (size_t i = tid; i < ntb; i += kThreads) cnt[i] += base[i % B] * C;
-    __syncthreads();
-    if (tid == 0) {
-        ps.top = top - R;
-        b.n_out = R;
+    // flat output start of every (tile, bin) and the totals for the trace
+    if (need <= kPlanWords && L.tb_off[k] != ~0u) {
+        const uint32_t* m = L.words;
+        for (uint32_t i = tid; i < need; i += bsz()) a.counts[i] = m[i] + base[i % B] * C;
+        for (uint32_t bin = tid; bin < B; bin += bsz()) a.totals[bin] = tot[bin];
+    } else {
+        for (size_t i = tid; i < (size_t)need; i += bsz()) a.counts[i] += base[i % B] * C;
     }
     __syncthreads();
 }
 
-__device__ inline void plan_transform(const Engine& e, uint32_t k, Batch& b, PlanShared& ps)
+__device__ inline void engine_plan(const Engine& e, PlanLds& L)
 {
-    const uint32_t tid = threadIdx.x, C = e.C;
-    const uint32_t R = (b.particles + C - 1u) / C;
+    const uint32_t tid = threadIdx.x, n = e.n_nodes, C = e.C;
+    load_states(e, L.st);
     if (tid == 0) {
-        ps.ok = 1u;
-        if (b.particles == 0u || R > e.reserve_bound[k]) {
-            set_status(&e.ctl[CTL_STATUS], GF_PLAN_MISMATCH);
-            ps.ok = 0u;
-        } else if (R > ps.top) {
-            set_status(&e.ctl[CTL_STATUS], GF_POOL_EXHAUSTED);
-            ps.ok = 0u;
+        L.top = e.ctl[CTL_TOP];
+        L.status = load_ctl(&e.ctl[CTL_STATUS]);
+        L.fail = 0u;
+        L.words_used = 0u;
+    }
+    for (uint32_t k = tid; k < n; k += bsz()) L.digest[k] = e.ctr[CNT_DIGEST0 + k];
+    __syncthreads();
+    if (L.status) return;
+    // pass 1: exact output chunk count of every node, in node order
+    for (uint32_t k = 0; k < n; ++k) {
+        if (!L.st[k].active) {
+            if (tid == 0) L.R[k] = 0u;
+            continue;
         }
-    }
-    __syncthreads();
-    if (!ps.ok) return;
-    const uint32_t top = ps.top;
-    for (uint32_t r = tid; r < R; r += kThreads) {
-        uint32_t idx = e.free_stack[top - 1u - r];
-        uint32_t rem = b.particles - r * C;
-        b.out[r] = gf_chunk_desc{idx, e.gens[idx], rem < C ? rem : C, 0u};
-    }
-    if (tid == 0) {
-        b.totals[0] = b.particles;
-        ps.top = top - R;
-        b.n_out = R;
-    }
-    __syncthreads();
-}
-
-// Last workgroup of KA.
-__device__ inline void engine_plan(const Engine& e, PlanShared& ps)
-{
-    const uint32_t tid = threadIdx.x;
-    if (tid == 0) ps.top = e.ctl[CTL_TOP];
-    __syncthreads();
-    for (uint32_t k = 0; k < e.n_nodes; ++k) {
-        Batch& b = e.batches[k];
-        if (!b.active) continue;
-        if (tid == 0) ps.stop = load_ctl(&e.ctl[CTL_STATUS]) != 0u ? 1u : 0u;
-        __syncthreads();
-        if (ps.stop) break;
-        const gf_node_desc& nd = e.nodes[k];
-        if (nd.role == GF_ROLE_SOURCE) {
-            if (tid == 0) {
-                if (ps.top == 0u) {
-                    set_status(&e.ctl[CTL_STATUS], GF_POOL_EXHAUSTED);
-                } else {
-                    uint32_t idx = e.free_stack[--ps.top];
-                    b.out[0] = gf_chunk_desc{idx, e.gens[idx], b.particles, 0u};
-                    b.n_out = 1u;
+        const uint32_t kind = e.arr[k].kind;
+        if (kind == KIND_PARTITION) {
+            plan_partition_counts(e, k, L);
+        } else if (tid == 0) {
+            if (kind == KIND_SOURCE) {
+                L.R[k] = 1u;
+            } else if (kind == KIND_SINK) {
+                L.R[k] = 0u;
+                L.st[k].digest_base = L.digest[k];
+                L.digest[k] += L.st[k].n_in;
+            } else {
+                uint32_t R = (L.st[k].particles + C - 1u) / C;
+                L.R[k] = R;
+                if (L.st[k].particles == 0u || R > e.arr[k].reserve_bound) {
+                    set_status(&e.ctl[CTL_STATUS], GF_PLAN_MISMATCH);
+                    L.fail = 1u;
                 }
             }
-            __syncthreads();
-            trace_record(e, k, b, 0u, nullptr, &ps.len);
-        } else if (nd.role == GF_ROLE_SINK) {
-            if (tid == 0) {
-                b.n_out = 0u;
-                b.digest_base = e.ctr[CNT_DIGEST0 + (k & 63u)];
-                e.ctr[CNT_DIGEST0 + (k & 63u)] += b.n_in;
-            }
-            __syncthreads();
-            trace_record(e, k, b, 0u, nullptr, &ps.len);
-        } else if (is_partition_op(nd.op)) {
-            plan_partition(e, k, b, ps);
-            if (ps.ok) trace_record(e, k, b, nd.bins, b.totals, &ps.len);
-        } else {
-            plan_transform(e, k, b, ps);
-            if (ps.ok) trace_record(e, k, b, 1u, b.totals, &ps.len);
         }
         __syncthreads();
     }
     if (tid == 0) {
-        e.ctl[CTL_TOP] = ps.top;
-        uint32_t live = e.S - ps.top;
+        uint32_t acc = 0;
+        for (uint32_t k = 0; k < n; ++k) {
+            L.off[k] = acc;
+            acc += L.R[k];
+        }
+        L.total_R = acc;
+        if (!L.fail && acc > L.top) {
+            set_status(&e.ctl[CTL_STATUS], GF_POOL_EXHAUSTED);
+            L.fail = 1u;
+        }
+    }
+    __syncthreads();
+    if (L.fail) return;
+    // pass 2: every pop of the step in one parallel round trip (node order, bin-major, k order)
+    const uint32_t top = L.top;
+    for (uint32_t j = tid; j < L.total_R; j += bsz()) {
+        uint32_t k = 0;
+        while (k + 1u < n && L.off[k + 1u] <= j) ++k;
+        const uint32_t r = j - L.off[k];
+        unsigned long long ent = e.free_stack[top - 1u - j];
+        gf_chunk_desc d;
+        d.index = (uint32_t)ent;
+        d.generation = (uint32_t)(ent >> 32);
+        const uint32_t kind = e.arr[k].kind;
+        if (kind == KIND_SOURCE) {
+            d.fill = L.st[k].particles;
+            d.bin = 0u;
+        } else if (kind == KIND_TRANSFORM) {
+            uint32_t rem = L.st[k].particles - r * C;
+            d.fill = rem < C ? rem : C;
+            d.bin = 0u;
+        } else {
+            const uint32_t B = e.nodes[k].bins;
+            const uint32_t* tot = L.tb_off[k] != ~0u ? L.words + L.tb_off[k] : e.arr[k].totals;
+            const uint32_t* base = tot + B;
+            uint32_t lo = 0, hi = B;  // largest bin with base[bin] <= r and a non-empty range
+            while (hi - lo > 1u) {
+                uint32_t mid = (lo + hi) >> 1;
+                if (base[mid] <= r) lo = mid;
+                else hi = mid;
+            }
+            while (lo + 1u < B && base[lo + 1u] <= r) ++lo;
+            uint32_t rem = tot[lo] - (r - base[lo]) * C;
+            d.fill = rem < C ? rem : C;
+            d.bin = lo;
+        }
+        e.arr[k].out[r] = d;
+    }
+    for (uint32_t k = tid; k < n; k += bsz()) {
+        L.st[k].n_out = L.R[k];
+        if (e.arr[k].kind == KIND_TRANSFORM) e.arr[k].totals[0] = L.st[k].particles;
+        e.ctr[CNT_DIGEST0 + k] = L.digest[k];
+    }
+    __syncthreads();
+    store_states(e, L.st);
+    if (tid == 0) {
+        uint32_t nt = top - L.total_R;
+        e.ctl[CTL_TOP] = nt;
+        unsigned long long live = e.S - nt;
         if (live > e.ctr[CNT_PEAK_LIVE]) e.ctr[CNT_PEAK_LIVE] = live;
     }
-    __syncthreads();
-}
-
-// Tile prefix / item prefix of a node's input list (one workgroup).
-__device__ inline void build_prefixes(Batch& b, uint32_t tile_items, uint32_t* sh)
-{
-    const uint32_t tid = threadIdx.x, n = b.n_in;
-    for (uint32_t i = tid; i < n; i += kThreads) {
-        b.tile_prefix[i] = (b.in[i].fill + tile_items - 1u) / tile_items;
-        b.item_prefix[i] = b.in[i].fill;
-    }
-    __syncthreads();
-    uint32_t nt = block_exscan_array(b.tile_prefix, n, sh);
-    uint32_t ni = block_exscan_array(b.item_prefix, n, sh);
-    if (tid == 0) {
-        b.tile_prefix[n] = nt;
-        b.item_prefix[n] = ni;
-        b.n_tiles = nt;
-        b.particles = ni;
+    if (e.trace_on) {
+        __syncthreads();
+        const uint32_t step = (uint32_t)e.ctr[CNT_STEP];
+        for (uint32_t k = 0; k < n; ++k) {
+            if (!L.st[k].active) continue;
+            const uint32_t kind = e.arr[k].kind;
+            Batch b = make_batch(e, k, L.st[k]);
+            if (kind == KIND_PARTITION) trace_record(e, k, b, e.nodes[k].bins, e.arr[k].totals, &L.len, step);
+            else if (kind == KIND_TRANSFORM) trace_record(e, k, b, 1u, e.arr[k].totals, &L.len, step);
+            else trace_record(e, k, b, 0u, nullptr, &L.len, step);
+        }
     }
     __syncthreads();
 }
 
-struct SelectShared {
-    uint32_t scan[kWaves];
-    uint32_t reserved;
-    uint32_t any;
-    uint32_t wanted;
-    unsigned long long head;
+// ------------------------------------------------------------------------------------------
+// KB last workgroup: retire + select
+// ------------------------------------------------------------------------------------------
+struct SelLds {
+    NodeSt st[kMaxNodes];
+    unsigned long long tail[kMaxNodes];
+    unsigned long long head[kMaxNodes];
+    uint32_t off[kMaxNodes + 1];
+    uint32_t lst_off[kMaxNodes + 1];  // staged ready descriptors of node k: lst[lst_off[k] ..)
+    uint32_t wanted[kMaxNodes];
+    uint32_t wcnt[kMaxWaves][kMaxNodes];
+    uint32_t scan[kMaxWaves];
+    uint32_t top, total, any, staged, overflow;
+    unsigned long long generated, sunk, step, batches, part_items, xf_items, src_items, sink_items;
+    gf_chunk_desc lst[kListCap];
 };
 
 __device__ inline uint64_t bound_of(uint64_t n, uint64_t C, uint64_t B)
@@ -340,156 +444,266 @@ __device__ inline uint64_t bound_of(uint64_t n, uint64_t C, uint64_t B)
     return r < n ? r : n;
 }
 
-// Last workgroup of KB, part 1: retire the current step in node order.
-__device__ inline void engine_retire(const Engine& e)
+__device__ inline void engine_retire(const Engine& e, SelLds& L)
 {
-    const uint32_t tid = threadIdx.x, lane = lane_id();
-    // ---- retire in node order
-    for (uint32_t k = 0; k < e.n_nodes; ++k) {
-        Batch& b = e.batches[k];
-        if (!b.active) continue;
-        const gf_node_desc& nd = e.nodes[k];
-        if (nd.role != GF_ROLE_SINK && tid < 64u) {
-            for (uint32_t o0 = 0; o0 < b.n_out; o0 += 64u) {
-                uint32_t o = o0 + lane;
-                bool v = o < b.n_out;
-                gf_chunk_desc d = v ? b.out[o] : gf_chunk_desc{0, 0, 0, 0};
-                uint32_t dst = v ? (uint32_t)e.routes[nd.route_offset + d.bin] : 0xFFFFFFFFu;
-                uint64_t peers = peer_mask(v, dst);
-                unsigned long long cur = v ? e.ring_tail[dst] : 0ull;
-                unsigned long long pos = cur + (unsigned long long)__popcll(peers & lanemask_lt());
-                if (v) e.rings[(size_t)dst * e.ring_cap + (size_t)(pos % e.ring_cap)] = d;
-                if (v && (peers & lanemask_lt()) == 0ull) e.ring_tail[dst] = cur + (unsigned long long)__popcll(peers);
-            }
+    const uint32_t tid = threadIdx.x, n = e.n_nodes, w = tid >> 6;
+    load_states(e, L.st);
+    if (tid == 0) {
+        L.top = e.ctl[CTL_TOP];
+        L.generated = e.ctr[CNT_GENERATED];
+        L.sunk = e.ctr[CNT_SUNK];
+        L.step = e.ctr[CNT_STEP];
+        L.batches = e.ctr[CNT_BATCHES];
+        L.part_items = e.ctr[CNT_PART_ITEMS];
+        L.xf_items = e.ctr[CNT_XF_ITEMS];
+        L.src_items = e.ctr[CNT_SRC_ITEMS];
+        L.sink_items = e.ctr[CNT_SINK_ITEMS];
+    }
+    __syncthreads();
+    for (uint32_t k = tid; k < n; k += bsz()) {
+        L.tail[k] = L.st[k].tail;
+        L.head[k] = L.st[k].head;
+    }
+    // outputs to route: node order, output order
+    if (tid == 0) {
+        uint32_t acc = 0;
+        for (uint32_t k = 0; k < n; ++k) {
+            L.off[k] = acc;
+            if (L.st[k].active && e.arr[k].kind != KIND_SINK) acc += L.st[k].n_out;
         }
-        if (nd.role != GF_ROLE_SOURCE) {
-            const uint32_t top = e.ctl[CTL_TOP];
-            for (uint32_t i = tid; i < b.n_in; i += kThreads) {
-                uint32_t idx = b.in[i].index;
-                e.gens[idx] += 1u;
-                e.free_stack[top + i] = idx;
-            }
-            __syncthreads();
-            if (tid == 0) e.ctl[CTL_TOP] = top + b.n_in;
+        L.off[n] = acc;
+        L.total = acc;
+    }
+    __syncthreads();
+    // route in chunks of 256 with an order-preserving per-destination rank
+    for (uint32_t j0 = 0; j0 < L.total; j0 += bsz()) {
+        const uint32_t j = j0 + tid;
+        const bool v = j < L.total;
+        gf_chunk_desc d{0, 0, 0, 0};
+        uint32_t dst = 0xFFFFFFFFu;
+        if (v) {
+            uint32_t k = 0;
+            while (k + 1u < n && L.off[k + 1u] <= j) ++k;
+            d = e.arr[k].out[j - L.off[k]];
+            dst = (uint32_t)e.routes[e.nodes[k].route_offset + d.bin];
         }
-        if (tid == 0) {
-            if (nd.role == GF_ROLE_SINK) {
-                e.ctr[CNT_SUNK] += b.particles;
-                e.ctr[CNT_SINK_ITEMS] += b.particles;
-            } else if (nd.role == GF_ROLE_SOURCE) {
-                e.ctr[CNT_SRC_ITEMS] += b.particles;
-            } else {
-                e.ctr[CNT_BATCHES] += 1u;
-                if (is_partition_op(nd.op)) e.ctr[CNT_PART_ITEMS] += b.particles;
-                else e.ctr[CNT_XF_ITEMS] += b.particles;
-            }
-            b.active = 0u;
+        for (uint32_t i = tid; i < kMaxWaves * kMaxNodes; i += bsz()) (&L.wcnt[0][0])[i] = 0u;
+        __syncthreads();
+        uint64_t peers = peer_mask(v, dst);
+        const uint32_t rank = (uint32_t)__popcll(peers & lanemask_lt());
+        if (v && (peers & lanemask_lt()) == 0ull) L.wcnt[w][dst] = (uint32_t)__popcll(peers);
+        __syncthreads();
+        if (v) {
+            uint32_t pre = 0;
+            for (uint32_t ww = 0; ww < w; ++ww) pre += L.wcnt[ww][dst];
+            unsigned long long pos = L.tail[dst] + pre + rank;
+            e.rings[(size_t)dst * e.ring_cap + (size_t)(pos % e.ring_cap)] = d;
+        }
+        __syncthreads();
+        for (uint32_t k = tid; k < n; k += bsz()) {
+            uint32_t sum = 0;
+            for (uint32_t ww = 0; ww < bwaves(); ++ww) sum += L.wcnt[ww][k];
+            L.tail[k] += sum;
         }
         __syncthreads();
     }
-    if (tid == 0) e.ctr[CNT_STEP] += 1u;
+    // release every consumed input to the free stack (generation + 1)
+    if (tid == 0) {
+        uint32_t acc = 0;
+        for (uint32_t k = 0; k < n; ++k) {
+            L.off[k] = acc;
+            if (L.st[k].active && e.arr[k].kind != KIND_SOURCE) acc += L.st[k].n_in;
+        }
+        L.off[n] = acc;
+        L.total = acc;
+    }
+    __syncthreads();
+    for (uint32_t j = tid; j < L.total; j += bsz()) {
+        uint32_t k = 0;
+        while (k + 1u < n && L.off[k + 1u] <= j) ++k;
+        gf_chunk_desc d = e.arr[k].in[j - L.off[k]];
+        e.free_stack[L.top + j] = (unsigned long long)d.index | ((unsigned long long)(d.generation + 1u) << 32);
+    }
+    __syncthreads();
+    if (tid == 0) {
+        L.top += L.total;
+        for (uint32_t k = 0; k < n; ++k) {
+            if (!L.st[k].active) continue;
+            const uint32_t kind = e.arr[k].kind;
+            if (kind == KIND_SINK) {
+                L.sunk += L.st[k].particles;
+                L.sink_items += L.st[k].particles;
+            } else if (kind == KIND_SOURCE) {
+                L.src_items += L.st[k].particles;
+            } else {
+                L.batches += 1u;
+                if (kind == KIND_PARTITION) L.part_items += L.st[k].particles;
+                else L.xf_items += L.st[k].particles;
+            }
+            L.st[k].active = 0u;
+        }
+        L.step += 1u;
+    }
     __syncthreads();
 }
 
-// Last workgroup of KB, part 2 (and the setup kernel): select the next step in node order.
-__device__ inline void engine_select(const Engine& e, SelectShared& ss)
+__device__ inline void engine_select(const Engine& e, SelLds& L)
 {
-    const uint32_t tid = threadIdx.x;
-    const uint32_t C = e.C;
+    const uint32_t tid = threadIdx.x, n = e.n_nodes, C = e.C, P = e.P, Q = e.Q;
+    // stage the candidate ready descriptors of every non-source node: <= Q per transform, all per sink
     if (tid == 0) {
-        ss.reserved = 0u;
-        ss.any = 0u;
+        uint32_t acc = 0;
+        L.overflow = 0u;
+        for (uint32_t k = 0; k < n; ++k) {
+            const uint32_t kind = e.arr[k].kind;
+            unsigned long long avail = L.tail[k] - L.head[k];
+            uint32_t want = kind == KIND_SOURCE ? 0u : kind == KIND_SINK ? (uint32_t)avail : (avail < Q ? (uint32_t)avail : Q);
+            L.lst_off[k] = acc;
+            L.wanted[k] = want;
+            acc += want;
+        }
+        L.lst_off[n] = acc;
+        L.staged = acc;
+        if (acc > kListCap) L.overflow = 1u;
     }
     __syncthreads();
-    const uint32_t P = e.P, Q = e.Q;
-    for (uint32_t k = 0; k < e.n_nodes; ++k) {
-        Batch& b = e.batches[k];
-        const gf_node_desc& nd = e.nodes[k];
-        if (tid == 0) {
-            ss.wanted = 0u;
-            b.active = 0u;
-            b.n_in = 0u;
-            b.n_out = 0u;
-            const uint32_t freec = e.ctl[CTL_TOP];
-            const unsigned long long head = e.ring_head[k], avail_ready = e.ring_tail[k] - head;
-            ss.head = head;
-            if (nd.role == GF_ROLE_SOURCE) {
-                unsigned long long gen = e.ctr[CNT_GENERATED], sunk = e.ctr[CNT_SUNK];
-                unsigned long long live = gen - sunk, remaining = e.N - gen;
+    if (!L.overflow) {
+        for (uint32_t j = tid; j < L.staged; j += bsz()) {
+            uint32_t k = 0;
+            while (k + 1u < n && L.lst_off[k + 1u] <= j) ++k;
+            unsigned long long pos = L.head[k] + (j - L.lst_off[k]);
+            L.lst[j] = e.rings[(size_t)k * e.ring_cap + (size_t)(pos % e.ring_cap)];
+        }
+    }
+    __syncthreads();
+    // decisions in node order (one thread, all operands in LDS)
+    if (tid == 0) {
+        uint32_t reserved = 0;
+        L.any = 0u;
+        for (uint32_t k = 0; k < n; ++k) {
+            NodeSt& s = L.st[k];
+            const uint32_t kind = e.arr[k].kind;
+            s.active = 0u;
+            s.n_in = 0u;
+            s.n_out = 0u;
+            s.n_tiles = 0u;
+            if (kind == KIND_SOURCE) {
+                unsigned long long live = L.generated - L.sunk, remaining = e.N - L.generated;
                 unsigned long long min_fill = C < P ? C : P;
                 if (remaining < min_fill) min_fill = remaining;
-                if (gen < e.N && live < P && (P - live) >= min_fill && freec - ss.reserved >= 1u) {
+                if (L.generated < e.N && live < P && (P - live) >= min_fill && L.top - reserved >= 1u) {
                     unsigned long long fill = remaining;
                     if (fill > P - live) fill = P - live;
                     if (fill > C) fill = C;
-                    b.particles = (uint32_t)fill;
-                    b.first_id = gen;
-                    b.n_tiles = (uint32_t)((fill + b.tile_items - 1u) / b.tile_items);
-                    e.ctr[CNT_GENERATED] = gen + fill;
-                    ss.reserved += 1u;
-                    b.active = 1u;
+                    s.particles = (uint32_t)fill;
+                    s.first_id = L.generated;
+                    s.n_tiles = (uint32_t)((fill + e.arr[k].tile_items - 1u) / e.arr[k].tile_items);
+                    L.generated += fill;
+                    reserved += 1u;
+                    s.active = 1u;
                 }
-            } else if (nd.role == GF_ROLE_TRANSFORM) {
-                uint32_t wanted = avail_ready < Q ? (uint32_t)avail_ready : Q;
+            } else if (kind == KIND_SINK) {
+                s.n_in = L.wanted[k];
+                s.active = s.n_in ? 1u : 0u;
+            } else {
+                uint32_t wanted = L.wanted[k];
                 unsigned long long parts = 0;
-                for (uint32_t i = 0; i < wanted; ++i) parts += e.rings[(size_t)k * e.ring_cap + (head + i) % e.ring_cap].fill;
+                for (uint32_t i = 0; i < wanted; ++i) {
+                    gf_chunk_desc d = L.overflow ? e.rings[(size_t)k * e.ring_cap + (size_t)((L.head[k] + i) % e.ring_cap)]
+                                                 : L.lst[L.lst_off[k] + i];
+                    parts += d.fill;
+                }
+                const uint32_t B = kind == KIND_PARTITION ? e.nodes[k].bins : 1u;
                 while (wanted) {
-                    uint64_t bound = bound_of(parts, C, nd.bins);
-                    if ((uint64_t)(freec - ss.reserved) >= bound) {
-                        ss.reserved += (uint32_t)bound;
+                    uint64_t bound = bound_of(parts, C, B);
+                    if ((uint64_t)(L.top - reserved) >= bound) {
+                        reserved += (uint32_t)bound;
                         break;
                     }
                     --wanted;
-                    parts -= e.rings[(size_t)k * e.ring_cap + (head + wanted) % e.ring_cap].fill;
+                    gf_chunk_desc d = L.overflow ? e.rings[(size_t)k * e.ring_cap + (size_t)((L.head[k] + wanted) % e.ring_cap)]
+                                                 : L.lst[L.lst_off[k] + wanted];
+                    parts -= d.fill;
                 }
-                ss.wanted = wanted;
-            } else {
-                ss.wanted = (uint32_t)avail_ready;
+                s.n_in = wanted;
+                s.active = wanted ? 1u : 0u;
             }
+            L.any |= s.active;
         }
-        __syncthreads();
-        const uint32_t wanted = ss.wanted;
-        if (nd.role != GF_ROLE_SOURCE && wanted) {
-            const unsigned long long head = ss.head;
-            for (uint32_t i = tid; i < wanted; i += kThreads)
-                b.in[i] = e.rings[(size_t)k * e.ring_cap + (head + i) % e.ring_cap];
-            __syncthreads();
-            if (tid == 0) {
-                b.n_in = wanted;
-                e.ring_head[k] = head + wanted;
-                b.active = 1u;
+    }
+    __syncthreads();
+    // input lists and tile / item prefixes of every active non-source node
+    for (uint32_t k = 0; k < n; ++k) {
+        NodeSt& s = L.st[k];
+        if (!s.active || e.arr[k].kind == KIND_SOURCE) continue;
+        const NodeArr& a = e.arr[k];
+        const uint32_t m = s.n_in, T = a.tile_items;
+        uint32_t carry_t = 0, carry_i = 0;
+        for (uint32_t i0 = 0; i0 < m; i0 += bsz()) {
+            const uint32_t i = i0 + tid;
+            gf_chunk_desc d{0, 0, 0, 0};
+            if (i < m) {
+                d = L.overflow ? e.rings[(size_t)k * e.ring_cap + (size_t)((L.head[k] + i) % e.ring_cap)] : L.lst[L.lst_off[k] + i];
+                a.in[i] = d;
             }
-            __syncthreads();
-            build_prefixes(b, b.tile_items, ss.scan);
+            uint32_t tt, ti;
+            uint32_t pt = block_exscan(i < m ? (d.fill + T - 1u) / T : 0u, L.scan, &tt);
+            uint32_t pi = block_exscan(i < m ? d.fill : 0u, L.scan, &ti);
+            if (i < m) {
+                a.tile_prefix[i] = carry_t + pt;
+                a.item_prefix[i] = carry_i + pi;
+            }
+            carry_t += tt;
+            carry_i += ti;
         }
-        if (tid == 0) ss.any |= b.active;
+        if (tid == 0) {
+            a.tile_prefix[m] = carry_t;
+            a.item_prefix[m] = carry_i;
+            s.n_tiles = carry_t;
+            s.particles = carry_i;
+            L.head[k] += m;
+        }
         __syncthreads();
     }
+    // work prefixes, ring cursors, termination
     if (tid == 0) {
         uint32_t ka = 0, kb = 0;
-        for (uint32_t k = 0; k < e.n_nodes; ++k) {
+        for (uint32_t k = 0; k < n; ++k) {
             e.ka_prefix[k] = ka;
             e.kb_prefix[k] = kb;
-            const Batch& b = e.batches[k];
-            if (b.active) {
-                if (e.nodes[k].role == GF_ROLE_TRANSFORM && is_partition_op(e.nodes[k].op)) ka += b.n_tiles;
-                kb += b.n_tiles;
+            NodeSt& s = L.st[k];
+            s.head = L.head[k];
+            s.tail = L.tail[k];
+            if (s.active) {
+                if (e.arr[k].kind == KIND_PARTITION) ka += s.n_tiles;
+                kb += s.n_tiles;
             }
         }
-        e.ka_prefix[e.n_nodes] = ka;
-        e.kb_prefix[e.n_nodes] = kb;
-        if (!ss.any) {
+        e.ka_prefix[n] = ka;
+        e.kb_prefix[n] = kb;
+        e.ctl[CTL_TOP] = L.top;
+        e.ctr[CNT_GENERATED] = L.generated;
+        e.ctr[CNT_SUNK] = L.sunk;
+        e.ctr[CNT_STEP] = L.step;
+        e.ctr[CNT_BATCHES] = L.batches;
+        e.ctr[CNT_PART_ITEMS] = L.part_items;
+        e.ctr[CNT_XF_ITEMS] = L.xf_items;
+        e.ctr[CNT_SRC_ITEMS] = L.src_items;
+        e.ctr[CNT_SINK_ITEMS] = L.sink_items;
+        if (!L.any) {
             bool drained = true;
-            for (uint32_t k = 0; k < e.n_nodes; ++k)
-                if (e.ring_tail[k] != e.ring_head[k]) drained = false;
-            if (drained && e.ctr[CNT_GENERATED] >= e.N) {
+            for (uint32_t k = 0; k < n; ++k)
+                if (L.tail[k] != L.head[k]) drained = false;
+            if (drained && L.generated >= e.N) {
                 e.ctl[CTL_DONE] = 1u;
-                if (e.ctl[CTL_TOP] != e.S) set_status(&e.ctl[CTL_STATUS], GF_BAD_ARGUMENT);
+                if (L.top != e.S) set_status(&e.ctl[CTL_STATUS], GF_BAD_ARGUMENT);
             } else {
                 set_status(&e.ctl[CTL_STATUS], GF_STALL);
             }
         }
     }
+    __syncthreads();
+    store_states(e, L.st);
     __syncthreads();
 }
 
@@ -501,21 +715,20 @@ __device__ inline uint32_t find_node(const uint32_t* prefix, uint32_t n, uint32_
     return k;
 }
 
-// Ticket: returns true in the last workgroup to arrive (all its global writes are fenced).
+// Ticket: true in the last workgroup to arrive. No fence: the caller's last workgroup reads no
+// data written by the other workgroups of the same launch, and every other workgroup has finished
+// all reads of the step's batch descriptors before it takes its ticket (the last workgroup
+// rewrites those descriptors for the next step). A device-scope fence per workgroup would cost
+// an L2 write-back each (about 50 ns serialised per workgroup on gfx950).
 __device__ inline bool last_block(uint32_t* ticket, uint32_t* flag_sh)
 {
-    __threadfence();
     __syncthreads();
     if (threadIdx.x == 0) {
         uint32_t t = atomicAdd(ticket, 1u);
         *flag_sh = (t == gridDim.x - 1u) ? 1u : 0u;
     }
     __syncthreads();
-    if (*flag_sh) {
-        __threadfence();
-        return true;
-    }
-    return false;
+    return *flag_sh != 0u;
 }
 
 }  // namespace gfd

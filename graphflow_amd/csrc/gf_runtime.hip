@@ -166,18 +166,23 @@ __global__ __launch_bounds__(kThreads) void k_engine_ka(Engine e)
 {
     extern __shared__ char smem[];
     __shared__ uint32_t flag;
-    __shared__ PlanShared ps;
     if (engine_stopped(e, &flag)) return;
     const uint32_t total = e.ka_prefix[e.n_nodes];
     for (uint32_t t = blockIdx.x; t < total; t += gridDim.x) {
-        uint32_t k = find_node(e.ka_prefix, e.n_nodes, t);
-        const Batch b = e.batches[k];
+        const uint32_t k = find_node(e.ka_prefix, e.n_nodes, t);
+        const NodeSt s = e.st[k];
+        const Batch b = make_batch(e, k, s);
         count_tile<P, R>(e.nodes[k], b, t - e.ka_prefix[k], e.cols, e.C, &e.ctl[CTL_STATUS], smem);
     }
-    if (last_block(&e.ctl[CTL_TICKET_A], &flag)) {
-        engine_plan(e, ps);
-        if (threadIdx.x == 0) e.ctl[CTL_TICKET_A] = 0u;
-    }
+}
+
+// One workgroup: the plan of every node of the step (reads KA's counts across the kernel boundary).
+__global__ __launch_bounds__(kThreads) void k_engine_plan(Engine e)
+{
+    extern __shared__ char smem[];
+    __shared__ uint32_t flag;
+    if (engine_stopped(e, &flag)) return;
+    engine_plan(e, *reinterpret_cast<PlanLds*>(smem));
 }
 
 template <class P, int R>
@@ -185,55 +190,137 @@ __global__ __launch_bounds__(kThreads) void k_engine_kb(Engine e, uint32_t stage
 {
     extern __shared__ char smem[];
     __shared__ uint32_t flag;
-    __shared__ SelectShared ss;
     if (engine_stopped(e, &flag)) return;
     const uint32_t total = e.kb_prefix[e.n_nodes];
     for (uint32_t t = blockIdx.x; t < total; t += gridDim.x) {
-        uint32_t k = find_node(e.kb_prefix, e.n_nodes, t);
+        const uint32_t k = find_node(e.kb_prefix, e.n_nodes, t);
         const gf_node_desc& nd = e.nodes[k];
-        const Batch b = e.batches[k];
+        const NodeSt s = e.st[k];
+        const Batch b = make_batch(e, k, s);
         const uint32_t lt = t - e.kb_prefix[k];
-        if (nd.role == GF_ROLE_SOURCE) {
+        const uint32_t kind = e.arr[k].kind;
+        if (kind == KIND_SOURCE) {
             source_tile<P>(nd, b, lt, e.cols, e.C, e.rp);
-        } else if (nd.role == GF_ROLE_SINK) {
+        } else if (kind == KIND_SINK) {
             sink_tile<P>(nd, b, lt, e.cols, e.C, &e.ctl[CTL_STATUS], smem);
-        } else if (is_partition_op(nd.op)) {
+        } else if (kind == KIND_PARTITION) {
+            const int32_t* fz = e.fuse + nd.route_offset;
             if ((stage_mask >> (k & 31u)) & 1u)
-                scatter_tile<P, R, true>(nd, b, lt, e.cols, e.C, &e.ctl[CTL_STATUS], smem);
+                scatter_tile<P, R, true>(nd, b, lt, e.cols, e.C, &e.ctl[CTL_STATUS], smem, e.nodes, fz, &e.rp);
             else
-                scatter_tile<P, R, false>(nd, b, lt, e.cols, e.C, &e.ctl[CTL_STATUS], smem);
+                scatter_tile<P, R, false>(nd, b, lt, e.cols, e.C, &e.ctl[CTL_STATUS], smem, e.nodes, fz, &e.rp);
         } else {
             transform_tile<P, R>(nd, b, lt, e.cols, e.C, e.rp);
         }
     }
     if (last_block(&e.ctl[CTL_TICKET_B], &flag)) {
-        engine_retire(e);
-        engine_select(e, ss);
+        SelLds& L = *reinterpret_cast<SelLds*>(smem);
+        engine_retire(e, L);
+        engine_select(e, L);
         if (threadIdx.x == 0) e.ctl[CTL_TICKET_B] = 0u;
+    }
+}
+
+// Micro engine: one persistent workgroup of G tile groups (256 threads each) runs whole steps with
+// no kernel boundary: every tile of a node is handed to the groups in rounds, the plan, retire and
+// select phases use all threads. The live working set stays in the L2 of one XCD, so each of the
+// step's dependent round trips is an L2 hit instead of a cross-XCD miss.
+template <class P, int R>
+__global__ __launch_bounds__(1024) void k_micro(Engine e, uint32_t stage_mask, uint32_t max_steps, uint32_t group_lds)
+{
+    extern __shared__ char smem[];
+    __shared__ uint32_t flag;
+    const uint32_t G = blockDim.x / kThreads, g = threadIdx.x / kThreads;
+    char* gsm = smem + (size_t)g * group_lds;
+    unsigned long long t0c = 0;
+    auto mark = [&](uint32_t phase) {
+        if (e.prof && threadIdx.x == 0) {
+            unsigned long long now = wall_clock64();
+            e.prof[phase] += now - t0c;
+            t0c = now;
+        }
+    };
+    for (uint32_t it = 0; it < max_steps; ++it) {
+        if (engine_stopped(e, &flag)) return;
+        if (e.prof && threadIdx.x == 0) {
+            t0c = wall_clock64();
+            e.prof[PROF_STEPS] += 1u;
+        }
+        for (uint32_t k = 0; k < e.n_nodes; ++k) {
+            const NodeSt s = e.st[k];
+            if (!s.active || e.arr[k].kind != KIND_PARTITION) continue;
+            const Batch b = make_batch(e, k, s);
+            for (uint32_t t0 = 0; t0 < s.n_tiles; t0 += G) {
+                const uint32_t t = t0 + g;
+                count_tile<P, R>(e.nodes[k], b, t < s.n_tiles ? t : kNoTile, e.cols, e.C, &e.ctl[CTL_STATUS], gsm);
+            }
+        }
+        __syncthreads();
+        mark(PROF_COUNT);
+        engine_plan(e, *reinterpret_cast<PlanLds*>(smem));
+        __syncthreads();
+        mark(PROF_PLAN);
+        if (engine_stopped(e, &flag)) return;  // a failed plan publishes nothing
+        for (uint32_t k = 0; k < e.n_nodes; ++k) {
+            const NodeSt s = e.st[k];
+            if (!s.active) continue;
+            const gf_node_desc& nd = e.nodes[k];
+            const Batch b = make_batch(e, k, s);
+            const uint32_t kind = e.arr[k].kind;
+            const bool stage = (stage_mask >> (k & 31u)) & 1u;
+            for (uint32_t t0 = 0; t0 < s.n_tiles; t0 += G) {
+                const uint32_t t = t0 + g < s.n_tiles ? t0 + g : kNoTile;
+                if (kind == KIND_SOURCE) {
+                    source_tile<P>(nd, b, t, e.cols, e.C, e.rp);
+                } else if (kind == KIND_SINK) {
+                    sink_tile<P>(nd, b, t, e.cols, e.C, &e.ctl[CTL_STATUS], gsm);
+                } else if (kind == KIND_PARTITION) {
+                    const int32_t* fz = e.fuse + nd.route_offset;
+                    if (stage)
+                        scatter_tile<P, R, true>(nd, b, t, e.cols, e.C, &e.ctl[CTL_STATUS], gsm, e.nodes, fz, &e.rp);
+                    else
+                        scatter_tile<P, R, false>(nd, b, t, e.cols, e.C, &e.ctl[CTL_STATUS], gsm, e.nodes, fz, &e.rp);
+                } else {
+                    transform_tile<P, R>(nd, b, t, e.cols, e.C, e.rp);
+                }
+            }
+            __syncthreads();
+            mark(kind == KIND_SOURCE ? PROF_SOURCE : kind == KIND_SINK ? PROF_SINK : kind == KIND_PARTITION ? PROF_PART : PROF_XF);
+        }
+        __syncthreads();
+        SelLds& L = *reinterpret_cast<SelLds*>(smem);
+        engine_retire(e, L);
+        mark(PROF_RETIRE);
+        engine_select(e, L);
+        __syncthreads();
+        mark(PROF_SELECT);
     }
 }
 
 __global__ __launch_bounds__(kThreads) void k_engine_setup(Engine e)
 {
-    __shared__ SelectShared ss;
+    extern __shared__ char smem[];
+    SelLds& L = *reinterpret_cast<SelLds*>(smem);
     const uint32_t tid = threadIdx.x;
-    for (uint32_t i = tid; i < e.S; i += kThreads) {
-        e.free_stack[i] = e.S - 1u - i;  // index 0 is handed out first (ChunkLedger.hpp:43-46)
-        e.gens[i] = 0u;
-    }
-    for (uint32_t k = tid; k < e.n_nodes; k += kThreads) {
-        e.ring_head[k] = 0ull;
-        e.ring_tail[k] = 0ull;
-        e.batches[k].active = 0u;
-        e.batches[k].n_in = 0u;
-        e.batches[k].n_out = 0u;
-    }
+    for (uint32_t i = tid; i < e.S; i += kThreads)
+        e.free_stack[i] = (unsigned long long)(e.S - 1u - i);  // index 0 first (ChunkLedger.hpp:43-46), generation 0
     for (uint32_t i = tid; i < CNT_WORDS; i += kThreads) e.ctr[i] = 0ull;
     for (uint32_t i = tid; i < CTL_WORDS; i += kThreads) e.ctl[i] = 0u;
+    if (e.prof)
+        for (uint32_t i = tid; i < PROF_WORDS; i += kThreads) e.prof[i] = 0ull;
+    for (uint32_t k = tid; k < e.n_nodes; k += kThreads) {
+        NodeSt z{};
+        L.st[k] = z;
+        L.head[k] = 0ull;
+        L.tail[k] = 0ull;
+    }
+    if (tid == 0) {
+        L.top = e.S;
+        L.generated = L.sunk = L.step = L.batches = 0ull;
+        L.part_items = L.xf_items = L.src_items = L.sink_items = 0ull;
+    }
     __syncthreads();
-    if (tid == 0) e.ctl[CTL_TOP] = e.S;
-    __syncthreads();
-    engine_select(e, ss);
+    engine_select(e, L);
 }
 
 // ---- dense histogram -> rows ------------------------------------------------------------------
@@ -774,6 +861,9 @@ struct EngineBuffers {
     Engine e{};
     uint32_t stage_mask = 0, r = 4, grid = 256;
     size_t lds_a = 0, lds_b = 0;
+    bool micro = false;
+    uint32_t group_lds = 0;
+    size_t micro_lds = 0;
     hipGraphExec_t exec = nullptr;
     uint32_t graph_steps = 0;
     ~EngineBuffers()
@@ -1078,7 +1168,9 @@ std::string shape_key(const gf_flow& f, const RunShape& s, uint64_t N)
     std::string k = std::to_string(f.payload) + "/" + std::to_string(s.C) + "/" + std::to_string(s.P) + "/" +
                     std::to_string(s.S) + "/" + std::to_string(s.Q) + "/" + std::to_string(s.tile_items) + "/" +
                     std::to_string(f.tuning.trace) + "/" + std::to_string(f.tuning.seed) + "/" + std::to_string(N) + "/" +
-                    std::to_string(f.tuning.steps_per_launch);
+                    std::to_string(f.tuning.steps_per_launch) + "/" + std::to_string(f.tuning.fuse_transforms) + "/" +
+                    std::to_string(f.tuning.max_workgroups) + "/" + std::to_string(f.tuning.device_engine) + "/" +
+                    std::to_string(f.tuning.use_graph);
     for (auto& n : f.nodes) {
         k += "|" + std::to_string(n.nd.role) + "," + std::to_string(n.nd.op) + "," + std::to_string(n.nd.bins);
         for (int32_t r : n.routes) k += "," + std::to_string(r);
@@ -1091,13 +1183,19 @@ std::string shape_key(const gf_flow& f, const RunShape& s, uint64_t N)
     return k;
 }
 
+template <class K>
+void set_lds(K kernel, size_t bytes)
+{
+    if (bytes > 64u * 1024u) HIPX(hipFuncSetAttribute((const void*)kernel, hipFuncAttributeMaxDynamicSharedMemorySize, (int)bytes));
+}
+
 void build_engine(gf_flow& f, uint64_t N, const RunShape& sh, uint32_t src)
 {
     auto eb = std::make_unique<EngineBuffers>();
     DevMem& m = eb->mem;
     Engine& e = eb->e;
     const uint32_t n = (uint32_t)f.nodes.size();
-    if (n > 64) fail(GF_BAD_ARGUMENT, "graphFlow: the device engine supports at most 64 nodes");
+    if (n > kMaxNodes) fail(GF_BAD_ARGUMENT, "graphFlow: the device engine supports at most 64 nodes");
     gf_pool& pool = *f.pool;
     e.n_nodes = n;
     e.C = sh.C;
@@ -1118,40 +1216,52 @@ void build_engine(gf_flow& f, uint64_t N, const RunShape& sh, uint32_t src)
     e.rp.ph.roulette_p = (float)t.photon_roulette_p;
     e.rp.ph.tally_bins = t.photon_tally_bins;
     std::vector<gf_node_desc> nds(n);
-    std::vector<int32_t> routes;
-    std::vector<uint32_t> rb(n, 0u);
-    uint32_t r = effective_r(f.payload, pick_tile_r(sh.tile_items));
+    std::vector<int32_t> routes, fuse;
+    const uint32_t r = effective_r(f.payload, pick_tile_r(sh.tile_items));
     const uint32_t rec = payload_bytes(f.payload);
-    size_t lds_a = 256, lds_b = 256;
-    uint32_t stage_mask = 0;
-    std::vector<Batch> hb(n);
-    uint32_t grid_tiles = 0;
+    size_t lds_a = sizeof(PlanLds), lds_b = sizeof(SelLds), group_lds = 256;
+    uint32_t stage_mask = 0, grid_tiles = 0;
+    std::vector<NodeArr> arr(n);
+    auto is_plain_xf = [&](int32_t k) {
+        return k >= 0 && f.nodes[k].nd.role == GF_ROLE_TRANSFORM && !is_partition(f.nodes[k].nd) && f.nodes[k].nd.bins == 1u;
+    };
     for (uint32_t k = 0; k < n; ++k) {
         nds[k] = f.nodes[k].nd;
         nds[k].route_offset = (uint32_t)routes.size();
-        routes.insert(routes.end(), f.nodes[k].routes.begin(), f.nodes[k].routes.end());
         const gf_node_desc& nd = nds[k];
         const bool part = nd.role == GF_ROLE_TRANSFORM && is_partition(nd);
+        for (int32_t dst : f.nodes[k].routes) {
+            // a plain transform fed by a partition bin runs inside that partition's scatter
+            if (part && t.fuse_transforms && is_plain_xf(dst)) {
+                fuse.push_back(dst);
+                routes.push_back(f.nodes[dst].routes[0]);
+            } else {
+                fuse.push_back(-1);
+                routes.push_back(dst);
+            }
+        }
         const uint32_t B = std::max<uint32_t>(nd.bins, 1u);
-        uint32_t cap_in = nd.role == GF_ROLE_SOURCE ? 1u : nd.role == GF_ROLE_SINK ? sh.S : sh.Q;
-        uint32_t cap_out = nd.role == GF_ROLE_SOURCE ? 1u : nd.role == GF_ROLE_SINK ? 1u : reserve_bound_of(nd, sh.C, sh.Q);
-        rb[k] = cap_out;
-        uint32_t cap_tiles = (uint32_t)(ceil_div(sh.C, sh.tile_items) * (nd.role == GF_ROLE_SINK ? 1u : cap_in));
-        Batch& b = hb[k];
-        std::memset(&b, 0, sizeof(b));
-        b.in = m.d<gf_chunk_desc>(cap_in);
-        b.tile_prefix = m.d<uint32_t>(cap_in + 1u);
-        b.item_prefix = m.d<uint32_t>(cap_in + 1u);
-        b.out = m.d<gf_chunk_desc>(cap_out);
-        b.counts = part ? m.d<uint32_t>((size_t)cap_tiles * B) : nullptr;
-        b.totals = m.d<uint32_t>(2u * B);
-        b.tile_items = sh.tile_items;
-        b.purpose = 2u * k;
+        NodeArr& a = arr[k];
+        std::memset(&a, 0, sizeof(a));
+        a.kind = nd.role == GF_ROLE_SOURCE ? KIND_SOURCE : nd.role == GF_ROLE_SINK ? KIND_SINK : part ? KIND_PARTITION : KIND_TRANSFORM;
+        a.cap_in = nd.role == GF_ROLE_SOURCE ? 1u : nd.role == GF_ROLE_SINK ? sh.S : sh.Q;
+        a.cap_out = nd.role == GF_ROLE_SOURCE ? 1u : nd.role == GF_ROLE_SINK ? 1u : reserve_bound_of(nd, sh.C, sh.Q);
+        a.reserve_bound = a.cap_out;
+        uint32_t cap_tiles = (uint32_t)(ceil_div(sh.C, sh.tile_items) * (nd.role == GF_ROLE_SINK ? 1u : a.cap_in));
+        a.in = m.d<gf_chunk_desc>(a.cap_in);
+        a.tile_prefix = m.d<uint32_t>(a.cap_in + 1u);
+        a.item_prefix = m.d<uint32_t>(a.cap_in + 1u);
+        a.out = m.d<gf_chunk_desc>(a.cap_out);
+        a.counts = part ? m.d<uint32_t>((size_t)cap_tiles * B) : nullptr;
+        a.totals = m.d<uint32_t>(2u * B);
+        a.tile_items = sh.tile_items;
+        a.purpose = 2u * k;
         if (part) {
             lds_a = std::max(lds_a, (size_t)B * 4u);
             bool stg = stage_fits(B, sh.tile_items, rec);
             if (stg && k < 32) stage_mask |= 1u << k;
             lds_b = std::max(lds_b, scatter_smem(B, sh.tile_items, rec, stg && k < 32));
+            group_lds = std::max(group_lds, std::max((size_t)B * 4u, scatter_smem(B, sh.tile_items, rec, stg && k < 32)));
             grid_tiles += cap_tiles;
         } else if (nd.role != GF_ROLE_SINK) {
             grid_tiles += cap_tiles;
@@ -1164,49 +1274,121 @@ void build_engine(gf_flow& f, uint64_t N, const RunShape& sh, uint32_t src)
             e.rp.zipf_values = nd.u[0];
         }
     }
-    if (lds_b > 64u * 1024u || lds_a > 64u * 1024u) fail(GF_BAD_ARGUMENT, "graphFlow: partition width exceeds the workgroup LDS budget");
+    if (lds_b > 160u * 1024u || lds_a > 160u * 1024u)
+        fail(GF_BAD_ARGUMENT, "graphFlow: partition width exceeds the workgroup LDS budget");
     gf_node_desc* dn = m.d<gf_node_desc>(n);
     HIPX(hipMemcpy(dn, nds.data(), sizeof(gf_node_desc) * n, hipMemcpyHostToDevice));
     int32_t* dr = m.d<int32_t>(std::max<size_t>(routes.size(), 1));
-    if (!routes.empty()) HIPX(hipMemcpy(dr, routes.data(), sizeof(int32_t) * routes.size(), hipMemcpyHostToDevice));
-    uint32_t* drb = m.d<uint32_t>(n);
-    HIPX(hipMemcpy(drb, rb.data(), sizeof(uint32_t) * n, hipMemcpyHostToDevice));
-    Batch* dbat = m.d<Batch>(n);
-    HIPX(hipMemcpy(dbat, hb.data(), sizeof(Batch) * n, hipMemcpyHostToDevice));
+    int32_t* df = m.d<int32_t>(std::max<size_t>(fuse.size(), 1));
+    if (!routes.empty()) {
+        HIPX(hipMemcpy(dr, routes.data(), sizeof(int32_t) * routes.size(), hipMemcpyHostToDevice));
+        HIPX(hipMemcpy(df, fuse.data(), sizeof(int32_t) * fuse.size(), hipMemcpyHostToDevice));
+    }
+    NodeArr* da = m.d<NodeArr>(n);
+    HIPX(hipMemcpy(da, arr.data(), sizeof(NodeArr) * n, hipMemcpyHostToDevice));
     e.nodes = dn;
     e.routes = dr;
-    e.reserve_bound = drb;
-    e.batches = dbat;
-    e.free_stack = m.d<uint32_t>(sh.S);
-    e.gens = m.d<uint32_t>(sh.S);
+    e.fuse = df;
+    e.arr = da;
+    e.st = m.d<NodeSt>(n);
+    e.free_stack = m.d<unsigned long long>(sh.S);
     e.rings = m.d<gf_chunk_desc>((size_t)n * sh.S);
-    e.ring_head = m.d<unsigned long long>(n);
-    e.ring_tail = m.d<unsigned long long>(n);
     e.ka_prefix = m.d<uint32_t>(n + 1u);
     e.kb_prefix = m.d<uint32_t>(n + 1u);
     e.ctl = m.d<uint32_t>(CTL_WORDS);
     e.ctr = m.d<unsigned long long>(CNT_WORDS);
+    e.trace_on = t.trace ? 1u : 0u;
     e.trace_cap = t.trace ? (unsigned long long)1 << 26 : 0ull;
     e.trace = t.trace ? m.d<uint32_t>(e.trace_cap) : nullptr;
-    eb->grid = std::min<uint32_t>(std::max<uint32_t>(grid_tiles, 1u), 1024u);
-    e.grid = eb->grid;
+    e.prof = t.use_graph == 2u ? m.d<unsigned long long>(PROF_WORDS) : nullptr;
+    const uint32_t gmax = t.max_workgroups ? t.max_workgroups : 64u;
+    eb->grid = std::min<uint32_t>(std::max<uint32_t>(grid_tiles, 1u), gmax);
     eb->r = r;
     eb->stage_mask = stage_mask;
     eb->lds_a = lds_a;
     eb->lds_b = lds_b;
     eb->key = shape_key(f, sh, N);
+    // engine choice: one persistent workgroup when the live items fit the L2 of one XCD (4 MiB)
+    // and its LDS holds four tile groups; the multi-workgroup step kernels otherwise
+    group_lds = (group_lds + 15u) & ~(size_t)15u;
+    eb->group_lds = (uint32_t)group_lds;
+    eb->micro_lds = std::max(std::max(sizeof(PlanLds), sizeof(SelLds)), 4u * group_lds);
+    const bool micro_fits = eb->micro_lds <= 160u * 1024u;
+    if (t.device_engine == 2u && !micro_fits) fail(GF_BAD_ARGUMENT, "graphFlow: micro engine LDS budget exceeded");
+    eb->micro = t.device_engine == 2u ||
+                (t.device_engine == 0u && micro_fits && (uint64_t)sh.P * rec <= (4ull << 20));
+    with_payload(f.payload, r, [&](auto pt, auto rc) {
+        using P = typename decltype(pt)::type;
+        constexpr int RR = decltype(rc)::value;
+        set_lds(k_engine_ka<P, RR>, lds_a);
+        set_lds(k_engine_kb<P, RR>, lds_b);
+        if (micro_fits) set_lds(k_micro<P, RR>, eb->micro_lds);
+    });
+    set_lds(k_engine_plan, sizeof(PlanLds));
     (void)src;
     f.eng = std::move(eb);
 }
 
-void launch_step(const EngineBuffers& eb, uint32_t payload, hipStream_t s)
+// One step: KA (classify/count tiles), KP (plan, one workgroup), KB (scatter/transform/source/sink
+// tiles, retire + select in the last workgroup). With `ev`, events are recorded around each kernel.
+void launch_step(const EngineBuffers& eb, uint32_t payload, hipStream_t s, hipEvent_t* ev = nullptr)
 {
     with_payload(payload, eb.r, [&](auto pt, auto rc) {
         using P = typename decltype(pt)::type;
         constexpr int RR = decltype(rc)::value;
+        if (ev) HIPX(hipEventRecord(ev[0], s));
         hipLaunchKernelGGL((k_engine_ka<P, RR>), dim3(eb.grid), dim3(kThreads), eb.lds_a, s, eb.e);
+        if (ev) HIPX(hipEventRecord(ev[1], s));
+        hipLaunchKernelGGL(k_engine_plan, dim3(1), dim3(kThreads), sizeof(PlanLds), s, eb.e);
+        if (ev) HIPX(hipEventRecord(ev[2], s));
         hipLaunchKernelGGL((k_engine_kb<P, RR>), dim3(eb.grid), dim3(kThreads), eb.lds_b, s, eb.e, eb.stage_mask);
+        if (ev) HIPX(hipEventRecord(ev[3], s));
     });
+}
+
+void finish_device(gf_flow& f, EngineBuffers& eb, gf_stats& st, const uint32_t* ctl, uint64_t launches);
+
+// Micro engine host loop: launches of one persistent workgroup, each running up to M steps.
+void run_micro(gf_flow& f, EngineBuffers& eb, uint64_t N, gf_stats& st, bool timed)
+{
+    hipStream_t s = f.ctx->stream;
+    const uint32_t M = f.tuning.steps_per_launch ? f.tuning.steps_per_launch : 2048u;
+    uint32_t ctl[CTL_WORDS];
+    uint64_t launches = 0, steps_launched = 0;
+    const uint64_t max_steps = 64ull * (N + 1024u) + 4096u;
+    hipEvent_t ev[2] = {nullptr, nullptr};
+    if (timed)
+        for (auto& x : ev) HIPX(hipEventCreate(&x));
+    for (int i = 0; i < 4; ++i) {
+        f.kt_ms[i] = 0.0;
+        f.kt_n[i] = 0;
+    }
+    for (;;) {
+        if (timed) HIPX(hipEventRecord(ev[0], s));
+        with_payload(f.payload, eb.r, [&](auto pt, auto rc) {
+            using P = typename decltype(pt)::type;
+            constexpr int RR = decltype(rc)::value;
+            hipLaunchKernelGGL((k_micro<P, RR>), dim3(1), dim3(1024), eb.micro_lds, s, eb.e, eb.stage_mask, M,
+                               eb.group_lds);
+        });
+        HIPX(hipGetLastError());
+        if (timed) HIPX(hipEventRecord(ev[1], s));
+        ++launches;
+        steps_launched += M;
+        HIPX(hipMemcpyAsync(ctl, eb.e.ctl, sizeof(ctl), hipMemcpyDeviceToHost, s));
+        HIPX(hipStreamSynchronize(s));
+        if (timed) {
+            float x = 0;
+            HIPX(hipEventElapsedTime(&x, ev[0], ev[1]));
+            f.kt_ms[3] += x;
+            f.kt_n[3]++;
+        }
+        if (ctl[CTL_STATUS] || ctl[CTL_DONE]) break;
+        if (steps_launched > max_steps) fail(GF_STALL, "graphFlow: device engine exceeded its step bound");
+    }
+    if (timed)
+        for (auto& x : ev) (void)hipEventDestroy(x);
+    finish_device(f, eb, st, ctl, launches);
 }
 
 void run_device(gf_flow& f, uint32_t src, uint64_t N, const RunShape& sh, gf_stats& st, bool timed)
@@ -1215,10 +1397,14 @@ void run_device(gf_flow& f, uint32_t src, uint64_t N, const RunShape& sh, gf_sta
     if (!f.eng || f.eng->key != key) build_engine(f, N, sh, src);
     EngineBuffers& eb = *f.eng;
     hipStream_t s = f.ctx->stream;
-    hipLaunchKernelGGL(k_engine_setup, dim3(1), dim3(kThreads), 0, s, eb.e);
+    hipLaunchKernelGGL(k_engine_setup, dim3(1), dim3(kThreads), sizeof(SelLds), s, eb.e);
     HIPX(hipGetLastError());
+    if (eb.micro) {
+        run_micro(f, eb, N, st, timed);
+        return;
+    }
     const uint32_t K = f.tuning.steps_per_launch ? f.tuning.steps_per_launch : 64u;
-    const bool use_graph = f.tuning.use_graph != 0 && !timed;
+    const bool use_graph = f.tuning.use_graph == 1u && !timed;
     if (use_graph && (!eb.exec || eb.graph_steps != K)) {
         if (eb.exec) {
             (void)hipGraphExecDestroy(eb.exec);
@@ -1241,47 +1427,45 @@ void run_device(gf_flow& f, uint32_t src, uint64_t N, const RunShape& sh, gf_sta
         f.kt_ms[i] = 0.0;
         f.kt_n[i] = 0;
     }
-    // the stall bound: every step retires or admits something, so a run needs at most this many
-    const uint64_t max_launches = 4ull * (N + 16u) * 64u + 1024u;
+    // host-side bound: every step admits, plans or retires something, so a correct run needs far
+    // fewer steps than this; exceeding it means a scheduling defect, reported as a stall
+    const uint64_t max_steps = 64ull * (N + 1024u) + 4096u;
+    uint64_t steps_launched = 0;
     for (;;) {
         if (use_graph) {
             HIPX(hipGraphLaunch(eb.exec, s));
-            launches += 2ull * K;
+            launches += 3ull * K;
         } else {
             for (uint32_t i = 0; i < K; ++i) {
                 if (timed) {
-                    with_payload(f.payload, eb.r, [&](auto pt, auto rc) {
-                        using P = typename decltype(pt)::type;
-                        constexpr int RR = decltype(rc)::value;
-                        HIPX(hipEventRecord(ev[0], s));
-                        hipLaunchKernelGGL((k_engine_ka<P, RR>), dim3(eb.grid), dim3(kThreads), eb.lds_a, s, eb.e);
-                        HIPX(hipEventRecord(ev[1], s));
-                        hipLaunchKernelGGL((k_engine_kb<P, RR>), dim3(eb.grid), dim3(kThreads), eb.lds_b, s, eb.e,
-                                           eb.stage_mask);
-                        HIPX(hipEventRecord(ev[2], s));
-                    });
-                    HIPX(hipEventSynchronize(ev[2]));
-                    float a = 0, b = 0;
-                    HIPX(hipEventElapsedTime(&a, ev[0], ev[1]));
-                    HIPX(hipEventElapsedTime(&b, ev[1], ev[2]));
-                    f.kt_ms[0] += a;
-                    f.kt_ms[1] += b;
-                    f.kt_n[0]++;
-                    f.kt_n[1]++;
+                    launch_step(eb, f.payload, s, ev);
+                    HIPX(hipEventSynchronize(ev[3]));
+                    for (int q = 0; q < 3; ++q) {
+                        float x = 0;
+                        HIPX(hipEventElapsedTime(&x, ev[q], ev[q + 1]));
+                        f.kt_ms[q] += x;
+                        f.kt_n[q]++;
+                    }
                 } else {
                     launch_step(eb, f.payload, s);
                 }
-                launches += 2;
+                launches += 3;
             }
         }
+        steps_launched += K;
         HIPX(hipGetLastError());
         HIPX(hipMemcpyAsync(ctl, eb.e.ctl, sizeof(ctl), hipMemcpyDeviceToHost, s));
         HIPX(hipStreamSynchronize(s));
         if (ctl[CTL_STATUS] || ctl[CTL_DONE]) break;
-        if (launches > max_launches) fail(GF_STALL, "graphFlow: device engine exceeded its step bound");
+        if (steps_launched > max_steps) fail(GF_STALL, "graphFlow: device engine exceeded its step bound");
     }
     if (timed)
         for (auto& x : ev) (void)hipEventDestroy(x);
+    finish_device(f, eb, st, ctl, launches);
+}
+
+void finish_device(gf_flow& f, EngineBuffers& eb, gf_stats& st, const uint32_t* ctl, uint64_t launches)
+{
     unsigned long long ctr[CNT_WORDS];
     HIPX(hipMemcpy(ctr, eb.e.ctr, sizeof(ctr), hipMemcpyDeviceToHost));
     if (f.tuning.trace) {
@@ -1682,6 +1866,15 @@ gf_status gf_flow_kernel_times(gf_flow* flow, double* ms, uint64_t* launches, ui
         launches[i] = flow->kt_n[i];
     }
     return GF_OK;
+}
+
+gf_status gf_flow_phase_profile(gf_flow* flow, uint64_t* ticks, uint32_t words)
+{
+    if (!flow || !ticks || !flow->eng || !flow->eng->e.prof) return GF_BAD_ARGUMENT;
+    return guard(&flow->err, [&] {
+        HIPX(hipMemcpy(ticks, flow->eng->e.prof, sizeof(uint64_t) * std::min<uint32_t>(words, PROF_WORDS),
+                       hipMemcpyDeviceToHost));
+    });
 }
 
 gf_status gf_flow_item_passes(gf_flow* flow, uint64_t* partition_items, uint64_t* transform_items)

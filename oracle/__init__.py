@@ -36,7 +36,7 @@ class NodeDesc(C.Structure):
 class RunCfg(C.Structure):
     _fields_ = [("seed", C.c_uint64), ("items", C.c_uint64), ("capacity", C.c_uint32),
                 ("live", C.c_uint32), ("batch", C.c_uint32), ("num_chunks", C.c_uint32),
-                ("mode", C.c_uint32), ("threads", C.c_uint32), ("layers", C.c_uint32),
+                ("mode", C.c_uint32), ("threads", C.c_uint32), ("fuse", C.c_uint32), ("layers", C.c_uint32),
                 ("tally_bins", C.c_uint32), ("layer_mfp", C.c_double), ("mu_t", C.c_double),
                 ("albedo", C.c_double), ("hg_g", C.c_double), ("roulette_w", C.c_double),
                 ("roulette_p", C.c_double)]
@@ -184,11 +184,11 @@ def make_nodes(graph):
     return arr, r
 
 
-def run(graph, *, seed, items, capacity, live=0, batch=1, num_chunks=0, mode=MODE_RUNNER, threads=1,
+def run(graph, *, seed, items, capacity, live=0, batch=1, num_chunks=0, mode=MODE_RUNNER, threads=1, fuse=False,
         trace_cap=0, hist_size=0, record=False, digest_cap=0, multiset_bins=0, lut=None, photon=None):
     nodes, routes = make_nodes(graph)
     cfg = RunCfg(seed=seed, items=items, capacity=capacity, live=live, batch=batch,
-                 num_chunks=num_chunks, mode=mode, threads=threads)
+                 num_chunks=num_chunks, mode=mode, threads=threads, fuse=1 if fuse else 0)
     if photon:
         for k, v in photon.items():
             setattr(cfg, k, v)

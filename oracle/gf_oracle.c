@@ -562,10 +562,19 @@ static int partition_classify(const or_ctx* x, const or_node_desc* nd, const or_
 /* Exact plan + output acquisition + stable scatter (Execution.hpp:2520-2642 and the scatter
  * kernels :1096-1187 / :1327-1502). Slot order inside a bin is the canonical stable order:
  * bin-major, then batch input order. */
+/* the plain transform node fused into bin `bin` of partition `nd`, or -1 */
+static int fused_of(const or_ctx* x, const or_node_desc* nd, uint32_t bin)
+{
+    if (!x->cfg->fuse || x->cfg->mode != OR_MODE_STEP) return -1;
+    int32_t d = x->routes[nd->route_offset + bin];
+    const or_node_desc* t = &x->nodes[d];
+    if (t->role == OR_ROLE_TRANSFORM && t->op >= OR_XF_SCALE_WEIGHT && t->op < OR_SNK_HIST && t->bins == 1u) return d;
+    return -1;
+}
+
 static int partition_plan_scatter(const or_ctx* x, const or_node_desc* nd, uint32_t purpose, or_pool* p,
                                   batch* b, uint32_t reserve_bound)
 {
-    (void)x;
     uint64_t sum = 0;
     for (uint32_t k = 0; k < nd->bins; ++k) sum += b->counts[k];
     if (sum != b->particles) return OR_PLAN_MISMATCH;
@@ -601,6 +610,8 @@ static int partition_plan_scatter(const or_ctx* x, const or_node_desc* nd, uint3
         or_item it = b->items[i];
         int u = or_update(nd, bin, &it);
         if (u != OR_OK && urc == OR_OK) urc = u;
+        int fz = fused_of(x, nd, bin);
+        if (fz >= 0) or_transform_item(x, &x->nodes[fz], 2u * (uint32_t)fz, &it);
         uint32_t chunk = b->out[base[bin] + j / p->C].index;
         p->slots[(size_t)chunk * p->C + j % p->C] = it;
     }
@@ -1024,8 +1035,11 @@ int or_run(const or_node_desc* nodes, uint32_t n_nodes, const int32_t* routes, c
                 batch* b = &lane[k];
                 if (!b->armed) continue;
                 if (nd->role != OR_ROLE_SINK)
-                    for (uint32_t o = 0; o < b->n_out; ++o)
-                        fifo_push(&ready[routes[nd->route_offset + b->out[o].bin]], b->out[o]);
+                    for (uint32_t o = 0; o < b->n_out; ++o) {
+                        int32_t d = routes[nd->route_offset + b->out[o].bin];
+                        if (is_partition(nd) && fused_of(&x, nd, b->out[o].bin) >= 0) d = routes[nodes[d].route_offset];
+                        fifo_push(&ready[d], b->out[o]);
+                    }
                 if (nd->role == OR_ROLE_SINK) sunk += b->particles;
                 if (nd->role != OR_ROLE_SOURCE)
                     for (uint32_t i = 0; i < b->n_in; ++i) pool_release(&pool, b->in[i].index);

@@ -120,6 +120,8 @@ typedef struct or_run_cfg {
     uint32_t num_chunks;/* S; 0 derives P + K*R */
     uint32_t mode;      /* OR_MODE_RUNNER / OR_MODE_STEP */
     uint32_t threads;   /* OpenMP threads for the batch loops (results do not depend on it) */
+    uint32_t fuse;      /* step mode: a plain transform fed by a partition bin runs inside that
+                           partition's scatter; its items go to the transform's destination */
     /* photon slab (OR_XF_PHOTON_STEP / OR_SNK_PHOTON_TALLY) */
     uint32_t layers;
     uint32_t tally_bins;
