@@ -29,12 +29,12 @@ def log(*a):
     print(*a, file=sys.stderr, flush=True)
 
 
-def build_feedback(g, torch, N, C, P, Q, schedule, seed=42, use_graph=1, fuse=None, tile_items=None):
+def build_feedback(g, torch, N, C, P, Q, schedule, seed=42, use_graph=1, fuse=False, engine=0, tile_items=None):
     counts = torch.zeros(g.FEEDBACK_HIST_N, dtype=torch.int64, device="cuda")
     flow = g.Flow(g.PAYLOAD_PARTICLE)
     src, *_ = g.feedback_network(flow, g.WeightHistogram(counts.data_ptr(), g.FEEDBACK_HIST_LO, g.FEEDBACK_HIST_N))
     flow.configure(g.Tuning(seed=seed, batch_chunks=Q, chunk_capacity=C, live_particles=P, schedule=schedule,
-                            use_graph=use_graph, tile_items=tile_items))
+                            use_graph=use_graph, tile_items=tile_items, fuse_transforms=fuse, device_engine=engine))
     return flow, src, counts
 
 
@@ -62,6 +62,8 @@ def main():
     ap.add_argument("--no-plain", action="store_true")
     ap.add_argument("--no-roofline", action="store_true")
     ap.add_argument("--verify", action="store_true", help="compare the sorted histogram with the oracle")
+    ap.add_argument("--engine", type=int, default=1, help="device engine: 0 autotuned, 1 wide, 2 micro")
+    ap.add_argument("--no-fuse", action="store_true", help="run A/B as separate transform batches")
     args = ap.parse_args()
 
     if args.gpus != 1:
@@ -74,7 +76,8 @@ def main():
 
     N, C, P, Q = int(args.items), args.capacity, args.live, args.batch
     log(f"[bench] batched form N={N} C={C} P={P} Q={Q}")
-    flow, src, counts = build_feedback(g, torch, N, C, P, Q, g.SCHEDULE_DEVICE)
+    fuse = not args.no_fuse
+    flow, src, counts = build_feedback(g, torch, N, C, P, Q, g.SCHEDULE_DEVICE, fuse=fuse, engine=args.engine)
     for i in range(args.warmup):
         dt, st = run_once(flow, src, counts, N, torch)
         log(f"[bench] warmup {i}: {dt:.3f} s, {N / dt:.4g} particles/s, steps={st['steps']}")
@@ -96,7 +99,10 @@ def main():
         "warmup": args.warmup, "ms_per_step": tmean * 1e3, "higher_is_better": True, "scaling": "weak",
         "vs_baseline": None, "dtype": "f32+u32", "data": "synthetic (device Philox source, seed 42)",
         "config": {"workload": f"particle_feedback N={N:.0e} C={C} P={P} Q={Q} seed=42",
-                   "form": "batched: device step engine, hipGraph", "engine_steps": int(st["steps"]),
+                   "form": ("batched: device step engine " + {0: "autotuned", 1: "wide (hipGraph of step kernels)",
+                                                                      2: "micro (one persistent workgroup)"}[args.engine]
+                            + (", A/B fused into the filter scatter" if fuse else "")),
+                   "engine_steps": int(st["steps"]),
                    "partition_item_passes": int(part_items), "transform_item_passes": int(xf_items),
                    "histogram_rows": int(len(rows)), "histogram_sum": row_sum, "histogram_sha256": sha},
     }
@@ -111,28 +117,32 @@ def main():
         log(f"[bench] histogram equals oracle: {ok}")
 
     if not args.no_roofline:
-        # one extra run with HIP events around every engine kernel (on the engine's stream)
+        # one extra run with HIP events around every engine kernel, on the engine's own stream
         flow.configure(g.Tuning(seed=42, batch_chunks=Q, chunk_capacity=C, live_particles=P,
-                                schedule=g.SCHEDULE_DEVICE, use_graph=2))
+                                schedule=g.SCHEDULE_DEVICE, use_graph=2, fuse_transforms=fuse,
+                                device_engine=args.engine))
         counts.zero_()
         flow.run(src, N)
         ms, n = flow.kernel_times()
         part_items, xf_items = flow.item_passes()
-        # algorithmic bytes of the step kernels: 24 B per partition item pass (one 12 B read and
-        # one 12 B write of the payload) and per transform item pass; the source writes 12 B and
-        # the sink reads 12 B per particle.
+        # algorithmic bytes: 24 B per partition or transform item pass (one 12 B read and one 12 B
+        # write of the payload), 12 B per generated particle (write) and per sunk particle (read)
         alg_bytes = 24 * part_items + 24 * xf_items + 12 * N + 12 * N
-        kb_ms, kb_n = ms[1], n[1]
-        achieved = alg_bytes / (kb_ms * 1e-3) / 1e9 if kb_ms > 0 else 0.0
+        if n[3]:  # micro engine: one persistent kernel does every phase
+            k_ms, k_n, kname = ms[3], n[3], "k_micro (whole steps in one workgroup)"
+            others = {}
+        else:
+            k_ms, k_n, kname = ms[2], n[2], "k_engine_kb (scatter+transform+source+sink of one step)"
+            others = {"ka_avg_launch_us": ms[0] * 1e3 / max(n[0], 1), "kp_avg_launch_us": ms[1] * 1e3 / max(n[1], 1)}
+        achieved = alg_bytes / (k_ms * 1e-3) / 1e9 if k_ms > 0 else 0.0
         result["roofline"] = {
-            "bound": "hbm", "kernel": "k_engine_kb (scatter+transform+source+sink of one step)",
-            "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS,
-            "traffic": None, "algorithmic_bytes_per_launch": alg_bytes / max(kb_n, 1),
-            "avg_launch_us": kb_ms * 1e3 / max(kb_n, 1), "launches": kb_n,
-            "ka_avg_launch_us": ms[0] * 1e3 / max(n[0], 1),
+            "bound": "hbm", "kernel": kname, "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+            "frac": achieved / HBM_PEAK_GBS, "traffic": None,
+            "algorithmic_bytes_per_launch": alg_bytes / max(k_n, 1), "avg_launch_us": k_ms * 1e3 / max(k_n, 1),
+            "launches": k_n, **others,
         }
-        log(f"[bench] roofline: KB avg {kb_ms * 1e3 / max(kb_n, 1):.2f} us over {kb_n} launches, "
-            f"KA avg {ms[0] * 1e3 / max(n[0], 1):.2f} us, achieved {achieved:.1f} GB/s")
+        log(f"[bench] roofline: {kname} avg {k_ms * 1e3 / max(k_n, 1):.2f} us over {k_n} launches, "
+            f"achieved {achieved:.1f} GB/s ({achieved / HBM_PEAK_GBS:.2e} of peak)")
 
     forms = {}
     if not args.no_plain:
