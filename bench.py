@@ -29,12 +29,19 @@ def log(*a):
     print(*a, file=sys.stderr, flush=True)
 
 
-def build_feedback(g, torch, N, C, P, Q, schedule, seed=42, use_graph=1, fuse=False, engine=0, tile_items=None):
+ENGINES = {0: "plain host-planned", 1: "wide (hipGraph of step kernels)", 2: "micro (one persistent workgroup)",
+           3: "team (one persistent launch, workgroups of one XCD, replicated scheduler)",
+           4: "team handing over to wide"}
+
+
+def build_feedback(g, torch, N, C, P, Q, schedule, seed=42, use_graph=1, fuse=False, engine=0, tile_items=None,
+                   team=0):
     counts = torch.zeros(g.FEEDBACK_HIST_N, dtype=torch.int64, device="cuda")
     flow = g.Flow(g.PAYLOAD_PARTICLE)
     src, *_ = g.feedback_network(flow, g.WeightHistogram(counts.data_ptr(), g.FEEDBACK_HIST_LO, g.FEEDBACK_HIST_N))
     flow.configure(g.Tuning(seed=seed, batch_chunks=Q, chunk_capacity=C, live_particles=P, schedule=schedule,
-                            use_graph=use_graph, tile_items=tile_items, fuse_transforms=fuse, device_engine=engine))
+                            use_graph=use_graph, tile_items=tile_items, fuse_transforms=fuse, device_engine=engine,
+                            max_workgroups=team or None))
     return flow, src, counts
 
 
@@ -62,7 +69,8 @@ def main():
     ap.add_argument("--no-plain", action="store_true")
     ap.add_argument("--no-roofline", action="store_true")
     ap.add_argument("--verify", action="store_true", help="compare the sorted histogram with the oracle")
-    ap.add_argument("--engine", type=int, default=1, help="device engine: 0 autotuned, 1 wide, 2 micro")
+    ap.add_argument("--engine", type=int, default=0, help="device engine: 0 autotuned, 1 wide, 2 micro, 3 team")
+    ap.add_argument("--team", type=int, default=0, help="team engine: workgroups in the team (0 = default)")
     ap.add_argument("--no-fuse", action="store_true", help="run A/B as separate transform batches")
     args = ap.parse_args()
 
@@ -77,7 +85,8 @@ def main():
     N, C, P, Q = int(args.items), args.capacity, args.live, args.batch
     log(f"[bench] batched form N={N} C={C} P={P} Q={Q}")
     fuse = not args.no_fuse
-    flow, src, counts = build_feedback(g, torch, N, C, P, Q, g.SCHEDULE_DEVICE, fuse=fuse, engine=args.engine)
+    flow, src, counts = build_feedback(g, torch, N, C, P, Q, g.SCHEDULE_DEVICE, fuse=fuse, engine=args.engine,
+                                       team=args.team)
     for i in range(args.warmup):
         dt, st = run_once(flow, src, counts, N, torch)
         log(f"[bench] warmup {i}: {dt:.3f} s, {N / dt:.4g} particles/s, steps={st['steps']}")
@@ -99,8 +108,7 @@ def main():
         "warmup": args.warmup, "ms_per_step": tmean * 1e3, "higher_is_better": True, "scaling": "weak",
         "vs_baseline": None, "dtype": "f32+u32", "data": "synthetic (device Philox source, seed 42)",
         "config": {"workload": f"particle_feedback N={N:.0e} C={C} P={P} Q={Q} seed=42",
-                   "form": ("batched: device step engine " + {0: "autotuned", 1: "wide (hipGraph of step kernels)",
-                                                                      2: "micro (one persistent workgroup)"}[args.engine]
+                   "form": ("batched: device engine " + ENGINES[int(st["engine"])]
                             + (", A/B fused into the filter scatter" if fuse else "")),
                    "engine_steps": int(st["steps"]),
                    "partition_item_passes": int(part_items), "transform_item_passes": int(xf_items),
@@ -120,15 +128,21 @@ def main():
         # one extra run with HIP events around every engine kernel, on the engine's own stream
         flow.configure(g.Tuning(seed=42, batch_chunks=Q, chunk_capacity=C, live_particles=P,
                                 schedule=g.SCHEDULE_DEVICE, use_graph=2, fuse_transforms=fuse,
-                                device_engine=args.engine))
+                                device_engine=args.engine, max_workgroups=args.team or None))
         counts.zero_()
-        flow.run(src, N)
+        tst = flow.run(src, N)
         ms, n = flow.kernel_times()
         part_items, xf_items = flow.item_passes()
         # algorithmic bytes: 24 B per partition or transform item pass (one 12 B read and one 12 B
         # write of the payload), 12 B per generated particle (write) and per sunk particle (read)
         alg_bytes = 24 * part_items + 24 * xf_items + 12 * N + 12 * N
-        if n[3]:  # micro engine: one persistent kernel does every phase
+        if n[3] and int(tst["engine"]) == 3:  # team engine: one persistent launch runs many steps
+            k_ms, k_n, kname = ms[3], n[3], "k_team (whole steps, team of workgroups on one XCD)"
+            phases, psteps = flow.phase_profile()
+            others = {"steps_per_launch": int(tst["steps"]) / max(n[3], 1), "avg_step_us": ms[3] * 1e3 / max(int(tst["steps"]), 1),
+                      "phase_us_per_step": {k: round(v, 3) for k, v in phases.items()}}
+            log(f"[bench] team phases (us/step over {psteps} steps): {others['phase_us_per_step']}")
+        elif n[3]:  # micro engine: one persistent kernel does every phase
             k_ms, k_n, kname = ms[3], n[3], "k_micro (whole steps in one workgroup)"
             others = {}
         else:

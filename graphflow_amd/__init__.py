@@ -98,7 +98,7 @@ class Stats(C.Structure):
     _fields_ = [("completed_batches", C.c_uint64), ("steps", C.c_uint64), ("generated", C.c_uint64),
                 ("sunk", C.c_uint64), ("pool_chunks", C.c_uint32), ("peak_live_chunks", C.c_uint32),
                 ("chunk_capacity", C.c_uint32), ("live_particles", C.c_uint32), ("kernel_launches", C.c_uint64),
-                ("wall_ms", C.c_double)]
+                ("wall_ms", C.c_double), ("engine", C.c_uint32), ("handoff_step", C.c_uint32)]
 
     def as_dict(self):
         return {k: getattr(self, k) for k, _ in self._fields_}
@@ -440,7 +440,7 @@ class Tuning:
     use_graph: int = 1
     fuse_transforms: bool = False
     max_workgroups: Optional[int] = None
-    device_engine: int = 0  # 0 autotuned, 1 wide step kernels, 2 one persistent workgroup
+    device_engine: int = 0  # 0 autotuned, 1 wide step kernels, 2 one persistent workgroup, 3 XCD team
     photon: Optional[dict] = None
 
     def validate(self):
@@ -557,9 +557,14 @@ class Flow:
         return list(ms), list(n)
 
     def phase_profile(self):
-        """Micro engine per-phase wall-clock microseconds per step (run with use_graph=2)."""
+        """Per-phase wall-clock microseconds per step of the micro or team engine (run with
+        use_graph=2; the device wall clock ticks at 100 MHz)."""
         a = (C.c_uint64 * 16)()
         _raise(lib().gf_flow_phase_profile(self.handle, a, 16))
+        if int(a[14]):
+            steps = int(a[14])
+            names = {9: "barrier_wait", 10: "counts", 11: "plan", 12: "segments", 13: "arrive_select"}
+            return {n: a[i] / 100.0 / steps for i, n in names.items()}, steps
         steps = max(int(a[8]), 1)
         names = ["count", "plan", "source", "partition", "transform", "sink", "retire", "select"]
         return {n: a[i] / 100.0 / steps for i, n in enumerate(names)}, int(a[8])

@@ -88,6 +88,17 @@ struct Cols {
     void* c[9];
 };
 
+// Relaxed agent-scope loads: they bypass the per-CU L1, so a workgroup reads what another
+// workgroup of the same XCD stored before a barrier, never a stale L1 line.
+__device__ inline uint32_t ld_u32(const void* p, size_t i)
+{
+    return __hip_atomic_load((const uint32_t*)p + i, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+__device__ inline uint64_t ld_u64(const void* p, size_t i)
+{
+    return __hip_atomic_load((const unsigned long long*)p + i, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+
 struct Particle {
     float w;
     uint32_t id, gen;
@@ -97,6 +108,14 @@ struct Particle {
         p.w = ((const float*)k.c[0])[s];
         p.id = ((const uint32_t*)k.c[1])[s];
         p.gen = ((const uint32_t*)k.c[2])[s];
+        return p;
+    }
+    __device__ static Particle load_team(const Cols& k, size_t s)
+    {
+        Particle p;
+        p.w = __uint_as_float(ld_u32(k.c[0], s));
+        p.id = ld_u32(k.c[1], s);
+        p.gen = ld_u32(k.c[2], s);
         return p;
     }
     __device__ void store(const Cols& k, size_t s) const
@@ -115,6 +134,13 @@ struct Keyed {
         Keyed p;
         p.key = ((const uint32_t*)k.c[0])[s];
         p.val = ((const uint64_t*)k.c[1])[s];
+        return p;
+    }
+    __device__ static Keyed load_team(const Cols& k, size_t s)
+    {
+        Keyed p;
+        p.key = ld_u32(k.c[0], s);
+        p.val = ld_u64(k.c[1], s);
         return p;
     }
     __device__ void store(const Cols& k, size_t s) const
@@ -139,6 +165,20 @@ struct Photon {
         p.w = ((const float*)k.c[6])[s];
         p.id = ((const uint32_t*)k.c[7])[s];
         p.step = ((const uint32_t*)k.c[8])[s];
+        return p;
+    }
+    __device__ static Photon load_team(const Cols& k, size_t s)
+    {
+        Photon p;
+        p.x = __uint_as_float(ld_u32(k.c[0], s));
+        p.y = __uint_as_float(ld_u32(k.c[1], s));
+        p.z = __uint_as_float(ld_u32(k.c[2], s));
+        p.ux = __uint_as_float(ld_u32(k.c[3], s));
+        p.uy = __uint_as_float(ld_u32(k.c[4], s));
+        p.uz = __uint_as_float(ld_u32(k.c[5], s));
+        p.w = __uint_as_float(ld_u32(k.c[6], s));
+        p.id = ld_u32(k.c[7], s);
+        p.step = ld_u32(k.c[8], s);
         return p;
     }
     __device__ void store(const Cols& k, size_t s) const

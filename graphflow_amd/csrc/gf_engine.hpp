@@ -67,7 +67,7 @@ struct Engine {
 enum { PROF_COUNT = 0, PROF_PLAN = 1, PROF_SOURCE = 2, PROF_PART = 3, PROF_XF = 4, PROF_SINK = 5, PROF_RETIRE = 6,
        PROF_SELECT = 7, PROF_STEPS = 8, PROF_WORDS = 16 };
 
-enum { CTL_STATUS = 0, CTL_DONE = 1, CTL_TOP = 2, CTL_TICKET_A = 3, CTL_TICKET_B = 4, CTL_WORDS = 8 };
+enum { CTL_STATUS = 0, CTL_DONE = 1, CTL_TOP = 2, CTL_TICKET_A = 3, CTL_TICKET_B = 4, CTL_HANDOFF = 5, CTL_WORDS = 8 };
 enum {
     CNT_GENERATED = 0, CNT_SUNK = 1, CNT_STEP = 2, CNT_BATCHES = 3, CNT_PART_ITEMS = 4, CNT_XF_ITEMS = 5,
     CNT_SRC_ITEMS = 6, CNT_SINK_ITEMS = 7, CNT_TRACE_LEN = 8, CNT_PEAK_LIVE = 9, CNT_FUSED_ITEMS = 10,

@@ -19,7 +19,7 @@
 #include <type_traits>
 #include <vector>
 
-#include "gf_engine.hpp"
+#include "gf_team.hpp"
 
 using namespace gfd;
 
@@ -297,7 +297,7 @@ __global__ __launch_bounds__(1024) void k_micro(Engine e, uint32_t stage_mask, u
     }
 }
 
-__global__ __launch_bounds__(kThreads) void k_engine_setup(Engine e)
+__global__ __launch_bounds__(kThreads) void k_engine_setup(Engine e, uint32_t do_select)
 {
     extern __shared__ char smem[];
     SelLds& L = *reinterpret_cast<SelLds*>(smem);
@@ -318,6 +318,41 @@ __global__ __launch_bounds__(kThreads) void k_engine_setup(Engine e)
         L.top = e.S;
         L.generated = L.sunk = L.step = L.batches = 0ull;
         L.part_items = L.xf_items = L.src_items = L.sink_items = 0ull;
+    }
+    __syncthreads();
+    if (do_select) {
+        engine_select(e, L);
+        return;
+    }
+    // pre-select state for the team engine: full free stack, empty rings, zero counters
+    for (uint32_t k = tid; k < e.n_nodes; k += kThreads) e.st[k] = L.st[k];
+    if (tid == 0) e.ctl[CTL_TOP] = e.S;
+}
+
+// Continue a run on the wide engine from the pre-select state another engine left behind.
+__global__ __launch_bounds__(kThreads) void k_engine_resume(Engine e)
+{
+    extern __shared__ char smem[];
+    SelLds& L = *reinterpret_cast<SelLds*>(smem);
+    const uint32_t tid = threadIdx.x;
+    load_states(e, L.st);
+    if (tid == 0) {
+        L.top = e.ctl[CTL_TOP];
+        L.generated = e.ctr[CNT_GENERATED];
+        L.sunk = e.ctr[CNT_SUNK];
+        L.step = e.ctr[CNT_STEP];
+        L.batches = e.ctr[CNT_BATCHES];
+        L.part_items = e.ctr[CNT_PART_ITEMS];
+        L.xf_items = e.ctr[CNT_XF_ITEMS];
+        L.src_items = e.ctr[CNT_SRC_ITEMS];
+        L.sink_items = e.ctr[CNT_SINK_ITEMS];
+        e.ctl[CTL_HANDOFF] = 0u;
+    }
+    __syncthreads();
+    for (uint32_t k = tid; k < e.n_nodes; k += kThreads) {
+        L.tail[k] = L.st[k].tail;
+        L.head[k] = L.st[k].head;
+        L.st[k].active = 0u;
     }
     __syncthreads();
     engine_select(e, L);
@@ -864,6 +899,10 @@ struct EngineBuffers {
     bool micro = false;
     uint32_t group_lds = 0;
     size_t micro_lds = 0;
+    // team engine
+    bool team = false;
+    TeamArgs ta{};
+    size_t cnt_bytes = 0;
     hipGraphExec_t exec = nullptr;
     uint32_t graph_steps = 0;
     ~EngineBuffers()
@@ -1315,8 +1354,38 @@ void build_engine(gf_flow& f, uint64_t N, const RunShape& sh, uint32_t src)
     eb->micro_lds = std::max(std::max(sizeof(PlanLds), sizeof(SelLds)), 4u * group_lds);
     const bool micro_fits = eb->micro_lds <= 160u * 1024u;
     if (t.device_engine == 2u && !micro_fits) fail(GF_BAD_ARGUMENT, "graphFlow: micro engine LDS budget exceeded");
-    eb->micro = t.device_engine == 2u ||
-                (t.device_engine == 0u && micro_fits && (uint64_t)sh.P * rec <= (4ull << 20));
+    // team engine: every partition narrow enough for a per-segment count row in LDS
+    uint32_t bb = 1;
+    bool team_ok = n <= kTeamNodes && routes.size() <= kTeamRoutes && sh.C / kSeg < (1u << 18);
+    for (uint32_t k = 0; k < n; ++k)
+        if (arr[k].kind == KIND_PARTITION) bb = std::max(bb, nds[k].bins + 1u);
+    if (bb > kTeamBins) team_ok = false;
+    const uint32_t tpc = (uint32_t)ceil_div(sh.C, kSeg);
+    const uint64_t cnt_words = (uint64_t)sh.S * 2u * tpc * bb;
+    if (cnt_words * 4u > (1ull << 31)) team_ok = false;
+    if (t.device_engine == 3u && !team_ok)
+        fail(GF_BAD_ARGUMENT, "graphFlow: the team engine needs at most 16 nodes and partitions of at most 31 bins");
+    eb->team = team_ok && (t.device_engine == 3u || t.device_engine == 0u);
+    if (eb->team) {
+        TeamArgs& ta = eb->ta;
+        ta.cnt = m.d<uint32_t>(cnt_words);
+        ta.zero_g = m.d<uint32_t>(1u + kTeamZero);
+        ta.tw = m.d<uint32_t>(TW_WORDS);
+        ta.K = std::min<uint32_t>(t.max_workgroups ? t.max_workgroups : 16u, 32u);
+        ta.tpc = tpc;
+        ta.bb = bb;
+        ta.max_steps = t.steps_per_launch ? t.steps_per_launch : 65536u;
+        uint32_t rc = 1;
+        while (rc * 2u <= kTeamRing / n) rc *= 2u;
+        ta.rc = std::min<uint32_t>(rc, 1024u);
+        ta.n_routes = (uint32_t)routes.size();
+        eb->cnt_bytes = cnt_words * 4u;
+        HIPX(hipFuncSetAttribute((const void*)k_team<Particle>, hipFuncAttributeMaxDynamicSharedMemorySize, (int)sizeof(TeamLds)));
+        HIPX(hipFuncSetAttribute((const void*)k_team<Keyed>, hipFuncAttributeMaxDynamicSharedMemorySize, (int)sizeof(TeamLds)));
+        HIPX(hipFuncSetAttribute((const void*)k_team<Photon>, hipFuncAttributeMaxDynamicSharedMemorySize, (int)sizeof(TeamLds)));
+    }
+    eb->micro = !eb->team && (t.device_engine == 2u ||
+                              (t.device_engine == 0u && micro_fits && (uint64_t)sh.P * rec <= (4ull << 20)));
     with_payload(f.payload, r, [&](auto pt, auto rc) {
         using P = typename decltype(pt)::type;
         constexpr int RR = decltype(rc)::value;
@@ -1391,18 +1460,93 @@ void run_micro(gf_flow& f, EngineBuffers& eb, uint64_t N, gf_stats& st, bool tim
     finish_device(f, eb, st, ctl, launches);
 }
 
+void run_wide(gf_flow& f, EngineBuffers& eb, uint64_t N, gf_stats& st, bool timed, uint64_t launches0);
+
+// Team engine host loop: each launch forms the team on one XCD and runs up to max_steps steps; a
+// step that exceeds the team's LDS limits hands the rest of the run to the wide engine.
+void run_team(gf_flow& f, EngineBuffers& eb, uint64_t N, gf_stats& st, bool timed)
+{
+    hipStream_t s = f.ctx->stream;
+    TeamArgs& ta = eb.ta;
+    ta.e = eb.e;
+    HIPX(hipMemsetAsync(ta.cnt, 0, eb.cnt_bytes, s));
+    HIPX(hipMemsetAsync(ta.zero_g, 0, sizeof(uint32_t), s));
+    uint32_t ctl[CTL_WORDS];
+    uint64_t launches = 0, steps_launched = 0;
+    const uint64_t max_steps = 64ull * (N + 1024u) + 4096u;
+    hipEvent_t ev[2] = {nullptr, nullptr};
+    if (timed)
+        for (auto& x : ev) HIPX(hipEventCreate(&x));
+    for (int i = 0; i < 4; ++i) {
+        f.kt_ms[i] = 0.0;
+        f.kt_n[i] = 0;
+    }
+    const uint32_t grid = 8u * ta.K;
+    for (;;) {
+        HIPX(hipMemsetAsync(ta.tw, 0, TW_WORDS * sizeof(uint32_t), s));
+        if (timed) HIPX(hipEventRecord(ev[0], s));
+        with_payload(f.payload, 1u, [&](auto pt, auto) {
+            using P = typename decltype(pt)::type;
+            hipLaunchKernelGGL(k_team<P>, dim3(grid), dim3(kTeamThreads), sizeof(TeamLds), s, ta);
+        });
+        HIPX(hipGetLastError());
+        if (timed) HIPX(hipEventRecord(ev[1], s));
+        ++launches;
+        steps_launched += ta.max_steps;
+        HIPX(hipMemcpyAsync(ctl, eb.e.ctl, sizeof(ctl), hipMemcpyDeviceToHost, s));
+        HIPX(hipStreamSynchronize(s));
+        if (timed) {
+            float x = 0;
+            HIPX(hipEventElapsedTime(&x, ev[0], ev[1]));
+            f.kt_ms[3] += x;
+            f.kt_n[3]++;
+        }
+        if (ctl[CTL_STATUS] || ctl[CTL_DONE]) break;
+        if (ctl[CTL_HANDOFF]) {
+            if (timed)
+                for (auto& x : ev) (void)hipEventDestroy(x);
+            unsigned long long step = 0;
+            HIPX(hipMemcpy(&step, eb.e.ctr + CNT_STEP, sizeof(step), hipMemcpyDeviceToHost));
+            st.handoff_step = (uint32_t)step;
+            st.engine = 4;
+            hipLaunchKernelGGL(k_engine_resume, dim3(1), dim3(kThreads), sizeof(SelLds), s, eb.e);
+            HIPX(hipGetLastError());
+            run_wide(f, eb, N, st, timed, launches);
+            st.engine = 4;
+            return;
+        }
+        if (steps_launched > max_steps) fail(GF_STALL, "graphFlow: device engine exceeded its step bound");
+    }
+    if (timed)
+        for (auto& x : ev) (void)hipEventDestroy(x);
+    finish_device(f, eb, st, ctl, launches);
+}
+
 void run_device(gf_flow& f, uint32_t src, uint64_t N, const RunShape& sh, gf_stats& st, bool timed)
 {
     const std::string key = shape_key(f, sh, N);
     if (!f.eng || f.eng->key != key) build_engine(f, N, sh, src);
     EngineBuffers& eb = *f.eng;
     hipStream_t s = f.ctx->stream;
-    hipLaunchKernelGGL(k_engine_setup, dim3(1), dim3(kThreads), sizeof(SelLds), s, eb.e);
+    hipLaunchKernelGGL(k_engine_setup, dim3(1), dim3(kThreads), sizeof(SelLds), s, eb.e, eb.team ? 0u : 1u);
     HIPX(hipGetLastError());
+    if (eb.team) {
+        st.engine = 3;
+        run_team(f, eb, N, st, timed);
+        return;
+    }
     if (eb.micro) {
+        st.engine = 2;
         run_micro(f, eb, N, st, timed);
         return;
     }
+    st.engine = 1;
+    run_wide(f, eb, N, st, timed, 0);
+}
+
+void run_wide(gf_flow& f, EngineBuffers& eb, uint64_t N, gf_stats& st, bool timed, uint64_t launches0)
+{
+    hipStream_t s = f.ctx->stream;
     const uint32_t K = f.tuning.steps_per_launch ? f.tuning.steps_per_launch : 64u;
     const bool use_graph = f.tuning.use_graph == 1u && !timed;
     if (use_graph && (!eb.exec || eb.graph_steps != K)) {
@@ -1419,7 +1563,7 @@ void run_device(gf_flow& f, uint32_t src, uint64_t N, const RunShape& sh, gf_sta
         eb.graph_steps = K;
     }
     uint32_t ctl[CTL_WORDS];
-    uint64_t launches = 0;
+    uint64_t launches = launches0;
     hipEvent_t ev[4] = {nullptr, nullptr, nullptr, nullptr};
     if (timed)
         for (auto& x : ev) HIPX(hipEventCreate(&x));

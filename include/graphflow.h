@@ -107,9 +107,12 @@ typedef struct gf_tuning {
     uint32_t use_graph;      /* 0 direct launches, 1 hipGraph of K steps, 2 direct launches timed with HIP events */
     uint32_t fuse_transforms;/* device schedule: apply a plain transform fed by a partition bin inside
                                 that partition's scatter and route its items to the transform's destination */
-    uint32_t max_workgroups; /* device schedule: workgroups per step kernel (0 -> 64) */
+    uint32_t max_workgroups; /* device schedule: workgroups per step kernel (0 -> 64); team size of
+                                the team engine (0 -> 16, at most 32) */
     uint32_t device_engine;  /* device schedule: 0 autotuned, 1 wide (multi-workgroup step kernels in a
-                                hipGraph), 2 micro (one persistent 1024-thread workgroup runs the steps) */
+                                hipGraph), 2 micro (one persistent 1024-thread workgroup runs the steps),
+                                3 team (one persistent launch, a team of workgroups on one XCD with a
+                                replicated scheduler and one barrier per step) */
     uint32_t photon_layers, photon_tally_bins;
     double photon_layer_mfp, photon_mu_t, photon_albedo, photon_hg_g, photon_roulette_w, photon_roulette_p;
 } gf_tuning;
@@ -119,6 +122,9 @@ typedef struct gf_stats {
     uint32_t pool_chunks, peak_live_chunks, chunk_capacity, live_particles;
     uint64_t kernel_launches;
     double wall_ms;
+    uint32_t engine;         /* engine that ran: 0 plain host-planned, 1 wide, 2 micro, 3 team,
+                                4 team handing over to wide */
+    uint32_t handoff_step;   /* step at which the team handed over (engine 4)              */
 } gf_stats;
 
 typedef struct gf_context gf_context;

@@ -37,9 +37,9 @@ def feedback_flow(N, *, distribution="parity", destinations=2, sink="record", mu
     return flow, src, buf
 
 
-def run_flow(flow, src, N, *, C, P, Q, schedule, fuse=False, engine=0, trace=True, tile_items=None):
+def run_flow(flow, src, N, *, C, P, Q, schedule, fuse=False, engine=0, trace=True, tile_items=None, **kw):
     flow.configure(g.Tuning(seed=SEED, batch_chunks=Q, chunk_capacity=C, live_particles=P, schedule=schedule,
-                            trace=trace, fuse_transforms=fuse, device_engine=engine, tile_items=tile_items))
+                            trace=trace, fuse_transforms=fuse, device_engine=engine, tile_items=tile_items, **kw))
     stats = flow.run(src, N)
     return stats, (flow.trace() if trace else None)
 

@@ -20,6 +20,10 @@ CASES = [
     (20000, 1024, 2048, 8, g.SCHEDULE_DEVICE, True, 2),
     (100000, 8192, 16384, 8, g.SCHEDULE_DEVICE, True, 2),
     (3000, 1000, 1500, 3, g.SCHEDULE_DEVICE, False, 1),  # non-power-of-two C, tail tiles
+    (20000, 1024, 2048, 8, g.SCHEDULE_DEVICE, False, 3),
+    (20000, 1024, 2048, 8, g.SCHEDULE_DEVICE, True, 3),
+    (100000, 8192, 16384, 8, g.SCHEDULE_DEVICE, True, 3),
+    (3000, 1000, 1500, 3, g.SCHEDULE_DEVICE, False, 3),  # non-power-of-two C, tail segments
 ]
 
 
@@ -60,7 +64,7 @@ def test_config1_histogram_rows_and_sha():
     assert g.rows_sha256(rows) == g.rows_sha256(ref_rows)
 
 
-@pytest.mark.parametrize("engine", [1, 2])
+@pytest.mark.parametrize("engine", [1, 2, 3])
 def test_config2_shape_histogram(engine):
     """Config 2 shape (C=8192, P=16384, Q=8) at N=2e6 on the batched form."""
     N = 2000000
@@ -75,12 +79,13 @@ def test_config2_shape_histogram(engine):
 @pytest.mark.parametrize("distribution,destinations", [("parity", 2), ("skew_90_10", 2), ("all_one", 2),
                                                        ("uniform", 16)])
 @pytest.mark.parametrize("fuse", [False, True])
-def test_routing_distributions(distribution, destinations, fuse):
+@pytest.mark.parametrize("engine", [1, 3])
+def test_routing_distributions(distribution, destinations, fuse, engine):
     """The four routing distributions of config 2 at C=8192, P=16384, Q=8 (reduced N)."""
     torch = torch_cuda()
     N = 60000
     flow, src, rec = feedback_flow(N, distribution=distribution, destinations=destinations)
-    stats, trace = run_flow(flow, src, N, C=8192, P=16384, Q=8, schedule=g.SCHEDULE_DEVICE, fuse=fuse, engine=1)
+    stats, trace = run_flow(flow, src, N, C=8192, P=16384, Q=8, schedule=g.SCHEDULE_DEVICE, fuse=fuse, engine=engine)
     ref = run_oracle(flow, N, C=8192, P=16384, Q=8, schedule=g.SCHEDULE_DEVICE, fuse=fuse, record=True)
     assert ref["status"] == O.OK, ref["message"]
     assert_same_trace(trace, ref["trace"])
@@ -92,11 +97,12 @@ def test_routing_distributions(distribution, destinations, fuse):
 
 
 def test_device_engines_agree_on_large_run():
-    """wide and micro engines produce the same trace on a config-2 shaped run (unfused)."""
+    """wide, micro and team engines produce the same trace on a config-2 shaped run (unfused)."""
     N = 50000
     traces = []
-    for engine in (1, 2):
+    for engine in (1, 2, 3):
         flow, src, _ = feedback_flow(N)
         _, trace = run_flow(flow, src, N, C=8192, P=16384, Q=8, schedule=g.SCHEDULE_DEVICE, engine=engine)
         traces.append(trace)
     assert_same_trace(traces[0], traces[1])
+    assert_same_trace(traces[0], traces[2])
