@@ -559,12 +559,19 @@ class Flow:
     def phase_profile(self):
         """Per-phase wall-clock microseconds per step of the micro or team engine (run with
         use_graph=2; the device wall clock ticks at 100 MHz)."""
-        a = (C.c_uint64 * 16)()
-        _raise(lib().gf_flow_phase_profile(self.handle, a, 16))
+        a = (C.c_uint64 * 32)()
+        _raise(lib().gf_flow_phase_profile(self.handle, a, 32))
         if int(a[14]):
             steps = int(a[14])
-            names = {9: "barrier_wait", 10: "counts", 11: "plan", 12: "segments", 13: "arrive_select"}
-            return {n: a[i] / 100.0 / steps for i, n in names.items()}, steps
+            names = {9: "barrier_wait", 10: "counts", 11: "plan", 12: "segments", 13: "arrive_select",
+                     15: "wave0_segments", 16: "wave4_retire", 17: "wave4_select"}
+            out = {n: a[i] / 100.0 / steps for i, n in names.items()}
+            cyc = {18: "sel_fills", 19: "sel_credit", 20: "sel_check", 21: "sel_lists", 22: "ret_out", 23: "ret_in",
+                   24: "plan_nodes", 25: "plan_pops", 26: "cnt_clear", 27: "cnt_scan", 28: "seg_rank",
+                   29: "seg_store", 30: "seg_agg", 31: "step"}
+            for i, nm in cyc.items():
+                out["cycles_" + nm] = a[i] / steps
+            return out, steps
         steps = max(int(a[8]), 1)
         names = ["count", "plan", "source", "partition", "transform", "sink", "retire", "select"]
         return {n: a[i] / 100.0 / steps for i, n in enumerate(names)}, int(a[8])

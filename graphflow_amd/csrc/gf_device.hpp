@@ -237,35 +237,46 @@ __device__ inline uint32_t route_of(uint32_t dist, uint32_t ndest, uint32_t h)
     }
 }
 
-// Classifier functors (pure functions of the payload, Concepts.hpp:134-145).
-__device__ inline uint32_t classify(const gf_node_desc& nd, const Particle& p)
+// Classifier functors (pure functions of the payload, Concepts.hpp:134-145). classify_p takes the
+// scalar parameters a classifier reads (op, bins, u[0], f[0]); the table classifier also needs the
+// descriptor and reports kNeedDesc there.
+constexpr uint32_t kNeedDesc = 0xFFFFFFFEu;
+
+__device__ inline uint32_t classify_p(uint32_t op, uint32_t bins, uint32_t u0, float f0, const Particle& p)
 {
-    switch (nd.op) {
+    switch (op) {
     case GF_CLS_WEIGHT_ROUTE:
-        if (p.w < nd.f[0]) return 0u;
-        return 1u + route_of(nd.u[0], nd.bins - 1u, fmix32(p.id));
+        if (p.w < f0) return 0u;
+        return 1u + route_of(u0, bins - 1u, fmix32(p.id));
     case GF_CLS_ID_MOD:
-        if (nd.u[0] == 1u) return 0u;
-        if (nd.u[0] == 2u) return (p.id % 8u == 0u) ? p.id % nd.bins : 0u;
-        return p.id % nd.bins;
-    case GF_CLS_LUT: {
-        uint32_t a = p.id < nd.u[0] ? p.id : nd.u[0] - 1u;
-        uint32_t g = p.gen < nd.u[1] ? p.gen : nd.u[1] - 1u;
-        return ((const uint32_t*)nd.ptr[0])[nd.u[3] + a * nd.u[1] + g];
-    }
+        if (u0 == 1u) return 0u;
+        if (u0 == 2u) return (p.id % 8u == 0u) ? p.id % bins : 0u;
+        return p.id % bins;
+    case GF_CLS_LUT: return kNeedDesc;
     default: return 0xFFFFFFFFu;
     }
 }
-__device__ inline uint32_t classify(const gf_node_desc& nd, const Keyed& p)
+__device__ inline uint32_t classify_p(uint32_t op, uint32_t bins, uint32_t, float, const Keyed& p)
 {
-    if (nd.op == GF_CLS_KEY_BUCKET) return fmix32(p.key) % nd.bins;
+    if (op == GF_CLS_KEY_BUCKET) return fmix32(p.key) % bins;
     return 0xFFFFFFFFu;
 }
-__device__ inline uint32_t classify(const gf_node_desc& nd, const Photon& p)
+__device__ inline uint32_t classify_p(uint32_t op, uint32_t, uint32_t, float, const Photon& p)
 {
-    if (nd.op == GF_CLS_PHOTON_ALIVE) return p.w > 0.0f ? 0u : 1u;
+    if (op == GF_CLS_PHOTON_ALIVE) return p.w > 0.0f ? 0u : 1u;
     return 0xFFFFFFFFu;
 }
+
+__device__ inline uint32_t classify(const gf_node_desc& nd, const Particle& p)
+{
+    const uint32_t c = classify_p(nd.op, nd.bins, nd.u[0], nd.f[0], p);
+    if (c != kNeedDesc) return c;
+    const uint32_t a = p.id < nd.u[0] ? p.id : nd.u[0] - 1u;
+    const uint32_t g = p.gen < nd.u[1] ? p.gen : nd.u[1] - 1u;
+    return ((const uint32_t*)nd.ptr[0])[nd.u[3] + a * nd.u[1] + g];
+}
+__device__ inline uint32_t classify(const gf_node_desc& nd, const Keyed& p) { return classify_p(nd.op, nd.bins, nd.u[0], nd.f[0], p); }
+__device__ inline uint32_t classify(const gf_node_desc& nd, const Photon& p) { return classify_p(nd.op, nd.bins, nd.u[0], nd.f[0], p); }
 
 // Partition updaters (UpdateResult, Concepts.hpp:33-39): 0 success, GF_INVALID, GF_OVERFLOW.
 __device__ inline uint32_t update(const gf_node_desc& nd, uint32_t bin, Particle& p)
@@ -337,13 +348,23 @@ __device__ inline void photon_step(const RunParams& rp, uint32_t purpose, Photon
     }
 }
 
-// Plain transform functors.
+// Plain transform functors. transform_p applies the transforms that need only (op, f[0]) and
+// reports whether it did.
+__device__ inline bool transform_p(uint32_t op, float f0, Particle& p)
+{
+    if (op == GF_XF_SCALE_WEIGHT) {
+        p.w = p.w * f0;
+        p.gen += 1u;
+        return true;
+    }
+    return false;
+}
+__device__ inline bool transform_p(uint32_t, float, Keyed&) { return false; }
+__device__ inline bool transform_p(uint32_t, float, Photon&) { return false; }
+
 __device__ inline void transform(const gf_node_desc& nd, const RunParams&, uint32_t, Particle& p)
 {
-    if (nd.op == GF_XF_SCALE_WEIGHT) {
-        p.w = p.w * nd.f[0];
-        p.gen += 1u;
-    }
+    transform_p(nd.op, nd.f[0], p);
 }
 __device__ inline void transform(const gf_node_desc&, const RunParams&, uint32_t, Keyed&) {}
 __device__ inline void transform(const gf_node_desc& nd, const RunParams& rp, uint32_t purpose, Photon& p)

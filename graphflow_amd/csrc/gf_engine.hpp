@@ -65,7 +65,7 @@ struct Engine {
 };
 
 enum { PROF_COUNT = 0, PROF_PLAN = 1, PROF_SOURCE = 2, PROF_PART = 3, PROF_XF = 4, PROF_SINK = 5, PROF_RETIRE = 6,
-       PROF_SELECT = 7, PROF_STEPS = 8, PROF_WORDS = 16 };
+       PROF_SELECT = 7, PROF_STEPS = 8, PROF_WORDS = 32 };
 
 enum { CTL_STATUS = 0, CTL_DONE = 1, CTL_TOP = 2, CTL_TICKET_A = 3, CTL_TICKET_B = 4, CTL_HANDOFF = 5, CTL_WORDS = 8 };
 enum {
@@ -121,15 +121,19 @@ __device__ inline uint32_t block_exscan_array(uint32_t* a, uint32_t n, uint32_t*
     return total;
 }
 
-// Wave-wide inclusive scan.
+// Wave-wide inclusive scan on the DPP network (no LDS round trip): Hillis-Steele inside each
+// 16-lane row with row_shr:1,2,4,8, then row_bcast:15 into rows 1 and 3 and row_bcast:31 into
+// rows 2 and 3. Lanes whose DPP source is outside the row read 0.
 __device__ inline uint32_t wave_inscan(uint32_t x)
 {
-    const uint32_t lane = lane_id();
-    for (int o = 1; o < 64; o <<= 1) {
-        uint32_t y = (uint32_t)__shfl_up((int)x, o);
-        if (lane >= (uint32_t)o) x += y;
-    }
-    return x;
+    int v = (int)x;
+    v += __builtin_amdgcn_update_dpp(0, v, 0x111, 0xF, 0xF, false);  // row_shr:1
+    v += __builtin_amdgcn_update_dpp(0, v, 0x112, 0xF, 0xF, false);  // row_shr:2
+    v += __builtin_amdgcn_update_dpp(0, v, 0x114, 0xF, 0xF, false);  // row_shr:4
+    v += __builtin_amdgcn_update_dpp(0, v, 0x118, 0xF, 0xF, false);  // row_shr:8
+    v += __builtin_amdgcn_update_dpp(0, v, 0x142, 0xA, 0xF, false);  // row_bcast:15 -> rows 1, 3
+    v += __builtin_amdgcn_update_dpp(0, v, 0x143, 0xC, 0xF, false);  // row_bcast:31 -> rows 2, 3
+    return (uint32_t)v;
 }
 
 __device__ inline Batch make_batch(const Engine& e, uint32_t k, const NodeSt& s)

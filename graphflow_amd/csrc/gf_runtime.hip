@@ -1356,7 +1356,7 @@ void build_engine(gf_flow& f, uint64_t N, const RunShape& sh, uint32_t src)
     if (t.device_engine == 2u && !micro_fits) fail(GF_BAD_ARGUMENT, "graphFlow: micro engine LDS budget exceeded");
     // team engine: every partition narrow enough for a per-segment count row in LDS
     uint32_t bb = 1;
-    bool team_ok = n <= kTeamNodes && routes.size() <= kTeamRoutes && sh.C / kSeg < (1u << 18);
+    bool team_ok = n <= kTeamNodes && routes.size() <= kTeamRoutes && ceil_div(sh.C, kSeg) <= (1u << 17);
     for (uint32_t k = 0; k < n; ++k)
         if (arr[k].kind == KIND_PARTITION) bb = std::max(bb, nds[k].bins + 1u);
     if (bb > kTeamBins) team_ok = false;
@@ -1364,7 +1364,7 @@ void build_engine(gf_flow& f, uint64_t N, const RunShape& sh, uint32_t src)
     const uint64_t cnt_words = (uint64_t)sh.S * 2u * tpc * bb;
     if (cnt_words * 4u > (1ull << 31)) team_ok = false;
     if (t.device_engine == 3u && !team_ok)
-        fail(GF_BAD_ARGUMENT, "graphFlow: the team engine needs at most 16 nodes and partitions of at most 31 bins");
+        fail(GF_BAD_ARGUMENT, "graphFlow: the team engine needs at most 32 nodes and partitions of at most 31 bins");
     eb->team = team_ok && (t.device_engine == 3u || t.device_engine == 0u);
     if (eb->team) {
         TeamArgs& ta = eb->ta;
@@ -1379,6 +1379,10 @@ void build_engine(gf_flow& f, uint64_t N, const RunShape& sh, uint32_t src)
         while (rc * 2u <= kTeamRing / n) rc *= 2u;
         ta.rc = std::min<uint32_t>(rc, 1024u);
         ta.n_routes = (uint32_t)routes.size();
+        ta.c_shift = 32u;
+        if ((sh.C & (sh.C - 1u)) == 0u)
+            for (uint32_t b = 0; b < 32u; ++b)
+                if ((1u << b) == sh.C) ta.c_shift = b;
         eb->cnt_bytes = cnt_words * 4u;
         HIPX(hipFuncSetAttribute((const void*)k_team<Particle>, hipFuncAttributeMaxDynamicSharedMemorySize, (int)sizeof(TeamLds)));
         HIPX(hipFuncSetAttribute((const void*)k_team<Keyed>, hipFuncAttributeMaxDynamicSharedMemorySize, (int)sizeof(TeamLds)));

@@ -31,7 +31,7 @@
 
 namespace gfd {
 
-constexpr uint32_t kTeamNodes = 16;
+constexpr uint32_t kTeamNodes = 32;
 constexpr uint32_t kTeamBins = 32;  // count row stride: bins of the widest partition + invalid slot
 constexpr uint32_t kTeamRoutes = 512;
 constexpr uint32_t kSeg = 256;
@@ -49,7 +49,34 @@ constexpr uint32_t kTeamSpin = 1u << 24;
 
 enum { TW_ARRIVED = 0, TW_XCC0 = 1, TW_BAR = 2, TW_ERR = 4, TW_REG = 8, TW_WORDS = 32 };  // TW_ERR + step parity
 enum { EXIT_NONE = 0, EXIT_DONE = 1, EXIT_STALL = 2, EXIT_HANDOFF = 3, EXIT_LIMIT = 4, EXIT_STATUS = 5, EXIT_SKIP = 6 };
-enum { PROF_T_WAIT = 9, PROF_T_COUNTS = 10, PROF_T_PLAN = 11, PROF_T_SEG = 12, PROF_T_SELECT = 13, PROF_T_STEPS = 14 };
+enum {
+    PROF_T_WAIT = 9, PROF_T_COUNTS = 10, PROF_T_PLAN = 11, PROF_T_SEG = 12, PROF_T_SELECT = 13, PROF_T_STEPS = 14,
+    PROF_T_SEGW = 15, PROF_T_RETIRE = 16, PROF_T_SELECTW = 17,
+    // shader-clock cycles of sub-phases (member 0, lane 0 of the wave doing the work)
+    PROF_C_SEL_FILLS = 18, PROF_C_SEL_CREDIT = 19, PROF_C_SEL_CHECK = 20, PROF_C_SEL_LISTS = 21,
+    PROF_C_RET_OUT = 22, PROF_C_RET_IN = 23, PROF_C_PLAN_NODES = 24, PROF_C_PLAN_POPS = 25,
+    PROF_C_CNT_CLEAR = 26, PROF_C_CNT_SCAN = 27, PROF_C_SEG_RANK = 28, PROF_C_SEG_STORE = 29, PROF_C_SEG_AGG = 30,
+    PROF_C_STEP = 31
+};
+
+// Sub-phase cycle counter: accumulates the shader-clock delta since the previous mark.
+struct Ticker {
+    unsigned long long* p;
+    unsigned long long t;
+    bool on;
+    __device__ Ticker(unsigned long long* prof, bool enabled) : p(prof), t(0), on(enabled && prof != nullptr)
+    {
+        if (on) t = __builtin_readcyclecounter();
+    }
+    __device__ void operator()(uint32_t slot)
+    {
+        if (on) {
+            const unsigned long long n = __builtin_readcyclecounter();
+            p[slot] += n - t;
+            t = n;
+        }
+    }
+};
 
 struct TeamArgs {
     Engine e;
@@ -57,6 +84,7 @@ struct TeamArgs {
     uint32_t* zero_g;  // [0] = n, [1..n] = chunk | (gen & 1) << 31 rows still to clear
     uint32_t* tw;      // team words, cleared before every launch
     uint32_t K, tpc, bb, max_steps, rc, n_routes;
+    uint32_t c_shift;  // log2(C) when C is a power of two, else 32
 };
 
 struct TeamLds {
@@ -73,7 +101,8 @@ struct TeamLds {
     unsigned long long first_id[kTeamNodes], digest_base[kTeamNodes];
     gf_chunk_desc in[kTeamIn];
     uint32_t in_items[kTeamIn];
-    uint32_t seg_map[kTeamSegs];  // node << 28 | input << 18 | segment of the chunk
+    uint32_t seg_map[kTeamSegs];  // seg_pack(node, input, segment of the chunk)
+    uint32_t seg_row[kTeamSegs];  // partition segments: first word of the segment's count row
     gf_chunk_desc out[kTeamOut];
     uint32_t pre[kTeamPrefix];
     uint32_t tot[kTeamNodes * kTeamBins];
@@ -84,6 +113,7 @@ struct TeamLds {
     uint32_t fs_base, top, zero_n, status, exit_code, recenter, total_out;
     uint32_t seg_err;  // a segment of this member reported a device status this step
     uint32_t reserved;  // credits the select phase reserved for this step's outputs
+    uint32_t prof_member;  // this member records sub-phase cycle counts
     uint32_t shv[8];
 };
 
@@ -96,7 +126,11 @@ __device__ inline void wave_sync()
     __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
 }
 
-__device__ inline uint32_t seg_pack(uint32_t k, uint32_t i, uint32_t gg) { return k << 28 | i << 18 | gg; }
+// Segment map entry: node (5 bits) | input chunk of the node's batch (10 bits) | segment (17 bits).
+__device__ inline uint32_t seg_pack(uint32_t k, uint32_t i, uint32_t gg) { return k << 27 | i << 17 | gg; }
+__device__ inline uint32_t seg_node(uint32_t sm) { return sm >> 27; }
+__device__ inline uint32_t seg_input(uint32_t sm) { return (sm >> 17) & 1023u; }
+__device__ inline uint32_t seg_part(uint32_t sm) { return sm & 0x1FFFFu; }
 
 // Rank of each valid lane among the lanes of its bin, after the `run` earlier items of that bin in
 // the same segment (lane b of `run` holds bin b's running count; bins < 64).
@@ -266,144 +300,200 @@ __device__ inline void team_flush(const TeamArgs& a, TeamLds& L)
     }
 }
 
+// ---- wave-level helpers ------------------------------------------------------------------------
+__device__ inline uint32_t rl(uint32_t v, uint32_t k) { return (uint32_t)__builtin_amdgcn_readlane((int)v, (int)k); }
+__device__ inline unsigned long long rl64(unsigned long long v, uint32_t k)
+{
+    return ((unsigned long long)rl((uint32_t)(v >> 32), k) << 32) | rl((uint32_t)v, k);
+}
+// Value of `v` in lane `src` (all lanes must be active).
+__device__ inline uint32_t from_lane(uint32_t v, uint32_t src) { return (uint32_t)__shfl((int)v, (int)src); }
+__device__ inline unsigned long long from_lane64(unsigned long long v, uint32_t src)
+{
+    return ((unsigned long long)from_lane((uint32_t)(v >> 32), src) << 32) | from_lane((uint32_t)v, src);
+}
+__device__ inline uint32_t wave_sum(uint32_t v) { return rl(wave_inscan(v), 63); }
+
+// Node whose [off, off + cnt) range holds item x: the last node j with cnt_j > 0 and off_j <= x
+// (off and cnt are lane-per-node values).
+__device__ inline uint32_t node_of(uint32_t x, uint32_t off, uint32_t cnt, uint32_t n)
+{
+    uint32_t kx = 0;
+    for (uint32_t j = 0; j < n; ++j)
+        if (rl(cnt, j) && x >= rl(off, j)) kx = j;
+    return kx;
+}
+
 // ---- select (wave 4): the next step's batches, or the reason to stop -------------------------
+// Lane k holds node k. Credits are handed out in node order by a scalar loop over the lanes'
+// values; every array write is lane-parallel.
 __device__ inline void team_select(const TeamArgs& a, TeamLds& L, bool at_limit)
 {
     const Engine& e = a.e;
     const uint32_t lane = lane_id(), n = e.n_nodes, C = e.C, rcm = a.rc - 1u;
-    if (lane == 0) {
-        const unsigned long long P = e.P, Q = e.Q;
-        uint32_t reserved = 0, in_tot = 0, seg_tot = 0, pre_tot = 0, any = 0;
-        unsigned long long gen = L.generated;
-        for (uint32_t k = 0; k < n; ++k) {
-            const uint32_t kind = L.kind[k];
-            const unsigned long long avail = L.tail[k] - L.head[k];
-            uint32_t nin = 0, parts = 0, nseg = 0, act = 0;
-            L.in_off[k] = in_tot;
-            L.seg_off[k] = seg_tot;
-            if (kind == KIND_SOURCE) {
-                const unsigned long long live = gen - L.sunk, remaining = e.N - gen;
-                unsigned long long min_fill = C < P ? C : P;
-                if (remaining < min_fill) min_fill = remaining;
-                if (gen < e.N && live < P && P - live >= min_fill && L.top - reserved >= 1u) {
-                    unsigned long long fill = remaining;
-                    if (fill > P - live) fill = P - live;
-                    if (fill > C) fill = C;
-                    parts = (uint32_t)fill;
-                    L.first_id[k] = gen;
-                    gen += fill;
-                    reserved += 1u;
-                    act = 1u;
-                    nseg = (parts + kSeg - 1u) / kSeg;
-                }
-            } else if (avail) {
-                const unsigned long long h = L.head[k];
-                const gf_chunk_desc* rg = L.ring + k * a.rc;
-                uint32_t want = kind == KIND_SINK ? (uint32_t)avail : (uint32_t)(avail < Q ? avail : Q);
-                for (uint32_t i = 0; i < want; ++i) parts += rg[(uint32_t)((h + i) & rcm)].fill;
-                if (kind != KIND_SINK) {
-                    const uint32_t B = kind == KIND_PARTITION ? L.nd[k].bins : 1u;
-                    while (want) {
-                        const uint64_t bound = bound_of(parts, C, B);
-                        if ((uint64_t)(L.top - reserved) >= bound) {
-                            reserved += (uint32_t)bound;
-                            break;
-                        }
-                        --want;
-                        parts -= rg[(uint32_t)((h + want) & rcm)].fill;
-                    }
-                }
-                nin = want;
-                act = nin ? 1u : 0u;
-                for (uint32_t i = 0; i < nin; ++i) nseg += (rg[(uint32_t)((h + i) & rcm)].fill + kSeg - 1u) / kSeg;
-                if (kind == KIND_PARTITION && act) {
-                    L.pre_off[k] = pre_tot;
-                    pre_tot += nseg * (L.nd[k].bins + 1u);
-                }
+    Ticker tk(e.prof, L.prof_member && lane == 0u);
+    const bool own = lane < n;
+    const uint32_t k = own ? lane : 0u;
+    const uint32_t kind = own ? L.kind[k] : 0xFFu;
+    const bool xf = kind == KIND_PARTITION || kind == KIND_TRANSFORM;
+    const uint32_t B = kind == KIND_PARTITION ? L.nd[k].bins : 1u;
+    const unsigned long long head = own ? L.head[k] : 0ull, tail = own ? L.tail[k] : 0ull;
+    const uint32_t avail = (uint32_t)(tail - head);
+    const gf_chunk_desc* rg = L.ring + k * a.rc;
+    uint32_t want = kind == KIND_SINK ? avail : xf ? (avail < e.Q ? avail : e.Q) : 0u;
+    uint32_t parts = 0;
+    for (uint32_t i = 0; i < want; ++i) parts += rg[(uint32_t)((head + i) & rcm)].fill;
+    uint32_t bound = xf ? (uint32_t)bound_of(parts, C, B) : 0u;
+    tk(PROF_C_SEL_FILLS);
+    // credits in node order
+    const uint32_t top = L.top;
+    uint32_t credits = top;
+    const unsigned long long gen0 = L.generated, sunk = L.sunk;
+    unsigned long long gen = gen0;
+    uint32_t src_fill = 0, src_node = 0xFFFFFFFFu;
+    for (uint32_t j = 0; j < n; ++j) {
+        const uint32_t kj = rl(kind, j);
+        if (kj == KIND_SOURCE) {
+            const unsigned long long P = e.P, live = gen - sunk, remaining = e.N - gen;
+            unsigned long long min_fill = C < P ? C : P;
+            if (remaining < min_fill) min_fill = remaining;
+            if (gen < e.N && live < P && P - live >= min_fill && credits >= 1u) {
+                unsigned long long fill = remaining;
+                if (fill > P - live) fill = P - live;
+                if (fill > C) fill = C;
+                src_fill = (uint32_t)fill;
+                src_node = j;
+                gen += fill;
+                credits -= 1u;
             }
-            L.active[k] = act;
-            L.n_in[k] = nin;
-            L.particles[k] = parts;
-            any |= act;
-            in_tot += nin;
-            seg_tot += nseg;
+        } else if (kj == KIND_PARTITION || kj == KIND_TRANSFORM) {
+            if (!rl(want, j)) continue;
+            if (rl(bound, j) > credits && lane == j) {
+                // largest prefix of the candidates whose bound fits the remaining credits
+                uint32_t m = want, p = parts, b = 0;
+                while (m) {
+                    b = (uint32_t)bound_of(p, C, B);
+                    if (b <= credits) break;
+                    --m;
+                    p -= rg[(uint32_t)((head + m) & rcm)].fill;
+                }
+                want = m;
+                parts = p;
+                bound = m ? b : 0u;
+            }
+            credits -= rl(bound, j);
         }
+    }
+    tk(PROF_C_SEL_CREDIT);
+    uint32_t nin = 0, act = 0, nseg = 0;
+    if (kind == KIND_SOURCE) {
+        act = lane == src_node ? 1u : 0u;
+        parts = act ? src_fill : 0u;
+        nseg = (parts + kSeg - 1u) / kSeg;
+    } else if (own) {
+        nin = want;
+        act = nin ? 1u : 0u;
+        for (uint32_t i = 0; i < nin; ++i) nseg += (rg[(uint32_t)((head + i) & rcm)].fill + kSeg - 1u) / kSeg;
+    }
+    const uint32_t prew = (kind == KIND_PARTITION && act) ? nseg * (B + 1u) : 0u;
+    const uint32_t i_inc = wave_inscan(nin), s_inc = wave_inscan(nseg), p_inc = wave_inscan(prew);
+    const uint32_t in_tot = rl(i_inc, 63), seg_tot = rl(s_inc, 63), pre_tot = rl(p_inc, 63);
+    const uint32_t reserved = top - credits;
+    const uint64_t actm = __ballot(act != 0u), pend = __ballot(own && tail != head);
+    uint32_t code = EXIT_NONE;
+    if (!actm) {
+        code = (!pend && gen0 >= e.N) ? EXIT_DONE : EXIT_STALL;
+    } else if (at_limit) {
+        code = EXIT_LIMIT;
+    } else {
+        const bool fits = in_tot <= kTeamIn && seg_tot <= kTeamSegs && pre_tot <= kTeamPrefix && reserved <= kTeamOut &&
+                          reserved <= kTeamMargin && in_tot <= kTeamMargin;
+        const bool room = !own || (avail - nin) + reserved <= a.rc;
+        if (!fits || __ballot(!room)) code = EXIT_HANDOFF;
+    }
+    tk(PROF_C_SEL_CHECK);
+    if (code) {
+        if (lane == 0) L.exit_code = code;
+        wave_sync();
+        return;
+    }
+    const uint32_t inoff = i_inc - nin, segoff = s_inc - nseg;
+    if (own) {
+        L.active[k] = act;
+        L.n_in[k] = nin;
+        L.particles[k] = parts;
+        L.in_off[k] = inoff;
+        L.seg_off[k] = segoff;
+        L.pre_off[k] = p_inc - prew;
+        L.head[k] = head + nin;
+        if (kind == KIND_SINK && act) {
+            const unsigned long long dg = L.digest[k];
+            L.digest_base[k] = dg;
+            L.digest[k] = dg + nin;
+        }
+        if (kind == KIND_SOURCE && act) L.first_id[k] = gen0;
+    }
+    if (lane == n) {
         L.in_off[n] = in_tot;
         L.seg_off[n] = seg_tot;
-        uint32_t code = EXIT_NONE;
-        if (!any) {
-            bool drained = true;
-            for (uint32_t k = 0; k < n; ++k)
-                if (L.tail[k] != L.head[k]) drained = false;
-            code = (drained && L.generated >= e.N) ? EXIT_DONE : EXIT_STALL;
-        } else if (at_limit) {
-            code = EXIT_LIMIT;
-        } else {
-            bool fits = in_tot <= kTeamIn && seg_tot <= kTeamSegs && pre_tot <= kTeamPrefix && reserved <= kTeamOut &&
-                        reserved <= kTeamMargin && in_tot <= kTeamMargin && in_tot <= kTeamZero;
-            for (uint32_t k = 0; k < n && fits; ++k)
-                if ((L.tail[k] - L.head[k]) - L.n_in[k] + reserved > a.rc) fits = false;
-            if (!fits) {
-                code = EXIT_HANDOFF;
-            } else {
-                for (uint32_t k = 0; k < n; ++k) {
-                    L.head[k] += L.n_in[k];
-                    if (L.kind[k] == KIND_SINK && L.active[k]) {
-                        L.digest_base[k] = L.digest[k];
-                        L.digest[k] += L.n_in[k];
-                    }
-                }
-                L.generated = gen;
-                L.reserved = reserved;
-                const uint32_t nb = L.top > kTeamStack / 2u ? L.top - kTeamStack / 2u : 0u;
-                L.recenter = (nb != L.fs_base && (L.top < L.fs_base + kTeamMargin || L.top + kTeamMargin > L.fs_base + kTeamStack)) ? 1u : 0u;
+    }
+    if (lane == 0) {
+        L.generated = gen;
+        L.reserved = reserved;
+        const uint32_t nb = top > kTeamStack / 2u ? top - kTeamStack / 2u : 0u;
+        const uint32_t fb = L.fs_base;
+        L.recenter = (nb != fb && (top < fb + kTeamMargin || top + kTeamMargin > fb + kTeamStack)) ? 1u : 0u;
+        L.exit_code = EXIT_NONE;
+    }
+    // input list, item prefixes, segment map and count rows, flattened over the step's inputs
+    // (node order; the source has no inputs)
+    const uint32_t nf = kind == KIND_SOURCE ? 0u : parts, nsf = kind == KIND_SOURCE ? 0u : nseg;
+    const uint32_t F = wave_inscan(nf) - nf, S = wave_inscan(nsf) - nsf;
+    uint32_t carry_f = 0, carry_s = 0;
+    for (uint32_t x0 = 0; x0 < in_tot; x0 += 64u) {
+        const uint32_t x = x0 + lane;
+        const bool v = x < in_tot;
+        const uint32_t kx = node_of(x, inoff, nin, n);
+        const uint32_t i = x - from_lane(inoff, kx);
+        const unsigned long long hx = from_lane64(head, kx);
+        const uint32_t Fx = from_lane(F, kx), Sx = from_lane(S, kx), sox = from_lane(segoff, kx);
+        const uint32_t kindx = from_lane(kind, kx);
+        gf_chunk_desc d{0, 0, 0, 0};
+        if (v) {
+            d = L.ring[kx * a.rc + (uint32_t)((hx + i) & rcm)];
+            L.in[x] = d;
+        }
+        const uint32_t fill = v ? d.fill : 0u, ns = (fill + kSeg - 1u) / kSeg;
+        const uint32_t fi = wave_inscan(fill), si = wave_inscan(ns);
+        if (v) {
+            L.in_items[x] = carry_f + fi - fill - Fx;
+            const uint32_t sp = sox + carry_s + si - ns - Sx;
+            const uint32_t row0 = ((d.index * 2u + (d.generation & 1u)) * a.tpc) * a.bb;
+            for (uint32_t g = 0; g < ns; ++g) {
+                L.seg_map[sp + g] = seg_pack(kx, i, g);
+                if (kindx == KIND_PARTITION) L.seg_row[sp + g] = row0 + g * a.bb;
             }
         }
-        L.exit_code = code;
+        carry_f += rl(fi, 63);
+        carry_s += rl(si, 63);
+    }
+    if (src_node != 0xFFFFFFFFu) {
+        const uint32_t so = rl(segoff, src_node), ns = rl(nseg, src_node);
+        for (uint32_t g = lane; g < ns; g += 64u) L.seg_map[so + g] = seg_pack(src_node, 0u, g);
     }
     wave_sync();
-    if (L.exit_code) return;
-    // input lists, item prefixes and segment maps
-    for (uint32_t k = 0; k < n; ++k) {
-        if (!L.active[k]) continue;
-        if (L.kind[k] == KIND_SOURCE) {
-            const uint32_t ns = L.seg_off[k + 1] - L.seg_off[k];
-            for (uint32_t g = lane; g < ns; g += 64u) L.seg_map[L.seg_off[k] + g] = seg_pack(k, 0u, g);
-            continue;
-        }
-        const uint32_t m = L.n_in[k];
-        const unsigned long long h0 = L.head[k] - m;
-        const gf_chunk_desc* rg = L.ring + k * a.rc;
-        uint32_t carry_i = 0, carry_s = 0;
-        for (uint32_t i0 = 0; i0 < m; i0 += 64u) {
-            const uint32_t i = i0 + lane;
-            gf_chunk_desc d{0, 0, 0, 0};
-            if (i < m) {
-                d = rg[(uint32_t)((h0 + i) & rcm)];
-                L.in[L.in_off[k] + i] = d;
-            }
-            const uint32_t fill = i < m ? d.fill : 0u, ns = (fill + kSeg - 1u) / kSeg;
-            const uint32_t inc_i = wave_inscan(fill), inc_s = wave_inscan(ns);
-            if (i < m) {
-                L.in_items[L.in_off[k] + i] = carry_i + inc_i - fill;
-                const uint32_t sp = L.seg_off[k] + carry_s + inc_s - ns;
-                for (uint32_t g = 0; g < ns; ++g) L.seg_map[sp + g] = seg_pack(k, i, g);
-            }
-            carry_i += (uint32_t)__shfl((int)inc_i, 63);
-            carry_s += (uint32_t)__shfl((int)inc_s, 63);
-        }
-    }
-    wave_sync();
+    tk(PROF_C_SEL_LISTS);
 }
 
-// ---- after the barrier: status, segment counts, clearing, free-stack window ----------------------
+// ---- after the barrier: status, clears, segment counts and their scan, free-stack window --------
 __device__ inline void team_counts(const TeamArgs& a, TeamLds& L, uint32_t me, uint32_t T, uint32_t it)
 {
     const Engine& e = a.e;
-    const uint32_t tid = threadIdx.x, n = e.n_nodes;
+    const uint32_t tid = threadIdx.x, n = e.n_nodes, w = tid >> 6, lane = lane_id();
     const uint32_t row = a.tpc * a.bb;
-    // errors members published before the barrier of the previous step (parity word: a member
-    // that is already one step ahead writes the other word)
+    Ticker tk(e.prof, L.prof_member && tid == 0u);
+    // errors members published before the previous barrier (parity word: a member already one
+    // step ahead writes the other word)
     if (tid == 0) {
         L.status = it ? ld_u32(a.tw, TW_ERR + ((it - 1u) & 1u)) : 0u;
         L.seg_err = 0u;
@@ -414,17 +504,26 @@ __device__ inline void team_counts(const TeamArgs& a, TeamLds& L, uint32_t me, u
         const uint32_t z = L.zero[x / row];
         a.cnt[((size_t)(z & 0x7FFFFFFFu) * 2u + (z >> 31)) * row + x % row] = 0u;
     }
-    // segment counts of every partition batch of this step
+    tk(PROF_C_CNT_CLEAR);
+    // wave b loads bin b's count of every segment of a partition batch and scans it in registers:
+    // L.pre[g][b] = items of bin b in the batch's segments before g, L.tot[b] = all of them
     for (uint32_t k = 0; k < n; ++k) {
         if (!L.active[k] || L.kind[k] != KIND_PARTITION) continue;
-        const uint32_t B1 = L.nd[k].bins + 1u, ns = L.seg_off[k + 1] - L.seg_off[k];
-        for (uint32_t x = tid; x < ns * B1; x += kTeamThreads) {
-            const uint32_t g = x / B1, b = x - g * B1;
-            const uint32_t sm = L.seg_map[L.seg_off[k] + g];
-            const gf_chunk_desc d = L.in[L.in_off[k] + ((sm >> 18) & 1023u)];
-            L.pre[L.pre_off[k] + x] = ld_u32(a.cnt, (((size_t)d.index * 2u + (d.generation & 1u)) * a.tpc + (sm & 0x3FFFFu)) * a.bb + b);
+        const uint32_t B1 = L.nd[k].bins + 1u, so = L.seg_off[k], ns = L.seg_off[k + 1] - so;
+        uint32_t* m = L.pre + L.pre_off[k];
+        for (uint32_t b = w; b < B1; b += kTeamWaves) {
+            uint32_t carry = 0;
+            for (uint32_t g0 = 0; g0 < ns; g0 += 64u) {
+                const uint32_t g = g0 + lane;
+                const uint32_t x = g < ns ? ld_u32(a.cnt, (size_t)L.seg_row[so + g] + b) : 0u;
+                const uint32_t inc = wave_inscan(x);
+                if (g < ns) m[g * B1 + b] = carry + inc - x;
+                carry += rl(inc, 63);
+            }
+            if (lane == 0) L.tot[k * kTeamBins + b] = carry;
         }
     }
+    tk(PROF_C_CNT_SCAN);
     // move the free-stack window so that this step's pops and pushes stay inside it
     if (L.recenter) {
         const uint32_t old = L.fs_base, top = L.top;
@@ -455,115 +554,119 @@ __device__ inline void team_counts(const TeamArgs& a, TeamLds& L, uint32_t me, u
     __syncthreads();
 }
 
-// ---- plan (all threads): totals, exact output counts, pops, flat starts ------------------------
+// ---- plan (wave 0): exact output counts, bin bases, checks, pops --------------------------------
+// Lane k holds node k; a partition's bins are handled one per lane.
 __device__ inline void team_plan(const TeamArgs& a, TeamLds& L)
 {
     const Engine& e = a.e;
-    const uint32_t tid = threadIdx.x, n = e.n_nodes, C = e.C, w = tid >> 6, lane = lane_id();
-    for (uint32_t k = 0; k < n; ++k) {
-        if (!L.active[k] || L.kind[k] != KIND_PARTITION) continue;
-        const uint32_t B1 = L.nd[k].bins + 1u, ns = L.seg_off[k + 1] - L.seg_off[k];
-        uint32_t* m = L.pre + L.pre_off[k];
-        for (uint32_t b = w; b < B1; b += kTeamWaves) {
-            uint32_t carry = 0;
-            for (uint32_t g0 = 0; g0 < ns; g0 += 64u) {
-                const uint32_t g = g0 + lane;
-                const uint32_t x = g < ns ? m[g * B1 + b] : 0u;
-                const uint32_t inc = wave_inscan(x);
-                if (g < ns) m[g * B1 + b] = carry + inc - x;
-                carry += (uint32_t)__shfl((int)inc, 63);
-            }
-            if (lane == 0) L.tot[k * kTeamBins + b] = carry;
-        }
-    }
-    __syncthreads();
-    if (tid < n) {
-        const uint32_t k = tid, kind = L.kind[k];
+    const uint32_t tid = threadIdx.x, n = e.n_nodes, C = e.C, lane = lane_id();
+    if (tid < 64u) {
+        Ticker tk(e.prof, L.prof_member && lane == 0u);
+        const bool own = lane < n;
+        const uint32_t k = own ? lane : 0u;
+        const uint32_t act = own ? L.active[k] : 0u, kind = own ? L.kind[k] : 0xFFu;
+        const uint32_t parts = own ? L.particles[k] : 0u, bound = own ? L.bound[k] : 0u;
         uint32_t R = 0, st = 0;
-        if (L.active[k]) {
-            if (kind == KIND_PARTITION) {
-                const uint32_t B = L.nd[k].bins;
-                uint32_t sum = 0;
-                for (uint32_t b = 0; b < B; ++b) {
-                    const uint32_t t = L.tot[k * kTeamBins + b];
-                    L.binbase[k * kTeamBins + b] = R;
-                    R += (t + C - 1u) / C;
-                    sum += t;
-                }
-                if (L.tot[k * kTeamBins + B] != 0u) st = GF_INVALID;
-                else if (sum != L.particles[k] || R > L.bound[k]) st = GF_PLAN_MISMATCH;
-            } else if (kind == KIND_TRANSFORM) {
-                R = (L.particles[k] + C - 1u) / C;
-                if (L.particles[k] == 0u || R > L.bound[k]) st = GF_PLAN_MISMATCH;
+        if (act) {
+            if (kind == KIND_TRANSFORM) {
+                R = (parts + C - 1u) / C;
+                if (parts == 0u || R > bound) st = GF_PLAN_MISMATCH;
             } else if (kind == KIND_SOURCE) {
                 R = 1u;
             }
         }
-        L.n_out[k] = R;
-        L.st_node[k] = st;
-    }
-    __syncthreads();
-    if (tid == 0) {
-        uint32_t off = 0, st = 0, inv = 0;
-        for (uint32_t k = 0; k < n; ++k) {
-            L.out_off[k] = off;
-            off += L.n_out[k];
-            if (L.st_node[k] == GF_INVALID) inv = 1u;
-            else if (!st && L.st_node[k]) st = L.st_node[k];
-        }
-        L.out_off[n] = off;
-        L.total_out = off;
-        if (inv) st = GF_INVALID;
-        if (!st && off > L.reserved) st = GF_PLAN_MISMATCH;
-        if (!st && off > L.top) st = GF_POOL_EXHAUSTED;
-        L.status = st;
-        if (st) set_status(&e.ctl[CTL_STATUS], st);
-    }
-    __syncthreads();
-    if (L.status) return;
-    // pops: output j of the step (node order, bin-major, k order) takes stack[top - 1 - j]
-    for (uint32_t j = tid; j < L.total_out; j += kTeamThreads) {
-        uint32_t k = 0;
-        while (k + 1u < n && L.out_off[k + 1u] <= j) ++k;
-        const uint32_t r = j - L.out_off[k], kind = L.kind[k];
-        const unsigned long long ent = L.fs[L.top - 1u - j - L.fs_base];
-        gf_chunk_desc d;
-        d.index = (uint32_t)ent;
-        d.generation = (uint32_t)(ent >> 32);
-        d.bin = 0u;
-        if (kind == KIND_SOURCE) {
-            d.fill = L.particles[k];
-        } else if (kind == KIND_TRANSFORM) {
-            const uint32_t rem = L.particles[k] - r * C;
-            d.fill = rem < C ? rem : C;
-        } else {
-            const uint32_t B = L.nd[k].bins;
-            uint32_t b = 0;
-            for (uint32_t x = 0; x < B; ++x) {
-                const uint32_t t = L.tot[k * kTeamBins + x], bb = L.binbase[k * kTeamBins + x];
-                if (t && bb <= r) b = x;
+        uint64_t pm = __ballot(act && kind == KIND_PARTITION);
+        uint32_t inv = 0, pk = ~0u, pB = 0, bbl = 0, totl = 0;  // bins of the last partition node, lane b
+        while (pm) {
+            const uint32_t kp = (uint32_t)(__ffsll((unsigned long long)pm) - 1);
+            pm &= pm - 1u;
+            const uint32_t B = L.nd[kp].bins;
+            const uint32_t t = lane <= B ? L.tot[kp * kTeamBins + lane] : 0u;
+            const uint32_t tv = lane < B ? t : 0u, r = (tv + C - 1u) / C;
+            const uint32_t ri = wave_inscan(r), ti = wave_inscan(tv);
+            if (lane < B) L.binbase[kp * kTeamBins + lane] = ri - r;
+            const uint32_t Rk = rl(ri, 63), sum = rl(ti, 63);
+            if (rl(t, B)) inv = 1u;
+            if (lane == kp) {
+                R = Rk;
+                if (sum != parts || Rk > bound) st = GF_PLAN_MISMATCH;
             }
-            const uint32_t rem = L.tot[k * kTeamBins + b] - (r - L.binbase[k * kTeamBins + b]) * C;
-            d.fill = rem < C ? rem : C;
-            d.bin = b;
+            pk = kp;
+            pB = B;
+            bbl = ri - r;
+            totl = tv;
         }
-        L.out[j] = d;
-    }
-    // flat output start of every (segment, bin)
-    for (uint32_t k = 0; k < n; ++k) {
-        if (!L.active[k] || L.kind[k] != KIND_PARTITION) continue;
-        const uint32_t B = L.nd[k].bins, B1 = B + 1u, ns = L.seg_off[k + 1] - L.seg_off[k];
-        uint32_t* m = L.pre + L.pre_off[k];
-        for (uint32_t x = tid; x < ns * B1; x += kTeamThreads) {
-            const uint32_t b = x % B1;
-            if (b < B) m[x] += L.binbase[k * kTeamBins + b] * C;
+        const uint64_t bad = __ballot(st != 0u);
+        uint32_t status = inv ? (uint32_t)GF_INVALID : bad ? rl(st, (uint32_t)(__ffsll((unsigned long long)bad) - 1)) : 0u;
+        const uint32_t oi = wave_inscan(R), off = oi - R, total = rl(oi, 63);
+        if (!status && total > L.reserved) status = GF_PLAN_MISMATCH;
+        if (!status && total > L.top) status = GF_POOL_EXHAUSTED;
+        if (own) {
+            L.n_out[k] = R;
+            L.out_off[k] = off;
         }
-    }
-    __syncthreads();
-    if (tid == 0) {
-        L.top -= L.total_out;
-        const unsigned long long live = e.S - L.top;
-        if (live > L.peak_live) L.peak_live = live;
+        if (lane == 0) {
+            L.out_off[n] = total;
+            L.total_out = total;
+            L.status = status;
+            if (status) set_status(&e.ctl[CTL_STATUS], status);
+        }
+        tk(PROF_C_PLAN_NODES);
+        if (!status) {
+            // output j of the step (node order, bin-major, k order) takes stack[top - 1 - j]
+            const uint32_t top = L.top, fbase = L.fs_base;
+            for (uint32_t j0 = 0; j0 < total; j0 += 64u) {
+                const uint32_t j = j0 + lane;
+                const uint32_t kx = node_of(j, off, R, n);
+                const uint32_t r = j - from_lane(off, kx), kk = from_lane(kind, kx), pp = from_lane(parts, kx);
+                if (j < total) {
+                    const unsigned long long ent = L.fs[top - 1u - j - fbase];
+                    gf_chunk_desc d;
+                    d.index = (uint32_t)ent;
+                    d.generation = (uint32_t)(ent >> 32);
+                    d.bin = 0u;
+                    if (kk == KIND_SOURCE) {
+                        d.fill = pp;
+                    } else if (kk == KIND_TRANSFORM) {
+                        const uint32_t rem = pp - r * C;
+                        d.fill = rem < C ? rem : C;
+                    } else {
+                        // the non-empty bin whose output range holds r
+                        uint32_t b = 0, tb = 0, bb = 0;
+                        if (kx == pk) {
+                            for (uint32_t x = 0; x < pB; ++x) {
+                                const uint32_t tx = rl(totl, x), bx = rl(bbl, x);
+                                if (tx && bx <= r) {
+                                    b = x;
+                                    tb = tx;
+                                    bb = bx;
+                                }
+                            }
+                        } else {
+                            const uint32_t B = L.nd[kx].bins;
+                            for (uint32_t x = 0; x < B; ++x) {
+                                const uint32_t tx = L.tot[kx * kTeamBins + x], bx = L.binbase[kx * kTeamBins + x];
+                                if (tx && bx <= r) {
+                                    b = x;
+                                    tb = tx;
+                                    bb = bx;
+                                }
+                            }
+                        }
+                        const uint32_t rem = tb - (r - bb) * C;
+                        d.fill = rem < C ? rem : C;
+                        d.bin = b;
+                    }
+                    L.out[j] = d;
+                }
+            }
+            if (lane == 0) {
+                L.top = top - total;
+                const unsigned long long live = e.S - (top - total);
+                if (live > L.peak_live) L.peak_live = live;
+            }
+        }
+        tk(PROF_C_PLAN_POPS);
     }
     __syncthreads();
 }
@@ -612,51 +715,85 @@ __device__ inline void team_trace(const TeamArgs& a, TeamLds& L)
     }
 }
 
+// Retire (wave 4 of every member): outputs to the destination rings in node and output order,
+// consumed inputs back to the free stack (generation + 1), partition inputs' count rows queued for
+// clearing in the next step, counters.
 __device__ inline void team_retire(const TeamArgs& a, TeamLds& L)
 {
     const Engine& e = a.e;
-    const uint32_t n = e.n_nodes, rcm = a.rc - 1u;
-    if (lane_id() == 0) {
-        // outputs to the destination rings: node order, output order
-        for (uint32_t k = 0; k < n; ++k) {
-            if (!L.active[k] || L.kind[k] == KIND_SINK) continue;
-            const bool part = L.kind[k] == KIND_PARTITION;
-            for (uint32_t r = 0; r < L.n_out[k]; ++r) {
-                const gf_chunk_desc d = L.out[L.out_off[k] + r];
-                const uint32_t dst = (uint32_t)L.route[L.route_off[k] + (part ? d.bin : 0u)];
-                const unsigned long long pos = L.tail[dst]++;
-                L.ring[dst * a.rc + (uint32_t)(pos & rcm)] = d;
-            }
+    const uint32_t lane = lane_id(), n = e.n_nodes, rcm = a.rc - 1u;
+    Ticker tk(e.prof, L.prof_member && lane == 0u);
+    const bool own = lane < n;
+    const uint32_t k = own ? lane : 0u;
+    const uint32_t kind = own ? L.kind[k] : 0xFFu, act = own ? L.active[k] : 0u;
+    const uint32_t nout = own ? L.n_out[k] : 0u, ooff = own ? L.out_off[k] : 0u, ro = own ? L.route_off[k] : 0u;
+    const uint32_t nin = own ? L.n_in[k] : 0u, inoff = own ? L.in_off[k] : 0u, parts = own ? L.particles[k] : 0u;
+    unsigned long long tail = own ? L.tail[k] : 0ull;
+    const uint32_t total_out = L.total_out, in_tot = L.in_off[n], top = L.top, fb = L.fs_base;
+    for (uint32_t x0 = 0; x0 < total_out; x0 += 64u) {
+        const uint32_t x = x0 + lane;
+        const uint32_t kx = node_of(x, ooff, nout, n);
+        const uint32_t kk = from_lane(kind, kx), rox = from_lane(ro, kx);
+        const bool v = x < total_out && kk != KIND_SINK;
+        gf_chunk_desc d{0, 0, 0, 0};
+        uint32_t dst = 0xFFFFFFFFu;
+        if (v) {
+            d = L.out[x];
+            dst = (uint32_t)L.route[rox + (kk == KIND_PARTITION ? d.bin : 0u)];
         }
-        // consumed inputs back to the free stack (generation + 1); partition inputs' count rows
-        // are cleared in the next step
-        uint32_t zn = 0;
-        for (uint32_t k = 0; k < n; ++k) {
-            if (!L.active[k]) continue;
-            const uint32_t kind = L.kind[k];
-            if (kind == KIND_SINK) {
-                L.sunk += L.particles[k];
-                L.sink_items += L.particles[k];
-            } else if (kind == KIND_SOURCE) {
-                L.src_items += L.particles[k];
-            } else {
-                L.batches += 1u;
-                if (kind == KIND_PARTITION) L.part_items += L.particles[k];
-                else L.xf_items += L.particles[k];
-            }
-            if (kind == KIND_SOURCE) continue;
-            for (uint32_t i = 0; i < L.n_in[k]; ++i) {
-                const gf_chunk_desc d = L.in[L.in_off[k] + i];
-                L.fs[L.top - L.fs_base] = (unsigned long long)d.index | ((unsigned long long)(d.generation + 1u) << 32);
-                L.top += 1u;
-                if (kind == KIND_PARTITION) L.zero[zn++] = d.index | (d.generation & 1u) << 31;
-            }
+        uint64_t pend = __ballot(v);
+        unsigned long long pos = 0;
+        while (pend) {
+            const int leader = __ffsll((unsigned long long)pend) - 1;
+            const uint32_t lb = rl(dst, (uint32_t)leader);
+            const bool mine = v && dst == lb;
+            const uint64_t m = __ballot(mine);
+            const unsigned long long tl = rl64(tail, lb);
+            if (mine) pos = tl + (uint32_t)__popcll(m & lanemask_lt());
+            if (lane == lb) tail += (uint32_t)__popcll(m);
+            pend &= ~m;
         }
-        for (uint32_t k = 0; k < n; ++k) L.active[k] = 0u;
+        if (v) L.ring[dst * a.rc + (uint32_t)(pos & rcm)] = d;
+    }
+    tk(PROF_C_RET_OUT);
+    uint32_t zn = 0;
+    for (uint32_t x0 = 0; x0 < in_tot; x0 += 64u) {
+        const uint32_t x = x0 + lane;
+        const bool v = x < in_tot;
+        const uint32_t kk = from_lane(kind, node_of(x, inoff, nin, n));
+        gf_chunk_desc d{0, 0, 0, 0};
+        if (v) {
+            d = L.in[x];
+            L.fs[top + x - fb] = (unsigned long long)d.index | ((unsigned long long)(d.generation + 1u) << 32);
+        }
+        const bool z = v && kk == KIND_PARTITION;
+        const uint64_t zm = __ballot(z);
+        if (z) L.zero[zn + (uint32_t)__popcll(zm & lanemask_lt())] = d.index | (d.generation & 1u) << 31;
+        zn += (uint32_t)__popcll(zm);
+    }
+    const bool a1 = act != 0u;
+    const uint32_t sink_p = wave_sum(a1 && kind == KIND_SINK ? parts : 0u);
+    const uint32_t src_p = wave_sum(a1 && kind == KIND_SOURCE ? parts : 0u);
+    const uint32_t part_p = wave_sum(a1 && kind == KIND_PARTITION ? parts : 0u);
+    const uint32_t xf_p = wave_sum(a1 && kind == KIND_TRANSFORM ? parts : 0u);
+    const uint32_t nb = (uint32_t)__popcll(__ballot(a1 && (kind == KIND_PARTITION || kind == KIND_TRANSFORM)));
+    if (own) {
+        L.tail[k] = tail;
+        L.active[k] = 0u;
+    }
+    if (lane == 0) {
+        L.top = top + in_tot;
         L.zero_n = zn;
+        L.sunk += sink_p;
+        L.sink_items += sink_p;
+        L.src_items += src_p;
+        L.part_items += part_p;
+        L.xf_items += xf_p;
+        L.batches += nb;
         L.step += 1u;
     }
     wave_sync();
+    tk(PROF_C_RET_IN);
 }
 
 // A failed item: the run's status word for the host, this member's flag for the team's stop decision.
@@ -680,95 +817,205 @@ __device__ inline uint32_t next_key(const TeamArgs& a, const TeamLds& L, uint32_
     return (((uint32_t)d.index * 2u + (d.generation & 1u)) * a.tpc + (off >> 8)) * a.bb + nb;
 }
 
+constexpr uint32_t kSegRounds = kSeg / 64u;
+
+__device__ inline uint32_t div_c(uint32_t x, uint32_t C, uint32_t shift) { return shift < 32u ? x >> shift : x / C; }
+
+// Records of one input segment, prefetched right after the barrier by the wave that will scatter it.
 template <class P>
-__device__ void team_partition_seg(const TeamArgs& a, const TeamLds& L, uint32_t k, uint32_t i, uint32_t gg, uint32_t gs)
+struct SegRecs {
+    P rec[kSegRounds];
+    uint32_t s;  // segment index, or ~0u
+};
+
+template <class P>
+__device__ inline void load_segment(const TeamArgs& a, const TeamLds& L, uint32_t s, SegRecs<P>& R)
 {
-    const Engine& e = a.e;
-    const uint32_t C = e.C, lane = lane_id();
-    const gf_node_desc& nd = L.nd[k];
-    const uint32_t B = nd.bins, ro = L.route_off[k];
+    const uint32_t sm = L.seg_map[s];
+    const uint32_t k = seg_node(sm), i = seg_input(sm), gg = seg_part(sm);
+    R.s = ~0u;
+    if (L.kind[k] == KIND_SOURCE) return;
     const gf_chunk_desc d = L.in[L.in_off[k] + i];
     const uint32_t off0 = gg * kSeg, cnt = d.fill - off0 < kSeg ? d.fill - off0 : kSeg;
-    const size_t slot0 = (size_t)d.index * C + off0;
-    const uint32_t* fst = L.pre + L.pre_off[k] + gs * (B + 1u);
-    const gf_chunk_desc* ob = L.out + L.out_off[k];
-    P rec[kSeg / 64];
+    const size_t slot0 = (size_t)d.index * a.e.C + off0;
+    const uint32_t lane = lane_id();
 #pragma unroll
-    for (uint32_t r = 0; r < kSeg / 64; ++r) {
+    for (uint32_t r = 0; r < kSegRounds; ++r) {
         const uint32_t it = r * 64u + lane;
-        if (it < cnt) rec[r] = P::load_team(e.cols, slot0 + it);
+        if (it < cnt) R.rec[r] = P::load_team(a.e.cols, slot0 + it);
     }
-    uint32_t run = 0;
-#pragma unroll
-    for (uint32_t r = 0; r < kSeg / 64; ++r) {
-        const uint32_t it = r * 64u + lane;
-        bool v = it < cnt;
-        uint32_t bin = ~0u;
-        if (v) {
-            bin = classify(nd, rec[r]);
-            if (bin >= B) {
-                seg_fail(a, L, GF_INVALID);
-                v = false;
-                bin = ~0u;
-            }
+    R.s = s;
+}
+
+// Per-bin constants of one partition node, bin b in lane b of the wave, loaded once per launch:
+// the fused transform (node, op, f[0]), the destination after fusion and, when that destination
+// is a partition, its classifier parameters (dop = 0 otherwise).
+struct BinTab {
+    uint32_t k;  // node described (wave-uniform), or ~0u
+    int32_t fz;
+    uint32_t dst, xop, dop, dbins, du0;
+    float xf0, df0;
+};
+
+__device__ inline void load_bintab(const TeamLds& L, uint32_t k, BinTab& t)
+{
+    const uint32_t lane = lane_id(), B = L.nd[k].bins, ro = L.route_off[k];
+    t.k = k;
+    t.fz = -1;
+    t.dst = 0u;
+    t.xop = 0u;
+    t.xf0 = 1.0f;
+    t.dop = 0u;
+    t.dbins = 1u;
+    t.du0 = 0u;
+    t.df0 = 0.0f;
+    if (lane < B) {
+        t.fz = L.fuse[ro + lane];
+        t.dst = (uint32_t)L.route[ro + lane];
+        if (t.fz >= 0) {
+            t.xop = L.nd[t.fz].op;
+            t.xf0 = L.nd[t.fz].f[0];
         }
-        const uint32_t pos = seg_rank(v, bin, run);
-        uint32_t key = ~0u;
-        if (v) {
-            if ((fst[bin] + pos) / C >= L.n_out[k]) {
-                seg_fail(a, L, GF_PLAN_MISMATCH);
-                v = false;
-            }
+        if (L.kind[t.dst] == KIND_PARTITION) {
+            const gf_node_desc& dn = L.nd[t.dst];
+            t.dop = dn.op;
+            t.dbins = dn.bins;
+            t.du0 = dn.u[0];
+            t.df0 = dn.f[0];
         }
-        if (v) {
-            const uint32_t flat = fst[bin] + pos;
-            const uint32_t q = flat / C, off = flat - q * C;
-            const gf_chunk_desc od = ob[q];
-            P p = rec[r];
-            const uint32_t u = update(nd, bin, p);
-            if (u) seg_fail(a, L, u);
-            const int32_t fz = L.fuse[ro + bin];
-            if (fz >= 0) transform(L.nd[fz], e.rp, 2u * (uint32_t)fz, p);
-            p.store(e.cols, (size_t)od.index * C + off);
-            key = next_key(a, L, (uint32_t)L.route[ro + bin], od, off, p);
+    }
+}
+
+// One lane's bin constants and flat output position.
+struct ItemBin {
+    uint32_t flat, dst, xop, dop, dbins, du0;
+    int32_t fz;
+    float xf0, df0;
+};
+
+// One pass over the distinct bins of a round (the bin id is wave-uniform inside the pass): the
+// rank of every lane inside the segment after `run` (lane b holds bin b's running count), its
+// flat output position from fb (lane b holds bin b's flat start for this segment), and the bin's
+// table entries, all through readlane.
+__device__ inline void seg_group(bool v, uint32_t bin, uint32_t& run, uint32_t fb, const BinTab& t, ItemBin& o)
+{
+    uint64_t act = __ballot(v);
+    const uint32_t lane = lane_id();
+    while (act) {
+        const uint32_t lb = rl(bin, (uint32_t)(__ffsll((unsigned long long)act) - 1));
+        const bool mine = v && bin == lb;
+        const uint64_t m = __ballot(mine);
+        const uint32_t base = rl(run, lb) + rl(fb, lb);
+        const uint32_t dst = rl(t.dst, lb), xop = rl(t.xop, lb), dop = rl(t.dop, lb), dbins = rl(t.dbins, lb),
+                       du0 = rl(t.du0, lb), fz = rl((uint32_t)t.fz, lb), xf0 = rl(__float_as_uint(t.xf0), lb),
+                       df0 = rl(__float_as_uint(t.df0), lb);
+        if (mine) {
+            o.flat = base + (uint32_t)__popcll(m & lanemask_lt());
+            o.dst = dst;
+            o.xop = xop;
+            o.dop = dop;
+            o.dbins = dbins;
+            o.du0 = du0;
+            o.fz = (int32_t)fz;
+            o.xf0 = __uint_as_float(xf0);
+            o.df0 = __uint_as_float(df0);
         }
-        agg_add(key != ~0u, key, a.cnt);
+        if (lane == lb) run += (uint32_t)__popcll(m);
+        act &= ~m;
     }
 }
 
 template <class P>
-__device__ void team_transform_seg(const TeamArgs& a, const TeamLds& L, uint32_t k, uint32_t i, uint32_t gg)
+__device__ void team_partition_seg(const TeamArgs& a, const TeamLds& L, uint32_t k, uint32_t i, uint32_t gg, uint32_t gs,
+                                   const SegRecs<P>& R, BinTab& t)
+{
+    const Engine& e = a.e;
+    const uint32_t C = e.C, lane = lane_id();
+    if (t.k != k) load_bintab(L, k, t);
+    const gf_node_desc& nd = L.nd[k];
+    const uint32_t B = nd.bins;
+    const gf_chunk_desc d = L.in[L.in_off[k] + i];
+    const uint32_t off0 = gg * kSeg, cnt = d.fill - off0 < kSeg ? d.fill - off0 : kSeg;
+    const uint32_t* pre = L.pre + L.pre_off[k] + gs * (B + 1u);
+    const uint32_t fb = lane < B ? L.binbase[k * kTeamBins + lane] * C + pre[lane] : 0u;
+    const gf_chunk_desc* ob = L.out + L.out_off[k];
+    const uint32_t nout = L.n_out[k];
+    Ticker tk(e.prof, L.prof_member && threadIdx.x == 0u);
+    bool v[kSegRounds];
+    uint32_t bin[kSegRounds], key[kSegRounds];
+    ItemBin ib[kSegRounds];
+#pragma unroll
+    for (uint32_t r = 0; r < kSegRounds; ++r) {
+        v[r] = r * 64u + lane < cnt;
+        bin[r] = ~0u;
+        if (v[r]) {
+            bin[r] = classify(nd, R.rec[r]);
+            if (bin[r] >= B) {
+                seg_fail(a, L, GF_INVALID);
+                v[r] = false;
+                bin[r] = ~0u;
+            }
+        }
+    }
+    uint32_t run = 0;
+#pragma unroll
+    for (uint32_t r = 0; r < kSegRounds; ++r) seg_group(v[r], bin[r], run, fb, t, ib[r]);
+    tk(PROF_C_SEG_RANK);
+#pragma unroll
+    for (uint32_t r = 0; r < kSegRounds; ++r) {
+        key[r] = ~0u;
+        if (!v[r]) continue;
+        const uint32_t q = div_c(ib[r].flat, C, a.c_shift), off = ib[r].flat - q * C;
+        if (q >= nout) {
+            seg_fail(a, L, GF_PLAN_MISMATCH);
+            continue;
+        }
+        const gf_chunk_desc od = ob[q];
+        P p = R.rec[r];
+        const uint32_t u = update(nd, bin[r], p);
+        if (u) seg_fail(a, L, u);
+        if (!transform_p(ib[r].xop, ib[r].xf0, p) && ib[r].fz >= 0) transform(L.nd[ib[r].fz], e.rp, 2u * (uint32_t)ib[r].fz, p);
+        p.store(e.cols, (size_t)od.index * C + off);
+        if (ib[r].dop) {
+            uint32_t nb = classify_p(ib[r].dop, ib[r].dbins, ib[r].du0, ib[r].df0, p);
+            if (nb == kNeedDesc) nb = classify(L.nd[ib[r].dst], p);
+            if (nb >= ib[r].dbins) nb = ib[r].dbins;
+            key[r] = (((uint32_t)od.index * 2u + (od.generation & 1u)) * a.tpc + (off >> 8)) * a.bb + nb;
+        }
+    }
+    tk(PROF_C_SEG_STORE);
+#pragma unroll
+    for (uint32_t r = 0; r < kSegRounds; ++r) agg_add(key[r] != ~0u, key[r], a.cnt);
+    tk(PROF_C_SEG_AGG);
+}
+
+template <class P>
+__device__ void team_transform_seg(const TeamArgs& a, const TeamLds& L, uint32_t k, uint32_t i, uint32_t gg,
+                                   const SegRecs<P>& R)
 {
     const Engine& e = a.e;
     const uint32_t C = e.C, lane = lane_id();
     const gf_node_desc& nd = L.nd[k];
     const gf_chunk_desc d = L.in[L.in_off[k] + i];
     const uint32_t off0 = gg * kSeg, cnt = d.fill - off0 < kSeg ? d.fill - off0 : kSeg;
-    const size_t slot0 = (size_t)d.index * C + off0;
     const uint32_t pos0 = L.in_items[L.in_off[k] + i] + off0;
     const gf_chunk_desc* ob = L.out + L.out_off[k];
-    const uint32_t dst = (uint32_t)L.route[L.route_off[k]];
-    P rec[kSeg / 64];
+    const uint32_t nout = L.n_out[k], dst = (uint32_t)L.route[L.route_off[k]];
 #pragma unroll
-    for (uint32_t r = 0; r < kSeg / 64; ++r) {
-        const uint32_t it = r * 64u + lane;
-        if (it < cnt) rec[r] = P::load_team(e.cols, slot0 + it);
-    }
-#pragma unroll
-    for (uint32_t r = 0; r < kSeg / 64; ++r) {
+    for (uint32_t r = 0; r < kSegRounds; ++r) {
         const uint32_t it = r * 64u + lane;
         uint32_t key = ~0u;
         if (it < cnt) {
-            P p = rec[r];
-            transform(nd, e.rp, L.purpose[k], p);
-            const uint32_t g = pos0 + it, q = g / C, off = g - q * C;
-            if (q >= L.n_out[k]) {
+            const uint32_t g = pos0 + it, q = div_c(g, C, a.c_shift), off = g - q * C;
+            if (q < nout) {
+                P p = R.rec[r];
+                transform(nd, e.rp, L.purpose[k], p);
+                const gf_chunk_desc od = ob[q];
+                p.store(e.cols, (size_t)od.index * C + off);
+                key = next_key(a, L, dst, od, off, p);
+            } else {
                 seg_fail(a, L, GF_PLAN_MISMATCH);
-                continue;
             }
-            const gf_chunk_desc od = ob[q];
-            p.store(e.cols, (size_t)od.index * C + off);
-            key = next_key(a, L, dst, od, off, p);
         }
         agg_add(key != ~0u, key, a.cnt);
     }
@@ -785,7 +1032,7 @@ __device__ void team_source_seg(const TeamArgs& a, const TeamLds& L, uint32_t k,
     const uint32_t dst = (uint32_t)L.route[L.route_off[k]];
     const unsigned long long id0 = L.first_id[k] + off0;
 #pragma unroll
-    for (uint32_t r = 0; r < kSeg / 64; ++r) {
+    for (uint32_t r = 0; r < kSegRounds; ++r) {
         const uint32_t it = r * 64u + lane;
         uint32_t key = ~0u;
         if (it < cnt) {
@@ -799,25 +1046,17 @@ __device__ void team_source_seg(const TeamArgs& a, const TeamLds& L, uint32_t k,
 }
 
 template <class P>
-__device__ void team_sink_seg(const TeamArgs& a, const TeamLds& L, uint32_t k, uint32_t i, uint32_t gg)
+__device__ void team_sink_seg(const TeamArgs& a, const TeamLds& L, uint32_t k, uint32_t i, uint32_t gg, const SegRecs<P>& R)
 {
-    const Engine& e = a.e;
-    const uint32_t C = e.C, lane = lane_id();
+    const uint32_t lane = lane_id();
     const gf_node_desc& nd = L.nd[k];
     const gf_chunk_desc d = L.in[L.in_off[k] + i];
     const uint32_t off0 = gg * kSeg, cnt = d.fill - off0 < kSeg ? d.fill - off0 : kSeg;
-    const size_t slot0 = (size_t)d.index * C + off0;
-    P rec[kSeg / 64];
-#pragma unroll
-    for (uint32_t r = 0; r < kSeg / 64; ++r) {
-        const uint32_t it = r * 64u + lane;
-        if (it < cnt) rec[r] = P::load_team(e.cols, slot0 + it);
-    }
     if constexpr (std::is_same_v<P, Particle>) {
 #pragma unroll
-        for (uint32_t r = 0; r < kSeg / 64; ++r) {
+        for (uint32_t r = 0; r < kSegRounds; ++r) {
             if (r * 64u + lane >= cnt) continue;
-            const Particle& p = rec[r];
+            const Particle& p = R.rec[r];
             if (nd.op == GF_SNK_HIST) {
                 const uint32_t v = nd.u[0] == 0u ? __float_as_uint(p.w) : p.gen;
                 const uint32_t rel = v - nd.u[1];
@@ -832,10 +1071,10 @@ __device__ void team_sink_seg(const TeamArgs& a, const TeamLds& L, uint32_t k, u
         if (nd.op != GF_SNK_CHUNK_DIGEST) return;
         uint64_t acc = 0, msum = 0;
 #pragma unroll
-        for (uint32_t r = 0; r < kSeg / 64; ++r) {
+        for (uint32_t r = 0; r < kSegRounds; ++r) {
             const uint32_t it = r * 64u + lane;
             if (it >= cnt) continue;
-            const uint64_t h = keyed_hash(rec[r]);
+            const uint64_t h = keyed_hash(R.rec[r]);
             acc += mix64(h ^ ((uint64_t)(off0 + it + 1u) * 0xD6E8FEB86659FD93ull));
             msum += h;
         }
@@ -862,19 +1101,41 @@ __device__ void team_sink_seg(const TeamArgs& a, const TeamLds& L, uint32_t k, u
     }
 }
 
-template <class P>
-__device__ inline void team_segments(const TeamArgs& a, const TeamLds& L, uint32_t me, uint32_t T, uint32_t w)
+// Segments [lo, hi) of this member, wave w takes lo + w, lo + w + 4, ...
+__device__ inline void member_range(const TeamLds& L, uint32_t n, uint32_t me, uint32_t T, uint32_t* lo, uint32_t* hi)
 {
-    const uint32_t G = L.seg_off[a.e.n_nodes];
-    const uint32_t lo = (uint32_t)((uint64_t)me * G / T), hi = (uint32_t)((uint64_t)(me + 1u) * G / T);
+    const uint32_t G = L.seg_off[n];
+    *lo = (uint32_t)((uint64_t)me * G / T);
+    *hi = (uint32_t)((uint64_t)(me + 1u) * G / T);
+}
+
+template <class P>
+__device__ inline void team_prefetch(const TeamArgs& a, const TeamLds& L, uint32_t me, uint32_t T, uint32_t w, SegRecs<P>& R)
+{
+    uint32_t lo, hi;
+    member_range(L, a.e.n_nodes, me, T, &lo, &hi);
+    R.s = ~0u;
+    if (lo + w < hi) load_segment<P>(a, L, lo + w, R);
+}
+
+template <class P>
+__device__ inline void team_segments(const TeamArgs& a, const TeamLds& L, uint32_t me, uint32_t T, uint32_t w, SegRecs<P>& R,
+                                     BinTab& t)
+{
+    uint32_t lo, hi;
+    member_range(L, a.e.n_nodes, me, T, &lo, &hi);
     for (uint32_t s = lo + w; s < hi; s += 4u) {
         const uint32_t sm = L.seg_map[s];
-        const uint32_t k = sm >> 28, i = (sm >> 18) & 1023u, gg = sm & 0x3FFFFu;
+        const uint32_t k = seg_node(sm), i = seg_input(sm), gg = seg_part(sm);
         const uint32_t kind = L.kind[k];
-        if (kind == KIND_PARTITION) team_partition_seg<P>(a, L, k, i, gg, s - L.seg_off[k]);
-        else if (kind == KIND_TRANSFORM) team_transform_seg<P>(a, L, k, i, gg);
-        else if (kind == KIND_SOURCE) team_source_seg<P>(a, L, k, gg);
-        else team_sink_seg<P>(a, L, k, i, gg);
+        if (kind == KIND_SOURCE) {
+            team_source_seg<P>(a, L, k, gg);
+            continue;
+        }
+        if (R.s != s) load_segment<P>(a, L, s, R);
+        if (kind == KIND_PARTITION) team_partition_seg<P>(a, L, k, i, gg, s - L.seg_off[k], R, t);
+        else if (kind == KIND_TRANSFORM) team_transform_seg<P>(a, L, k, i, gg, R);
+        else team_sink_seg<P>(a, L, k, i, gg, R);
     }
 }
 
@@ -891,6 +1152,7 @@ __global__ __launch_bounds__(kTeamThreads) void k_team(TeamArgs a)
     const Engine& e = a.e;
     const uint32_t tid = threadIdx.x, w = tid >> 6;
     const bool prof = e.prof != nullptr && me == 0u && tid == 0u;
+    const bool wprof = e.prof != nullptr && me == 0u && lane_id() == 0u;
     unsigned long long tc = prof ? wall_clock64() : 0ull;
     auto mark = [&](uint32_t ph) {
         if (prof) {
@@ -899,6 +1161,7 @@ __global__ __launch_bounds__(kTeamThreads) void k_team(TeamArgs a)
             tc = now;
         }
     };
+    if (tid == 0u) L.prof_member = (e.prof != nullptr && me == 0u) ? 1u : 0u;
     team_load(a, L);
     if (L.exit_code == EXIT_SKIP) {
         if (me == 0u && tid == 0u) e.ctl[CTL_HANDOFF] = 1u;
@@ -907,6 +1170,10 @@ __global__ __launch_bounds__(kTeamThreads) void k_team(TeamArgs a)
     if (w == 4u) team_select(a, L, a.max_steps == 0u);
     __syncthreads();
     uint32_t target = 0;
+    unsigned long long cyc = 0;
+    SegRecs<P> R;
+    BinTab tab;
+    tab.k = ~0u;
     for (uint32_t it = 0; L.exit_code == EXIT_NONE; ++it) {
         if (it) {
             if (tid == 0u && !team_wait(a, target)) {
@@ -917,18 +1184,21 @@ __global__ __launch_bounds__(kTeamThreads) void k_team(TeamArgs a)
             mark(PROF_T_WAIT);
             if (L.status) break;
         }
+        if (w < 4u) team_prefetch<P>(a, L, me, T, w, R);
         team_counts(a, L, me, T, it);
         if (L.status) break;
         mark(PROF_T_COUNTS);
         team_plan(a, L);
         if (L.status) break;
         mark(PROF_T_PLAN);
+        const unsigned long long w0 = wprof ? wall_clock64() : 0ull;
         if (w < 4u) {
-            team_segments<P>(a, L, me, T, w);
+            team_segments<P>(a, L, me, T, w, R, tab);
         } else {
             if (me == 0u && e.trace_on) team_trace(a, L);
             team_retire(a, L);
         }
+        if (wprof && (w == 0u || w == 4u)) e.prof[w == 0u ? PROF_T_SEGW : PROF_T_RETIRE] += wall_clock64() - w0;
         __builtin_amdgcn_s_waitcnt(0);
         __syncthreads();
         mark(PROF_T_SEG);
@@ -940,10 +1210,19 @@ __global__ __launch_bounds__(kTeamThreads) void k_team(TeamArgs a)
             }
             __hip_atomic_fetch_add(&a.tw[TW_BAR], 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
         }
-        if (w == 4u) team_select(a, L, it + 1u >= a.max_steps);
+        if (w == 4u) {
+            const unsigned long long s0 = wprof ? wall_clock64() : 0ull;
+            team_select(a, L, it + 1u >= a.max_steps);
+            if (wprof) e.prof[PROF_T_SELECTW] += wall_clock64() - s0;
+        }
         __syncthreads();
         mark(PROF_T_SELECT);
         if (prof) e.prof[PROF_T_STEPS] += 1u;
+        if (prof) {
+            const unsigned long long c = __builtin_readcyclecounter();
+            if (it) e.prof[PROF_C_STEP] += c - cyc;
+            cyc = c;
+        }
     }
     if (me == 0u && !L.status) team_flush(a, L);
 }
