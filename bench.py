@@ -71,6 +71,7 @@ def main():
     ap.add_argument("--verify", action="store_true", help="compare the sorted histogram with the oracle")
     ap.add_argument("--engine", type=int, default=0, help="device engine: 0 autotuned, 1 wide, 2 micro, 3 team")
     ap.add_argument("--team", type=int, default=0, help="team engine: workgroups in the team (0 = default)")
+    ap.add_argument("--members", action="store_true", help="team engine: print per-member phase times")
     ap.add_argument("--no-fuse", action="store_true", help="run A/B as separate transform batches")
     args = ap.parse_args()
 
@@ -142,6 +143,10 @@ def main():
             others = {"steps_per_launch": int(tst["steps"]) / max(n[3], 1), "avg_step_us": ms[3] * 1e3 / max(int(tst["steps"]), 1),
                       "phase_us_per_step": {k: round(v, 3) for k, v in phases.items()}}
             log(f"[bench] team phases (us/step over {psteps} steps): {others['phase_us_per_step']}")
+            if args.members:
+                for m, r in enumerate(flow.member_profile(args.team or 32)):
+                    log(f"[bench] member {m:2d}: wait {r[0]:.2f} countplan {r[1]:.2f} scatter {r[2]:.2f} "
+                        f"sched {r[3]:.2f} us, segments {r[4]:.2f} sinks {r[5]:.2f} per step")
         elif n[3]:  # micro engine: one persistent kernel does every phase
             k_ms, k_n, kname = ms[3], n[3], "k_micro (whole steps in one workgroup)"
             others = {}

@@ -17,7 +17,8 @@ from dataclasses import dataclass, field
 from typing import List, Optional, Sequence
 
 HERE = os.path.dirname(os.path.abspath(__file__))
-LIB_PATH = os.path.join(HERE, "libgraphflow_amd.so")
+# GRAPHFLOW_AMD_LIB selects another build of the same library (for side-by-side timing)
+LIB_PATH = os.environ.get("GRAPHFLOW_AMD_LIB") or os.path.join(HERE, "libgraphflow_amd.so")
 
 # ---- status (graphflow.h gf_status) -----------------------------------------------------------
 OK, INVALID, OVERFLOW, STALL, POOL_EXHAUSTED, BAD_ARGUMENT, PLAN_MISMATCH, DEVICE_ERROR, STAGE_FAILED = range(9)
@@ -563,11 +564,11 @@ class Flow:
         _raise(lib().gf_flow_phase_profile(self.handle, a, 32))
         if int(a[14]):
             steps = int(a[14])
-            names = {9: "barrier_wait", 10: "counts", 11: "plan", 12: "segments", 13: "arrive_select",
+            names = {9: "barrier_wait", 10: "prefetch", 11: "counts_plan", 12: "segments", 13: "arrive",
                      15: "wave0_segments", 16: "wave4_retire", 17: "wave4_select"}
             out = {n: a[i] / 100.0 / steps for i, n in names.items()}
             cyc = {18: "sel_fills", 19: "sel_credit", 20: "sel_check", 21: "sel_lists", 22: "ret_out", 23: "ret_in",
-                   24: "plan_nodes", 25: "plan_pops", 26: "cnt_clear", 27: "cnt_scan", 28: "seg_rank",
+                   24: "plan_nodes", 25: "plan_pops", 26: "sel_inputs", 27: "cnt_scan", 28: "seg_rank",
                    29: "seg_store", 30: "seg_agg", 31: "step"}
             for i, nm in cyc.items():
                 out["cycles_" + nm] = a[i] / steps
@@ -575,6 +576,20 @@ class Flow:
         steps = max(int(a[8]), 1)
         names = ["count", "plan", "source", "partition", "transform", "sink", "retire", "select"]
         return {n: a[i] / 100.0 / steps for i, n in enumerate(names)}, int(a[8])
+
+    def member_profile(self, members=32):
+        """Team engine, timed run: per member [barrier wait, counts+plan, scatter phase, scheduler
+        wave] in microseconds per step, and segments scattered / sink segments per step."""
+        words = 32 + 64 * 8
+        a = (C.c_uint64 * words)()
+        _raise(lib().gf_flow_phase_profile(self.handle, a, words))
+        steps = max(int(a[14]), 1)
+        rows = []
+        for m in range(members):
+            b = 32 + 8 * m
+            rows.append([a[b] / 100.0 / steps, a[b + 1] / 100.0 / steps, a[b + 2] / 100.0 / steps,
+                         a[b + 3] / 100.0 / steps, a[b + 4] / steps, a[b + 5] / steps])
+        return rows
 
     def item_passes(self):
         a, b = C.c_uint64(), C.c_uint64()

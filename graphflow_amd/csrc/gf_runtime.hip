@@ -1371,7 +1371,9 @@ void build_engine(gf_flow& f, uint64_t N, const RunShape& sh, uint32_t src)
         ta.cnt = m.d<uint32_t>(cnt_words);
         ta.zero_g = m.d<uint32_t>(1u + kTeamZero);
         ta.tw = m.d<uint32_t>(TW_WORDS);
-        ta.K = std::min<uint32_t>(t.max_workgroups ? t.max_workgroups : 16u, 32u);
+        // default team: every CU of one XCD (the device's CUs over its 8 XCDs)
+        const uint32_t xcd_cus = std::max<uint32_t>(1u, f.ctx->cus / 8u);
+        ta.K = std::min<uint32_t>(t.max_workgroups ? t.max_workgroups : xcd_cus, 32u);
         ta.tpc = tpc;
         ta.bb = bb;
         ta.max_steps = t.steps_per_launch ? t.steps_per_launch : 65536u;
@@ -1379,6 +1381,7 @@ void build_engine(gf_flow& f, uint64_t N, const RunShape& sh, uint32_t src)
         while (rc * 2u <= kTeamRing / n) rc *= 2u;
         ta.rc = std::min<uint32_t>(rc, 1024u);
         ta.n_routes = (uint32_t)routes.size();
+        ta.mprof = t.use_graph == 2u ? m.d<unsigned long long>(64u * kMemberProf) : nullptr;
         ta.c_shift = 32u;
         if ((sh.C & (sh.C - 1u)) == 0u)
             for (uint32_t b = 0; b < 32u; ++b)
@@ -1474,6 +1477,7 @@ void run_team(gf_flow& f, EngineBuffers& eb, uint64_t N, gf_stats& st, bool time
     TeamArgs& ta = eb.ta;
     ta.e = eb.e;
     HIPX(hipMemsetAsync(ta.cnt, 0, eb.cnt_bytes, s));
+    if (ta.mprof) HIPX(hipMemsetAsync(ta.mprof, 0, 64u * kMemberProf * sizeof(unsigned long long), s));
     HIPX(hipMemsetAsync(ta.zero_g, 0, sizeof(uint32_t), s));
     uint32_t ctl[CTL_WORDS];
     uint64_t launches = 0, steps_launched = 0;
@@ -2022,6 +2026,10 @@ gf_status gf_flow_phase_profile(gf_flow* flow, uint64_t* ticks, uint32_t words)
     return guard(&flow->err, [&] {
         HIPX(hipMemcpy(ticks, flow->eng->e.prof, sizeof(uint64_t) * std::min<uint32_t>(words, PROF_WORDS),
                        hipMemcpyDeviceToHost));
+        // words beyond the phase totals: per-member team totals, kMemberProf words per member
+        if (words > PROF_WORDS && flow->eng->team && flow->eng->ta.mprof)
+            HIPX(hipMemcpy(ticks + PROF_WORDS, flow->eng->ta.mprof,
+                           sizeof(uint64_t) * std::min<uint32_t>(words - PROF_WORDS, 64u * kMemberProf), hipMemcpyDeviceToHost));
     });
 }
 

@@ -6,18 +6,23 @@
 // scheduling decision is ever broadcast. The only data the members exchange per step are the
 // records themselves and per-segment bin counts:
 //
-//   * a segment is 256 consecutive slots of one chunk;
+//   * a segment is kSeg consecutive slots of one chunk;
 //   * whoever writes a record into a chunk that is routed to a partition node also classifies the
 //     record with that node's classifier and adds one to cnt[chunk][gen&1][segment][bin] (a
 //     wave-aggregated atomic; the last slot of the row counts invalid bins);
 //   * when the partition node later takes the chunk, every member reads those counts right
 //     after the barrier and derives the exact plan (per-bin totals, output chunk counts, the flat
-//     start of every (segment, bin)) before it has loaded a single record.
+//     start of every (segment, bin)) while the records it will scatter are still in flight.
 //
 // The stable canonical order (bin-major, then batch input order) follows from the per-segment
 // prefixes plus a wave-level rank inside each segment, so a member scatters its segments without
 // seeing any other member's records. The scheduling rules are those of engine_select/engine_plan/
 // engine_retire (sched/GraphRunner.hpp:902-1133 applied once per node per step).
+//
+// Per step:   barrier -> counts (+ records prefetched) -> plan -> scatter (waves 0-2) while wave 3
+// retires the step and selects the next one into the other half of the double-buffered batch
+// state -> wait for the stores -> arrive. One wave per SIMD, so the scheduler never shares issue
+// slots with a scattering wave.
 //
 // Coherence: stores are plain vector stores (they write through the CU's L1 into the XCD's L2);
 // every load of data written by another member in the same launch is a relaxed agent-scope load,
@@ -34,16 +39,22 @@ namespace gfd {
 constexpr uint32_t kTeamNodes = 32;
 constexpr uint32_t kTeamBins = 32;  // count row stride: bins of the widest partition + invalid slot
 constexpr uint32_t kTeamRoutes = 512;
-constexpr uint32_t kSeg = 256;
+constexpr uint32_t kSegShift = 7;
+constexpr uint32_t kSeg = 1u << kSegShift;  // slots per segment
+constexpr uint32_t kSegRounds = kSeg / 64u;
 constexpr uint32_t kTeamRing = 2048;   // ready descriptors over all nodes
 constexpr uint32_t kTeamStack = 2048;  // free-stack window
 constexpr uint32_t kTeamMargin = 512;  // pops or pushes of one step
-constexpr uint32_t kTeamIn = 1024;     // input chunks of one step
+constexpr uint32_t kTeamIn = 512;      // input chunks of one step
 constexpr uint32_t kTeamOut = 512;     // output chunks of one step
-constexpr uint32_t kTeamSegs = 2048;   // segments of one step
+constexpr uint32_t kTeamSegs = 1024;   // segments of one step
 constexpr uint32_t kTeamPrefix = 4096; // [segment][bin] words of the step's partitions
-constexpr uint32_t kTeamZero = 1024;   // consumed count rows cleared in the next step
-constexpr uint32_t kTeamWaves = 5;     // waves 0-3 scatter segments, wave 4 schedules
+constexpr uint32_t kTeamZero = 512;    // consumed count rows cleared in the next step
+constexpr uint32_t kTeamWaves = 5;     // waves 0-2 scatter segments, wave 3 schedules, wave 4 sinks
+constexpr uint32_t kWorkers = 3;
+constexpr uint32_t kSchedWave = 3;
+constexpr uint32_t kSinkWave = 4;
+constexpr uint32_t kClrThreads = 64u * 4u;  // waves that wait for their stores before the barrier
 constexpr uint32_t kTeamThreads = 64u * kTeamWaves;
 constexpr uint32_t kTeamSpin = 1u << 24;
 
@@ -59,20 +70,21 @@ enum {
     PROF_C_STEP = 31
 };
 
-// Sub-phase cycle counter: accumulates the shader-clock delta since the previous mark.
+// Sub-phase cycle counter: adds the shader-clock delta since the previous mark to an LDS
+// accumulator (a fire-and-forget LDS atomic, so the measurement does not stall the wave); member 0
+// writes the accumulators to the profile buffer when the launch ends.
 struct Ticker {
-    unsigned long long* p;
+    unsigned long long* p;  // LDS accumulators, or nullptr
     unsigned long long t;
-    bool on;
-    __device__ Ticker(unsigned long long* prof, bool enabled) : p(prof), t(0), on(enabled && prof != nullptr)
+    __device__ Ticker(unsigned long long* acc, bool enabled) : p(enabled ? acc : nullptr), t(0)
     {
-        if (on) t = __builtin_readcyclecounter();
+        if (p) t = __builtin_readcyclecounter();
     }
     __device__ void operator()(uint32_t slot)
     {
-        if (on) {
+        if (p) {
             const unsigned long long n = __builtin_readcyclecounter();
-            p[slot] += n - t;
+            atomicAdd(&p[slot], n - t);
             t = n;
         }
     }
@@ -85,6 +97,27 @@ struct TeamArgs {
     uint32_t* tw;      // team words, cleared before every launch
     uint32_t K, tpc, bb, max_steps, rc, n_routes;
     uint32_t c_shift;  // log2(C) when C is a power of two, else 32
+    unsigned long long* mprof;  // optional per-member wall-clock totals, kMemberProf words per member
+};
+
+// Per-member profile words: barrier wait, counts+plan, scatter phase, scheduler wave, segments
+// scattered by the worker waves, sink segments.
+enum { MP_WAIT = 0, MP_COUNTPLAN = 1, MP_SEG = 2, MP_SCHED = 3, MP_NSEG = 4, MP_NSINK = 5, kMemberProf = 8 };
+
+// One step's selected batches; two of them alternate so that the scheduler wave can select step
+// s+1 while the other waves still scatter step s.
+struct StepBuf {
+    uint32_t active[kTeamNodes], n_in[kTeamNodes], particles[kTeamNodes], pre_off[kTeamNodes];
+    uint32_t in_off[kTeamNodes + 1], seg_off[kTeamNodes + 1], nseg[kTeamNodes];
+    uint32_t g_main;    // segments [0, g_main) belong to non-sink nodes, [g_main, seg_off[n]) to sinks
+    unsigned long long first_id[kTeamNodes], digest_base[kTeamNodes];
+    uint32_t reserved;  // credits reserved for the step's outputs
+    uint32_t recenter;  // move the free-stack window before the step's pops
+    uint32_t sum_sink;  // items the step's sinks take
+    uint32_t part_mask; // active partition nodes of the step (bit k = node k)
+    gf_chunk_desc in[kTeamIn];
+    uint32_t in_items[kTeamIn];  // items of the node's batch before input i
+    uint32_t in_seg[kTeamIn];    // first segment (step numbering) of input i
 };
 
 struct TeamLds {
@@ -94,28 +127,23 @@ struct TeamLds {
     unsigned long long head[kTeamNodes], tail[kTeamNodes];
     gf_chunk_desc ring[kTeamRing];
     unsigned long long fs[kTeamStack];
-    // selected batches
-    uint32_t active[kTeamNodes], n_in[kTeamNodes], particles[kTeamNodes], n_out[kTeamNodes], pre_off[kTeamNodes];
-    uint32_t in_off[kTeamNodes + 1], seg_off[kTeamNodes + 1], out_off[kTeamNodes + 1];
-    uint32_t st_node[kTeamNodes];
-    unsigned long long first_id[kTeamNodes], digest_base[kTeamNodes];
-    gf_chunk_desc in[kTeamIn];
-    uint32_t in_items[kTeamIn];
-    uint32_t seg_map[kTeamSegs];  // seg_pack(node, input, segment of the chunk)
-    uint32_t seg_row[kTeamSegs];  // partition segments: first word of the segment's count row
+    StepBuf sb[2];
+    // plan of the current step
+    uint32_t n_out[kTeamNodes], out_off[kTeamNodes + 1];
     gf_chunk_desc out[kTeamOut];
+    uint32_t out_ridx[kTeamOut];  // route-table entry of output j
     uint32_t pre[kTeamPrefix];
     uint32_t tot[kTeamNodes * kTeamBins];
     uint32_t binbase[kTeamNodes * kTeamBins];
-    uint32_t zero[kTeamZero];
+    uint32_t zero[kTeamZero];  // count rows to clear in the launch's first step
     unsigned long long generated, sunk, step, batches, part_items, xf_items, src_items, sink_items, peak_live, trace_len;
     unsigned long long digest[kTeamNodes];
-    uint32_t fs_base, top, zero_n, status, exit_code, recenter, total_out;
-    uint32_t seg_err;  // a segment of this member reported a device status this step
-    uint32_t reserved;  // credits the select phase reserved for this step's outputs
-    uint32_t prof_member;  // this member records sub-phase cycle counts
-    uint32_t shv[8];
+    uint32_t fs_base, top, zero_n, status, exit_code, total_out, err_prev;
+    uint32_t seg_err;      // a segment of this member reported a device status this step
+    uint32_t prof_member;  // this member records sub-phase timings
+    unsigned long long prof_acc[PROF_WORDS];
 };
+static_assert(sizeof(TeamLds) + 64u <= 160u * 1024u, "team scheduler state must fit one CU's LDS");
 
 __device__ inline uint32_t xcc_id() { return __builtin_amdgcn_s_getreg(20 | (3 << 11)) & 0xFu; }  // HW_REG_XCC_ID[3:0]
 
@@ -126,30 +154,37 @@ __device__ inline void wave_sync()
     __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
 }
 
-// Segment map entry: node (5 bits) | input chunk of the node's batch (10 bits) | segment (17 bits).
-__device__ inline uint32_t seg_pack(uint32_t k, uint32_t i, uint32_t gg) { return k << 27 | i << 17 | gg; }
-__device__ inline uint32_t seg_node(uint32_t sm) { return sm >> 27; }
-__device__ inline uint32_t seg_input(uint32_t sm) { return (sm >> 17) & 1023u; }
-__device__ inline uint32_t seg_part(uint32_t sm) { return sm & 0x1FFFFu; }
-
-// Rank of each valid lane among the lanes of its bin, after the `run` earlier items of that bin in
-// the same segment (lane b of `run` holds bin b's running count; bins < 64).
-__device__ inline uint32_t seg_rank(bool v, uint32_t bin, uint32_t& run)
+// ---- wave-level helpers ------------------------------------------------------------------------
+__device__ inline uint32_t rl(uint32_t v, uint32_t k) { return (uint32_t)__builtin_amdgcn_readlane((int)v, (int)k); }
+__device__ inline unsigned long long rl64(unsigned long long v, uint32_t k)
 {
-    uint64_t act = __ballot(v);
-    uint32_t pos = 0;
-    const uint32_t lane = lane_id();
-    while (act) {
-        const int leader = __ffsll((unsigned long long)act) - 1;
-        const uint32_t lb = (uint32_t)__builtin_amdgcn_readlane((int)bin, leader);
-        const bool mine = v && bin == lb;
-        const uint64_t m = __ballot(mine);
-        const uint32_t base = (uint32_t)__builtin_amdgcn_readlane((int)run, (int)lb);
-        if (mine) pos = base + (uint32_t)__popcll(m & lanemask_lt());
-        if (lane == lb) run += (uint32_t)__popcll(m);
-        act &= ~m;
-    }
-    return pos;
+    return ((unsigned long long)rl((uint32_t)(v >> 32), k) << 32) | rl((uint32_t)v, k);
+}
+// Value of `v` in lane `src` (all lanes must be active).
+__device__ inline uint32_t from_lane(uint32_t v, uint32_t src) { return (uint32_t)__builtin_amdgcn_ds_bpermute((int)(src << 2), (int)v); }
+__device__ inline unsigned long long from_lane64(unsigned long long v, uint32_t src)
+{
+    return ((unsigned long long)from_lane((uint32_t)(v >> 32), src) << 32) | from_lane((uint32_t)v, src);
+}
+__device__ inline uint32_t wave_sum(uint32_t v) { return rl(wave_inscan(v), 63); }
+// Value of `v` in lane y (per-lane y < m, m small): a scalar loop over the candidate lanes, usable
+// in divergent code where ds_bpermute would read inactive lanes.
+__device__ inline uint32_t from_lane_nd(uint32_t v, uint32_t y, uint32_t m)
+{
+    uint32_t r = 0;
+    for (uint32_t j = 0; j < m; ++j)
+        if (y == j) r = rl(v, j);
+    return r;
+}
+
+// Node whose [off, off + cnt) range holds item x: the last node j with cnt_j > 0 and off_j <= x
+// (off and cnt are lane-per-node values).
+__device__ inline uint32_t node_of(uint32_t x, uint32_t off, uint32_t cnt, uint32_t n)
+{
+    uint32_t kx = 0;
+    for (uint32_t j = 0; j < n; ++j)
+        if (rl(cnt, j) && x >= rl(off, j)) kx = j;
+    return kx;
 }
 
 // arr[key] += number of lanes with that key, one atomic per distinct key of the wave.
@@ -159,12 +194,14 @@ __device__ inline void agg_add(bool v, uint32_t key, uint32_t* arr)
     const uint32_t lane = lane_id();
     while (act) {
         const int leader = __ffsll((unsigned long long)act) - 1;
-        const uint32_t lk = (uint32_t)__builtin_amdgcn_readlane((int)key, leader);
+        const uint32_t lk = rl(key, (uint32_t)leader);
         const uint64_t m = __ballot(v && key == lk);
         if (lane == (uint32_t)leader) atomicAdd(&arr[lk], (uint32_t)__popcll(m));
         act &= ~m;
     }
 }
+
+__device__ inline uint32_t div_c(uint32_t x, uint32_t C, uint32_t shift) { return shift < 32u ? x >> shift : x / C; }
 
 // ---- team formation --------------------------------------------------------------------------
 // Every workgroup of the grid registers under its XCC id; the members are the first K of the XCC
@@ -221,7 +258,8 @@ __device__ inline void team_load(const TeamArgs& a, TeamLds& L)
         L.head[tid] = e.st[tid].head;
         L.tail[tid] = e.st[tid].tail;
         L.digest[tid] = e.ctr[CNT_DIGEST0 + tid];
-        L.active[tid] = 0u;
+        L.sb[0].active[tid] = 0u;
+        L.sb[1].active[tid] = 0u;
     }
     if (tid == 0) {
         L.top = e.ctl[CTL_TOP];
@@ -238,8 +276,9 @@ __device__ inline void team_load(const TeamArgs& a, TeamLds& L)
         L.peak_live = e.ctr[CNT_PEAK_LIVE];
         L.zero_n = a.zero_g[0];
         L.status = 0u;
+        L.err_prev = 0u;
+        L.seg_err = 0u;
         L.exit_code = EXIT_NONE;
-        L.recenter = 0u;
         uint32_t big = 0;
         for (uint32_t k = 0; k < n; ++k)
             if (e.st[k].tail - e.st[k].head > a.rc) big = 1u;
@@ -257,8 +296,10 @@ __device__ inline void team_load(const TeamArgs& a, TeamLds& L)
     __syncthreads();
 }
 
-// Member 0 writes the pre-select state back in the wide engine's layout.
-__device__ inline void team_flush(const TeamArgs& a, TeamLds& L)
+// Member 0 writes the pre-select state back in the wide engine's layout, with the count rows the
+// last executed step consumed (cleared by the next launch's first step): the partition inputs of
+// Bl, or the rows this launch was handed when it executed no step.
+__device__ inline void team_flush(const TeamArgs& a, TeamLds& L, const StepBuf* Bl)
 {
     const Engine& e = a.e;
     const uint32_t tid = threadIdx.x, n = e.n_nodes, rcm = a.rc - 1u;
@@ -275,9 +316,23 @@ __device__ inline void team_flush(const TeamArgs& a, TeamLds& L)
         e.ctr[CNT_DIGEST0 + tid] = L.digest[tid];
     }
     for (uint32_t i = tid; i < L.top - L.fs_base; i += kTeamThreads) e.free_stack[L.fs_base + i] = L.fs[i];
-    for (uint32_t i = tid; i < L.zero_n; i += kTeamThreads) a.zero_g[1 + i] = L.zero[i];
+    uint32_t zn = 0;
+    if (Bl) {
+        for (uint32_t pmk = Bl->part_mask; pmk; pmk &= pmk - 1u) {
+            const uint32_t k = (uint32_t)(__ffs(pmk) - 1);
+            const uint32_t io = Bl->in_off[k], m = Bl->n_in[k];
+            for (uint32_t i = tid; i < m; i += kTeamThreads) {
+                const gf_chunk_desc d = Bl->in[io + i];
+                a.zero_g[1 + zn + i] = d.index | (d.generation & 1u) << 31;
+            }
+            zn += m;
+        }
+    } else {
+        for (uint32_t i = tid; i < L.zero_n; i += kTeamThreads) a.zero_g[1 + i] = L.zero[i];
+        zn = L.zero_n;
+    }
     if (tid == 0) {
-        a.zero_g[0] = L.zero_n;
+        a.zero_g[0] = zn;
         e.ctl[CTL_TOP] = L.top;
         e.ctr[CNT_GENERATED] = L.generated;
         e.ctr[CNT_SUNK] = L.sunk;
@@ -300,104 +355,164 @@ __device__ inline void team_flush(const TeamArgs& a, TeamLds& L)
     }
 }
 
-// ---- wave-level helpers ------------------------------------------------------------------------
-__device__ inline uint32_t rl(uint32_t v, uint32_t k) { return (uint32_t)__builtin_amdgcn_readlane((int)v, (int)k); }
-__device__ inline unsigned long long rl64(unsigned long long v, uint32_t k)
-{
-    return ((unsigned long long)rl((uint32_t)(v >> 32), k) << 32) | rl((uint32_t)v, k);
-}
-// Value of `v` in lane `src` (all lanes must be active).
-__device__ inline uint32_t from_lane(uint32_t v, uint32_t src) { return (uint32_t)__shfl((int)v, (int)src); }
-__device__ inline unsigned long long from_lane64(unsigned long long v, uint32_t src)
-{
-    return ((unsigned long long)from_lane((uint32_t)(v >> 32), src) << 32) | from_lane((uint32_t)v, src);
-}
-__device__ inline uint32_t wave_sum(uint32_t v) { return rl(wave_inscan(v), 63); }
+// ---- scheduler registers (wave 3) ---------------------------------------------------------------
+// Lane k = node k, kept in registers for the whole launch: ring cursors, the fill and segment sums
+// of the node's ready chunks, the sink digest count and the node's static kind and bins.
+struct SchedRegs {
+    unsigned long long head, tail, digest;
+    unsigned long long items;  // items of the node's batches in this launch
+    uint32_t rsum, rseg, kind, bins;
+    uint32_t nbat;             // batches of the node in this launch
+};
 
-// Node whose [off, off + cnt) range holds item x: the last node j with cnt_j > 0 and off_j <= x
-// (off and cnt are lane-per-node values).
-__device__ inline uint32_t node_of(uint32_t x, uint32_t off, uint32_t cnt, uint32_t n)
+__device__ inline void sched_load(const TeamArgs& a, const TeamLds& L, SchedRegs& g)
 {
-    uint32_t kx = 0;
-    for (uint32_t j = 0; j < n; ++j)
-        if (rl(cnt, j) && x >= rl(off, j)) kx = j;
-    return kx;
+    const uint32_t lane = lane_id(), rcm = a.rc - 1u;
+    const bool own = lane < a.e.n_nodes;
+    const uint32_t k = own ? lane : 0u;
+    g.kind = own ? L.kind[k] : 0xFFu;
+    g.bins = g.kind == KIND_PARTITION ? L.nd[k].bins : 1u;
+    g.head = own ? L.head[k] : 0ull;
+    g.tail = own ? L.tail[k] : 0ull;
+    g.digest = own ? L.digest[k] : 0ull;
+    g.rsum = 0u;
+    g.rseg = 0u;
+    g.items = 0ull;
+    g.nbat = 0u;
+    for (unsigned long long p = g.head; p < g.tail; ++p) {
+        const uint32_t f = L.ring[k * a.rc + (uint32_t)(p & rcm)].fill;
+        g.rsum += f;
+        g.rseg += (f + kSeg - 1u) >> kSegShift;
+    }
 }
 
-// ---- select (wave 4): the next step's batches, or the reason to stop -------------------------
-// Lane k holds node k. Credits are handed out in node order by a scalar loop over the lanes'
-// values; every array write is lane-parallel.
-__device__ inline void team_select(const TeamArgs& a, TeamLds& L, bool at_limit)
+__device__ inline void sched_store(const TeamArgs& a, TeamLds& L, const SchedRegs& g)
+{
+    const uint32_t lane = lane_id();
+    if (lane < a.e.n_nodes) {
+        L.head[lane] = g.head;
+        L.tail[lane] = g.tail;
+        L.digest[lane] = g.digest;
+    }
+    // launch totals per kind
+    unsigned long long src = 0, part = 0, xf = 0, nb = 0;
+    for (uint32_t j = 0; j < a.e.n_nodes; ++j) {
+        const uint32_t kj = rl(g.kind, j);
+        const unsigned long long it = rl64(g.items, j);
+        if (kj == KIND_SOURCE) src += it;
+        else if (kj == KIND_PARTITION) part += it;
+        else if (kj == KIND_TRANSFORM) xf += it;
+        nb += rl(g.nbat, j);
+    }
+    if (lane == 0) {
+        L.src_items += src;
+        L.part_items += part;
+        L.xf_items += xf;
+        L.batches += nb;
+    }
+    wave_sync();
+}
+
+// ---- select (wave 3): the next step's batches into Bn, or the reason to stop ------------------
+// Credits are handed out in node order by a scalar loop over the nodes that may take a batch; a
+// node that takes its whole ring needs no ring reads (the ring sums are in registers).
+__device__ inline void team_select(const TeamArgs& a, TeamLds& L, StepBuf& Bn, bool at_limit, SchedRegs& g)
 {
     const Engine& e = a.e;
     const uint32_t lane = lane_id(), n = e.n_nodes, C = e.C, rcm = a.rc - 1u;
-    Ticker tk(e.prof, L.prof_member && lane == 0u);
+    Ticker tk(L.prof_acc, L.prof_member && lane == 0u);
     const bool own = lane < n;
     const uint32_t k = own ? lane : 0u;
-    const uint32_t kind = own ? L.kind[k] : 0xFFu;
+    const uint32_t kind = g.kind, B = g.bins;
     const bool xf = kind == KIND_PARTITION || kind == KIND_TRANSFORM;
-    const uint32_t B = kind == KIND_PARTITION ? L.nd[k].bins : 1u;
-    const unsigned long long head = own ? L.head[k] : 0ull, tail = own ? L.tail[k] : 0ull;
+    const unsigned long long head = g.head, tail = g.tail;
     const uint32_t avail = (uint32_t)(tail - head);
     const gf_chunk_desc* rg = L.ring + k * a.rc;
-    uint32_t want = kind == KIND_SINK ? avail : xf ? (avail < e.Q ? avail : e.Q) : 0u;
-    uint32_t parts = 0;
-    for (uint32_t i = 0; i < want; ++i) parts += rg[(uint32_t)((head + i) & rcm)].fill;
+    uint32_t want = 0, parts = 0, nseg = 0;
+    if (kind == KIND_SINK || (xf && avail <= e.Q)) {
+        want = avail;
+        parts = g.rsum;
+        nseg = g.rseg;
+    } else if (xf) {
+        want = e.Q;
+        for (uint32_t i = 0; i < want; ++i) {
+            const uint32_t f = rg[(uint32_t)((head + i) & rcm)].fill;
+            parts += f;
+            nseg += (f + kSeg - 1u) >> kSegShift;
+        }
+    }
     uint32_t bound = xf ? (uint32_t)bound_of(parts, C, B) : 0u;
     tk(PROF_C_SEL_FILLS);
-    // credits in node order
+    // credits in node order, over the nodes that may take a batch
     const uint32_t top = L.top;
     uint32_t credits = top;
     const unsigned long long gen0 = L.generated, sunk = L.sunk;
     unsigned long long gen = gen0;
     uint32_t src_fill = 0, src_node = 0xFFFFFFFFu;
-    for (uint32_t j = 0; j < n; ++j) {
-        const uint32_t kj = rl(kind, j);
-        if (kj == KIND_SOURCE) {
-            const unsigned long long P = e.P, live = gen - sunk, remaining = e.N - gen;
-            unsigned long long min_fill = C < P ? C : P;
-            if (remaining < min_fill) min_fill = remaining;
-            if (gen < e.N && live < P && P - live >= min_fill && credits >= 1u) {
-                unsigned long long fill = remaining;
-                if (fill > P - live) fill = P - live;
-                if (fill > C) fill = C;
-                src_fill = (uint32_t)fill;
+    // the source's admission (live < P, room for min(C, P, remaining)) does not depend on credits
+    // beyond the one chunk it needs; the source lane evaluates it up front
+    uint32_t adm = 0;
+    if (kind == KIND_SOURCE) {
+        const unsigned long long P = e.P, live = gen0 - sunk, remaining = e.N - gen0;
+        unsigned long long min_fill = C < P ? C : P;
+        if (remaining < min_fill) min_fill = remaining;
+        if (gen0 < e.N && live < P && P - live >= min_fill) {
+            unsigned long long fill = remaining;
+            if (fill > P - live) fill = P - live;
+            if (fill > C) fill = C;
+            adm = (uint32_t)fill;
+        }
+    }
+    uint64_t cand = __ballot(adm != 0u || (xf && want));
+    while (cand) {
+        const uint32_t j = (uint32_t)(__ffsll((unsigned long long)cand) - 1);
+        cand &= cand - 1u;
+        if (rl(kind, j) == KIND_SOURCE) {
+            if (credits >= 1u) {
+                src_fill = rl(adm, j);
                 src_node = j;
-                gen += fill;
+                gen += src_fill;
                 credits -= 1u;
             }
-        } else if (kj == KIND_PARTITION || kj == KIND_TRANSFORM) {
-            if (!rl(want, j)) continue;
+        } else {
             if (rl(bound, j) > credits && lane == j) {
                 // largest prefix of the candidates whose bound fits the remaining credits
-                uint32_t m = want, p = parts, b = 0;
+                uint32_t m = want, p = parts, sg = nseg, b = 0;
                 while (m) {
                     b = (uint32_t)bound_of(p, C, B);
                     if (b <= credits) break;
                     --m;
-                    p -= rg[(uint32_t)((head + m) & rcm)].fill;
+                    const uint32_t f = rg[(uint32_t)((head + m) & rcm)].fill;
+                    p -= f;
+                    sg -= (f + kSeg - 1u) >> kSegShift;
                 }
                 want = m;
                 parts = p;
+                nseg = sg;
                 bound = m ? b : 0u;
             }
             credits -= rl(bound, j);
         }
     }
     tk(PROF_C_SEL_CREDIT);
-    uint32_t nin = 0, act = 0, nseg = 0;
+    uint32_t nin = 0, act = 0;
     if (kind == KIND_SOURCE) {
         act = lane == src_node ? 1u : 0u;
         parts = act ? src_fill : 0u;
-        nseg = (parts + kSeg - 1u) / kSeg;
-    } else if (own) {
+        nseg = (parts + kSeg - 1u) >> kSegShift;
+    } else if (own && want) {
         nin = want;
-        act = nin ? 1u : 0u;
-        for (uint32_t i = 0; i < nin; ++i) nseg += (rg[(uint32_t)((head + i) & rcm)].fill + kSeg - 1u) / kSeg;
+        act = 1u;
+    } else {
+        parts = 0u;
+        nseg = 0u;
     }
     const uint32_t prew = (kind == KIND_PARTITION && act) ? nseg * (B + 1u) : 0u;
-    const uint32_t i_inc = wave_inscan(nin), s_inc = wave_inscan(nseg), p_inc = wave_inscan(prew);
-    const uint32_t in_tot = rl(i_inc, 63), seg_tot = rl(s_inc, 63), pre_tot = rl(p_inc, 63);
+    // segments are numbered in node order, the sinks' after everybody else's
+    const uint32_t nsm = kind == KIND_SINK ? 0u : nseg, nss = kind == KIND_SINK ? nseg : 0u;
+    const uint32_t i_inc = wave_inscan(nin), sm_inc = wave_inscan(nsm), ss_inc = wave_inscan(nss), p_inc = wave_inscan(prew);
+    const uint32_t g_main = rl(sm_inc, 63);
+    const uint32_t in_tot = rl(i_inc, 63), seg_tot = g_main + rl(ss_inc, 63), pre_tot = rl(p_inc, 63);
     const uint32_t reserved = top - credits;
     const uint64_t actm = __ballot(act != 0u), pend = __ballot(own && tail != head);
     uint32_t code = EXIT_NONE;
@@ -417,156 +532,214 @@ __device__ inline void team_select(const TeamArgs& a, TeamLds& L, bool at_limit)
         wave_sync();
         return;
     }
-    const uint32_t inoff = i_inc - nin, segoff = s_inc - nseg;
+    const uint32_t inoff = i_inc - nin, segoff = kind == KIND_SINK ? g_main + ss_inc - nss : sm_inc - nsm;
     if (own) {
-        L.active[k] = act;
-        L.n_in[k] = nin;
-        L.particles[k] = parts;
-        L.in_off[k] = inoff;
-        L.seg_off[k] = segoff;
-        L.pre_off[k] = p_inc - prew;
-        L.head[k] = head + nin;
+        Bn.active[k] = act;
+        Bn.n_in[k] = nin;
+        Bn.particles[k] = parts;
+        Bn.in_off[k] = inoff;
+        Bn.seg_off[k] = segoff;
+        Bn.nseg[k] = nseg;
+        Bn.pre_off[k] = p_inc - prew;
         if (kind == KIND_SINK && act) {
-            const unsigned long long dg = L.digest[k];
-            L.digest_base[k] = dg;
-            L.digest[k] = dg + nin;
+            Bn.digest_base[k] = g.digest;
+            g.digest += nin;
         }
-        if (kind == KIND_SOURCE && act) L.first_id[k] = gen0;
+        if (kind == KIND_SOURCE && act) Bn.first_id[k] = gen0;
+        if (act && kind != KIND_SOURCE) {
+            g.head = head + nin;
+            g.rsum -= parts;
+            g.rseg -= nseg;
+        }
     }
+    if (act) {
+        g.items += parts;
+        if (xf) g.nbat += 1u;
+    }
+    uint32_t sum_sink = 0;
+    for (uint64_t sm = __ballot(act && kind == KIND_SINK); sm; sm &= sm - 1u)
+        sum_sink += rl(parts, (uint32_t)(__ffsll((unsigned long long)sm) - 1));
     if (lane == n) {
-        L.in_off[n] = in_tot;
-        L.seg_off[n] = seg_tot;
+        Bn.in_off[n] = in_tot;
+        Bn.seg_off[n] = seg_tot;
     }
+    const uint32_t pmask = (uint32_t)__ballot(act && kind == KIND_PARTITION);  // nodes < 32
     if (lane == 0) {
         L.generated = gen;
-        L.reserved = reserved;
+        Bn.g_main = g_main;
+        Bn.reserved = reserved;
+        Bn.sum_sink = sum_sink;
+        Bn.part_mask = pmask;
         const uint32_t nb = top > kTeamStack / 2u ? top - kTeamStack / 2u : 0u;
         const uint32_t fb = L.fs_base;
-        L.recenter = (nb != fb && (top < fb + kTeamMargin || top + kTeamMargin > fb + kTeamStack)) ? 1u : 0u;
+        Bn.recenter = (nb != fb && (top < fb + kTeamMargin || top + kTeamMargin > fb + kTeamStack)) ? 1u : 0u;
         L.exit_code = EXIT_NONE;
     }
-    // input list, item prefixes, segment map and count rows, flattened over the step's inputs
-    // (node order; the source has no inputs)
-    const uint32_t nf = kind == KIND_SOURCE ? 0u : parts, nsf = kind == KIND_SOURCE ? 0u : nseg;
-    const uint32_t F = wave_inscan(nf) - nf, S = wave_inscan(nsf) - nsf;
-    uint32_t carry_f = 0, carry_s = 0;
-    for (uint32_t x0 = 0; x0 < in_tot; x0 += 64u) {
-        const uint32_t x = x0 + lane;
-        const bool v = x < in_tot;
-        const uint32_t kx = node_of(x, inoff, nin, n);
-        const uint32_t i = x - from_lane(inoff, kx);
-        const unsigned long long hx = from_lane64(head, kx);
-        const uint32_t Fx = from_lane(F, kx), Sx = from_lane(S, kx), sox = from_lane(segoff, kx);
-        const uint32_t kindx = from_lane(kind, kx);
-        gf_chunk_desc d{0, 0, 0, 0};
-        if (v) {
-            d = L.ring[kx * a.rc + (uint32_t)((hx + i) & rcm)];
-            L.in[x] = d;
-        }
-        const uint32_t fill = v ? d.fill : 0u, ns = (fill + kSeg - 1u) / kSeg;
-        const uint32_t fi = wave_inscan(fill), si = wave_inscan(ns);
-        if (v) {
-            L.in_items[x] = carry_f + fi - fill - Fx;
-            const uint32_t sp = sox + carry_s + si - ns - Sx;
-            const uint32_t row0 = ((d.index * 2u + (d.generation & 1u)) * a.tpc) * a.bb;
-            for (uint32_t g = 0; g < ns; ++g) {
-                L.seg_map[sp + g] = seg_pack(kx, i, g);
-                if (kindx == KIND_PARTITION) L.seg_row[sp + g] = row0 + g * a.bb;
+    // per active node (node order): its inputs' descriptors and first segments, and for plain
+    // transforms the items before each input
+    for (uint64_t actn = __ballot(act && kind != KIND_SOURCE); actn; actn &= actn - 1u) {
+        const uint32_t kk = (uint32_t)(__ffsll((unsigned long long)actn) - 1);
+        const uint32_t m = rl(nin, kk), io = rl(inoff, kk), so = rl(segoff, kk);
+        const unsigned long long h = rl64(head, kk);
+        const bool plain = rl(kind, kk) == KIND_TRANSFORM;
+        const gf_chunk_desc* rk = L.ring + kk * a.rc;
+        uint32_t carry_f = 0, carry_s = 0;
+        for (uint32_t y0 = 0; y0 < m; y0 += 64u) {
+            const uint32_t y = y0 + lane;
+            const bool v = y < m;
+            gf_chunk_desc d{0, 0, 0, 0};
+            if (v) d = rk[(uint32_t)((h + y) & rcm)];
+            const uint32_t ns = (d.fill + kSeg - 1u) >> kSegShift, si = wave_inscan(ns);
+            const uint32_t first = so + carry_s + si - ns;
+            if (v) {
+                Bn.in[io + y] = d;
+                Bn.in_seg[io + y] = first;
+            }
+            carry_s += rl(si, 63);
+            if (plain) {
+                const uint32_t fi = wave_inscan(d.fill);
+                if (v) Bn.in_items[io + y] = carry_f + fi - d.fill;
+                carry_f += rl(fi, 63);
             }
         }
-        carry_f += rl(fi, 63);
-        carry_s += rl(si, 63);
-    }
-    if (src_node != 0xFFFFFFFFu) {
-        const uint32_t so = rl(segoff, src_node), ns = rl(nseg, src_node);
-        for (uint32_t g = lane; g < ns; g += 64u) L.seg_map[so + g] = seg_pack(src_node, 0u, g);
     }
     wave_sync();
     tk(PROF_C_SEL_LISTS);
 }
 
-// ---- after the barrier: status, clears, segment counts and their scan, free-stack window --------
-__device__ inline void team_counts(const TeamArgs& a, TeamLds& L, uint32_t me, uint32_t T, uint32_t it)
+// Move the free-stack window so that the step's pops and pushes stay inside it (all threads).
+__device__ inline void team_recenter(const TeamArgs& a, TeamLds& L, uint32_t me)
 {
     const Engine& e = a.e;
-    const uint32_t tid = threadIdx.x, n = e.n_nodes, w = tid >> 6, lane = lane_id();
-    const uint32_t row = a.tpc * a.bb;
-    Ticker tk(e.prof, L.prof_member && tid == 0u);
+    const uint32_t tid = threadIdx.x;
+    const uint32_t old = L.fs_base, top = L.top;
+    const uint32_t nb = top > kTeamStack / 2u ? top - kTeamStack / 2u : 0u;
+    constexpr uint32_t per = (kTeamStack + kTeamThreads - 1u) / kTeamThreads;
+    unsigned long long keep[per];
+    // entries of [max(old, nb), top) stay in LDS at a new offset; member 0 spills [old, nb)
+#pragma unroll
+    for (uint32_t r = 0; r < per; ++r) {
+        const uint32_t gi = (nb > old ? nb : old) + tid + r * kTeamThreads;
+        keep[r] = gi < top ? L.fs[gi - old] : 0ull;
+    }
+    if (me == 0 && tid < kClrThreads)
+        for (uint32_t gi = old + tid; gi < nb && gi < top; gi += kClrThreads) e.free_stack[gi] = L.fs[gi - old];
+    __syncthreads();
+#pragma unroll
+    for (uint32_t r = 0; r < per; ++r) {
+        const uint32_t gi = (nb > old ? nb : old) + tid + r * kTeamThreads;
+        if (gi < top) L.fs[gi - nb] = keep[r];
+    }
+    // entries [nb, old) come back from global memory (written by an earlier spill or launch)
+    for (uint32_t gi = nb + tid; gi < old; gi += kTeamThreads) L.fs[gi - nb] = ld_u64(e.free_stack, gi);
+    __syncthreads();
+    if (tid == 0) L.fs_base = nb;
+    __syncthreads();
+}
+
+// ---- after the barrier: counts (all waves) and plan (wave 0) -----------------------------------
+// Every wave clears its share of the count rows the previous step consumed and scans some bins of
+// every partition batch (lane = segment): L.pre[g][b] = items of bin b in the batch's segments
+// before g, L.tot[b] = all of them. After one workgroup barrier wave 0 derives every node's exact
+// output count, checks the batch and pops the step's output chunks.
+__device__ inline void team_countplan(const TeamArgs& a, TeamLds& L, const StepBuf& Bc, const StepBuf* Bp, uint32_t me,
+                                      uint32_t T, uint32_t it)
+{
+    const Engine& e = a.e;
+    const uint32_t tid = threadIdx.x, w = tid >> 6, lane = lane_id(), n = e.n_nodes, C = e.C;
+    if (Bc.recenter) team_recenter(a, L, me);
+    Ticker tk(L.prof_acc, L.prof_member && tid == 0u);
     // errors members published before the previous barrier (parity word: a member already one
     // step ahead writes the other word)
-    if (tid == 0) {
-        L.status = it ? ld_u32(a.tw, TW_ERR + ((it - 1u) & 1u)) : 0u;
-        L.seg_err = 0u;
+    const uint32_t err = (tid == 64u && it) ? ld_u32(a.tw, TW_ERR + ((it - 1u) & 1u)) : 0u;
+    // clear the count rows of the chunks partitions consumed in the previous step (a chunk's rows
+    // are one contiguous block of tpc * bb words)
+    const uint32_t blk = a.tpc * a.bb;
+    if (Bp) {
+        for (uint32_t pmk = Bp->part_mask; pmk; pmk &= pmk - 1u) {
+            const uint32_t k = (uint32_t)(__ffs(pmk) - 1);
+            const uint32_t io = Bp->in_off[k], zw = Bp->n_in[k] * blk;
+            for (uint32_t x = me * kClrThreads + tid; tid < kClrThreads && x < zw; x += T * kClrThreads) {
+                const gf_chunk_desc d = Bp->in[io + x / blk];
+                a.cnt[((size_t)d.index * 2u + (d.generation & 1u)) * blk + x % blk] = 0u;
+            }
+        }
+    } else {
+        const uint32_t zw = L.zero_n * blk;
+        for (uint32_t x = me * kClrThreads + tid; tid < kClrThreads && x < zw; x += T * kClrThreads) {
+            const uint32_t z = L.zero[x / blk];
+            a.cnt[((size_t)(z & 0x7FFFFFFFu) * 2u + (z >> 31)) * blk + x % blk] = 0u;
+        }
     }
-    // clear the count rows of the chunks partitions consumed in the previous step
-    const uint32_t zw = L.zero_n * row;
-    for (uint32_t x = me * kTeamThreads + tid; x < zw; x += T * kTeamThreads) {
-        const uint32_t z = L.zero[x / row];
-        a.cnt[((size_t)(z & 0x7FFFFFFFu) * 2u + (z >> 31)) * row + x % row] = 0u;
-    }
-    tk(PROF_C_CNT_CLEAR);
-    // wave b loads bin b's count of every segment of a partition batch and scans it in registers:
-    // L.pre[g][b] = items of bin b in the batch's segments before g, L.tot[b] = all of them
-    for (uint32_t k = 0; k < n; ++k) {
-        if (!L.active[k] || L.kind[k] != KIND_PARTITION) continue;
-        const uint32_t B1 = L.nd[k].bins + 1u, so = L.seg_off[k], ns = L.seg_off[k + 1] - so;
-        uint32_t* m = L.pre + L.pre_off[k];
+    for (uint32_t pmk = Bc.part_mask; pmk; pmk &= pmk - 1u) {
+        const uint32_t k = (uint32_t)(__ffs(pmk) - 1);
+        const uint32_t B1 = L.nd[k].bins + 1u, so = Bc.seg_off[k], ns = Bc.nseg[k];
+        const uint32_t io = Bc.in_off[k], mi = Bc.n_in[k];
+        uint32_t* m = L.pre + Bc.pre_off[k];
+        // first segment and count-row base of input `lane` (inputs of a partition batch <= 64
+        // keep them in registers; more inputs search LDS)
+        const gf_chunk_desc dl = lane < mi ? Bc.in[io + lane] : gf_chunk_desc{0, 0, 0, 0};
+        const uint32_t fl = lane < mi ? Bc.in_seg[io + lane] : 0xFFFFFFFFu;
+        const uint32_t rowl = ((dl.index * 2u + (dl.generation & 1u)) * a.tpc) * a.bb;
+        auto row_of = [&](uint32_t g) -> size_t {
+            const uint32_t gs = so + g;
+            uint32_t f = rl(fl, 0), r = rl(rowl, 0);
+            if (mi <= 64u) {
+                for (uint32_t y = 1; y < mi; ++y)
+                    if (rl(fl, y) <= gs) {
+                        f = rl(fl, y);
+                        r = rl(rowl, y);
+                    }
+            } else {
+                for (uint32_t y = 1; y < mi; ++y) {
+                    const uint32_t fy = Bc.in_seg[io + y];
+                    if (fy <= gs) {
+                        const gf_chunk_desc d = Bc.in[io + y];
+                        f = fy;
+                        r = ((d.index * 2u + (d.generation & 1u)) * a.tpc) * a.bb;
+                    }
+                }
+            }
+            return (size_t)r + (size_t)(gs - f) * a.bb;
+        };
         for (uint32_t b = w; b < B1; b += kTeamWaves) {
             uint32_t carry = 0;
-            for (uint32_t g0 = 0; g0 < ns; g0 += 64u) {
-                const uint32_t g = g0 + lane;
-                const uint32_t x = g < ns ? ld_u32(a.cnt, (size_t)L.seg_row[so + g] + b) : 0u;
-                const uint32_t inc = wave_inscan(x);
-                if (g < ns) m[g * B1 + b] = carry + inc - x;
-                carry += rl(inc, 63);
+            for (uint32_t g0 = 0; g0 < ns; g0 += 128u) {
+                const uint32_t ga = g0 + lane, gb = ga + 64u;
+                const bool va = ga < ns, vb = gb < ns;
+                const size_t ra = row_of(ga), rb = row_of(gb);
+                const uint32_t xa = va ? ld_u32(a.cnt, ra + b) : 0u;
+                const uint32_t xb = vb ? ld_u32(a.cnt, rb + b) : 0u;
+                const uint32_t ia = wave_inscan(xa), ib = wave_inscan(xb);
+                const uint32_t ta = rl(ia, 63);
+                if (va) m[ga * B1 + b] = carry + ia - xa;
+                if (vb) m[gb * B1 + b] = carry + ta + ib - xb;
+                carry += ta + rl(ib, 63);
             }
             if (lane == 0) L.tot[k * kTeamBins + b] = carry;
         }
     }
-    tk(PROF_C_CNT_SCAN);
-    // move the free-stack window so that this step's pops and pushes stay inside it
-    if (L.recenter) {
-        const uint32_t old = L.fs_base, top = L.top;
-        const uint32_t nb = top > kTeamStack / 2u ? top - kTeamStack / 2u : 0u;
-        constexpr uint32_t per = (kTeamStack + kTeamThreads - 1u) / kTeamThreads;
-        unsigned long long keep[per];
-        // entries of [max(old, nb), top) stay in LDS at a new offset; member 0 spills [old, nb)
-#pragma unroll
-        for (uint32_t r = 0; r < per; ++r) {
-            const uint32_t gi = (nb > old ? nb : old) + tid + r * kTeamThreads;
-            keep[r] = gi < top ? L.fs[gi - old] : 0ull;
-        }
-        if (me == 0)
-            for (uint32_t gi = old + tid; gi < nb && gi < top; gi += kTeamThreads) e.free_stack[gi] = L.fs[gi - old];
-        __syncthreads();
-#pragma unroll
-        for (uint32_t r = 0; r < per; ++r) {
-            const uint32_t gi = (nb > old ? nb : old) + tid + r * kTeamThreads;
-            if (gi < top) L.fs[gi - nb] = keep[r];
-        }
-        // entries [nb, old) come back from global memory (written by an earlier spill or launch)
-        for (uint32_t gi = nb + tid; gi < old; gi += kTeamThreads) L.fs[gi - nb] = ld_u64(e.free_stack, gi);
-        if (tid == 0) {
-            L.fs_base = nb;
-            L.recenter = 0u;
-        }
+    if (tid == 64u) {
+        L.seg_err = 0u;
+        L.err_prev = err;
     }
+    tk(PROF_C_CNT_SCAN);
     __syncthreads();
-}
-
-// ---- plan (wave 0): exact output counts, bin bases, checks, pops --------------------------------
-// Lane k holds node k; a partition's bins are handled one per lane.
-__device__ inline void team_plan(const TeamArgs& a, TeamLds& L)
-{
-    const Engine& e = a.e;
-    const uint32_t tid = threadIdx.x, n = e.n_nodes, C = e.C, lane = lane_id();
-    if (tid < 64u) {
-        Ticker tk(e.prof, L.prof_member && lane == 0u);
+    if (w == 0u) {
+        // lanes 0..31: nodes; lanes 32..63: bins of the first partition node of the step
         const bool own = lane < n;
         const uint32_t k = own ? lane : 0u;
-        const uint32_t act = own ? L.active[k] : 0u, kind = own ? L.kind[k] : 0xFFu;
-        const uint32_t parts = own ? L.particles[k] : 0u, bound = own ? L.bound[k] : 0u;
-        uint32_t R = 0, st = 0;
+        const uint32_t act = own ? Bc.active[k] : 0u, kind = own ? L.kind[k] : 0xFFu;
+        const uint32_t parts = own ? Bc.particles[k] : 0u, bound = own ? L.bound[k] : 0u, ro = own ? L.route_off[k] : 0u;
+        const uint32_t pm = Bc.part_mask;
+        const uint32_t kp = pm ? (uint32_t)(__ffs(pm) - 1) : 0u;
+        const uint32_t B = pm ? L.nd[kp].bins : 0u;
+        const uint32_t bl = lane - 32u;  // bin of this lane (lanes >= 32)
+        const uint32_t t = (pm && lane >= 32u && bl <= B) ? L.tot[kp * kTeamBins + bl] : 0u;
+        const uint32_t tv = (lane >= 32u && bl < B) ? t : 0u, r = (tv + C - 1u) / C;
+        const uint32_t ri = wave_inscan(r), ti = wave_inscan(tv);
+        const uint32_t bbase = ri - r;  // first output of bin bl within the partition's outputs
+        uint32_t R = 0, st = 0, inv = 0;
         if (act) {
             if (kind == KIND_TRANSFORM) {
                 R = (parts + C - 1u) / C;
@@ -575,31 +748,36 @@ __device__ inline void team_plan(const TeamArgs& a, TeamLds& L)
                 R = 1u;
             }
         }
-        uint64_t pm = __ballot(act && kind == KIND_PARTITION);
-        uint32_t inv = 0, pk = ~0u, pB = 0, bbl = 0, totl = 0;  // bins of the last partition node, lane b
-        while (pm) {
-            const uint32_t kp = (uint32_t)(__ffsll((unsigned long long)pm) - 1);
-            pm &= pm - 1u;
-            const uint32_t B = L.nd[kp].bins;
-            const uint32_t t = lane <= B ? L.tot[kp * kTeamBins + lane] : 0u;
-            const uint32_t tv = lane < B ? t : 0u, r = (tv + C - 1u) / C;
-            const uint32_t ri = wave_inscan(r), ti = wave_inscan(tv);
-            if (lane < B) L.binbase[kp * kTeamBins + lane] = ri - r;
+        if (pm) {
             const uint32_t Rk = rl(ri, 63), sum = rl(ti, 63);
-            if (rl(t, B)) inv = 1u;
+            if (rl(t, 32u + B)) inv = 1u;
             if (lane == kp) {
                 R = Rk;
                 if (sum != parts || Rk > bound) st = GF_PLAN_MISMATCH;
             }
-            pk = kp;
-            pB = B;
-            bbl = ri - r;
-            totl = tv;
+            if (lane >= 32u && bl < B) L.binbase[kp * kTeamBins + bl] = bbase;
+        }
+        // further partition nodes of the same step (rare): one pass each through LDS
+        for (uint32_t pmk = pm & (pm - 1u); pmk; pmk &= pmk - 1u) {
+            const uint32_t kq = (uint32_t)(__ffs(pmk) - 1);
+            const uint32_t Bq = L.nd[kq].bins;
+            const uint32_t tq = lane <= Bq ? L.tot[kq * kTeamBins + lane] : 0u;
+            const uint32_t tqv = lane < Bq ? tq : 0u, rq = (tqv + C - 1u) / C;
+            const uint32_t rqi = wave_inscan(rq), tqi = wave_inscan(tqv);
+            if (lane < Bq) L.binbase[kq * kTeamBins + lane] = rqi - rq;
+            // wave-wide values first: a readlane inside the lane branch would let the scans move
+            // under that branch's exec mask
+            const uint32_t Rq = rl(rqi, 63), sq = rl(tqi, 63);
+            if (rl(tq, Bq)) inv = 1u;
+            if (lane == kq) {
+                R = Rq;
+                if (sq != parts || Rq > bound) st = GF_PLAN_MISMATCH;
+            }
         }
         const uint64_t bad = __ballot(st != 0u);
         uint32_t status = inv ? (uint32_t)GF_INVALID : bad ? rl(st, (uint32_t)(__ffsll((unsigned long long)bad) - 1)) : 0u;
-        const uint32_t oi = wave_inscan(R), off = oi - R, total = rl(oi, 63);
-        if (!status && total > L.reserved) status = GF_PLAN_MISMATCH;
+        const uint32_t oi = wave_inscan(own ? R : 0u), off = oi - R, total = rl(oi, 63);
+        if (!status && total > Bc.reserved) status = GF_PLAN_MISMATCH;
         if (!status && total > L.top) status = GF_POOL_EXHAUSTED;
         if (own) {
             L.n_out[k] = R;
@@ -615,49 +793,57 @@ __device__ inline void team_plan(const TeamArgs& a, TeamLds& L)
         if (!status) {
             // output j of the step (node order, bin-major, k order) takes stack[top - 1 - j]
             const uint32_t top = L.top, fbase = L.fs_base;
-            for (uint32_t j0 = 0; j0 < total; j0 += 64u) {
-                const uint32_t j = j0 + lane;
-                const uint32_t kx = node_of(j, off, R, n);
-                const uint32_t r = j - from_lane(off, kx), kk = from_lane(kind, kx), pp = from_lane(parts, kx);
-                if (j < total) {
+            auto emit = [&](uint32_t j0, uint32_t cnt, uint32_t fill0, uint32_t rem0, uint32_t bin, uint32_t ridx) {
+                // outputs j0 .. j0+cnt-1: fills C, C, ..., the rest of rem0 (fill0 = C or the whole)
+                for (uint32_t q = lane; q < cnt; q += 64u) {
+                    const uint32_t j = j0 + q;
                     const unsigned long long ent = L.fs[top - 1u - j - fbase];
+                    const uint32_t rem = rem0 - q * fill0;
                     gf_chunk_desc d;
                     d.index = (uint32_t)ent;
                     d.generation = (uint32_t)(ent >> 32);
-                    d.bin = 0u;
-                    if (kk == KIND_SOURCE) {
-                        d.fill = pp;
-                    } else if (kk == KIND_TRANSFORM) {
-                        const uint32_t rem = pp - r * C;
-                        d.fill = rem < C ? rem : C;
-                    } else {
-                        // the non-empty bin whose output range holds r
-                        uint32_t b = 0, tb = 0, bb = 0;
-                        if (kx == pk) {
-                            for (uint32_t x = 0; x < pB; ++x) {
-                                const uint32_t tx = rl(totl, x), bx = rl(bbl, x);
-                                if (tx && bx <= r) {
-                                    b = x;
-                                    tb = tx;
-                                    bb = bx;
-                                }
-                            }
-                        } else {
-                            const uint32_t B = L.nd[kx].bins;
-                            for (uint32_t x = 0; x < B; ++x) {
-                                const uint32_t tx = L.tot[kx * kTeamBins + x], bx = L.binbase[kx * kTeamBins + x];
-                                if (tx && bx <= r) {
-                                    b = x;
-                                    tb = tx;
-                                    bb = bx;
-                                }
-                            }
-                        }
-                        const uint32_t rem = tb - (r - bb) * C;
-                        d.fill = rem < C ? rem : C;
-                        d.bin = b;
-                    }
+                    d.fill = rem < fill0 ? rem : fill0;
+                    d.bin = bin;
                     L.out[j] = d;
+                    L.out_ridx[j] = ridx;
+                }
+            };
+            // node lanes emit their source / transform outputs and bin lanes the first partition's,
+            // all lanes at once
+            const uint32_t kpo = rl(off, kp), kpr = rl(ro, kp);
+            uint32_t cnt = 0, j0 = 0, fill0 = C, rem0 = 0, bin = 0, ridx = 0;
+            if (own && R && (kind == KIND_SOURCE || kind == KIND_TRANSFORM)) {
+                cnt = R;
+                j0 = off;
+                fill0 = kind == KIND_SOURCE ? parts : C;
+                rem0 = parts;
+                ridx = ro;
+            } else if (lane >= 32u && r) {
+                cnt = r;
+                j0 = kpo + bbase;
+                rem0 = tv;
+                bin = bl;
+                ridx = kpr + bl;
+            }
+            for (uint32_t q = 0; q < cnt; ++q) {
+                const uint32_t j = j0 + q;
+                const unsigned long long ent = L.fs[top - 1u - j - fbase];
+                const uint32_t rem = rem0 - q * fill0;
+                gf_chunk_desc d;
+                d.index = (uint32_t)ent;
+                d.generation = (uint32_t)(ent >> 32);
+                d.fill = rem < fill0 ? rem : fill0;
+                d.bin = bin;
+                L.out[j] = d;
+                L.out_ridx[j] = ridx;
+            }
+            // further partition nodes of the step, one bin at a time
+            for (uint32_t pmk = pm & (pm - 1u); pmk; pmk &= pmk - 1u) {
+                const uint32_t kq = (uint32_t)(__ffs(pmk) - 1);
+                const uint32_t Bq = L.nd[kq].bins, base = rl(off, kq), rok = rl(ro, kq);
+                for (uint32_t x = 0; x < Bq; ++x) {
+                    const uint32_t tx = L.tot[kq * kTeamBins + x];
+                    if (tx) emit(base + L.binbase[kq * kTeamBins + x], (tx + C - 1u) / C, C, tx, x, rok + x);
                 }
             }
             if (lane == 0) {
@@ -671,15 +857,15 @@ __device__ inline void team_plan(const TeamArgs& a, TeamLds& L)
     __syncthreads();
 }
 
-// ---- trace (member 0) and retire (wave 4 of every member) --------------------------------------
-__device__ inline void team_trace(const TeamArgs& a, TeamLds& L)
+// ---- trace (member 0) and retire (wave 3 of every member) --------------------------------------
+__device__ inline void team_trace(const TeamArgs& a, TeamLds& L, const StepBuf& Bc)
 {
     const Engine& e = a.e;
     const uint32_t lane = lane_id(), n = e.n_nodes;
     for (uint32_t k = 0; k < n; ++k) {
-        if (!L.active[k]) continue;
+        if (!Bc.active[k]) continue;
         const uint32_t kind = L.kind[k];
-        const uint32_t nin = kind == KIND_SOURCE ? 0u : L.n_in[k];
+        const uint32_t nin = kind == KIND_SOURCE ? 0u : Bc.n_in[k];
         const uint32_t nb = kind == KIND_PARTITION ? L.nd[k].bins : kind == KIND_TRANSFORM ? 1u : 0u;
         const uint32_t R = L.n_out[k];
         const unsigned long long len = 5ull + 3ull * nin + nb + 4ull * R, p0 = L.trace_len;
@@ -694,13 +880,13 @@ __device__ inline void team_trace(const TeamArgs& a, TeamLds& L)
             put(4 + 3ull * nin + nb, R);
         }
         for (uint32_t i = lane; i < nin; i += 64u) {
-            const gf_chunk_desc d = L.in[L.in_off[k] + i];
+            const gf_chunk_desc d = Bc.in[Bc.in_off[k] + i];
             put(3 + 3ull * i, d.index);
             put(4 + 3ull * i, d.generation);
             put(5 + 3ull * i, d.fill);
         }
         for (uint32_t b = lane; b < nb; b += 64u)
-            put(4 + 3ull * nin + b, kind == KIND_PARTITION ? L.tot[k * kTeamBins + b] : L.particles[k]);
+            put(4 + 3ull * nin + b, kind == KIND_PARTITION ? L.tot[k * kTeamBins + b] : Bc.particles[k]);
         const unsigned long long o0 = 5ull + 3ull * nin + nb;
         for (uint32_t i = lane; i < R; i += 64u) {
             const gf_chunk_desc d = L.out[L.out_off[k] + i];
@@ -715,81 +901,51 @@ __device__ inline void team_trace(const TeamArgs& a, TeamLds& L)
     }
 }
 
-// Retire (wave 4 of every member): outputs to the destination rings in node and output order,
-// consumed inputs back to the free stack (generation + 1), partition inputs' count rows queued for
-// clearing in the next step, counters.
-__device__ inline void team_retire(const TeamArgs& a, TeamLds& L)
+// Retire (wave 3 of every member): outputs to the destination rings in node and output order,
+// consumed inputs back to the free stack (generation + 1), counters.
+__device__ inline void team_retire(const TeamArgs& a, TeamLds& L, StepBuf& Bc, SchedRegs& g)
 {
     const Engine& e = a.e;
     const uint32_t lane = lane_id(), n = e.n_nodes, rcm = a.rc - 1u;
-    Ticker tk(e.prof, L.prof_member && lane == 0u);
-    const bool own = lane < n;
-    const uint32_t k = own ? lane : 0u;
-    const uint32_t kind = own ? L.kind[k] : 0xFFu, act = own ? L.active[k] : 0u;
-    const uint32_t nout = own ? L.n_out[k] : 0u, ooff = own ? L.out_off[k] : 0u, ro = own ? L.route_off[k] : 0u;
-    const uint32_t nin = own ? L.n_in[k] : 0u, inoff = own ? L.in_off[k] : 0u, parts = own ? L.particles[k] : 0u;
-    unsigned long long tail = own ? L.tail[k] : 0ull;
-    const uint32_t total_out = L.total_out, in_tot = L.in_off[n], top = L.top, fb = L.fs_base;
+    Ticker tk(L.prof_acc, L.prof_member && lane == 0u);
+    const uint32_t total_out = L.total_out, in_tot = Bc.in_off[n], top = L.top, fb = L.fs_base;
     for (uint32_t x0 = 0; x0 < total_out; x0 += 64u) {
         const uint32_t x = x0 + lane;
-        const uint32_t kx = node_of(x, ooff, nout, n);
-        const uint32_t kk = from_lane(kind, kx), rox = from_lane(ro, kx);
-        const bool v = x < total_out && kk != KIND_SINK;
+        const bool v = x < total_out;
         gf_chunk_desc d{0, 0, 0, 0};
         uint32_t dst = 0xFFFFFFFFu;
         if (v) {
             d = L.out[x];
-            dst = (uint32_t)L.route[rox + (kk == KIND_PARTITION ? d.bin : 0u)];
+            dst = (uint32_t)L.route[L.out_ridx[x]];
         }
+        const uint32_t ns = (d.fill + kSeg - 1u) >> kSegShift;
         uint64_t pend = __ballot(v);
         unsigned long long pos = 0;
         while (pend) {
-            const int leader = __ffsll((unsigned long long)pend) - 1;
-            const uint32_t lb = rl(dst, (uint32_t)leader);
+            const uint32_t lb = rl(dst, (uint32_t)(__ffsll((unsigned long long)pend) - 1));
             const bool mine = v && dst == lb;
             const uint64_t m = __ballot(mine);
-            const unsigned long long tl = rl64(tail, lb);
+            const unsigned long long tl = rl64(g.tail, lb);
             if (mine) pos = tl + (uint32_t)__popcll(m & lanemask_lt());
-            if (lane == lb) tail += (uint32_t)__popcll(m);
+            const uint32_t fs = wave_sum(mine ? d.fill : 0u), ss = wave_sum(mine ? ns : 0u);
+            if (lane == lb) {
+                g.tail += (uint32_t)__popcll(m);
+                g.rsum += fs;
+                g.rseg += ss;
+            }
             pend &= ~m;
         }
         if (v) L.ring[dst * a.rc + (uint32_t)(pos & rcm)] = d;
     }
     tk(PROF_C_RET_OUT);
-    uint32_t zn = 0;
-    for (uint32_t x0 = 0; x0 < in_tot; x0 += 64u) {
-        const uint32_t x = x0 + lane;
-        const bool v = x < in_tot;
-        const uint32_t kk = from_lane(kind, node_of(x, inoff, nin, n));
-        gf_chunk_desc d{0, 0, 0, 0};
-        if (v) {
-            d = L.in[x];
-            L.fs[top + x - fb] = (unsigned long long)d.index | ((unsigned long long)(d.generation + 1u) << 32);
-        }
-        const bool z = v && kk == KIND_PARTITION;
-        const uint64_t zm = __ballot(z);
-        if (z) L.zero[zn + (uint32_t)__popcll(zm & lanemask_lt())] = d.index | (d.generation & 1u) << 31;
-        zn += (uint32_t)__popcll(zm);
-    }
-    const bool a1 = act != 0u;
-    const uint32_t sink_p = wave_sum(a1 && kind == KIND_SINK ? parts : 0u);
-    const uint32_t src_p = wave_sum(a1 && kind == KIND_SOURCE ? parts : 0u);
-    const uint32_t part_p = wave_sum(a1 && kind == KIND_PARTITION ? parts : 0u);
-    const uint32_t xf_p = wave_sum(a1 && kind == KIND_TRANSFORM ? parts : 0u);
-    const uint32_t nb = (uint32_t)__popcll(__ballot(a1 && (kind == KIND_PARTITION || kind == KIND_TRANSFORM)));
-    if (own) {
-        L.tail[k] = tail;
-        L.active[k] = 0u;
+    for (uint32_t x = lane; x < in_tot; x += 64u) {
+        const gf_chunk_desc d = Bc.in[x];
+        L.fs[top + x - fb] = (unsigned long long)d.index | ((unsigned long long)(d.generation + 1u) << 32);
     }
     if (lane == 0) {
         L.top = top + in_tot;
-        L.zero_n = zn;
-        L.sunk += sink_p;
-        L.sink_items += sink_p;
-        L.src_items += src_p;
-        L.part_items += part_p;
-        L.xf_items += xf_p;
-        L.batches += nb;
+        L.sunk += Bc.sum_sink;
+        L.sink_items += Bc.sum_sink;
         L.step += 1u;
     }
     wave_sync();
@@ -814,12 +970,8 @@ __device__ inline uint32_t next_key(const TeamArgs& a, const TeamLds& L, uint32_
     const gf_node_desc& nd = L.nd[dst];
     uint32_t nb = classify(nd, rec);
     if (nb >= nd.bins) nb = nd.bins;
-    return (((uint32_t)d.index * 2u + (d.generation & 1u)) * a.tpc + (off >> 8)) * a.bb + nb;
+    return (((uint32_t)d.index * 2u + (d.generation & 1u)) * a.tpc + (off >> kSegShift)) * a.bb + nb;
 }
-
-constexpr uint32_t kSegRounds = kSeg / 64u;
-
-__device__ inline uint32_t div_c(uint32_t x, uint32_t C, uint32_t shift) { return shift < 32u ? x >> shift : x / C; }
 
 // Records of one input segment, prefetched right after the barrier by the wave that will scatter it.
 template <class P>
@@ -828,14 +980,43 @@ struct SegRecs {
     uint32_t s;  // segment index, or ~0u
 };
 
-template <class P>
-__device__ inline void load_segment(const TeamArgs& a, const TeamLds& L, uint32_t s, SegRecs<P>& R)
+// Node, input chunk of the node's batch and segment of that chunk holding step segment s
+// (wave-uniform): a ballot over the nodes' segment ranges, then over the node's inputs.
+struct SegLoc {
+    uint32_t k, i, gg;
+};
+
+__device__ inline SegLoc locate(const TeamLds& L, const StepBuf& Bc, uint32_t n, uint32_t s)
 {
-    const uint32_t sm = L.seg_map[s];
-    const uint32_t k = seg_node(sm), i = seg_input(sm), gg = seg_part(sm);
+    const uint32_t lane = lane_id();
+    const uint32_t lo = lane < n ? Bc.seg_off[lane] : 0xFFFFFFFFu, hi = lane < n ? lo + Bc.nseg[lane] : 0u;
+    SegLoc r;
+    r.k = (uint32_t)(__ffsll((unsigned long long)__ballot(lane < n && lo <= s && s < hi)) - 1);
+    const uint32_t so = rl(lo, r.k);
+    r.i = 0u;
+    r.gg = s - so;
+    if (L.kind[r.k] == KIND_SOURCE) return r;
+    const uint32_t io = Bc.in_off[r.k], m = Bc.n_in[r.k];
+    for (uint32_t y0 = 0; y0 < m; y0 += 64u) {
+        const uint32_t f = y0 + lane < m ? Bc.in_seg[io + y0 + lane] : 0xFFFFFFFFu;
+        const uint64_t ym = __ballot(f <= s);
+        if (ym) {
+            const uint32_t hb = 63u - (uint32_t)__clzll((unsigned long long)ym);
+            r.i = y0 + hb;
+            r.gg = s - rl(f, hb);
+        }
+    }
+    return r;
+}
+
+template <class P>
+__device__ inline void load_segment(const TeamArgs& a, const TeamLds& L, const StepBuf& Bc, uint32_t s, SegRecs<P>& R)
+{
+    const SegLoc loc = locate(L, Bc, a.e.n_nodes, s);
+    const uint32_t k = loc.k, i = loc.i, gg = loc.gg;
     R.s = ~0u;
     if (L.kind[k] == KIND_SOURCE) return;
-    const gf_chunk_desc d = L.in[L.in_off[k] + i];
+    const gf_chunk_desc d = Bc.in[Bc.in_off[k] + i];
     const uint32_t off0 = gg * kSeg, cnt = d.fill - off0 < kSeg ? d.fill - off0 : kSeg;
     const size_t slot0 = (size_t)d.index * a.e.C + off0;
     const uint32_t lane = lane_id();
@@ -847,13 +1028,13 @@ __device__ inline void load_segment(const TeamArgs& a, const TeamLds& L, uint32_
     R.s = s;
 }
 
-// Per-bin constants of one partition node, bin b in lane b of the wave, loaded once per launch:
-// the fused transform (node, op, f[0]), the destination after fusion and, when that destination
-// is a partition, its classifier parameters (dop = 0 otherwise).
+// Per-bin constants of one partition node, bin b in lane b of the wave, loaded once per launch
+// and packed for one ds_bpermute per value: w0 = destination after fusion | (fused node + 1) << 8
+// | fused op << 16 | destination classifier op << 24 (0 when the destination is not a
+// partition), w1 = destination bins | destination u[0] << 16, and the two f[0] parameters.
 struct BinTab {
     uint32_t k;  // node described (wave-uniform), or ~0u
-    int32_t fz;
-    uint32_t dst, xop, dop, dbins, du0;
+    uint32_t w0, w1;
     float xf0, df0;
 };
 
@@ -861,111 +1042,92 @@ __device__ inline void load_bintab(const TeamLds& L, uint32_t k, BinTab& t)
 {
     const uint32_t lane = lane_id(), B = L.nd[k].bins, ro = L.route_off[k];
     t.k = k;
-    t.fz = -1;
-    t.dst = 0u;
-    t.xop = 0u;
+    t.w0 = 0u;
+    t.w1 = 1u;
     t.xf0 = 1.0f;
-    t.dop = 0u;
-    t.dbins = 1u;
-    t.du0 = 0u;
     t.df0 = 0.0f;
     if (lane < B) {
-        t.fz = L.fuse[ro + lane];
-        t.dst = (uint32_t)L.route[ro + lane];
-        if (t.fz >= 0) {
-            t.xop = L.nd[t.fz].op;
-            t.xf0 = L.nd[t.fz].f[0];
+        const int32_t fz = L.fuse[ro + lane];
+        const uint32_t dst = (uint32_t)L.route[ro + lane];
+        uint32_t xop = 0, dop = 0;
+        if (fz >= 0) {
+            xop = L.nd[fz].op;
+            t.xf0 = L.nd[fz].f[0];
         }
-        if (L.kind[t.dst] == KIND_PARTITION) {
-            const gf_node_desc& dn = L.nd[t.dst];
-            t.dop = dn.op;
-            t.dbins = dn.bins;
-            t.du0 = dn.u[0];
+        if (L.kind[dst] == KIND_PARTITION) {
+            const gf_node_desc& dn = L.nd[dst];
+            dop = dn.op;
+            t.w1 = dn.bins | (dn.u[0] & 0xFFFFu) << 16;
             t.df0 = dn.f[0];
         }
+        t.w0 = dst | (uint32_t)(fz + 1) << 8 | xop << 16 | dop << 24;
     }
 }
 
-// One lane's bin constants and flat output position.
-struct ItemBin {
-    uint32_t flat, dst, xop, dop, dbins, du0;
-    int32_t fz;
-    float xf0, df0;
-};
-
-// One pass over the distinct bins of a round (the bin id is wave-uniform inside the pass): the
-// rank of every lane inside the segment after `run` (lane b holds bin b's running count), its
-// flat output position from fb (lane b holds bin b's flat start for this segment), and the bin's
-// table entries, all through readlane.
-__device__ inline void seg_group(bool v, uint32_t bin, uint32_t& run, uint32_t fb, const BinTab& t, ItemBin& o)
+// Rank of every valid lane inside the segment after `run` earlier items of its bin (lane b of run
+// holds bin b's running count; bins < 64).
+__device__ inline uint32_t seg_rank(bool v, uint32_t bin, uint32_t& run)
 {
     uint64_t act = __ballot(v);
+    uint32_t pos = 0;
     const uint32_t lane = lane_id();
     while (act) {
         const uint32_t lb = rl(bin, (uint32_t)(__ffsll((unsigned long long)act) - 1));
         const bool mine = v && bin == lb;
         const uint64_t m = __ballot(mine);
-        const uint32_t base = rl(run, lb) + rl(fb, lb);
-        const uint32_t dst = rl(t.dst, lb), xop = rl(t.xop, lb), dop = rl(t.dop, lb), dbins = rl(t.dbins, lb),
-                       du0 = rl(t.du0, lb), fz = rl((uint32_t)t.fz, lb), xf0 = rl(__float_as_uint(t.xf0), lb),
-                       df0 = rl(__float_as_uint(t.df0), lb);
-        if (mine) {
-            o.flat = base + (uint32_t)__popcll(m & lanemask_lt());
-            o.dst = dst;
-            o.xop = xop;
-            o.dop = dop;
-            o.dbins = dbins;
-            o.du0 = du0;
-            o.fz = (int32_t)fz;
-            o.xf0 = __uint_as_float(xf0);
-            o.df0 = __uint_as_float(df0);
-        }
+        const uint32_t base = rl(run, lb);
+        if (mine) pos = base + (uint32_t)__popcll(m & lanemask_lt());
         if (lane == lb) run += (uint32_t)__popcll(m);
         act &= ~m;
     }
+    return pos;
 }
 
 template <class P>
-__device__ void team_partition_seg(const TeamArgs& a, const TeamLds& L, uint32_t k, uint32_t i, uint32_t gg, uint32_t gs,
-                                   const SegRecs<P>& R, BinTab& t)
+__device__ void team_partition_seg(const TeamArgs& a, const TeamLds& L, const StepBuf& Bc, uint32_t k, uint32_t i, uint32_t gg,
+                                   uint32_t gs, const SegRecs<P>& R, BinTab& t)
 {
     const Engine& e = a.e;
     const uint32_t C = e.C, lane = lane_id();
     if (t.k != k) load_bintab(L, k, t);
     const gf_node_desc& nd = L.nd[k];
     const uint32_t B = nd.bins;
-    const gf_chunk_desc d = L.in[L.in_off[k] + i];
+    const gf_chunk_desc d = Bc.in[Bc.in_off[k] + i];
     const uint32_t off0 = gg * kSeg, cnt = d.fill - off0 < kSeg ? d.fill - off0 : kSeg;
-    const uint32_t* pre = L.pre + L.pre_off[k] + gs * (B + 1u);
-    const uint32_t fb = lane < B ? L.binbase[k * kTeamBins + lane] * C + pre[lane] : 0u;
+    const uint32_t* pre = L.pre + Bc.pre_off[k] + gs * (B + 1u);
+    const uint32_t fb = lane < B ? L.binbase[k * kTeamBins + lane] * C + pre[lane] : 0u;  // flat start of bin `lane`
     const gf_chunk_desc* ob = L.out + L.out_off[k];
     const uint32_t nout = L.n_out[k];
-    Ticker tk(e.prof, L.prof_member && threadIdx.x == 0u);
+    Ticker tk(const_cast<unsigned long long*>(L.prof_acc), L.prof_member && threadIdx.x == 0u);
     bool v[kSegRounds];
-    uint32_t bin[kSegRounds], key[kSegRounds];
-    ItemBin ib[kSegRounds];
+    uint32_t bin[kSegRounds], pos[kSegRounds], key[kSegRounds];
 #pragma unroll
     for (uint32_t r = 0; r < kSegRounds; ++r) {
         v[r] = r * 64u + lane < cnt;
-        bin[r] = ~0u;
+        bin[r] = 0u;
         if (v[r]) {
             bin[r] = classify(nd, R.rec[r]);
             if (bin[r] >= B) {
                 seg_fail(a, L, GF_INVALID);
                 v[r] = false;
-                bin[r] = ~0u;
+                bin[r] = 0u;
             }
         }
     }
     uint32_t run = 0;
 #pragma unroll
-    for (uint32_t r = 0; r < kSegRounds; ++r) seg_group(v[r], bin[r], run, fb, t, ib[r]);
+    for (uint32_t r = 0; r < kSegRounds; ++r) pos[r] = seg_rank(v[r], bin[r], run);
     tk(PROF_C_SEG_RANK);
 #pragma unroll
     for (uint32_t r = 0; r < kSegRounds; ++r) {
+        // the bin's constants for every lane at once
+        const uint32_t flat = from_lane(fb, bin[r]) + pos[r];
+        const uint32_t w0 = from_lane(t.w0, bin[r]), w1 = from_lane(t.w1, bin[r]);
+        const float xf0 = __uint_as_float(from_lane(__float_as_uint(t.xf0), bin[r]));
+        const float df0 = __uint_as_float(from_lane(__float_as_uint(t.df0), bin[r]));
         key[r] = ~0u;
         if (!v[r]) continue;
-        const uint32_t q = div_c(ib[r].flat, C, a.c_shift), off = ib[r].flat - q * C;
+        const uint32_t q = div_c(flat, C, a.c_shift), off = flat - q * C;
         if (q >= nout) {
             seg_fail(a, L, GF_PLAN_MISMATCH);
             continue;
@@ -974,13 +1136,16 @@ __device__ void team_partition_seg(const TeamArgs& a, const TeamLds& L, uint32_t
         P p = R.rec[r];
         const uint32_t u = update(nd, bin[r], p);
         if (u) seg_fail(a, L, u);
-        if (!transform_p(ib[r].xop, ib[r].xf0, p) && ib[r].fz >= 0) transform(L.nd[ib[r].fz], e.rp, 2u * (uint32_t)ib[r].fz, p);
+        const int32_t fz = (int32_t)((w0 >> 8) & 0xFFu) - 1;
+        if (!transform_p((w0 >> 16) & 0xFFu, xf0, p) && fz >= 0) transform(L.nd[fz], e.rp, 2u * (uint32_t)fz, p);
         p.store(e.cols, (size_t)od.index * C + off);
-        if (ib[r].dop) {
-            uint32_t nb = classify_p(ib[r].dop, ib[r].dbins, ib[r].du0, ib[r].df0, p);
-            if (nb == kNeedDesc) nb = classify(L.nd[ib[r].dst], p);
-            if (nb >= ib[r].dbins) nb = ib[r].dbins;
-            key[r] = (((uint32_t)od.index * 2u + (od.generation & 1u)) * a.tpc + (off >> 8)) * a.bb + nb;
+        const uint32_t dop = w0 >> 24;
+        if (dop) {
+            const uint32_t dbins = w1 & 0xFFFFu;
+            uint32_t nb = classify_p(dop, dbins, w1 >> 16, df0, p);
+            if (nb == kNeedDesc) nb = classify(L.nd[w0 & 0xFFu], p);
+            if (nb >= dbins) nb = dbins;
+            key[r] = (((uint32_t)od.index * 2u + (od.generation & 1u)) * a.tpc + (off >> kSegShift)) * a.bb + nb;
         }
     }
     tk(PROF_C_SEG_STORE);
@@ -990,15 +1155,15 @@ __device__ void team_partition_seg(const TeamArgs& a, const TeamLds& L, uint32_t
 }
 
 template <class P>
-__device__ void team_transform_seg(const TeamArgs& a, const TeamLds& L, uint32_t k, uint32_t i, uint32_t gg,
+__device__ void team_transform_seg(const TeamArgs& a, const TeamLds& L, const StepBuf& Bc, uint32_t k, uint32_t i, uint32_t gg,
                                    const SegRecs<P>& R)
 {
     const Engine& e = a.e;
     const uint32_t C = e.C, lane = lane_id();
     const gf_node_desc& nd = L.nd[k];
-    const gf_chunk_desc d = L.in[L.in_off[k] + i];
+    const gf_chunk_desc d = Bc.in[Bc.in_off[k] + i];
     const uint32_t off0 = gg * kSeg, cnt = d.fill - off0 < kSeg ? d.fill - off0 : kSeg;
-    const uint32_t pos0 = L.in_items[L.in_off[k] + i] + off0;
+    const uint32_t pos0 = Bc.in_items[Bc.in_off[k] + i] + off0;
     const gf_chunk_desc* ob = L.out + L.out_off[k];
     const uint32_t nout = L.n_out[k], dst = (uint32_t)L.route[L.route_off[k]];
 #pragma unroll
@@ -1022,15 +1187,15 @@ __device__ void team_transform_seg(const TeamArgs& a, const TeamLds& L, uint32_t
 }
 
 template <class P>
-__device__ void team_source_seg(const TeamArgs& a, const TeamLds& L, uint32_t k, uint32_t gg)
+__device__ void team_source_seg(const TeamArgs& a, const TeamLds& L, const StepBuf& Bc, uint32_t k, uint32_t gg)
 {
     const Engine& e = a.e;
     const uint32_t C = e.C, lane = lane_id();
     const gf_node_desc& nd = L.nd[k];
     const gf_chunk_desc od = L.out[L.out_off[k]];
-    const uint32_t fill = L.particles[k], off0 = gg * kSeg, cnt = fill - off0 < kSeg ? fill - off0 : kSeg;
+    const uint32_t fill = Bc.particles[k], off0 = gg * kSeg, cnt = fill - off0 < kSeg ? fill - off0 : kSeg;
     const uint32_t dst = (uint32_t)L.route[L.route_off[k]];
-    const unsigned long long id0 = L.first_id[k] + off0;
+    const unsigned long long id0 = Bc.first_id[k] + off0;
 #pragma unroll
     for (uint32_t r = 0; r < kSegRounds; ++r) {
         const uint32_t it = r * 64u + lane;
@@ -1046,11 +1211,12 @@ __device__ void team_source_seg(const TeamArgs& a, const TeamLds& L, uint32_t k,
 }
 
 template <class P>
-__device__ void team_sink_seg(const TeamArgs& a, const TeamLds& L, uint32_t k, uint32_t i, uint32_t gg, const SegRecs<P>& R)
+__device__ void team_sink_seg(const TeamArgs& a, const TeamLds& L, const StepBuf& Bc, uint32_t k, uint32_t i, uint32_t gg,
+                              const SegRecs<P>& R)
 {
     const uint32_t lane = lane_id();
     const gf_node_desc& nd = L.nd[k];
-    const gf_chunk_desc d = L.in[L.in_off[k] + i];
+    const gf_chunk_desc d = Bc.in[Bc.in_off[k] + i];
     const uint32_t off0 = gg * kSeg, cnt = d.fill - off0 < kSeg ? d.fill - off0 : kSeg;
     if constexpr (std::is_same_v<P, Particle>) {
 #pragma unroll
@@ -1083,7 +1249,7 @@ __device__ void team_sink_seg(const TeamArgs& a, const TeamLds& L, uint32_t k, u
             msum += __shfl_down(msum, o);
         }
         if (lane == 0) {
-            const uint64_t entry = L.digest_base[k] + i;
+            const uint64_t entry = Bc.digest_base[k] + i;
             if (nd.ptr[0] && entry < nd.size[0]) {
                 uint32_t* log = (uint32_t*)nd.ptr[0] + 4 * entry;
                 if (gg == 0) {
@@ -1101,41 +1267,62 @@ __device__ void team_sink_seg(const TeamArgs& a, const TeamLds& L, uint32_t k, u
     }
 }
 
-// Segments [lo, hi) of this member, wave w takes lo + w, lo + w + 4, ...
-__device__ inline void member_range(const TeamLds& L, uint32_t n, uint32_t me, uint32_t T, uint32_t* lo, uint32_t* hi)
+// Segments [lo, hi) of this member: worker waves take lo + o, lo + o + kWorkers, ... in the order
+// o = 0 for wave 1, 1 for wave 2, 2 for wave 0 (wave 0 computes the plan first, so it scatters only
+// when the member has more than two segments).
+__device__ inline void member_range(const StepBuf& Bc, uint32_t n, uint32_t me, uint32_t T, uint32_t* lo, uint32_t* hi)
 {
-    const uint32_t G = L.seg_off[n];
+    (void)n;
+    const uint32_t G = Bc.g_main;
     *lo = (uint32_t)((uint64_t)me * G / T);
     *hi = (uint32_t)((uint64_t)(me + 1u) * G / T);
 }
 
+// Sink segments of this member, taken by its sink wave. The sink wave does not wait for its
+// atomics and stores before the barrier (nothing in the run reads them); its loads complete before
+// it arrives because every sink op consumes the loaded record.
 template <class P>
-__device__ inline void team_prefetch(const TeamArgs& a, const TeamLds& L, uint32_t me, uint32_t T, uint32_t w, SegRecs<P>& R)
+__device__ inline void team_sinks(const TeamArgs& a, const TeamLds& L, const StepBuf& Bc, uint32_t me, uint32_t T)
 {
-    uint32_t lo, hi;
-    member_range(L, a.e.n_nodes, me, T, &lo, &hi);
-    R.s = ~0u;
-    if (lo + w < hi) load_segment<P>(a, L, lo + w, R);
+    const uint32_t G0 = Bc.g_main, Gs = Bc.seg_off[a.e.n_nodes] - G0;
+    const uint32_t lo = G0 + (uint32_t)((uint64_t)me * Gs / T), hi = G0 + (uint32_t)((uint64_t)(me + 1u) * Gs / T);
+    SegRecs<P> R;
+    for (uint32_t s = lo; s < hi; ++s) {
+        const SegLoc loc = locate(L, Bc, a.e.n_nodes, s);
+        load_segment<P>(a, L, Bc, s, R);
+        team_sink_seg<P>(a, L, Bc, loc.k, loc.i, loc.gg, R);
+    }
 }
 
 template <class P>
-__device__ inline void team_segments(const TeamArgs& a, const TeamLds& L, uint32_t me, uint32_t T, uint32_t w, SegRecs<P>& R,
-                                     BinTab& t)
+__device__ inline void team_prefetch(const TeamArgs& a, const TeamLds& L, const StepBuf& Bc, uint32_t me, uint32_t T, uint32_t w,
+                                     SegRecs<P>& R)
 {
     uint32_t lo, hi;
-    member_range(L, a.e.n_nodes, me, T, &lo, &hi);
-    for (uint32_t s = lo + w; s < hi; s += 4u) {
-        const uint32_t sm = L.seg_map[s];
-        const uint32_t k = seg_node(sm), i = seg_input(sm), gg = seg_part(sm);
+    member_range(Bc, a.e.n_nodes, me, T, &lo, &hi);
+    const uint32_t o = (w + kWorkers - 1u) % kWorkers;
+    R.s = ~0u;
+    if (lo + o < hi) load_segment<P>(a, L, Bc, lo + o, R);
+}
+
+template <class P>
+__device__ inline void team_segments(const TeamArgs& a, const TeamLds& L, const StepBuf& Bc, uint32_t me, uint32_t T, uint32_t w,
+                                     SegRecs<P>& R, BinTab& t)
+{
+    uint32_t lo, hi;
+    member_range(Bc, a.e.n_nodes, me, T, &lo, &hi);
+    const uint32_t o = (w + kWorkers - 1u) % kWorkers;
+    for (uint32_t s = lo + o; s < hi; s += kWorkers) {
+        const SegLoc loc = locate(L, Bc, a.e.n_nodes, s);
+        const uint32_t k = loc.k, i = loc.i, gg = loc.gg;
         const uint32_t kind = L.kind[k];
         if (kind == KIND_SOURCE) {
-            team_source_seg<P>(a, L, k, gg);
+            team_source_seg<P>(a, L, Bc, k, gg);
             continue;
         }
-        if (R.s != s) load_segment<P>(a, L, s, R);
-        if (kind == KIND_PARTITION) team_partition_seg<P>(a, L, k, i, gg, s - L.seg_off[k], R, t);
-        else if (kind == KIND_TRANSFORM) team_transform_seg<P>(a, L, k, i, gg, R);
-        else team_sink_seg<P>(a, L, k, i, gg, R);
+        if (R.s != s) load_segment<P>(a, L, Bc, s, R);
+        if (kind == KIND_PARTITION) team_partition_seg<P>(a, L, Bc, k, i, gg, s - Bc.seg_off[k], R, t);
+        else team_transform_seg<P>(a, L, Bc, k, i, gg, R);
     }
 }
 
@@ -1152,56 +1339,91 @@ __global__ __launch_bounds__(kTeamThreads) void k_team(TeamArgs a)
     const Engine& e = a.e;
     const uint32_t tid = threadIdx.x, w = tid >> 6;
     const bool prof = e.prof != nullptr && me == 0u && tid == 0u;
+    unsigned long long* mp = a.mprof ? a.mprof + (size_t)me * kMemberProf : nullptr;
+    unsigned long long mt = mp ? wall_clock64() : 0ull;
+    auto mmark = [&](uint32_t slot) {
+        if (mp && tid == 0u) {
+            const unsigned long long now = wall_clock64();
+            atomicAdd(&mp[slot], now - mt);
+            mt = now;
+        }
+    };
     const bool wprof = e.prof != nullptr && me == 0u && lane_id() == 0u;
     unsigned long long tc = prof ? wall_clock64() : 0ull;
     auto mark = [&](uint32_t ph) {
         if (prof) {
             const unsigned long long now = wall_clock64();
-            e.prof[ph] += now - tc;
+            atomicAdd(&L.prof_acc[ph], now - tc);
             tc = now;
         }
     };
     if (tid == 0u) L.prof_member = (e.prof != nullptr && me == 0u) ? 1u : 0u;
+    if (tid < PROF_WORDS) L.prof_acc[tid] = 0ull;
     team_load(a, L);
     if (L.exit_code == EXIT_SKIP) {
         if (me == 0u && tid == 0u) e.ctl[CTL_HANDOFF] = 1u;
         return;
     }
-    if (w == 4u) team_select(a, L, a.max_steps == 0u);
+    SchedRegs sg;
+    if (w == kSchedWave) {
+        sched_load(a, L, sg);
+        team_select(a, L, L.sb[0], a.max_steps == 0u, sg);
+    }
     __syncthreads();
     uint32_t target = 0;
     unsigned long long cyc = 0;
     SegRecs<P> R;
     BinTab tab;
     tab.k = ~0u;
-    for (uint32_t it = 0; L.exit_code == EXIT_NONE; ++it) {
+    int32_t last = -1;  // last executed step of this launch
+    for (uint32_t it = 0;; ++it) {
+        StepBuf& Bc = L.sb[it & 1u];
+        StepBuf& Bn = L.sb[(it + 1u) & 1u];  // the previous step's batches until select overwrites them
         if (it) {
+            // the previous step's records of every member are in memory
             if (tid == 0u && !team_wait(a, target)) {
                 set_status(&e.ctl[CTL_STATUS], GF_DEVICE_ERROR);
                 L.status = GF_DEVICE_ERROR;
             }
             __syncthreads();
             mark(PROF_T_WAIT);
+            mmark(MP_WAIT);
             if (L.status) break;
         }
-        if (w < 4u) team_prefetch<P>(a, L, me, T, w, R);
-        team_counts(a, L, me, T, it);
-        if (L.status) break;
+        if (L.exit_code != EXIT_NONE) break;
+        R.s = ~0u;
+        if (w >= 1u && w < kWorkers) team_prefetch<P>(a, L, Bc, me, T, w, R);  // wave 0 starts on the counts
         mark(PROF_T_COUNTS);
-        team_plan(a, L);
-        if (L.status) break;
+        team_countplan(a, L, Bc, it ? &Bn : nullptr, me, T, it);
         mark(PROF_T_PLAN);
+        mmark(MP_COUNTPLAN);
+        if (L.err_prev || L.status) break;
         const unsigned long long w0 = wprof ? wall_clock64() : 0ull;
-        if (w < 4u) {
-            team_segments<P>(a, L, me, T, w, R, tab);
-        } else {
-            if (me == 0u && e.trace_on) team_trace(a, L);
-            team_retire(a, L);
+        const unsigned long long w0m = mp ? wall_clock64() : 0ull;
+        if (mp && tid == 0u) {
+            uint32_t lo, hi;
+            member_range(Bc, e.n_nodes, me, T, &lo, &hi);
+            const uint32_t Gs = Bc.seg_off[e.n_nodes] - Bc.g_main;
+            atomicAdd(&mp[MP_NSEG], (unsigned long long)(hi - lo));
+            atomicAdd(&mp[MP_NSINK], (unsigned long long)((uint64_t)(me + 1u) * Gs / T - (uint64_t)me * Gs / T));
         }
-        if (wprof && (w == 0u || w == 4u)) e.prof[w == 0u ? PROF_T_SEGW : PROF_T_RETIRE] += wall_clock64() - w0;
-        __builtin_amdgcn_s_waitcnt(0);
+        if (w < kWorkers) {
+            team_segments<P>(a, L, Bc, me, T, w, R, tab);
+            if (wprof && w == 0u) atomicAdd(&L.prof_acc[PROF_T_SEGW], wall_clock64() - w0);
+        } else if (w == kSinkWave) {
+            team_sinks<P>(a, L, Bc, me, T);
+        } else {
+            // the scheduler wave retires this step and selects the next one meanwhile
+            if (me == 0u && e.trace_on) team_trace(a, L, Bc);
+            team_retire(a, L, Bc, sg);
+            const unsigned long long w1 = wprof ? wall_clock64() : 0ull;
+            if (wprof) atomicAdd(&L.prof_acc[PROF_T_RETIRE], w1 - w0);
+            team_select(a, L, Bn, it + 1u >= a.max_steps, sg);
+            if (wprof) atomicAdd(&L.prof_acc[PROF_T_SELECTW], wall_clock64() - w1);
+            if (mp && lane_id() == 0u) atomicAdd(&mp[MP_SCHED], wall_clock64() - w0m);
+        }
+        if (w != kSinkWave) __builtin_amdgcn_s_waitcnt(0);
         __syncthreads();
-        mark(PROF_T_SEG);
         target += T;
         if (tid == 0u) {
             if (L.seg_err) {
@@ -1210,21 +1432,24 @@ __global__ __launch_bounds__(kTeamThreads) void k_team(TeamArgs a)
             }
             __hip_atomic_fetch_add(&a.tw[TW_BAR], 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
         }
-        if (w == 4u) {
-            const unsigned long long s0 = wprof ? wall_clock64() : 0ull;
-            team_select(a, L, it + 1u >= a.max_steps);
-            if (wprof) e.prof[PROF_T_SELECTW] += wall_clock64() - s0;
-        }
-        __syncthreads();
+        mark(PROF_T_SEG);
+        mmark(MP_SEG);
         mark(PROF_T_SELECT);
-        if (prof) e.prof[PROF_T_STEPS] += 1u;
+        last = (int32_t)it;
         if (prof) {
+            L.prof_acc[PROF_T_STEPS] += 1u;
             const unsigned long long c = __builtin_readcyclecounter();
-            if (it) e.prof[PROF_C_STEP] += c - cyc;
+            if (it) L.prof_acc[PROF_C_STEP] += c - cyc;
             cyc = c;
         }
     }
-    if (me == 0u && !L.status) team_flush(a, L);
+    if (w == kSchedWave) sched_store(a, L, sg);
+    __syncthreads();
+    if (me == 0u && !L.status && !L.err_prev) team_flush(a, L, last >= 0 ? &L.sb[last & 1] : nullptr);
+    if (e.prof != nullptr && me == 0u) {
+        __syncthreads();
+        if (tid < PROF_WORDS) e.prof[tid] += L.prof_acc[tid];
+    }
 }
 
 }  // namespace gfd

@@ -108,7 +108,7 @@ typedef struct gf_tuning {
     uint32_t fuse_transforms;/* device schedule: apply a plain transform fed by a partition bin inside
                                 that partition's scatter and route its items to the transform's destination */
     uint32_t max_workgroups; /* device schedule: workgroups per step kernel (0 -> 64); team size of
-                                the team engine (0 -> 16, at most 32) */
+                                the team engine (0 -> the CUs of one XCD, at most 32) */
     uint32_t device_engine;  /* device schedule: 0 autotuned, 1 wide (multi-workgroup step kernels in a
                                 hipGraph), 2 micro (one persistent 1024-thread workgroup runs the steps),
                                 3 team (one persistent launch, a team of workgroups on one XCD with a
