@@ -54,6 +54,66 @@ def run_once(flow, src, counts, N, torch):
     return time.perf_counter() - t0, st
 
 
+TRAFFIC_FILE = os.path.join(os.path.dirname(os.path.abspath(__file__)), "profiles", "k_team_feedback_N1e8_hbm_traffic.txt")
+
+
+def pmc_summary(fetch_csv, write_csv, workload, out):
+    """Turn two rocprofv3 --pmc runs (FETCH_SIZE, then WRITE_SIZE; counter_collection.csv, KiB per
+    dispatch) of `bench.py --copy-calib 1024` into HBM bytes per k_team launch. Each counter is
+    scaled by what it reads for the calibration copy of known size (the last device copy of the
+    run: 1 GiB read and 1 GiB written)."""
+    import csv
+
+    def per_dispatch(path, counter):
+        d = {}
+        for r in csv.DictReader(open(path)):
+            if r["Counter_Name"] != counter:
+                continue
+            k = (int(r["Dispatch_Id"]), r["Kernel_Name"])
+            d[k] = d.get(k, 0.0) + float(r["Counter_Value"])
+        return d
+
+    lines = ["# HBM traffic of the team engine kernel (k_team), from two rocprofv3 --pmc passes:",
+             "#   cd /tmp && rocprofv3 --pmc FETCH_SIZE --output-format csv -d $R/gpurun_out/pmc_fetch -o run -- "
+             "python $R/bench.py --no-cpu --no-plain --no-roofline --steps 1 --warmup 0 --items 1e8 --copy-calib 1024",
+             "#   (the same with --pmc WRITE_SIZE into $R/gpurun_out/pmc_write; R = the repository root)",
+             "#   python bench.py --pmc-summary gpurun_out/pmc_fetch/run_counter_collection.csv "
+             "gpurun_out/pmc_write/run_counter_collection.csv <this file>",
+             f"workload={workload}"]
+    per = {}
+    for counter, path in (("FETCH_SIZE", fetch_csv), ("WRITE_SIZE", write_csv)):
+        d = per_dispatch(path, counter)
+        copies = [(k, v) for k, v in d.items() if "copyBuffer" in k[1]]
+        cal_k, cal_v = max(copies, key=lambda kv: kv[0][0])
+        factor = float(1 << 20) / cal_v  # KiB the 1 GiB copy moved / KiB the counter read
+        team = [v for k, v in sorted(d.items()) if "k_team" in k[1]]
+        per[counter] = [v * factor * 1024.0 for v in team]
+        lines.append(f"{counter}: calibration copy dispatch {cal_k[0]} read {cal_v:.1f} KiB for 1048576 KiB "
+                     f"-> factor {factor:.4f}; k_team dispatches (KiB): {', '.join(f'{v:.1f}' for v in team)}")
+    launches = len(per["FETCH_SIZE"])
+    total = sum(per["FETCH_SIZE"]) + sum(per["WRITE_SIZE"])
+    lines.append(f"launches={launches}")
+    lines.append(f"fetch_bytes_total={sum(per['FETCH_SIZE']):.0f}")
+    lines.append(f"write_bytes_total={sum(per['WRITE_SIZE']):.0f}")
+    lines.append(f"traffic_bytes_per_launch={total / max(launches, 1):.0f}")
+    with open(out, "w") as f:
+        f.write("\n".join(lines) + "\n")
+    print("\n".join(lines))
+
+
+def traffic_for(workload, launches):
+    """HBM bytes per k_team launch from the committed PMC summary of this workload, or None."""
+    import re
+
+    try:
+        kv = dict(m.groups() for m in (re.match(r"^([a-z_]+)=(.*)$", l.strip()) for l in open(TRAFFIC_FILE)) if m)
+    except OSError:
+        return None
+    if kv.get("workload") != workload or int(kv.get("launches", -1)) != launches:
+        return None
+    return float(kv["traffic_bytes_per_launch"])
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -71,9 +131,22 @@ def main():
     ap.add_argument("--verify", action="store_true", help="compare the sorted histogram with the oracle")
     ap.add_argument("--engine", type=int, default=0, help="device engine: 0 autotuned, 1 wide, 2 micro, 3 team")
     ap.add_argument("--team", type=int, default=0, help="team engine: workgroups in the team (0 = default)")
-    ap.add_argument("--members", action="store_true", help="team engine: print per-member phase times")
+    ap.add_argument("--phases", action="store_true", help="device engines: one more run with the per-phase profile")
+    ap.add_argument("--members", action="store_true", help="team engine: with --phases, per-member phase times")
     ap.add_argument("--no-fuse", action="store_true", help="run A/B as separate transform batches")
+    ap.add_argument("--copy-calib", type=float, default=0.0,
+                    help="after the runs, copy this many MiB device to device once (the known-size kernel that "
+                         "scales rocprofv3 FETCH_SIZE / WRITE_SIZE readings to bytes)")
+    ap.add_argument("--pmc-summary", nargs=3, metavar=("FETCH_CSV", "WRITE_CSV", "OUT"),
+                    help="write the HBM traffic summary of two rocprofv3 --pmc runs and exit")
     args = ap.parse_args()
+
+    N, C, P, Q = int(args.items), args.capacity, args.live, args.batch
+    fuse = not args.no_fuse
+    wkey = f"particle_feedback N={N:.0e} C={C} P={P} Q={Q} seed=42 fuse={int(fuse)} team={args.team or 'xcd'}"
+    if args.pmc_summary:
+        pmc_summary(*args.pmc_summary[:2], wkey, args.pmc_summary[2])
+        return
 
     if args.gpus != 1:
         print(json.dumps({"error": "this build runs on one GPU; --gpus above 1 is not supported "
@@ -83,9 +156,7 @@ def main():
     import torch
     import graphflow_amd as g
 
-    N, C, P, Q = int(args.items), args.capacity, args.live, args.batch
     log(f"[bench] batched form N={N} C={C} P={P} Q={Q}")
-    fuse = not args.no_fuse
     flow, src, counts = build_feedback(g, torch, N, C, P, Q, g.SCHEDULE_DEVICE, fuse=fuse, engine=args.engine,
                                        team=args.team)
     for i in range(args.warmup):
@@ -137,16 +208,14 @@ def main():
         # algorithmic bytes: 24 B per partition or transform item pass (one 12 B read and one 12 B
         # write of the payload), 12 B per generated particle (write) and per sunk particle (read)
         alg_bytes = 24 * part_items + 24 * xf_items + 12 * N + 12 * N
+        traffic = None
         if n[3] and int(tst["engine"]) == 3:  # team engine: one persistent launch runs many steps
             k_ms, k_n, kname = ms[3], n[3], "k_team (whole steps, team of workgroups on one XCD)"
-            phases, psteps = flow.phase_profile()
-            others = {"steps_per_launch": int(tst["steps"]) / max(n[3], 1), "avg_step_us": ms[3] * 1e3 / max(int(tst["steps"]), 1),
-                      "phase_us_per_step": {k: round(v, 3) for k, v in phases.items()}}
-            log(f"[bench] team phases (us/step over {psteps} steps): {others['phase_us_per_step']}")
-            if args.members:
-                for m, r in enumerate(flow.member_profile(args.team or 32)):
-                    log(f"[bench] member {m:2d}: wait {r[0]:.2f} countplan {r[1]:.2f} scatter {r[2]:.2f} "
-                        f"sched {r[3]:.2f} us, segments {r[4]:.2f} sinks {r[5]:.2f} per step")
+            others = {"steps_per_launch": int(tst["steps"]) / max(n[3], 1),
+                      "avg_step_us": ms[3] * 1e3 / max(int(tst["steps"]), 1)}
+            traffic = traffic_for(wkey, int(k_n))
+            if traffic is not None:
+                others["traffic_source"] = os.path.relpath(TRAFFIC_FILE, os.path.dirname(os.path.abspath(__file__)))
         elif n[3]:  # micro engine: one persistent kernel does every phase
             k_ms, k_n, kname = ms[3], n[3], "k_micro (whole steps in one workgroup)"
             others = {}
@@ -156,12 +225,26 @@ def main():
         achieved = alg_bytes / (k_ms * 1e-3) / 1e9 if k_ms > 0 else 0.0
         result["roofline"] = {
             "bound": "hbm", "kernel": kname, "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-            "frac": achieved / HBM_PEAK_GBS, "traffic": None,
+            "frac": achieved / HBM_PEAK_GBS, "traffic": traffic,
             "algorithmic_bytes_per_launch": alg_bytes / max(k_n, 1), "avg_launch_us": k_ms * 1e3 / max(k_n, 1),
             "launches": k_n, **others,
         }
         log(f"[bench] roofline: {kname} avg {k_ms * 1e3 / max(k_n, 1):.2f} us over {k_n} launches, "
-            f"achieved {achieved:.1f} GB/s ({achieved / HBM_PEAK_GBS:.2e} of peak)")
+            f"achieved {achieved:.1f} GB/s ({achieved / HBM_PEAK_GBS:.2e} of peak), traffic/launch {traffic}")
+
+    if args.phases:
+        # per-phase device profile (slows the profiled member; not used for any reported rate)
+        flow.configure(g.Tuning(seed=42, batch_chunks=Q, chunk_capacity=C, live_particles=P,
+                                schedule=g.SCHEDULE_DEVICE, use_graph=3, fuse_transforms=fuse,
+                                device_engine=args.engine, max_workgroups=args.team or None))
+        counts.zero_()
+        pst = flow.run(src, N)
+        phases, psteps = flow.phase_profile()
+        log(f"[bench] phases (us/step over {psteps} steps): { {k: round(v, 3) for k, v in phases.items()} }")
+        if args.members and int(pst["engine"]) == 3:
+            for m, r in enumerate(flow.member_profile(args.team or 32)):
+                log(f"[bench] member {m:2d}: wait {r[0]:.2f} countplan {r[1]:.2f} scatter {r[2]:.2f} "
+                    f"sched {r[3]:.2f} us, segments {r[4]:.2f} sinks {r[5]:.2f} per step")
 
     forms = {}
     if not args.no_plain:
@@ -186,6 +269,15 @@ def main():
                       f"C={C} P={P} Q={Q} seed 42, single thread, status={r['status']}",
         }
         log(f"[bench] cpu baseline N={Nc}: {Nc / dt:.4g} particles/s ({dt:.1f} s)")
+
+    if args.copy_calib > 0:
+        nbytes = int(args.copy_calib * (1 << 20)) // 4 * 4
+        a = torch.ones(nbytes // 4, dtype=torch.float32, device="cuda")
+        b = torch.empty_like(a)
+        torch.cuda.synchronize()
+        b.copy_(a)
+        torch.cuda.synchronize()
+        log(f"[bench] copy calibration: {nbytes} bytes read and {nbytes} bytes written by one device copy")
 
     print(json.dumps(result), flush=True)
 

@@ -559,7 +559,7 @@ class Flow:
 
     def phase_profile(self):
         """Per-phase wall-clock microseconds per step of the micro or team engine (run with
-        use_graph=2; the device wall clock ticks at 100 MHz)."""
+        use_graph=3; the device wall clock ticks at 100 MHz)."""
         a = (C.c_uint64 * 32)()
         _raise(lib().gf_flow_phase_profile(self.handle, a, 32))
         if int(a[14]):
@@ -579,7 +579,8 @@ class Flow:
 
     def member_profile(self, members=32):
         """Team engine, timed run: per member [barrier wait, counts+plan, scatter phase, scheduler
-        wave] in microseconds per step, and segments scattered / sink segments per step."""
+        wave] in microseconds per step, and segments scattered / sink segments per step (run with
+        use_graph=3)."""
         words = 32 + 64 * 8
         a = (C.c_uint64 * words)()
         _raise(lib().gf_flow_phase_profile(self.handle, a, words))

@@ -1339,7 +1339,7 @@ void build_engine(gf_flow& f, uint64_t N, const RunShape& sh, uint32_t src)
     e.trace_on = t.trace ? 1u : 0u;
     e.trace_cap = t.trace ? (unsigned long long)1 << 26 : 0ull;
     e.trace = t.trace ? m.d<uint32_t>(e.trace_cap) : nullptr;
-    e.prof = t.use_graph == 2u ? m.d<unsigned long long>(PROF_WORDS) : nullptr;
+    e.prof = t.use_graph == 3u ? m.d<unsigned long long>(PROF_WORDS) : nullptr;
     const uint32_t gmax = t.max_workgroups ? t.max_workgroups : 64u;
     eb->grid = std::min<uint32_t>(std::max<uint32_t>(grid_tiles, 1u), gmax);
     eb->r = r;
@@ -1381,7 +1381,7 @@ void build_engine(gf_flow& f, uint64_t N, const RunShape& sh, uint32_t src)
         while (rc * 2u <= kTeamRing / n) rc *= 2u;
         ta.rc = std::min<uint32_t>(rc, 1024u);
         ta.n_routes = (uint32_t)routes.size();
-        ta.mprof = t.use_graph == 2u ? m.d<unsigned long long>(64u * kMemberProf) : nullptr;
+        ta.mprof = t.use_graph == 3u ? m.d<unsigned long long>(64u * kMemberProf) : nullptr;
         ta.c_shift = 32u;
         if ((sh.C & (sh.C - 1u)) == 0u)
             for (uint32_t b = 0; b < 32u; ++b)
@@ -1986,7 +1986,7 @@ gf_status gf_flow_run(gf_flow* flow, uint32_t source, uint64_t items, gf_stats* 
         if (items == 0) {
             // nothing to admit
         } else if (flow->tuning.schedule == GF_SCHEDULE_DEVICE) {
-            run_device(*flow, source, items, sh, st, flow->tuning.use_graph == 2u);
+            run_device(*flow, source, items, sh, st, flow->tuning.use_graph >= 2u);
         } else {
             run_host(*flow, source, items, sh, st);
         }

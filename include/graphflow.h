@@ -104,7 +104,8 @@ typedef struct gf_tuning {
     uint32_t schedule;       /* GF_SCHEDULE_*                           */
     uint32_t steps_per_launch; /* device engine steps per graph launch (0 -> 64) */
     uint32_t trace;          /* record a batch trace                    */
-    uint32_t use_graph;      /* 0 direct launches, 1 hipGraph of K steps, 2 direct launches timed with HIP events */
+    uint32_t use_graph;      /* 0 direct launches, 1 hipGraph of K steps, 2 direct launches timed with HIP events,
+                                3 as 2 plus the device engines' per-phase wall-clock profile */
     uint32_t fuse_transforms;/* device schedule: apply a plain transform fed by a partition bin inside
                                 that partition's scatter and route its items to the transform's destination */
     uint32_t max_workgroups; /* device schedule: workgroups per step kernel (0 -> 64); team size of
@@ -187,8 +188,11 @@ gf_status gf_flow_last_error(gf_flow* flow, char* buffer, uint32_t size);
 gf_status gf_flow_trace(gf_flow* flow, uint32_t* host, uint64_t cap, uint64_t* len);
 /* kernel timing of the last device-engine run: per kernel kind total ms and launches */
 gf_status gf_flow_kernel_times(gf_flow* flow, double* ms, uint64_t* launches, uint32_t kinds);
-/* micro engine with use_graph = 2: wall-clock ticks (100 MHz) per step phase: count, plan, source,
- * partition scatter, transform, sink, retire, select, and the number of steps */
+/* micro or team engine run with use_graph = 3: wall-clock ticks (100 MHz) per step phase and the
+ * number of steps (micro: count, plan, source, partition scatter, transform, sink, retire, select,
+ * steps; team: words 9-17 phase ticks, word 14 steps, words 18-31 shader cycles of sub-phases, then
+ * 8 words per team member: barrier wait, counts+plan, scatter phase, scheduler wave, segments,
+ * sink segments) */
 gf_status gf_flow_phase_profile(gf_flow* flow, uint64_t* ticks, uint32_t words);
 /* algorithmic bytes moved by the last run's partition/transform/source/sink item passes */
 gf_status gf_flow_item_passes(gf_flow* flow, uint64_t* partition_items, uint64_t* transform_items);
