@@ -662,5 +662,26 @@ def feedback_network(flow: Flow, sink_op: Op, threshold: float = 1e-3, distribut
     return src, filt, dests, sink
 
 
+def photon_network(flow: Flow, tally_ptr: int):
+    """Photon transport as a flow (examples/photon_scattering/PhotonScattering.hpp:84-102, with the
+    event routing folded into one alive/done partition): Generate -> Step -> Alive; Alive.alive ->
+    Step, Alive.done -> Done. Step is one flight + absorption tally + roulette + Henyey-Greenstein
+    scatter; the absorbed weight goes to tally_ptr (double[tally_bins], depth bins over the slab).
+    The step node is node 1, so its random stream purpose is 2."""
+    src = flow.source("generate", PhotonSource())
+    step = flow.transform("step", PhotonStep(tally_ptr))
+    alive = flow.partition("alive", PhotonAlive())
+    done = flow.sink("done", PhotonDone())
+    flow.connect(src, step)
+    flow.connect(step, alive)
+    flow.connect(alive.out(0), step)
+    flow.connect(alive.out(1), done)
+    return src, step, alive, done
+
+
+PHOTON_DEFAULTS = {"layers": 10, "tally_bins": 1024, "layer_mfp": 1.0, "mu_t": 1.0, "albedo": 0.9, "hg_g": 0.9,
+                   "roulette_w": 1e-4, "roulette_p": 0.1}
+
+
 FEEDBACK_HIST_LO = 0x33800000  # bits of 2^-24, the smallest U(0,1] weight
 FEEDBACK_HIST_N = 0x3F800000 - 0x33800000 + 1  # up to bits of 1.0

@@ -329,8 +329,13 @@ __device__ inline void photon_step(const RunParams& rp, uint32_t purpose, Photon
     }
     float g = c.g;
     float xi = uniform_co(r[1]);
-    float tmp = (1.0f - g * g) / (1.0f - g + 2.0f * g * xi);
-    float ct = (1.0f + g * g - tmp * tmp) / (2.0f * g);
+    float ct;
+    if (g != 0.0f) {
+        const float tmp = (1.0f - g * g) / (1.0f - g + 2.0f * g * xi);
+        ct = (1.0f + g * g - tmp * tmp) / (2.0f * g);
+    } else {
+        ct = 2.0f * xi - 1.0f;  // isotropic
+    }
     ct = fminf(1.0f, fmaxf(-1.0f, ct));
     float st = sqrtf(1.0f - ct * ct);
     float phi = 6.2831853071795864f * uniform_co(r[2]);

@@ -1024,6 +1024,39 @@ void trace_host(std::vector<uint32_t>& tr, bool on, uint32_t node, uint64_t step
 }
 
 // Plain form: GraphRunner::run with every batch completed before the next loop iteration.
+// Photon step parameters of a run (the slab of the photon transport flow). Zero layers, tally
+// bins, layer thickness or mu_t take the defaults 10, 1024, 1 mfp and 1.
+PhotonParams photon_params_of(const gf_tuning& t)
+{
+    const uint32_t layers = t.photon_layers ? t.photon_layers : 10u;
+    const double mfp = t.photon_layer_mfp > 0 ? t.photon_layer_mfp : 1.0, mu_t = t.photon_mu_t > 0 ? t.photon_mu_t : 1.0;
+    PhotonParams ph;
+    ph.L = (float)(layers * mfp / mu_t);
+    ph.mu_t = (float)mu_t;
+    ph.albedo = (float)t.photon_albedo;
+    ph.g = (float)t.photon_hg_g;
+    ph.roulette_w = (float)t.photon_roulette_w;
+    ph.roulette_p = (float)t.photon_roulette_p;
+    ph.tally_bins = t.photon_tally_bins ? t.photon_tally_bins : 1024u;
+    return ph;
+}
+
+// The slab of the photon transport workload (10 layers of 1 mfp, mu_t 1, albedo 0.9, g 0.9,
+// roulette below 1e-4 with survival 1/10): what a stand-alone photon stage runs with.
+gf_tuning default_photon_tuning()
+{
+    gf_tuning t{};
+    t.photon_layers = 10u;
+    t.photon_tally_bins = 1024u;
+    t.photon_layer_mfp = 1.0;
+    t.photon_mu_t = 1.0;
+    t.photon_albedo = 0.9;
+    t.photon_hg_g = 0.9;
+    t.photon_roulette_w = 1e-4;
+    t.photon_roulette_p = 0.1;
+    return t;
+}
+
 void run_host(gf_flow& f, uint32_t src, uint64_t N, const RunShape& sh, gf_stats& st)
 {
     gf_pool& pool = *f.pool;
@@ -1032,6 +1065,7 @@ void run_host(gf_flow& f, uint32_t src, uint64_t N, const RunShape& sh, gf_stats
     for (uint32_t k = 0; k < n; ++k) {
         stages[k] = std::make_unique<gf_stage>();
         stages[k]->init(&pool, f.nodes[k].nd, k, f.tuning.seed, Q, sh.tile_items);
+        stages[k]->rp.ph = photon_params_of(f.tuning);
     }
     hipStream_t s = f.ctx->stream;
     struct Lane {
@@ -1247,13 +1281,7 @@ void build_engine(gf_flow& f, uint64_t N, const RunShape& sh, uint32_t src)
     e.rp.seed = f.tuning.seed;
     e.rp.C = sh.C;
     const gf_tuning& t = f.tuning;
-    e.rp.ph.L = (float)(t.photon_layers * t.photon_layer_mfp / (t.photon_mu_t > 0 ? t.photon_mu_t : 1.0));
-    e.rp.ph.mu_t = (float)t.photon_mu_t;
-    e.rp.ph.albedo = (float)t.photon_albedo;
-    e.rp.ph.g = (float)t.photon_hg_g;
-    e.rp.ph.roulette_w = (float)t.photon_roulette_w;
-    e.rp.ph.roulette_p = (float)t.photon_roulette_p;
-    e.rp.ph.tally_bins = t.photon_tally_bins;
+    e.rp.ph = photon_params_of(t);
     std::vector<gf_node_desc> nds(n);
     std::vector<int32_t> routes, fuse;
     const uint32_t r = effective_r(f.payload, pick_tile_r(sh.tile_items));
@@ -1832,6 +1860,7 @@ gf_status gf_stage_create(gf_pool* pool, const gf_node_desc* node, uint32_t node
     return guard(&pool->ctx->err, [&] {
         auto s = std::make_unique<gf_stage>();
         s->init(pool, *node, node_key, seed, batch_chunks, tile_items);
+        s->rp.ph = photon_params_of(default_photon_tuning());
         *out = s.release();
     });
 }

@@ -107,6 +107,9 @@ def lib():
                                             C.POINTER(C.c_float), u32p, u32p, C.c_uint32]
         _lib.or_photon_scalar.argtypes = [C.c_uint64, C.c_uint64, C.POINTER(RunCfg),
                                           C.POINTER(C.c_double), u64p, C.c_uint32]
+        f64p = C.POINTER(C.c_double)
+        _lib.or_hg_cos_f64.argtypes = [C.c_double, f64p, f64p, C.c_uint64]
+        _lib.or_hg_rotate_f64.argtypes = [f64p, f64p, f64p, f64p, C.c_uint64]
     return _lib
 
 
@@ -253,6 +256,22 @@ def photon_scalar(seed, photons, photon_cfg, threads=1):
     steps = C.c_uint64(0)
     lib().or_photon_scalar(seed, photons, C.byref(cfg), _p(tally, C.c_double), C.byref(steps), threads)
     return tally, steps.value
+
+
+def hg_cos(g, xi):
+    xi = np.ascontiguousarray(xi, np.float64)
+    out = np.zeros_like(xi)
+    lib().or_hg_cos_f64(C.c_double(g), _p(xi, C.c_double), _p(out, C.c_double), len(xi))
+    return out
+
+
+def hg_rotate(u, ct, phi):
+    u = np.ascontiguousarray(u, np.float64).reshape(-1, 3)
+    ct = np.ascontiguousarray(ct, np.float64)
+    phi = np.ascontiguousarray(phi, np.float64)
+    out = np.zeros_like(u)
+    lib().or_hg_rotate_f64(_p(u, C.c_double), _p(ct, C.c_double), _p(phi, C.c_double), _p(out, C.c_double), len(ct))
+    return out
 
 
 def zipf_table(values, s):

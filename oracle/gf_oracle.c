@@ -25,8 +25,12 @@
  *                                        (tests/integration/test_batching.cpp:577-602),
  *                                        credit-shrunk prefix (test_batching.cpp:550-575)
  *   or_zipf_table/or_zipf_sample         closed-form Zipf probabilities (top-1 / top-10 mass)
- *   or_photon_scalar                     analytic limits: albedo 0 -> Beer-Lambert absorption profile,
- *                                        weight conservation identity
+ *   or_hg_cos_f64                        closed-form HG moments <ct> = g, <ct^2> = (1 + 2 g^2) / 3
+ *   or_hg_rotate_f64                     unit length and <u, u'> = ct for any u, both frames
+ *   or_photon_scalar                     albedo 0 -> Beer-Lambert absorption profile N (e^-mu z0 - e^-mu z1)
+ *                                        per depth bin (two mu_t), roulette unbiasedness (same mean
+ *                                        absorbed weight with and without roulette), fp32 flow
+ *                                        restatement within the stated per-bin tolerance
  */
 #include "gf_oracle.h"
 
@@ -357,8 +361,13 @@ static void or_photon_step_f32(const or_run_cfg* c, uint64_t seed, uint32_t purp
     }
     float g = (float)c->hg_g;
     float xi = (float)(r[1] >> 8) * (1.0f / 16777216.0f);
-    float tmp = (1.0f - g * g) / (1.0f - g + 2.0f * g * xi);
-    float ct = (1.0f + g * g - tmp * tmp) / (2.0f * g);
+    float ct;
+    if (g != 0.0f) {
+        float tmp = (1.0f - g * g) / (1.0f - g + 2.0f * g * xi);
+        ct = (1.0f + g * g - tmp * tmp) / (2.0f * g);
+    } else {
+        ct = 2.0f * xi - 1.0f;
+    }
     if (ct > 1.0f) ct = 1.0f;
     if (ct < -1.0f) ct = -1.0f;
     float st = sqrtf(1.0f - ct * ct);
@@ -1116,6 +1125,63 @@ int or_feedback_scalar(uint64_t seed, uint64_t items, float threshold, uint32_t 
     return OR_OK;
 }
 
+/* Henyey-Greenstein polar cosine by inversion of the HG cumulative distribution (Henyey &
+ * Greenstein 1941; inversion as published for MCML, Wang, Jacques & Zheng 1995):
+ * ct = (1 + g^2 - ((1 - g^2) / (1 - g + 2 g xi))^2) / (2 g), isotropic 2 xi - 1 when g = 0.
+ * (The reference example scatters isotropically, examples/photon_scattering/PhotonFunctors.hpp:172-190;
+ * the anisotropic phase function comes from the photon transport workload definition.) */
+static double or_hg_cos(double g, double xi)
+{
+    double ct;
+    if (g != 0.0) {
+        double tmp = (1.0 - g * g) / (1.0 - g + 2.0 * g * xi);
+        ct = (1.0 + g * g - tmp * tmp) / (2.0 * g);
+    } else {
+        ct = 2.0 * xi - 1.0;
+    }
+    if (ct > 1.0) ct = 1.0;
+    if (ct < -1.0) ct = -1.0;
+    return ct;
+}
+
+/* New direction after scattering by polar cosine ct and azimuth phi about the old direction u
+ * (the standard MCML rotation; near the poles the frame is the lab frame). */
+static void or_hg_rotate(double* ux, double* uy, double* uz, double ct, double phi)
+{
+    double st = sqrt(1.0 - ct * ct), cp = cos(phi), sp = sin(phi);
+    if (fabs(*uz) > 0.99999) {
+        *ux = st * cp;
+        *uy = st * sp;
+        *uz = *uz > 0.0 ? ct : -ct;
+    } else {
+        double tt = sqrt(1.0 - *uz * *uz);
+        double nux = st * (*ux * *uz * cp - *uy * sp) / tt + *ux * ct;
+        double nuy = st * (*uy * *uz * cp + *ux * sp) / tt + *uy * ct;
+        double nuz = -st * cp * tt + *uz * ct;
+        *ux = nux;
+        *uy = nuy;
+        *uz = nuz;
+    }
+}
+
+int or_hg_cos_f64(double g, const double* xi, double* ct, uint64_t n)
+{
+    for (uint64_t i = 0; i < n; ++i) ct[i] = or_hg_cos(g, xi[i]);
+    return OR_OK;
+}
+
+int or_hg_rotate_f64(const double* u, const double* ct, const double* phi, double* out, uint64_t n)
+{
+    for (uint64_t i = 0; i < n; ++i) {
+        double x = u[3 * i], y = u[3 * i + 1], z = u[3 * i + 2];
+        or_hg_rotate(&x, &y, &z, ct[i], phi[i]);
+        out[3 * i] = x;
+        out[3 * i + 1] = y;
+        out[3 * i + 2] = z;
+    }
+    return OR_OK;
+}
+
 /* FP64 photon histories on the same Philox words as the fp32 device step. */
 int or_photon_scalar(uint64_t seed, uint64_t photons, const or_run_cfg* c, double* tally, uint64_t* steps_out,
                      uint32_t threads)
@@ -1152,24 +1218,9 @@ int or_photon_scalar(uint64_t seed, uint64_t photons, const or_run_cfg* c, doubl
                 if (u < c->roulette_p) w /= c->roulette_p;
                 else break;
             }
-            double g = c->hg_g;
-            double xi = (double)(r[1] >> 8) * (1.0 / 16777216.0);
-            double tmp = (1.0 - g * g) / (1.0 - g + 2.0 * g * xi);
-            double ct = (1.0 + g * g - tmp * tmp) / (2.0 * g);
-            if (ct > 1.0) ct = 1.0;
-            if (ct < -1.0) ct = -1.0;
-            double st = sqrt(1.0 - ct * ct);
-            double phi = 6.283185307179586 * ((double)(r[2] >> 8) * (1.0 / 16777216.0));
-            double cp = cos(phi), sp = sin(phi);
-            if (fabs(uz) > 0.99999) {
-                ux = st * cp; uy = st * sp; uz = uz > 0.0 ? ct : -ct;
-            } else {
-                double tt = sqrt(1.0 - uz * uz);
-                double nux = st * (ux * uz * cp - uy * sp) / tt + ux * ct;
-                double nuy = st * (uy * uz * cp + ux * sp) / tt + uy * ct;
-                double nuz = -st * cp * tt + uz * ct;
-                ux = nux; uy = nuy; uz = nuz;
-            }
+            const double ct = or_hg_cos(c->hg_g, (double)(r[1] >> 8) * (1.0 / 16777216.0));
+            const double phi = 6.283185307179586 * ((double)(r[2] >> 8) * (1.0 / 16777216.0));
+            or_hg_rotate(&ux, &uy, &uz, ct, phi);
         }
     }
     for (int k = 0; k < nt; ++k)

@@ -171,6 +171,8 @@ int or_feedback_scalar(uint64_t seed, uint64_t items, float threshold, uint32_t 
                        const float* multipliers, uint32_t* weight_bits, uint32_t* gens, uint32_t threads);
 
 /* FP64 photon histories, schedule free, same RNG words as the device (ScalarTransport.hpp:87-211 pattern). */
+int or_hg_cos_f64(double g, const double* xi, double* ct, uint64_t n);
+int or_hg_rotate_f64(const double* u, const double* ct, const double* phi, double* out, uint64_t n);
 int or_photon_scalar(uint64_t seed, uint64_t photons, const or_run_cfg* cfg, double* tally, uint64_t* steps_out,
                      uint32_t threads);
 
