@@ -114,6 +114,40 @@ def traffic_for(workload, launches):
     return float(kv["traffic_bytes_per_launch"])
 
 
+def photon_forms(g, torch, N, Np, C, P, Q, verify):
+    """Photon transport (10 layers of 1 mfp, albedo 0.9, HG g=0.9, roulette 1e-4 / 0.1, 1024 depth
+    bins): the batched persistent form (team engine) and the plain host-planned one-batch form,
+    photons/s and photon steps/s; with verify, the worst per-bin relative difference of the batched
+    tallies from the FP64 CPU oracle on the same Philox words."""
+    cfg = dict(g.PHOTON_DEFAULTS)
+    out = {}
+    for name, n, kw in (("photon_batched_team", N, dict(schedule=g.SCHEDULE_DEVICE, batch_chunks=Q)),
+                        ("photon_plain_host_planned_Q1", Np, dict(schedule=g.SCHEDULE_HOST, batch_chunks=1))):
+        tally = torch.zeros(cfg["tally_bins"], dtype=torch.float64, device="cuda")
+        flow = g.Flow(g.PAYLOAD_PHOTON)
+        src, *_ = g.photon_network(flow, tally.data_ptr())
+        flow.configure(g.Tuning(seed=42, chunk_capacity=C, live_particles=P, photon=cfg, **kw))
+        flow.run(src, min(n, 100000))  # warm-up: engine build, pool
+        tally.zero_()
+        dt, st = run_once(flow, src, tally, n, torch)
+        part_items, xf_items = flow.item_passes()
+        photon_steps = part_items  # one alive-partition pass per photon step (the step is fused into it)
+        rec = {"value": n / dt, "unit": "photons/s", "items": n, "photon_steps_per_s": photon_steps / dt,
+               "photon_steps": int(photon_steps), "engine_steps": int(st["steps"])}
+        if verify and name == "photon_batched_team":
+            import numpy as np
+            import oracle as O
+            ref, ref_steps = O.photon_scalar(42, n, cfg, threads=min(os.cpu_count() or 1, 16))
+            t = tally.cpu().numpy()
+            rec["fp64_oracle_max_bin_rel_err"] = float((np.abs(t - ref) / ref).max())
+            rec["fp64_oracle_total_rel_err"] = float(abs(t.sum() - ref.sum()) / ref.sum())
+            rec["fp64_oracle_photon_steps"] = int(ref_steps)
+        out[name] = rec
+        log(f"[bench] {name} N={n}: {n / dt:.4g} photons/s, {photon_steps / dt:.4g} photon steps/s "
+            + (f"(max bin rel err vs FP64 {rec['fp64_oracle_max_bin_rel_err']:.2e})" if "fp64_oracle_max_bin_rel_err" in rec else ""))
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -134,6 +168,8 @@ def main():
     ap.add_argument("--phases", action="store_true", help="device engines: one more run with the per-phase profile")
     ap.add_argument("--members", action="store_true", help="team engine: with --phases, per-member phase times")
     ap.add_argument("--no-fuse", action="store_true", help="run A/B as separate transform batches")
+    ap.add_argument("--photons", type=float, default=1e7, help="photon transport form: photons (0 = skip)")
+    ap.add_argument("--plain-photons", type=float, default=1e6)
     ap.add_argument("--copy-calib", type=float, default=0.0,
                     help="after the runs, copy this many MiB device to device once (the known-size kernel that "
                          "scales rocprofv3 FETCH_SIZE / WRITE_SIZE readings to bytes)")
@@ -253,6 +289,9 @@ def main():
         dt, _ = run_once(pflow, psrc, pcounts, Np, torch)
         forms["plain_host_planned_Q1"] = {"value": Np / dt, "unit": "particles/s", "items": Np}
         log(f"[bench] plain form (host planned, Q=1) N={Np}: {Np / dt:.4g} particles/s")
+    if args.photons > 0:
+        forms.update(photon_forms(g, torch, int(args.photons), int(args.plain_photons), C, P, Q,
+                                  verify=not args.no_cpu))
     result["forms"] = forms
 
     if not args.no_cpu:

@@ -92,7 +92,8 @@ class TuningC(C.Structure):
                 ("use_graph", C.c_uint32), ("fuse_transforms", C.c_uint32), ("max_workgroups", C.c_uint32),
                 ("device_engine", C.c_uint32), ("photon_layers", C.c_uint32), ("photon_tally_bins", C.c_uint32),
                 ("photon_layer_mfp", C.c_double), ("photon_mu_t", C.c_double), ("photon_albedo", C.c_double),
-                ("photon_hg_g", C.c_double), ("photon_roulette_w", C.c_double), ("photon_roulette_p", C.c_double)]
+                ("photon_hg_g", C.c_double), ("photon_roulette_w", C.c_double), ("photon_roulette_p", C.c_double),
+                ("team_xcd", C.c_uint32), ("reserved0", C.c_uint32)]
 
 
 class Stats(C.Structure):
@@ -443,6 +444,7 @@ class Tuning:
     max_workgroups: Optional[int] = None
     device_engine: int = 0  # 0 autotuned, 1 wide step kernels, 2 one persistent workgroup, 3 XCD team
     photon: Optional[dict] = None
+    team_xcd: Optional[int] = None  # team engine: run the team on this XCD (None: where it first starts)
 
     def validate(self):
         for name in ("batch_chunks", "chunk_capacity", "live_particles", "num_chunks", "tile_items",
@@ -457,7 +459,8 @@ class Tuning:
                     tile_items=self.tile_items or 0, schedule=self.schedule,
                     steps_per_launch=self.steps_per_launch or 0, trace=1 if self.trace else 0,
                     use_graph=self.use_graph, fuse_transforms=1 if self.fuse_transforms else 0,
-                    max_workgroups=self.max_workgroups or 0, device_engine=self.device_engine)
+                    max_workgroups=self.max_workgroups or 0, device_engine=self.device_engine,
+                    team_xcd=0 if self.team_xcd is None else self.team_xcd + 1)
         ph = self.photon or {}
         t.photon_layers = ph.get("layers", 10)
         t.photon_tally_bins = ph.get("tally_bins", 1024)

@@ -1243,7 +1243,14 @@ std::string shape_key(const gf_flow& f, const RunShape& s, uint64_t N)
                     std::to_string(f.tuning.trace) + "/" + std::to_string(f.tuning.seed) + "/" + std::to_string(N) + "/" +
                     std::to_string(f.tuning.steps_per_launch) + "/" + std::to_string(f.tuning.fuse_transforms) + "/" +
                     std::to_string(f.tuning.max_workgroups) + "/" + std::to_string(f.tuning.device_engine) + "/" +
-                    std::to_string(f.tuning.use_graph);
+                    std::to_string(f.tuning.use_graph) + "/" + std::to_string(f.tuning.team_xcd);
+    {
+        const PhotonParams ph = photon_params_of(f.tuning);
+        char buf[160];
+        std::snprintf(buf, sizeof buf, "/%a/%a/%a/%a/%a/%a/%u", ph.L, ph.mu_t, ph.albedo, ph.g, ph.roulette_w,
+                      ph.roulette_p, ph.tally_bins);
+        k += buf;
+    }
     for (auto& n : f.nodes) {
         k += "|" + std::to_string(n.nd.role) + "," + std::to_string(n.nd.op) + "," + std::to_string(n.nd.bins);
         for (int32_t r : n.routes) k += "," + std::to_string(r);
@@ -1402,6 +1409,7 @@ void build_engine(gf_flow& f, uint64_t N, const RunShape& sh, uint32_t src)
         // default team: every CU of one XCD (the device's CUs over its 8 XCDs)
         const uint32_t xcd_cus = std::max<uint32_t>(1u, f.ctx->cus / 8u);
         ta.K = std::min<uint32_t>(t.max_workgroups ? t.max_workgroups : xcd_cus, 32u);
+        ta.xcd = t.team_xcd;
         ta.tpc = tpc;
         ta.bb = bb;
         ta.max_steps = t.steps_per_launch ? t.steps_per_launch : 65536u;

@@ -58,7 +58,10 @@ constexpr uint32_t kClrThreads = 64u * 4u;  // waves that wait for their stores 
 constexpr uint32_t kTeamThreads = 64u * kTeamWaves;
 constexpr uint32_t kTeamSpin = 1u << 24;
 
-enum { TW_ARRIVED = 0, TW_XCC0 = 1, TW_BAR = 2, TW_ERR = 4, TW_REG = 8, TW_WORDS = 32 };  // TW_ERR + step parity
+// Team words: the team's XCC (+1), the step barrier, the per-step-parity error words, the closed
+// team size (bit 31 set once the leader closed registration) and one registration counter per XCC.
+enum { TW_XCC0 = 1, TW_BAR = 2, TW_ERR = 4, TW_TEAM = 6, TW_REG = 8, TW_WORDS = 32 };
+constexpr uint32_t kJoinSpin = 1u << 12;  // leader's wait for the rest of the team (polls of L2)
 enum { EXIT_NONE = 0, EXIT_DONE = 1, EXIT_STALL = 2, EXIT_HANDOFF = 3, EXIT_LIMIT = 4, EXIT_STATUS = 5, EXIT_SKIP = 6 };
 enum {
     PROF_T_WAIT = 9, PROF_T_COUNTS = 10, PROF_T_PLAN = 11, PROF_T_SEG = 12, PROF_T_SELECT = 13, PROF_T_STEPS = 14,
@@ -97,6 +100,7 @@ struct TeamArgs {
     uint32_t* tw;      // team words, cleared before every launch
     uint32_t K, tpc, bb, max_steps, rc, n_routes;
     uint32_t c_shift;  // log2(C) when C is a power of two, else 32
+    uint32_t xcd;      // XCC the team must use, plus one; 0: the XCC of the first workgroup to run
     unsigned long long* mprof;  // optional per-member wall-clock totals, kMemberProf words per member
 };
 
@@ -204,27 +208,49 @@ __device__ inline void agg_add(bool v, uint32_t key, uint32_t* arr)
 __device__ inline uint32_t div_c(uint32_t x, uint32_t C, uint32_t shift) { return shift < 32u ? x >> shift : x / C; }
 
 // ---- team formation --------------------------------------------------------------------------
-// Every workgroup of the grid registers under its XCC id; the members are the first K of the XCC
-// that block 0 runs on. Returns the member rank (or ~0u) in sh[0] and the team size in sh[1].
+// The team lives on one XCC: the requested one (a.xcd = XCC + 1), or the XCC of the first
+// workgroup that runs. Workgroups elsewhere leave at once, so the launch never waits for CUs that
+// other kernels hold. On the team's XCC the first K to register are candidates; the first of them
+// (the leader) waits a bounded time for the other K - 1, then closes the team at the count it saw,
+// and candidates that registered after that leave too. Returns the member rank (or ~0u) in sh[0]
+// and the team size in sh[1].
 __device__ inline void team_join(const TeamArgs& a, uint32_t* sh)
 {
     if (threadIdx.x == 0) {
         uint32_t* tw = a.tw;
         const uint32_t x = xcc_id();
-        if (blockIdx.x == 0) atomicExch(&tw[TW_XCC0], x + 1u);
-        const uint32_t slot = atomicAdd(&tw[TW_REG + x], 1u);
-        __builtin_amdgcn_s_waitcnt(0);
-        atomicAdd(&tw[TW_ARRIVED], 1u);
-        uint32_t spins = 0, ok = 1u;
-        while (ld_u32(tw, TW_ARRIVED) < gridDim.x)
-            if (++spins > kTeamSpin) {
-                ok = 0u;
-                break;
+        uint32_t x0;
+        if (a.xcd) {
+            x0 = a.xcd - 1u;
+        } else {
+            const uint32_t prev = atomicCAS(&tw[TW_XCC0], 0u, x + 1u);
+            x0 = prev ? prev - 1u : x;
+        }
+        uint32_t mine = ~0u, T = 0u, ok = 1u;
+        if (x == x0) {
+            const uint32_t slot = atomicAdd(&tw[TW_REG + x], 1u);
+            if (slot < a.K) {
+                if (slot == 0u) {
+                    uint32_t spins = 0, reg = ld_u32(tw, TW_REG + x);
+                    while (reg < a.K && ++spins < kJoinSpin) reg = ld_u32(tw, TW_REG + x);
+                    T = reg < a.K ? reg : a.K;
+                    atomicExch(&tw[TW_TEAM], 0x80000000u | T);
+                } else {
+                    uint32_t spins = 0, w = ld_u32(tw, TW_TEAM);
+                    while (!(w & 0x80000000u)) {
+                        if (++spins > kTeamSpin) {
+                            ok = 0u;
+                            break;
+                        }
+                        w = ld_u32(tw, TW_TEAM);
+                    }
+                    T = w & 0x7FFFFFFFu;
+                }
+                if (ok && slot < T) mine = slot;
             }
-        const uint32_t x0 = ld_u32(tw, TW_XCC0) - 1u;
-        const uint32_t reg = ld_u32(tw, TW_REG + (x0 & 15u));
-        sh[0] = (ok && x == x0 && slot < a.K) ? slot : ~0u;
-        sh[1] = reg < a.K ? reg : a.K;
+        }
+        sh[0] = mine;
+        sh[1] = T;
         if (!ok) set_status(&a.e.ctl[CTL_STATUS], GF_DEVICE_ERROR);
     }
     __syncthreads();

@@ -116,6 +116,9 @@ typedef struct gf_tuning {
                                 replicated scheduler and one barrier per step) */
     uint32_t photon_layers, photon_tally_bins;
     double photon_layer_mfp, photon_mu_t, photon_albedo, photon_hg_g, photon_roulette_w, photon_roulette_p;
+    uint32_t team_xcd;       /* team engine: XCD the team runs on, plus one (0 -> the XCD of the first
+                                workgroup to start); flows that run side by side use different XCDs */
+    uint32_t reserved0;
 } gf_tuning;
 
 typedef struct gf_stats {
