@@ -149,6 +149,30 @@ SIGNATURES = {
     "gf_hist_rows": (C.c_int, [_V, _V, C.c_uint64, C.c_uint32, _P(C.c_uint64), C.c_uint64, _P(C.c_uint64)]),
 }
 
+class ServerConfigC(C.Structure):
+    _fields_ = [("slots", C.c_uint32), ("team_size", C.c_uint32), ("memory_budget", C.c_uint64),
+                ("max_items", C.c_uint64), ("topology_bytes", C.c_uint64 * 3)]
+
+
+class JobDescC(C.Structure):
+    _fields_ = [("topology", C.c_uint32), ("reserved", C.c_uint32), ("items", C.c_uint64), ("seed", C.c_uint64)]
+
+
+class JobResultC(C.Structure):
+    _fields_ = [("job", C.c_uint64), ("topology", C.c_uint32), ("slot", C.c_uint32), ("status", C.c_uint32),
+                ("engine", C.c_uint32), ("items", C.c_uint64), ("steps", C.c_uint64), ("submit_ms", C.c_double),
+                ("start_ms", C.c_double), ("end_ms", C.c_double), ("digest", C.c_uint64), ("tally_sum", C.c_double)]
+
+
+SIGNATURES.update({
+    "gf_server_create": (C.c_int, [_V, _P(ServerConfigC), _P(_V)]),
+    "gf_server_submit": (C.c_int, [_V, _P(JobDescC), _P(C.c_uint64)]),
+    "gf_server_poll": (C.c_int, [_V, _P(JobResultC), C.c_uint32, _P(C.c_uint32), C.c_uint32]),
+    "gf_server_job_tally": (C.c_int, [_V, C.c_uint64, _P(C.c_double), C.c_uint32]),
+    "gf_server_last_error": (C.c_int, [_V, C.c_char_p, C.c_uint32]),
+    "gf_server_destroy": (C.c_int, [_V]),
+})
+
 _lib = None
 
 
@@ -663,6 +687,96 @@ def feedback_network(flow: Flow, sink_op: Op, threshold: float = 1e-3, distribut
         flow.connect(filt.out(1 + d), node)
         flow.connect(node, filt)
     return src, filt, dests, sink
+
+
+JOB_FEEDBACK, JOB_HASH, JOB_PHOTON = 0, 1, 2
+JOB_NAMES = {JOB_FEEDBACK: "feedback", JOB_HASH: "hash", JOB_PHOTON: "photon"}
+
+
+class Server:
+    """Serving loop over the C ABI: independent flow jobs (JOB_FEEDBACK particle feedback network,
+    JOB_HASH keyed murmur3 pipeline with 4096 buckets, JOB_PHOTON photon transport) admitted into
+    slots that run side by side on the device (gf_server_* in include/graphflow.h)."""
+
+    def __init__(self, slots: int = 8, team_size: int = 0, memory_budget: int = 0, max_items: int = 0,
+                 ctx: Optional[Context] = None):
+        self.ctx = ctx or default_context()
+        self.cfg = ServerConfigC(slots=slots, team_size=team_size, memory_budget=memory_budget, max_items=max_items)
+        h = _V()
+        _raise(lib().gf_server_create(self.ctx.handle, C.byref(self.cfg), C.byref(h)))
+        self.handle = h
+
+    def submit(self, topology: int, items: int, seed: int = 42) -> int:
+        j = JobDescC(topology=topology, items=items, seed=seed)
+        jid = C.c_uint64()
+        _raise(lib().gf_server_submit(self.handle, C.byref(j), C.byref(jid)))
+        return jid.value
+
+    def poll(self, wait_all: bool = False, cap: int = 8192) -> list:
+        arr = (JobResultC * cap)()
+        n = C.c_uint32()
+        _raise(lib().gf_server_poll(self.handle, arr, cap, C.byref(n), 1 if wait_all else 0))
+        out = []
+        for r in arr[:n.value]:
+            out.append({f: getattr(r, f) for f, _ in JobResultC._fields_})
+        return out
+
+    def tally(self, job: int, bins: int = 1024):
+        import numpy as np
+        t = np.zeros(bins, np.float64)
+        _raise(lib().gf_server_job_tally(self.handle, job, t.ctypes.data_as(C.POINTER(C.c_double)), bins))
+        return t
+
+    def last_error(self) -> str:
+        b = C.create_string_buffer(512)
+        lib().gf_server_last_error(self.handle, b, 512)
+        return b.value.decode()
+
+    def close(self):
+        if getattr(self, "handle", None):
+            lib().gf_server_destroy(self.handle)
+            self.handle = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+def record_digest(weight_bits) -> int:
+    """The served feedback result digest of per-id weight bits: sum over ids of mix64(id << 32 | bits)
+    modulo 2^64 (numpy; the same value the device computes)."""
+    import numpy as np
+    w = np.asarray(weight_bits, np.uint64)
+    i = np.arange(len(w), dtype=np.uint64)
+    with np.errstate(over="ignore"):
+        v = (i << np.uint64(32)) | w
+        return int(mix64_np(v).sum(dtype=np.uint64))
+
+
+def multiset_digest(sums, counts) -> int:
+    """The served hash result digest of per-bucket (hash sum, count) multiset pairs."""
+    import numpy as np
+    s = np.asarray(sums, np.uint64)
+    c = np.asarray(counts, np.uint64)
+    b = np.arange(len(s), dtype=np.uint64)
+    with np.errstate(over="ignore"):
+        d = mix64_np((b << np.uint64(48)) ^ s) + mix64_np(c * np.uint64(0x9E3779B97F4A7C15) + b)
+        return int(d.sum(dtype=np.uint64))
+
+
+def mix64_np(v):
+    """SplitMix64 finalizer over a numpy uint64 array (wrapping arithmetic)."""
+    import numpy as np
+    v = np.asarray(v, np.uint64).copy()
+    with np.errstate(over="ignore"):
+        v ^= v >> np.uint64(30)
+        v *= np.uint64(0xBF58476D1CE4E5B9)
+        v ^= v >> np.uint64(27)
+        v *= np.uint64(0x94D049BB133111EB)
+        v ^= v >> np.uint64(31)
+    return v
 
 
 def photon_network(flow: Flow, tally_ptr: int):

@@ -1238,9 +1238,13 @@ void run_host(gf_flow& f, uint32_t src, uint64_t N, const RunShape& sh, gf_stats
 
 std::string shape_key(const gf_flow& f, const RunShape& s, uint64_t N)
 {
-    std::string k = std::to_string(f.payload) + "/" + std::to_string(s.C) + "/" + std::to_string(s.P) + "/" +
+    // N, P and the seed are per-run scalars of the engine (set by run_device), not part of its build;
+    // only whether P makes the persistent one-workgroup engine eligible is
+    (void)N;
+    const bool small_p = (uint64_t)s.P * payload_bytes(f.payload) <= (4ull << 20);
+    std::string k = std::to_string(f.payload) + "/" + std::to_string(s.C) + "/" + std::to_string(small_p) + "/" +
                     std::to_string(s.S) + "/" + std::to_string(s.Q) + "/" + std::to_string(s.tile_items) + "/" +
-                    std::to_string(f.tuning.trace) + "/" + std::to_string(f.tuning.seed) + "/" + std::to_string(N) + "/" +
+                    std::to_string(f.tuning.trace) + "/" +
                     std::to_string(f.tuning.steps_per_launch) + "/" + std::to_string(f.tuning.fuse_transforms) + "/" +
                     std::to_string(f.tuning.max_workgroups) + "/" + std::to_string(f.tuning.device_engine) + "/" +
                     std::to_string(f.tuning.use_graph) + "/" + std::to_string(f.tuning.team_xcd);
@@ -1571,6 +1575,18 @@ void run_device(gf_flow& f, uint32_t src, uint64_t N, const RunShape& sh, gf_sta
     const std::string key = shape_key(f, sh, N);
     if (!f.eng || f.eng->key != key) build_engine(f, N, sh, src);
     EngineBuffers& eb = *f.eng;
+    if (eb.e.N != N || eb.e.P != sh.P || eb.e.rp.seed != f.tuning.seed) {
+        // same build, new run scalars: every kernel takes the engine by value, so only a captured
+        // graph has to be captured again
+        eb.e.N = eb.ta.e.N = N;
+        eb.e.P = eb.ta.e.P = sh.P;
+        eb.e.rp.seed = eb.ta.e.rp.seed = f.tuning.seed;
+        if (eb.exec) {
+            (void)hipGraphExecDestroy(eb.exec);
+            eb.exec = nullptr;
+        }
+        eb.graph_steps = 0;
+    }
     hipStream_t s = f.ctx->stream;
     hipLaunchKernelGGL(k_engine_setup, dim3(1), dim3(kThreads), sizeof(SelLds), s, eb.e, eb.team ? 0u : 1u);
     HIPX(hipGetLastError());
@@ -2114,3 +2130,5 @@ gf_status gf_hist_rows(gf_context* ctx, const uint64_t* counts, uint64_t n, uint
 }
 
 }  // extern "C"
+
+#include "gf_server.hpp"

@@ -200,6 +200,52 @@ gf_status gf_flow_phase_profile(gf_flow* flow, uint64_t* ticks, uint32_t words);
 /* algorithmic bytes moved by the last run's partition/transform/source/sink item passes */
 gf_status gf_flow_item_passes(gf_flow* flow, uint64_t* partition_items, uint64_t* transform_items);
 
+/* ---- serving: independent flow jobs admitted into slots as they arrive ------------------------
+ * No counterpart in the reference (its runner executes one flow per run). Jobs are one of three
+ * topologies: the particle feedback network (result: digest of the per-id weight bits), the keyed
+ * murmur3 hash pipeline with 4096 buckets (result: digest of the per-bucket multisets) and photon
+ * transport (result: the 1024-bin absorbed-weight tally). Each slot owns a stream and keeps one
+ * resident flow per topology; a job is admitted first come, first served into a free slot when its
+ * topology's flow is resident there or fits the free memory budget. Submission, polling and
+ * destruction may be called from one host thread while the slots run. */
+enum { GF_JOB_FEEDBACK = 0, GF_JOB_HASH = 1, GF_JOB_PHOTON = 2 };
+
+typedef struct gf_server_config {
+    uint32_t slots;             /* jobs running side by side (0 -> 8, one per XCD)                  */
+    uint32_t team_size;         /* team engine workgroups per job (0 -> CUs of one XCD minus 4)     */
+    uint64_t memory_budget;     /* device bytes the slots' flows may hold (0 -> 64 GiB)             */
+    uint64_t max_items;         /* largest feedback job (0 -> 1e7; sizes its result buffer)         */
+    uint64_t topology_bytes[3]; /* out: device bytes a slot holds per topology                      */
+} gf_server_config;
+
+typedef struct gf_job_desc {
+    uint32_t topology;          /* GF_JOB_* */
+    uint32_t reserved;
+    uint64_t items;             /* N: particles, keys or photons */
+    uint64_t seed;
+} gf_job_desc;
+
+typedef struct gf_job_result {
+    uint64_t job;
+    uint32_t topology, slot;
+    uint32_t status;            /* gf_status of the job's run */
+    uint32_t engine;            /* gf_stats.engine of the run */
+    uint64_t items, steps;
+    double submit_ms, start_ms, end_ms; /* server clock: submitted, started on its slot, finished */
+    uint64_t digest;            /* feedback / hash result digest (order independent) */
+    double tally_sum;           /* photon: total absorbed weight */
+} gf_job_result;
+
+typedef struct gf_server gf_server;
+gf_status gf_server_create(gf_context* ctx, const gf_server_config* cfg, gf_server** out);
+gf_status gf_server_submit(gf_server* server, const gf_job_desc* job, uint64_t* job_id);
+/* completed jobs since the last poll (up to cap); wait_all = 1 first waits until every submitted
+ * job has finished */
+gf_status gf_server_poll(gf_server* server, gf_job_result* out, uint32_t cap, uint32_t* n, uint32_t wait_all);
+gf_status gf_server_job_tally(gf_server* server, uint64_t job, double* tally, uint32_t bins);
+gf_status gf_server_last_error(gf_server* server, char* buffer, uint32_t size);
+gf_status gf_server_destroy(gf_server* server);
+
 /* ---- device histogram rows: compact a dense u64 count array into ascending (value,count) -- */
 gf_status gf_hist_rows(gf_context* ctx, const uint64_t* device_counts, uint64_t n, uint32_t lo,
                        uint64_t* host_rows /* 2 per row */, uint64_t cap, uint64_t* rows);

@@ -1,0 +1,77 @@
+"""Serving: a mixed stream of independent flow jobs admitted into slots that run side by side.
+
+Each served job's result must equal the same job run alone (exact digests for the particle
+feedback and keyed hash jobs; photon tallies equal up to the order of the fp64 atomic additions,
+1e-12 relative), and equal the CPU oracle: the feedback digest from the oracle's per-id weights,
+the hash digest from the oracle's per-bucket multisets, the photon tally within the per-bin
+tolerance of the photon tests (5e-3 relative plus the weight of two histories moved to a
+neighbouring bin by fp32 rounding).
+"""
+import numpy as np
+import pytest
+
+import graphflow_amd as g
+import oracle as O
+from gpu_util import torch_cuda
+
+pytestmark = pytest.mark.gpu
+
+JOBS = [(g.JOB_FEEDBACK, 20000, 1), (g.JOB_HASH, 30000, 2), (g.JOB_PHOTON, 20000, 3),
+        (g.JOB_FEEDBACK, 3000, 4), (g.JOB_HASH, 4096, 5), (g.JOB_PHOTON, 5000, 6),
+        (g.JOB_FEEDBACK, 120000, 7), (g.JOB_HASH, 100000, 8), (g.JOB_PHOTON, 50000, 9),
+        (g.JOB_FEEDBACK, 1, 10), (g.JOB_HASH, 8193, 11), (g.JOB_PHOTON, 1000, 12)]
+
+
+def serve(jobs, slots):
+    torch_cuda()
+    sv = g.Server(slots=slots, max_items=200000)
+    ids = [sv.submit(t, n, s) for t, n, s in jobs]
+    res = {r["job"]: r for r in sv.poll(wait_all=True)}
+    out = []
+    for jid in ids:
+        r = res[jid]
+        assert r["status"] == 0, (r, sv.last_error())
+        r = dict(r)
+        if r["topology"] == g.JOB_PHOTON:
+            r["tally"] = sv.tally(jid)
+        out.append(r)
+    sv.close()
+    return out
+
+
+def hash_graph():
+    return [dict(role=O.ROLE_SOURCE, op="SRC_KEYED_ZIPF", bins=1, u=[1 << 20], f=[1.1], routes=[1]),
+            dict(role=O.ROLE_TRANSFORM, op="CLS_KEY_BUCKET", bins=4096, routes=[2] * 4096),
+            dict(role=O.ROLE_SINK, op="SNK_CHUNK_DIGEST", bins=0, routes=[])]
+
+
+def test_mixed_stream_equals_alone_and_oracle():
+    mixed = serve(JOBS, slots=8)
+    alone = serve(JOBS, slots=1)
+    assert len({r["slot"] for r in mixed}) > 1
+    for (topo, n, seed), m, a in zip(JOBS, mixed, alone):
+        assert m["items"] == n and a["items"] == n
+        if topo == g.JOB_PHOTON:
+            assert np.allclose(m["tally"], a["tally"], rtol=1e-12, atol=0.0)
+            ref, _ = O.photon_scalar(seed, n, dict(g.PHOTON_DEFAULTS), threads=16)
+            assert (np.abs(m["tally"] - ref) <= 5e-3 * ref + 2.0).all()
+        else:
+            assert m["digest"] == a["digest"], (topo, n, seed)
+            if topo == g.JOB_FEEDBACK:
+                w, _ = O.feedback_scalar(seed, n, threads=16)
+                assert m["digest"] == g.record_digest(w), (n, seed)
+            else:
+                ref = O.run(hash_graph(), seed=seed, items=n, capacity=8192, live=16384, batch=8, mode=O.MODE_STEP,
+                            multiset_bins=4096)
+                assert ref["status"] == O.OK
+                assert m["digest"] == g.multiset_digest(ref["multiset"], ref["multiset_items"]), (n, seed)
+
+
+def test_latency_accounting_and_slots():
+    jobs = [(g.JOB_FEEDBACK, 10000, s) for s in range(16)]
+    res = serve(jobs, slots=4)
+    for r in res:
+        assert r["submit_ms"] <= r["start_ms"] <= r["end_ms"]
+        assert r["engine"] == 3  # the team engine, on the slot's XCD
+    assert {r["slot"] for r in res} <= {0, 1, 2, 3}
+    assert len({r["digest"] for r in res}) == 16  # different seeds, different results
