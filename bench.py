@@ -148,6 +148,92 @@ def photon_forms(g, torch, N, Np, C, P, Q, verify):
     return out
 
 
+def served_form(g, torch, jobs, rate, max_items, seed=42):
+    """Served mixed stream: `jobs` independent flow jobs, Poisson arrivals at `rate` per second,
+    N ~ log-uniform[1e3, max_items], topology uniform over {particle feedback, keyed hash pipeline,
+    photon transport}; aggregate items/s and p50/p99 job latency (submit to finish), next to the
+    same jobs run one at a time with the plain host-planned form (Q=1)."""
+    import threading
+    import numpy as np
+
+    rng = np.random.default_rng(seed)
+    topo = rng.integers(0, 3, jobs)
+    items = np.exp(rng.uniform(np.log(1e3), np.log(max_items), jobs)).astype(np.int64)
+    arrive = np.cumsum(rng.exponential(1.0 / rate, jobs))
+    seeds = 1000 + np.arange(jobs)
+    sv = g.Server(slots=8, max_items=int(max_items) + 1, prewarm=True)  # every slot builds its flows first
+    ids = [None] * jobs
+
+    def submitter():
+        t0 = time.perf_counter()
+        for i in range(jobs):
+            d = arrive[i] - (time.perf_counter() - t0)
+            if d > 0:
+                time.sleep(d)
+            ids[i] = sv.submit(int(topo[i]), int(items[i]), int(seeds[i]))
+
+    th = threading.Thread(target=submitter)
+    th.start()
+    th.join()
+    res = sv.poll(wait_all=True)
+    sv.close()
+    bad = [r for r in res if r["status"] != 0]
+    lat = np.array([r["end_ms"] - r["submit_ms"] for r in res])
+    t_first = min(r["submit_ms"] for r in res)
+    t_last = max(r["end_ms"] for r in res)
+    total = int(items.sum())
+    served = total / ((t_last - t_first) * 1e-3)
+    per_topo = {}
+    for t in range(3):
+        rs = [r for r in res if r["topology"] == t]
+        if rs:
+            per_topo[g.JOB_NAMES[t]] = {
+                "jobs": len(rs), "items": int(sum(r["items"] for r in rs)),
+                "mean_service_ms": float(np.mean([r["end_ms"] - r["start_ms"] for r in rs])),
+                "service_items_per_s": float(sum(r["items"] for r in rs) / max(1e-9, 1e-3 * sum(r["end_ms"] - r["start_ms"] for r in rs)))}
+    # plain form: every 4th job of the same stream one at a time, host-planned one batch at a time
+    sample = list(range(0, jobs, 4))
+    plain_t = 0.0
+    tally = torch.zeros(1024, dtype=torch.float64, device="cuda")
+    rec = torch.zeros(int(max_items) + 1, dtype=torch.int32, device="cuda")
+    ms = torch.zeros(2 * 4096, dtype=torch.int64, device="cuda")
+    flows = {}
+    for t in range(3):
+        f = g.Flow([g.PAYLOAD_PARTICLE, g.PAYLOAD_KEYED, g.PAYLOAD_PHOTON][t])
+        if t == 0:
+            src, *_ = g.feedback_network(f, g.RecordWeight(rec.data_ptr(), int(max_items) + 1))
+        elif t == 1:
+            src = f.source("keys", g.ZipfKeys(1 << 20, 1.1))
+            part = f.partition("bucket", g.KeyBucket(4096))
+            snk = f.sink("digest", g.ChunkDigest(0, 0, ms.data_ptr(), 4096))
+            f.connect(src, part)
+            for b in range(4096):
+                f.connect(part, snk, b)
+        else:
+            src, *_ = g.photon_network(f, tally.data_ptr())
+        flows[t] = (f, src)
+    for i in sample:
+        f, src = flows[int(topo[i])]
+        f.configure(g.Tuning(seed=int(seeds[i]), batch_chunks=1, chunk_capacity=8192, live_particles=16384,
+                             schedule=g.SCHEDULE_HOST, photon=dict(g.PHOTON_DEFAULTS)))
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        f.run(src, int(items[i]))
+        torch.cuda.synchronize()
+        plain_t += time.perf_counter() - t0
+    plain = int(items[sample].sum()) / plain_t
+    out = {"value": served, "unit": "items/s", "jobs": jobs, "items": total, "arrival_rate_per_s": rate,
+           "per_topology": per_topo, "plain_sample_jobs": len(sample),
+           "n_range": [1000, int(max_items)], "slots": 8, "failed_jobs": len(bad),
+           "p50_latency_ms": float(np.percentile(lat, 50)), "p99_latency_ms": float(np.percentile(lat, 99)),
+           "makespan_s": (t_last - t_first) * 1e-3, "plain_one_at_a_time_items_per_s": plain,
+           "speedup_vs_plain": served / plain,
+           "jobs_per_topology": {g.JOB_NAMES[t]: int((topo == t).sum()) for t in range(3)}}
+    log(f"[bench] served mixed stream: {jobs} jobs, {total} items, {served:.4g} items/s, p50 {out['p50_latency_ms']:.2f} ms, "
+        f"p99 {out['p99_latency_ms']:.2f} ms; plain one at a time {plain:.4g} items/s ({served / plain:.1f}x)")
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -170,6 +256,9 @@ def main():
     ap.add_argument("--no-fuse", action="store_true", help="run A/B as separate transform batches")
     ap.add_argument("--photons", type=float, default=1e7, help="photon transport form: photons (0 = skip)")
     ap.add_argument("--plain-photons", type=float, default=1e6)
+    ap.add_argument("--serve-jobs", type=int, default=256, help="served mixed stream: jobs (0 = skip)")
+    ap.add_argument("--serve-rate", type=float, default=1000.0, help="served mixed stream: Poisson arrivals per second")
+    ap.add_argument("--serve-max-items", type=float, default=1e6, help="served mixed stream: N ~ log-uniform[1e3, this]")
     ap.add_argument("--copy-calib", type=float, default=0.0,
                     help="after the runs, copy this many MiB device to device once (the known-size kernel that "
                          "scales rocprofv3 FETCH_SIZE / WRITE_SIZE readings to bytes)")
@@ -292,6 +381,8 @@ def main():
     if args.photons > 0:
         forms.update(photon_forms(g, torch, int(args.photons), int(args.plain_photons), C, P, Q,
                                   verify=not args.no_cpu))
+    if args.serve_jobs > 0:
+        forms["served_mixed_stream"] = served_form(g, torch, args.serve_jobs, args.serve_rate, args.serve_max_items)
     result["forms"] = forms
 
     if not args.no_cpu:

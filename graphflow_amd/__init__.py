@@ -151,7 +151,8 @@ SIGNATURES = {
 
 class ServerConfigC(C.Structure):
     _fields_ = [("slots", C.c_uint32), ("team_size", C.c_uint32), ("memory_budget", C.c_uint64),
-                ("max_items", C.c_uint64), ("topology_bytes", C.c_uint64 * 3)]
+                ("max_items", C.c_uint64), ("prewarm", C.c_uint32), ("reserved", C.c_uint32),
+                ("topology_bytes", C.c_uint64 * 3)]
 
 
 class JobDescC(C.Structure):
@@ -699,9 +700,10 @@ class Server:
     slots that run side by side on the device (gf_server_* in include/graphflow.h)."""
 
     def __init__(self, slots: int = 8, team_size: int = 0, memory_budget: int = 0, max_items: int = 0,
-                 ctx: Optional[Context] = None):
+                 prewarm: bool = False, ctx: Optional[Context] = None):
         self.ctx = ctx or default_context()
-        self.cfg = ServerConfigC(slots=slots, team_size=team_size, memory_budget=memory_budget, max_items=max_items)
+        self.cfg = ServerConfigC(slots=slots, team_size=team_size, memory_budget=memory_budget, max_items=max_items,
+                                 prewarm=1 if prewarm else 0)
         h = _V()
         _raise(lib().gf_server_create(self.ctx.handle, C.byref(self.cfg), C.byref(h)))
         self.handle = h

@@ -1273,6 +1273,14 @@ void set_lds(K kernel, size_t bytes)
     if (bytes > 64u * 1024u) HIPX(hipFuncSetAttribute((const void*)kernel, hipFuncAttributeMaxDynamicSharedMemorySize, (int)bytes));
 }
 
+// Stream-ordered host<->device copy on the flow's stream, then wait for it: never touches the
+// legacy stream, so it is safe while another thread captures a graph on its own stream.
+void copy_sync(void* dst, const void* src, size_t bytes, hipMemcpyKind kind, hipStream_t s)
+{
+    HIPX(hipMemcpyAsync(dst, src, bytes, kind, s));
+    HIPX(hipStreamSynchronize(s));
+}
+
 void build_engine(gf_flow& f, uint64_t N, const RunShape& sh, uint32_t src)
 {
     auto eb = std::make_unique<EngineBuffers>();
@@ -1347,7 +1355,7 @@ void build_engine(gf_flow& f, uint64_t N, const RunShape& sh, uint32_t src)
         if (nd.op == GF_SRC_KEYED_ZIPF) {
             auto z = zipf_table(nd.u[0], (double)nd.f[0]);
             double* dz = m.d<double>(z.size());
-            HIPX(hipMemcpy(dz, z.data(), z.size() * sizeof(double), hipMemcpyHostToDevice));
+            copy_sync(dz, z.data(), z.size() * sizeof(double), hipMemcpyHostToDevice, f.ctx->stream);
             e.rp.zipf = dz;
             e.rp.zipf_values = nd.u[0];
         }
@@ -1355,15 +1363,15 @@ void build_engine(gf_flow& f, uint64_t N, const RunShape& sh, uint32_t src)
     if (lds_b > 160u * 1024u || lds_a > 160u * 1024u)
         fail(GF_BAD_ARGUMENT, "graphFlow: partition width exceeds the workgroup LDS budget");
     gf_node_desc* dn = m.d<gf_node_desc>(n);
-    HIPX(hipMemcpy(dn, nds.data(), sizeof(gf_node_desc) * n, hipMemcpyHostToDevice));
+    copy_sync(dn, nds.data(), sizeof(gf_node_desc) * n, hipMemcpyHostToDevice, f.ctx->stream);
     int32_t* dr = m.d<int32_t>(std::max<size_t>(routes.size(), 1));
     int32_t* df = m.d<int32_t>(std::max<size_t>(fuse.size(), 1));
     if (!routes.empty()) {
-        HIPX(hipMemcpy(dr, routes.data(), sizeof(int32_t) * routes.size(), hipMemcpyHostToDevice));
-        HIPX(hipMemcpy(df, fuse.data(), sizeof(int32_t) * fuse.size(), hipMemcpyHostToDevice));
+        copy_sync(dr, routes.data(), sizeof(int32_t) * routes.size(), hipMemcpyHostToDevice, f.ctx->stream);
+        copy_sync(df, fuse.data(), sizeof(int32_t) * fuse.size(), hipMemcpyHostToDevice, f.ctx->stream);
     }
     NodeArr* da = m.d<NodeArr>(n);
-    HIPX(hipMemcpy(da, arr.data(), sizeof(NodeArr) * n, hipMemcpyHostToDevice));
+    copy_sync(da, arr.data(), sizeof(NodeArr) * n, hipMemcpyHostToDevice, f.ctx->stream);
     e.nodes = dn;
     e.routes = dr;
     e.fuse = df;
@@ -1554,7 +1562,7 @@ void run_team(gf_flow& f, EngineBuffers& eb, uint64_t N, gf_stats& st, bool time
             if (timed)
                 for (auto& x : ev) (void)hipEventDestroy(x);
             unsigned long long step = 0;
-            HIPX(hipMemcpy(&step, eb.e.ctr + CNT_STEP, sizeof(step), hipMemcpyDeviceToHost));
+            copy_sync(&step, eb.e.ctr + CNT_STEP, sizeof(step), hipMemcpyDeviceToHost, f.ctx->stream);
             st.handoff_step = (uint32_t)step;
             st.engine = 4;
             hipLaunchKernelGGL(k_engine_resume, dim3(1), dim3(kThreads), sizeof(SelLds), s, eb.e);
@@ -1671,11 +1679,11 @@ void run_wide(gf_flow& f, EngineBuffers& eb, uint64_t N, gf_stats& st, bool time
 void finish_device(gf_flow& f, EngineBuffers& eb, gf_stats& st, const uint32_t* ctl, uint64_t launches)
 {
     unsigned long long ctr[CNT_WORDS];
-    HIPX(hipMemcpy(ctr, eb.e.ctr, sizeof(ctr), hipMemcpyDeviceToHost));
+    copy_sync(ctr, eb.e.ctr, sizeof(ctr), hipMemcpyDeviceToHost, f.ctx->stream);
     if (f.tuning.trace) {
         uint64_t len = std::min<uint64_t>(ctr[CNT_TRACE_LEN], eb.e.trace_cap);
         f.trace.resize(len);
-        if (len) HIPX(hipMemcpy(f.trace.data(), eb.e.trace, len * sizeof(uint32_t), hipMemcpyDeviceToHost));
+        if (len) copy_sync(f.trace.data(), eb.e.trace, len * sizeof(uint32_t), hipMemcpyDeviceToHost, f.ctx->stream);
     }
     st.completed_batches = ctr[CNT_BATCHES];
     st.steps = ctr[CNT_STEP];

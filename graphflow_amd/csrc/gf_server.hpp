@@ -76,6 +76,7 @@ struct gf_server {
     std::unordered_map<uint64_t, std::vector<double>> tallies;
     uint64_t next_id = 1, submitted = 0, finished = 0;
     uint64_t mem_in_use = 0;
+    uint32_t warm = 0;  // slots done with their prewarm
     bool stop = false;
     std::vector<std::thread> workers;
     std::chrono::steady_clock::time_point t0;
@@ -179,9 +180,7 @@ SlotFlow make_slot_flow(gf_server& sv, gf_context* ctx, uint32_t topo, uint32_t 
     t.live_particles = topo == GF_JOB_PHOTON ? kServePhotonLive : kServeLive;
     t.schedule = GF_SCHEDULE_DEVICE;
     t.fuse_transforms = 1;
-    // direct launches: a graph capture on one slot's stream would make the synchronous copies of
-    // another slot's engine build fail while it lasts
-    t.use_graph = 0;
+    t.use_graph = 1;
     const bool team = topo != GF_JOB_HASH;
     t.device_engine = team ? 3u : 1u;
     t.max_workgroups = team ? sv.cfg.team_size : 0u;
@@ -264,6 +263,30 @@ void server_worker(gf_server* sv, uint32_t slot)
         return;
     }
     SlotFlow flows[3];
+    if (sv->cfg.prewarm) {
+        // build every topology's flow and engine up front (one tiny job each), within the budget
+        for (uint32_t tp = 0; tp < 3u; ++tp) {
+            {
+                std::lock_guard<std::mutex> lk(sv->mu);
+                if (sv->mem_in_use + sv->cfg.topology_bytes[tp] > sv->cfg.memory_budget) continue;
+                sv->mem_in_use += sv->cfg.topology_bytes[tp];
+            }
+            std::string msg;
+            gf_job_result r{};
+            std::vector<double> tally;
+            ServerJob w{0, gf_job_desc{tp, 0u, 64u, 1u}, 0.0};
+            if (guard(&msg, [&] {
+                    flows[tp] = make_slot_flow(*sv, ctx, tp, slot);
+                    run_job(*sv, ctx, flows[tp], w, r, &tally);
+                }) != GF_OK) {
+                std::lock_guard<std::mutex> lk(sv->mu);
+                sv->err = msg;
+            }
+        }
+        std::lock_guard<std::mutex> lk(sv->mu);
+        sv->warm++;
+    }
+    sv->cv.notify_all();
     for (;;) {
         ServerJob j;
         uint64_t need = 0;
@@ -340,6 +363,10 @@ gf_status gf_server_create(gf_context* ctx, const gf_server_config* cfg, gf_serv
         sv->t0 = std::chrono::steady_clock::now();
         gf_server* p = sv.get();
         for (uint32_t i = 0; i < c.slots; ++i) p->workers.emplace_back(server_worker, p, i);
+        if (c.prewarm) {
+            std::unique_lock<std::mutex> lk(p->mu);
+            p->cv.wait(lk, [&] { return p->warm == c.slots || !p->err.empty(); });
+        }
         *out = sv.release();
     });
 }

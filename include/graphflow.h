@@ -215,6 +215,8 @@ typedef struct gf_server_config {
     uint32_t team_size;         /* team engine workgroups per job (0 -> CUs of one XCD minus 4)     */
     uint64_t memory_budget;     /* device bytes the slots' flows may hold (0 -> 64 GiB)             */
     uint64_t max_items;         /* largest feedback job (0 -> 1e7; sizes its result buffer)         */
+    uint32_t prewarm;           /* 1: every slot builds its three flows before create returns       */
+    uint32_t reserved;
     uint64_t topology_bytes[3]; /* out: device bytes a slot holds per topology                      */
 } gf_server_config;
 
