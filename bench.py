@@ -45,6 +45,16 @@ def build_feedback(g, torch, N, C, P, Q, schedule, seed=42, use_graph=1, fuse=Fa
     return flow, src, counts
 
 
+def tuning_of(g, N, C, P, Q, fuse, args, tuned, use_graph):
+    """The timed configuration again (the autotuner's pick when it ran), with another use_graph."""
+    eng = tuned["device_engine"] if tuned else args.engine
+    wg = tuned["max_workgroups"] if tuned else (args.team or None)
+    tile = tuned["tile_items"] if tuned else None
+    return g.Tuning(seed=42, batch_chunks=Q, chunk_capacity=C, live_particles=P, schedule=g.SCHEDULE_DEVICE,
+                    use_graph=use_graph, fuse_transforms=fuse, device_engine=eng, max_workgroups=wg or None,
+                    tile_items=tile or None)
+
+
 def run_once(flow, src, counts, N, torch):
     counts.zero_()
     torch.cuda.synchronize()
@@ -251,6 +261,7 @@ def main():
     ap.add_argument("--verify", action="store_true", help="compare the sorted histogram with the oracle")
     ap.add_argument("--engine", type=int, default=0, help="device engine: 0 autotuned, 1 wide, 2 micro, 3 team")
     ap.add_argument("--team", type=int, default=0, help="team engine: workgroups in the team (0 = default)")
+    ap.add_argument("--no-autotune", action="store_true", help="skip the autotuner's probe runs")
     ap.add_argument("--phases", action="store_true", help="device engines: one more run with the per-phase profile")
     ap.add_argument("--members", action="store_true", help="team engine: with --phases, per-member phase times")
     ap.add_argument("--no-fuse", action="store_true", help="run A/B as separate transform batches")
@@ -284,6 +295,10 @@ def main():
     log(f"[bench] batched form N={N} C={C} P={P} Q={Q}")
     flow, src, counts = build_feedback(g, torch, N, C, P, Q, g.SCHEDULE_DEVICE, fuse=fuse, engine=args.engine,
                                        team=args.team)
+    tuned = None
+    if not args.no_autotune and args.engine == 0 and not args.team:
+        tuned = flow.autotune(src, probe_items=min(N, 2_000_000))
+        log(f"[bench] autotuner picked {tuned}")
     for i in range(args.warmup):
         dt, st = run_once(flow, src, counts, N, torch)
         log(f"[bench] warmup {i}: {dt:.3f} s, {N / dt:.4g} particles/s, steps={st['steps']}")
@@ -309,7 +324,8 @@ def main():
                             + (", A/B fused into the filter scatter" if fuse else "")),
                    "engine_steps": int(st["steps"]),
                    "partition_item_passes": int(part_items), "transform_item_passes": int(xf_items),
-                   "histogram_rows": int(len(rows)), "histogram_sum": row_sum, "histogram_sha256": sha},
+                   "histogram_rows": int(len(rows)), "histogram_sum": row_sum, "histogram_sha256": sha,
+                   "autotuned": tuned},
     }
 
     if args.verify:
@@ -323,9 +339,7 @@ def main():
 
     if not args.no_roofline:
         # one extra run with HIP events around every engine kernel, on the engine's own stream
-        flow.configure(g.Tuning(seed=42, batch_chunks=Q, chunk_capacity=C, live_particles=P,
-                                schedule=g.SCHEDULE_DEVICE, use_graph=2, fuse_transforms=fuse,
-                                device_engine=args.engine, max_workgroups=args.team or None))
+        flow.configure(tuning_of(g, N, C, P, Q, fuse, args, tuned, use_graph=2))
         counts.zero_()
         tst = flow.run(src, N)
         ms, n = flow.kernel_times()
@@ -338,7 +352,9 @@ def main():
             k_ms, k_n, kname = ms[3], n[3], "k_team (whole steps, team of workgroups on one XCD)"
             others = {"steps_per_launch": int(tst["steps"]) / max(n[3], 1),
                       "avg_step_us": ms[3] * 1e3 / max(int(tst["steps"]), 1)}
-            traffic = traffic_for(wkey, int(k_n))
+            kteam = (tuned or {}).get("max_workgroups") or args.team or 32
+            traffic = traffic_for(wkey.replace("team=xcd", f"team={kteam}").replace(f"team={args.team}", f"team={kteam}"),
+                                  int(k_n))
             if traffic is not None:
                 others["traffic_source"] = os.path.relpath(TRAFFIC_FILE, os.path.dirname(os.path.abspath(__file__)))
         elif n[3]:  # micro engine: one persistent kernel does every phase
@@ -359,9 +375,7 @@ def main():
 
     if args.phases:
         # per-phase device profile (slows the profiled member; not used for any reported rate)
-        flow.configure(g.Tuning(seed=42, batch_chunks=Q, chunk_capacity=C, live_particles=P,
-                                schedule=g.SCHEDULE_DEVICE, use_graph=3, fuse_transforms=fuse,
-                                device_engine=args.engine, max_workgroups=args.team or None))
+        flow.configure(tuning_of(g, N, C, P, Q, fuse, args, tuned, use_graph=3))
         counts.zero_()
         pst = flow.run(src, N)
         phases, psteps = flow.phase_profile()

@@ -166,6 +166,7 @@ class JobResultC(C.Structure):
 
 
 SIGNATURES.update({
+    "gf_flow_autotune": (C.c_int, [_V, C.c_uint32, C.c_uint64, _P(TuningC), _P(C.c_double)]),
     "gf_server_create": (C.c_int, [_V, _P(ServerConfigC), _P(_V)]),
     "gf_server_submit": (C.c_int, [_V, _P(JobDescC), _P(C.c_uint64)]),
     "gf_server_poll": (C.c_int, [_V, _P(JobResultC), C.c_uint32, _P(C.c_uint32), C.c_uint32]),
@@ -570,6 +571,17 @@ class Flow:
         rc = lib().gf_flow_run(self.handle, source.key, items, C.byref(st))
         _raise(rc, self._err() if rc else "")
         return st.as_dict()
+
+    def autotune(self, source: NodeHandle, probe_items: int = 0) -> dict:
+        """Pick the fastest device-schedule configuration for the knobs left open (device_engine 0,
+        batch_chunks unset) by probe runs; configures the flow with it. The probe runs write the
+        flow's sinks (clear them before the real run)."""
+        t = TuningC()
+        ms = C.c_double()
+        rc = lib().gf_flow_autotune(self.handle, source.key, probe_items, C.byref(t), C.byref(ms))
+        _raise(rc, self._err() if rc else "")
+        return {"device_engine": t.device_engine, "max_workgroups": t.max_workgroups, "batch_chunks": t.batch_chunks,
+                "tile_items": t.tile_items, "probe_ms": ms.value}
 
     def trace(self):
         import numpy as np

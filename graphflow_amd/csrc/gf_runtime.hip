@@ -2031,6 +2031,84 @@ gf_status gf_flow_configure(gf_flow* flow, const gf_tuning* t)
     });
 }
 
+// Autotuner: probe runs over the device-schedule knobs the caller left open (engine 0, Q 0), on
+// this flow's own graph, payload and routing; the fastest candidate configures the flow.
+// Candidates: team engine with 16/24/32 workgroups (at most the CUs of one XCD), wide engine with
+// 1024- or 4096-item tiles and 64 or 256 workgroups, the one-workgroup engine; Q in {4, 8, 16}
+// when unset. None of them changes a result: every engine runs the same step semantics, and Q
+// changes batch composition but no particle's trajectory.
+gf_status gf_flow_autotune(gf_flow* flow, uint32_t source, uint64_t probe_items, gf_tuning* chosen, double* best_ms)
+{
+    if (!flow || !chosen) return GF_BAD_ARGUMENT;
+    return guard(&flow->err, [&] {
+        check_graph(*flow, source);
+        const gf_tuning base = flow->tuning;
+        if (base.schedule != GF_SCHEDULE_DEVICE) {
+            *chosen = base;
+            return;
+        }
+        const uint64_t n = probe_items ? probe_items : (1ull << 21);
+        std::vector<gf_tuning> cand;
+        std::vector<uint32_t> qs;
+        if (base.batch_chunks) qs.push_back(base.batch_chunks);
+        else qs = {4u, 8u, 16u};
+        const uint32_t xcd_cus = std::max<uint32_t>(1u, std::min<uint32_t>(32u, flow->ctx->cus / 8u));
+        for (uint32_t q : qs) {
+            gf_tuning t = base;
+            t.batch_chunks = q;
+            t.use_graph = base.use_graph >= 2u ? 1u : base.use_graph;
+            if (base.device_engine == 0u || base.device_engine == 3u)
+                for (uint32_t k : {16u, 24u, 32u})
+                    if (k <= xcd_cus || k == 16u) {
+                        gf_tuning c = t;
+                        c.device_engine = 3u;
+                        c.max_workgroups = std::min(k, xcd_cus);
+                        cand.push_back(c);
+                    }
+            if (base.device_engine == 0u || base.device_engine == 1u)
+                for (uint32_t ti : {1024u, 4096u})
+                    for (uint32_t wg : {64u, 256u}) {
+                        gf_tuning c = t;
+                        c.device_engine = 1u;
+                        c.tile_items = ti;
+                        c.max_workgroups = wg;
+                        cand.push_back(c);
+                    }
+            if (base.device_engine == 0u || base.device_engine == 2u) {
+                gf_tuning c = t;
+                c.device_engine = 2u;
+                cand.push_back(c);
+            }
+        }
+        double best = 1e300;
+        gf_tuning pick = base;
+        for (const gf_tuning& c : cand) {
+            flow->tuning = c;
+            gf_stats st{};
+            double ms = 1e300;
+            const gf_status rc = guard(nullptr, [&] {
+                for (int rep = 0; rep < 2; ++rep) {  // the first run builds the engine
+                    RunShape sh = resolve(*flow, n);
+                    ensure_pool(*flow, sh);
+                    auto t0 = std::chrono::steady_clock::now();
+                    run_device(*flow, source, n, sh, st, false);
+                    HIPX(hipStreamSynchronize(flow->ctx->stream));
+                    ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
+                }
+            });
+            if (rc == GF_OK && ms < best) {
+                best = ms;
+                pick = c;
+            }
+        }
+        if (best >= 1e300) fail(GF_BAD_ARGUMENT, "autotune: no candidate ran");
+        flow->tuning = pick;
+        flow->tuning.use_graph = base.use_graph;
+        *chosen = flow->tuning;
+        if (best_ms) *best_ms = best;
+    });
+}
+
 gf_status gf_flow_run(gf_flow* flow, uint32_t source, uint64_t items, gf_stats* stats)
 {
     if (!flow) return GF_BAD_ARGUMENT;
