@@ -1,0 +1,9 @@
+#!/bin/sh
+# Builds the C++ Flow API example against the in-tree library (tests/cpp/_build/flow_example).
+set -e
+HERE=$(cd "$(dirname "$0")" && pwd)
+ROOT=$(cd "$HERE/../.." && pwd)
+mkdir -p "$HERE/_build"
+g++ -std=c++20 -O2 -Wall -D__HIP_PLATFORM_AMD__ -I"$ROOT/include" -I/opt/rocm/include "$HERE/flow_example.cpp" \
+    -L"$ROOT/graphflow_amd" -lgraphflow_amd -L/opt/rocm/lib -lamdhip64 \
+    -Wl,-rpath,"\$ORIGIN/../../../graphflow_amd" -Wl,-rpath,/opt/rocm/lib -o "$HERE/_build/flow_example"
