@@ -19,10 +19,10 @@
 // seeing any other member's records. The scheduling rules are those of engine_select/engine_plan/
 // engine_retire (sched/GraphRunner.hpp:902-1133 applied once per node per step).
 //
-// Per step:   barrier -> counts (+ records prefetched) -> plan -> scatter (waves 0-2) while wave 3
+// Per step:   barrier -> counts (+ records prefetched) -> plan -> scatter (waves 0-5) while wave 6
 // retires the step and selects the next one into the other half of the double-buffered batch
-// state -> wait for the stores -> arrive. One wave per SIMD, so the scheduler never shares issue
-// slots with a scattering wave.
+// state, and wave 7 runs the sink segments -> wait for the stores -> arrive. Two waves per SIMD:
+// a member has enough worker waves that most of them scatter one segment per step.
 //
 // Coherence: stores are plain vector stores (they write through the CU's L1 into the XCD's L2);
 // every load of data written by another member in the same launch is a relaxed agent-scope load,
@@ -50,10 +50,17 @@ constexpr uint32_t kTeamOut = 512;     // output chunks of one step
 constexpr uint32_t kTeamSegs = 1024;   // segments of one step
 constexpr uint32_t kTeamPrefix = 4096; // [segment][bin] words of the step's partitions
 constexpr uint32_t kTeamZero = 512;    // consumed count rows cleared in the next step
-constexpr uint32_t kTeamWaves = 5;     // waves 0-2 scatter segments, wave 3 schedules, wave 4 sinks
-constexpr uint32_t kWorkers = 3;
-constexpr uint32_t kSchedWave = 3;
-constexpr uint32_t kSinkWave = 4;
+constexpr uint32_t kTeamWaves = 8;     // waves 0-5 scatter segments, wave 6 schedules, wave 7 sinks
+constexpr uint32_t kWorkers = 6;
+constexpr uint32_t kSchedWave = 6;
+constexpr uint32_t kSinkWave = 7;
+// Order in which the waves take the bins of the count scan: first the waves that issued no
+// prefetch (wave 0, the scheduler and the sink wave), whose count loads are not queued behind
+// record loads.
+__device__ constexpr uint32_t scan_rank(uint32_t w)
+{
+    return w == 0u ? 0u : w >= kWorkers ? w - kWorkers + 1u : w + (kTeamWaves - kWorkers);
+}
 constexpr uint32_t kClrThreads = 64u * 4u;  // waves that wait for their stores before the barrier
 constexpr uint32_t kTeamThreads = 64u * kTeamWaves;
 constexpr uint32_t kTeamSpin = 1u << 24;
@@ -381,7 +388,7 @@ __device__ inline void team_flush(const TeamArgs& a, TeamLds& L, const StepBuf* 
     }
 }
 
-// ---- scheduler registers (wave 3) ---------------------------------------------------------------
+// ---- scheduler registers (wave 6) ---------------------------------------------------------------
 // Lane k = node k, kept in registers for the whole launch: ring cursors, the fill and segment sums
 // of the node's ready chunks, the sink digest count and the node's static kind and bins.
 struct SchedRegs {
@@ -439,7 +446,7 @@ __device__ inline void sched_store(const TeamArgs& a, TeamLds& L, const SchedReg
     wave_sync();
 }
 
-// ---- select (wave 3): the next step's batches into Bn, or the reason to stop ------------------
+// ---- select (wave 6): the next step's batches into Bn, or the reason to stop ------------------
 // Credits are handed out in node order by a scalar loop over the nodes that may take a batch; a
 // node that takes its whole ring needs no ring reads (the ring sums are in registers).
 __device__ inline void team_select(const TeamArgs& a, TeamLds& L, StepBuf& Bn, bool at_limit, SchedRegs& g)
@@ -728,7 +735,7 @@ __device__ inline void team_countplan(const TeamArgs& a, TeamLds& L, const StepB
             }
             return (size_t)r + (size_t)(gs - f) * a.bb;
         };
-        for (uint32_t b = w; b < B1; b += kTeamWaves) {
+        for (uint32_t b = scan_rank(w); b < B1; b += kTeamWaves) {
             uint32_t carry = 0;
             for (uint32_t g0 = 0; g0 < ns; g0 += 128u) {
                 const uint32_t ga = g0 + lane, gb = ga + 64u;
@@ -883,7 +890,7 @@ __device__ inline void team_countplan(const TeamArgs& a, TeamLds& L, const StepB
     __syncthreads();
 }
 
-// ---- trace (member 0) and retire (wave 3 of every member) --------------------------------------
+// ---- trace (member 0) and retire (wave 6 of every member) --------------------------------------
 __device__ inline void team_trace(const TeamArgs& a, TeamLds& L, const StepBuf& Bc)
 {
     const Engine& e = a.e;
@@ -927,7 +934,7 @@ __device__ inline void team_trace(const TeamArgs& a, TeamLds& L, const StepBuf& 
     }
 }
 
-// Retire (wave 3 of every member): outputs to the destination rings in node and output order,
+// Retire (wave 6 of every member): outputs to the destination rings in node and output order,
 // consumed inputs back to the free stack (generation + 1), counters.
 __device__ inline void team_retire(const TeamArgs& a, TeamLds& L, StepBuf& Bc, SchedRegs& g)
 {
@@ -985,7 +992,7 @@ __device__ inline void seg_fail(const TeamArgs& a, const TeamLds& L, uint32_t co
     const_cast<TeamLds&>(L).seg_err = 1u;
 }
 
-// ---- segment work (waves 0-3) ---------------------------------------------------------------------
+// ---- segment work (waves 0-5 and the sink wave) ---------------------------------------------------------------------
 // Next-node count key of a record stored at (chunk d, offset off), or ~0u when the destination is
 // not a partition.
 template <class P>
