@@ -158,6 +158,30 @@ def photon_forms(g, torch, N, Np, C, P, Q, verify):
     return out
 
 
+def hash_form(g, torch, N):
+    """Keyed hash pipeline (Zipf(1.1) keys over 2^20 values, murmur3 finalizer into 4096 buckets,
+    C=8192) at Q in {8, 16}: keys/s on the device engine, one partition pass per key (24 B of
+    algorithmic traffic per key)."""
+    out = {}
+    bins = 4096
+    ms = torch.zeros(2 * bins, dtype=torch.int64, device="cuda")
+    for Q in (8, 16):
+        f = g.Flow(g.PAYLOAD_KEYED)
+        src = f.source("keys", g.ZipfKeys(1 << 20, 1.1))
+        part = f.partition("bucket", g.KeyBucket(bins))
+        snk = f.sink("digest", g.ChunkDigest(0, 0, ms.data_ptr(), bins))
+        f.connect(src, part)
+        for b in range(bins):
+            f.connect(part, snk, b)
+        f.configure(g.Tuning(seed=42, batch_chunks=Q, chunk_capacity=8192, schedule=g.SCHEDULE_DEVICE))
+        f.run(src, 100000)
+        dt, st = run_once(f, src, ms, N, torch)
+        out[f"hash_pipeline_B4096_Q{Q}"] = {"value": N / dt, "unit": "keys/s", "items": N, "engine_steps": int(st["steps"]),
+                                           "engine": int(st["engine"])}
+        log(f"[bench] hash pipeline B=4096 Q={Q} N={N}: {N / dt:.4g} keys/s ({int(st['steps'])} steps)")
+    return out
+
+
 def served_form(g, torch, jobs, rate, max_items, seed=42):
     """Served mixed stream: `jobs` independent flow jobs, Poisson arrivals at `rate` per second,
     N ~ log-uniform[1e3, max_items], topology uniform over {particle feedback, keyed hash pipeline,
@@ -267,6 +291,7 @@ def main():
     ap.add_argument("--no-fuse", action="store_true", help="run A/B as separate transform batches")
     ap.add_argument("--photons", type=float, default=1e7, help="photon transport form: photons (0 = skip)")
     ap.add_argument("--plain-photons", type=float, default=1e6)
+    ap.add_argument("--hash-items", type=float, default=4e6, help="keyed hash pipeline form: keys (0 = skip)")
     ap.add_argument("--serve-jobs", type=int, default=256, help="served mixed stream: jobs (0 = skip)")
     ap.add_argument("--serve-rate", type=float, default=1000.0, help="served mixed stream: Poisson arrivals per second")
     ap.add_argument("--serve-max-items", type=float, default=1e6, help="served mixed stream: N ~ log-uniform[1e3, this]")
@@ -395,6 +420,8 @@ def main():
     if args.photons > 0:
         forms.update(photon_forms(g, torch, int(args.photons), int(args.plain_photons), C, P, Q,
                                   verify=not args.no_cpu))
+    if args.hash_items > 0:
+        forms.update(hash_form(g, torch, int(args.hash_items)))
     if args.serve_jobs > 0:
         forms["served_mixed_stream"] = served_form(g, torch, args.serve_jobs, args.serve_rate, args.serve_max_items)
     result["forms"] = forms
