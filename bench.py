@@ -175,11 +175,11 @@ def hash_form(g, torch, N):
         for b in range(bins):
             f.connect(part, snk, b)
         f.configure(g.Tuning(seed=42, batch_chunks=Q, chunk_capacity=8192, schedule=g.SCHEDULE_DEVICE))
-        f.run(src, 100000)
+        tuned = f.autotune(src, probe_items=min(N, 1_000_000))
         dt, st = run_once(f, src, ms, N, torch)
         out[f"hash_pipeline_B4096_Q{Q}"] = {"value": N / dt, "unit": "keys/s", "items": N, "engine_steps": int(st["steps"]),
-                                           "engine": int(st["engine"])}
-        log(f"[bench] hash pipeline B=4096 Q={Q} N={N}: {N / dt:.4g} keys/s ({int(st['steps'])} steps)")
+                                           "engine": int(st["engine"]), "autotuned": tuned}
+        log(f"[bench] hash pipeline B=4096 Q={Q} N={N}: {N / dt:.4g} keys/s ({int(st['steps'])} steps, {tuned})")
     return out
 
 

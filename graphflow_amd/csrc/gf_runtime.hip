@@ -2034,7 +2034,7 @@ gf_status gf_flow_configure(gf_flow* flow, const gf_tuning* t)
 // Autotuner: probe runs over the device-schedule knobs the caller left open (engine 0, Q 0), on
 // this flow's own graph, payload and routing; the fastest candidate configures the flow.
 // Candidates: team engine with 16/24/32 workgroups (at most the CUs of one XCD), wide engine with
-// 1024- or 4096-item tiles and 64 or 256 workgroups, the one-workgroup engine; Q in {4, 8, 16}
+// 1024- or 4096-item tiles and 64, 256 or 1024 workgroups, the one-workgroup engine; Q in {4, 8, 16}
 // when unset. None of them changes a result: every engine runs the same step semantics, and Q
 // changes batch composition but no particle's trajectory.
 gf_status gf_flow_autotune(gf_flow* flow, uint32_t source, uint64_t probe_items, gf_tuning* chosen, double* best_ms)
@@ -2067,7 +2067,7 @@ gf_status gf_flow_autotune(gf_flow* flow, uint32_t source, uint64_t probe_items,
                     }
             if (base.device_engine == 0u || base.device_engine == 1u)
                 for (uint32_t ti : {1024u, 4096u})
-                    for (uint32_t wg : {64u, 256u}) {
+                    for (uint32_t wg : {64u, 256u, 1024u}) {
                         gf_tuning c = t;
                         c.device_engine = 1u;
                         c.tile_items = ti;

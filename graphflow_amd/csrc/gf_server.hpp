@@ -183,7 +183,8 @@ SlotFlow make_slot_flow(gf_server& sv, gf_context* ctx, uint32_t topo, uint32_t 
     t.use_graph = 1;
     const bool team = topo != GF_JOB_HASH;
     t.device_engine = team ? 3u : 1u;
-    t.max_workgroups = team ? sv.cfg.team_size : 0u;
+    // the hash pipeline's wide engine: a wide grid spreads the sink's many small per-bucket chunks
+    t.max_workgroups = team ? sv.cfg.team_size : 256u;
     t.team_xcd = team ? 1u + slot % 8u : 0u;
     ck(gf_flow_configure(f, &t), "configure");
     const RunShape sh = resolve(*f, t.live_particles);

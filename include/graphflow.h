@@ -189,7 +189,7 @@ gf_status gf_flow_configure(gf_flow* flow, const gf_tuning* tuning);
 gf_status gf_flow_run(gf_flow* flow, uint32_t source, uint64_t items, gf_stats* stats);
 /* autotuner (no counterpart in the reference): probe runs of probe_items (0 -> 2^21) over the
  * device-schedule knobs left open in the flow's tuning (device_engine 0: team sizes, wide-engine
- * tile sizes and grids, the one-workgroup engine; batch_chunks 0: Q in {4, 8, 16}); the fastest
+ * tile sizes and grids of 64/256/1024 workgroups, the one-workgroup engine; batch_chunks 0: Q in {4, 8, 16}); the fastest
  * configures the flow and is returned with its probe time. The probe runs write the flow's sinks. */
 gf_status gf_flow_autotune(gf_flow* flow, uint32_t source, uint64_t probe_items, gf_tuning* chosen, double* best_ms);
 gf_status gf_flow_last_error(gf_flow* flow, char* buffer, uint32_t size);
