@@ -84,12 +84,12 @@ def pmc_summary(fetch_csv, write_csv, workload, out):
         return d
 
     lines = ["# HBM traffic of the team engine kernel (k_team), from two rocprofv3 --pmc passes:",
-             "#   cd /tmp && rocprofv3 --pmc FETCH_SIZE --output-format csv -d $R/gpurun_out/pmc_fetch -o run -- "
+             "#   cd /tmp && rocprofv3 --pmc FETCH_SIZE --output-format csv -d $OUT/pmc_fetch -o run -- "
              "python $R/bench.py --no-cpu --no-plain --photons 0 --serve-jobs 0 --hash-items 0 --no-autotune --no-roofline "
              "--steps 1 --warmup 0 --items 1e8 --copy-calib 1024",
-             "#   (the same with --pmc WRITE_SIZE into $R/gpurun_out/pmc_write; R = the repository root)",
-             "#   python bench.py --pmc-summary gpurun_out/pmc_fetch/run_counter_collection.csv "
-             "gpurun_out/pmc_write/run_counter_collection.csv <this file>",
+             "#   (the same with --pmc WRITE_SIZE into $OUT/pmc_write; R = the repository root, OUT = an output directory)",
+             "#   python bench.py --pmc-summary $OUT/pmc_fetch/run_counter_collection.csv "
+             "$OUT/pmc_write/run_counter_collection.csv <this file>",
              f"workload={workload}"]
     per = {}
     for counter, path in (("FETCH_SIZE", fetch_csv), ("WRITE_SIZE", write_csv)):
@@ -314,7 +314,7 @@ def main():
 
     if args.gpus != 1:
         print(json.dumps({"error": "this build runs on one GPU; --gpus above 1 is not supported "
-                                   "(the path shards by particle-id range, see DESIGN.md)"}))
+                                   "(the path would shard by particle-id range, one flow per GPU)"}))
         sys.exit(2)
 
     import torch

@@ -1101,7 +1101,7 @@ int or_partition_items(const or_node_desc* node, const uint32_t* lut, uint64_t s
     return OR_OK;
 }
 
-/* Schedule-free feedback model (tests/reference/ScalarModel.hpp:84-133 pattern, BASELINE functors). */
+/* Schedule-free feedback model (tests/reference/ScalarModel.hpp:84-133 pattern, feedback-network functors). */
 int or_feedback_scalar(uint64_t seed, uint64_t items, float threshold, uint32_t distribution, uint32_t ndest,
                        const float* multipliers, uint32_t* weight_bits, uint32_t* gens, uint32_t threads)
 {

@@ -17,7 +17,7 @@
  *   - transform batch            include/graphFlow/flow/Execution.hpp:708-776, 1805-1901
  *   - scheduler loop             include/graphFlow/sched/GraphRunner.hpp:716-1257
  *   - pool derivation            include/graphFlow/flow/Flow.hpp:580-626
- * with the BASELINE functors (fp32 feedback network, murmur3/Zipf hash pipeline, layered-slab
+ * with the benchmark functors (fp32 feedback network, murmur3/Zipf hash pipeline, layered-slab
  * photon transport in FP64).
  *
  * Pinning status of every function is listed in the header comment of gf_oracle.c.

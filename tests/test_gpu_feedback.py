@@ -1,4 +1,4 @@
-"""GPU parity of the fp32 feedback network (BASELINE configs 1 and 2): every planned batch
+"""GPU parity of the fp32 feedback network (N=1e6 plain and N=1e7 batched workloads): every planned batch
 (inputs, per-bin counts, output chunk indices, generations, fills, bins) and every final weight
 must equal the oracle's, for the plain host-planned form and both device engines."""
 import numpy as np
@@ -49,7 +49,7 @@ def test_feedback_trace_and_weights(N, C, P, Q, schedule, fuse, engine):
 
 
 def test_config1_histogram_rows_and_sha():
-    """N=1e6, C=1024, P=2048, Q=1, seed 42 on the plain form; sorted histogram
+    """Workload N=1e6, C=1024, P=2048, Q=1, seed 42 on the plain form; sorted histogram
     rows bit-exact with the oracle, rows summing to N."""
     N = 1000000
     flow, src, counts = feedback_flow(N, sink="hist")

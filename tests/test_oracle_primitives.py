@@ -268,7 +268,7 @@ def test_feedback_network_matches_schedule_free_model(mode):
     # S = P + K*R: Filter outputChunkBound(C, C, 3) = 3, A and B one each (Flow.hpp:580-626)
     assert r["pool_chunks"] == 2048 + 5
     assert r["sunk"] == n
-    # mean passes: A-routed particles need ~9 halvings, B ~21 (BASELINE functors)
+    # mean passes: A-routed particles need ~9 halvings, B ~21 (feedback-network functors)
     assert 13.0 < gens.mean() < 17.0
 
 
