@@ -10,19 +10,23 @@
 //   * whoever writes a record into a chunk that is routed to a partition node also classifies the
 //     record with that node's classifier and adds one to cnt[chunk][gen&1][segment][bin] (a
 //     wave-aggregated atomic; the last slot of the row counts invalid bins);
-//   * when the partition node later takes the chunk, every member reads those counts right
-//     after the barrier and derives the exact plan (per-bin totals, output chunk counts, the flat
-//     start of every (segment, bin)) while the records it will scatter are still in flight.
+//   * when the partition node later takes the chunk, the counts give the exact plan: per-bin
+//     totals, output chunk counts and the flat start of every (segment, bin).
 //
 // The stable canonical order (bin-major, then batch input order) follows from the per-segment
 // prefixes plus a wave-level rank inside each segment, so a member scatters its segments without
 // seeing any other member's records. The scheduling rules are those of engine_select/engine_plan/
 // engine_retire (sched/GraphRunner.hpp:902-1133 applied once per node per step).
 //
-// Per step:   barrier -> counts (+ records prefetched) -> plan -> scatter (waves 0-5) while wave 6
-// retires the step and selects the next one into the other half of the double-buffered batch
-// state, and wave 7 runs the sink segments -> wait for the stores -> arrive. Two waves per SIMD:
-// a member has enough worker waves that most of them scatter one segment per step.
+// Per step, after the barrier, with no workgroup barrier until the step's end:
+//   * waves 0-5 each load their segment's records and the count rows of the segment's node, derive
+//     from them alone where the segment's items go (bin totals -> output chunk of each bin, the
+//     prefix of the segment's earlier items -> slot) and scatter;
+//   * wave 6 loads the same count rows, plans the whole step (pops the output chunks), retires it
+//     and selects the next step into the other half of the double-buffered batch state;
+//   * wave 7 runs the sink segments.
+// Then every wave waits for its stores, the member arrives, and wave 6 pushes the step's inputs on
+// the free stack (after the workgroup barrier, because the scatter waves read the popped entries).
 //
 // Coherence: stores are plain vector stores (they write through the CU's L1 into the XCD's L2);
 // every load of data written by another member in the same launch is a relaxed agent-scope load,
@@ -48,26 +52,22 @@ constexpr uint32_t kTeamMargin = 512;  // pops or pushes of one step
 constexpr uint32_t kTeamIn = 512;      // input chunks of one step
 constexpr uint32_t kTeamOut = 512;     // output chunks of one step
 constexpr uint32_t kTeamSegs = 1024;   // segments of one step
-constexpr uint32_t kTeamPrefix = 4096; // [segment][bin] words of the step's partitions
 constexpr uint32_t kTeamZero = 512;    // consumed count rows cleared in the next step
 constexpr uint32_t kTeamWaves = 8;     // waves 0-5 scatter segments, wave 6 schedules, wave 7 sinks
 constexpr uint32_t kWorkers = 6;
 constexpr uint32_t kSchedWave = 6;
 constexpr uint32_t kSinkWave = 7;
-// Order in which the waves take the bins of the count scan: first the waves that issued no
-// prefetch (wave 0, the scheduler and the sink wave), whose count loads are not queued behind
-// record loads.
-__device__ constexpr uint32_t scan_rank(uint32_t w)
-{
-    return w == 0u ? 0u : w >= kWorkers ? w - kWorkers + 1u : w + (kTeamWaves - kWorkers);
-}
-constexpr uint32_t kClrThreads = 64u * 4u;  // waves that wait for their stores before the barrier
+constexpr uint32_t kClrThreads = 64u * 4u;  // threads of member 0 that spill the free-stack window
 constexpr uint32_t kTeamThreads = 64u * kTeamWaves;
 constexpr uint32_t kTeamSpin = 1u << 24;
 
-// Team words: the team's XCC (+1), the step barrier, the per-step-parity error words, the closed
-// team size (bit 31 set once the leader closed registration) and one registration counter per XCC.
-enum { TW_XCC0 = 1, TW_BAR = 2, TW_ERR = 4, TW_TEAM = 6, TW_REG = 8, TW_WORDS = 32 };
+// Team words: the team's XCC (+1), the step barrier, the closed team size (bit 31 set once the
+// leader closed registration) and one registration counter per XCC. The barrier is one 64-bit
+// word: arrivals in bits 0-31, members that failed an item in an even step in bits 32-47 and in
+// an odd step in bits 48-63, so a member learns of the previous step's failures in the same load
+// that ends its wait.
+enum { TW_XCC0 = 1, TW_BAR = 2, TW_TEAM = 6, TW_REG = 8, TW_WORDS = 32 };
+constexpr uint32_t kBarWord = TW_BAR / 2u;  // index of the barrier in 64-bit words
 constexpr uint32_t kJoinSpin = 1u << 12;  // leader's wait for the rest of the team (polls of L2)
 enum { EXIT_NONE = 0, EXIT_DONE = 1, EXIT_STALL = 2, EXIT_HANDOFF = 3, EXIT_LIMIT = 4, EXIT_STATUS = 5, EXIT_SKIP = 6 };
 enum {
@@ -118,8 +118,10 @@ enum { MP_WAIT = 0, MP_COUNTPLAN = 1, MP_SEG = 2, MP_SCHED = 3, MP_NSEG = 4, MP_
 // One step's selected batches; two of them alternate so that the scheduler wave can select step
 // s+1 while the other waves still scatter step s.
 struct StepBuf {
-    uint32_t active[kTeamNodes], n_in[kTeamNodes], particles[kTeamNodes], pre_off[kTeamNodes];
+    uint32_t active[kTeamNodes], n_in[kTeamNodes], particles[kTeamNodes];
+    uint32_t ob[kTeamNodes];  // outputs of the sources and transforms before node k (node order)
     uint32_t in_off[kTeamNodes + 1], seg_off[kTeamNodes + 1], nseg[kTeamNodes];
+    uint32_t top0;      // free-stack top when the step starts: output j takes entry top0 - 1 - j
     uint32_t g_main;    // segments [0, g_main) belong to non-sink nodes, [g_main, seg_off[n]) to sinks
     unsigned long long first_id[kTeamNodes], digest_base[kTeamNodes];
     uint32_t reserved;  // credits reserved for the step's outputs
@@ -143,7 +145,6 @@ struct TeamLds {
     uint32_t n_out[kTeamNodes], out_off[kTeamNodes + 1];
     gf_chunk_desc out[kTeamOut];
     uint32_t out_ridx[kTeamOut];  // route-table entry of output j
-    uint32_t pre[kTeamPrefix];
     uint32_t tot[kTeamNodes * kTeamBins];
     uint32_t binbase[kTeamNodes * kTeamBins];
     uint32_t zero[kTeamZero];  // count rows to clear in the launch's first step
@@ -263,13 +264,16 @@ __device__ inline void team_join(const TeamArgs& a, uint32_t* sh)
     __syncthreads();
 }
 
-// Thread 0: wait until every member has arrived `target` times in total (bounded spin).
-__device__ inline bool team_wait(const TeamArgs& a, uint32_t target)
+// Thread 0: wait until every member has arrived `target` times in total (bounded spin); `word` is
+// the barrier word that ended the wait.
+__device__ inline bool team_wait(const TeamArgs& a, uint32_t target, unsigned long long& word)
 {
     uint32_t spins = 0;
-    while (ld_u32(a.tw, TW_BAR) < target)
+    for (;;) {
+        word = ld_u64(a.tw, kBarWord);
+        if ((uint32_t)word >= target) return true;
         if (++spins > kTeamSpin) return false;
-    return true;
+    }
 }
 
 // ---- state load / flush ------------------------------------------------------------------------
@@ -540,12 +544,13 @@ __device__ inline void team_select(const TeamArgs& a, TeamLds& L, StepBuf& Bn, b
         parts = 0u;
         nseg = 0u;
     }
-    const uint32_t prew = (kind == KIND_PARTITION && act) ? nseg * (B + 1u) : 0u;
+    // outputs of the sources and transforms (the partitions' follow from the counts)
+    const uint32_t rnp = !act ? 0u : kind == KIND_SOURCE ? 1u : kind == KIND_TRANSFORM ? (parts + C - 1u) / C : 0u;
     // segments are numbered in node order, the sinks' after everybody else's
     const uint32_t nsm = kind == KIND_SINK ? 0u : nseg, nss = kind == KIND_SINK ? nseg : 0u;
-    const uint32_t i_inc = wave_inscan(nin), sm_inc = wave_inscan(nsm), ss_inc = wave_inscan(nss), p_inc = wave_inscan(prew);
+    const uint32_t i_inc = wave_inscan(nin), sm_inc = wave_inscan(nsm), ss_inc = wave_inscan(nss), o_inc = wave_inscan(rnp);
     const uint32_t g_main = rl(sm_inc, 63);
-    const uint32_t in_tot = rl(i_inc, 63), seg_tot = g_main + rl(ss_inc, 63), pre_tot = rl(p_inc, 63);
+    const uint32_t in_tot = rl(i_inc, 63), seg_tot = g_main + rl(ss_inc, 63);
     const uint32_t reserved = top - credits;
     const uint64_t actm = __ballot(act != 0u), pend = __ballot(own && tail != head);
     uint32_t code = EXIT_NONE;
@@ -554,8 +559,8 @@ __device__ inline void team_select(const TeamArgs& a, TeamLds& L, StepBuf& Bn, b
     } else if (at_limit) {
         code = EXIT_LIMIT;
     } else {
-        const bool fits = in_tot <= kTeamIn && seg_tot <= kTeamSegs && pre_tot <= kTeamPrefix && reserved <= kTeamOut &&
-                          reserved <= kTeamMargin && in_tot <= kTeamMargin;
+        const bool fits = in_tot <= kTeamIn && seg_tot <= kTeamSegs && reserved <= kTeamOut && reserved <= kTeamMargin &&
+                          in_tot <= kTeamMargin;
         const bool room = !own || (avail - nin) + reserved <= a.rc;
         if (!fits || __ballot(!room)) code = EXIT_HANDOFF;
     }
@@ -573,7 +578,7 @@ __device__ inline void team_select(const TeamArgs& a, TeamLds& L, StepBuf& Bn, b
         Bn.in_off[k] = inoff;
         Bn.seg_off[k] = segoff;
         Bn.nseg[k] = nseg;
-        Bn.pre_off[k] = p_inc - prew;
+        Bn.ob[k] = o_inc - rnp;
         if (kind == KIND_SINK && act) {
             Bn.digest_base[k] = g.digest;
             g.digest += nin;
@@ -603,6 +608,7 @@ __device__ inline void team_select(const TeamArgs& a, TeamLds& L, StepBuf& Bn, b
         Bn.reserved = reserved;
         Bn.sum_sink = sum_sink;
         Bn.part_mask = pmask;
+        Bn.top0 = top;
         const uint32_t nb = top > kTeamStack / 2u ? top - kTeamStack / 2u : 0u;
         const uint32_t fb = L.fs_base;
         Bn.recenter = (nb != fb && (top < fb + kTeamMargin || top + kTeamMargin > fb + kTeamStack)) ? 1u : 0u;
@@ -670,195 +676,187 @@ __device__ inline void team_recenter(const TeamArgs& a, TeamLds& L, uint32_t me)
     __syncthreads();
 }
 
-// ---- after the barrier: counts (all waves) and plan (wave 0) -----------------------------------
-// Every wave clears its share of the count rows the previous step consumed and scans some bins of
-// every partition batch (lane = segment): L.pre[g][b] = items of bin b in the batch's segments
-// before g, L.tot[b] = all of them. After one workgroup barrier wave 0 derives every node's exact
-// output count, checks the batch and pops the step's output chunks.
-__device__ inline void team_countplan(const TeamArgs& a, TeamLds& L, const StepBuf& Bc, const StepBuf* Bp, uint32_t me,
-                                      uint32_t T, uint32_t it)
+// ---- count rows ----------------------------------------------------------------------------------
+// Wave 6: clear this member's share of the count rows the previous step's partitions consumed (or,
+// in a launch's first step, the rows the previous launch left). A chunk's rows are one contiguous
+// block of tpc * bb words; its next use as a partition input has the other generation parity, so
+// the rows are zero again before any producer counts into them.
+__device__ inline void team_clear_rows(const TeamArgs& a, const TeamLds& L, const StepBuf* Bp, uint32_t me, uint32_t T)
 {
-    const Engine& e = a.e;
-    const uint32_t tid = threadIdx.x, w = tid >> 6, lane = lane_id(), n = e.n_nodes, C = e.C;
-    if (Bc.recenter) team_recenter(a, L, me);
-    Ticker tk(L.prof_acc, L.prof_member && tid == 0u);
-    // errors members published before the previous barrier (parity word: a member already one
-    // step ahead writes the other word)
-    const uint32_t err = (tid == 64u && it) ? ld_u32(a.tw, TW_ERR + ((it - 1u) & 1u)) : 0u;
-    // clear the count rows of the chunks partitions consumed in the previous step (a chunk's rows
-    // are one contiguous block of tpc * bb words)
-    const uint32_t blk = a.tpc * a.bb;
+    const uint32_t lane = lane_id(), blk = a.tpc * a.bb;
     if (Bp) {
         for (uint32_t pmk = Bp->part_mask; pmk; pmk &= pmk - 1u) {
             const uint32_t k = (uint32_t)(__ffs(pmk) - 1);
             const uint32_t io = Bp->in_off[k], zw = Bp->n_in[k] * blk;
-            for (uint32_t x = me * kClrThreads + tid; tid < kClrThreads && x < zw; x += T * kClrThreads) {
+            for (uint32_t x = me * 64u + lane; x < zw; x += T * 64u) {
                 const gf_chunk_desc d = Bp->in[io + x / blk];
                 a.cnt[((size_t)d.index * 2u + (d.generation & 1u)) * blk + x % blk] = 0u;
             }
         }
     } else {
         const uint32_t zw = L.zero_n * blk;
-        for (uint32_t x = me * kClrThreads + tid; tid < kClrThreads && x < zw; x += T * kClrThreads) {
+        for (uint32_t x = me * 64u + lane; x < zw; x += T * 64u) {
             const uint32_t z = L.zero[x / blk];
             a.cnt[((size_t)(z & 0x7FFFFFFFu) * 2u + (z >> 31)) * blk + x % blk] = 0u;
         }
     }
-    for (uint32_t pmk = Bc.part_mask; pmk; pmk &= pmk - 1u) {
-        const uint32_t k = (uint32_t)(__ffs(pmk) - 1);
-        const uint32_t B1 = L.nd[k].bins + 1u, so = Bc.seg_off[k], ns = Bc.nseg[k];
-        const uint32_t io = Bc.in_off[k], mi = Bc.n_in[k];
-        uint32_t* m = L.pre + Bc.pre_off[k];
-        // first segment and count-row base of input `lane` (inputs of a partition batch <= 64
-        // keep them in registers; more inputs search LDS)
-        const gf_chunk_desc dl = lane < mi ? Bc.in[io + lane] : gf_chunk_desc{0, 0, 0, 0};
-        const uint32_t fl = lane < mi ? Bc.in_seg[io + lane] : 0xFFFFFFFFu;
-        const uint32_t rowl = ((dl.index * 2u + (dl.generation & 1u)) * a.tpc) * a.bb;
-        auto row_of = [&](uint32_t g) -> size_t {
-            const uint32_t gs = so + g;
-            uint32_t f = rl(fl, 0), r = rl(rowl, 0);
-            if (mi <= 64u) {
-                for (uint32_t y = 1; y < mi; ++y)
-                    if (rl(fl, y) <= gs) {
-                        f = rl(fl, y);
-                        r = rl(rowl, y);
-                    }
-            } else {
-                for (uint32_t y = 1; y < mi; ++y) {
-                    const uint32_t fy = Bc.in_seg[io + y];
-                    if (fy <= gs) {
-                        const gf_chunk_desc d = Bc.in[io + y];
-                        f = fy;
-                        r = ((d.index * 2u + (d.generation & 1u)) * a.tpc) * a.bb;
-                    }
+}
+
+// Per-bin sums of the count rows of partition node k's segments [g0, g1) (segments numbered within
+// the node's batch, in batch input order): lane lofs + b receives in `all` the sum of slot b over
+// [g0, g1) and in `lo` the sum over [g0, split) (b < B1; slot B1 - 1 counts invalid bins). One
+// relaxed agent-scope load per (segment, slot), two segments per lane and iteration, then one
+// wave scan per slot.
+__device__ inline void seg_sums(const TeamArgs& a, const StepBuf& Bc, uint32_t k, uint32_t B1, uint32_t g0, uint32_t split,
+                                uint32_t g1, uint32_t lofs, uint32_t& lo, uint32_t& all)
+{
+    const uint32_t lane = lane_id(), so = Bc.seg_off[k], io = Bc.in_off[k], mi = Bc.n_in[k];
+    // first segment (step numbering) and count-row base of input `lane`; batches of more than 64
+    // inputs search LDS
+    const gf_chunk_desc dl = lane < mi ? Bc.in[io + lane] : gf_chunk_desc{0, 0, 0, 0};
+    const uint32_t fl = lane < mi ? Bc.in_seg[io + lane] : 0xFFFFFFFFu;
+    const uint32_t rowl = ((dl.index * 2u + (dl.generation & 1u)) * a.tpc) * a.bb;
+    auto row_of = [&](uint32_t g) -> size_t {
+        const uint32_t gs = so + g;
+        uint32_t f = rl(fl, 0), r = rl(rowl, 0);
+        if (mi <= 64u) {
+            for (uint32_t y = 1; y < mi; ++y) {
+                const uint32_t fy = rl(fl, y), ry = rl(rowl, y);
+                const bool c = fy <= gs;
+                f = c ? fy : f;
+                r = c ? ry : r;
+            }
+        } else {
+            for (uint32_t y = 1; y < mi; ++y) {
+                const uint32_t fy = Bc.in_seg[io + y];
+                if (fy <= gs) {
+                    const gf_chunk_desc d = Bc.in[io + y];
+                    f = fy;
+                    r = ((d.index * 2u + (d.generation & 1u)) * a.tpc) * a.bb;
                 }
             }
-            return (size_t)r + (size_t)(gs - f) * a.bb;
-        };
-        for (uint32_t b = scan_rank(w); b < B1; b += kTeamWaves) {
-            uint32_t carry = 0;
-            for (uint32_t g0 = 0; g0 < ns; g0 += 128u) {
-                const uint32_t ga = g0 + lane, gb = ga + 64u;
-                const bool va = ga < ns, vb = gb < ns;
-                const size_t ra = row_of(ga), rb = row_of(gb);
-                const uint32_t xa = va ? ld_u32(a.cnt, ra + b) : 0u;
-                const uint32_t xb = vb ? ld_u32(a.cnt, rb + b) : 0u;
-                const uint32_t ia = wave_inscan(xa), ib = wave_inscan(xb);
-                const uint32_t ta = rl(ia, 63);
-                if (va) m[ga * B1 + b] = carry + ia - xa;
-                if (vb) m[gb * B1 + b] = carry + ta + ib - xb;
-                carry += ta + rl(ib, 63);
+        }
+        return (size_t)r + (size_t)(gs - f) * a.bb;
+    };
+    lo = 0u;
+    all = 0u;
+    for (uint32_t h = g0; h < g1; h += 128u) {
+        const uint32_t ga = h + lane, gb = ga + 64u;
+        const bool va = ga < g1, vb = gb < g1;
+        const size_t ra = row_of(ga), rb = row_of(gb);
+        for (uint32_t c0 = 0; c0 < B1; c0 += 4u) {
+            uint32_t xa[4], xb[4];
+#pragma unroll
+            for (uint32_t q = 0; q < 4u; ++q) {
+                const bool in = c0 + q < B1;
+                xa[q] = va && in ? ld_u32(a.cnt, ra + c0 + q) : 0u;
+                xb[q] = vb && in ? ld_u32(a.cnt, rb + c0 + q) : 0u;
             }
-            if (lane == 0) L.tot[k * kTeamBins + b] = carry;
+#pragma unroll
+            for (uint32_t q = 0; q < 4u; ++q) {
+                if (c0 + q >= B1) break;
+                const uint32_t ia = wave_inscan(xa[q]), ib = wave_inscan(xb[q]);
+                const uint32_t ta = rl(ia, 63), s = ta + rl(ib, 63);
+                uint32_t l = 0u;
+                if (split >= h + 128u) l = s;
+                else if (split > h + 64u) l = ta + rl(ib, split - h - 65u);
+                else if (split == h + 64u) l = ta;
+                else if (split > h) l = rl(ia, split - h - 1u);
+                const bool mine = lane == lofs + c0 + q;
+                all += mine ? s : 0u;
+                lo += mine ? l : 0u;
+            }
         }
     }
-    if (tid == 64u) {
-        L.seg_err = 0u;
-        L.err_prev = err;
+}
+
+// ---- plan (wave 6) ------------------------------------------------------------------------------
+// Every node's exact output count from the count rows (partitions) or the batch (sources,
+// transforms), the checks of the batch, and the step's output chunks popped off the free stack.
+__device__ inline void team_plan(const TeamArgs& a, TeamLds& L, const StepBuf& Bc, const SchedRegs& g)
+{
+    const Engine& e = a.e;
+    const uint32_t lane = lane_id(), n = e.n_nodes, C = e.C;
+    Ticker tk(L.prof_acc, L.prof_member && lane == 0u);
+    // lanes 0..31: nodes; lanes 32..63: bins of the first partition node of the step
+    const bool own = lane < n;
+    const uint32_t k = own ? lane : 0u;
+    const uint32_t act = own ? Bc.active[k] : 0u, kind = g.kind;
+    const uint32_t parts = own ? Bc.particles[k] : 0u, bound = own ? L.bound[k] : 0u, ro = own ? L.route_off[k] : 0u;
+    const uint32_t pm = Bc.part_mask;
+    const uint32_t kp = pm ? (uint32_t)(__ffs(pm) - 1) : 0u;
+    const uint32_t B = pm ? rl(g.bins, kp) : 0u;
+    uint32_t t = 0u;
+    if (pm) {
+        uint32_t unused;
+        seg_sums(a, Bc, kp, B + 1u, 0u, 0u, Bc.nseg[kp], 32u, unused, t);
     }
     tk(PROF_C_CNT_SCAN);
-    __syncthreads();
-    if (w == 0u) {
-        // lanes 0..31: nodes; lanes 32..63: bins of the first partition node of the step
-        const bool own = lane < n;
-        const uint32_t k = own ? lane : 0u;
-        const uint32_t act = own ? Bc.active[k] : 0u, kind = own ? L.kind[k] : 0xFFu;
-        const uint32_t parts = own ? Bc.particles[k] : 0u, bound = own ? L.bound[k] : 0u, ro = own ? L.route_off[k] : 0u;
-        const uint32_t pm = Bc.part_mask;
-        const uint32_t kp = pm ? (uint32_t)(__ffs(pm) - 1) : 0u;
-        const uint32_t B = pm ? L.nd[kp].bins : 0u;
-        const uint32_t bl = lane - 32u;  // bin of this lane (lanes >= 32)
-        const uint32_t t = (pm && lane >= 32u && bl <= B) ? L.tot[kp * kTeamBins + bl] : 0u;
-        const uint32_t tv = (lane >= 32u && bl < B) ? t : 0u, r = (tv + C - 1u) / C;
-        const uint32_t ri = wave_inscan(r), ti = wave_inscan(tv);
-        const uint32_t bbase = ri - r;  // first output of bin bl within the partition's outputs
-        uint32_t R = 0, st = 0, inv = 0;
-        if (act) {
-            if (kind == KIND_TRANSFORM) {
-                R = (parts + C - 1u) / C;
-                if (parts == 0u || R > bound) st = GF_PLAN_MISMATCH;
-            } else if (kind == KIND_SOURCE) {
-                R = 1u;
-            }
+    const uint32_t bl = lane - 32u;  // bin of this lane (lanes >= 32)
+    const uint32_t tv = (lane >= 32u && bl < B) ? t : 0u, r = (tv + C - 1u) / C;
+    const uint32_t ri = wave_inscan(r), ti = wave_inscan(tv);
+    const uint32_t bbase = ri - r;  // first output of bin bl within the partition's outputs
+    uint32_t R = 0, st = 0, inv = 0;
+    if (act) {
+        if (kind == KIND_TRANSFORM) {
+            R = (parts + C - 1u) / C;
+            if (parts == 0u || R > bound) st = GF_PLAN_MISMATCH;
+        } else if (kind == KIND_SOURCE) {
+            R = 1u;
         }
-        if (pm) {
-            const uint32_t Rk = rl(ri, 63), sum = rl(ti, 63);
-            if (rl(t, 32u + B)) inv = 1u;
-            if (lane == kp) {
-                R = Rk;
-                if (sum != parts || Rk > bound) st = GF_PLAN_MISMATCH;
-            }
-            if (lane >= 32u && bl < B) L.binbase[kp * kTeamBins + bl] = bbase;
+    }
+    if (pm) {
+        const uint32_t Rk = rl(ri, 63), sum = rl(ti, 63);
+        if (rl(t, 32u + B)) inv = 1u;
+        if (lane == kp) {
+            R = Rk;
+            if (sum != parts || Rk > bound) st = GF_PLAN_MISMATCH;
         }
-        // further partition nodes of the same step (rare): one pass each through LDS
-        for (uint32_t pmk = pm & (pm - 1u); pmk; pmk &= pmk - 1u) {
-            const uint32_t kq = (uint32_t)(__ffs(pmk) - 1);
-            const uint32_t Bq = L.nd[kq].bins;
-            const uint32_t tq = lane <= Bq ? L.tot[kq * kTeamBins + lane] : 0u;
-            const uint32_t tqv = lane < Bq ? tq : 0u, rq = (tqv + C - 1u) / C;
-            const uint32_t rqi = wave_inscan(rq), tqi = wave_inscan(tqv);
-            if (lane < Bq) L.binbase[kq * kTeamBins + lane] = rqi - rq;
-            // wave-wide values first: a readlane inside the lane branch would let the scans move
-            // under that branch's exec mask
-            const uint32_t Rq = rl(rqi, 63), sq = rl(tqi, 63);
-            if (rl(tq, Bq)) inv = 1u;
-            if (lane == kq) {
-                R = Rq;
-                if (sq != parts || Rq > bound) st = GF_PLAN_MISMATCH;
-            }
+        if (lane >= 32u && bl < B) L.binbase[kp * kTeamBins + bl] = bbase;
+        if (lane >= 32u && bl <= B) L.tot[kp * kTeamBins + bl] = t;
+    }
+    // further partition nodes of the same step (rare): bins in lanes 0.., one pass each
+    for (uint32_t pmk = pm & (pm - 1u); pmk; pmk &= pmk - 1u) {
+        const uint32_t kq = (uint32_t)(__ffs(pmk) - 1);
+        const uint32_t Bq = rl(g.bins, kq);
+        uint32_t unused, tq;
+        seg_sums(a, Bc, kq, Bq + 1u, 0u, 0u, Bc.nseg[kq], 0u, unused, tq);
+        const uint32_t tqv = lane < Bq ? tq : 0u, rq = (tqv + C - 1u) / C;
+        const uint32_t rqi = wave_inscan(rq), tqi = wave_inscan(tqv);
+        if (lane < Bq) L.binbase[kq * kTeamBins + lane] = rqi - rq;
+        if (lane <= Bq) L.tot[kq * kTeamBins + lane] = tq;
+        // wave-wide values first: a readlane inside the lane branch would let the scans move
+        // under that branch's exec mask
+        const uint32_t Rq = rl(rqi, 63), sq = rl(tqi, 63);
+        if (rl(tq, Bq)) inv = 1u;
+        if (lane == kq) {
+            R = Rq;
+            if (sq != parts || Rq > bound) st = GF_PLAN_MISMATCH;
         }
-        const uint64_t bad = __ballot(st != 0u);
-        uint32_t status = inv ? (uint32_t)GF_INVALID : bad ? rl(st, (uint32_t)(__ffsll((unsigned long long)bad) - 1)) : 0u;
-        const uint32_t oi = wave_inscan(own ? R : 0u), off = oi - R, total = rl(oi, 63);
-        if (!status && total > Bc.reserved) status = GF_PLAN_MISMATCH;
-        if (!status && total > L.top) status = GF_POOL_EXHAUSTED;
-        if (own) {
-            L.n_out[k] = R;
-            L.out_off[k] = off;
-        }
-        if (lane == 0) {
-            L.out_off[n] = total;
-            L.total_out = total;
-            L.status = status;
-            if (status) set_status(&e.ctl[CTL_STATUS], status);
-        }
-        tk(PROF_C_PLAN_NODES);
-        if (!status) {
-            // output j of the step (node order, bin-major, k order) takes stack[top - 1 - j]
-            const uint32_t top = L.top, fbase = L.fs_base;
-            auto emit = [&](uint32_t j0, uint32_t cnt, uint32_t fill0, uint32_t rem0, uint32_t bin, uint32_t ridx) {
-                // outputs j0 .. j0+cnt-1: fills C, C, ..., the rest of rem0 (fill0 = C or the whole)
-                for (uint32_t q = lane; q < cnt; q += 64u) {
-                    const uint32_t j = j0 + q;
-                    const unsigned long long ent = L.fs[top - 1u - j - fbase];
-                    const uint32_t rem = rem0 - q * fill0;
-                    gf_chunk_desc d;
-                    d.index = (uint32_t)ent;
-                    d.generation = (uint32_t)(ent >> 32);
-                    d.fill = rem < fill0 ? rem : fill0;
-                    d.bin = bin;
-                    L.out[j] = d;
-                    L.out_ridx[j] = ridx;
-                }
-            };
-            // node lanes emit their source / transform outputs and bin lanes the first partition's,
-            // all lanes at once
-            const uint32_t kpo = rl(off, kp), kpr = rl(ro, kp);
-            uint32_t cnt = 0, j0 = 0, fill0 = C, rem0 = 0, bin = 0, ridx = 0;
-            if (own && R && (kind == KIND_SOURCE || kind == KIND_TRANSFORM)) {
-                cnt = R;
-                j0 = off;
-                fill0 = kind == KIND_SOURCE ? parts : C;
-                rem0 = parts;
-                ridx = ro;
-            } else if (lane >= 32u && r) {
-                cnt = r;
-                j0 = kpo + bbase;
-                rem0 = tv;
-                bin = bl;
-                ridx = kpr + bl;
-            }
-            for (uint32_t q = 0; q < cnt; ++q) {
+    }
+    wave_sync();
+    const uint64_t bad = __ballot(st != 0u);
+    uint32_t status = inv ? (uint32_t)GF_INVALID : bad ? rl(st, (uint32_t)(__ffsll((unsigned long long)bad) - 1)) : 0u;
+    const uint32_t oi = wave_inscan(own ? R : 0u), off = oi - R, total = rl(oi, 63);
+    if (!status && total > Bc.reserved) status = GF_PLAN_MISMATCH;
+    if (!status && total > L.top) status = GF_POOL_EXHAUSTED;
+    if (own) {
+        L.n_out[k] = R;
+        L.out_off[k] = off;
+    }
+    if (lane == 0) {
+        L.out_off[n] = total;
+        L.total_out = total;
+        L.status = status;
+        if (status) set_status(&e.ctl[CTL_STATUS], status);
+    }
+    tk(PROF_C_PLAN_NODES);
+    if (!status) {
+        // output j of the step (node order, bin-major, k order) takes stack[top - 1 - j]
+        const uint32_t top = L.top, fbase = L.fs_base;
+        auto emit = [&](uint32_t j0, uint32_t cnt, uint32_t fill0, uint32_t rem0, uint32_t bin, uint32_t ridx) {
+            // outputs j0 .. j0+cnt-1: fills C, C, ..., the rest of rem0 (fill0 = C or the whole)
+            for (uint32_t q = lane; q < cnt; q += 64u) {
                 const uint32_t j = j0 + q;
                 const unsigned long long ent = L.fs[top - 1u - j - fbase];
                 const uint32_t rem = rem0 - q * fill0;
@@ -870,24 +868,53 @@ __device__ inline void team_countplan(const TeamArgs& a, TeamLds& L, const StepB
                 L.out[j] = d;
                 L.out_ridx[j] = ridx;
             }
-            // further partition nodes of the step, one bin at a time
-            for (uint32_t pmk = pm & (pm - 1u); pmk; pmk &= pmk - 1u) {
-                const uint32_t kq = (uint32_t)(__ffs(pmk) - 1);
-                const uint32_t Bq = L.nd[kq].bins, base = rl(off, kq), rok = rl(ro, kq);
-                for (uint32_t x = 0; x < Bq; ++x) {
-                    const uint32_t tx = L.tot[kq * kTeamBins + x];
-                    if (tx) emit(base + L.binbase[kq * kTeamBins + x], (tx + C - 1u) / C, C, tx, x, rok + x);
-                }
-            }
-            if (lane == 0) {
-                L.top = top - total;
-                const unsigned long long live = e.S - (top - total);
-                if (live > L.peak_live) L.peak_live = live;
+        };
+        // node lanes emit their source / transform outputs and bin lanes the first partition's,
+        // all lanes at once
+        const uint32_t kpo = rl(off, kp), kpr = rl(ro, kp);
+        uint32_t cnt = 0, j0 = 0, fill0 = C, rem0 = 0, bin = 0, ridx = 0;
+        if (own && R && (kind == KIND_SOURCE || kind == KIND_TRANSFORM)) {
+            cnt = R;
+            j0 = off;
+            fill0 = kind == KIND_SOURCE ? parts : C;
+            rem0 = parts;
+            ridx = ro;
+        } else if (lane >= 32u && r) {
+            cnt = r;
+            j0 = kpo + bbase;
+            rem0 = tv;
+            bin = bl;
+            ridx = kpr + bl;
+        }
+        for (uint32_t q = 0; q < cnt; ++q) {
+            const uint32_t j = j0 + q;
+            const unsigned long long ent = L.fs[top - 1u - j - fbase];
+            const uint32_t rem = rem0 - q * fill0;
+            gf_chunk_desc d;
+            d.index = (uint32_t)ent;
+            d.generation = (uint32_t)(ent >> 32);
+            d.fill = rem < fill0 ? rem : fill0;
+            d.bin = bin;
+            L.out[j] = d;
+            L.out_ridx[j] = ridx;
+        }
+        // further partition nodes of the step, one bin at a time
+        for (uint32_t pmk = pm & (pm - 1u); pmk; pmk &= pmk - 1u) {
+            const uint32_t kq = (uint32_t)(__ffs(pmk) - 1);
+            const uint32_t Bq = rl(g.bins, kq), base = rl(off, kq), rok = rl(ro, kq);
+            for (uint32_t x = 0; x < Bq; ++x) {
+                const uint32_t tx = L.tot[kq * kTeamBins + x];
+                if (tx) emit(base + L.binbase[kq * kTeamBins + x], (tx + C - 1u) / C, C, tx, x, rok + x);
             }
         }
-        tk(PROF_C_PLAN_POPS);
+        if (lane == 0) {
+            L.top = top - total;
+            const unsigned long long live = e.S - (top - total);
+            if (live > L.peak_live) L.peak_live = live;
+        }
     }
-    __syncthreads();
+    wave_sync();
+    tk(PROF_C_PLAN_POPS);
 }
 
 // ---- trace (member 0) and retire (wave 6 of every member) --------------------------------------
@@ -935,13 +962,13 @@ __device__ inline void team_trace(const TeamArgs& a, TeamLds& L, const StepBuf& 
 }
 
 // Retire (wave 6 of every member): outputs to the destination rings in node and output order,
-// consumed inputs back to the free stack (generation + 1), counters.
+// the free-stack top and the counters.
 __device__ inline void team_retire(const TeamArgs& a, TeamLds& L, StepBuf& Bc, SchedRegs& g)
 {
     const Engine& e = a.e;
     const uint32_t lane = lane_id(), n = e.n_nodes, rcm = a.rc - 1u;
     Ticker tk(L.prof_acc, L.prof_member && lane == 0u);
-    const uint32_t total_out = L.total_out, in_tot = Bc.in_off[n], top = L.top, fb = L.fs_base;
+    const uint32_t total_out = L.total_out, in_tot = Bc.in_off[n], top = L.top;
     for (uint32_t x0 = 0; x0 < total_out; x0 += 64u) {
         const uint32_t x = x0 + lane;
         const bool v = x < total_out;
@@ -971,10 +998,8 @@ __device__ inline void team_retire(const TeamArgs& a, TeamLds& L, StepBuf& Bc, S
         if (v) L.ring[dst * a.rc + (uint32_t)(pos & rcm)] = d;
     }
     tk(PROF_C_RET_OUT);
-    for (uint32_t x = lane; x < in_tot; x += 64u) {
-        const gf_chunk_desc d = Bc.in[x];
-        L.fs[top + x - fb] = (unsigned long long)d.index | ((unsigned long long)(d.generation + 1u) << 32);
-    }
+    // the consumed inputs go back on the free stack in team_push, once the scatter waves no longer
+    // read the popped entries; the top moves now, for the next step's credits
     if (lane == 0) {
         L.top = top + in_tot;
         L.sunk += Bc.sum_sink;
@@ -983,6 +1008,18 @@ __device__ inline void team_retire(const TeamArgs& a, TeamLds& L, StepBuf& Bc, S
     }
     wave_sync();
     tk(PROF_C_RET_IN);
+}
+
+// Wave 6, after the step's workgroup barrier: the step's consumed inputs back on the free stack
+// (generation + 1), where its pops were taken from.
+__device__ inline void team_push(const TeamArgs& a, TeamLds& L, const StepBuf& Bc)
+{
+    const uint32_t lane = lane_id(), in_tot = Bc.in_off[a.e.n_nodes];
+    const uint32_t ptop = Bc.top0 - L.total_out, fb = L.fs_base;
+    for (uint32_t x = lane; x < in_tot; x += 64u) {
+        const gf_chunk_desc d = Bc.in[x];
+        L.fs[ptop + x - fb] = (unsigned long long)d.index | ((unsigned long long)(d.generation + 1u) << 32);
+    }
 }
 
 // A failed item: the run's status word for the host, this member's flag for the team's stop decision.
@@ -1116,9 +1153,88 @@ __device__ inline uint32_t seg_rank(bool v, uint32_t bin, uint32_t& run)
     return pos;
 }
 
+// Where one node's outputs go, as a scatter wave derives it from the batch and the count rows
+// alone: the node's outputs are entries base .. base + R - 1 of the step's pops, and (partitions)
+// the items of bin b in segment gs start at flat slot bb_b + pre_b of the node's outputs.
+struct Place {
+    uint32_t k;    // node described (wave-uniform), or ~0u
+    uint32_t ok;   // the node's batch agrees with its counts (otherwise wave 6 fails the step)
+    uint32_t base, R;
+    uint32_t gs;   // node-local segment `pre` belongs to
+    uint32_t bb;   // lane b: binbase of bin b times C
+    uint32_t pre;  // lane b: items of bin b in the node's segments before gs
+};
+
+// Output count of partition node k from its count rows.
+__device__ inline uint32_t part_outputs(const TeamArgs& a, const TeamLds& L, const StepBuf& Bc, uint32_t k)
+{
+    const uint32_t B = L.nd[k].bins, C = a.e.C;
+    uint32_t unused, tot;
+    seg_sums(a, Bc, k, B + 1u, 0u, 0u, Bc.nseg[k], 0u, unused, tot);
+    const uint32_t tv = lane_id() < B ? tot : 0u;
+    return wave_sum((tv + C - 1u) / C);
+}
+
+// Placement of node k's segment gs (node-local numbering). A wave's segments come in increasing
+// order, so a later segment of the same partition only adds the rows in between to the prefix.
+__device__ inline void place_node(const TeamArgs& a, const TeamLds& L, const StepBuf& Bc, uint32_t k, uint32_t kind, uint32_t gs,
+                                  Place& pl)
+{
+    const uint32_t C = a.e.C, lane = lane_id();
+    if (pl.k == k) {
+        if (kind != KIND_PARTITION || gs == pl.gs) return;
+        if (gs > pl.gs) {
+            uint32_t add, unused;
+            seg_sums(a, Bc, k, L.nd[k].bins + 1u, pl.gs, gs, gs, 0u, add, unused);
+            pl.pre += add;
+            pl.gs = gs;
+            return;
+        }
+    }
+    pl.k = k;
+    pl.gs = gs;
+    uint32_t base = Bc.ob[k];
+    for (uint32_t pmk = Bc.part_mask & ((1u << k) - 1u); pmk; pmk &= pmk - 1u)
+        base += part_outputs(a, L, Bc, (uint32_t)(__ffs(pmk) - 1));
+    pl.base = base;
+    pl.ok = 1u;
+    pl.bb = 0u;
+    pl.pre = 0u;
+    const uint32_t parts = Bc.particles[k];
+    if (kind == KIND_SOURCE) {
+        pl.R = 1u;
+    } else if (kind == KIND_TRANSFORM) {
+        pl.R = (parts + C - 1u) / C;
+    } else {
+        const uint32_t B = L.nd[k].bins;
+        uint32_t pre, tot;
+        seg_sums(a, Bc, k, B + 1u, 0u, gs, Bc.nseg[k], 0u, pre, tot);
+        const uint32_t tv = lane < B ? tot : 0u, r = (tv + C - 1u) / C, ri = wave_inscan(r);
+        const uint32_t Rk = rl(ri, 63), sum = wave_sum(tv), inv = rl(tot, B);
+        pl.R = Rk;
+        pl.ok = (inv == 0u && sum == parts && Rk <= L.bound[k]) ? 1u : 0u;
+        pl.bb = lane < B ? (ri - r) * C : 0u;
+        pl.pre = pre;
+    }
+}
+
+// Output chunk of pop j of the step (entry top0 - 1 - j of the free stack), or false when the
+// entry is not in the LDS window (only a failed plan gets there).
+__device__ inline bool pop_desc(const TeamLds& L, const StepBuf& Bc, uint32_t j, gf_chunk_desc& od)
+{
+    const uint32_t top0 = Bc.top0, fb = L.fs_base;
+    if (j >= top0 - fb) return false;
+    const unsigned long long ent = L.fs[top0 - 1u - j - fb];
+    od.index = (uint32_t)ent;
+    od.generation = (uint32_t)(ent >> 32);
+    od.fill = 0u;
+    od.bin = 0u;
+    return true;
+}
+
 template <class P>
 __device__ void team_partition_seg(const TeamArgs& a, const TeamLds& L, const StepBuf& Bc, uint32_t k, uint32_t i, uint32_t gg,
-                                   uint32_t gs, const SegRecs<P>& R, BinTab& t)
+                                   const SegRecs<P>& R, BinTab& t, const Place& pl)
 {
     const Engine& e = a.e;
     const uint32_t C = e.C, lane = lane_id();
@@ -1127,10 +1243,7 @@ __device__ void team_partition_seg(const TeamArgs& a, const TeamLds& L, const St
     const uint32_t B = nd.bins;
     const gf_chunk_desc d = Bc.in[Bc.in_off[k] + i];
     const uint32_t off0 = gg * kSeg, cnt = d.fill - off0 < kSeg ? d.fill - off0 : kSeg;
-    const uint32_t* pre = L.pre + Bc.pre_off[k] + gs * (B + 1u);
-    const uint32_t fb = lane < B ? L.binbase[k * kTeamBins + lane] * C + pre[lane] : 0u;  // flat start of bin `lane`
-    const gf_chunk_desc* ob = L.out + L.out_off[k];
-    const uint32_t nout = L.n_out[k];
+    const uint32_t fb = lane < B ? pl.bb + pl.pre : 0u;  // flat start of bin `lane` in this segment
     Ticker tk(const_cast<unsigned long long*>(L.prof_acc), L.prof_member && threadIdx.x == 0u);
     bool v[kSegRounds];
     uint32_t bin[kSegRounds], pos[kSegRounds], key[kSegRounds];
@@ -1161,11 +1274,11 @@ __device__ void team_partition_seg(const TeamArgs& a, const TeamLds& L, const St
         key[r] = ~0u;
         if (!v[r]) continue;
         const uint32_t q = div_c(flat, C, a.c_shift), off = flat - q * C;
-        if (q >= nout) {
+        gf_chunk_desc od;
+        if (q >= pl.R || !pop_desc(L, Bc, pl.base + q, od)) {
             seg_fail(a, L, GF_PLAN_MISMATCH);
             continue;
         }
-        const gf_chunk_desc od = ob[q];
         P p = R.rec[r];
         const uint32_t u = update(nd, bin[r], p);
         if (u) seg_fail(a, L, u);
@@ -1189,7 +1302,7 @@ __device__ void team_partition_seg(const TeamArgs& a, const TeamLds& L, const St
 
 template <class P>
 __device__ void team_transform_seg(const TeamArgs& a, const TeamLds& L, const StepBuf& Bc, uint32_t k, uint32_t i, uint32_t gg,
-                                   const SegRecs<P>& R)
+                                   const SegRecs<P>& R, const Place& pl)
 {
     const Engine& e = a.e;
     const uint32_t C = e.C, lane = lane_id();
@@ -1197,18 +1310,17 @@ __device__ void team_transform_seg(const TeamArgs& a, const TeamLds& L, const St
     const gf_chunk_desc d = Bc.in[Bc.in_off[k] + i];
     const uint32_t off0 = gg * kSeg, cnt = d.fill - off0 < kSeg ? d.fill - off0 : kSeg;
     const uint32_t pos0 = Bc.in_items[Bc.in_off[k] + i] + off0;
-    const gf_chunk_desc* ob = L.out + L.out_off[k];
-    const uint32_t nout = L.n_out[k], dst = (uint32_t)L.route[L.route_off[k]];
+    const uint32_t dst = (uint32_t)L.route[L.route_off[k]];
 #pragma unroll
     for (uint32_t r = 0; r < kSegRounds; ++r) {
         const uint32_t it = r * 64u + lane;
         uint32_t key = ~0u;
         if (it < cnt) {
             const uint32_t g = pos0 + it, q = div_c(g, C, a.c_shift), off = g - q * C;
-            if (q < nout) {
+            gf_chunk_desc od;
+            if (q < pl.R && pop_desc(L, Bc, pl.base + q, od)) {
                 P p = R.rec[r];
                 transform(nd, e.rp, L.purpose[k], p);
-                const gf_chunk_desc od = ob[q];
                 p.store(e.cols, (size_t)od.index * C + off);
                 key = next_key(a, L, dst, od, off, p);
             } else {
@@ -1220,12 +1332,16 @@ __device__ void team_transform_seg(const TeamArgs& a, const TeamLds& L, const St
 }
 
 template <class P>
-__device__ void team_source_seg(const TeamArgs& a, const TeamLds& L, const StepBuf& Bc, uint32_t k, uint32_t gg)
+__device__ void team_source_seg(const TeamArgs& a, const TeamLds& L, const StepBuf& Bc, uint32_t k, uint32_t gg, const Place& pl)
 {
     const Engine& e = a.e;
     const uint32_t C = e.C, lane = lane_id();
     const gf_node_desc& nd = L.nd[k];
-    const gf_chunk_desc od = L.out[L.out_off[k]];
+    gf_chunk_desc od;
+    if (!pop_desc(L, Bc, pl.base, od)) {
+        seg_fail(a, L, GF_PLAN_MISMATCH);
+        return;
+    }
     const uint32_t fill = Bc.particles[k], off0 = gg * kSeg, cnt = fill - off0 < kSeg ? fill - off0 : kSeg;
     const uint32_t dst = (uint32_t)L.route[L.route_off[k]];
     const unsigned long long id0 = Bc.first_id[k] + off0;
@@ -1300,9 +1416,8 @@ __device__ void team_sink_seg(const TeamArgs& a, const TeamLds& L, const StepBuf
     }
 }
 
-// Segments [lo, hi) of this member: worker waves take lo + o, lo + o + kWorkers, ... in the order
-// o = 0 for wave 1, 1 for wave 2, 2 for wave 0 (wave 0 computes the plan first, so it scatters only
-// when the member has more than two segments).
+// Segments [lo, hi) of this member (the step's non-sink segments split evenly over the members):
+// worker wave w takes lo + w, lo + w + kWorkers, ...
 __device__ inline void member_range(const StepBuf& Bc, uint32_t n, uint32_t me, uint32_t T, uint32_t* lo, uint32_t* hi)
 {
     (void)n;
@@ -1333,9 +1448,8 @@ __device__ inline void team_prefetch(const TeamArgs& a, const TeamLds& L, const 
 {
     uint32_t lo, hi;
     member_range(Bc, a.e.n_nodes, me, T, &lo, &hi);
-    const uint32_t o = (w + kWorkers - 1u) % kWorkers;
     R.s = ~0u;
-    if (lo + o < hi) load_segment<P>(a, L, Bc, lo + o, R);
+    if (lo + w < hi) load_segment<P>(a, L, Bc, lo + w, R);
 }
 
 template <class P>
@@ -1344,18 +1458,21 @@ __device__ inline void team_segments(const TeamArgs& a, const TeamLds& L, const 
 {
     uint32_t lo, hi;
     member_range(Bc, a.e.n_nodes, me, T, &lo, &hi);
-    const uint32_t o = (w + kWorkers - 1u) % kWorkers;
-    for (uint32_t s = lo + o; s < hi; s += kWorkers) {
+    Place pl;
+    pl.k = ~0u;
+    for (uint32_t s = lo + w; s < hi; s += kWorkers) {
         const SegLoc loc = locate(L, Bc, a.e.n_nodes, s);
         const uint32_t k = loc.k, i = loc.i, gg = loc.gg;
         const uint32_t kind = L.kind[k];
+        place_node(a, L, Bc, k, kind, s - Bc.seg_off[k], pl);
+        if (!pl.ok) continue;
         if (kind == KIND_SOURCE) {
-            team_source_seg<P>(a, L, Bc, k, gg);
+            team_source_seg<P>(a, L, Bc, k, gg, pl);
             continue;
         }
         if (R.s != s) load_segment<P>(a, L, Bc, s, R);
-        if (kind == KIND_PARTITION) team_partition_seg<P>(a, L, Bc, k, i, gg, s - Bc.seg_off[k], R, t);
-        else team_transform_seg<P>(a, L, Bc, k, i, gg, R);
+        if (kind == KIND_PARTITION) team_partition_seg<P>(a, L, Bc, k, i, gg, R, t, pl);
+        else team_transform_seg<P>(a, L, Bc, k, i, gg, R, pl);
     }
 }
 
@@ -1413,24 +1530,24 @@ __global__ __launch_bounds__(kTeamThreads) void k_team(TeamArgs a)
         StepBuf& Bc = L.sb[it & 1u];
         StepBuf& Bn = L.sb[(it + 1u) & 1u];  // the previous step's batches until select overwrites them
         if (it) {
-            // the previous step's records of every member are in memory
-            if (tid == 0u && !team_wait(a, target)) {
-                set_status(&e.ctl[CTL_STATUS], GF_DEVICE_ERROR);
-                L.status = GF_DEVICE_ERROR;
+            // the previous step's records and counts of every member are in memory
+            if (tid == 0u) {
+                unsigned long long word = 0ull;
+                if (!team_wait(a, target, word)) {
+                    set_status(&e.ctl[CTL_STATUS], GF_DEVICE_ERROR);
+                    L.status = GF_DEVICE_ERROR;
+                }
+                L.err_prev = (uint32_t)(word >> (32u + 16u * ((it - 1u) & 1u))) & 0xFFFFu;
             }
             __syncthreads();
             mark(PROF_T_WAIT);
             mmark(MP_WAIT);
-            if (L.status) break;
+            if (L.status || L.err_prev) break;
         }
         if (L.exit_code != EXIT_NONE) break;
-        R.s = ~0u;
-        if (w >= 1u && w < kWorkers) team_prefetch<P>(a, L, Bc, me, T, w, R);  // wave 0 starts on the counts
+        if (Bc.recenter) team_recenter(a, L, me);
         mark(PROF_T_COUNTS);
-        team_countplan(a, L, Bc, it ? &Bn : nullptr, me, T, it);
-        mark(PROF_T_PLAN);
         mmark(MP_COUNTPLAN);
-        if (L.err_prev || L.status) break;
         const unsigned long long w0 = wprof ? wall_clock64() : 0ull;
         const unsigned long long w0m = mp ? wall_clock64() : 0ull;
         if (mp && tid == 0u) {
@@ -1441,32 +1558,40 @@ __global__ __launch_bounds__(kTeamThreads) void k_team(TeamArgs a)
             atomicAdd(&mp[MP_NSINK], (unsigned long long)((uint64_t)(me + 1u) * Gs / T - (uint64_t)me * Gs / T));
         }
         if (w < kWorkers) {
+            R.s = ~0u;
+            team_prefetch<P>(a, L, Bc, me, T, w, R);
             team_segments<P>(a, L, Bc, me, T, w, R, tab);
             if (wprof && w == 0u) atomicAdd(&L.prof_acc[PROF_T_SEGW], wall_clock64() - w0);
         } else if (w == kSinkWave) {
             team_sinks<P>(a, L, Bc, me, T);
         } else {
-            // the scheduler wave retires this step and selects the next one meanwhile
-            if (me == 0u && e.trace_on) team_trace(a, L, Bc);
-            team_retire(a, L, Bc, sg);
-            const unsigned long long w1 = wprof ? wall_clock64() : 0ull;
-            if (wprof) atomicAdd(&L.prof_acc[PROF_T_RETIRE], w1 - w0);
-            team_select(a, L, Bn, it + 1u >= a.max_steps, sg);
-            if (wprof) atomicAdd(&L.prof_acc[PROF_T_SELECTW], wall_clock64() - w1);
+            // the scheduler wave plans this step, retires it and selects the next one
+            team_clear_rows(a, L, it ? &Bn : nullptr, me, T);
+            team_plan(a, L, Bc, sg);
+            if (!L.status) {
+                if (me == 0u && e.trace_on) team_trace(a, L, Bc);
+                const unsigned long long w1 = wprof ? wall_clock64() : 0ull;
+                team_retire(a, L, Bc, sg);
+                const unsigned long long w2 = wprof ? wall_clock64() : 0ull;
+                if (wprof) atomicAdd(&L.prof_acc[PROF_T_RETIRE], w2 - w1);
+                team_select(a, L, Bn, it + 1u >= a.max_steps, sg);
+                if (wprof) atomicAdd(&L.prof_acc[PROF_T_SELECTW], wall_clock64() - w2);
+            }
             if (mp && lane_id() == 0u) atomicAdd(&mp[MP_SCHED], wall_clock64() - w0m);
         }
         if (w != kSinkWave) __builtin_amdgcn_s_waitcnt(0);
         __syncthreads();
         target += T;
         if (tid == 0u) {
-            if (L.seg_err) {
-                atomicOr(&a.tw[TW_ERR + (it & 1u)], 1u);
-                __builtin_amdgcn_s_waitcnt(0);
-            }
-            __hip_atomic_fetch_add(&a.tw[TW_BAR], 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+            // arrive, with this member's item failures of the step in the step parity's field
+            const unsigned long long add = 1ull + (L.seg_err ? 1ull << (32u + 16u * (it & 1u)) : 0ull);
+            __hip_atomic_fetch_add((unsigned long long*)a.tw + kBarWord, add, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+            L.seg_err = 0u;
         }
         mark(PROF_T_SEG);
         mmark(MP_SEG);
+        if (L.status) break;
+        if (w == kSchedWave) team_push(a, L, Bc);
         mark(PROF_T_SELECT);
         last = (int32_t)it;
         if (prof) {
