@@ -408,8 +408,9 @@ def main():
         log(f"[bench] phases (us/step over {psteps} steps): { {k: round(v, 3) for k, v in phases.items()} }")
         if args.members and int(pst["engine"]) == 3:
             for m, r in enumerate(flow.member_profile(args.team or 32)):
-                log(f"[bench] member {m:2d}: wait {r[0]:.2f} countplan {r[1]:.2f} scatter {r[2]:.2f} "
-                    f"sched {r[3]:.2f} us, segments {r[4]:.2f} sinks {r[5]:.2f} per step")
+                log(f"[bench] member {m:2d}: wait {r[0]:.2f} start {r[1]:.2f} scatter {r[2]:.2f} "
+                    f"sched {r[3]:.2f} us, segments {r[4]:.2f} sinks {r[5]:.2f} per step, wave work "
+                    + " ".join(f"{x:.2f}" for x in r[6:]))
 
     forms = {}
     if not args.no_plain:

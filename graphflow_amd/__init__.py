@@ -618,18 +618,21 @@ class Flow:
         return {n: a[i] / 100.0 / steps for i, n in enumerate(names)}, int(a[8])
 
     def member_profile(self, members=32):
-        """Team engine, timed run: per member [barrier wait, counts+plan, scatter phase, scheduler
-        wave] in microseconds per step, and segments scattered / sink segments per step (run with
+        """Team engine, timed run: per member [barrier wait, step start, scatter phase, scheduler
+        wave] in microseconds per step, segments scattered / sink segments per step, then the
+        eight waves' work time from the step's start in microseconds per step (run with
         use_graph=3)."""
-        words = 32 + 64 * 8
+        stride = 16
+        words = 32 + 64 * stride
         a = (C.c_uint64 * words)()
         _raise(lib().gf_flow_phase_profile(self.handle, a, words))
         steps = max(int(a[14]), 1)
         rows = []
         for m in range(members):
-            b = 32 + 8 * m
+            b = 32 + stride * m
             rows.append([a[b] / 100.0 / steps, a[b + 1] / 100.0 / steps, a[b + 2] / 100.0 / steps,
-                         a[b + 3] / 100.0 / steps, a[b + 4] / steps, a[b + 5] / steps])
+                         a[b + 3] / 100.0 / steps, a[b + 4] / steps, a[b + 5] / steps]
+                        + [a[b + 8 + w] / 100.0 / steps for w in range(8)])
         return rows
 
     def item_passes(self):

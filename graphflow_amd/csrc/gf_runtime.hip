@@ -1407,6 +1407,7 @@ void build_engine(gf_flow& f, uint64_t N, const RunShape& sh, uint32_t src)
     for (uint32_t k = 0; k < n; ++k)
         if (arr[k].kind == KIND_PARTITION) bb = std::max(bb, nds[k].bins + 1u);
     if (bb > kTeamBins) team_ok = false;
+    bb = (bb + 3u) & ~3u;  // rows of whole 16-byte words: one 128-bit load per four slots
     const uint32_t tpc = (uint32_t)ceil_div(sh.C, kSeg);
     const uint64_t cnt_words = (uint64_t)sh.S * 2u * tpc * bb;
     if (cnt_words * 4u > (1ull << 31)) team_ok = false;
@@ -1424,6 +1425,7 @@ void build_engine(gf_flow& f, uint64_t N, const RunShape& sh, uint32_t src)
         ta.xcd = t.team_xcd;
         ta.tpc = tpc;
         ta.bb = bb;
+        ta.cnt_bytes = (uint32_t)(cnt_words * 4u);
         ta.max_steps = t.steps_per_launch ? t.steps_per_launch : 65536u;
         uint32_t rc = 1;
         while (rc * 2u <= kTeamRing / n) rc *= 2u;

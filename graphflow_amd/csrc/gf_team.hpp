@@ -106,14 +106,16 @@ struct TeamArgs {
     uint32_t* zero_g;  // [0] = n, [1..n] = chunk | (gen & 1) << 31 rows still to clear
     uint32_t* tw;      // team words, cleared before every launch
     uint32_t K, tpc, bb, max_steps, rc, n_routes;
+    uint32_t cnt_bytes;  // size of cnt (below 2 GiB)
     uint32_t c_shift;  // log2(C) when C is a power of two, else 32
     uint32_t xcd;      // XCC the team must use, plus one; 0: the XCC of the first workgroup to run
     unsigned long long* mprof;  // optional per-member wall-clock totals, kMemberProf words per member
 };
 
-// Per-member profile words: barrier wait, counts+plan, scatter phase, scheduler wave, segments
-// scattered by the worker waves, sink segments.
-enum { MP_WAIT = 0, MP_COUNTPLAN = 1, MP_SEG = 2, MP_SCHED = 3, MP_NSEG = 4, MP_NSINK = 5, kMemberProf = 8 };
+// Per-member profile words: barrier wait, step start (window move), scatter phase, scheduler wave,
+// segments scattered by the worker waves, sink segments, and from MP_WAVE on the time from the
+// step's start until each wave has finished its work.
+enum { MP_WAIT = 0, MP_COUNTPLAN = 1, MP_SEG = 2, MP_SCHED = 3, MP_NSEG = 4, MP_NSINK = 5, MP_WAVE = 8, kMemberProf = 16 };
 
 // One step's selected batches; two of them alternate so that the scheduler wave can select step
 // s+1 while the other waves still scatter step s.
@@ -154,6 +156,7 @@ struct TeamLds {
     uint32_t seg_err;      // a segment of this member reported a device status this step
     uint32_t prof_member;  // this member records sub-phase timings
     unsigned long long prof_acc[PROF_WORDS];
+    unsigned long long wave_acc[kTeamWaves];  // member profile: per-wave work time
 };
 static_assert(sizeof(TeamLds) + 64u <= 160u * 1024u, "team scheduler state must fit one CU's LDS");
 
@@ -704,19 +707,24 @@ __device__ inline void team_clear_rows(const TeamArgs& a, const TeamLds& L, cons
 
 // Per-bin sums of the count rows of partition node k's segments [g0, g1) (segments numbered within
 // the node's batch, in batch input order): lane lofs + b receives in `all` the sum of slot b over
-// [g0, g1) and in `lo` the sum over [g0, split) (b < B1; slot B1 - 1 counts invalid bins). One
-// relaxed agent-scope load per (segment, slot), two segments per lane and iteration, then one
-// wave scan per slot.
+// [g0, g1) and in `lo` the sum over [g0, split) (b < B1; slot B1 - 1 counts invalid bins). Rows are
+// bb words (a multiple of four); each lane takes two segments and loads their rows four slots at
+// a time with 128-bit agent-coherent (sc1) buffer loads, which read zero past the array's end, so
+// lanes without a segment load from there. Then one wave scan per slot.
 __device__ inline void seg_sums(const TeamArgs& a, const StepBuf& Bc, uint32_t k, uint32_t B1, uint32_t g0, uint32_t split,
                                 uint32_t g1, uint32_t lofs, uint32_t& lo, uint32_t& all)
 {
+    typedef uint32_t u32x4 __attribute__((ext_vector_type(4)));
+    constexpr int kSc1 = 16;  // cache policy: agent scope (bypasses the CU's L1)
+    const __amdgpu_buffer_rsrc_t rs = __builtin_amdgcn_make_buffer_rsrc((void*)a.cnt, (short)0, (int)a.cnt_bytes, 0x00020000);
     const uint32_t lane = lane_id(), so = Bc.seg_off[k], io = Bc.in_off[k], mi = Bc.n_in[k];
     // first segment (step numbering) and count-row base of input `lane`; batches of more than 64
     // inputs search LDS
     const gf_chunk_desc dl = lane < mi ? Bc.in[io + lane] : gf_chunk_desc{0, 0, 0, 0};
     const uint32_t fl = lane < mi ? Bc.in_seg[io + lane] : 0xFFFFFFFFu;
     const uint32_t rowl = ((dl.index * 2u + (dl.generation & 1u)) * a.tpc) * a.bb;
-    auto row_of = [&](uint32_t g) -> size_t {
+    // byte offset of the row of segment g, or one past the array when g >= g1
+    auto row_of = [&](uint32_t g) -> uint32_t {
         const uint32_t gs = so + g;
         uint32_t f = rl(fl, 0), r = rl(rowl, 0);
         if (mi <= 64u) {
@@ -736,26 +744,20 @@ __device__ inline void seg_sums(const TeamArgs& a, const StepBuf& Bc, uint32_t k
                 }
             }
         }
-        return (size_t)r + (size_t)(gs - f) * a.bb;
+        return g < g1 ? (r + (gs - f) * a.bb) * 4u : a.cnt_bytes;
     };
     lo = 0u;
     all = 0u;
     for (uint32_t h = g0; h < g1; h += 128u) {
-        const uint32_t ga = h + lane, gb = ga + 64u;
-        const bool va = ga < g1, vb = gb < g1;
-        const size_t ra = row_of(ga), rb = row_of(gb);
+        const uint32_t ra = row_of(h + lane), rb = row_of(h + 64u + lane);
         for (uint32_t c0 = 0; c0 < B1; c0 += 4u) {
-            uint32_t xa[4], xb[4];
-#pragma unroll
-            for (uint32_t q = 0; q < 4u; ++q) {
-                const bool in = c0 + q < B1;
-                xa[q] = va && in ? ld_u32(a.cnt, ra + c0 + q) : 0u;
-                xb[q] = vb && in ? ld_u32(a.cnt, rb + c0 + q) : 0u;
-            }
+            const u32x4 va = __builtin_amdgcn_raw_buffer_load_b128(rs, ra + c0 * 4u, 0, kSc1);
+            const u32x4 vb = __builtin_amdgcn_raw_buffer_load_b128(rs, rb + c0 * 4u, 0, kSc1);
 #pragma unroll
             for (uint32_t q = 0; q < 4u; ++q) {
                 if (c0 + q >= B1) break;
-                const uint32_t ia = wave_inscan(xa[q]), ib = wave_inscan(xb[q]);
+                const uint32_t xa = va[q], xb = vb[q];
+                const uint32_t ia = wave_inscan(xa), ib = wave_inscan(xb);
                 const uint32_t ta = rl(ia, 63), s = ta + rl(ib, 63);
                 uint32_t l = 0u;
                 if (split >= h + 128u) l = s;
@@ -1033,11 +1035,10 @@ __device__ inline void seg_fail(const TeamArgs& a, const TeamLds& L, uint32_t co
 // Next-node count key of a record stored at (chunk d, offset off), or ~0u when the destination is
 // not a partition.
 template <class P>
-__device__ inline uint32_t next_key(const TeamArgs& a, const TeamLds& L, uint32_t dst, const gf_chunk_desc& d, uint32_t off,
+__device__ inline uint32_t next_key(const TeamArgs& a, uint32_t dkind, const gf_node_desc& nd, const gf_chunk_desc& d, uint32_t off,
                                     const P& rec)
 {
-    if (L.kind[dst] != KIND_PARTITION) return ~0u;
-    const gf_node_desc& nd = L.nd[dst];
+    if (dkind != KIND_PARTITION) return ~0u;
     uint32_t nb = classify(nd, rec);
     if (nb >= nd.bins) nb = nd.bins;
     return (((uint32_t)d.index * 2u + (d.generation & 1u)) * a.tpc + (off >> kSegShift)) * a.bb + nb;
@@ -1053,7 +1054,7 @@ struct SegRecs {
 // Node, input chunk of the node's batch and segment of that chunk holding step segment s
 // (wave-uniform): a ballot over the nodes' segment ranges, then over the node's inputs.
 struct SegLoc {
-    uint32_t k, i, gg;
+    uint32_t k, i, gg, kind;
 };
 
 __device__ inline SegLoc locate(const TeamLds& L, const StepBuf& Bc, uint32_t n, uint32_t s)
@@ -1065,7 +1066,8 @@ __device__ inline SegLoc locate(const TeamLds& L, const StepBuf& Bc, uint32_t n,
     const uint32_t so = rl(lo, r.k);
     r.i = 0u;
     r.gg = s - so;
-    if (L.kind[r.k] == KIND_SOURCE) return r;
+    r.kind = L.kind[r.k];
+    if (r.kind == KIND_SOURCE) return r;
     const uint32_t io = Bc.in_off[r.k], m = Bc.n_in[r.k];
     for (uint32_t y0 = 0; y0 < m; y0 += 64u) {
         const uint32_t f = y0 + lane < m ? Bc.in_seg[io + y0 + lane] : 0xFFFFFFFFu;
@@ -1080,12 +1082,11 @@ __device__ inline SegLoc locate(const TeamLds& L, const StepBuf& Bc, uint32_t n,
 }
 
 template <class P>
-__device__ inline void load_segment(const TeamArgs& a, const TeamLds& L, const StepBuf& Bc, uint32_t s, SegRecs<P>& R)
+__device__ inline void load_segment(const TeamArgs& a, const StepBuf& Bc, const SegLoc& loc, uint32_t s, SegRecs<P>& R)
 {
-    const SegLoc loc = locate(L, Bc, a.e.n_nodes, s);
     const uint32_t k = loc.k, i = loc.i, gg = loc.gg;
     R.s = ~0u;
-    if (L.kind[k] == KIND_SOURCE) return;
+    if (loc.kind == KIND_SOURCE) return;
     const gf_chunk_desc d = Bc.in[Bc.in_off[k] + i];
     const uint32_t off0 = gg * kSeg, cnt = d.fill - off0 < kSeg ? d.fill - off0 : kSeg;
     const size_t slot0 = (size_t)d.index * a.e.C + off0;
@@ -1163,6 +1164,8 @@ struct Place {
     uint32_t gs;   // node-local segment `pre` belongs to
     uint32_t bb;   // lane b: binbase of bin b times C
     uint32_t pre;  // lane b: items of bin b in the node's segments before gs
+    uint32_t fs0;  // LDS index of the step's pop 0 (the free stack's top entry when the step starts)
+    uint32_t room; // pops the LDS window holds
 };
 
 // Output count of partition node k from its count rows.
@@ -1193,6 +1196,8 @@ __device__ inline void place_node(const TeamArgs& a, const TeamLds& L, const Ste
     }
     pl.k = k;
     pl.gs = gs;
+    pl.room = Bc.top0 - L.fs_base;
+    pl.fs0 = pl.room - 1u;
     uint32_t base = Bc.ob[k];
     for (uint32_t pmk = Bc.part_mask & ((1u << k) - 1u); pmk; pmk &= pmk - 1u)
         base += part_outputs(a, L, Bc, (uint32_t)(__ffs(pmk) - 1));
@@ -1220,11 +1225,10 @@ __device__ inline void place_node(const TeamArgs& a, const TeamLds& L, const Ste
 
 // Output chunk of pop j of the step (entry top0 - 1 - j of the free stack), or false when the
 // entry is not in the LDS window (only a failed plan gets there).
-__device__ inline bool pop_desc(const TeamLds& L, const StepBuf& Bc, uint32_t j, gf_chunk_desc& od)
+__device__ inline bool pop_desc(const TeamLds& L, const Place& pl, uint32_t j, gf_chunk_desc& od)
 {
-    const uint32_t top0 = Bc.top0, fb = L.fs_base;
-    if (j >= top0 - fb) return false;
-    const unsigned long long ent = L.fs[top0 - 1u - j - fb];
+    if (j >= pl.room) return false;
+    const unsigned long long ent = L.fs[pl.fs0 - j];
     od.index = (uint32_t)ent;
     od.generation = (uint32_t)(ent >> 32);
     od.fill = 0u;
@@ -1239,7 +1243,7 @@ __device__ void team_partition_seg(const TeamArgs& a, const TeamLds& L, const St
     const Engine& e = a.e;
     const uint32_t C = e.C, lane = lane_id();
     if (t.k != k) load_bintab(L, k, t);
-    const gf_node_desc& nd = L.nd[k];
+    const gf_node_desc nd = L.nd[k];
     const uint32_t B = nd.bins;
     const gf_chunk_desc d = Bc.in[Bc.in_off[k] + i];
     const uint32_t off0 = gg * kSeg, cnt = d.fill - off0 < kSeg ? d.fill - off0 : kSeg;
@@ -1275,7 +1279,7 @@ __device__ void team_partition_seg(const TeamArgs& a, const TeamLds& L, const St
         if (!v[r]) continue;
         const uint32_t q = div_c(flat, C, a.c_shift), off = flat - q * C;
         gf_chunk_desc od;
-        if (q >= pl.R || !pop_desc(L, Bc, pl.base + q, od)) {
+        if (q >= pl.R || !pop_desc(L, pl, pl.base + q, od)) {
             seg_fail(a, L, GF_PLAN_MISMATCH);
             continue;
         }
@@ -1306,11 +1310,12 @@ __device__ void team_transform_seg(const TeamArgs& a, const TeamLds& L, const St
 {
     const Engine& e = a.e;
     const uint32_t C = e.C, lane = lane_id();
-    const gf_node_desc& nd = L.nd[k];
+    const gf_node_desc nd = L.nd[k];
     const gf_chunk_desc d = Bc.in[Bc.in_off[k] + i];
     const uint32_t off0 = gg * kSeg, cnt = d.fill - off0 < kSeg ? d.fill - off0 : kSeg;
     const uint32_t pos0 = Bc.in_items[Bc.in_off[k] + i] + off0;
-    const uint32_t dst = (uint32_t)L.route[L.route_off[k]];
+    const uint32_t dst = (uint32_t)L.route[L.route_off[k]], dkind = L.kind[dst], purpose = L.purpose[k];
+    const gf_node_desc dnd = L.nd[dst];
 #pragma unroll
     for (uint32_t r = 0; r < kSegRounds; ++r) {
         const uint32_t it = r * 64u + lane;
@@ -1318,11 +1323,11 @@ __device__ void team_transform_seg(const TeamArgs& a, const TeamLds& L, const St
         if (it < cnt) {
             const uint32_t g = pos0 + it, q = div_c(g, C, a.c_shift), off = g - q * C;
             gf_chunk_desc od;
-            if (q < pl.R && pop_desc(L, Bc, pl.base + q, od)) {
+            if (q < pl.R && pop_desc(L, pl, pl.base + q, od)) {
                 P p = R.rec[r];
-                transform(nd, e.rp, L.purpose[k], p);
+                transform(nd, e.rp, purpose, p);
                 p.store(e.cols, (size_t)od.index * C + off);
-                key = next_key(a, L, dst, od, off, p);
+                key = next_key(a, dkind, dnd, od, off, p);
             } else {
                 seg_fail(a, L, GF_PLAN_MISMATCH);
             }
@@ -1336,14 +1341,15 @@ __device__ void team_source_seg(const TeamArgs& a, const TeamLds& L, const StepB
 {
     const Engine& e = a.e;
     const uint32_t C = e.C, lane = lane_id();
-    const gf_node_desc& nd = L.nd[k];
+    const gf_node_desc nd = L.nd[k];
     gf_chunk_desc od;
-    if (!pop_desc(L, Bc, pl.base, od)) {
+    if (!pop_desc(L, pl, pl.base, od)) {
         seg_fail(a, L, GF_PLAN_MISMATCH);
         return;
     }
     const uint32_t fill = Bc.particles[k], off0 = gg * kSeg, cnt = fill - off0 < kSeg ? fill - off0 : kSeg;
-    const uint32_t dst = (uint32_t)L.route[L.route_off[k]];
+    const uint32_t dst = (uint32_t)L.route[L.route_off[k]], dkind = L.kind[dst], purpose = L.purpose[k];
+    const gf_node_desc dnd = L.nd[dst];
     const unsigned long long id0 = Bc.first_id[k] + off0;
 #pragma unroll
     for (uint32_t r = 0; r < kSegRounds; ++r) {
@@ -1351,9 +1357,9 @@ __device__ void team_source_seg(const TeamArgs& a, const TeamLds& L, const StepB
         uint32_t key = ~0u;
         if (it < cnt) {
             P p;
-            source(nd, e.rp, L.purpose[k], id0 + it, p);
+            source(nd, e.rp, purpose, id0 + it, p);
             p.store(e.cols, (size_t)od.index * C + off0 + it);
-            key = next_key(a, L, dst, od, off0 + it, p);
+            key = next_key(a, dkind, dnd, od, off0 + it, p);
         }
         agg_add(key != ~0u, key, a.cnt);
     }
@@ -1364,7 +1370,7 @@ __device__ void team_sink_seg(const TeamArgs& a, const TeamLds& L, const StepBuf
                               const SegRecs<P>& R)
 {
     const uint32_t lane = lane_id();
-    const gf_node_desc& nd = L.nd[k];
+    const gf_node_desc nd = L.nd[k];
     const gf_chunk_desc d = Bc.in[Bc.in_off[k] + i];
     const uint32_t off0 = gg * kSeg, cnt = d.fill - off0 < kSeg ? d.fill - off0 : kSeg;
     if constexpr (std::is_same_v<P, Particle>) {
@@ -1437,42 +1443,31 @@ __device__ inline void team_sinks(const TeamArgs& a, const TeamLds& L, const Ste
     SegRecs<P> R;
     for (uint32_t s = lo; s < hi; ++s) {
         const SegLoc loc = locate(L, Bc, a.e.n_nodes, s);
-        load_segment<P>(a, L, Bc, s, R);
+        load_segment<P>(a, Bc, loc, s, R);
         team_sink_seg<P>(a, L, Bc, loc.k, loc.i, loc.gg, R);
     }
 }
 
-template <class P>
-__device__ inline void team_prefetch(const TeamArgs& a, const TeamLds& L, const StepBuf& Bc, uint32_t me, uint32_t T, uint32_t w,
-                                     SegRecs<P>& R)
-{
-    uint32_t lo, hi;
-    member_range(Bc, a.e.n_nodes, me, T, &lo, &hi);
-    R.s = ~0u;
-    if (lo + w < hi) load_segment<P>(a, L, Bc, lo + w, R);
-}
-
+// Worker wave w: segments lo + w, lo + w + kWorkers, ... of this member. The first segment's
+// records are requested before its count rows, so both loads are in flight together.
 template <class P>
 __device__ inline void team_segments(const TeamArgs& a, const TeamLds& L, const StepBuf& Bc, uint32_t me, uint32_t T, uint32_t w,
-                                     SegRecs<P>& R, BinTab& t)
+                                     BinTab& t)
 {
     uint32_t lo, hi;
     member_range(Bc, a.e.n_nodes, me, T, &lo, &hi);
     Place pl;
     pl.k = ~0u;
+    SegRecs<P> R;
     for (uint32_t s = lo + w; s < hi; s += kWorkers) {
         const SegLoc loc = locate(L, Bc, a.e.n_nodes, s);
-        const uint32_t k = loc.k, i = loc.i, gg = loc.gg;
-        const uint32_t kind = L.kind[k];
+        load_segment<P>(a, Bc, loc, s, R);
+        const uint32_t k = loc.k, kind = loc.kind;
         place_node(a, L, Bc, k, kind, s - Bc.seg_off[k], pl);
         if (!pl.ok) continue;
-        if (kind == KIND_SOURCE) {
-            team_source_seg<P>(a, L, Bc, k, gg, pl);
-            continue;
-        }
-        if (R.s != s) load_segment<P>(a, L, Bc, s, R);
-        if (kind == KIND_PARTITION) team_partition_seg<P>(a, L, Bc, k, i, gg, R, t, pl);
-        else team_transform_seg<P>(a, L, Bc, k, i, gg, R, pl);
+        if (kind == KIND_SOURCE) team_source_seg<P>(a, L, Bc, k, loc.gg, pl);
+        else if (kind == KIND_PARTITION) team_partition_seg<P>(a, L, Bc, k, loc.i, loc.gg, R, t, pl);
+        else team_transform_seg<P>(a, L, Bc, k, loc.i, loc.gg, R, pl);
     }
 }
 
@@ -1509,6 +1504,7 @@ __global__ __launch_bounds__(kTeamThreads) void k_team(TeamArgs a)
     };
     if (tid == 0u) L.prof_member = (e.prof != nullptr && me == 0u) ? 1u : 0u;
     if (tid < PROF_WORDS) L.prof_acc[tid] = 0ull;
+    if (tid < kTeamWaves) L.wave_acc[tid] = 0ull;
     team_load(a, L);
     if (L.exit_code == EXIT_SKIP) {
         if (me == 0u && tid == 0u) e.ctl[CTL_HANDOFF] = 1u;
@@ -1522,7 +1518,6 @@ __global__ __launch_bounds__(kTeamThreads) void k_team(TeamArgs a)
     __syncthreads();
     uint32_t target = 0;
     unsigned long long cyc = 0;
-    SegRecs<P> R;
     BinTab tab;
     tab.k = ~0u;
     int32_t last = -1;  // last executed step of this launch
@@ -1548,6 +1543,7 @@ __global__ __launch_bounds__(kTeamThreads) void k_team(TeamArgs a)
         if (Bc.recenter) team_recenter(a, L, me);
         mark(PROF_T_COUNTS);
         mmark(MP_COUNTPLAN);
+        const unsigned long long ts = mp ? wall_clock64() : 0ull;
         const unsigned long long w0 = wprof ? wall_clock64() : 0ull;
         const unsigned long long w0m = mp ? wall_clock64() : 0ull;
         if (mp && tid == 0u) {
@@ -1558,9 +1554,7 @@ __global__ __launch_bounds__(kTeamThreads) void k_team(TeamArgs a)
             atomicAdd(&mp[MP_NSINK], (unsigned long long)((uint64_t)(me + 1u) * Gs / T - (uint64_t)me * Gs / T));
         }
         if (w < kWorkers) {
-            R.s = ~0u;
-            team_prefetch<P>(a, L, Bc, me, T, w, R);
-            team_segments<P>(a, L, Bc, me, T, w, R, tab);
+            team_segments<P>(a, L, Bc, me, T, w, tab);
             if (wprof && w == 0u) atomicAdd(&L.prof_acc[PROF_T_SEGW], wall_clock64() - w0);
         } else if (w == kSinkWave) {
             team_sinks<P>(a, L, Bc, me, T);
@@ -1579,6 +1573,7 @@ __global__ __launch_bounds__(kTeamThreads) void k_team(TeamArgs a)
             }
             if (mp && lane_id() == 0u) atomicAdd(&mp[MP_SCHED], wall_clock64() - w0m);
         }
+        if (mp && lane_id() == 0u) atomicAdd(&L.wave_acc[w], wall_clock64() - ts);
         if (w != kSinkWave) __builtin_amdgcn_s_waitcnt(0);
         __syncthreads();
         target += T;
@@ -1603,6 +1598,7 @@ __global__ __launch_bounds__(kTeamThreads) void k_team(TeamArgs a)
     }
     if (w == kSchedWave) sched_store(a, L, sg);
     __syncthreads();
+    if (mp && tid < kTeamWaves) mp[MP_WAVE + tid] = L.wave_acc[tid];
     if (me == 0u && !L.status && !L.err_prev) team_flush(a, L, last >= 0 ? &L.sb[last & 1] : nullptr);
     if (e.prof != nullptr && me == 0u) {
         __syncthreads();
