@@ -52,7 +52,7 @@ constexpr uint32_t kTeamMargin = 512;  // pops or pushes of one step
 constexpr uint32_t kTeamIn = 512;      // input chunks of one step
 constexpr uint32_t kTeamOut = 512;     // output chunks of one step
 constexpr uint32_t kTeamSegs = 1024;   // segments of one step
-constexpr uint32_t kTeamZero = 512;    // consumed count rows cleared in the next step
+constexpr uint32_t kTeamZero = 512;    // count rows consumed by one step's partitions, cleared in the next
 constexpr uint32_t kTeamWaves = 8;     // waves 0-5 scatter segments, wave 6 schedules, wave 7 sinks
 constexpr uint32_t kWorkers = 6;
 constexpr uint32_t kSchedWave = 6;
@@ -149,7 +149,7 @@ struct TeamLds {
     uint32_t out_ridx[kTeamOut];  // route-table entry of output j
     uint32_t tot[kTeamNodes * kTeamBins];
     uint32_t binbase[kTeamNodes * kTeamBins];
-    uint32_t zero[kTeamZero];  // count rows to clear in the launch's first step
+    uint32_t zero[kTeamZero];  // count rows to clear in the next step: chunk | (generation & 1) << 31
     unsigned long long generated, sunk, step, batches, part_items, xf_items, src_items, sink_items, peak_live, trace_len;
     unsigned long long digest[kTeamNodes];
     uint32_t fs_base, top, zero_n, status, exit_code, total_out, err_prev;
@@ -337,9 +337,9 @@ __device__ inline void team_load(const TeamArgs& a, TeamLds& L)
 }
 
 // Member 0 writes the pre-select state back in the wide engine's layout, with the count rows the
-// last executed step consumed (cleared by the next launch's first step): the partition inputs of
-// Bl, or the rows this launch was handed when it executed no step.
-__device__ inline void team_flush(const TeamArgs& a, TeamLds& L, const StepBuf* Bl)
+// last executed step consumed (cleared by the next launch's first step), or the rows this launch was
+// handed when it executed no step.
+__device__ inline void team_flush(const TeamArgs& a, TeamLds& L)
 {
     const Engine& e = a.e;
     const uint32_t tid = threadIdx.x, n = e.n_nodes, rcm = a.rc - 1u;
@@ -356,23 +356,9 @@ __device__ inline void team_flush(const TeamArgs& a, TeamLds& L, const StepBuf* 
         e.ctr[CNT_DIGEST0 + tid] = L.digest[tid];
     }
     for (uint32_t i = tid; i < L.top - L.fs_base; i += kTeamThreads) e.free_stack[L.fs_base + i] = L.fs[i];
-    uint32_t zn = 0;
-    if (Bl) {
-        for (uint32_t pmk = Bl->part_mask; pmk; pmk &= pmk - 1u) {
-            const uint32_t k = (uint32_t)(__ffs(pmk) - 1);
-            const uint32_t io = Bl->in_off[k], m = Bl->n_in[k];
-            for (uint32_t i = tid; i < m; i += kTeamThreads) {
-                const gf_chunk_desc d = Bl->in[io + i];
-                a.zero_g[1 + zn + i] = d.index | (d.generation & 1u) << 31;
-            }
-            zn += m;
-        }
-    } else {
-        for (uint32_t i = tid; i < L.zero_n; i += kTeamThreads) a.zero_g[1 + i] = L.zero[i];
-        zn = L.zero_n;
-    }
+    for (uint32_t i = tid; i < L.zero_n; i += kTeamThreads) a.zero_g[1 + i] = L.zero[i];
     if (tid == 0) {
-        a.zero_g[0] = zn;
+        a.zero_g[0] = L.zero_n;
         e.ctl[CTL_TOP] = L.top;
         e.ctr[CNT_GENERATED] = L.generated;
         e.ctr[CNT_SUNK] = L.sunk;
@@ -680,28 +666,16 @@ __device__ inline void team_recenter(const TeamArgs& a, TeamLds& L, uint32_t me)
 }
 
 // ---- count rows ----------------------------------------------------------------------------------
-// Wave 6: clear this member's share of the count rows the previous step's partitions consumed (or,
+// Sink wave: clear this member's share of the count rows the previous step's partitions consumed (or,
 // in a launch's first step, the rows the previous launch left). A chunk's rows are one contiguous
 // block of tpc * bb words; its next use as a partition input has the other generation parity, so
 // the rows are zero again before any producer counts into them.
-__device__ inline void team_clear_rows(const TeamArgs& a, const TeamLds& L, const StepBuf* Bp, uint32_t me, uint32_t T)
+__device__ inline void team_clear_rows(const TeamArgs& a, const TeamLds& L, uint32_t me, uint32_t T)
 {
-    const uint32_t lane = lane_id(), blk = a.tpc * a.bb;
-    if (Bp) {
-        for (uint32_t pmk = Bp->part_mask; pmk; pmk &= pmk - 1u) {
-            const uint32_t k = (uint32_t)(__ffs(pmk) - 1);
-            const uint32_t io = Bp->in_off[k], zw = Bp->n_in[k] * blk;
-            for (uint32_t x = me * 64u + lane; x < zw; x += T * 64u) {
-                const gf_chunk_desc d = Bp->in[io + x / blk];
-                a.cnt[((size_t)d.index * 2u + (d.generation & 1u)) * blk + x % blk] = 0u;
-            }
-        }
-    } else {
-        const uint32_t zw = L.zero_n * blk;
-        for (uint32_t x = me * 64u + lane; x < zw; x += T * 64u) {
-            const uint32_t z = L.zero[x / blk];
-            a.cnt[((size_t)(z & 0x7FFFFFFFu) * 2u + (z >> 31)) * blk + x % blk] = 0u;
-        }
+    const uint32_t lane = lane_id(), blk = a.tpc * a.bb, zw = L.zero_n * blk;
+    for (uint32_t x = me * 64u + lane; x < zw; x += T * 64u) {
+        const uint32_t z = L.zero[x / blk];
+        a.cnt[((size_t)(z & 0x7FFFFFFFu) * 2u + (z >> 31)) * blk + x % blk] = 0u;
     }
 }
 
@@ -785,7 +759,7 @@ __device__ inline void team_plan(const TeamArgs& a, TeamLds& L, const StepBuf& B
     const uint32_t k = own ? lane : 0u;
     const uint32_t act = own ? Bc.active[k] : 0u, kind = g.kind;
     const uint32_t parts = own ? Bc.particles[k] : 0u, bound = own ? L.bound[k] : 0u, ro = own ? L.route_off[k] : 0u;
-    const uint32_t pm = Bc.part_mask;
+    const uint32_t pm = Bc.part_mask, reserved = Bc.reserved, top = L.top, fbase = L.fs_base;
     const uint32_t kp = pm ? (uint32_t)(__ffs(pm) - 1) : 0u;
     const uint32_t B = pm ? rl(g.bins, kp) : 0u;
     uint32_t t = 0u;
@@ -840,8 +814,8 @@ __device__ inline void team_plan(const TeamArgs& a, TeamLds& L, const StepBuf& B
     const uint64_t bad = __ballot(st != 0u);
     uint32_t status = inv ? (uint32_t)GF_INVALID : bad ? rl(st, (uint32_t)(__ffsll((unsigned long long)bad) - 1)) : 0u;
     const uint32_t oi = wave_inscan(own ? R : 0u), off = oi - R, total = rl(oi, 63);
-    if (!status && total > Bc.reserved) status = GF_PLAN_MISMATCH;
-    if (!status && total > L.top) status = GF_POOL_EXHAUSTED;
+    if (!status && total > reserved) status = GF_PLAN_MISMATCH;
+    if (!status && total > top) status = GF_POOL_EXHAUSTED;
     if (own) {
         L.n_out[k] = R;
         L.out_off[k] = off;
@@ -855,7 +829,6 @@ __device__ inline void team_plan(const TeamArgs& a, TeamLds& L, const StepBuf& B
     tk(PROF_C_PLAN_NODES);
     if (!status) {
         // output j of the step (node order, bin-major, k order) takes stack[top - 1 - j]
-        const uint32_t top = L.top, fbase = L.fs_base;
         auto emit = [&](uint32_t j0, uint32_t cnt, uint32_t fill0, uint32_t rem0, uint32_t bin, uint32_t ridx) {
             // outputs j0 .. j0+cnt-1: fills C, C, ..., the rest of rem0 (fill0 = C or the whole)
             for (uint32_t q = lane; q < cnt; q += 64u) {
@@ -1013,7 +986,8 @@ __device__ inline void team_retire(const TeamArgs& a, TeamLds& L, StepBuf& Bc, S
 }
 
 // Wave 6, after the step's workgroup barrier: the step's consumed inputs back on the free stack
-// (generation + 1), where its pops were taken from.
+// (generation + 1), where its pops were taken from, and the partition inputs' count rows on the
+// list the sink wave clears in the next step.
 __device__ inline void team_push(const TeamArgs& a, TeamLds& L, const StepBuf& Bc)
 {
     const uint32_t lane = lane_id(), in_tot = Bc.in_off[a.e.n_nodes];
@@ -1022,6 +996,18 @@ __device__ inline void team_push(const TeamArgs& a, TeamLds& L, const StepBuf& B
         const gf_chunk_desc d = Bc.in[x];
         L.fs[ptop + x - fb] = (unsigned long long)d.index | ((unsigned long long)(d.generation + 1u) << 32);
     }
+    uint32_t zn = 0;
+    for (uint32_t pmk = Bc.part_mask; pmk; pmk &= pmk - 1u) {
+        const uint32_t k = (uint32_t)(__ffs(pmk) - 1);
+        const uint32_t io = Bc.in_off[k], m = Bc.n_in[k];
+        for (uint32_t i = lane; i < m; i += 64u) {
+            const gf_chunk_desc d = Bc.in[io + i];
+            L.zero[zn + i] = d.index | (d.generation & 1u) << 31;
+        }
+        zn += m;
+    }
+    if (lane == 0) L.zero_n = zn;
+    wave_sync();
 }
 
 // A failed item: the run's status word for the host, this member's flag for the team's stop decision.
@@ -1432,9 +1418,8 @@ __device__ inline void member_range(const StepBuf& Bc, uint32_t n, uint32_t me, 
     *hi = (uint32_t)((uint64_t)(me + 1u) * G / T);
 }
 
-// Sink segments of this member, taken by its sink wave. The sink wave does not wait for its
-// atomics and stores before the barrier (nothing in the run reads them); its loads complete before
-// it arrives because every sink op consumes the loaded record.
+// Sink segments of this member, taken by its sink wave (after it cleared its share of the count
+// rows).
 template <class P>
 __device__ inline void team_sinks(const TeamArgs& a, const TeamLds& L, const StepBuf& Bc, uint32_t me, uint32_t T)
 {
@@ -1520,7 +1505,6 @@ __global__ __launch_bounds__(kTeamThreads) void k_team(TeamArgs a)
     unsigned long long cyc = 0;
     BinTab tab;
     tab.k = ~0u;
-    int32_t last = -1;  // last executed step of this launch
     for (uint32_t it = 0;; ++it) {
         StepBuf& Bc = L.sb[it & 1u];
         StepBuf& Bn = L.sb[(it + 1u) & 1u];  // the previous step's batches until select overwrites them
@@ -1557,10 +1541,10 @@ __global__ __launch_bounds__(kTeamThreads) void k_team(TeamArgs a)
             team_segments<P>(a, L, Bc, me, T, w, tab);
             if (wprof && w == 0u) atomicAdd(&L.prof_acc[PROF_T_SEGW], wall_clock64() - w0);
         } else if (w == kSinkWave) {
+            team_clear_rows(a, L, me, T);
             team_sinks<P>(a, L, Bc, me, T);
         } else {
             // the scheduler wave plans this step, retires it and selects the next one
-            team_clear_rows(a, L, it ? &Bn : nullptr, me, T);
             team_plan(a, L, Bc, sg);
             if (!L.status) {
                 if (me == 0u && e.trace_on) team_trace(a, L, Bc);
@@ -1574,7 +1558,7 @@ __global__ __launch_bounds__(kTeamThreads) void k_team(TeamArgs a)
             if (mp && lane_id() == 0u) atomicAdd(&mp[MP_SCHED], wall_clock64() - w0m);
         }
         if (mp && lane_id() == 0u) atomicAdd(&L.wave_acc[w], wall_clock64() - ts);
-        if (w != kSinkWave) __builtin_amdgcn_s_waitcnt(0);
+        __builtin_amdgcn_s_waitcnt(0);
         __syncthreads();
         target += T;
         if (tid == 0u) {
@@ -1588,7 +1572,6 @@ __global__ __launch_bounds__(kTeamThreads) void k_team(TeamArgs a)
         if (L.status) break;
         if (w == kSchedWave) team_push(a, L, Bc);
         mark(PROF_T_SELECT);
-        last = (int32_t)it;
         if (prof) {
             L.prof_acc[PROF_T_STEPS] += 1u;
             const unsigned long long c = __builtin_readcyclecounter();
@@ -1599,7 +1582,7 @@ __global__ __launch_bounds__(kTeamThreads) void k_team(TeamArgs a)
     if (w == kSchedWave) sched_store(a, L, sg);
     __syncthreads();
     if (mp && tid < kTeamWaves) mp[MP_WAVE + tid] = L.wave_acc[tid];
-    if (me == 0u && !L.status && !L.err_prev) team_flush(a, L, last >= 0 ? &L.sb[last & 1] : nullptr);
+    if (me == 0u && !L.status && !L.err_prev) team_flush(a, L);
     if (e.prof != nullptr && me == 0u) {
         __syncthreads();
         if (tid < PROF_WORDS) e.prof[tid] += L.prof_acc[tid];
