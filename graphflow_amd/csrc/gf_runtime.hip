@@ -902,6 +902,7 @@ struct EngineBuffers {
     // team engine
     bool team = false;
     TeamArgs ta{};
+    TeamArgs* ta_dev = nullptr;  // the launch's arguments, read by the kernel from device memory
     size_t cnt_bytes = 0;
     hipGraphExec_t exec = nullptr;
     uint32_t graph_steps = 0;
@@ -1417,6 +1418,7 @@ void build_engine(gf_flow& f, uint64_t N, const RunShape& sh, uint32_t src)
     if (eb->team) {
         TeamArgs& ta = eb->ta;
         ta.cnt = m.d<uint32_t>(cnt_words);
+        eb->ta_dev = m.d<TeamArgs>(1);
         ta.zero_g = m.d<uint32_t>(1u + kTeamZero);
         ta.tw = m.d<uint32_t>(TW_WORDS);
         // default team: every CU of one XCD (the device's CUs over its 8 XCDs)
@@ -1540,12 +1542,13 @@ void run_team(gf_flow& f, EngineBuffers& eb, uint64_t N, gf_stats& st, bool time
         f.kt_n[i] = 0;
     }
     const uint32_t grid = 8u * ta.K;
+    copy_sync(eb.ta_dev, &ta, sizeof(TeamArgs), hipMemcpyHostToDevice, s);
     for (;;) {
         HIPX(hipMemsetAsync(ta.tw, 0, TW_WORDS * sizeof(uint32_t), s));
         if (timed) HIPX(hipEventRecord(ev[0], s));
         with_payload(f.payload, 1u, [&](auto pt, auto) {
             using P = typename decltype(pt)::type;
-            hipLaunchKernelGGL(k_team<P>, dim3(grid), dim3(kTeamThreads), sizeof(TeamLds), s, ta);
+            hipLaunchKernelGGL(k_team<P>, dim3(grid), dim3(kTeamThreads), sizeof(TeamLds), s, (const TeamArgs*)eb.ta_dev);
         });
         HIPX(hipGetLastError());
         if (timed) HIPX(hipEventRecord(ev[1], s));

@@ -1457,9 +1457,22 @@ __device__ inline void team_segments(const TeamArgs& a, const TeamLds& L, const 
 }
 
 // ---- the persistent team kernel ---------------------------------------------------------------
-template <class P>
-__global__ __launch_bounds__(kTeamThreads) void k_team(TeamArgs a)
+// A copy of `p` the compiler cannot see through: loads of the arguments through it stay inside the
+// step loop, where the scalar cache serves them, instead of holding every argument any wave role
+// uses in scalar registers for the whole launch (which spills them into vector-register lanes).
+typedef const TeamArgs __attribute__((address_space(4))) ConstTeamArgs;  // constant address space: scalar loads
+__device__ inline const TeamArgs& fresh_args(const TeamArgs* ga)
 {
+    ConstTeamArgs* p = (ConstTeamArgs*)ga;
+    asm volatile("" : "+s"(p));
+    return *(const TeamArgs*)p;
+}
+
+// The launch's arguments are read from device memory (`ga`).
+template <class P>
+__global__ __launch_bounds__(kTeamThreads) void k_team(const TeamArgs* __restrict__ ga)
+{
+    const TeamArgs& a = *ga;
     extern __shared__ char smem[];
     TeamLds& L = *reinterpret_cast<TeamLds*>(smem);
     __shared__ uint32_t sh[4];
@@ -1506,6 +1519,8 @@ __global__ __launch_bounds__(kTeamThreads) void k_team(TeamArgs a)
     BinTab tab;
     tab.k = ~0u;
     for (uint32_t it = 0;; ++it) {
+        const TeamArgs& a = fresh_args(ga);
+        const Engine& e = a.e;
         StepBuf& Bc = L.sb[it & 1u];
         StepBuf& Bn = L.sb[(it + 1u) & 1u];  // the previous step's batches until select overwrites them
         if (it) {
