@@ -22,10 +22,10 @@
 //   * waves 0-5 each load their segment's records and the count rows of the segment's node, derive
 //     from them alone where the segment's items go (bin totals -> output chunk of each bin, the
 //     prefix of the segment's earlier items -> slot) and scatter;
-//   * wave 6 loads the same count rows, plans the whole step (pops the output chunks), retires it
+//   * wave 7 loads the same count rows, plans the whole step (pops the output chunks), retires it
 //     and selects the next step into the other half of the double-buffered batch state;
-//   * wave 7 runs the sink segments.
-// Then every wave waits for its stores, the member arrives, and wave 6 pushes the step's inputs on
+//   * wave 6 runs the sink segments.
+// Then every wave waits for its stores, the member arrives, and wave 7 pushes the step's inputs on
 // the free stack (after the workgroup barrier, because the scatter waves read the popped entries).
 //
 // Coherence: stores are plain vector stores (they write through the CU's L1 into the XCD's L2);
@@ -53,10 +53,13 @@ constexpr uint32_t kTeamIn = 512;      // input chunks of one step
 constexpr uint32_t kTeamOut = 512;     // output chunks of one step
 constexpr uint32_t kTeamSegs = 1024;   // segments of one step
 constexpr uint32_t kTeamZero = 512;    // count rows consumed by one step's partitions, cleared in the next
-constexpr uint32_t kTeamWaves = 8;     // waves 0-5 scatter segments, wave 6 schedules, wave 7 sinks
+constexpr uint32_t kTeamWaves = 8;     // waves 0-5 scatter segments, wave 6 sinks, wave 7 schedules
 constexpr uint32_t kWorkers = 6;
-constexpr uint32_t kSchedWave = 6;
-constexpr uint32_t kSinkWave = 7;
+constexpr uint32_t kSchedWave = 7;
+constexpr uint32_t kSinkWave = 6;
+// Order in which the worker waves take a member's segments. Waves w and w + 4 share a SIMD; the
+// wave beside the scheduler (wave 3) comes last, so the scheduler rarely competes for issue slots.
+__device__ constexpr uint32_t worker_rank(uint32_t w) { return w < 3u ? w : w == 3u ? 5u : w - 1u; }
 constexpr uint32_t kClrThreads = 64u * 4u;  // threads of member 0 that spill the free-stack window
 constexpr uint32_t kTeamThreads = 64u * kTeamWaves;
 constexpr uint32_t kTeamSpin = 1u << 24;
@@ -381,7 +384,7 @@ __device__ inline void team_flush(const TeamArgs& a, TeamLds& L)
     }
 }
 
-// ---- scheduler registers (wave 6) ---------------------------------------------------------------
+// ---- scheduler registers (wave 7) ---------------------------------------------------------------
 // Lane k = node k, kept in registers for the whole launch: ring cursors, the fill and segment sums
 // of the node's ready chunks, the sink digest count and the node's static kind and bins.
 struct SchedRegs {
@@ -439,7 +442,7 @@ __device__ inline void sched_store(const TeamArgs& a, TeamLds& L, const SchedReg
     wave_sync();
 }
 
-// ---- select (wave 6): the next step's batches into Bn, or the reason to stop ------------------
+// ---- select (wave 7): the next step's batches into Bn, or the reason to stop ------------------
 // Credits are handed out in node order by a scalar loop over the nodes that may take a batch; a
 // node that takes its whole ring needs no ring reads (the ring sums are in registers).
 __device__ inline void team_select(const TeamArgs& a, TeamLds& L, StepBuf& Bn, bool at_limit, SchedRegs& g)
@@ -746,7 +749,7 @@ __device__ inline void seg_sums(const TeamArgs& a, const StepBuf& Bc, uint32_t k
     }
 }
 
-// ---- plan (wave 6) ------------------------------------------------------------------------------
+// ---- plan (wave 7) ------------------------------------------------------------------------------
 // Every node's exact output count from the count rows (partitions) or the batch (sources,
 // transforms), the checks of the batch, and the step's output chunks popped off the free stack.
 __device__ inline void team_plan(const TeamArgs& a, TeamLds& L, const StepBuf& Bc, const SchedRegs& g)
@@ -892,7 +895,7 @@ __device__ inline void team_plan(const TeamArgs& a, TeamLds& L, const StepBuf& B
     tk(PROF_C_PLAN_POPS);
 }
 
-// ---- trace (member 0) and retire (wave 6 of every member) --------------------------------------
+// ---- trace (member 0) and retire (wave 7 of every member) --------------------------------------
 __device__ inline void team_trace(const TeamArgs& a, TeamLds& L, const StepBuf& Bc)
 {
     const Engine& e = a.e;
@@ -936,7 +939,7 @@ __device__ inline void team_trace(const TeamArgs& a, TeamLds& L, const StepBuf& 
     }
 }
 
-// Retire (wave 6 of every member): outputs to the destination rings in node and output order,
+// Retire (wave 7 of every member): outputs to the destination rings in node and output order,
 // the free-stack top and the counters.
 __device__ inline void team_retire(const TeamArgs& a, TeamLds& L, StepBuf& Bc, SchedRegs& g)
 {
@@ -1145,7 +1148,7 @@ __device__ inline uint32_t seg_rank(bool v, uint32_t bin, uint32_t& run)
 // the items of bin b in segment gs start at flat slot bb_b + pre_b of the node's outputs.
 struct Place {
     uint32_t k;    // node described (wave-uniform), or ~0u
-    uint32_t ok;   // the node's batch agrees with its counts (otherwise wave 6 fails the step)
+    uint32_t ok;   // the node's batch agrees with its counts (otherwise wave 7 fails the step)
     uint32_t base, R;
     uint32_t gs;   // node-local segment `pre` belongs to
     uint32_t bb;   // lane b: binbase of bin b times C
@@ -1409,7 +1412,7 @@ __device__ void team_sink_seg(const TeamArgs& a, const TeamLds& L, const StepBuf
 }
 
 // Segments [lo, hi) of this member (the step's non-sink segments split evenly over the members):
-// worker wave w takes lo + w, lo + w + kWorkers, ...
+// worker wave w takes lo + r, lo + r + kWorkers, ... with r = worker_rank(w).
 __device__ inline void member_range(const StepBuf& Bc, uint32_t n, uint32_t me, uint32_t T, uint32_t* lo, uint32_t* hi)
 {
     (void)n;
@@ -1433,8 +1436,8 @@ __device__ inline void team_sinks(const TeamArgs& a, const TeamLds& L, const Ste
     }
 }
 
-// Worker wave w: segments lo + w, lo + w + kWorkers, ... of this member. The first segment's
-// records are requested before its count rows, so both loads are in flight together.
+// Worker wave w: segments lo + r, lo + r + kWorkers, ... of this member (r = worker_rank(w)). A
+// segment's records are requested before its count rows, so both loads are in flight together.
 template <class P>
 __device__ inline void team_segments(const TeamArgs& a, const TeamLds& L, const StepBuf& Bc, uint32_t me, uint32_t T, uint32_t w,
                                      BinTab& t)
@@ -1444,7 +1447,7 @@ __device__ inline void team_segments(const TeamArgs& a, const TeamLds& L, const 
     Place pl;
     pl.k = ~0u;
     SegRecs<P> R;
-    for (uint32_t s = lo + w; s < hi; s += kWorkers) {
+    for (uint32_t s = lo + worker_rank(w); s < hi; s += kWorkers) {
         const SegLoc loc = locate(L, Bc, a.e.n_nodes, s);
         load_segment<P>(a, Bc, loc, s, R);
         const uint32_t k = loc.k, kind = loc.kind;
