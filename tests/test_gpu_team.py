@@ -119,3 +119,37 @@ def test_invalid_bin_is_a_batch_error_and_nothing_is_published():
         flow.run(src, 64)
     torch.cuda.synchronize()
     assert (rec.cpu().numpy() == -1).all()
+
+
+@pytest.mark.parametrize("fuse", [False, True])
+def test_two_partitions_and_later_transforms_in_one_step(fuse):
+    """Two partitions and (unfused) two transforms after them in node order are active in the same
+    steps, so a scatter wave's output range includes the outputs of the partitions before its node,
+    which it derives from their count rows. Trace and final weights equal the oracle's."""
+    torch = torch_cuda()
+    N = 20000
+    rec = torch.full((N,), -1, dtype=torch.int32, device="cuda")
+    flow = g.Flow(g.PAYLOAD_PARTICLE)
+    src = flow.source("generate", g.UniformParticles())
+    p1 = flow.partition("route", g.WeightRoute(1e-3, "parity", 2))
+    p2 = flow.partition("split", g.IdMod(2))
+    a = flow.transform("A", g.ScaleWeight(0.5))
+    b = flow.transform("B", g.ScaleWeight(0.75))
+    sink = flow.sink("sink", g.RecordWeight(rec.data_ptr(), N))
+    flow.connect(src, p1)
+    flow.connect(p1.out(0), sink)
+    flow.connect(p1.out(1), a)
+    flow.connect(p1.out(2), p2)
+    flow.connect(p2.out(0), a)
+    flow.connect(p2.out(1), b)
+    flow.connect(a, p1)
+    flow.connect(b, p1)
+    log(f"[team] two partitions fuse={fuse}")
+    stats, trace = run_flow(flow, src, N, C=256, P=2048, Q=4, schedule=g.SCHEDULE_DEVICE, fuse=fuse, engine=3)
+    ref = run_oracle(flow, N, C=256, P=2048, Q=4, schedule=g.SCHEDULE_DEVICE, fuse=fuse, record=True)
+    assert ref["status"] == O.OK, ref["message"]
+    assert stats["engine"] == 3
+    assert_same_trace(trace, ref["trace"])
+    torch.cuda.synchronize()
+    assert (rec.cpu().numpy().view(np.uint32) == ref["record"]).all()
+    assert stats["sunk"] == N
