@@ -265,14 +265,21 @@ __device__ inline void plan_partition_counts(const Engine& e, uint32_t k, PlanLd
             L.words_used += 2u * B;
         }
     } else {
+        // thread per bin (consecutive bins: coalesced), 16 tiles loaded before any is rewritten
         tot = a.totals;
         base = a.totals + B;
+        uint32_t* cn = a.counts;
         for (uint32_t bin = tid; bin < B; bin += bsz()) {
             uint32_t acc = 0;
-            for (uint32_t t = 0; t < nt; ++t) {
-                uint32_t x = a.counts[(size_t)t * B + bin];
-                a.counts[(size_t)t * B + bin] = acc;
-                acc += x;
+            for (uint32_t t0 = 0; t0 < nt; t0 += 16u) {
+                uint32_t x[16];
+#pragma unroll
+                for (uint32_t j = 0; j < 16u; ++j) x[j] = t0 + j < nt ? cn[(size_t)(t0 + j) * B + bin] : 0u;
+#pragma unroll
+                for (uint32_t j = 0; j < 16u; ++j) {
+                    if (t0 + j < nt) cn[(size_t)(t0 + j) * B + bin] = acc;
+                    acc += x[j];
+                }
             }
             tot[bin] = acc;
         }
@@ -300,7 +307,18 @@ __device__ inline void plan_partition_counts(const Engine& e, uint32_t k, PlanLd
         for (uint32_t i = tid; i < need; i += bsz()) a.counts[i] = m[i] + base[i % B] * C;
         for (uint32_t bin = tid; bin < B; bin += bsz()) a.totals[bin] = tot[bin];
     } else {
-        for (size_t i = tid; i < (size_t)need; i += bsz()) a.counts[i] += base[i % B] * C;
+        uint32_t* cn = a.counts;
+        for (uint32_t bin = tid; bin < B; bin += bsz()) {
+            const uint32_t add = base[bin] * C;
+            for (uint32_t t0 = 0; t0 < nt; t0 += 16u) {
+                uint32_t x[16];
+#pragma unroll
+                for (uint32_t j = 0; j < 16u; ++j) x[j] = t0 + j < nt ? cn[(size_t)(t0 + j) * B + bin] : 0u;
+#pragma unroll
+                for (uint32_t j = 0; j < 16u; ++j)
+                    if (t0 + j < nt) cn[(size_t)(t0 + j) * B + bin] = x[j] + add;
+            }
+        }
     }
     __syncthreads();
 }
@@ -359,40 +377,47 @@ __device__ inline void engine_plan(const Engine& e, PlanLds& L)
     }
     __syncthreads();
     if (L.fail) return;
-    // pass 2: every pop of the step in one parallel round trip (node order, bin-major, k order)
+    // pass 2: every pop of the step (node order, bin-major, k order): output j takes
+    // stack[top - 1 - j]; a partition's outputs are written per bin (thread per bin), the others
+    // per output
     const uint32_t top = L.top;
-    for (uint32_t j = tid; j < L.total_R; j += bsz()) {
-        uint32_t k = 0;
-        while (k + 1u < n && L.off[k + 1u] <= j) ++k;
-        const uint32_t r = j - L.off[k];
-        unsigned long long ent = e.free_stack[top - 1u - j];
-        gf_chunk_desc d;
-        d.index = (uint32_t)ent;
-        d.generation = (uint32_t)(ent >> 32);
-        const uint32_t kind = e.arr[k].kind;
-        if (kind == KIND_SOURCE) {
-            d.fill = L.st[k].particles;
-            d.bin = 0u;
-        } else if (kind == KIND_TRANSFORM) {
-            uint32_t rem = L.st[k].particles - r * C;
-            d.fill = rem < C ? rem : C;
-            d.bin = 0u;
-        } else {
+    for (uint32_t k = 0; k < n; ++k) {
+        if (!L.st[k].active || !L.R[k]) continue;
+        const uint32_t kind = e.arr[k].kind, o = L.off[k];
+        gf_chunk_desc* out = e.arr[k].out;
+        if (kind == KIND_PARTITION) {
             const uint32_t B = e.nodes[k].bins;
             const uint32_t* tot = L.tb_off[k] != ~0u ? L.words + L.tb_off[k] : e.arr[k].totals;
             const uint32_t* base = tot + B;
-            uint32_t lo = 0, hi = B;  // largest bin with base[bin] <= r and a non-empty range
-            while (hi - lo > 1u) {
-                uint32_t mid = (lo + hi) >> 1;
-                if (base[mid] <= r) lo = mid;
-                else hi = mid;
+            for (uint32_t bin = tid; bin < B; bin += bsz()) {
+                const uint32_t tb = tot[bin], b0 = base[bin];
+                for (uint32_t q = 0, rem = tb; rem; ++q) {
+                    const unsigned long long ent = e.free_stack[top - 1u - (o + b0 + q)];
+                    gf_chunk_desc d;
+                    d.index = (uint32_t)ent;
+                    d.generation = (uint32_t)(ent >> 32);
+                    d.fill = rem < C ? rem : C;
+                    d.bin = bin;
+                    out[b0 + q] = d;
+                    rem -= d.fill;
+                }
             }
-            while (lo + 1u < B && base[lo + 1u] <= r) ++lo;
-            uint32_t rem = tot[lo] - (r - base[lo]) * C;
-            d.fill = rem < C ? rem : C;
-            d.bin = lo;
+        } else {
+            for (uint32_t r = tid; r < L.R[k]; r += bsz()) {
+                const unsigned long long ent = e.free_stack[top - 1u - (o + r)];
+                gf_chunk_desc d;
+                d.index = (uint32_t)ent;
+                d.generation = (uint32_t)(ent >> 32);
+                if (kind == KIND_SOURCE) {
+                    d.fill = L.st[k].particles;
+                } else {
+                    const uint32_t rem = L.st[k].particles - r * C;
+                    d.fill = rem < C ? rem : C;
+                }
+                d.bin = 0u;
+                out[r] = d;
+            }
         }
-        e.arr[k].out[r] = d;
     }
     for (uint32_t k = tid; k < n; k += bsz()) {
         L.st[k].n_out = L.R[k];
