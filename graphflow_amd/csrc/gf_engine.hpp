@@ -1,17 +1,18 @@
 // Device-resident step engine of the batched form.
 //
-// One step = two kernels:
-//   KA: classify+count every partition tile of the step; the last workgroup to finish (atomic
-//       ticket, no spinning) plans every node in node order: per-tile flat offsets, exact output
-//       chunk counts, pops from the device free stack, output descriptors, trace.
-//   KB: scatter / transform / source / sink tiles of every active node; the last workgroup
-//       retires the step (routes outputs to the destination ready rings, releases inputs to the
-//       free stack) and selects the next step's batches (source admission against P, largest
-//       credit-fitting prefix of <= Q ready chunks per transform node, all ready chunks per sink).
+// One step = four kernels, no host work in between:
+//   KA: classify+count every partition tile of the step;
+//   KP: one workgroup plans every node in node order: per-tile flat offsets, exact output chunk
+//       counts, pops from the device free stack, output descriptors, trace;
+//   KB: scatter / transform / source / sink tiles of every active node;
+//   KR: one workgroup retires the step (routes outputs to the destination ready rings, releases
+//       inputs to the free stack) and selects the next step's batches (source admission against
+//       P, largest credit-fitting prefix of <= Q ready chunks per transform node, all ready chunks
+//       per sink).
 // The scheduling rules are those of GraphRunner::run (sched/GraphRunner.hpp:902-1133) applied
 // once per node per step; the host only replays a hipGraph of K steps and polls a done flag.
 //
-// The last-workgroup phases are latency bound: they keep the whole control state in LDS and
+// The one-workgroup phases are latency bound: they keep the whole control state in LDS and
 // touch global memory in a constant number of parallel round trips per step.
 #pragma once
 

@@ -177,7 +177,7 @@ __global__ __launch_bounds__(kThreads) void k_engine_ka(Engine e)
 }
 
 // One workgroup: the plan of every node of the step (reads KA's counts across the kernel boundary).
-__global__ __launch_bounds__(kThreads) void k_engine_plan(Engine e)
+__global__ __launch_bounds__(1024) void k_engine_plan(Engine e)
 {
     extern __shared__ char smem[];
     __shared__ uint32_t flag;
@@ -213,12 +213,18 @@ __global__ __launch_bounds__(kThreads) void k_engine_kb(Engine e, uint32_t stage
             transform_tile<P, R>(nd, b, lt, e.cols, e.C, e.rp);
         }
     }
-    if (last_block(&e.ctl[CTL_TICKET_B], &flag)) {
-        SelLds& L = *reinterpret_cast<SelLds*>(smem);
-        engine_retire(e, L);
-        engine_select(e, L);
-        if (threadIdx.x == 0) e.ctl[CTL_TICKET_B] = 0u;
-    }
+}
+
+// Retire + select of the step (one workgroup of 1024 threads, after every tile of KB): outputs to
+// the destination rings, inputs back on the free stack, the next step's batches.
+__global__ __launch_bounds__(1024) void k_engine_kr(Engine e)
+{
+    extern __shared__ char smem[];
+    __shared__ uint32_t flag;
+    if (engine_stopped(e, &flag)) return;
+    SelLds& L = *reinterpret_cast<SelLds*>(smem);
+    engine_retire(e, L);
+    engine_select(e, L);
 }
 
 // Micro engine: one persistent workgroup of G tile groups (256 threads each) runs whole steps with
@@ -1306,7 +1312,8 @@ void build_engine(gf_flow& f, uint64_t N, const RunShape& sh, uint32_t src)
     std::vector<int32_t> routes, fuse;
     const uint32_t r = effective_r(f.payload, pick_tile_r(sh.tile_items));
     const uint32_t rec = payload_bytes(f.payload);
-    size_t lds_a = sizeof(PlanLds), lds_b = sizeof(SelLds), group_lds = 256;
+    // KA and KB hold only their tiles' LDS (the plan, retire and select run in KP and KR)
+    size_t lds_a = 256, lds_b = 256, group_lds = 256;
     uint32_t stage_mask = 0, grid_tiles = 0;
     std::vector<NodeArr> arr(n);
     auto is_plain_xf = [&](int32_t k) {
@@ -1453,12 +1460,14 @@ void build_engine(gf_flow& f, uint64_t N, const RunShape& sh, uint32_t src)
         if (micro_fits) set_lds(k_micro<P, RR>, eb->micro_lds);
     });
     set_lds(k_engine_plan, sizeof(PlanLds));
+    set_lds(k_engine_kr, sizeof(SelLds));
     (void)src;
     f.eng = std::move(eb);
 }
 
 // One step: KA (classify/count tiles), KP (plan, one workgroup), KB (scatter/transform/source/sink
-// tiles, retire + select in the last workgroup). With `ev`, events are recorded around each kernel.
+// tiles), KR (retire + select, one workgroup). With `ev`, events are recorded around KA, KP and
+// KB + KR.
 void launch_step(const EngineBuffers& eb, uint32_t payload, hipStream_t s, hipEvent_t* ev = nullptr)
 {
     with_payload(payload, eb.r, [&](auto pt, auto rc) {
@@ -1467,9 +1476,10 @@ void launch_step(const EngineBuffers& eb, uint32_t payload, hipStream_t s, hipEv
         if (ev) HIPX(hipEventRecord(ev[0], s));
         hipLaunchKernelGGL((k_engine_ka<P, RR>), dim3(eb.grid), dim3(kThreads), eb.lds_a, s, eb.e);
         if (ev) HIPX(hipEventRecord(ev[1], s));
-        hipLaunchKernelGGL(k_engine_plan, dim3(1), dim3(kThreads), sizeof(PlanLds), s, eb.e);
+        hipLaunchKernelGGL(k_engine_plan, dim3(1), dim3(1024), sizeof(PlanLds), s, eb.e);
         if (ev) HIPX(hipEventRecord(ev[2], s));
         hipLaunchKernelGGL((k_engine_kb<P, RR>), dim3(eb.grid), dim3(kThreads), eb.lds_b, s, eb.e, eb.stage_mask);
+        hipLaunchKernelGGL(k_engine_kr, dim3(1), dim3(1024), sizeof(SelLds), s, eb.e);
         if (ev) HIPX(hipEventRecord(ev[3], s));
     });
 }
@@ -1651,7 +1661,7 @@ void run_wide(gf_flow& f, EngineBuffers& eb, uint64_t N, gf_stats& st, bool time
     for (;;) {
         if (use_graph) {
             HIPX(hipGraphLaunch(eb.exec, s));
-            launches += 3ull * K;
+            launches += 4ull * K;
         } else {
             for (uint32_t i = 0; i < K; ++i) {
                 if (timed) {
@@ -1666,7 +1676,7 @@ void run_wide(gf_flow& f, EngineBuffers& eb, uint64_t N, gf_stats& st, bool time
                 } else {
                     launch_step(eb, f.payload, s);
                 }
-                launches += 3;
+                launches += 4;
             }
         }
         steps_launched += K;
