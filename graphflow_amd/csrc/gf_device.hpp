@@ -443,6 +443,9 @@ __device__ inline uint64_t peer_mask(bool valid, uint32_t bin)
 // Chunk of tile t: largest c with prefix[c] <= t (prefix has n+1 entries).
 __device__ inline uint32_t find_chunk(const uint32_t* prefix, uint32_t n, uint32_t t)
 {
+    // common case, two independent loads: every chunk before t holds exactly one tile (chunks of at
+    // most tile_items items, e.g. the many small per-bucket chunks a sink takes)
+    if (t < n && prefix[t] == t && prefix[t + 1u] > t) return t;
     uint32_t lo = 0, hi = n;  // invariant: prefix[lo] <= t < prefix[hi]
     while (hi - lo > 1u) {
         uint32_t mid = (lo + hi) >> 1;
