@@ -153,3 +153,12 @@ def test_two_partitions_and_later_transforms_in_one_step(fuse):
     torch.cuda.synchronize()
     assert (rec.cpu().numpy().view(np.uint32) == ref["record"]).all()
     assert stats["sunk"] == N
+
+
+@pytest.mark.parametrize("fuse", [False, True])
+def test_batches_of_more_than_64_inputs(fuse):
+    """C=1, Q=200: partition batches of up to 200 one-item chunks (more than one wave of inputs and
+    more than 128 segments per batch), still within the team's per-step limits; the trace and the
+    final weights equal the oracle's."""
+    st = _feedback_parity(3000, 1, 600, 200, fuse)
+    assert st["engine"] == 3
