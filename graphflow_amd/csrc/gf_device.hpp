@@ -719,4 +719,58 @@ __device__ void sink_tile(const gf_node_desc& nd, const Batch& b, uint32_t t, co
     }
 }
 
+// Sink of one tile by one wave (same results as sink_tile): for sinks that take many small chunks
+// per step, the waves of a workgroup each run a tile, so their latency chains overlap.
+template <class P>
+__device__ void sink_tile_wave(const gf_node_desc& nd, const Batch& b, uint32_t t, const Cols& cols, uint32_t C,
+                               uint32_t* status)
+{
+    const uint32_t lane = lane_id();
+    TileSpan s = tile_span(b, b.in, b.n_in, t, C);
+    if constexpr (std::is_same_v<P, Particle>) {
+        for (uint32_t i = lane; i < s.count; i += 64u) {
+            Particle p = Particle::load(cols, s.base + i);
+            if (nd.op == GF_SNK_HIST) {
+                uint32_t v = nd.u[0] == 0u ? __float_as_uint(p.w) : p.gen;
+                uint32_t rel = v - nd.u[1];
+                if (v < nd.u[1] || rel >= nd.u[2]) set_status(status, GF_INVALID);
+                else atomicAdd((unsigned long long*)nd.ptr[0] + rel, 1ull);
+            } else if (nd.op == GF_SNK_RECORD_WEIGHT) {
+                if ((uint64_t)p.id < nd.size[0]) ((uint32_t*)nd.ptr[0])[p.id] = __float_as_uint(p.w);
+                else set_status(status, GF_INVALID);
+            }
+        }
+    } else if constexpr (std::is_same_v<P, Keyed>) {
+        if (nd.op != GF_SNK_CHUNK_DIGEST) return;
+        const gf_chunk_desc d = s.count ? b.in[s.chunk] : gf_chunk_desc{0, 0, 0, 0};
+        uint64_t acc = 0, msum = 0;
+        for (uint32_t i = lane; i < s.count; i += 64u) {
+            Keyed p = Keyed::load(cols, s.base + i);
+            uint64_t h = keyed_hash(p);
+            acc += mix64(h ^ ((uint64_t)(s.offset + i + 1u) * 0xD6E8FEB86659FD93ull));
+            msum += h;
+        }
+        for (int o = 32; o > 0; o >>= 1) {
+            acc += __shfl_down(acc, o);
+            msum += __shfl_down(msum, o);
+        }
+        if (lane == 0 && s.count) {
+            uint64_t entry = b.digest_base + s.chunk;
+            if (nd.ptr[0] && entry < nd.size[0]) {
+                uint32_t* log = (uint32_t*)nd.ptr[0] + 4 * entry;
+                if (s.offset == 0) {
+                    log[0] = d.bin;
+                    log[1] = d.fill;
+                }
+                atomicAdd((unsigned long long*)(log + 2), (unsigned long long)acc);
+            }
+            if (nd.ptr[1] && d.bin < nd.size[1]) {
+                unsigned long long* ms = (unsigned long long*)nd.ptr[1] + 2 * d.bin;
+                atomicAdd(ms, (unsigned long long)msum);
+                atomicAdd(ms + 1, (unsigned long long)s.count);
+            }
+        }
+    }
+}
+
 }  // namespace gfd

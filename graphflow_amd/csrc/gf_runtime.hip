@@ -185,15 +185,23 @@ __global__ __launch_bounds__(1024) void k_engine_plan(Engine e)
     engine_plan(e, *reinterpret_cast<PlanLds*>(smem));
 }
 
+// Tiles round-robin over the workgroups; the tiles of the sinks in `wave_sinks` (sinks fed by wide
+// partitions, whose chunks are small) round-robin over the waves instead.
 template <class P, int R>
-__global__ __launch_bounds__(kThreads) void k_engine_kb(Engine e, uint32_t stage_mask)
+__global__ __launch_bounds__(kThreads) void k_engine_kb(Engine e, uint32_t stage_mask, uint32_t wave_sinks)
 {
     extern __shared__ char smem[];
     __shared__ uint32_t flag;
     if (engine_stopped(e, &flag)) return;
-    const uint32_t total = e.kb_prefix[e.n_nodes];
-    for (uint32_t t = blockIdx.x; t < total; t += gridDim.x) {
+    const uint32_t total = e.kb_prefix[e.n_nodes], G = gridDim.x;
+    for (uint32_t t = blockIdx.x; t < total; t += G) {
         const uint32_t k = find_node(e.kb_prefix, e.n_nodes, t);
+        if (k < 32u && ((wave_sinks >> k) & 1u)) {
+            // skip the node's tiles: the first tile of this workgroup's round-robin at or after them
+            const uint32_t end = e.kb_prefix[k + 1u];
+            t += (end - t + G - 1u) / G * G - G;
+            continue;
+        }
         const gf_node_desc& nd = e.nodes[k];
         const NodeSt s = e.st[k];
         const Batch b = make_batch(e, k, s);
@@ -212,6 +220,16 @@ __global__ __launch_bounds__(kThreads) void k_engine_kb(Engine e, uint32_t stage
         } else {
             transform_tile<P, R>(nd, b, lt, e.cols, e.C, e.rp);
         }
+    }
+    const uint32_t w = threadIdx.x >> 6, W = G * (uint32_t)kWaves;
+    for (uint32_t ws = wave_sinks; ws; ws &= ws - 1u) {
+        const uint32_t k = (uint32_t)(__ffs(ws) - 1);
+        if (k >= e.n_nodes || !e.st[k].active) continue;
+        const uint32_t t0 = e.kb_prefix[k], t1 = e.kb_prefix[k + 1u];
+        const gf_node_desc& nd = e.nodes[k];
+        const Batch b = make_batch(e, k, e.st[k]);
+        for (uint32_t t = t0 + blockIdx.x * (uint32_t)kWaves + w; t < t1; t += W)
+            sink_tile_wave<P>(nd, b, t - t0, e.cols, e.C, &e.ctl[CTL_STATUS]);
     }
 }
 
@@ -900,7 +918,7 @@ struct EngineBuffers {
     DevMem mem;
     std::string key;
     Engine e{};
-    uint32_t stage_mask = 0, r = 4, grid = 256;
+    uint32_t stage_mask = 0, wave_sinks = 0, r = 4, grid = 256;
     size_t lds_a = 0, lds_b = 0;
     bool micro = false;
     uint32_t group_lds = 0;
@@ -1399,6 +1417,16 @@ void build_engine(gf_flow& f, uint64_t N, const RunShape& sh, uint32_t src)
     eb->grid = std::min<uint32_t>(std::max<uint32_t>(grid_tiles, 1u), gmax);
     eb->r = r;
     eb->stage_mask = stage_mask;
+    // sinks fed by a partition of 64 or more bins take many small chunks per step: wave tiles
+    uint32_t wave_sinks = 0;
+    for (uint32_t k = 0; k < n; ++k) {
+        if (arr[k].kind != KIND_PARTITION || nds[k].bins < 64u) continue;
+        for (uint32_t b = 0; b < nds[k].bins; ++b) {
+            const int32_t dst = routes[nds[k].route_offset + b];
+            if (dst >= 0 && dst < 32 && arr[dst].kind == KIND_SINK) wave_sinks |= 1u << dst;
+        }
+    }
+    eb->wave_sinks = wave_sinks;
     eb->lds_a = lds_a;
     eb->lds_b = lds_b;
     eb->key = shape_key(f, sh, N);
@@ -1478,7 +1506,8 @@ void launch_step(const EngineBuffers& eb, uint32_t payload, hipStream_t s, hipEv
         if (ev) HIPX(hipEventRecord(ev[1], s));
         hipLaunchKernelGGL(k_engine_plan, dim3(1), dim3(1024), sizeof(PlanLds), s, eb.e);
         if (ev) HIPX(hipEventRecord(ev[2], s));
-        hipLaunchKernelGGL((k_engine_kb<P, RR>), dim3(eb.grid), dim3(kThreads), eb.lds_b, s, eb.e, eb.stage_mask);
+        hipLaunchKernelGGL((k_engine_kb<P, RR>), dim3(eb.grid), dim3(kThreads), eb.lds_b, s, eb.e, eb.stage_mask,
+                           eb.wave_sinks);
         hipLaunchKernelGGL(k_engine_kr, dim3(1), dim3(1024), sizeof(SelLds), s, eb.e);
         if (ev) HIPX(hipEventRecord(ev[3], s));
     });
