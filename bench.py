@@ -159,14 +159,15 @@ def photon_forms(g, torch, N, Np, C, P, Q, verify):
     return out
 
 
-def hash_form(g, torch, N):
+def hash_form(g, torch, N, qs=(8, 16), autotune=True):
     """Keyed hash pipeline (Zipf(1.1) keys over 2^20 values, murmur3 finalizer into 4096 buckets,
-    C=8192) at Q in {8, 16}: keys/s on the device engine, one partition pass per key (24 B of
-    algorithmic traffic per key)."""
+    C=8192) at each Q of `qs`: keys/s on the device engine, one partition pass per key (24 B of
+    algorithmic traffic per key). Without autotune: the wide engine, 1024 workgroups, 1024-item
+    tiles."""
     out = {}
     bins = 4096
     ms = torch.zeros(2 * bins, dtype=torch.int64, device="cuda")
-    for Q in (8, 16):
+    for Q in qs:
         f = g.Flow(g.PAYLOAD_KEYED)
         src = f.source("keys", g.ZipfKeys(1 << 20, 1.1))
         part = f.partition("bucket", g.KeyBucket(bins))
@@ -174,8 +175,13 @@ def hash_form(g, torch, N):
         f.connect(src, part)
         for b in range(bins):
             f.connect(part, snk, b)
-        f.configure(g.Tuning(seed=42, batch_chunks=Q, chunk_capacity=8192, schedule=g.SCHEDULE_DEVICE))
-        tuned = f.autotune(src, probe_items=min(N, 1_000_000))
+        if autotune:
+            f.configure(g.Tuning(seed=42, batch_chunks=Q, chunk_capacity=8192, schedule=g.SCHEDULE_DEVICE))
+            tuned = f.autotune(src, probe_items=min(N, 1_000_000))
+        else:
+            tuned = {"device_engine": 1, "max_workgroups": 1024, "batch_chunks": Q, "tile_items": 1024}
+            f.configure(g.Tuning(seed=42, batch_chunks=Q, chunk_capacity=8192, schedule=g.SCHEDULE_DEVICE,
+                                 device_engine=1, max_workgroups=1024, tile_items=1024))
         dt, st = run_once(f, src, ms, N, torch)
         out[f"hash_pipeline_B4096_Q{Q}"] = {"value": N / dt, "unit": "keys/s", "items": N, "engine_steps": int(st["steps"]),
                                            "engine": int(st["engine"]), "autotuned": tuned}
@@ -294,6 +300,7 @@ def main():
     ap.add_argument("--photons", type=float, default=1e7, help="photon transport form: photons (0 = skip)")
     ap.add_argument("--plain-photons", type=float, default=1e6)
     ap.add_argument("--hash-items", type=float, default=4e6, help="keyed hash pipeline form: keys (0 = skip)")
+    ap.add_argument("--hash-q", default="8,16", help="keyed hash pipeline form: batch sizes Q, comma separated")
     ap.add_argument("--serve-jobs", type=int, default=256, help="served mixed stream: jobs (0 = skip)")
     ap.add_argument("--serve-rate", type=float, default=1000.0, help="served mixed stream: Poisson arrivals per second")
     ap.add_argument("--serve-max-items", type=float, default=1e6, help="served mixed stream: N ~ log-uniform[1e3, this]")
@@ -423,7 +430,8 @@ def main():
         forms.update(photon_forms(g, torch, int(args.photons), int(args.plain_photons), C, P, Q,
                                   verify=not args.no_cpu))
     if args.hash_items > 0:
-        forms.update(hash_form(g, torch, int(args.hash_items)))
+        forms.update(hash_form(g, torch, int(args.hash_items), tuple(int(q) for q in args.hash_q.split(",")),
+                               autotune=not args.no_autotune))
     if args.serve_jobs > 0:
         sat = served_form(g, torch, args.serve_jobs, args.serve_rate, args.serve_max_items)
         forms["served_mixed_stream"] = sat
