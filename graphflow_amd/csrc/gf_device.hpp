@@ -507,8 +507,14 @@ __device__ void count_tile(const gf_node_desc& nd, const Batch& b, uint32_t t, c
                 v = false;
             }
         }
-        uint64_t peers = peer_mask(v, bin);
-        if (v && (peers & lanemask_lt()) == 0ull) atomicAdd(&tcnt[bin], (uint32_t)__popcll(peers));
+        if (B >= 64u) {
+            // wide partitions: a wave's items mostly fall into distinct bins, so one LDS atomic
+            // per item beats aggregating per distinct bin
+            if (v) atomicAdd(&tcnt[bin], 1u);
+        } else {
+            uint64_t peers = peer_mask(v, bin);
+            if (v && (peers & lanemask_lt()) == 0ull) atomicAdd(&tcnt[bin], (uint32_t)__popcll(peers));
+        }
     }
     __syncthreads();
     if (t != kNoTile) {
