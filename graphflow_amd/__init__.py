@@ -604,12 +604,12 @@ class Flow:
         _raise(lib().gf_flow_phase_profile(self.handle, a, 32))
         if int(a[14]):
             steps = int(a[14])
-            names = {9: "barrier_wait", 10: "prefetch", 11: "counts_plan", 12: "segments", 13: "arrive",
-                     15: "wave0_segments", 16: "wave4_retire", 17: "wave4_select"}
+            names = {9: "barrier_wait", 10: "step_start", 12: "step_work", 13: "after_barrier",
+                     15: "wave0_segments", 16: "scheduler_retire", 17: "scheduler_select"}
             out = {n: a[i] / 100.0 / steps for i, n in names.items()}
             cyc = {18: "sel_fills", 19: "sel_credit", 20: "sel_check", 21: "sel_lists", 22: "ret_out", 23: "ret_in",
-                   24: "plan_nodes", 25: "plan_pops", 26: "sel_inputs", 27: "cnt_scan", 28: "seg_rank",
-                   29: "seg_store", 30: "seg_agg", 31: "step"}
+                   24: "plan_nodes", 25: "plan_pops", 27: "cnt_scan", 28: "seg_rank", 29: "seg_store", 30: "seg_agg",
+                   31: "step"}
             for i, nm in cyc.items():
                 out["cycles_" + nm] = a[i] / steps
             return out, steps

@@ -79,7 +79,7 @@ enum {
     // shader-clock cycles of sub-phases (member 0, lane 0 of the wave doing the work)
     PROF_C_SEL_FILLS = 18, PROF_C_SEL_CREDIT = 19, PROF_C_SEL_CHECK = 20, PROF_C_SEL_LISTS = 21,
     PROF_C_RET_OUT = 22, PROF_C_RET_IN = 23, PROF_C_PLAN_NODES = 24, PROF_C_PLAN_POPS = 25,
-    PROF_C_CNT_CLEAR = 26, PROF_C_CNT_SCAN = 27, PROF_C_SEG_RANK = 28, PROF_C_SEG_STORE = 29, PROF_C_SEG_AGG = 30,
+    PROF_C_CNT_SCAN = 27, PROF_C_SEG_RANK = 28, PROF_C_SEG_STORE = 29, PROF_C_SEG_AGG = 30,
     PROF_C_STEP = 31
 };
 
@@ -1033,7 +1033,7 @@ __device__ inline uint32_t next_key(const TeamArgs& a, uint32_t dkind, const gf_
     return (((uint32_t)d.index * 2u + (d.generation & 1u)) * a.tpc + (off >> kSegShift)) * a.bb + nb;
 }
 
-// Records of one input segment, prefetched right after the barrier by the wave that will scatter it.
+// Records of one input segment, requested by the wave that scatters it before it reads the count rows.
 template <class P>
 struct SegRecs {
     P rec[kSegRounds];
