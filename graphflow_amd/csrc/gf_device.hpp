@@ -440,6 +440,20 @@ __device__ inline uint64_t peer_mask(bool valid, uint32_t bin)
     return peers;
 }
 
+// The same set from one ballot per bin bit (bins < 2^nbits): a lane's peers are the valid lanes
+// that agree with it on every bit. Costs nbits ballots however many bins the wave holds, so wide
+// partitions use it.
+__device__ inline uint64_t peer_mask_bits(bool valid, uint32_t bin, uint32_t nbits)
+{
+    uint64_t peers = __ballot(valid);
+    for (uint32_t j = 0; j < nbits; ++j) {
+        const bool bit = ((bin >> j) & 1u) != 0u;
+        const uint64_t m = __ballot(valid && bit);
+        peers &= bit ? m : ~m;
+    }
+    return valid ? peers : 0ull;
+}
+
 // Chunk of tile t: largest c with prefix[c] <= t (prefix has n+1 entries).
 __device__ inline uint32_t find_chunk(const uint32_t* prefix, uint32_t n, uint32_t t)
 {
@@ -567,7 +581,7 @@ __device__ void scatter_tile(const gf_node_desc& nd, const Batch& b, uint32_t t,
                 bin[r] = 0xFFFFFFFFu;
             }
         }
-        uint64_t peers = peer_mask(v, bin[r]);
+        uint64_t peers = B > 64u ? peer_mask_bits(v, bin[r], 32u - (uint32_t)__clz((int)(B - 1u))) : peer_mask(v, bin[r]);
         uint32_t cur = v ? wcnt[w * B + bin[r]] : 0u;
         rank[r] = cur + (uint32_t)__popcll(peers & lanemask_lt());
         // every lane of the wave has read `cur` before a leader stores the new running count
