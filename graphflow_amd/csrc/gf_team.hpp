@@ -687,10 +687,13 @@ __device__ inline void team_clear_rows(const TeamArgs& a, const TeamLds& L, uint
 // [g0, g1) and in `lo` the sum over [g0, split) (b < B1; slot B1 - 1 counts invalid bins). Rows are
 // bb words (a multiple of four); each lane takes two segments and loads their rows four slots at
 // a time with 128-bit agent-coherent (sc1) buffer loads, which read zero past the array's end, so
-// lanes without a segment load from there. Then one wave scan per slot.
+// lanes without a segment load from there. A row entry counts at most kSeg items, so two slots
+// share one word as 16-bit fields whose sums over the 128 rows of a pass stay below 2^15: one
+// wave reduction per slot pair for the totals and one for the prefix.
 __device__ inline void seg_sums(const TeamArgs& a, const StepBuf& Bc, uint32_t k, uint32_t B1, uint32_t g0, uint32_t split,
                                 uint32_t g1, uint32_t lofs, uint32_t& lo, uint32_t& all)
 {
+    static_assert(kSeg * 128u < 65536u, "16-bit count fields");
     typedef uint32_t u32x4 __attribute__((ext_vector_type(4)));
     constexpr int kSc1 = 16;  // cache policy: agent scope (bypasses the CU's L1)
     const __amdgpu_buffer_rsrc_t rs = __builtin_amdgcn_make_buffer_rsrc((void*)a.cnt, (short)0, (int)a.cnt_bytes, 0x00020000);
@@ -700,50 +703,61 @@ __device__ inline void seg_sums(const TeamArgs& a, const StepBuf& Bc, uint32_t k
     const gf_chunk_desc dl = lane < mi ? Bc.in[io + lane] : gf_chunk_desc{0, 0, 0, 0};
     const uint32_t fl = lane < mi ? Bc.in_seg[io + lane] : 0xFFFFFFFFu;
     const uint32_t rowl = ((dl.index * 2u + (dl.generation & 1u)) * a.tpc) * a.bb;
-    // byte offset of the row of segment g, or one past the array when g >= g1
-    auto row_of = [&](uint32_t g) -> uint32_t {
-        const uint32_t gs = so + g;
-        uint32_t f = rl(fl, 0), r = rl(rowl, 0);
+    lo = 0u;
+    all = 0u;
+    for (uint32_t h = g0; h < g1; h += 128u) {
+        // the input holding segment h + lane (a) and h + 64 + lane (b): the last one starting at or before it
+        const uint32_t ga = so + h + lane, gb = ga + 64u;
+        uint32_t fa = rl(fl, 0), ra = rl(rowl, 0), fb = fa, rb = ra;
         if (mi <= 64u) {
             for (uint32_t y = 1; y < mi; ++y) {
                 const uint32_t fy = rl(fl, y), ry = rl(rowl, y);
-                const bool c = fy <= gs;
-                f = c ? fy : f;
-                r = c ? ry : r;
+                const bool ca = fy <= ga, cb = fy <= gb;
+                fa = ca ? fy : fa;
+                ra = ca ? ry : ra;
+                fb = cb ? fy : fb;
+                rb = cb ? ry : rb;
             }
         } else {
             for (uint32_t y = 1; y < mi; ++y) {
                 const uint32_t fy = Bc.in_seg[io + y];
-                if (fy <= gs) {
+                if (fy <= gb) {
                     const gf_chunk_desc d = Bc.in[io + y];
-                    f = fy;
-                    r = ((d.index * 2u + (d.generation & 1u)) * a.tpc) * a.bb;
+                    const uint32_t ry = ((d.index * 2u + (d.generation & 1u)) * a.tpc) * a.bb;
+                    if (fy <= ga) {
+                        fa = fy;
+                        ra = ry;
+                    }
+                    fb = fy;
+                    rb = ry;
                 }
             }
         }
-        return g < g1 ? (r + (gs - f) * a.bb) * 4u : a.cnt_bytes;
-    };
-    lo = 0u;
-    all = 0u;
-    for (uint32_t h = g0; h < g1; h += 128u) {
-        const uint32_t ra = row_of(h + lane), rb = row_of(h + 64u + lane);
+        // byte offsets of the two rows, one past the array for segments >= g1
+        const uint32_t oa = h + lane < g1 ? (ra + (ga - fa) * a.bb) * 4u : a.cnt_bytes;
+        const uint32_t ob = h + 64u + lane < g1 ? (rb + (gb - fb) * a.bb) * 4u : a.cnt_bytes;
+        // rows before `split` in this pass: none, all, or those of the lanes below it
+        const bool none = split <= h, whole = split >= h + 128u;
+        const bool pa = h + lane < split, pb = h + 64u + lane < split;
         for (uint32_t c0 = 0; c0 < B1; c0 += 4u) {
-            const u32x4 va = __builtin_amdgcn_raw_buffer_load_b128(rs, ra + c0 * 4u, 0, kSc1);
-            const u32x4 vb = __builtin_amdgcn_raw_buffer_load_b128(rs, rb + c0 * 4u, 0, kSc1);
-#pragma unroll
-            for (uint32_t q = 0; q < 4u; ++q) {
-                if (c0 + q >= B1) break;
-                const uint32_t xa = va[q], xb = vb[q];
-                const uint32_t ia = wave_inscan(xa), ib = wave_inscan(xb);
-                const uint32_t ta = rl(ia, 63), s = ta + rl(ib, 63);
-                uint32_t l = 0u;
-                if (split >= h + 128u) l = s;
-                else if (split > h + 64u) l = ta + rl(ib, split - h - 65u);
-                else if (split == h + 64u) l = ta;
-                else if (split > h) l = rl(ia, split - h - 1u);
-                const bool mine = lane == lofs + c0 + q;
-                all += mine ? s : 0u;
-                lo += mine ? l : 0u;
+            const u32x4 va = __builtin_amdgcn_raw_buffer_load_b128(rs, oa + c0 * 4u, 0, kSc1);
+            const u32x4 vb = __builtin_amdgcn_raw_buffer_load_b128(rs, ob + c0 * 4u, 0, kSc1);
+            const uint32_t a01 = va[0] | va[1] << 16, a23 = va[2] | va[3] << 16;
+            const uint32_t b01 = vb[0] | vb[1] << 16, b23 = vb[2] | vb[3] << 16;
+            const uint32_t t01 = rl(wave_inscan(a01 + b01), 63), t23 = rl(wave_inscan(a23 + b23), 63);
+            uint32_t p01 = 0u, p23 = 0u;
+            if (whole) {
+                p01 = t01;
+                p23 = t23;
+            } else if (!none) {
+                p01 = rl(wave_inscan((pa ? a01 : 0u) + (pb ? b01 : 0u)), 63);
+                p23 = rl(wave_inscan((pa ? a23 : 0u) + (pb ? b23 : 0u)), 63);
+            }
+            // slot c0 + q goes to lane lofs + c0 + q
+            const uint32_t q = lane - lofs - c0, sh = (q & 1u) * 16u;
+            if (q < 4u && c0 + q < B1) {
+                all += ((q < 2u ? t01 : t23) >> sh) & 0xFFFFu;
+                lo += ((q < 2u ? p01 : p23) >> sh) & 0xFFFFu;
             }
         }
     }
