@@ -220,6 +220,21 @@ __device__ inline void agg_add(bool v, uint32_t key, uint32_t* arr)
 }
 
 __device__ inline uint32_t div_c(uint32_t x, uint32_t C, uint32_t shift) { return shift < 32u ? x >> shift : x / C; }
+// ceil(x / C), with a shift when C is a power of two
+__device__ inline uint32_t ceil_c(uint32_t x, uint32_t C, uint32_t shift)
+{
+    const uint32_t q = div_c(x, C, shift);
+    return q + (x - q * C != 0u ? 1u : 0u);
+}
+
+// Output-chunk bound of a batch of n items into B bins: min(ceil(n / C) + min(B, n) - 1, n), in
+// 32-bit arithmetic
+__device__ inline uint32_t team_bound(uint32_t n, uint32_t C, uint32_t shift, uint32_t B)
+{
+    if (n == 0u) return 0u;
+    const uint32_t per = ceil_c(n, C, shift), extra = (B < n ? B : n) - 1u;
+    return per > n - extra ? n : per + extra;
+}
 
 // ---- team formation --------------------------------------------------------------------------
 // The team lives on one XCC: the requested one (a.xcd = XCC + 1), or the XCC of the first
@@ -470,7 +485,7 @@ __device__ inline void team_select(const TeamArgs& a, TeamLds& L, StepBuf& Bn, b
             nseg += (f + kSeg - 1u) >> kSegShift;
         }
     }
-    uint32_t bound = xf ? (uint32_t)bound_of(parts, C, B) : 0u;
+    uint32_t bound = xf ? team_bound(parts, C, a.c_shift, B) : 0u;
     tk(PROF_C_SEL_FILLS);
     // credits in node order, over the nodes that may take a batch
     const uint32_t top = L.top;
@@ -508,7 +523,7 @@ __device__ inline void team_select(const TeamArgs& a, TeamLds& L, StepBuf& Bn, b
                 // largest prefix of the candidates whose bound fits the remaining credits
                 uint32_t m = want, p = parts, sg = nseg, b = 0;
                 while (m) {
-                    b = (uint32_t)bound_of(p, C, B);
+                    b = team_bound(p, C, a.c_shift, B);
                     if (b <= credits) break;
                     --m;
                     const uint32_t f = rg[(uint32_t)((head + m) & rcm)].fill;
@@ -537,10 +552,16 @@ __device__ inline void team_select(const TeamArgs& a, TeamLds& L, StepBuf& Bn, b
         nseg = 0u;
     }
     // outputs of the sources and transforms (the partitions' follow from the counts)
-    const uint32_t rnp = !act ? 0u : kind == KIND_SOURCE ? 1u : kind == KIND_TRANSFORM ? (parts + C - 1u) / C : 0u;
+    const uint32_t rnp = !act ? 0u : kind == KIND_SOURCE ? 1u : kind == KIND_TRANSFORM ? ceil_c(parts, C, a.c_shift) : 0u;
     // segments are numbered in node order, the sinks' after everybody else's
     const uint32_t nsm = kind == KIND_SINK ? 0u : nseg, nss = kind == KIND_SINK ? nseg : 0u;
-    const uint32_t i_inc = wave_inscan(nin), sm_inc = wave_inscan(nsm), ss_inc = wave_inscan(nss), o_inc = wave_inscan(rnp);
+    // two scans of 16-bit field pairs: inputs and outputs (each below kTeamRing + kTeamNodes over
+    // all nodes), and the segment counts clamped to kTeamSegs + 1 (a clamped count fails the fit
+    // check below either way, and 32 of them stay below 2^16)
+    static_assert(kTeamRing + kTeamNodes < 65536u && kTeamNodes * (kTeamSegs + 1u) < 65536u, "16-bit scan fields");
+    const uint32_t ksm = nsm <= kTeamSegs ? nsm : kTeamSegs + 1u, kss = nss <= kTeamSegs ? nss : kTeamSegs + 1u;
+    const uint32_t io_inc = wave_inscan(nin | rnp << 16), s_inc = wave_inscan(ksm | kss << 16);
+    const uint32_t i_inc = io_inc & 0xFFFFu, o_inc = io_inc >> 16, sm_inc = s_inc & 0xFFFFu, ss_inc = s_inc >> 16;
     const uint32_t g_main = rl(sm_inc, 63);
     const uint32_t in_tot = rl(i_inc, 63), seg_tot = g_main + rl(ss_inc, 63);
     const uint32_t reserved = top - credits;
@@ -689,9 +710,10 @@ __device__ inline void team_clear_rows(const TeamArgs& a, const TeamLds& L, uint
 // a time with 128-bit agent-coherent (sc1) buffer loads, which read zero past the array's end, so
 // lanes without a segment load from there. A row entry counts at most kSeg items, so two slots
 // share one word as 16-bit fields whose sums over the 128 rows of a pass stay below 2^15: one
-// wave reduction per slot pair for the totals and one for the prefix.
+// wave reduction per slot pair for the totals and one for the prefix. `tsum` (wave-uniform)
+// receives the sum of `all` over the slots.
 __device__ inline void seg_sums(const TeamArgs& a, const StepBuf& Bc, uint32_t k, uint32_t B1, uint32_t g0, uint32_t split,
-                                uint32_t g1, uint32_t lofs, uint32_t& lo, uint32_t& all)
+                                uint32_t g1, uint32_t lofs, uint32_t& lo, uint32_t& all, uint32_t& tsum)
 {
     static_assert(kSeg * 128u < 65536u, "16-bit count fields");
     typedef uint32_t u32x4 __attribute__((ext_vector_type(4)));
@@ -705,6 +727,7 @@ __device__ inline void seg_sums(const TeamArgs& a, const StepBuf& Bc, uint32_t k
     const uint32_t rowl = ((dl.index * 2u + (dl.generation & 1u)) * a.tpc) * a.bb;
     lo = 0u;
     all = 0u;
+    tsum = 0u;
     for (uint32_t h = g0; h < g1; h += 128u) {
         // the input holding segment h + lane (a) and h + 64 + lane (b): the last one starting at or before it
         const uint32_t ga = so + h + lane, gb = ga + 64u;
@@ -745,6 +768,8 @@ __device__ inline void seg_sums(const TeamArgs& a, const StepBuf& Bc, uint32_t k
             const uint32_t a01 = va[0] | va[1] << 16, a23 = va[2] | va[3] << 16;
             const uint32_t b01 = vb[0] | vb[1] << 16, b23 = vb[2] | vb[3] << 16;
             const uint32_t t01 = rl(wave_inscan(a01 + b01), 63), t23 = rl(wave_inscan(a23 + b23), 63);
+            const uint32_t nq = B1 - c0;  // slots of this group
+            tsum += (t01 & 0xFFFFu) + (nq > 1u ? t01 >> 16 : 0u) + (nq > 2u ? t23 & 0xFFFFu : 0u) + (nq > 3u ? t23 >> 16 : 0u);
             uint32_t p01 = 0u, p23 = 0u;
             if (whole) {
                 p01 = t01;
@@ -779,31 +804,31 @@ __device__ inline void team_plan(const TeamArgs& a, TeamLds& L, const StepBuf& B
     const uint32_t pm = Bc.part_mask, reserved = Bc.reserved, top = L.top, fbase = L.fs_base;
     const uint32_t kp = pm ? (uint32_t)(__ffs(pm) - 1) : 0u;
     const uint32_t B = pm ? rl(g.bins, kp) : 0u;
-    uint32_t t = 0u;
+    uint32_t t = 0u, tall = 0u;  // tall: items over all slots (equals the valid bins' sum unless slot B counts)
     if (pm) {
         uint32_t unused;
-        seg_sums(a, Bc, kp, B + 1u, 0u, 0u, Bc.nseg[kp], 32u, unused, t);
+        seg_sums(a, Bc, kp, B + 1u, 0u, 0u, Bc.nseg[kp], 32u, unused, t, tall);
     }
     tk(PROF_C_CNT_SCAN);
     const uint32_t bl = lane - 32u;  // bin of this lane (lanes >= 32)
-    const uint32_t tv = (lane >= 32u && bl < B) ? t : 0u, r = (tv + C - 1u) / C;
-    const uint32_t ri = wave_inscan(r), ti = wave_inscan(tv);
+    const uint32_t tv = (lane >= 32u && bl < B) ? t : 0u, r = ceil_c(tv, C, a.c_shift);
+    const uint32_t ri = wave_inscan(r);
     const uint32_t bbase = ri - r;  // first output of bin bl within the partition's outputs
     uint32_t R = 0, st = 0, inv = 0;
     if (act) {
         if (kind == KIND_TRANSFORM) {
-            R = (parts + C - 1u) / C;
+            R = ceil_c(parts, C, a.c_shift);
             if (parts == 0u || R > bound) st = GF_PLAN_MISMATCH;
         } else if (kind == KIND_SOURCE) {
             R = 1u;
         }
     }
     if (pm) {
-        const uint32_t Rk = rl(ri, 63), sum = rl(ti, 63);
+        const uint32_t Rk = rl(ri, 63);
         if (rl(t, 32u + B)) inv = 1u;
         if (lane == kp) {
             R = Rk;
-            if (sum != parts || Rk > bound) st = GF_PLAN_MISMATCH;
+            if (tall != parts || Rk > bound) st = GF_PLAN_MISMATCH;
         }
         if (lane >= 32u && bl < B) L.binbase[kp * kTeamBins + bl] = bbase;
         if (lane >= 32u && bl <= B) L.tot[kp * kTeamBins + bl] = t;
@@ -812,15 +837,15 @@ __device__ inline void team_plan(const TeamArgs& a, TeamLds& L, const StepBuf& B
     for (uint32_t pmk = pm & (pm - 1u); pmk; pmk &= pmk - 1u) {
         const uint32_t kq = (uint32_t)(__ffs(pmk) - 1);
         const uint32_t Bq = rl(g.bins, kq);
-        uint32_t unused, tq;
-        seg_sums(a, Bc, kq, Bq + 1u, 0u, 0u, Bc.nseg[kq], 0u, unused, tq);
-        const uint32_t tqv = lane < Bq ? tq : 0u, rq = (tqv + C - 1u) / C;
-        const uint32_t rqi = wave_inscan(rq), tqi = wave_inscan(tqv);
+        uint32_t unused, tq, sq;
+        seg_sums(a, Bc, kq, Bq + 1u, 0u, 0u, Bc.nseg[kq], 0u, unused, tq, sq);
+        const uint32_t tqv = lane < Bq ? tq : 0u, rq = ceil_c(tqv, C, a.c_shift);
+        const uint32_t rqi = wave_inscan(rq);
         if (lane < Bq) L.binbase[kq * kTeamBins + lane] = rqi - rq;
         if (lane <= Bq) L.tot[kq * kTeamBins + lane] = tq;
         // wave-wide values first: a readlane inside the lane branch would let the scans move
         // under that branch's exec mask
-        const uint32_t Rq = rl(rqi, 63), sq = rl(tqi, 63);
+        const uint32_t Rq = rl(rqi, 63);
         if (rl(tq, Bq)) inv = 1u;
         if (lane == kq) {
             R = Rq;
@@ -896,7 +921,7 @@ __device__ inline void team_plan(const TeamArgs& a, TeamLds& L, const StepBuf& B
             const uint32_t Bq = rl(g.bins, kq), base = rl(off, kq), rok = rl(ro, kq);
             for (uint32_t x = 0; x < Bq; ++x) {
                 const uint32_t tx = L.tot[kq * kTeamBins + x];
-                if (tx) emit(base + L.binbase[kq * kTeamBins + x], (tx + C - 1u) / C, C, tx, x, rok + x);
+                if (tx) emit(base + L.binbase[kq * kTeamBins + x], ceil_c(tx, C, a.c_shift), C, tx, x, rok + x);
             }
         }
         if (lane == 0) {
@@ -979,11 +1004,13 @@ __device__ inline void team_retire(const TeamArgs& a, TeamLds& L, StepBuf& Bc, S
             const uint64_t m = __ballot(mine);
             const unsigned long long tl = rl64(g.tail, lb);
             if (mine) pos = tl + (uint32_t)__popcll(m & lanemask_lt());
-            const uint32_t fs = wave_sum(mine ? d.fill : 0u), ss = wave_sum(mine ? ns : 0u);
+            // fills and segment counts as one sum: the step's outputs hold at most kTeamSegs * kSeg
+            // items (18 bits) in at most kTeamSegs + kTeamOut segments
+            const uint32_t fss = wave_sum(mine ? d.fill | ns << 18 : 0u);
             if (lane == lb) {
                 g.tail += (uint32_t)__popcll(m);
-                g.rsum += fs;
-                g.rseg += ss;
+                g.rsum += fss & 0x3FFFFu;
+                g.rseg += fss >> 18;
             }
             pend &= ~m;
         }
@@ -1175,10 +1202,10 @@ struct Place {
 __device__ inline uint32_t part_outputs(const TeamArgs& a, const TeamLds& L, const StepBuf& Bc, uint32_t k)
 {
     const uint32_t B = L.nd[k].bins, C = a.e.C;
-    uint32_t unused, tot;
-    seg_sums(a, Bc, k, B + 1u, 0u, 0u, Bc.nseg[k], 0u, unused, tot);
+    uint32_t unused, tot, ts;
+    seg_sums(a, Bc, k, B + 1u, 0u, 0u, Bc.nseg[k], 0u, unused, tot, ts);
     const uint32_t tv = lane_id() < B ? tot : 0u;
-    return wave_sum((tv + C - 1u) / C);
+    return wave_sum(ceil_c(tv, C, a.c_shift));
 }
 
 // Placement of node k's segment gs (node-local numbering). A wave's segments come in increasing
@@ -1190,8 +1217,8 @@ __device__ inline void place_node(const TeamArgs& a, const TeamLds& L, const Ste
     if (pl.k == k) {
         if (kind != KIND_PARTITION || gs == pl.gs) return;
         if (gs > pl.gs) {
-            uint32_t add, unused;
-            seg_sums(a, Bc, k, L.nd[k].bins + 1u, pl.gs, gs, gs, 0u, add, unused);
+            uint32_t add, unused, us;
+            seg_sums(a, Bc, k, L.nd[k].bins + 1u, pl.gs, gs, gs, 0u, add, unused, us);
             pl.pre += add;
             pl.gs = gs;
             return;
@@ -1212,13 +1239,13 @@ __device__ inline void place_node(const TeamArgs& a, const TeamLds& L, const Ste
     if (kind == KIND_SOURCE) {
         pl.R = 1u;
     } else if (kind == KIND_TRANSFORM) {
-        pl.R = (parts + C - 1u) / C;
+        pl.R = ceil_c(parts, C, a.c_shift);
     } else {
         const uint32_t B = L.nd[k].bins;
-        uint32_t pre, tot;
-        seg_sums(a, Bc, k, B + 1u, 0u, gs, Bc.nseg[k], 0u, pre, tot);
-        const uint32_t tv = lane < B ? tot : 0u, r = (tv + C - 1u) / C, ri = wave_inscan(r);
-        const uint32_t Rk = rl(ri, 63), sum = wave_sum(tv), inv = rl(tot, B);
+        uint32_t pre, tot, sum;  // sum: items over all slots, the valid bins' sum when slot B is 0
+        seg_sums(a, Bc, k, B + 1u, 0u, gs, Bc.nseg[k], 0u, pre, tot, sum);
+        const uint32_t tv = lane < B ? tot : 0u, r = ceil_c(tv, C, a.c_shift), ri = wave_inscan(r);
+        const uint32_t Rk = rl(ri, 63), inv = rl(tot, B);
         pl.R = Rk;
         pl.ok = (inv == 0u && sum == parts && Rk <= L.bound[k]) ? 1u : 0u;
         pl.bb = lane < B ? (ri - r) * C : 0u;
@@ -1431,8 +1458,9 @@ __device__ inline void member_range(const StepBuf& Bc, uint32_t n, uint32_t me, 
 {
     (void)n;
     const uint32_t G = Bc.g_main;
-    *lo = (uint32_t)((uint64_t)me * G / T);
-    *hi = (uint32_t)((uint64_t)(me + 1u) * G / T);
+    // me < T and G <= kTeamSegs: the products fit 32 bits
+    *lo = me * G / T;
+    *hi = (me + 1u) * G / T;
 }
 
 // Sink segments of this member, taken by its sink wave (after it cleared its share of the count
@@ -1441,7 +1469,7 @@ template <class P>
 __device__ inline void team_sinks(const TeamArgs& a, const TeamLds& L, const StepBuf& Bc, uint32_t me, uint32_t T)
 {
     const uint32_t G0 = Bc.g_main, Gs = Bc.seg_off[a.e.n_nodes] - G0;
-    const uint32_t lo = G0 + (uint32_t)((uint64_t)me * Gs / T), hi = G0 + (uint32_t)((uint64_t)(me + 1u) * Gs / T);
+    const uint32_t lo = G0 + me * Gs / T, hi = G0 + (me + 1u) * Gs / T;
     SegRecs<P> R;
     for (uint32_t s = lo; s < hi; ++s) {
         const SegLoc loc = locate(L, Bc, a.e.n_nodes, s);
@@ -1567,7 +1595,7 @@ __global__ __launch_bounds__(kTeamThreads) void k_team(const TeamArgs* __restric
             member_range(Bc, e.n_nodes, me, T, &lo, &hi);
             const uint32_t Gs = Bc.seg_off[e.n_nodes] - Bc.g_main;
             atomicAdd(&mp[MP_NSEG], (unsigned long long)(hi - lo));
-            atomicAdd(&mp[MP_NSINK], (unsigned long long)((uint64_t)(me + 1u) * Gs / T - (uint64_t)me * Gs / T));
+            atomicAdd(&mp[MP_NSINK], (unsigned long long)((me + 1u) * Gs / T - me * Gs / T));
         }
         if (w < kWorkers) {
             team_segments<P>(a, L, Bc, me, T, w, tab);
