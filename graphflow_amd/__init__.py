@@ -608,7 +608,8 @@ class Flow:
                      15: "wave0_segments", 16: "scheduler_retire", 17: "scheduler_select"}
             out = {n: a[i] / 100.0 / steps for i, n in names.items()}
             cyc = {18: "sel_fills", 19: "sel_credit", 20: "sel_check", 21: "sel_lists", 22: "ret_out", 23: "ret_in",
-                   24: "plan_nodes", 25: "plan_pops", 27: "cnt_scan", 28: "seg_rank", 29: "seg_store", 30: "seg_agg",
+                   24: "plan_nodes", 25: "plan_pops", 27: "cnt_scan", 26: "seg_locate", 7: "seg_place", 28: "seg_rank",
+                   29: "seg_store", 30: "seg_agg",
                    31: "step"}
             for i, nm in cyc.items():
                 out["cycles_" + nm] = a[i] / steps

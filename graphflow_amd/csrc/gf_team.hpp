@@ -79,8 +79,8 @@ enum {
     // shader-clock cycles of sub-phases (member 0, lane 0 of the wave doing the work)
     PROF_C_SEL_FILLS = 18, PROF_C_SEL_CREDIT = 19, PROF_C_SEL_CHECK = 20, PROF_C_SEL_LISTS = 21,
     PROF_C_RET_OUT = 22, PROF_C_RET_IN = 23, PROF_C_PLAN_NODES = 24, PROF_C_PLAN_POPS = 25,
-    PROF_C_CNT_SCAN = 27, PROF_C_SEG_RANK = 28, PROF_C_SEG_STORE = 29, PROF_C_SEG_AGG = 30,
-    PROF_C_STEP = 31
+    PROF_C_SEG_LOCATE = 26, PROF_C_CNT_SCAN = 27, PROF_C_SEG_RANK = 28, PROF_C_SEG_STORE = 29, PROF_C_SEG_AGG = 30,
+    PROF_C_STEP = 31, PROF_C_SEG_PLACE = 7  // slot 7 is free in the team engine
 };
 
 // Sub-phase cycle counter: adds the shader-clock delta since the previous mark to an LDS
@@ -1489,11 +1489,14 @@ __device__ inline void team_segments(const TeamArgs& a, const TeamLds& L, const 
     Place pl;
     pl.k = ~0u;
     SegRecs<P> R;
+    Ticker tk(const_cast<unsigned long long*>(L.prof_acc), L.prof_member && threadIdx.x == 0u);
     for (uint32_t s = lo + worker_rank(w); s < hi; s += kWorkers) {
         const SegLoc loc = locate(L, Bc, a.e.n_nodes, s);
         load_segment<P>(a, Bc, loc, s, R);
+        tk(PROF_C_SEG_LOCATE);
         const uint32_t k = loc.k, kind = loc.kind;
         place_node(a, L, Bc, k, kind, s - Bc.seg_off[k], pl);
+        tk(PROF_C_SEG_PLACE);
         if (!pl.ok) continue;
         if (kind == KIND_SOURCE) team_source_seg<P>(a, L, Bc, k, loc.gg, pl);
         else if (kind == KIND_PARTITION) team_partition_seg<P>(a, L, Bc, k, loc.i, loc.gg, R, t, pl);
