@@ -46,6 +46,7 @@ constexpr uint32_t kTeamRoutes = 512;
 constexpr uint32_t kSegShift = 7;
 constexpr uint32_t kSeg = 1u << kSegShift;  // slots per segment
 constexpr uint32_t kSegRounds = kSeg / 64u;
+static_assert(kSegRounds == 2u, "segments of two 64-lane rounds");
 constexpr uint32_t kTeamRing = 2048;   // ready descriptors over all nodes
 constexpr uint32_t kTeamStack = 2048;  // free-stack window
 constexpr uint32_t kTeamMargin = 512;  // pops or pushes of one step
@@ -205,17 +206,20 @@ __device__ inline uint32_t node_of(uint32_t x, uint32_t off, uint32_t cnt, uint3
     return kx;
 }
 
-// arr[key] += number of lanes with that key, one atomic per distinct key of the wave.
-__device__ inline void agg_add(bool v, uint32_t key, uint32_t* arr)
+// arr[key] += number of a segment's items with that key, over both 64-lane rounds (key ~0u: none):
+// one atomic per distinct key of the segment, so the items of one destination segment in the two
+// rounds share it.
+__device__ inline void seg_agg_add(const uint32_t (&key)[2], uint32_t* arr)
 {
-    uint64_t act = __ballot(v);
+    uint64_t a0 = __ballot(key[0] != ~0u), a1 = __ballot(key[1] != ~0u);
     const uint32_t lane = lane_id();
-    while (act) {
-        const int leader = __ffsll((unsigned long long)act) - 1;
-        const uint32_t lk = rl(key, (uint32_t)leader);
-        const uint64_t m = __ballot(v && key == lk);
-        if (lane == (uint32_t)leader) atomicAdd(&arr[lk], (uint32_t)__popcll(m));
-        act &= ~m;
+    while (a0 | a1) {
+        const uint32_t lk = a0 ? rl(key[0], (uint32_t)(__ffsll((unsigned long long)a0) - 1))
+                               : rl(key[1], (uint32_t)(__ffsll((unsigned long long)a1) - 1));
+        const uint64_t m0 = __ballot(key[0] == lk), m1 = __ballot(key[1] == lk);
+        if (lane == 0u) atomicAdd(&arr[lk], (uint32_t)(__popcll(m0) + __popcll(m1)));
+        a0 &= ~m0;
+        a1 &= ~m1;
     }
 }
 
@@ -1329,8 +1333,7 @@ __device__ void team_partition_seg(const TeamArgs& a, const TeamLds& L, const St
         }
     }
     tk(PROF_C_SEG_STORE);
-#pragma unroll
-    for (uint32_t r = 0; r < kSegRounds; ++r) agg_add(key[r] != ~0u, key[r], a.cnt);
+    seg_agg_add(key, a.cnt);
     tk(PROF_C_SEG_AGG);
 }
 
@@ -1346,10 +1349,11 @@ __device__ void team_transform_seg(const TeamArgs& a, const TeamLds& L, const St
     const uint32_t pos0 = Bc.in_items[Bc.in_off[k] + i] + off0;
     const uint32_t dst = (uint32_t)L.route[L.route_off[k]], dkind = L.kind[dst], purpose = L.purpose[k];
     const gf_node_desc dnd = L.nd[dst];
+    uint32_t key[kSegRounds];
 #pragma unroll
     for (uint32_t r = 0; r < kSegRounds; ++r) {
         const uint32_t it = r * 64u + lane;
-        uint32_t key = ~0u;
+        key[r] = ~0u;
         if (it < cnt) {
             const uint32_t g = pos0 + it, q = div_c(g, C, a.c_shift), off = g - q * C;
             gf_chunk_desc od;
@@ -1357,13 +1361,13 @@ __device__ void team_transform_seg(const TeamArgs& a, const TeamLds& L, const St
                 P p = R.rec[r];
                 transform(nd, e.rp, purpose, p);
                 p.store(e.cols, (size_t)od.index * C + off);
-                key = next_key(a, dkind, dnd, od, off, p);
+                key[r] = next_key(a, dkind, dnd, od, off, p);
             } else {
                 seg_fail(a, L, GF_PLAN_MISMATCH);
             }
         }
-        agg_add(key != ~0u, key, a.cnt);
     }
+    seg_agg_add(key, a.cnt);
 }
 
 template <class P>
@@ -1381,18 +1385,19 @@ __device__ void team_source_seg(const TeamArgs& a, const TeamLds& L, const StepB
     const uint32_t dst = (uint32_t)L.route[L.route_off[k]], dkind = L.kind[dst], purpose = L.purpose[k];
     const gf_node_desc dnd = L.nd[dst];
     const unsigned long long id0 = Bc.first_id[k] + off0;
+    uint32_t key[kSegRounds];
 #pragma unroll
     for (uint32_t r = 0; r < kSegRounds; ++r) {
         const uint32_t it = r * 64u + lane;
-        uint32_t key = ~0u;
+        key[r] = ~0u;
         if (it < cnt) {
             P p;
             source(nd, e.rp, purpose, id0 + it, p);
             p.store(e.cols, (size_t)od.index * C + off0 + it);
-            key = next_key(a, dkind, dnd, od, off0 + it, p);
+            key[r] = next_key(a, dkind, dnd, od, off0 + it, p);
         }
-        agg_add(key != ~0u, key, a.cnt);
     }
+    seg_agg_add(key, a.cnt);
 }
 
 template <class P>
