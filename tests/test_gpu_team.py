@@ -40,9 +40,10 @@ def test_relaunch_keeps_state(steps_per_launch):
     _feedback_parity(12000, 1024, 2048, 4, True, steps_per_launch=steps_per_launch)
 
 
-@pytest.mark.parametrize("C", [1, 3, 255, 256, 257, 1000])
+@pytest.mark.parametrize("C", [1, 3, 127, 128, 129, 255, 256, 257, 1000])
 def test_segment_edges(C):
-    """Capacities below, at and around the 256-slot segment, ragged tails."""
+    """Capacities below, at and around the 128-slot segment and twice it (a chunk of one
+    segment, of one segment and a one-item tail, of two), ragged tails, non-power-of-two C."""
     N = 3000 if C >= 255 else 600
     _feedback_parity(N, C, max(2 * C, 8), 3, False)
 
