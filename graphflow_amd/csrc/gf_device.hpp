@@ -549,12 +549,15 @@ __host__ __device__ inline size_t scatter_smem(uint32_t bins, uint32_t tile_item
 // Stable scatter of one tile (Execution.hpp:1096-1187 / :1327-1502 semantics with the canonical
 // order): the records are held in registers between the ranking and the write; with staging the
 // tile is reordered in LDS so consecutive lanes write consecutive output slots of one bin run.
-// b.counts[t*B + bin] holds the flat output position of the tile's first item of `bin`; flat
-// position f lives at slot out[f / C].index * C + f % C.
+// b.counts[t*B + bin] holds the flat output position of the tile's first item of `bin`, less
+// bin_base[bin] * C when bin_base is given (the device engines keep each bin's first output chunk
+// beside the bin totals instead of adding it to every tile's entry); flat position f lives at
+// slot out[f / C].index * C + f % C.
 template <class P, int R, bool Stage>
 __device__ void scatter_tile(const gf_node_desc& nd, const Batch& b, uint32_t t, const Cols& cols, uint32_t C,
                              uint32_t* status, char* smem, const gf_node_desc* all = nullptr,
-                             const int32_t* fuse = nullptr, const RunParams* rp = nullptr)
+                             const int32_t* fuse = nullptr, const RunParams* rp = nullptr,
+                             const uint32_t* bin_base = nullptr)
 {
     const uint32_t B = nd.bins, tid = ltid(), w = tid >> 6, lane = lane_id();
     uint32_t* wcnt = (uint32_t*)smem;      // [kWaves][B] running, then exclusive wave prefix
@@ -617,7 +620,7 @@ __device__ void scatter_tile(const gf_node_desc& nd, const Batch& b, uint32_t t,
             if (fuse && fuse[k] >= 0) transform(all[fuse[k]], *rp, 2u * (uint32_t)fuse[k], rec[r]);
             uint32_t p = tstart[k] + within;
             stage[p] = rec[r];
-            flat[p] = toff[k] + within;
+            flat[p] = toff[k] + (bin_base ? bin_base[k] * C : 0u) + within;
         }
         __syncthreads();
         for (uint32_t j = tid; j < s.count; j += kThreads) {
@@ -631,7 +634,7 @@ __device__ void scatter_tile(const gf_node_desc& nd, const Batch& b, uint32_t t,
         for (int r = 0; r < R; ++r) {
             if (bin[r] == 0xFFFFFFFFu) continue;
             uint32_t k = bin[r];
-            uint32_t f = toff[k] + wcnt[w * B + k] + rank[r];
+            uint32_t f = toff[k] + (bin_base ? bin_base[k] * C : 0u) + wcnt[w * B + k] + rank[r];
             P p = rec[r];
             uint32_t u = update(nd, k, p);
             if (u) set_status(status, u);

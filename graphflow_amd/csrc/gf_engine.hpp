@@ -302,23 +302,15 @@ __device__ inline void plan_partition_counts(const Engine& e, uint32_t k, PlanLd
             L.fail = 1u;
         }
     }
-    // flat output start of every (tile, bin) and the totals for the trace
+    // every (tile, bin)'s items of the bin in earlier tiles, and beside the totals (for the trace)
+    // each bin's first output chunk: a scatter tile adds that chunk's flat start itself, so the
+    // [tile][bin] matrix is not rewritten a second time
     if (need <= kPlanWords && L.tb_off[k] != ~0u) {
         const uint32_t* m = L.words;
-        for (uint32_t i = tid; i < need; i += bsz()) a.counts[i] = m[i] + base[i % B] * C;
-        for (uint32_t bin = tid; bin < B; bin += bsz()) a.totals[bin] = tot[bin];
-    } else {
-        uint32_t* cn = a.counts;
+        for (uint32_t i = tid; i < need; i += bsz()) a.counts[i] = m[i];
         for (uint32_t bin = tid; bin < B; bin += bsz()) {
-            const uint32_t add = base[bin] * C;
-            for (uint32_t t0 = 0; t0 < nt; t0 += 16u) {
-                uint32_t x[16];
-#pragma unroll
-                for (uint32_t j = 0; j < 16u; ++j) x[j] = t0 + j < nt ? cn[(size_t)(t0 + j) * B + bin] : 0u;
-#pragma unroll
-                for (uint32_t j = 0; j < 16u; ++j)
-                    if (t0 + j < nt) cn[(size_t)(t0 + j) * B + bin] = x[j] + add;
-            }
+            a.totals[bin] = tot[bin];
+            a.totals[B + bin] = base[bin];
         }
     }
     __syncthreads();

@@ -214,9 +214,9 @@ __global__ __launch_bounds__(kThreads) void k_engine_kb(Engine e, uint32_t stage
         } else if (kind == KIND_PARTITION) {
             const int32_t* fz = e.fuse + nd.route_offset;
             if ((stage_mask >> (k & 31u)) & 1u)
-                scatter_tile<P, R, true>(nd, b, lt, e.cols, e.C, &e.ctl[CTL_STATUS], smem, e.nodes, fz, &e.rp);
+                scatter_tile<P, R, true>(nd, b, lt, e.cols, e.C, &e.ctl[CTL_STATUS], smem, e.nodes, fz, &e.rp, b.totals + nd.bins);
             else
-                scatter_tile<P, R, false>(nd, b, lt, e.cols, e.C, &e.ctl[CTL_STATUS], smem, e.nodes, fz, &e.rp);
+                scatter_tile<P, R, false>(nd, b, lt, e.cols, e.C, &e.ctl[CTL_STATUS], smem, e.nodes, fz, &e.rp, b.totals + nd.bins);
         } else {
             transform_tile<P, R>(nd, b, lt, e.cols, e.C, e.rp);
         }
@@ -301,9 +301,9 @@ __global__ __launch_bounds__(1024) void k_micro(Engine e, uint32_t stage_mask, u
                 } else if (kind == KIND_PARTITION) {
                     const int32_t* fz = e.fuse + nd.route_offset;
                     if (stage)
-                        scatter_tile<P, R, true>(nd, b, t, e.cols, e.C, &e.ctl[CTL_STATUS], gsm, e.nodes, fz, &e.rp);
+                        scatter_tile<P, R, true>(nd, b, t, e.cols, e.C, &e.ctl[CTL_STATUS], gsm, e.nodes, fz, &e.rp, b.totals + nd.bins);
                     else
-                        scatter_tile<P, R, false>(nd, b, t, e.cols, e.C, &e.ctl[CTL_STATUS], gsm, e.nodes, fz, &e.rp);
+                        scatter_tile<P, R, false>(nd, b, t, e.cols, e.C, &e.ctl[CTL_STATUS], gsm, e.nodes, fz, &e.rp, b.totals + nd.bins);
                 } else {
                     transform_tile<P, R>(nd, b, t, e.cols, e.C, e.rp);
                 }
