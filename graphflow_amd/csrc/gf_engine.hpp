@@ -447,6 +447,7 @@ struct SelLds {
     NodeSt st[kMaxNodes];
     unsigned long long tail[kMaxNodes];
     unsigned long long head[kMaxNodes];
+    uint32_t ring_at[kMaxNodes];  // ring slot of node k's tail (retire) / head (select): cursor % ring_cap
     uint32_t off[kMaxNodes + 1];
     uint32_t lst_off[kMaxNodes + 1];  // staged ready descriptors of node k: lst[lst_off[k] ..)
     uint32_t wanted[kMaxNodes];
@@ -485,6 +486,7 @@ __device__ inline void engine_retire(const Engine& e, SelLds& L)
     for (uint32_t k = tid; k < n; k += bsz()) {
         L.tail[k] = L.st[k].tail;
         L.head[k] = L.st[k].head;
+        L.ring_at[k] = (uint32_t)(L.st[k].tail % e.ring_cap);  // the one 64-bit remainder per node
     }
     // outputs to route: node order, output order
     if (tid == 0) {
@@ -518,14 +520,20 @@ __device__ inline void engine_retire(const Engine& e, SelLds& L)
         if (v) {
             uint32_t pre = 0;
             for (uint32_t ww = 0; ww < w; ++ww) pre += L.wcnt[ww][dst];
-            unsigned long long pos = L.tail[dst] + pre + rank;
-            e.rings[(size_t)dst * e.ring_cap + (size_t)(pos % e.ring_cap)] = d;
+            // slot of tail + pre + rank: a ring never holds more than ring_cap (= S) chunks, so
+            // pre + rank < ring_cap and one subtraction wraps it
+            uint32_t slot = L.ring_at[dst] + pre + rank;
+            if (slot >= e.ring_cap) slot -= e.ring_cap;
+            e.rings[(size_t)dst * e.ring_cap + slot] = d;
         }
         __syncthreads();
         for (uint32_t k = tid; k < n; k += bsz()) {
             uint32_t sum = 0;
             for (uint32_t ww = 0; ww < bwaves(); ++ww) sum += L.wcnt[ww][k];
             L.tail[k] += sum;
+            uint32_t at = L.ring_at[k] + sum;
+            if (at >= e.ring_cap) at -= e.ring_cap;
+            L.ring_at[k] = at;
         }
         __syncthreads();
     }
@@ -588,13 +596,19 @@ __device__ inline void engine_select(const Engine& e, SelLds& L)
         L.staged = acc;
         if (acc > kListCap) L.overflow = 1u;
     }
+    for (uint32_t k = tid; k < n; k += bsz()) L.ring_at[k] = (uint32_t)(L.head[k] % e.ring_cap);
     __syncthreads();
+    // descriptor i of node k's ready chunks (i < the node's ring occupancy <= ring_cap)
+    auto ring_entry = [&](uint32_t k, uint32_t i) -> const gf_chunk_desc& {
+        uint32_t slot = L.ring_at[k] + i;
+        if (slot >= e.ring_cap) slot -= e.ring_cap;
+        return e.rings[(size_t)k * e.ring_cap + slot];
+    };
     if (!L.overflow) {
         for (uint32_t j = tid; j < L.staged; j += bsz()) {
             uint32_t k = 0;
             while (k + 1u < n && L.lst_off[k + 1u] <= j) ++k;
-            unsigned long long pos = L.head[k] + (j - L.lst_off[k]);
-            L.lst[j] = e.rings[(size_t)k * e.ring_cap + (size_t)(pos % e.ring_cap)];
+            L.lst[j] = ring_entry(k, j - L.lst_off[k]);
         }
     }
     __syncthreads();
@@ -631,8 +645,7 @@ __device__ inline void engine_select(const Engine& e, SelLds& L)
                 uint32_t wanted = L.wanted[k];
                 unsigned long long parts = 0;
                 for (uint32_t i = 0; i < wanted; ++i) {
-                    gf_chunk_desc d = L.overflow ? e.rings[(size_t)k * e.ring_cap + (size_t)((L.head[k] + i) % e.ring_cap)]
-                                                 : L.lst[L.lst_off[k] + i];
+                    gf_chunk_desc d = L.overflow ? ring_entry(k, i) : L.lst[L.lst_off[k] + i];
                     parts += d.fill;
                 }
                 const uint32_t B = kind == KIND_PARTITION ? e.nodes[k].bins : 1u;
@@ -643,8 +656,7 @@ __device__ inline void engine_select(const Engine& e, SelLds& L)
                         break;
                     }
                     --wanted;
-                    gf_chunk_desc d = L.overflow ? e.rings[(size_t)k * e.ring_cap + (size_t)((L.head[k] + wanted) % e.ring_cap)]
-                                                 : L.lst[L.lst_off[k] + wanted];
+                    gf_chunk_desc d = L.overflow ? ring_entry(k, wanted) : L.lst[L.lst_off[k] + wanted];
                     parts -= d.fill;
                 }
                 s.n_in = wanted;
@@ -665,7 +677,7 @@ __device__ inline void engine_select(const Engine& e, SelLds& L)
             const uint32_t i = i0 + tid;
             gf_chunk_desc d{0, 0, 0, 0};
             if (i < m) {
-                d = L.overflow ? e.rings[(size_t)k * e.ring_cap + (size_t)((L.head[k] + i) % e.ring_cap)] : L.lst[L.lst_off[k] + i];
+                d = L.overflow ? ring_entry(k, i) : L.lst[L.lst_off[k] + i];
                 a.in[i] = d;
             }
             uint32_t tt, ti;
