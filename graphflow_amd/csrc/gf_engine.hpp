@@ -467,6 +467,8 @@ __device__ inline uint64_t bound_of(uint64_t n, uint64_t C, uint64_t B)
     return r < n ? r : n;
 }
 
+constexpr uint32_t kAhead = 4;  // block-sized rounds of descriptor loads issued together
+
 __device__ inline void engine_retire(const Engine& e, SelLds& L)
 {
     const uint32_t tid = threadIdx.x, n = e.n_nodes, w = tid >> 6;
@@ -499,43 +501,58 @@ __device__ inline void engine_retire(const Engine& e, SelLds& L)
         L.total = acc;
     }
     __syncthreads();
-    // route in chunks of 256 with an order-preserving per-destination rank
-    for (uint32_t j0 = 0; j0 < L.total; j0 += bsz()) {
-        const uint32_t j = j0 + tid;
-        const bool v = j < L.total;
-        gf_chunk_desc d{0, 0, 0, 0};
-        uint32_t dst = 0xFFFFFFFFu;
-        if (v) {
-            uint32_t k = 0;
-            while (k + 1u < n && L.off[k + 1u] <= j) ++k;
-            d = e.arr[k].out[j - L.off[k]];
-            dst = (uint32_t)e.routes[e.nodes[k].route_offset + d.bin];
+    // route one block-sized round at a time with an order-preserving per-destination rank; the
+    // descriptors and destinations of kAhead rounds are loaded together
+    for (uint32_t g0 = 0; g0 < L.total; g0 += kAhead * bsz()) {
+        gf_chunk_desc dd[kAhead];
+        uint32_t dk[kAhead], ds[kAhead];
+#pragma unroll
+        for (uint32_t u = 0; u < kAhead; ++u) {
+            const uint32_t j = g0 + u * bsz() + tid;
+            dd[u] = gf_chunk_desc{0, 0, 0, 0};
+            dk[u] = 0u;
+            if (j < L.total) {
+                uint32_t k = 0;
+                while (k + 1u < n && L.off[k + 1u] <= j) ++k;
+                dk[u] = k;
+                dd[u] = e.arr[k].out[j - L.off[k]];
+            }
         }
-        for (uint32_t i = tid; i < kMaxWaves * kMaxNodes; i += bsz()) (&L.wcnt[0][0])[i] = 0u;
-        __syncthreads();
-        uint64_t peers = peer_mask(v, dst);
-        const uint32_t rank = (uint32_t)__popcll(peers & lanemask_lt());
-        if (v && (peers & lanemask_lt()) == 0ull) L.wcnt[w][dst] = (uint32_t)__popcll(peers);
-        __syncthreads();
-        if (v) {
-            uint32_t pre = 0;
-            for (uint32_t ww = 0; ww < w; ++ww) pre += L.wcnt[ww][dst];
-            // slot of tail + pre + rank: a ring never holds more than ring_cap (= S) chunks, so
-            // pre + rank < ring_cap and one subtraction wraps it
-            uint32_t slot = L.ring_at[dst] + pre + rank;
-            if (slot >= e.ring_cap) slot -= e.ring_cap;
-            e.rings[(size_t)dst * e.ring_cap + slot] = d;
+#pragma unroll
+        for (uint32_t u = 0; u < kAhead; ++u)
+            ds[u] = g0 + u * bsz() + tid < L.total ? (uint32_t)e.routes[e.nodes[dk[u]].route_offset + dd[u].bin] : 0xFFFFFFFFu;
+#pragma unroll
+        for (uint32_t u = 0; u < kAhead; ++u) {
+            if (g0 + u * bsz() >= L.total) break;
+            const bool v = g0 + u * bsz() + tid < L.total;
+            const gf_chunk_desc d = dd[u];
+            const uint32_t dst = ds[u];
+            for (uint32_t i = tid; i < kMaxWaves * kMaxNodes; i += bsz()) (&L.wcnt[0][0])[i] = 0u;
+            __syncthreads();
+            uint64_t peers = peer_mask(v, dst);
+            const uint32_t rank = (uint32_t)__popcll(peers & lanemask_lt());
+            if (v && (peers & lanemask_lt()) == 0ull) L.wcnt[w][dst] = (uint32_t)__popcll(peers);
+            __syncthreads();
+            if (v) {
+                uint32_t pre = 0;
+                for (uint32_t ww = 0; ww < w; ++ww) pre += L.wcnt[ww][dst];
+                // slot of tail + pre + rank: a ring never holds more than ring_cap (= S) chunks, so
+                // pre + rank < ring_cap and one subtraction wraps it
+                uint32_t slot = L.ring_at[dst] + pre + rank;
+                if (slot >= e.ring_cap) slot -= e.ring_cap;
+                e.rings[(size_t)dst * e.ring_cap + slot] = d;
+            }
+            __syncthreads();
+            for (uint32_t k = tid; k < n; k += bsz()) {
+                uint32_t sum = 0;
+                for (uint32_t ww = 0; ww < bwaves(); ++ww) sum += L.wcnt[ww][k];
+                L.tail[k] += sum;
+                uint32_t at = L.ring_at[k] + sum;
+                if (at >= e.ring_cap) at -= e.ring_cap;
+                L.ring_at[k] = at;
+            }
+            __syncthreads();
         }
-        __syncthreads();
-        for (uint32_t k = tid; k < n; k += bsz()) {
-            uint32_t sum = 0;
-            for (uint32_t ww = 0; ww < bwaves(); ++ww) sum += L.wcnt[ww][k];
-            L.tail[k] += sum;
-            uint32_t at = L.ring_at[k] + sum;
-            if (at >= e.ring_cap) at -= e.ring_cap;
-            L.ring_at[k] = at;
-        }
-        __syncthreads();
     }
     // release every consumed input to the free stack (generation + 1)
     if (tid == 0) {
@@ -548,11 +565,24 @@ __device__ inline void engine_retire(const Engine& e, SelLds& L)
         L.total = acc;
     }
     __syncthreads();
-    for (uint32_t j = tid; j < L.total; j += bsz()) {
-        uint32_t k = 0;
-        while (k + 1u < n && L.off[k + 1u] <= j) ++k;
-        gf_chunk_desc d = e.arr[k].in[j - L.off[k]];
-        e.free_stack[L.top + j] = (unsigned long long)d.index | ((unsigned long long)(d.generation + 1u) << 32);
+    for (uint32_t g0 = tid; g0 < L.total; g0 += kAhead * bsz()) {
+        gf_chunk_desc dd[kAhead];
+#pragma unroll
+        for (uint32_t u = 0; u < kAhead; ++u) {
+            const uint32_t j = g0 + u * bsz();
+            dd[u] = gf_chunk_desc{0, 0, 0, 0};
+            if (j < L.total) {
+                uint32_t k = 0;
+                while (k + 1u < n && L.off[k + 1u] <= j) ++k;
+                dd[u] = e.arr[k].in[j - L.off[k]];
+            }
+        }
+#pragma unroll
+        for (uint32_t u = 0; u < kAhead; ++u) {
+            const uint32_t j = g0 + u * bsz();
+            if (j < L.total)
+                e.free_stack[L.top + j] = (unsigned long long)dd[u].index | ((unsigned long long)(dd[u].generation + 1u) << 32);
+        }
     }
     __syncthreads();
     if (tid == 0) {
@@ -673,22 +703,32 @@ __device__ inline void engine_select(const Engine& e, SelLds& L)
         const NodeArr& a = e.arr[k];
         const uint32_t m = s.n_in, T = a.tile_items;
         uint32_t carry_t = 0, carry_i = 0;
-        for (uint32_t i0 = 0; i0 < m; i0 += bsz()) {
-            const uint32_t i = i0 + tid;
-            gf_chunk_desc d{0, 0, 0, 0};
-            if (i < m) {
-                d = L.overflow ? ring_entry(k, i) : L.lst[L.lst_off[k] + i];
-                a.in[i] = d;
+        for (uint32_t g0 = 0; g0 < m; g0 += kAhead * bsz()) {
+            // the descriptors of kAhead rounds loaded together, then one round at a time
+            gf_chunk_desc dd[kAhead];
+#pragma unroll
+            for (uint32_t u = 0; u < kAhead; ++u) {
+                const uint32_t i = g0 + u * bsz() + tid;
+                dd[u] = gf_chunk_desc{0, 0, 0, 0};
+                if (i < m) dd[u] = L.overflow ? ring_entry(k, i) : L.lst[L.lst_off[k] + i];
             }
-            uint32_t tt, ti;
-            uint32_t pt = block_exscan(i < m ? (d.fill + T - 1u) / T : 0u, L.scan, &tt);
-            uint32_t pi = block_exscan(i < m ? d.fill : 0u, L.scan, &ti);
-            if (i < m) {
-                a.tile_prefix[i] = carry_t + pt;
-                a.item_prefix[i] = carry_i + pi;
+#pragma unroll
+            for (uint32_t u = 0; u < kAhead; ++u) {
+                const uint32_t i0 = g0 + u * bsz();
+                if (i0 >= m) break;
+                const uint32_t i = i0 + tid;
+                const gf_chunk_desc d = dd[u];
+                if (i < m) a.in[i] = d;
+                uint32_t tt, ti;
+                uint32_t pt = block_exscan(i < m ? (d.fill + T - 1u) / T : 0u, L.scan, &tt);
+                uint32_t pi = block_exscan(i < m ? d.fill : 0u, L.scan, &ti);
+                if (i < m) {
+                    a.tile_prefix[i] = carry_t + pt;
+                    a.item_prefix[i] = carry_i + pi;
+                }
+                carry_t += tt;
+                carry_i += ti;
             }
-            carry_t += tt;
-            carry_i += ti;
         }
         if (tid == 0) {
             a.tile_prefix[m] = carry_t;
