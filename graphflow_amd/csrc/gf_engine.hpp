@@ -704,31 +704,38 @@ __device__ inline void engine_select(const Engine& e, SelLds& L)
         const uint32_t m = s.n_in, T = a.tile_items;
         uint32_t carry_t = 0, carry_i = 0;
         for (uint32_t g0 = 0; g0 < m; g0 += kAhead * bsz()) {
-            // the descriptors of kAhead rounds loaded together, then one round at a time
+            // kAhead consecutive inputs per thread: their loads together, one pair of block scans
+            // over the threads' sums
             gf_chunk_desc dd[kAhead];
+            uint32_t tl[kAhead], st = 0u, sf = 0u;
 #pragma unroll
             for (uint32_t u = 0; u < kAhead; ++u) {
-                const uint32_t i = g0 + u * bsz() + tid;
+                const uint32_t i = g0 + tid * kAhead + u;
                 dd[u] = gf_chunk_desc{0, 0, 0, 0};
                 if (i < m) dd[u] = L.overflow ? ring_entry(k, i) : L.lst[L.lst_off[k] + i];
             }
 #pragma unroll
             for (uint32_t u = 0; u < kAhead; ++u) {
-                const uint32_t i0 = g0 + u * bsz();
-                if (i0 >= m) break;
-                const uint32_t i = i0 + tid;
-                const gf_chunk_desc d = dd[u];
-                if (i < m) a.in[i] = d;
-                uint32_t tt, ti;
-                uint32_t pt = block_exscan(i < m ? (d.fill + T - 1u) / T : 0u, L.scan, &tt);
-                uint32_t pi = block_exscan(i < m ? d.fill : 0u, L.scan, &ti);
+                tl[u] = (dd[u].fill + T - 1u) / T;  // 0 for the empty entries past m
+                st += tl[u];
+                sf += dd[u].fill;
+            }
+            uint32_t tt, ti;
+            uint32_t pt = block_exscan(st, L.scan, &tt);
+            uint32_t pi = block_exscan(sf, L.scan, &ti);
+#pragma unroll
+            for (uint32_t u = 0; u < kAhead; ++u) {
+                const uint32_t i = g0 + tid * kAhead + u;
                 if (i < m) {
+                    a.in[i] = dd[u];
                     a.tile_prefix[i] = carry_t + pt;
                     a.item_prefix[i] = carry_i + pi;
                 }
-                carry_t += tt;
-                carry_i += ti;
+                pt += tl[u];
+                pi += dd[u].fill;
             }
+            carry_t += tt;
+            carry_i += ti;
         }
         if (tid == 0) {
             a.tile_prefix[m] = carry_t;
