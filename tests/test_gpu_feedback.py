@@ -106,3 +106,23 @@ def test_device_engines_agree_on_large_run():
         traces.append(trace)
     assert_same_trace(traces[0], traces[1])
     assert_same_trace(traces[0], traces[2])
+
+
+@pytest.mark.parametrize("distribution", ["parity", "skew_90_10", "all_one", "uniform"])
+def test_config2_full_size_histogram_equals_golden(distribution):
+    """Config 2 at full size: N=1e8, C=8192, P=16384, Q=8, seed 42, batched team engine with A/B
+    fused; the sorted histogram must equal the oracle's (SHA-256 of the rows, row count, sum = N)
+    recorded in tests/golden/feedback_N1e8.json."""
+    import json
+    import os
+
+    fx = json.load(open(os.path.join(os.path.dirname(__file__), "golden", "feedback_N1e8.json")))
+    case = fx["cases"][distribution]
+    N = fx["items"]
+    flow, src, counts = feedback_flow(N, sink="hist", distribution=distribution, destinations=case["destinations"])
+    stats, _ = run_flow(flow, src, N, C=8192, P=16384, Q=8, schedule=g.SCHEDULE_DEVICE, fuse=True, engine=3,
+                        trace=False)
+    rows = g.hist_rows(counts.data_ptr(), g.FEEDBACK_HIST_N, g.FEEDBACK_HIST_LO)
+    assert stats["sunk"] == N
+    assert len(rows) == case["rows"] and int(rows[:, 1].sum()) == N
+    assert g.rows_sha256(rows) == case["sha256"]

@@ -115,27 +115,112 @@ def test_philox_known_answers():
         [0xD16CFE09, 0x94FDCCEB, 0x5001E420, 0x24126EA1]
 
 
-def _mix64_py(v):
-    m = (1 << 64) - 1
-    v ^= v >> 30
-    v = (v * 0xBF58476D1CE4E5B9) & m
-    v ^= v >> 27
-    v = (v * 0x94D049BB133111EB) & m
-    v ^= v >> 31
-    return v
+M64 = (1 << 64) - 1
 
 
-def test_stream_key_closed_form_and_uniqueness():
-    # rng/ParticleRng.hpp:57-94; uniqueness property of tests/unit/test_philox.cpp:48-61
+def test_mix64_splitmix64_published_outputs():
+    # SplitMix64 (Steele, Lea, Flood, OOPSLA 2014; Vigna's reference splitmix64.c): every output is
+    # the finalizer of state += 0x9E3779B97F4A7C15. The first five outputs from state 1234567 are
+    # the generator's published test vector; a wrong shift or multiplier fails all of them.
     L = O.lib()
-    for seed, i, ev, pur in [(42, 7, 0, 0), (0, 0, 0, 0), (2**64 - 1, 2**32 - 1, 0xFFFF, 0xFFFF)]:
-        packed = (i & 0xFFFFFFFF) | ((ev & 0xFFFF) << 32) | ((pur & 0xFFFF) << 48)
-        assert L.or_stream_key(seed, i, ev, pur) == _mix64_py(packed ^ _mix64_py(seed))
-    keys = {L.or_stream_key(42, i, e, p) for i in range(32) for e in range(16) for p in range(8)}
-    assert len(keys) == 32 * 16 * 8
-    # SplitMix64 finalizer of 0 is 0, of 1 the published value
+    published = [6457827717110365317, 3203168211198807973, 9817491932198370423,
+                 4593380528125082431, 16408922859458223821]
+    state = 1234567
+    for want in published:
+        state = (state + 0x9E3779B97F4A7C15) & M64
+        assert L.or_mix64(state) == want
     assert L.or_mix64(0) == 0
-    assert L.or_mix64(1) == _mix64_py(1)
+
+
+def _unxorshift(y, s):
+    # inverse of x ^ (x >> s): fixed point of x = y ^ (x >> s), reached after 64 / s rounds
+    x = y
+    for _ in range(64 // s + 1):
+        x = y ^ (x >> s)
+    return x
+
+
+def _unmix64(v):
+    # the finalizer's inverse from its algebra: undo each xor-shift, multiply by the modular inverse
+    v = _unxorshift(v, 31)
+    v = (v * pow(0x94D049BB133111EB, -1, 1 << 64)) & M64
+    v = _unxorshift(v, 27)
+    v = (v * pow(0xBF58476D1CE4E5B9, -1, 1 << 64)) & M64
+    return _unxorshift(v, 30)
+
+
+def test_stream_key_bit_layout_through_the_inverse_mixer():
+    # rng/ParticleRng.hpp:35-38,56-94: id in bits 0-31, event in 32-47, purpose in 48-63, the packed
+    # position xored with mix64(seed) before the final mix. Undoing the final mix with the
+    # independently derived inverse exposes the packed position, so each field's bit range is
+    # checked on its own (a swapped field, a wrong shift or a missing seed mix fails).
+    L = O.lib()
+    assert all(_unmix64(L.or_mix64(v)) == v for v in (0, 1, 12345, 2**63, M64, 0xDEADBEEFCAFEF00D))
+    for seed in (0, 42, M64):
+        s = L.or_mix64(seed)
+        pos = lambda i, e, p: _unmix64(L.or_stream_key(seed, i, e, p)) ^ s
+        assert pos(0, 0, 0) == 0
+        assert pos(7, 0, 0) == 7 and pos(0xFFFFFFFF, 0, 0) == 0xFFFFFFFF
+        assert pos(0, 1, 0) == 1 << 32 and pos(0, 0xFFFF, 0) == 0xFFFF << 32
+        assert pos(0, 0, 1) == 1 << 48 and pos(0, 0, 0xFFFF) == 0xFFFF << 48
+        assert pos(3, 5, 9) == 3 | 5 << 32 | 9 << 48
+        # fields are masked to their bit budget, never carried into the next field
+        assert pos(2**32 + 3, 0, 0) == 3 and pos(0, 0x10005, 0) == 5 << 32
+    # uniqueness property of tests/unit/test_philox.cpp:48-61 (64 ids x 64 events x 32 purposes)
+    keys = {L.or_stream_key(42, i, e, p) for i in range(64) for e in range(64) for p in range(32)}
+    assert len(keys) == 64 * 64 * 32
+
+
+def _insert(keys, counts, bin_):
+    import ctypes as C
+    u = C.POINTER(C.c_uint32)
+    return O.lib().or_hash_table_insert(keys.ctypes.data_as(u), counts.ctypes.data_as(u), len(keys), bin_)
+
+
+def _table(entries):
+    return np.full(entries, 0xFFFFFFFF, np.uint32), np.zeros(entries, np.uint32)
+
+
+def test_hash_table_exhaustion():
+    # tests/integration/test_hash_bins.cpp:504-546: HashBinTable<4> has nextPow2(2*4) = 8 entries,
+    # so the ninth distinct bin of a tile finds the table full; bins already present still count
+    L = O.lib()
+    keys, counts = _table(L.or_next_pow2(2 * 4))
+    assert len(keys) == 8
+    assert [_insert(keys, counts, b) for b in range(8)] == [1] * 8
+    assert all(_insert(keys, counts, b) == 0 for b in range(8, 16))
+    assert _insert(keys, counts, 3) == 1
+    assert sorted(keys.tolist()) == list(range(8))
+    assert counts[keys.tolist().index(3)] == 2 and counts.sum() == 9
+
+
+def test_hash_table_probe_sequence():
+    # BinTable.hpp:90-121: home slot (bin * 0x9E3779B1) & mask, linear probing on collision
+    L = O.lib()
+    keys, counts = _table(16)
+    home = lambda b: (b * 0x9E3779B1) & 15
+    assert _insert(keys, counts, 5) == 1 and keys[home(5)] == 5
+    # 5 + 16 has the same home slot (16 * odd = 0 mod 16): it lands one slot further
+    assert home(21) == home(5)
+    assert _insert(keys, counts, 21) == 1 and keys[(home(5) + 1) & 15] == 21
+    assert _insert(keys, counts, 37) == 1 and keys[(home(5) + 2) & 15] == 37
+    # the probe wraps past the last slot
+    keys, counts = _table(8)
+    b = next(x for x in range(100) if (x * 0x9E3779B1) & 7 == 7)
+    assert _insert(keys, counts, b) == 1 and _insert(keys, counts, b + 8) == 1
+    assert keys[7] == b and keys[0] == b + 8
+
+
+def test_hash_table_counts_equal_a_brute_force_histogram():
+    rng = np.random.default_rng(7)
+    for entries, distinct in ((256, 128), (512, 300), (64, 64)):
+        keys, counts = _table(entries)
+        bins = rng.choice(1 << 20, distinct, replace=False)[rng.integers(0, distinct, 4 * entries)]
+        assert all(_insert(keys, counts, int(b)) == 1 for b in bins)
+        occ = keys != 0xFFFFFFFF
+        got = dict(zip(keys[occ].tolist(), counts[occ].tolist()))
+        v, c = np.unique(bins, return_counts=True)
+        assert got == dict(zip(v.tolist(), c.tolist()))
 
 
 def test_fmix32_known_values():
@@ -302,3 +387,18 @@ def test_stall_is_reported():
          dict(role=2, op="SNK_HIST", bins=0, u=[2, 0, 8])]
     r = O.run(g, seed=1, items=64, capacity=8, live=64, batch=1, num_chunks=3, mode=O.MODE_RUNNER, hist_size=8)
     assert r["status"] == O.STALL
+
+
+def test_feedback_golden_fixture_headline():
+    # tests/golden/feedback_N1e8.json holds the N=1e8 histograms of the four routing distributions
+    # from the schedule-free model; recompute the headline (parity routing) case and its SHA
+    import json
+    import os
+
+    fx = json.load(open(os.path.join(os.path.dirname(__file__), "golden", "feedback_N1e8.json")))
+    case = fx["cases"]["parity"]
+    w, _ = O.feedback_scalar(fx["seed"], fx["items"], 1e-3, 0, 2, (0.5, 0.75), threads=os.cpu_count() or 1)
+    v, c = np.unique(w, return_counts=True)
+    assert len(v) == case["rows"] and int(c.sum()) == case["sum"] == fx["items"]
+    text = "".join(f"{a},{b}\n" for a, b in zip(v.tolist(), c.tolist())).encode()
+    assert hashlib.sha256(text).hexdigest() == case["sha256"]
