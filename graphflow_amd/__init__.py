@@ -25,6 +25,7 @@ OK, INVALID, OVERFLOW, STALL, POOL_EXHAUSTED, BAD_ARGUMENT, PLAN_MISMATCH, DEVIC
 PAYLOAD_PARTICLE, PAYLOAD_KEYED, PAYLOAD_PHOTON = 0, 1, 2
 ROLE_SOURCE, ROLE_TRANSFORM, ROLE_SINK = 0, 1, 2
 SCHEDULE_HOST, SCHEDULE_DEVICE = 0, 1
+BIN_PATH_AUTO, BIN_PATH_DENSE, BIN_PATH_HASH = 0, 1, 2
 
 OPS = {
     "SRC_PARTICLE_UNIFORM": 1, "SRC_KEYED_ZIPF": 2, "SRC_PHOTON": 3,
@@ -93,7 +94,8 @@ class TuningC(C.Structure):
                 ("device_engine", C.c_uint32), ("photon_layers", C.c_uint32), ("photon_tally_bins", C.c_uint32),
                 ("photon_layer_mfp", C.c_double), ("photon_mu_t", C.c_double), ("photon_albedo", C.c_double),
                 ("photon_hg_g", C.c_double), ("photon_roulette_w", C.c_double), ("photon_roulette_p", C.c_double),
-                ("team_xcd", C.c_uint32), ("reserved0", C.c_uint32)]
+                ("team_xcd", C.c_uint32), ("inflight_batches", C.c_uint32), ("bin_path", C.c_uint32),
+                ("hash_table_entries", C.c_uint32), ("stall_timeout_ms", C.c_uint32), ("reserved1", C.c_uint32)]
 
 
 class Stats(C.Structure):
@@ -471,10 +473,14 @@ class Tuning:
     device_engine: int = 0  # 0 autotuned, 1 wide step kernels, 2 one persistent workgroup, 3 XCD team
     photon: Optional[dict] = None
     team_xcd: Optional[int] = None  # team engine: run the team on this XCD (None: where it first starts)
+    inflight_batches: Optional[int] = None  # batches in flight per node (Tuning::inflightBatches)
+    bin_path: int = 0  # BIN_PATH_AUTO (hash above 1024 bins), BIN_PATH_DENSE, BIN_PATH_HASH
+    hash_table_entries: Optional[int] = None  # hash path: slots of a tile's bin table (None: 2 * tile items)
+    stall_timeout_ms: Optional[int] = None  # plain schedule: longest wait for a device stage (Tuning::stallTimeout)
 
     def validate(self):
         for name in ("batch_chunks", "chunk_capacity", "live_particles", "num_chunks", "tile_items",
-                     "steps_per_launch"):
+                     "steps_per_launch", "inflight_batches", "hash_table_entries", "stall_timeout_ms"):
             v = getattr(self, name)
             if v is not None and v == 0:
                 raise ValueError(f"graphFlow::Tuning: {name} must be non-zero")
@@ -486,7 +492,9 @@ class Tuning:
                     steps_per_launch=self.steps_per_launch or 0, trace=1 if self.trace else 0,
                     use_graph=self.use_graph, fuse_transforms=1 if self.fuse_transforms else 0,
                     max_workgroups=self.max_workgroups or 0, device_engine=self.device_engine,
-                    team_xcd=0 if self.team_xcd is None else self.team_xcd + 1)
+                    team_xcd=0 if self.team_xcd is None else self.team_xcd + 1,
+                    inflight_batches=self.inflight_batches or 0, bin_path=self.bin_path,
+                    hash_table_entries=self.hash_table_entries or 0, stall_timeout_ms=self.stall_timeout_ms or 0)
         ph = self.photon or {}
         t.photon_layers = ph.get("layers", 10)
         t.photon_tally_bins = ph.get("tally_bins", 1024)

@@ -10,7 +10,12 @@
 //       P, largest credit-fitting prefix of <= Q ready chunks per transform node, all ready chunks
 //       per sink).
 // The scheduling rules are those of GraphRunner::run (sched/GraphRunner.hpp:902-1133) applied
-// once per node per step; the host only replays a hipGraph of K steps and polls a done flag.
+// once per lane per step; the host only replays a hipGraph of K steps and polls a done flag.
+//
+// Lanes (Tuning::inflightBatches, flow/Tuning.hpp:34-41; GraphRunner.hpp:766-771): every source
+// and transform node runs up to D batches per step. The engine's nodes are the lanes, in (node,
+// lane) order; the lanes of a node share its ready ring, which belongs to its first lane
+// (ring_of), and routes name that first lane. A sink keeps one lane: it takes all ready chunks.
 //
 // The one-workgroup phases are latency bound: they keep the whole control state in LDS and
 // touch global memory in a constant number of parallel round trips per step.
@@ -20,7 +25,7 @@
 
 namespace gfd {
 
-constexpr uint32_t kMaxNodes = 64;
+constexpr uint32_t kMaxNodes = 128;  // engine nodes (lanes)
 constexpr uint32_t kListCap = 2048;   // chunk descriptors staged in LDS by the select phase
 constexpr uint32_t kPlanWords = 8192; // [tile][bin] words of one partition planned in LDS
 
@@ -40,7 +45,13 @@ struct NodeArr {
     uint32_t* counts;
     uint32_t* totals;
     uint32_t tile_items, purpose, cap_in, cap_out, reserve_bound, kind;  // kind: 0 src, 1 part, 2 xf, 3 sink
+    uint32_t flags;  // NODE_STAGE: scatter stages its tile in LDS; NODE_WAVE_SINK: one tile per wave
+    uint32_t real;   // the graph node this lane belongs to (trace records, stream purpose = 2 * real)
+    uint32_t lanes;  // first lane of a node: the node's lanes (it stages candidates for all of them)
+    uint32_t ring;   // first lane of a node: index of the node's ready ring in `rings`
 };
+
+enum { NODE_STAGE = 1u, NODE_WAVE_SINK = 2u };
 
 enum { KIND_SOURCE = 0, KIND_PARTITION = 1, KIND_TRANSFORM = 2, KIND_SINK = 3 };
 
@@ -49,9 +60,11 @@ struct Engine {
     uint64_t N;
     RunParams rp;
     Cols cols;
-    const gf_node_desc* nodes;
-    const int32_t* routes;  // effective route per (node, bin), fused transforms bypassed
-    const int32_t* fuse;    // per (node, bin): transform node applied inside the scatter, or -1
+    const gf_node_desc* nodes;       // per lane (a copy of its node's descriptor)
+    const gf_node_desc* real_nodes;  // per graph node (fused transforms are looked up here)
+    const int32_t* routes;  // effective route per (node, bin) to the destination's first lane, fused transforms bypassed
+    const int32_t* fuse;    // per (node, bin): graph node of the transform applied inside the scatter, or -1
+    const uint32_t* ring_of;  // per lane: the lane whose ring holds its node's ready chunks
     const NodeArr* arr;
     NodeSt* st;
     unsigned long long* free_stack;  // entry = index | generation << 32
@@ -194,7 +207,7 @@ __device__ inline void trace_record(const Engine& e, uint32_t k, const Batch& b,
         if (p0 + p < e.trace_cap) e.trace[p0 + p] = v;
     };
     if (tid == 0) {
-        put(0, 0xB47C0000u | k);
+        put(0, 0xB47C0000u | e.arr[k].real);
         put(1, step);
         put(2, b.n_in);
         put(3 + 3ull * b.n_in, nbins);
@@ -451,6 +464,8 @@ struct SelLds {
     uint32_t off[kMaxNodes + 1];
     uint32_t lst_off[kMaxNodes + 1];  // staged ready descriptors of node k: lst[lst_off[k] ..)
     uint32_t wanted[kMaxNodes];
+    uint32_t taken[kMaxNodes];  // first lane of a node: ready chunks its lanes took this step
+    uint32_t in_at[kMaxNodes];  // lane: position of its first input in its node's ready chunks
     uint32_t wcnt[kMaxWaves][kMaxNodes];
     uint32_t scan[kMaxWaves];
     uint32_t top, total, any, staged, overflow;
@@ -540,7 +555,7 @@ __device__ inline void engine_retire(const Engine& e, SelLds& L)
                 // pre + rank < ring_cap and one subtraction wraps it
                 uint32_t slot = L.ring_at[dst] + pre + rank;
                 if (slot >= e.ring_cap) slot -= e.ring_cap;
-                e.rings[(size_t)dst * e.ring_cap + slot] = d;
+                e.rings[(size_t)e.arr[dst].ring * e.ring_cap + slot] = d;
             }
             __syncthreads();
             for (uint32_t k = tid; k < n; k += bsz()) {
@@ -610,16 +625,20 @@ __device__ inline void engine_retire(const Engine& e, SelLds& L)
 __device__ inline void engine_select(const Engine& e, SelLds& L)
 {
     const uint32_t tid = threadIdx.x, n = e.n_nodes, C = e.C, P = e.P, Q = e.Q;
-    // stage the candidate ready descriptors of every non-source node: <= Q per transform, all per sink
+    // stage the candidate ready descriptors of every non-source node's ring: <= Q per transform
+    // lane, all per sink (the first lane of a node stages for all its lanes)
     if (tid == 0) {
         uint32_t acc = 0;
         L.overflow = 0u;
         for (uint32_t k = 0; k < n; ++k) {
             const uint32_t kind = e.arr[k].kind;
             unsigned long long avail = L.tail[k] - L.head[k];
-            uint32_t want = kind == KIND_SOURCE ? 0u : kind == KIND_SINK ? (uint32_t)avail : (avail < Q ? (uint32_t)avail : Q);
+            const unsigned long long cap = (unsigned long long)Q * e.arr[k].lanes;
+            uint32_t want = kind == KIND_SOURCE || e.ring_of[k] != k ? 0u : kind == KIND_SINK ? (uint32_t)avail
+                            : (avail < cap ? (uint32_t)avail : (uint32_t)cap);
             L.lst_off[k] = acc;
             L.wanted[k] = want;
+            L.taken[k] = 0u;
             acc += want;
         }
         L.lst_off[n] = acc;
@@ -632,7 +651,7 @@ __device__ inline void engine_select(const Engine& e, SelLds& L)
     auto ring_entry = [&](uint32_t k, uint32_t i) -> const gf_chunk_desc& {
         uint32_t slot = L.ring_at[k] + i;
         if (slot >= e.ring_cap) slot -= e.ring_cap;
-        return e.rings[(size_t)k * e.ring_cap + slot];
+        return e.rings[(size_t)e.arr[k].ring * e.ring_cap + slot];
     };
     if (!L.overflow) {
         for (uint32_t j = tid; j < L.staged; j += bsz()) {
@@ -671,11 +690,14 @@ __device__ inline void engine_select(const Engine& e, SelLds& L)
             } else if (kind == KIND_SINK) {
                 s.n_in = L.wanted[k];
                 s.active = s.n_in ? 1u : 0u;
+                L.in_at[k] = 0u;
             } else {
-                uint32_t wanted = L.wanted[k];
+                // this lane's candidates follow the chunks the node's earlier lanes took
+                const uint32_t l = e.ring_of[k], at = L.taken[l];
+                uint32_t wanted = L.wanted[l] - at < Q ? L.wanted[l] - at : Q;
                 unsigned long long parts = 0;
                 for (uint32_t i = 0; i < wanted; ++i) {
-                    gf_chunk_desc d = L.overflow ? ring_entry(k, i) : L.lst[L.lst_off[k] + i];
+                    gf_chunk_desc d = L.overflow ? ring_entry(l, at + i) : L.lst[L.lst_off[l] + at + i];
                     parts += d.fill;
                 }
                 const uint32_t B = kind == KIND_PARTITION ? e.nodes[k].bins : 1u;
@@ -686,11 +708,13 @@ __device__ inline void engine_select(const Engine& e, SelLds& L)
                         break;
                     }
                     --wanted;
-                    gf_chunk_desc d = L.overflow ? ring_entry(k, wanted) : L.lst[L.lst_off[k] + wanted];
+                    gf_chunk_desc d = L.overflow ? ring_entry(l, at + wanted) : L.lst[L.lst_off[l] + at + wanted];
                     parts -= d.fill;
                 }
                 s.n_in = wanted;
                 s.active = wanted ? 1u : 0u;
+                L.in_at[k] = at;
+                L.taken[l] = at + wanted;
             }
             L.any |= s.active;
         }
@@ -701,7 +725,7 @@ __device__ inline void engine_select(const Engine& e, SelLds& L)
         NodeSt& s = L.st[k];
         if (!s.active || e.arr[k].kind == KIND_SOURCE) continue;
         const NodeArr& a = e.arr[k];
-        const uint32_t m = s.n_in, T = a.tile_items;
+        const uint32_t m = s.n_in, T = a.tile_items, l = e.ring_of[k], at = L.in_at[k];
         uint32_t carry_t = 0, carry_i = 0;
         for (uint32_t g0 = 0; g0 < m; g0 += kAhead * bsz()) {
             // kAhead consecutive inputs per thread: their loads together, one pair of block scans
@@ -712,7 +736,7 @@ __device__ inline void engine_select(const Engine& e, SelLds& L)
             for (uint32_t u = 0; u < kAhead; ++u) {
                 const uint32_t i = g0 + tid * kAhead + u;
                 dd[u] = gf_chunk_desc{0, 0, 0, 0};
-                if (i < m) dd[u] = L.overflow ? ring_entry(k, i) : L.lst[L.lst_off[k] + i];
+                if (i < m) dd[u] = L.overflow ? ring_entry(l, at + i) : L.lst[L.lst_off[l] + at + i];
             }
 #pragma unroll
             for (uint32_t u = 0; u < kAhead; ++u) {
@@ -742,7 +766,7 @@ __device__ inline void engine_select(const Engine& e, SelLds& L)
             a.item_prefix[m] = carry_i;
             s.n_tiles = carry_t;
             s.particles = carry_i;
-            L.head[k] += m;
+            L.head[l] += m;
         }
         __syncthreads();
     }

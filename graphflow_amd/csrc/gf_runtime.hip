@@ -16,6 +16,7 @@
 #include <deque>
 #include <memory>
 #include <string>
+#include <thread>
 #include <type_traits>
 #include <vector>
 
@@ -185,10 +186,10 @@ __global__ __launch_bounds__(1024) void k_engine_plan(Engine e)
     engine_plan(e, *reinterpret_cast<PlanLds*>(smem));
 }
 
-// Tiles round-robin over the workgroups; the tiles of the sinks in `wave_sinks` (sinks fed by wide
-// partitions, whose chunks are small) round-robin over the waves instead.
+// Tiles round-robin over the workgroups; the tiles of the sinks flagged NODE_WAVE_SINK (sinks fed by
+// wide partitions, whose chunks are small) round-robin over the waves instead.
 template <class P, int R>
-__global__ __launch_bounds__(kThreads) void k_engine_kb(Engine e, uint32_t stage_mask, uint32_t wave_sinks)
+__global__ __launch_bounds__(kThreads) void k_engine_kb(Engine e)
 {
     extern __shared__ char smem[];
     __shared__ uint32_t flag;
@@ -196,7 +197,7 @@ __global__ __launch_bounds__(kThreads) void k_engine_kb(Engine e, uint32_t stage
     const uint32_t total = e.kb_prefix[e.n_nodes], G = gridDim.x;
     for (uint32_t t = blockIdx.x; t < total; t += G) {
         const uint32_t k = find_node(e.kb_prefix, e.n_nodes, t);
-        if (k < 32u && ((wave_sinks >> k) & 1u)) {
+        if (e.arr[k].flags & NODE_WAVE_SINK) {
             // skip the node's tiles: the first tile of this workgroup's round-robin at or after them
             const uint32_t end = e.kb_prefix[k + 1u];
             t += (end - t + G - 1u) / G * G - G;
@@ -213,18 +214,17 @@ __global__ __launch_bounds__(kThreads) void k_engine_kb(Engine e, uint32_t stage
             sink_tile<P>(nd, b, lt, e.cols, e.C, &e.ctl[CTL_STATUS], smem);
         } else if (kind == KIND_PARTITION) {
             const int32_t* fz = e.fuse + nd.route_offset;
-            if ((stage_mask >> (k & 31u)) & 1u)
-                scatter_tile<P, R, true>(nd, b, lt, e.cols, e.C, &e.ctl[CTL_STATUS], smem, e.nodes, fz, &e.rp, b.totals + nd.bins);
+            if (e.arr[k].flags & NODE_STAGE)
+                scatter_tile<P, R, true>(nd, b, lt, e.cols, e.C, &e.ctl[CTL_STATUS], smem, e.real_nodes, fz, &e.rp, b.totals + nd.bins);
             else
-                scatter_tile<P, R, false>(nd, b, lt, e.cols, e.C, &e.ctl[CTL_STATUS], smem, e.nodes, fz, &e.rp, b.totals + nd.bins);
+                scatter_tile<P, R, false>(nd, b, lt, e.cols, e.C, &e.ctl[CTL_STATUS], smem, e.real_nodes, fz, &e.rp, b.totals + nd.bins);
         } else {
             transform_tile<P, R>(nd, b, lt, e.cols, e.C, e.rp);
         }
     }
     const uint32_t w = threadIdx.x >> 6, W = G * (uint32_t)kWaves;
-    for (uint32_t ws = wave_sinks; ws; ws &= ws - 1u) {
-        const uint32_t k = (uint32_t)(__ffs(ws) - 1);
-        if (k >= e.n_nodes || !e.st[k].active) continue;
+    for (uint32_t k = 0; k < e.n_nodes; ++k) {
+        if (!(e.arr[k].flags & NODE_WAVE_SINK) || !e.st[k].active) continue;
         const uint32_t t0 = e.kb_prefix[k], t1 = e.kb_prefix[k + 1u];
         const gf_node_desc& nd = e.nodes[k];
         const Batch b = make_batch(e, k, e.st[k]);
@@ -250,7 +250,7 @@ __global__ __launch_bounds__(1024) void k_engine_kr(Engine e)
 // select phases use all threads. The live working set stays in the L2 of one XCD, so each of the
 // step's dependent round trips is an L2 hit instead of a cross-XCD miss.
 template <class P, int R>
-__global__ __launch_bounds__(1024) void k_micro(Engine e, uint32_t stage_mask, uint32_t max_steps, uint32_t group_lds)
+__global__ __launch_bounds__(1024) void k_micro(Engine e, uint32_t max_steps, uint32_t group_lds)
 {
     extern __shared__ char smem[];
     __shared__ uint32_t flag;
@@ -291,7 +291,7 @@ __global__ __launch_bounds__(1024) void k_micro(Engine e, uint32_t stage_mask, u
             const gf_node_desc& nd = e.nodes[k];
             const Batch b = make_batch(e, k, s);
             const uint32_t kind = e.arr[k].kind;
-            const bool stage = (stage_mask >> (k & 31u)) & 1u;
+            const bool stage = (e.arr[k].flags & NODE_STAGE) != 0u;
             for (uint32_t t0 = 0; t0 < s.n_tiles; t0 += G) {
                 const uint32_t t = t0 + g < s.n_tiles ? t0 + g : kNoTile;
                 if (kind == KIND_SOURCE) {
@@ -301,9 +301,9 @@ __global__ __launch_bounds__(1024) void k_micro(Engine e, uint32_t stage_mask, u
                 } else if (kind == KIND_PARTITION) {
                     const int32_t* fz = e.fuse + nd.route_offset;
                     if (stage)
-                        scatter_tile<P, R, true>(nd, b, t, e.cols, e.C, &e.ctl[CTL_STATUS], gsm, e.nodes, fz, &e.rp, b.totals + nd.bins);
+                        scatter_tile<P, R, true>(nd, b, t, e.cols, e.C, &e.ctl[CTL_STATUS], gsm, e.real_nodes, fz, &e.rp, b.totals + nd.bins);
                     else
-                        scatter_tile<P, R, false>(nd, b, t, e.cols, e.C, &e.ctl[CTL_STATUS], gsm, e.nodes, fz, &e.rp, b.totals + nd.bins);
+                        scatter_tile<P, R, false>(nd, b, t, e.cols, e.C, &e.ctl[CTL_STATUS], gsm, e.real_nodes, fz, &e.rp, b.totals + nd.bins);
                 } else {
                     transform_tile<P, R>(nd, b, t, e.cols, e.C, e.rp);
                 }
@@ -593,6 +593,7 @@ struct gf_stage {
     uint64_t first_id = 0;
     bool failed = false, launched = false;
     uint64_t launches = 0;
+    uint32_t bin_path = GF_BIN_PATH_AUTO, hash_entries = 0, stall_ms = 0;
     std::string err;
 
     ~gf_stage()
@@ -607,9 +608,15 @@ struct gf_stage {
         return is_partition(nd) ? 1 : 2;
     }
 
-    void init(gf_pool* p, const gf_node_desc& d, uint32_t k, uint64_t sd, uint32_t q, uint32_t ti)
+    void init(gf_pool* p, const gf_node_desc& d, uint32_t k, uint64_t sd, uint32_t q, uint32_t ti,
+              const gf_tuning* t = nullptr)
     {
         pool = p;
+        if (t) {
+            bin_path = t->bin_path;
+            hash_entries = t->hash_table_entries;
+            stall_ms = t->stall_timeout_ms;
+        }
         nd = d;
         key = k;
         seed = sd;
@@ -649,6 +656,25 @@ struct gf_stage {
             HIPX(hipMemcpy(d_zipf, t.data(), t.size() * sizeof(double), hipMemcpyHostToDevice));
             rp.zipf = d_zipf;
             rp.zipf_values = nd.u[0];
+        }
+    }
+
+    // Wait for the launched device stage. With a stall timeout (Tuning::stallTimeout,
+    // GraphRunner.hpp:1742) a stage that has not completed in time fails the run with GF_STALL.
+    void wait_stage()
+    {
+        if (!stall_ms) {
+            HIPX(hipEventSynchronize(ev));
+            return;
+        }
+        const auto t0 = std::chrono::steady_clock::now();
+        for (;;) {
+            const hipError_t q = hipEventQuery(ev);
+            if (q == hipSuccess) return;
+            if (q != hipErrorNotReady) HIPX(q);
+            if (std::chrono::steady_clock::now() - t0 > std::chrono::milliseconds(stall_ms))
+                fail(GF_STALL, "graphFlow: scheduler stall, a device stage did not complete within the stall timeout");
+            std::this_thread::yield();
         }
     }
 
@@ -791,7 +817,7 @@ struct gf_stage {
     {
         throw_if_failed();
         allocated = 0;
-        HIPX(hipEventSynchronize(ev));
+        wait_stage();
         const int kd = kind();
         if (kd == 0) {
             check_status("source");
@@ -1086,12 +1112,17 @@ void run_host(gf_flow& f, uint32_t src, uint64_t N, const RunShape& sh, gf_stats
 {
     gf_pool& pool = *f.pool;
     const uint32_t n = (uint32_t)f.nodes.size(), C = sh.C, P = sh.P, Q = sh.Q;
-    std::vector<std::unique_ptr<gf_stage>> stages(n);
-    for (uint32_t k = 0; k < n; ++k) {
-        stages[k] = std::make_unique<gf_stage>();
-        stages[k]->init(&pool, f.nodes[k].nd, k, f.tuning.seed, Q, sh.tile_items);
-        stages[k]->rp.ph = photon_params_of(f.tuning);
-    }
+    // lanes (Tuning::inflightBatches, GraphRunner.hpp:766-771): each lane owns its stage, batch,
+    // leases and output credit; lane j of node k is lanes[k * D + j]
+    const uint32_t D = f.tuning.inflight_batches ? f.tuning.inflight_batches : 1u;
+    std::vector<std::unique_ptr<gf_stage>> stages((size_t)n * D);
+    for (uint32_t k = 0; k < n; ++k)
+        for (uint32_t j = 0; j < D; ++j) {
+            auto& sg = stages[(size_t)k * D + j];
+            sg = std::make_unique<gf_stage>();
+            sg->init(&pool, f.nodes[k].nd, k, f.tuning.seed, Q, sh.tile_items, &f.tuning);
+            sg->rp.ph = photon_params_of(f.tuning);
+        }
     hipStream_t s = f.ctx->stream;
     struct Lane {
         bool armed = false;
@@ -1099,7 +1130,12 @@ void run_host(gf_flow& f, uint32_t src, uint64_t N, const RunShape& sh, gf_stats
         uint32_t reserved = 0;
         uint64_t particles = 0;
     };
-    std::vector<Lane> lanes(n);
+    std::vector<Lane> lanes((size_t)n * D);
+    auto free_lane = [&](uint32_t k) -> int {
+        for (uint32_t j = 0; j < D; ++j)
+            if (!lanes[(size_t)k * D + j].armed) return (int)j;
+        return -1;
+    };
     std::vector<std::deque<gf_chunk_desc>> ready(n);
     uint64_t generated = 0, sunk = 0, iter = 0, batches = 0;
     uint32_t reserved = 0, rr = 0;
@@ -1108,22 +1144,18 @@ void run_host(gf_flow& f, uint32_t src, uint64_t N, const RunShape& sh, gf_stats
     f.part_items = f.xf_items = 0;
     auto teardown = [&]() {
         (void)hipStreamSynchronize(s);
+        for (size_t i = 0; i < lanes.size(); ++i) {
+            stages[i]->abort();
+            if (lanes[i].armed)
+                for (auto& d : lanes[i].batch) {
+                    try {
+                        pool.release(gf_chunk_handle{d.index, d.generation});
+                    } catch (...) {
+                    }
+                }
+            lanes[i].armed = false;
+        }
         for (uint32_t k = 0; k < n; ++k) {
-            stages[k]->abort();
-            if (lanes[k].armed && f.nodes[k].nd.role != GF_ROLE_SOURCE)
-                for (auto& d : lanes[k].batch) {
-                    try {
-                        pool.release(gf_chunk_handle{d.index, d.generation});
-                    } catch (...) {
-                    }
-                }
-            if (lanes[k].armed && f.nodes[k].nd.role == GF_ROLE_SOURCE)
-                for (auto& d : lanes[k].batch) {
-                    try {
-                        pool.release(gf_chunk_handle{d.index, d.generation});
-                    } catch (...) {
-                    }
-                }
             for (auto& d : ready[k]) {
                 try {
                     pool.release(gf_chunk_handle{d.index, d.generation});
@@ -1131,70 +1163,71 @@ void run_host(gf_flow& f, uint32_t src, uint64_t N, const RunShape& sh, gf_stats
                 }
             }
             ready[k].clear();
-            lanes[k].armed = false;
         }
     };
     try {
         for (;;) {
             bool progress = false;
-            // 1) retire
-            for (uint32_t k = 0; k < n; ++k) {
-                Lane& L = lanes[k];
-                if (!L.armed) continue;
-                gf_stage& stg = *stages[k];
-                const gf_node_desc& nd = f.nodes[k].nd;
-                uint32_t allocated = 0;
-                bool more = stg.advance(s, allocated);
-                if (stg.kind() == 1 && more)
-                    trace_host(f.trace, tr, k, iter, L.batch, nd.bins, stg.totals.data(), stg.pending);
-                if (allocated) {
-                    if (allocated > L.reserved) fail(GF_BAD_ARGUMENT, "graphFlow: node acquired more chunks than reserved");
-                    reserved -= L.reserved;
-                    L.reserved = 0;
-                } else if (!more && L.reserved) {
-                    reserved -= L.reserved;
-                    L.reserved = 0;
-                }
-                if (!more) {
-                    std::vector<gf_chunk_desc> outs;
-                    stg.finish(outs);
-                    if (nd.role != GF_ROLE_SINK)
-                        for (auto& d : outs) {
-                            if (d.bin >= f.nodes[k].routes.size()) fail(GF_BAD_ARGUMENT, "graphFlow: node produced an unroutable bin");
-                            ready[(uint32_t)f.nodes[k].routes[d.bin]].push_back(d);
-                        }
-                    if (nd.role == GF_ROLE_SINK) sunk += L.particles;
-                    if (nd.role != GF_ROLE_SOURCE)
-                        for (auto& d : L.batch) pool.release(gf_chunk_handle{d.index, d.generation});
-                    if (nd.role == GF_ROLE_TRANSFORM) {
-                        ++batches;
-                        if (is_partition(nd)) f.part_items += L.particles;
-                        else f.xf_items += L.particles;
+            // 1) retire, every lane of every node
+            for (uint32_t k = 0; k < n; ++k)
+                for (uint32_t j = 0; j < D; ++j) {
+                    Lane& L = lanes[(size_t)k * D + j];
+                    if (!L.armed) continue;
+                    gf_stage& stg = *stages[(size_t)k * D + j];
+                    const gf_node_desc& nd = f.nodes[k].nd;
+                    uint32_t allocated = 0;
+                    bool more = stg.advance(s, allocated);
+                    if (stg.kind() == 1 && more)
+                        trace_host(f.trace, tr, k, iter, L.batch, nd.bins, stg.totals.data(), stg.pending);
+                    if (allocated) {
+                        if (allocated > L.reserved) fail(GF_BAD_ARGUMENT, "graphFlow: node acquired more chunks than reserved");
+                        reserved -= L.reserved;
+                        L.reserved = 0;
+                    } else if (!more && L.reserved) {
+                        reserved -= L.reserved;
+                        L.reserved = 0;
                     }
-                    L.armed = false;
+                    if (!more) {
+                        std::vector<gf_chunk_desc> outs;
+                        stg.finish(outs);
+                        if (nd.role != GF_ROLE_SINK)
+                            for (auto& d : outs) {
+                                if (d.bin >= f.nodes[k].routes.size()) fail(GF_BAD_ARGUMENT, "graphFlow: node produced an unroutable bin");
+                                ready[(uint32_t)f.nodes[k].routes[d.bin]].push_back(d);
+                            }
+                        if (nd.role == GF_ROLE_SINK) sunk += L.particles;
+                        if (nd.role != GF_ROLE_SOURCE)
+                            for (auto& d : L.batch) pool.release(gf_chunk_handle{d.index, d.generation});
+                        if (nd.role == GF_ROLE_TRANSFORM) {
+                            ++batches;
+                            if (is_partition(nd)) f.part_items += L.particles;
+                            else f.xf_items += L.particles;
+                        }
+                        L.armed = false;
+                    }
+                    progress = true;
                 }
-                progress = true;
-            }
-            // 2) admission
+            // 2) admission into the source's first free lane
             {
                 uint64_t live = generated - sunk, remaining = N - generated;
                 uint64_t min_fill = std::min<uint64_t>(std::min<uint64_t>(C, P), remaining);
                 bool within = live < P, budget = live < P && (P - live) >= min_fill;
-                if (generated < N && !lanes[src].armed && within && budget && pool.free_list.size() > reserved) {
+                const int j = free_lane(src);
+                if (generated < N && j >= 0 && within && budget && pool.free_list.size() > reserved) {
                     uint64_t fill = std::min<uint64_t>(std::min<uint64_t>(remaining, P - live), C);
                     gf_chunk_handle h = pool.acquire();
                     gf_chunk_desc d{h.index, h.generation, (uint32_t)fill, 0u};
-                    Lane& L = lanes[src];
+                    Lane& L = lanes[(size_t)src * D + (uint32_t)j];
                     L.batch = {d};
                     L.particles = fill;
                     L.armed = true;
-                    stages[src]->launch(&d, 1, generated, s);
+                    stages[(size_t)src * D + (uint32_t)j]->launch(&d, 1, generated, s);
                     trace_host(f.trace, tr, src, iter, {}, 0, nullptr, L.batch);
                     generated += fill;
                     progress = true;
                 }
             }
-            // 3) dispatch one batch: sinks first, then round robin
+            // 3) dispatch one batch into a free lane: sinks first, then round robin
             {
                 bool dispatched = false;
                 for (int pass = 0; pass < 2 && !dispatched; ++pass)
@@ -1202,8 +1235,10 @@ void run_host(gf_flow& f, uint32_t src, uint64_t N, const RunShape& sh, gf_stats
                         uint32_t k = pass == 0 ? kk : (rr + kk) % n;
                         const gf_node_desc& nd = f.nodes[k].nd;
                         if (pass == 0 && nd.role != GF_ROLE_SINK) continue;
-                        Lane& L = lanes[k];
-                        if (ready[k].empty() || L.armed) continue;
+                        if (ready[k].empty()) continue;
+                        const int j = free_lane(k);
+                        if (j < 0) continue;
+                        Lane& L = lanes[(size_t)k * D + (uint32_t)j];
                         size_t wanted = nd.role == GF_ROLE_TRANSFORM ? std::min<size_t>(Q, ready[k].size()) : 1u;
                         uint32_t reserve = 0;
                         if (nd.role == GF_ROLE_TRANSFORM) {
@@ -1227,11 +1262,12 @@ void run_host(gf_flow& f, uint32_t src, uint64_t N, const RunShape& sh, gf_stats
                         L.particles = 0;
                         for (auto& d : L.batch) L.particles += d.fill;
                         L.armed = true;
-                        stages[k]->launch(L.batch.data(), (uint32_t)L.batch.size(), 0, s);
+                        gf_stage& stg = *stages[(size_t)k * D + (uint32_t)j];
+                        stg.launch(L.batch.data(), (uint32_t)L.batch.size(), 0, s);
                         if (nd.role == GF_ROLE_SINK) trace_host(f.trace, tr, k, iter, L.batch, 0, nullptr, {});
-                        if (stages[k]->kind() == 2) {
+                        if (stg.kind() == 2) {
                             uint32_t tot = (uint32_t)L.particles;
-                            trace_host(f.trace, tr, k, iter, L.batch, 1, &tot, stages[k]->pending);
+                            trace_host(f.trace, tr, k, iter, L.batch, 1, &tot, stg.pending);
                         }
                         dispatched = true;
                     }
@@ -1240,7 +1276,9 @@ void run_host(gf_flow& f, uint32_t src, uint64_t N, const RunShape& sh, gf_stats
             }
             bool drained = true;
             for (uint32_t k = 0; k < n; ++k)
-                if (lanes[k].armed || !ready[k].empty()) drained = false;
+                if (!ready[k].empty()) drained = false;
+            for (auto& L : lanes)
+                if (L.armed) drained = false;
             ++iter;
             if (generated >= N && drained && reserved == 0) break;
             if (!progress) fail(GF_STALL, "graphFlow: scheduler stall, no progress possible");
@@ -1312,9 +1350,21 @@ void build_engine(gf_flow& f, uint64_t N, const RunShape& sh, uint32_t src)
     DevMem& m = eb->mem;
     Engine& e = eb->e;
     const uint32_t n = (uint32_t)f.nodes.size();
-    if (n > kMaxNodes) fail(GF_BAD_ARGUMENT, "graphFlow: the device engine supports at most 64 nodes");
+    const gf_tuning& t = f.tuning;
+    // lanes: D per source and transform node, one per sink (Tuning::inflightBatches)
+    const uint32_t D = t.inflight_batches ? t.inflight_batches : 1u;
+    std::vector<uint32_t> first_lane(n), lanes_of(n);
+    uint32_t nv = 0;
+    for (uint32_t k = 0; k < n; ++k) {
+        first_lane[k] = nv;
+        lanes_of[k] = f.nodes[k].nd.role == GF_ROLE_SINK ? 1u : D;
+        nv += lanes_of[k];
+    }
+    if (nv > kMaxNodes)
+        fail(GF_BAD_ARGUMENT, "graphFlow: the device engine supports at most " + std::to_string(kMaxNodes) +
+                                  " lanes (nodes times inflight batches)");
     gf_pool& pool = *f.pool;
-    e.n_nodes = n;
+    e.n_nodes = nv;
     e.C = sh.C;
     e.P = sh.P;
     e.Q = sh.Q;
@@ -1322,61 +1372,83 @@ void build_engine(gf_flow& f, uint64_t N, const RunShape& sh, uint32_t src)
     e.ring_cap = sh.S;
     e.N = N;
     e.cols = pool.dcols();
-    e.rp.seed = f.tuning.seed;
+    e.rp.seed = t.seed;
     e.rp.C = sh.C;
-    const gf_tuning& t = f.tuning;
     e.rp.ph = photon_params_of(t);
-    std::vector<gf_node_desc> nds(n);
+    std::vector<gf_node_desc> real(n), nds(nv);
     std::vector<int32_t> routes, fuse;
+    std::vector<uint32_t> ring_of(nv);
     const uint32_t r = effective_r(f.payload, pick_tile_r(sh.tile_items));
     const uint32_t rec = payload_bytes(f.payload);
     // KA and KB hold only their tiles' LDS (the plan, retire and select run in KP and KR)
     size_t lds_a = 256, lds_b = 256, group_lds = 256;
-    uint32_t stage_mask = 0, grid_tiles = 0;
-    std::vector<NodeArr> arr(n);
+    uint32_t grid_tiles = 0;
+    std::vector<NodeArr> arr(nv);
     auto is_plain_xf = [&](int32_t k) {
         return k >= 0 && f.nodes[k].nd.role == GF_ROLE_TRANSFORM && !is_partition(f.nodes[k].nd) && f.nodes[k].nd.bins == 1u;
     };
     for (uint32_t k = 0; k < n; ++k) {
-        nds[k] = f.nodes[k].nd;
-        nds[k].route_offset = (uint32_t)routes.size();
-        const gf_node_desc& nd = nds[k];
-        const bool part = nd.role == GF_ROLE_TRANSFORM && is_partition(nd);
+        real[k] = f.nodes[k].nd;
+        real[k].route_offset = (uint32_t)routes.size();
+        const bool part = real[k].role == GF_ROLE_TRANSFORM && is_partition(real[k]);
         for (int32_t dst : f.nodes[k].routes) {
-            // a plain transform fed by a partition bin runs inside that partition's scatter
+            // a plain transform fed by a partition bin runs inside that partition's scatter; routes
+            // name the destination's first lane, which owns the destination's ready ring
             if (part && t.fuse_transforms && is_plain_xf(dst)) {
                 fuse.push_back(dst);
-                routes.push_back(f.nodes[dst].routes[0]);
+                routes.push_back((int32_t)first_lane[f.nodes[dst].routes[0]]);
             } else {
                 fuse.push_back(-1);
-                routes.push_back(dst);
+                routes.push_back((int32_t)first_lane[dst]);
             }
         }
+    }
+    // sinks fed by a partition of 64 or more bins take many small chunks per step: wave tiles
+    std::vector<bool> wave_sink(n, false);
+    for (uint32_t k = 0; k < n; ++k) {
+        if (!(real[k].role == GF_ROLE_TRANSFORM && is_partition(real[k])) || real[k].bins < 64u) continue;
+        for (int32_t dst : f.nodes[k].routes)
+            if (dst >= 0 && f.nodes[dst].nd.role == GF_ROLE_SINK) wave_sink[dst] = true;
+    }
+    uint32_t rings = 0;
+    for (uint32_t k = 0; k < n; ++k) {
+        const gf_node_desc& nd = real[k];
+        const bool part = nd.role == GF_ROLE_TRANSFORM && is_partition(nd);
         const uint32_t B = std::max<uint32_t>(nd.bins, 1u);
-        NodeArr& a = arr[k];
-        std::memset(&a, 0, sizeof(a));
-        a.kind = nd.role == GF_ROLE_SOURCE ? KIND_SOURCE : nd.role == GF_ROLE_SINK ? KIND_SINK : part ? KIND_PARTITION : KIND_TRANSFORM;
-        a.cap_in = nd.role == GF_ROLE_SOURCE ? 1u : nd.role == GF_ROLE_SINK ? sh.S : sh.Q;
-        a.cap_out = nd.role == GF_ROLE_SOURCE ? 1u : nd.role == GF_ROLE_SINK ? 1u : reserve_bound_of(nd, sh.C, sh.Q);
-        a.reserve_bound = a.cap_out;
-        uint32_t cap_tiles = (uint32_t)(ceil_div(sh.C, sh.tile_items) * (nd.role == GF_ROLE_SINK ? 1u : a.cap_in));
-        a.in = m.d<gf_chunk_desc>(a.cap_in);
-        a.tile_prefix = m.d<uint32_t>(a.cap_in + 1u);
-        a.item_prefix = m.d<uint32_t>(a.cap_in + 1u);
-        a.out = m.d<gf_chunk_desc>(a.cap_out);
-        a.counts = part ? m.d<uint32_t>((size_t)cap_tiles * B) : nullptr;
-        a.totals = m.d<uint32_t>(2u * B);
-        a.tile_items = sh.tile_items;
-        a.purpose = 2u * k;
-        if (part) {
-            lds_a = std::max(lds_a, (size_t)B * 4u);
-            bool stg = stage_fits(B, sh.tile_items, rec);
-            if (stg && k < 32) stage_mask |= 1u << k;
-            lds_b = std::max(lds_b, scatter_smem(B, sh.tile_items, rec, stg && k < 32));
-            group_lds = std::max(group_lds, std::max((size_t)B * 4u, scatter_smem(B, sh.tile_items, rec, stg && k < 32)));
-            grid_tiles += cap_tiles;
-        } else if (nd.role != GF_ROLE_SINK) {
-            grid_tiles += cap_tiles;
+        const uint32_t ring = rings++;
+        for (uint32_t j = 0; j < lanes_of[k]; ++j) {
+            const uint32_t v = first_lane[k] + j;
+            nds[v] = nd;
+            ring_of[v] = first_lane[k];
+            NodeArr& a = arr[v];
+            std::memset(&a, 0, sizeof(a));
+            a.kind = nd.role == GF_ROLE_SOURCE ? KIND_SOURCE : nd.role == GF_ROLE_SINK ? KIND_SINK : part ? KIND_PARTITION : KIND_TRANSFORM;
+            a.cap_in = nd.role == GF_ROLE_SOURCE ? 1u : nd.role == GF_ROLE_SINK ? sh.S : sh.Q;
+            a.cap_out = nd.role == GF_ROLE_SOURCE ? 1u : nd.role == GF_ROLE_SINK ? 1u : reserve_bound_of(nd, sh.C, sh.Q);
+            a.reserve_bound = a.cap_out;
+            a.real = k;
+            a.lanes = j == 0 ? lanes_of[k] : 0u;
+            a.ring = ring;
+            uint32_t cap_tiles = (uint32_t)(ceil_div(sh.C, sh.tile_items) * (nd.role == GF_ROLE_SINK ? 1u : a.cap_in));
+            a.in = m.d<gf_chunk_desc>(a.cap_in);
+            a.tile_prefix = m.d<uint32_t>(a.cap_in + 1u);
+            a.item_prefix = m.d<uint32_t>(a.cap_in + 1u);
+            a.out = m.d<gf_chunk_desc>(a.cap_out);
+            a.counts = part ? m.d<uint32_t>((size_t)cap_tiles * B) : nullptr;
+            a.totals = m.d<uint32_t>(2u * B);
+            a.tile_items = sh.tile_items;
+            a.purpose = 2u * k;
+            if (wave_sink[k]) a.flags |= NODE_WAVE_SINK;
+            if (part) {
+                lds_a = std::max(lds_a, (size_t)B * 4u);
+                bool stg = stage_fits(B, sh.tile_items, rec);
+                if (stg) a.flags |= NODE_STAGE;
+                lds_b = std::max(lds_b, scatter_smem(B, sh.tile_items, rec, stg));
+                group_lds = std::max(group_lds, std::max((size_t)B * 4u, scatter_smem(B, sh.tile_items, rec, stg)));
+                grid_tiles += cap_tiles;
+            } else if (nd.role != GF_ROLE_SINK) {
+                grid_tiles += cap_tiles;
+            }
         }
         if (nd.op == GF_SRC_KEYED_ZIPF) {
             auto z = zipf_table(nd.u[0], (double)nd.f[0]);
@@ -1388,25 +1460,31 @@ void build_engine(gf_flow& f, uint64_t N, const RunShape& sh, uint32_t src)
     }
     if (lds_b > 160u * 1024u || lds_a > 160u * 1024u)
         fail(GF_BAD_ARGUMENT, "graphFlow: partition width exceeds the workgroup LDS budget");
-    gf_node_desc* dn = m.d<gf_node_desc>(n);
-    copy_sync(dn, nds.data(), sizeof(gf_node_desc) * n, hipMemcpyHostToDevice, f.ctx->stream);
+    gf_node_desc* dn = m.d<gf_node_desc>(nv);
+    copy_sync(dn, nds.data(), sizeof(gf_node_desc) * nv, hipMemcpyHostToDevice, f.ctx->stream);
+    gf_node_desc* drn = m.d<gf_node_desc>(n);
+    copy_sync(drn, real.data(), sizeof(gf_node_desc) * n, hipMemcpyHostToDevice, f.ctx->stream);
     int32_t* dr = m.d<int32_t>(std::max<size_t>(routes.size(), 1));
     int32_t* df = m.d<int32_t>(std::max<size_t>(fuse.size(), 1));
     if (!routes.empty()) {
         copy_sync(dr, routes.data(), sizeof(int32_t) * routes.size(), hipMemcpyHostToDevice, f.ctx->stream);
         copy_sync(df, fuse.data(), sizeof(int32_t) * fuse.size(), hipMemcpyHostToDevice, f.ctx->stream);
     }
-    NodeArr* da = m.d<NodeArr>(n);
-    copy_sync(da, arr.data(), sizeof(NodeArr) * n, hipMemcpyHostToDevice, f.ctx->stream);
+    uint32_t* dring = m.d<uint32_t>(nv);
+    copy_sync(dring, ring_of.data(), sizeof(uint32_t) * nv, hipMemcpyHostToDevice, f.ctx->stream);
+    NodeArr* da = m.d<NodeArr>(nv);
+    copy_sync(da, arr.data(), sizeof(NodeArr) * nv, hipMemcpyHostToDevice, f.ctx->stream);
     e.nodes = dn;
+    e.real_nodes = drn;
     e.routes = dr;
     e.fuse = df;
+    e.ring_of = dring;
     e.arr = da;
-    e.st = m.d<NodeSt>(n);
+    e.st = m.d<NodeSt>(nv);
     e.free_stack = m.d<unsigned long long>(sh.S);
-    e.rings = m.d<gf_chunk_desc>((size_t)n * sh.S);
-    e.ka_prefix = m.d<uint32_t>(n + 1u);
-    e.kb_prefix = m.d<uint32_t>(n + 1u);
+    e.rings = m.d<gf_chunk_desc>((size_t)rings * sh.S);
+    e.ka_prefix = m.d<uint32_t>(nv + 1u);
+    e.kb_prefix = m.d<uint32_t>(nv + 1u);
     e.ctl = m.d<uint32_t>(CTL_WORDS);
     e.ctr = m.d<unsigned long long>(CNT_WORDS);
     e.trace_on = t.trace ? 1u : 0u;
@@ -1416,17 +1494,6 @@ void build_engine(gf_flow& f, uint64_t N, const RunShape& sh, uint32_t src)
     const uint32_t gmax = t.max_workgroups ? t.max_workgroups : 64u;
     eb->grid = std::min<uint32_t>(std::max<uint32_t>(grid_tiles, 1u), gmax);
     eb->r = r;
-    eb->stage_mask = stage_mask;
-    // sinks fed by a partition of 64 or more bins take many small chunks per step: wave tiles
-    uint32_t wave_sinks = 0;
-    for (uint32_t k = 0; k < n; ++k) {
-        if (arr[k].kind != KIND_PARTITION || nds[k].bins < 64u) continue;
-        for (uint32_t b = 0; b < nds[k].bins; ++b) {
-            const int32_t dst = routes[nds[k].route_offset + b];
-            if (dst >= 0 && dst < 32 && arr[dst].kind == KIND_SINK) wave_sinks |= 1u << dst;
-        }
-    }
-    eb->wave_sinks = wave_sinks;
     eb->lds_a = lds_a;
     eb->lds_b = lds_b;
     eb->key = shape_key(f, sh, N);
@@ -1437,11 +1504,12 @@ void build_engine(gf_flow& f, uint64_t N, const RunShape& sh, uint32_t src)
     eb->micro_lds = std::max(std::max(sizeof(PlanLds), sizeof(SelLds)), 4u * group_lds);
     const bool micro_fits = eb->micro_lds <= 160u * 1024u;
     if (t.device_engine == 2u && !micro_fits) fail(GF_BAD_ARGUMENT, "graphFlow: micro engine LDS budget exceeded");
-    // team engine: every partition narrow enough for a per-segment count row in LDS
+    // team engine: every partition narrow enough for a per-segment count row in LDS; its replicated
+    // scheduler runs one lane per node
     uint32_t bb = 1;
-    bool team_ok = n <= kTeamNodes && routes.size() <= kTeamRoutes && ceil_div(sh.C, kSeg) <= (1u << 17);
+    bool team_ok = D == 1u && n <= kTeamNodes && routes.size() <= kTeamRoutes && ceil_div(sh.C, kSeg) <= (1u << 17);
     for (uint32_t k = 0; k < n; ++k)
-        if (arr[k].kind == KIND_PARTITION) bb = std::max(bb, nds[k].bins + 1u);
+        if (is_partition(real[k]) && real[k].role == GF_ROLE_TRANSFORM) bb = std::max(bb, real[k].bins + 1u);
     if (bb > kTeamBins) team_ok = false;
     bb = (bb + 3u) & ~3u;  // rows of whole 16-byte words: one 128-bit load per four slots
     const uint32_t tpc = (uint32_t)ceil_div(sh.C, kSeg);
@@ -1506,8 +1574,7 @@ void launch_step(const EngineBuffers& eb, uint32_t payload, hipStream_t s, hipEv
         if (ev) HIPX(hipEventRecord(ev[1], s));
         hipLaunchKernelGGL(k_engine_plan, dim3(1), dim3(1024), sizeof(PlanLds), s, eb.e);
         if (ev) HIPX(hipEventRecord(ev[2], s));
-        hipLaunchKernelGGL((k_engine_kb<P, RR>), dim3(eb.grid), dim3(kThreads), eb.lds_b, s, eb.e, eb.stage_mask,
-                           eb.wave_sinks);
+        hipLaunchKernelGGL((k_engine_kb<P, RR>), dim3(eb.grid), dim3(kThreads), eb.lds_b, s, eb.e);
         hipLaunchKernelGGL(k_engine_kr, dim3(1), dim3(1024), sizeof(SelLds), s, eb.e);
         if (ev) HIPX(hipEventRecord(ev[3], s));
     });
@@ -1535,8 +1602,7 @@ void run_micro(gf_flow& f, EngineBuffers& eb, uint64_t N, gf_stats& st, bool tim
         with_payload(f.payload, eb.r, [&](auto pt, auto rc) {
             using P = typename decltype(pt)::type;
             constexpr int RR = decltype(rc)::value;
-            hipLaunchKernelGGL((k_micro<P, RR>), dim3(1), dim3(1024), eb.micro_lds, s, eb.e, eb.stage_mask, M,
-                               eb.group_lds);
+            hipLaunchKernelGGL((k_micro<P, RR>), dim3(1), dim3(1024), eb.micro_lds, s, eb.e, M, eb.group_lds);
         });
         HIPX(hipGetLastError());
         if (timed) HIPX(hipEventRecord(ev[1], s));
@@ -2071,6 +2137,10 @@ gf_status gf_flow_configure(gf_flow* flow, const gf_tuning* t)
     if (!flow || !t) return GF_BAD_ARGUMENT;
     return guard(&flow->err, [&] {
         if (t->schedule > GF_SCHEDULE_DEVICE) fail(GF_BAD_ARGUMENT, "graphFlow::Tuning: unknown schedule");
+        if (t->bin_path > GF_BIN_PATH_HASH) fail(GF_BAD_ARGUMENT, "graphFlow::Tuning: unknown bin path");
+        if (t->inflight_batches > kMaxNodes) fail(GF_BAD_ARGUMENT, "graphFlow::Tuning: inflightBatches exceeds the engine's lanes");
+        if (t->inflight_batches > 1u && t->schedule == GF_SCHEDULE_DEVICE && t->device_engine == 3u)
+            fail(GF_BAD_ARGUMENT, "graphFlow::Tuning: the team engine runs one batch per node and step (inflightBatches 1)");
         flow->tuning = *t;
     });
 }

@@ -118,8 +118,18 @@ typedef struct gf_tuning {
     double photon_layer_mfp, photon_mu_t, photon_albedo, photon_hg_g, photon_roulette_w, photon_roulette_p;
     uint32_t team_xcd;       /* team engine: XCD the team runs on, plus one (0 -> the XCD of the first
                                 workgroup to start); flows that run side by side use different XCDs */
-    uint32_t reserved0;
+    uint32_t inflight_batches; /* Tuning::inflightBatches: batches in flight per source and transform node
+                                  (0 -> 1); plain schedule: lanes of the host runner; device schedule: a
+                                  node runs up to this many batches per step (wide and micro engines) */
+    uint32_t bin_path;       /* Tuning::binPath: GF_BIN_PATH_* (hash above 1024 bins when automatic)      */
+    uint32_t hash_table_entries; /* hash path: slots of a tile's bin table (0 -> nextPow2(2 * tile items), which
+                                    never fills); a tile with more distinct bins fails its batch (GF_INVALID) */
+    uint32_t stall_timeout_ms; /* Tuning::stallTimeout: plain schedule, longest wait for a device stage
+                                  before the run fails with GF_STALL (0 -> 120000)                         */
+    uint32_t reserved1;
 } gf_tuning;
+
+enum { GF_BIN_PATH_AUTO = 0, GF_BIN_PATH_DENSE = 1, GF_BIN_PATH_HASH = 2 };
 
 typedef struct gf_stats {
     uint64_t completed_batches, steps, generated, sunk;

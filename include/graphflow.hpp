@@ -14,6 +14,7 @@
 #pragma once
 
 #include <cstdint>
+#include <chrono>
 #include <optional>
 #include <stdexcept>
 #include <string>
@@ -154,10 +155,16 @@ inline Op PhotonDone() { return op(GF_ROLE_SINK, GF_SNK_PHOTON_TALLY, 0); }
 
 // ---- tuning and stats (flow/Tuning.hpp:27-145, sched/GraphRunner.hpp:154-210) --------------------
 enum class Schedule : std::uint32_t { plain = GF_SCHEDULE_HOST, device = GF_SCHEDULE_DEVICE };
+// BinPath (core/BinPath.hpp): dense LDS counters or per-tile hash tables; automatic = hash above 1024 bins
+enum class BinPath : std::uint32_t { automatic = GF_BIN_PATH_AUTO, dense = GF_BIN_PATH_DENSE, hash = GF_BIN_PATH_HASH };
 
 struct Tuning {
     std::uint64_t seed = 0;
     std::optional<std::uint32_t> batchChunks, chunkCapacity, liveParticles, numChunks, tileSize, stepsPerLaunch;
+    std::optional<std::uint32_t> inflightBatches;           // Tuning::inflightBatches
+    BinPath binPath = BinPath::automatic;                   // Tuning::binPath
+    std::optional<std::uint32_t> hashTableEntries;          // hash path: slots of a tile's bin table
+    std::optional<std::chrono::milliseconds> stallTimeout;  // Tuning::stallTimeout
     Schedule schedule = Schedule::device;
     bool trace = false;
     bool fuseTransforms = false;
@@ -181,6 +188,10 @@ struct Tuning {
         t.max_workgroups = teamSize.value_or(0);
         t.device_engine = deviceEngine;
         t.team_xcd = teamXcd ? *teamXcd + 1u : 0u;
+        t.inflight_batches = inflightBatches.value_or(0);
+        t.bin_path = (std::uint32_t)binPath;
+        t.hash_table_entries = hashTableEntries.value_or(0);
+        t.stall_timeout_ms = stallTimeout ? (std::uint32_t)stallTimeout->count() : 0u;
         t.photon_layers = 10;
         t.photon_tally_bins = 1024;
         t.photon_layer_mfp = 1.0;

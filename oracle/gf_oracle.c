@@ -503,6 +503,7 @@ typedef struct batch {
     or_item* items;  /* gathered items in batch order */
     uint32_t* bins;  /* classified bin per item */
     uint64_t particles;
+    uint64_t first_id; /* source lanes: id of the chunk's first item */
     uint32_t reserved;
     int stage;       /* runner: 0 launched, 1 planned */
     int armed;
@@ -745,6 +746,37 @@ static uint32_t reserve_bound_of(const or_node_desc* nd, uint32_t C, uint32_t Q)
 
 static int is_partition(const or_node_desc* nd) { return nd->op >= OR_CLS_WEIGHT_ROUTE && nd->op < OR_XF_SCALE_WEIGHT; }
 
+/* Bin path of a partition (BinPath.hpp): hash above 1024 bins unless the run fixes one. */
+static int uses_hash_path(const or_run_cfg* cfg, const or_node_desc* nd)
+{
+    return cfg->bin_path == 2u || (cfg->bin_path == 0u && nd->bins > 1024u);
+}
+
+/* Hash-path classify check (Execution.hpp:1207-1307): every tile of hash_tile items of every input
+ * chunk inserts its bins into a table of hash_entries slots (BinTable.hpp:101-121); a tile with
+ * more distinct bins than slots fails the batch before any output is planned. With hash_entries 0
+ * the table has nextPow2(2 * tile) slots and can never fill. */
+static int hash_tables_fit(const or_run_cfg* cfg, const or_node_desc* nd, const batch* b)
+{
+    const uint32_t T = cfg->hash_tile, E = cfg->hash_entries;
+    if (!uses_hash_path(cfg, nd) || E == 0u || T == 0u) return 1;
+    uint32_t* keys = (uint32_t*)malloc(sizeof(uint32_t) * E);
+    uint32_t* cnt = (uint32_t*)malloc(sizeof(uint32_t) * E);
+    int ok = 1;
+    uint64_t item = 0;
+    for (uint32_t c = 0; c < b->n_in && ok; ++c) {
+        for (uint32_t off = 0; off < b->in[c].fill && ok; off += T) {
+            const uint32_t n = b->in[c].fill - off < T ? b->in[c].fill - off : T;
+            for (uint32_t e = 0; e < E; ++e) { keys[e] = 0xFFFFFFFFu; cnt[e] = 0u; }
+            for (uint32_t i = 0; i < n; ++i)
+                if (!or_hash_table_insert(keys, cnt, E, b->bins[item + off + i])) { ok = 0; break; }
+        }
+        item += b->in[c].fill;
+    }
+    free(keys); free(cnt);
+    return ok;
+}
+
 int or_run(const or_node_desc* nodes, uint32_t n_nodes, const int32_t* routes, const uint32_t* lut,
            const or_run_cfg* cfg, or_run_out* out)
 {
@@ -755,6 +787,8 @@ int or_run(const or_node_desc* nodes, uint32_t n_nodes, const int32_t* routes, c
     out->batches = out->steps = out->generated = out->sunk = 0;
     or_ctx x = {nodes, n_nodes, routes, lut, cfg, NULL, 0, out};
     uint32_t C = cfg->capacity, Q = cfg->batch ? cfg->batch : 1u, P = 0, S = 0;
+    /* Tuning::inflightBatches (Tuning.hpp:34-41): lanes per node (GraphRunner.hpp:766-771) */
+    const uint32_t D = cfg->inflight ? cfg->inflight : 1u;
     int rc = or_derive_pool(nodes, n_nodes, cfg->items, C, cfg->live, &P, &S);
     if (rc) { fail(out, rc, "pool derivation"); return rc; }
     if (cfg->num_chunks) S = cfg->num_chunks;
@@ -784,91 +818,96 @@ int or_run(const or_node_desc* nodes, uint32_t n_nodes, const int32_t* routes, c
     or_pool pool;
     if (pool_init(&pool, S, C)) { fail(out, OR_BAD_ARGUMENT, "pool allocation"); return OR_BAD_ARGUMENT; }
 
-    uint32_t maxb = 1;
-    for (uint32_t k = 0; k < n_nodes; ++k) if (nodes[k].bins > maxb) maxb = nodes[k].bins;
     fifo* ready = (fifo*)calloc(n_nodes, sizeof(fifo));
-    batch* lane = (batch*)calloc(n_nodes, sizeof(batch));
+    /* lane j of node k is lane[k * D + j]; every lane owns its batch, leases and credit */
+    batch* lane = (batch*)calloc((size_t)n_nodes * D, sizeof(batch));
     uint64_t item_cap = (uint64_t)Q * C;
-    for (uint32_t k = 0; k < n_nodes; ++k) {
-        uint32_t cap_in = nodes[k].role == OR_ROLE_SINK ? S : Q;
-        lane[k].in = (desc*)malloc(sizeof(desc) * (cap_in + 1u));
-        uint32_t cap_out = nodes[k].role == OR_ROLE_SOURCE ? 1u : reserve_bound_of(&nodes[k], C, Q) + 1u;
-        lane[k].out = (desc*)malloc(sizeof(desc) * cap_out);
-        lane[k].counts = (uint32_t*)malloc(sizeof(uint32_t) * (nodes[k].bins + 1u));
-        if (nodes[k].role == OR_ROLE_TRANSFORM) {
-            lane[k].items = (or_item*)malloc(sizeof(or_item) * item_cap);
-            lane[k].bins = (uint32_t*)malloc(sizeof(uint32_t) * item_cap);
+    for (uint32_t k = 0; k < n_nodes; ++k)
+        for (uint32_t j = 0; j < D; ++j) {
+            batch* b = &lane[k * D + j];
+            uint32_t cap_in = nodes[k].role == OR_ROLE_SINK ? S : Q;
+            b->in = (desc*)malloc(sizeof(desc) * (cap_in + 1u));
+            uint32_t cap_out = nodes[k].role == OR_ROLE_SOURCE ? 1u : reserve_bound_of(&nodes[k], C, Q) + 1u;
+            b->out = (desc*)malloc(sizeof(desc) * cap_out);
+            b->counts = (uint32_t*)malloc(sizeof(uint32_t) * (nodes[k].bins + 1u));
+            if (nodes[k].role == OR_ROLE_TRANSFORM) {
+                b->items = (or_item*)malloc(sizeof(or_item) * item_cap);
+                b->bins = (uint32_t*)malloc(sizeof(uint32_t) * item_cap);
+            }
         }
-    }
     uint64_t N = cfg->items, generated = 0, sunk = 0;
     uint32_t reserved = 0, rr = 0;
     uint64_t iter = 0;
-    uint64_t first_id_of_src = 0;
 
 #define FAIL_RUN(code, msg) do { fail(out, code, msg); goto done; } while (0)
 
     if (cfg->mode == OR_MODE_RUNNER) {
-        /* GraphRunner::run (GraphRunner.hpp:716-1257) with every launched batch complete by the
-         * next loop iteration: one legal interleaving of the reference's event polling. */
+        /* GraphRunner::run (GraphRunner.hpp:716-1257) with every launched device stage complete by
+         * the next loop iteration: one legal interleaving of the reference's event polling. A
+         * partition batch spans two iterations (classify, then plan + scatter), so with several
+         * lanes a node can dispatch its next batch while the previous one is still in flight. */
         for (;;) {
             int progress = 0;
-            /* 1) retire */
-            for (uint32_t k = 0; k < n_nodes; ++k) {
-                batch* b = &lane[k];
-                if (!b->armed) continue;
-                const or_node_desc* nd = &nodes[k];
-                uint32_t purpose = 2u * k;
-                int more = 0;
-                uint32_t allocated = 0;
-                if (nd->role == OR_ROLE_TRANSFORM && is_partition(nd)) {
-                    if (b->stage == 0) {
-                        int r2 = partition_plan_scatter(&x, nd, purpose + 1u, &pool, b, reserve_bound_of(nd, C, Q));
-                        if (r2 == OR_OVERFLOW) FAIL_RUN(OR_OVERFLOW, "partition scatter: checked updater overflow");
-                        if (r2 == OR_INVALID) FAIL_RUN(OR_INVALID, "partition scatter: invalid updater result");
-                        if (r2) FAIL_RUN(r2, "partition plan failed");
-                        trace_batch(out, k, iter, b, nd->bins);
-                        allocated = b->n_out;
-                        b->stage = 1;
-                        more = 1;
-                    }
-                } else if (nd->role == OR_ROLE_TRANSFORM) {
-                    allocated = b->n_out;
-                }
-                if (allocated) {
-                    if (allocated > b->reserved) FAIL_RUN(OR_BAD_ARGUMENT, "node acquired more chunks than reserved");
-                    reserved -= b->reserved;
-                    b->reserved = 0;
-                } else if (!more && b->reserved) {
-                    reserved -= b->reserved;
-                    b->reserved = 0;
-                }
-                if (!more) {
-                    /* finish: route outputs (GraphRunner.hpp:808-852) */
-                    if (nd->role != OR_ROLE_SINK)
-                        for (uint32_t o = 0; o < b->n_out; ++o) {
-                            int32_t d = routes[nd->route_offset + b->out[o].bin];
-                            fifo_push(&ready[d], b->out[o]);
+            /* 1) retire, every lane of every node */
+            for (uint32_t k = 0; k < n_nodes; ++k)
+                for (uint32_t j = 0; j < D; ++j) {
+                    batch* b = &lane[k * D + j];
+                    if (!b->armed) continue;
+                    const or_node_desc* nd = &nodes[k];
+                    uint32_t purpose = 2u * k;
+                    int more = 0;
+                    uint32_t allocated = 0;
+                    if (nd->role == OR_ROLE_TRANSFORM && is_partition(nd)) {
+                        if (b->stage == 0) {
+                            int r2 = partition_plan_scatter(&x, nd, purpose + 1u, &pool, b, reserve_bound_of(nd, C, Q));
+                            if (r2 == OR_OVERFLOW) FAIL_RUN(OR_OVERFLOW, "partition scatter: checked updater overflow");
+                            if (r2 == OR_INVALID) FAIL_RUN(OR_INVALID, "partition scatter: invalid updater result");
+                            if (r2) FAIL_RUN(r2, "partition plan failed");
+                            trace_batch(out, k, iter, b, nd->bins);
+                            allocated = b->n_out;
+                            b->stage = 1;
+                            more = 1;
                         }
-                    if (nd->role == OR_ROLE_SINK) sunk += b->particles;
-                    if (nd->role != OR_ROLE_SOURCE)
-                        for (uint32_t i = 0; i < b->n_in; ++i) pool_release(&pool, b->in[i].index);
-                    if (nd->role == OR_ROLE_TRANSFORM) out->batches++;
-                    b->armed = 0;
+                    } else if (nd->role == OR_ROLE_TRANSFORM) {
+                        allocated = b->n_out;
+                    }
+                    if (allocated) {
+                        if (allocated > b->reserved) FAIL_RUN(OR_BAD_ARGUMENT, "node acquired more chunks than reserved");
+                        reserved -= b->reserved;
+                        b->reserved = 0;
+                    } else if (!more && b->reserved) {
+                        reserved -= b->reserved;
+                        b->reserved = 0;
+                    }
+                    if (!more) {
+                        /* finish: route outputs (GraphRunner.hpp:808-852) */
+                        if (nd->role != OR_ROLE_SINK)
+                            for (uint32_t o = 0; o < b->n_out; ++o) {
+                                int32_t d = routes[nd->route_offset + b->out[o].bin];
+                                fifo_push(&ready[d], b->out[o]);
+                            }
+                        if (nd->role == OR_ROLE_SINK) sunk += b->particles;
+                        if (nd->role != OR_ROLE_SOURCE)
+                            for (uint32_t i = 0; i < b->n_in; ++i) pool_release(&pool, b->in[i].index);
+                        if (nd->role == OR_ROLE_TRANSFORM) out->batches++;
+                        b->armed = 0;
+                    }
+                    progress = 1;
                 }
-                progress = 1;
-            }
-            /* 2) admission (GraphRunner.hpp:902-983) */
+            /* 2) admission into the source's first free lane (GraphRunner.hpp:902-983) */
             {
                 uint64_t live = generated - sunk, remaining = N - generated;
                 uint64_t min_fill = C < P ? C : P;
                 if (remaining < min_fill) min_fill = remaining;
                 int within = live < P;
                 int budget = live < P && (P - live) >= min_fill;
-                if (generated < N && !lane[src].armed && within && budget && pool.top > reserved) {
+                batch* b = NULL;
+                for (uint32_t j = 0; j < D && !b; ++j)
+                    if (!lane[(uint32_t)src * D + j].armed) b = &lane[(uint32_t)src * D + j];
+                if (generated < N && b && within && budget && pool.top > reserved) {
                     uint64_t fill = remaining;
                     if (fill > P - live) fill = P - live;
                     if (fill > C) fill = C;
-                    batch* b = &lane[src];
                     uint32_t idx = 0;
                     pool_acquire(&pool, &idx);
                     b->n_in = 0;
@@ -878,13 +917,13 @@ int or_run(const or_node_desc* nodes, uint32_t n_nodes, const int32_t* routes, c
                     b->particles = fill;
                     source_batch(&x, &nodes[src], 2u * (uint32_t)src, &pool, b, generated);
                     trace_batch(out, (uint32_t)src, iter, b, 0);
-                    first_id_of_src = generated;
                     generated += fill;
                     b->armed = 1;
                     progress = 1;
                 }
             }
-            /* 3) dispatch one batch: sinks first, then round robin (GraphRunner.hpp:985-1133) */
+            /* 3) dispatch one batch into a free lane: sinks first, then round robin
+             *    (GraphRunner.hpp:985-1133) */
             {
                 int dispatched = 0;
                 for (int pass = 0; pass < 2 && !dispatched; ++pass) {
@@ -892,8 +931,11 @@ int or_run(const or_node_desc* nodes, uint32_t n_nodes, const int32_t* routes, c
                         uint32_t k = pass == 0 ? kk : (rr + kk) % n_nodes;
                         const or_node_desc* nd = &nodes[k];
                         if (pass == 0 && nd->role != OR_ROLE_SINK) continue;
-                        batch* b = &lane[k];
-                        if (fifo_size(&ready[k]) == 0 || b->armed) continue;
+                        if (fifo_size(&ready[k]) == 0) continue;
+                        batch* b = NULL;
+                        for (uint32_t j = 0; j < D && !b; ++j)
+                            if (!lane[k * D + j].armed) b = &lane[k * D + j];
+                        if (!b) continue;
                         uint64_t wanted = nd->role == OR_ROLE_TRANSFORM ? (fifo_size(&ready[k]) < Q ? fifo_size(&ready[k]) : Q) : 1u;
                         uint32_t reserve = 0;
                         if (nd->role == OR_ROLE_TRANSFORM) {
@@ -925,6 +967,7 @@ int or_run(const or_node_desc* nodes, uint32_t n_nodes, const int32_t* routes, c
                         } else if (is_partition(nd)) {
                             int r2 = partition_classify(&x, nd, &pool, b);
                             if (r2) FAIL_RUN(r2, "partition classify: device status reports an invalid classifier result");
+                            if (!hash_tables_fit(cfg, nd, b)) FAIL_RUN(OR_INVALID, "partition classify: hash bin table full");
                         } else {
                             int r2 = transform_batch(&x, nd, purpose, &pool, b, reserve_bound_of(nd, C, Q));
                             if (r2) FAIL_RUN(r2, "transform batch failed");
@@ -939,68 +982,82 @@ int or_run(const or_node_desc* nodes, uint32_t n_nodes, const int32_t* routes, c
             if (out->status) goto done;
             /* 4) termination */
             int drained = 1;
-            for (uint32_t k = 0; k < n_nodes; ++k)
-                if (lane[k].armed || fifo_size(&ready[k])) { drained = 0; break; }
+            for (uint32_t k = 0; k < n_nodes && drained; ++k) {
+                if (fifo_size(&ready[k])) drained = 0;
+                for (uint32_t j = 0; j < D; ++j) if (lane[k * D + j].armed) drained = 0;
+            }
             iter++;
             if (generated >= N && drained && reserved == 0) break;
             if (!progress) FAIL_RUN(OR_STALL, "scheduler stall: no progress possible");
         }
     } else {
-        /* Step schedule of the device-resident batched form: per step every node takes at most
-         * one batch, selected in node order against the remaining credits; execution pops output
-         * chunks in node order; retirement routes outputs and releases inputs in node order. */
+        /* Step schedule of the device-resident batched form. Per step, in node order, every lane of
+         * a node takes at most one batch: a source lane admits one chunk (the admission rule of
+         * GraphRunner.hpp:902-983 with the step's earlier admissions counted), a transform lane the
+         * largest credit-fitting prefix of <= Q ready chunks after the earlier lanes' batches, and
+         * a sink all ready chunks in one batch. Execution pops output chunks in (node, lane)
+         * order; retirement routes outputs and releases inputs in the same order. */
         for (;;) {
             uint32_t res = 0;
             int any = 0;
             for (uint32_t k = 0; k < n_nodes; ++k) {
                 const or_node_desc* nd = &nodes[k];
-                batch* b = &lane[k];
-                b->armed = 0;
-                b->n_in = 0;
-                b->n_out = 0;
-                if (nd->role == OR_ROLE_SOURCE) {
-                    uint64_t live = generated - sunk, remaining = N - generated;
-                    uint64_t min_fill = C < P ? C : P;
-                    if (remaining < min_fill) min_fill = remaining;
-                    if (generated < N && live < P && (P - live) >= min_fill && pool.top - res >= 1u) {
-                        uint64_t fill = remaining;
-                        if (fill > P - live) fill = P - live;
-                        if (fill > C) fill = C;
-                        b->particles = fill;
-                        first_id_of_src = generated;
-                        generated += fill;
-                        res += 1;
-                        b->armed = 1;
+                int stop = 0;
+                for (uint32_t j = 0; j < D; ++j) {
+                    batch* b = &lane[k * D + j];
+                    b->armed = 0;
+                    b->n_in = 0;
+                    b->n_out = 0;
+                    if (stop) continue;
+                    if (nd->role == OR_ROLE_SOURCE) {
+                        uint64_t live = generated - sunk, remaining = N - generated;
+                        uint64_t min_fill = C < P ? C : P;
+                        if (remaining < min_fill) min_fill = remaining;
+                        if (generated < N && live < P && (P - live) >= min_fill && pool.top - res >= 1u) {
+                            uint64_t fill = remaining;
+                            if (fill > P - live) fill = P - live;
+                            if (fill > C) fill = C;
+                            b->particles = fill;
+                            b->first_id = generated;
+                            generated += fill;
+                            res += 1;
+                            b->armed = 1;
+                        } else {
+                            stop = 1;
+                        }
+                    } else if (nd->role == OR_ROLE_TRANSFORM) {
+                        uint64_t wanted = fifo_size(&ready[k]) < Q ? fifo_size(&ready[k]) : Q, parts = 0;
+                        for (uint64_t i = 0; i < wanted; ++i) parts += fifo_at(&ready[k], i).fill;
+                        uint32_t reserve = 0;
+                        while (wanted) {
+                            uint64_t bound;
+                            or_output_chunk_bound(parts, C, nd->bins, &bound);
+                            if (pool.top - res >= bound) { reserve = (uint32_t)bound; break; }
+                            --wanted;
+                            parts -= fifo_at(&ready[k], wanted).fill;
+                        }
+                        if (wanted) {
+                            b->n_in = (uint32_t)wanted;
+                            for (uint64_t i = 0; i < wanted; ++i) b->in[i] = fifo_at(&ready[k], i);
+                            ready[k].head += wanted;
+                            res += reserve;
+                            b->armed = 1;
+                        } else {
+                            stop = 1;
+                        }
+                    } else {
+                        uint64_t n = fifo_size(&ready[k]);
+                        if (n) {
+                            b->n_in = (uint32_t)n;
+                            b->particles = 0;
+                            for (uint64_t i = 0; i < n; ++i) { b->in[i] = fifo_at(&ready[k], i); b->particles += b->in[i].fill; }
+                            ready[k].head += n;
+                            b->armed = 1;
+                        }
+                        stop = 1; /* one sink batch per step */
                     }
-                } else if (nd->role == OR_ROLE_TRANSFORM) {
-                    uint64_t wanted = fifo_size(&ready[k]) < Q ? fifo_size(&ready[k]) : Q, parts = 0;
-                    for (uint64_t i = 0; i < wanted; ++i) parts += fifo_at(&ready[k], i).fill;
-                    uint32_t reserve = 0;
-                    while (wanted) {
-                        uint64_t bound;
-                        or_output_chunk_bound(parts, C, nd->bins, &bound);
-                        if (pool.top - res >= bound) { reserve = (uint32_t)bound; break; }
-                        --wanted;
-                        parts -= fifo_at(&ready[k], wanted).fill;
-                    }
-                    if (wanted) {
-                        b->n_in = (uint32_t)wanted;
-                        for (uint64_t i = 0; i < wanted; ++i) b->in[i] = fifo_at(&ready[k], i);
-                        ready[k].head += wanted;
-                        res += reserve;
-                        b->armed = 1;
-                    }
-                } else {
-                    uint64_t n = fifo_size(&ready[k]);
-                    if (n) {
-                        b->n_in = (uint32_t)n;
-                        b->particles = 0;
-                        for (uint64_t i = 0; i < n; ++i) { b->in[i] = fifo_at(&ready[k], i); b->particles += b->in[i].fill; }
-                        ready[k].head += n;
-                        b->armed = 1;
-                    }
+                    any |= b->armed;
                 }
-                any |= b->armed;
             }
             if (!any) {
                 int drained = 1;
@@ -1008,53 +1065,56 @@ int or_run(const or_node_desc* nodes, uint32_t n_nodes, const int32_t* routes, c
                 if (generated >= N && drained) break;
                 FAIL_RUN(OR_STALL, "scheduler stall: no progress possible");
             }
-            /* execute in node order */
-            for (uint32_t k = 0; k < n_nodes; ++k) {
-                const or_node_desc* nd = &nodes[k];
-                batch* b = &lane[k];
-                if (!b->armed) continue;
-                uint32_t purpose = 2u * k;
-                if (nd->role == OR_ROLE_SOURCE) {
-                    uint32_t idx = 0;
-                    pool_acquire(&pool, &idx);
-                    b->n_out = 1;
-                    b->out[0].index = idx; b->out[0].gen = pool.gen[idx];
-                    b->out[0].fill = (uint32_t)b->particles; b->out[0].bin = 0;
-                    source_batch(&x, nd, purpose, &pool, b, first_id_of_src);
-                    trace_batch(out, k, iter, b, 0);
-                } else if (nd->role == OR_ROLE_SINK) {
-                    sink_batch(&x, nd, &pool, b);
-                    trace_batch(out, k, iter, b, 0);
-                } else if (is_partition(nd)) {
-                    int r2 = partition_classify(&x, nd, &pool, b);
-                    if (r2) FAIL_RUN(r2, "partition classify: device status reports an invalid classifier result");
-                    r2 = partition_plan_scatter(&x, nd, purpose + 1u, &pool, b, reserve_bound_of(nd, C, Q));
-                    if (r2 == OR_OVERFLOW) FAIL_RUN(OR_OVERFLOW, "partition scatter: checked updater overflow");
-                    if (r2) FAIL_RUN(r2, "partition plan failed");
-                    trace_batch(out, k, iter, b, nd->bins);
-                } else {
-                    int r2 = transform_batch(&x, nd, purpose, &pool, b, reserve_bound_of(nd, C, Q));
-                    if (r2) FAIL_RUN(r2, "transform batch failed");
-                    trace_batch(out, k, iter, b, 1);
-                }
-            }
-            /* retire in node order */
-            for (uint32_t k = 0; k < n_nodes; ++k) {
-                const or_node_desc* nd = &nodes[k];
-                batch* b = &lane[k];
-                if (!b->armed) continue;
-                if (nd->role != OR_ROLE_SINK)
-                    for (uint32_t o = 0; o < b->n_out; ++o) {
-                        int32_t d = routes[nd->route_offset + b->out[o].bin];
-                        if (is_partition(nd) && fused_of(&x, nd, b->out[o].bin) >= 0) d = routes[nodes[d].route_offset];
-                        fifo_push(&ready[d], b->out[o]);
+            /* execute in (node, lane) order */
+            for (uint32_t k = 0; k < n_nodes; ++k)
+                for (uint32_t j = 0; j < D; ++j) {
+                    const or_node_desc* nd = &nodes[k];
+                    batch* b = &lane[k * D + j];
+                    if (!b->armed) continue;
+                    uint32_t purpose = 2u * k;
+                    if (nd->role == OR_ROLE_SOURCE) {
+                        uint32_t idx = 0;
+                        pool_acquire(&pool, &idx);
+                        b->n_out = 1;
+                        b->out[0].index = idx; b->out[0].gen = pool.gen[idx];
+                        b->out[0].fill = (uint32_t)b->particles; b->out[0].bin = 0;
+                        source_batch(&x, nd, purpose, &pool, b, b->first_id);
+                        trace_batch(out, k, iter, b, 0);
+                    } else if (nd->role == OR_ROLE_SINK) {
+                        sink_batch(&x, nd, &pool, b);
+                        trace_batch(out, k, iter, b, 0);
+                    } else if (is_partition(nd)) {
+                        int r2 = partition_classify(&x, nd, &pool, b);
+                        if (r2) FAIL_RUN(r2, "partition classify: device status reports an invalid classifier result");
+                        if (!hash_tables_fit(cfg, nd, b)) FAIL_RUN(OR_INVALID, "partition classify: hash bin table full");
+                        r2 = partition_plan_scatter(&x, nd, purpose + 1u, &pool, b, reserve_bound_of(nd, C, Q));
+                        if (r2 == OR_OVERFLOW) FAIL_RUN(OR_OVERFLOW, "partition scatter: checked updater overflow");
+                        if (r2) FAIL_RUN(r2, "partition plan failed");
+                        trace_batch(out, k, iter, b, nd->bins);
+                    } else {
+                        int r2 = transform_batch(&x, nd, purpose, &pool, b, reserve_bound_of(nd, C, Q));
+                        if (r2) FAIL_RUN(r2, "transform batch failed");
+                        trace_batch(out, k, iter, b, 1);
                     }
-                if (nd->role == OR_ROLE_SINK) sunk += b->particles;
-                if (nd->role != OR_ROLE_SOURCE)
-                    for (uint32_t i = 0; i < b->n_in; ++i) pool_release(&pool, b->in[i].index);
-                if (nd->role == OR_ROLE_TRANSFORM) out->batches++;
-                b->armed = 0;
-            }
+                }
+            /* retire in (node, lane) order */
+            for (uint32_t k = 0; k < n_nodes; ++k)
+                for (uint32_t j = 0; j < D; ++j) {
+                    const or_node_desc* nd = &nodes[k];
+                    batch* b = &lane[k * D + j];
+                    if (!b->armed) continue;
+                    if (nd->role != OR_ROLE_SINK)
+                        for (uint32_t o = 0; o < b->n_out; ++o) {
+                            int32_t d = routes[nd->route_offset + b->out[o].bin];
+                            if (is_partition(nd) && fused_of(&x, nd, b->out[o].bin) >= 0) d = routes[nodes[d].route_offset];
+                            fifo_push(&ready[d], b->out[o]);
+                        }
+                    if (nd->role == OR_ROLE_SINK) sunk += b->particles;
+                    if (nd->role != OR_ROLE_SOURCE)
+                        for (uint32_t i = 0; i < b->n_in; ++i) pool_release(&pool, b->in[i].index);
+                    if (nd->role == OR_ROLE_TRANSFORM) out->batches++;
+                    b->armed = 0;
+                }
             iter++;
         }
     }
@@ -1064,16 +1124,17 @@ done:
     out->generated = generated;
     out->sunk = sunk;
     out->peak_live_chunks = pool.peak_live;
-    for (uint32_t k = 0; k < n_nodes; ++k) {
-        free(ready[k].d); free(lane[k].in); free(lane[k].out); free(lane[k].counts);
-        free(lane[k].items); free(lane[k].bins);
+    for (uint32_t k = 0; k < n_nodes; ++k) free(ready[k].d);
+    for (uint64_t i = 0; i < (uint64_t)n_nodes * D; ++i) {
+        free(lane[i].in); free(lane[i].out); free(lane[i].counts);
+        free(lane[i].items); free(lane[i].bins);
     }
     free(ready); free(lane);
     free(x.zipf);
     pool_free(&pool);
-    (void)maxb;
     return (int)out->status;
 }
+
 
 /* One partition batch over explicit arrays: bins, totals, and the stable rank j of every item
  * inside its bin (the slot is then table[B + binBase[bin] + j/C], j % C). */

@@ -1,0 +1,64 @@
+"""Lanes on the GPU (Tuning::inflightBatches, flow/Tuning.hpp:34-41; sched/GraphRunner.hpp:766-771,
+1318-1330): the host runner with D lanes per node, and the wide and one-workgroup step engines
+running up to D batches per node and step. Every batch record (inputs, counts, output chunks,
+generations, fills) must equal the oracle's schedule with the same depth."""
+import numpy as np
+import pytest
+
+import graphflow_amd as g
+import oracle as O
+from gpu_util import SEED, assert_same_trace, feedback_flow, run_flow, run_oracle, torch_cuda
+
+pytestmark = pytest.mark.gpu
+
+
+@pytest.mark.parametrize("depth", [1, 2, 4])
+@pytest.mark.parametrize("schedule,engine,fuse", [(g.SCHEDULE_HOST, 0, False), (g.SCHEDULE_DEVICE, 1, False),
+                                                  (g.SCHEDULE_DEVICE, 1, True), (g.SCHEDULE_DEVICE, 2, True)])
+def test_feedback_lanes_trace(depth, schedule, engine, fuse):
+    torch = torch_cuda()
+    N, C, P, Q = 20000, 512, 4096, 4
+    flow, src, rec = feedback_flow(N)
+    stats, trace = run_flow(flow, src, N, C=C, P=P, Q=Q, schedule=schedule, fuse=fuse, engine=engine,
+                            inflight_batches=depth)
+    ref = run_oracle(flow, N, C=C, P=P, Q=Q, schedule=schedule, fuse=fuse, record=True, inflight=depth)
+    assert ref["status"] == O.OK, ref["message"]
+    assert_same_trace(trace, ref["trace"])
+    assert stats["steps"] == ref["steps"]
+    w, _ = O.feedback_scalar(SEED, N)
+    torch.cuda.synchronize()
+    assert (rec.cpu().numpy().view(np.uint32) == w).all()
+
+
+@pytest.mark.parametrize("depth", [2, 4])
+def test_hash_pipeline_lanes_trace(depth):
+    """Config 3 shape with several batches of the 4096-bucket partition per step."""
+    torch = torch_cuda()
+    N, B, C, Q = 300000, 4096, 8192, 4
+    cap = 1 << 17
+    log = torch.zeros(4 * cap, dtype=torch.int32, device="cuda")
+    mset = torch.zeros(2 * B, dtype=torch.int64, device="cuda")
+    flow = g.Flow(g.PAYLOAD_KEYED)
+    src = flow.source("keys", g.ZipfKeys(1 << 20, 1.1))
+    part = flow.partition("bucket", g.KeyBucket(B))
+    sink = flow.sink("digest", g.ChunkDigest(log.data_ptr(), cap, mset.data_ptr(), B))
+    flow.connect(src, part)
+    for b in range(B):
+        flow.connect(part, sink, b)
+    P = 1 << 17
+    flow.configure(g.Tuning(seed=SEED, batch_chunks=Q, chunk_capacity=C, live_particles=P, schedule=g.SCHEDULE_DEVICE,
+                            trace=True, inflight_batches=depth, device_engine=1))
+    st = flow.run(src, N)
+    ref = run_oracle(flow, N, C=C, P=P, Q=Q, schedule=g.SCHEDULE_DEVICE, digest_cap=cap, multiset_bins=B,
+                     inflight=depth)
+    assert ref["status"] == O.OK, ref["message"]
+    assert_same_trace(flow.trace(), ref["trace"])
+    torch.cuda.synchronize()
+    ms = mset.cpu().numpy().view(np.uint64).reshape(-1, 2)
+    assert (ms[:, 0] == ref["multiset"]).all() and int(ms[:, 1].sum()) == N == st["sunk"]
+
+
+def test_team_engine_rejects_lanes():
+    flow, src, _ = feedback_flow(1000)
+    with pytest.raises(ValueError):
+        flow.configure(g.Tuning(seed=SEED, schedule=g.SCHEDULE_DEVICE, device_engine=3, inflight_batches=2))
