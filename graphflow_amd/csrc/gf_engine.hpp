@@ -76,6 +76,7 @@ struct Engine {
     uint32_t* trace;
     unsigned long long trace_cap;
     unsigned long long* prof;  // optional per-phase wall-clock totals (micro engine), PROF_*
+    unsigned long long* lb;    // look-back words of the grid plan, one per lane
 };
 
 enum { PROF_COUNT = 0, PROF_PLAN = 1, PROF_SOURCE = 2, PROF_PART = 3, PROF_XF = 4, PROF_SINK = 5, PROF_RETIRE = 6,
@@ -625,8 +626,9 @@ __device__ inline void engine_retire(const Engine& e, SelLds& L)
 __device__ inline void engine_select(const Engine& e, SelLds& L)
 {
     const uint32_t tid = threadIdx.x, n = e.n_nodes, C = e.C, P = e.P, Q = e.Q;
-    // stage the candidate ready descriptors of every non-source node's ring: <= Q per transform
-    // lane, all per sink (the first lane of a node stages for all its lanes)
+    // stage the candidate ready descriptors of every transform's ring in LDS: <= Q per lane (the
+    // first lane of a node stages for all its lanes). A sink takes all its ready chunks, which its
+    // input list reads straight from the ring.
     if (tid == 0) {
         uint32_t acc = 0;
         L.overflow = 0u;
@@ -639,7 +641,7 @@ __device__ inline void engine_select(const Engine& e, SelLds& L)
             L.lst_off[k] = acc;
             L.wanted[k] = want;
             L.taken[k] = 0u;
-            acc += want;
+            if (kind != KIND_SINK) acc += want;
         }
         L.lst_off[n] = acc;
         L.staged = acc;
@@ -736,7 +738,8 @@ __device__ inline void engine_select(const Engine& e, SelLds& L)
             for (uint32_t u = 0; u < kAhead; ++u) {
                 const uint32_t i = g0 + tid * kAhead + u;
                 dd[u] = gf_chunk_desc{0, 0, 0, 0};
-                if (i < m) dd[u] = L.overflow ? ring_entry(l, at + i) : L.lst[L.lst_off[l] + at + i];
+                if (i < m)
+                    dd[u] = L.overflow || a.kind == KIND_SINK ? ring_entry(l, at + i) : L.lst[L.lst_off[l] + at + i];
             }
 #pragma unroll
             for (uint32_t u = 0; u < kAhead; ++u) {
@@ -787,6 +790,7 @@ __device__ inline void engine_select(const Engine& e, SelLds& L)
         e.ka_prefix[n] = ka;
         e.kb_prefix[n] = kb;
         e.ctl[CTL_TOP] = L.top;
+        e.ctl[CTL_TICKET_A] = 0u;  // the next step's grid plan takes its lanes from ticket 0
         e.ctr[CNT_GENERATED] = L.generated;
         e.ctr[CNT_SUNK] = L.sunk;
         e.ctr[CNT_STEP] = L.step;
@@ -820,20 +824,211 @@ __device__ inline uint32_t find_node(const uint32_t* prefix, uint32_t n, uint32_
     return k;
 }
 
-// Ticket: true in the last workgroup to arrive. No fence: the caller's last workgroup reads no
-// data written by the other workgroups of the same launch, and every other workgroup has finished
-// all reads of the step's batch descriptors before it takes its ticket (the last workgroup
-// rewrites those descriptors for the next step). A device-scope fence per workgroup would cost
-// an L2 write-back each (about 50 ns serialised per workgroup on gfx950).
-__device__ inline bool last_block(uint32_t* ticket, uint32_t* flag_sh)
+// ------------------------------------------------------------------------------------------
+// KP over the grid: one workgroup per active lane, pop offsets by a decoupled look-back
+// ------------------------------------------------------------------------------------------
+// Workgroups take the step's active lanes in node order by ticket (CTL_TICKET_A, cleared by the
+// select of the previous step). A lane's workgroup plans its batch alone: per-(tile, bin) prefix
+// over tiles, bin totals, exact output chunk count R and bin bases. Its output chunks start at the
+// step's pop position O = sum of R over the earlier lanes, found by a decoupled look-back over the
+// lanes' published words (Merrill & Garland, "Single-pass Parallel Prefix Scan with Decoupled
+// Look-back", 2016): word = epoch << 36 | flag << 34 | value, flag 1 = the lane's R, flag 2 = the
+// inclusive prefix through the lane. The epoch is the step number (28 bits; a word of a step 2^28
+// steps earlier is never read), so no word is ever cleared. A
+// workgroup publishes its R before it waits, and waits only on lanes with earlier tickets, which
+// are held by running workgroups: every wait ends.
+enum { LB_AGG = 1ull, LB_INC = 2ull };
+
+__device__ inline unsigned long long lb_word(uint32_t epoch, unsigned long long flag, unsigned long long v)
 {
-    __syncthreads();
-    if (threadIdx.x == 0) {
-        uint32_t t = atomicAdd(ticket, 1u);
-        *flag_sh = (t == gridDim.x - 1u) ? 1u : 0u;
+    return ((unsigned long long)(epoch & 0xFFFFFFFu) << 36) | (flag << 34) | (v & ((1ull << 34) - 1ull));
+}
+
+__device__ inline void lb_publish(unsigned long long* w, unsigned long long v)
+{
+    __hip_atomic_store(w, v, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
+}
+
+// Exclusive prefix of the published values before unit u (thread 0 only).
+__device__ inline unsigned long long lb_lookback(unsigned long long* words, uint32_t u, uint32_t epoch)
+{
+    unsigned long long acc = 0;
+    for (int32_t j = (int32_t)u - 1; j >= 0; --j) {
+        unsigned long long w;
+        for (;;) {
+            w = __hip_atomic_load(&words[j], __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT);
+            if ((uint32_t)(w >> 36) == (epoch & 0xFFFFFFFu) && ((w >> 34) & 3ull) != 0ull) break;
+            __builtin_amdgcn_s_sleep(1);
+        }
+        acc += w & ((1ull << 34) - 1ull);
+        if (((w >> 34) & 3ull) == LB_INC) break;
+    }
+    return acc;
+}
+
+struct GridPlanLds {
+    uint32_t unit, units, v, status, fail;
+    uint32_t scan[kMaxWaves];
+    unsigned long long off, top;
+    uint32_t act[kMaxNodes];  // active lanes in node order
+};
+
+// Plan of one partition lane: per (tile, bin) exclusive prefix over tiles in place, totals[bin],
+// totals[B + bin] = the bin's first output chunk; returns R, *items = the classified items.
+__device__ inline uint32_t plan_lane_counts(const Engine& e, uint32_t k, uint32_t nt, uint32_t* scan, uint32_t* items)
+{
+    const NodeArr& a = e.arr[k];
+    const uint32_t B = e.nodes[k].bins, C = e.C, tid = threadIdx.x;
+    uint32_t carry_r = 0, carry_n = 0;
+    for (uint32_t b0 = 0; b0 < B; b0 += bsz()) {
+        const uint32_t bin = b0 + tid;
+        uint32_t tot = 0;
+        if (bin < B) {
+            for (uint32_t t0 = 0; t0 < nt; t0 += 8u) {
+                uint32_t x[8];
+#pragma unroll
+                for (uint32_t j = 0; j < 8u; ++j) x[j] = t0 + j < nt ? a.counts[(size_t)(t0 + j) * B + bin] : 0u;
+#pragma unroll
+                for (uint32_t j = 0; j < 8u; ++j) {
+                    if (t0 + j < nt) a.counts[(size_t)(t0 + j) * B + bin] = tot;
+                    tot += x[j];
+                }
+            }
+            a.totals[bin] = tot;
+        }
+        const uint32_t ch = bin < B ? (tot + C - 1u) / C : 0u;
+        uint32_t sum_r, sum_n;
+        const uint32_t ex = block_exscan(ch, scan, &sum_r);
+        block_exscan(tot, scan, &sum_n);
+        if (bin < B) a.totals[B + bin] = carry_r + ex;
+        carry_r += sum_r;
+        carry_n += sum_n;
+    }
+    *items = carry_n;
+    return carry_r;
+}
+
+__device__ inline void engine_plan_grid(const Engine& e, GridPlanLds& L)
+{
+    const uint32_t tid = threadIdx.x, n = e.n_nodes, C = e.C;
+    if (tid == 0) {
+        L.status = load_ctl(&e.ctl[CTL_STATUS]);
+        uint32_t u = 0;
+        for (uint32_t k = 0; k < n; ++k)
+            if (e.st[k].active) L.act[u++] = k;
+        L.units = u;
     }
     __syncthreads();
-    return *flag_sh != 0u;
+    if (L.status) return;
+    const uint32_t epoch = (uint32_t)e.ctr[CNT_STEP];
+    for (;;) {
+        if (tid == 0) {
+            L.unit = atomicAdd(&e.ctl[CTL_TICKET_A], 1u);
+            L.fail = 0u;
+            L.top = e.ctl[CTL_TOP];  // read before this unit publishes (the last unit rewrites it)
+        }
+        __syncthreads();
+        const uint32_t u = L.unit;
+        if (u >= L.units) return;
+        const uint32_t k = L.act[u];
+        NodeSt s = e.st[k];
+        const NodeArr& a = e.arr[k];
+        const uint32_t kind = a.kind;
+        uint32_t R = 0;
+        if (kind == KIND_PARTITION) {
+            uint32_t items = 0;
+            R = plan_lane_counts(e, k, s.n_tiles, L.scan, &items);
+            if (tid == 0 && (items != s.particles || R > a.reserve_bound)) {
+                set_status(&e.ctl[CTL_STATUS], GF_PLAN_MISMATCH);
+                L.fail = 1u;
+            }
+        } else if (kind == KIND_SOURCE) {
+            R = 1u;
+        } else if (kind == KIND_TRANSFORM) {
+            R = (s.particles + C - 1u) / C;
+            if (tid == 0 && (s.particles == 0u || R > a.reserve_bound)) {
+                set_status(&e.ctl[CTL_STATUS], GF_PLAN_MISMATCH);
+                L.fail = 1u;
+            }
+        }
+        __syncthreads();
+        if (tid == 0) {
+            lb_publish(&e.lb[u], lb_word(epoch, u == 0u ? LB_INC : LB_AGG, R));
+            const unsigned long long off = lb_lookback(e.lb, u, epoch);
+            if (u != 0u) lb_publish(&e.lb[u], lb_word(epoch, LB_INC, off + R));
+            L.off = off;
+            if (off + R > L.top && !L.fail) {
+                set_status(&e.ctl[CTL_STATUS], GF_POOL_EXHAUSTED);
+                L.fail = 1u;
+            }
+            if (kind == KIND_SINK) {
+                s.digest_base = e.ctr[CNT_DIGEST0 + k];
+                e.ctr[CNT_DIGEST0 + k] = s.digest_base + s.n_in;
+                e.st[k].digest_base = s.digest_base;
+            }
+            e.st[k].n_out = R;
+            if (kind == KIND_TRANSFORM) a.totals[0] = s.particles;
+            if (u + 1u == L.units) {
+                const uint32_t nt = (uint32_t)(L.top - (off + R));
+                e.ctl[CTL_TOP] = nt;
+                const unsigned long long live = e.S - nt;
+                if (live > e.ctr[CNT_PEAK_LIVE]) e.ctr[CNT_PEAK_LIVE] = live;
+            }
+        }
+        __syncthreads();
+        if (L.fail || !R) continue;
+        // pops: output j of the step takes stack[top - 1 - j]; bin-major, then chunk order
+        const uint32_t top = (uint32_t)L.top, o = (uint32_t)L.off;
+        gf_chunk_desc* out = a.out;
+        if (kind == KIND_PARTITION) {
+            const uint32_t B = e.nodes[k].bins;
+            for (uint32_t bin = tid; bin < B; bin += bsz()) {
+                const uint32_t tb = a.totals[bin], b0 = a.totals[B + bin];
+                for (uint32_t q = 0, rem = tb; rem; ++q) {
+                    const unsigned long long ent = e.free_stack[top - 1u - (o + b0 + q)];
+                    gf_chunk_desc d;
+                    d.index = (uint32_t)ent;
+                    d.generation = (uint32_t)(ent >> 32);
+                    d.fill = rem < C ? rem : C;
+                    d.bin = bin;
+                    out[b0 + q] = d;
+                    rem -= d.fill;
+                }
+            }
+        } else {
+            for (uint32_t r = tid; r < R; r += bsz()) {
+                const unsigned long long ent = e.free_stack[top - 1u - (o + r)];
+                gf_chunk_desc d;
+                d.index = (uint32_t)ent;
+                d.generation = (uint32_t)(ent >> 32);
+                if (kind == KIND_SOURCE) {
+                    d.fill = s.particles;
+                } else {
+                    const uint32_t rem = s.particles - r * C;
+                    d.fill = rem < C ? rem : C;
+                }
+                d.bin = 0u;
+                out[r] = d;
+            }
+        }
+        __syncthreads();
+    }
+}
+
+// Trace records of a step planned over the grid (one workgroup, node order).
+__device__ inline void engine_trace_step(const Engine& e, unsigned long long* len_sh)
+{
+    if (load_ctl(&e.ctl[CTL_STATUS])) return;
+    const uint32_t step = (uint32_t)e.ctr[CNT_STEP];
+    for (uint32_t k = 0; k < e.n_nodes; ++k) {
+        const NodeSt s = e.st[k];
+        if (!s.active) continue;
+        const uint32_t kind = e.arr[k].kind;
+        Batch b = make_batch(e, k, s);
+        if (kind == KIND_PARTITION) trace_record(e, k, b, e.nodes[k].bins, e.arr[k].totals, len_sh, step);
+        else if (kind == KIND_TRANSFORM) trace_record(e, k, b, 1u, e.arr[k].totals, len_sh, step);
+        else trace_record(e, k, b, 0u, nullptr, len_sh, step);
+    }
 }
 
 }  // namespace gfd

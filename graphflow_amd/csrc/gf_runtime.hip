@@ -177,13 +177,21 @@ __global__ __launch_bounds__(kThreads) void k_engine_ka(Engine e)
     }
 }
 
-// One workgroup: the plan of every node of the step (reads KA's counts across the kernel boundary).
+// The plan of every lane of the step over the grid: one workgroup per active lane, pop offsets by
+// a decoupled look-back (reads KA's counts across the kernel boundary).
 __global__ __launch_bounds__(1024) void k_engine_plan(Engine e)
 {
-    extern __shared__ char smem[];
+    __shared__ GridPlanLds L;
     __shared__ uint32_t flag;
     if (engine_stopped(e, &flag)) return;
-    engine_plan(e, *reinterpret_cast<PlanLds*>(smem));
+    engine_plan_grid(e, L);
+}
+
+// Trace records of the planned step (one workgroup; only when the run records a trace).
+__global__ __launch_bounds__(1024) void k_engine_trace(Engine e)
+{
+    __shared__ unsigned long long len;
+    engine_trace_step(e, &len);
 }
 
 // Tiles round-robin over the workgroups; the tiles of the sinks flagged NODE_WAVE_SINK (sinks fed by
@@ -335,6 +343,7 @@ __global__ __launch_bounds__(kThreads) void k_engine_setup(Engine e, uint32_t do
     for (uint32_t k = tid; k < e.n_nodes; k += kThreads) {
         NodeSt z{};
         L.st[k] = z;
+        e.lb[k] = 0ull;  // epoch 0, no flag: not published
         L.head[k] = 0ull;
         L.tail[k] = 0ull;
     }
@@ -1491,6 +1500,7 @@ void build_engine(gf_flow& f, uint64_t N, const RunShape& sh, uint32_t src)
     e.trace_cap = t.trace ? (unsigned long long)1 << 26 : 0ull;
     e.trace = t.trace ? m.d<uint32_t>(e.trace_cap) : nullptr;
     e.prof = t.use_graph == 3u ? m.d<unsigned long long>(PROF_WORDS) : nullptr;
+    e.lb = m.d<unsigned long long>(nv);
     const uint32_t gmax = t.max_workgroups ? t.max_workgroups : 64u;
     eb->grid = std::min<uint32_t>(std::max<uint32_t>(grid_tiles, 1u), gmax);
     eb->r = r;
@@ -1555,7 +1565,6 @@ void build_engine(gf_flow& f, uint64_t N, const RunShape& sh, uint32_t src)
         set_lds(k_engine_kb<P, RR>, lds_b);
         if (micro_fits) set_lds(k_micro<P, RR>, eb->micro_lds);
     });
-    set_lds(k_engine_plan, sizeof(PlanLds));
     set_lds(k_engine_kr, sizeof(SelLds));
     (void)src;
     f.eng = std::move(eb);
@@ -1572,7 +1581,8 @@ void launch_step(const EngineBuffers& eb, uint32_t payload, hipStream_t s, hipEv
         if (ev) HIPX(hipEventRecord(ev[0], s));
         hipLaunchKernelGGL((k_engine_ka<P, RR>), dim3(eb.grid), dim3(kThreads), eb.lds_a, s, eb.e);
         if (ev) HIPX(hipEventRecord(ev[1], s));
-        hipLaunchKernelGGL(k_engine_plan, dim3(1), dim3(1024), sizeof(PlanLds), s, eb.e);
+        hipLaunchKernelGGL(k_engine_plan, dim3(eb.e.n_nodes), dim3(1024), 0, s, eb.e);
+        if (eb.e.trace_on) hipLaunchKernelGGL(k_engine_trace, dim3(1), dim3(1024), 0, s, eb.e);
         if (ev) HIPX(hipEventRecord(ev[2], s));
         hipLaunchKernelGGL((k_engine_kb<P, RR>), dim3(eb.grid), dim3(kThreads), eb.lds_b, s, eb.e);
         hipLaunchKernelGGL(k_engine_kr, dim3(1), dim3(1024), sizeof(SelLds), s, eb.e);
