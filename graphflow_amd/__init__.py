@@ -130,6 +130,12 @@ SIGNATURES = {
     "gf_pool_write": (C.c_int, [_V, C.c_uint32, C.c_uint64, _V, C.c_uint64]),
     "gf_pool_read": (C.c_int, [_V, C.c_uint32, C.c_uint64, _V, C.c_uint64]),
     "gf_stage_create": (C.c_int, [_V, _P(NodeDesc), C.c_uint32, C.c_uint64, C.c_uint32, C.c_uint32, _P(_V)]),
+    "gf_stage_create_tuned": (C.c_int, [_V, _P(NodeDesc), C.c_uint32, _P(TuningC), _P(_V)]),
+    "gf_stage_last_error": (C.c_int, [_V, C.c_char_p, C.c_uint32]),
+    "gf_pool_lease_count": (C.c_int, [_V, ChunkHandle, _P(C.c_uint32)]),
+    "gf_pool_is_live": (C.c_int, [_V, ChunkHandle, _P(C.c_int)]),
+    "gf_pool_generation": (C.c_int, [_V, C.c_uint32, _P(C.c_uint32)]),
+    "gf_pool_validate_live": (C.c_int, [_V, ChunkHandle]),
     "gf_stage_destroy": (C.c_int, [_V]),
     "gf_stage_launch": (C.c_int, [_V, _P(ChunkDesc), C.c_uint32, C.c_uint64, _V]),
     "gf_stage_advance": (C.c_int, [_V, _V, _P(C.c_uint32), _P(C.c_int)]),
@@ -277,6 +283,24 @@ class Pool:
         _raise(lib().gf_pool_counts(self.handle, a))
         return dict(zip(["num_chunks", "capacity", "free", "live", "peak_live", "peak_leases"], list(a)))
 
+    def lease_count(self, h) -> int:
+        o = C.c_uint32()
+        _raise(lib().gf_pool_lease_count(self.handle, h, C.byref(o)))
+        return o.value
+
+    def is_live(self, h) -> bool:
+        o = C.c_int()
+        _raise(lib().gf_pool_is_live(self.handle, h, C.byref(o)))
+        return bool(o.value)
+
+    def generation(self, index: int) -> int:
+        o = C.c_uint32()
+        _raise(lib().gf_pool_generation(self.handle, index, C.byref(o)))
+        return o.value
+
+    def validate_live(self, h):
+        _raise(lib().gf_pool_validate_live(self.handle, h))
+
     def base(self, h) -> int:
         o = C.c_uint64()
         _raise(lib().gf_pool_base(self.handle, h, C.byref(o)))
@@ -304,20 +328,32 @@ class Stage:
     """One node's batch protocol: launch / advance / finish / abort (flow/Execution.hpp:1537-1561)."""
 
     def __init__(self, pool: Pool, node: NodeDesc, node_key: int = 0, seed: int = 0, batch_chunks: int = 1,
-                 tile_items: int = 0):
+                 tile_items: int = 0, **tuning):
+        """tuning: further Tuning fields (bin_path, hash_table_entries, stall_timeout_ms, photon)."""
         self.pool = pool
         self.node = node
         h = _V()
-        _raise(lib().gf_stage_create(pool.handle, C.byref(node), node_key, seed, batch_chunks, tile_items, C.byref(h)))
+        if tuning:
+            t = Tuning(seed=seed, batch_chunks=batch_chunks, tile_items=tile_items or None, **tuning).c()
+            _raise(lib().gf_stage_create_tuned(pool.handle, C.byref(node), node_key, C.byref(t), C.byref(h)))
+        else:
+            _raise(lib().gf_stage_create(pool.handle, C.byref(node), node_key, seed, batch_chunks, tile_items,
+                                         C.byref(h)))
         self.handle = h
+
+    def _check(self, code: int):
+        if code != OK:
+            buf = C.create_string_buffer(512)
+            lib().gf_stage_last_error(self.handle, buf, 512)
+            _raise(code, buf.value.decode())
 
     def launch(self, chunks: Sequence[tuple], first_id: int = 0):
         arr = (ChunkDesc * len(chunks))(*[ChunkDesc(*c) for c in chunks])
-        _raise(lib().gf_stage_launch(self.handle, arr, len(chunks), first_id, None))
+        self._check(lib().gf_stage_launch(self.handle, arr, len(chunks), first_id, None))
 
     def advance(self):
         a, m = C.c_uint32(), C.c_int()
-        _raise(lib().gf_stage_advance(self.handle, None, C.byref(a), C.byref(m)))
+        self._check(lib().gf_stage_advance(self.handle, None, C.byref(a), C.byref(m)))
         return bool(m.value), a.value
 
     def finish(self, cap: int = 1 << 16):

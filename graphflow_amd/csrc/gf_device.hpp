@@ -538,6 +538,164 @@ __device__ void count_tile(const gf_node_desc& nd, const Batch& b, uint32_t t, c
     __syncthreads();
 }
 
+// ------------------------------------------------------------------------------------------
+// Hash bin path (BinPath::hash, core/BinPath.hpp): partitions wider than the dense LDS counters
+// count and rank a tile's items through an open-addressing table of E = 2^k slots in LDS
+// (graph/BinTable.hpp:69-137: home slot (bin * 0x9E3779B1) & (E - 1), linear probing, a bin's
+// key set once by compare-and-swap). LDS grows with the tile, not with the partition width, so a
+// partition may have up to 65536 bins. A tile with more distinct bins than slots fails its batch
+// before any output is planned (Execution.hpp:1207-1307).
+// ------------------------------------------------------------------------------------------
+constexpr uint32_t kEmptyKey = 0xFFFFFFFFu;
+constexpr uint32_t kNoSlot = 0xFFFFFFFFu;
+constexpr uint32_t kStatusTableFull = 17u;  // device status word: a tile's bin table is full (-> GF_INVALID)
+
+__device__ inline uint32_t hash_home(uint32_t bin, uint32_t mask) { return (bin * 0x9E3779B1u) & mask; }
+
+// Slot of `bin`, inserting it; kNoSlot when every slot holds another bin.
+__device__ inline uint32_t hash_insert(uint32_t* keys, uint32_t mask, uint32_t bin)
+{
+    uint32_t slot = hash_home(bin, mask);
+    for (uint32_t p = 0; p <= mask; ++p) {
+        const uint32_t old = atomicCAS(&keys[slot], kEmptyKey, bin);
+        if (old == kEmptyKey || old == bin) return slot;
+        slot = (slot + 1u) & mask;
+    }
+    return kNoSlot;
+}
+
+// Slot of `bin` in a built table, kNoSlot when absent.
+__device__ inline uint32_t hash_find(const uint32_t* keys, uint32_t mask, uint32_t bin)
+{
+    uint32_t slot = hash_home(bin, mask);
+    for (uint32_t p = 0; p <= mask; ++p) {
+        const uint32_t k = keys[slot];
+        if (k == bin) return slot;
+        if (k == kEmptyKey) return kNoSlot;
+        slot = (slot + 1u) & mask;
+    }
+    return kNoSlot;
+}
+
+// LDS bytes of count_tile_hash / scatter_tile_hash with E slots.
+__host__ __device__ inline size_t count_hash_smem(uint32_t E) { return (size_t)E * 8u + 16u; }
+__host__ __device__ inline size_t scatter_hash_smem(uint32_t E) { return (size_t)E * 4u * (kWaves + 2u) + 16u; }
+
+// Hash-path classify of one tile: the tile's bins counted in the LDS table, then the tile's dense
+// count row written from it (the plan scans [tile][bin] rows whatever the bin path).
+template <class P, int R>
+__device__ void count_tile_hash(const gf_node_desc& nd, const Batch& b, uint32_t t, const Cols& cols, uint32_t C,
+                                uint32_t* status, char* smem, uint32_t E)
+{
+    const uint32_t B = nd.bins, tid = ltid(), w = tid >> 6, lane = lane_id(), mask = E - 1u;
+    uint32_t* keys = (uint32_t*)smem;
+    uint32_t* cnt = keys + E;
+    uint32_t* full = cnt + E;
+    for (uint32_t i = tid; i < E; i += kThreads) {
+        keys[i] = kEmptyKey;
+        cnt[i] = 0u;
+    }
+    if (tid == 0) *full = 0u;
+    __syncthreads();
+    TileSpan s = tile_span(b, b.in, b.n_in, t, C);
+#pragma unroll
+    for (int r = 0; r < R; ++r) {
+        const uint32_t i = w * 64u * R + (uint32_t)r * 64u + lane;
+        if (i < s.count) {
+            P p = P::load(cols, s.base + i);
+            const uint32_t bin = classify(nd, p);
+            if (bin >= B) {
+                set_status(status, GF_INVALID);
+            } else {
+                const uint32_t slot = hash_insert(keys, mask, bin);
+                if (slot == kNoSlot) *full = 1u;
+                else atomicAdd(&cnt[slot], 1u);
+            }
+        }
+    }
+    __syncthreads();
+    if (tid == 0 && *full) set_status(status, kStatusTableFull);
+    if (t != kNoTile) {
+        uint32_t* dst = b.counts + (size_t)t * B;
+        for (uint32_t bin = tid; bin < B; bin += kThreads) {
+            const uint32_t slot = hash_find(keys, mask, bin);
+            dst[bin] = slot == kNoSlot ? 0u : cnt[slot];
+        }
+    }
+    __syncthreads();
+}
+
+// Hash-path stable scatter of one tile: scatter_tile's ranking with the per-wave running counts
+// kept per table slot instead of per bin.
+template <class P, int R>
+__device__ void scatter_tile_hash(const gf_node_desc& nd, const Batch& b, uint32_t t, const Cols& cols, uint32_t C,
+                                  uint32_t* status, char* smem, uint32_t E, const gf_node_desc* all = nullptr,
+                                  const int32_t* fuse = nullptr, const RunParams* rp = nullptr,
+                                  const uint32_t* bin_base = nullptr)
+{
+    const uint32_t B = nd.bins, tid = ltid(), w = tid >> 6, lane = lane_id(), mask = E - 1u;
+    uint32_t* keys = (uint32_t*)smem;
+    uint32_t* wcnt = keys + E;            // [kWaves][E] running, then exclusive wave prefix
+    uint32_t* full = wcnt + kWaves * E;
+    for (uint32_t i = tid; i < E; i += kThreads) keys[i] = kEmptyKey;
+    for (uint32_t i = tid; i < kWaves * E; i += kThreads) wcnt[i] = 0u;
+    if (tid == 0) *full = 0u;
+    __syncthreads();
+    TileSpan s = tile_span(b, b.in, b.n_in, t, C);
+    P rec[R];
+    uint32_t bin[R], slot[R], rank[R];
+#pragma unroll
+    for (int r = 0; r < R; ++r) {
+        const uint32_t i = w * 64u * R + (uint32_t)r * 64u + lane;
+        bool v = i < s.count;
+        bin[r] = 0xFFFFFFFFu;
+        slot[r] = kNoSlot;
+        if (v) {
+            rec[r] = P::load(cols, s.base + i);
+            bin[r] = classify(nd, rec[r]);
+            if (bin[r] >= B) {
+                set_status(status, GF_INVALID);
+                v = false;
+            } else {
+                slot[r] = hash_insert(keys, mask, bin[r]);
+                if (slot[r] == kNoSlot) {
+                    *full = 1u;
+                    v = false;
+                }
+            }
+            if (!v) bin[r] = 0xFFFFFFFFu;
+        }
+        const uint64_t peers = peer_mask_bits(v, slot[r], 32u - (uint32_t)__clz((int)(E - 1u)));
+        const uint32_t cur = v ? wcnt[w * E + slot[r]] : 0u;
+        rank[r] = cur + (uint32_t)__popcll(peers & lanemask_lt());
+        if (v && (peers & lanemask_lt()) == 0ull) wcnt[w * E + slot[r]] = cur + (uint32_t)__popcll(peers);
+    }
+    __syncthreads();
+    if (tid == 0 && *full) set_status(status, kStatusTableFull);
+    for (uint32_t k = tid; k < E; k += kThreads) {
+        uint32_t acc = 0;
+        for (int ww = 0; ww < kWaves; ++ww) {
+            const uint32_t x = wcnt[ww * E + k];
+            wcnt[ww * E + k] = acc;
+            acc += x;
+        }
+    }
+    __syncthreads();
+    const uint32_t* toff = b.counts + (size_t)(t == kNoTile ? 0u : t) * B;
+#pragma unroll
+    for (int r = 0; r < R; ++r) {
+        if (bin[r] == 0xFFFFFFFFu) continue;
+        const uint32_t k = bin[r];
+        const uint32_t f = toff[k] + (bin_base ? bin_base[k] * C : 0u) + wcnt[w * E + slot[r]] + rank[r];
+        P p = rec[r];
+        const uint32_t u = update(nd, k, p);
+        if (u) set_status(status, u);
+        if (fuse && fuse[k] >= 0) transform(all[fuse[k]], *rp, 2u * (uint32_t)fuse[k], p);
+        p.store(cols, (size_t)b.out[f / C].index * C + f % C);
+    }
+    __syncthreads();
+}
+
 // LDS bytes of scatter_tile.
 __host__ __device__ inline size_t scatter_smem(uint32_t bins, uint32_t tile_items, uint32_t rec_bytes, bool stage)
 {

@@ -49,6 +49,8 @@ struct NodeArr {
     uint32_t real;   // the graph node this lane belongs to (trace records, stream purpose = 2 * real)
     uint32_t lanes;  // first lane of a node: the node's lanes (it stages candidates for all of them)
     uint32_t ring;   // first lane of a node: index of the node's ready ring in `rings`
+    uint32_t hash_entries;  // partitions on the hash bin path: slots of a tile's LDS bin table; 0: dense path
+    uint32_t pad;
 };
 
 enum { NODE_STAGE = 1u, NODE_WAVE_SINK = 2u };

@@ -75,6 +75,12 @@ uint64_t output_chunk_bound(uint64_t n, uint64_t C, uint64_t B)
     return r < n ? r : n;
 }
 
+template <class K>
+void set_lds(K kernel, size_t bytes)
+{
+    if (bytes > 64u * 1024u) HIPX(hipFuncSetAttribute((const void*)kernel, hipFuncAttributeMaxDynamicSharedMemorySize, (int)bytes));
+}
+
 bool is_partition(const gf_node_desc& nd) { return nd.op >= GF_CLS_WEIGHT_ROUTE && nd.op < GF_XF_SCALE_WEIGHT; }
 
 uint32_t reserve_bound_of(const gf_node_desc& nd, uint32_t C, uint32_t Q)
@@ -82,6 +88,37 @@ uint32_t reserve_bound_of(const gf_node_desc& nd, uint32_t C, uint32_t Q)
     uint64_t b = output_chunk_bound((uint64_t)C * Q, C, is_partition(nd) ? nd.bins : 1u);
     if (b > UINT32_MAX) fail(GF_OVERFLOW, "partition batch output-chunk bound overflow");
     return (uint32_t)b;
+}
+
+// Bin path of a partition (core/BinPath.hpp). The dense path keeps 24 bytes of LDS per bin in a
+// scatter tile, which on MI355X's 160 KB LDS holds up to 4096 bins; automatic selects the hash
+// path above that (the reference's limit, Config.hpp:174, is derived from its GPUs' shared memory).
+constexpr uint32_t kDenseBinLimit = 4096u;
+constexpr uint32_t kHashTileLimit = 1024u;  // hash path tiles: a tile's table of 2 * 1024 slots fits LDS
+
+uint32_t next_pow2(uint32_t v)
+{
+    uint32_t r = 1;
+    while (r < v) r <<= 1;
+    return r;
+}
+
+bool on_hash_path(const gf_node_desc& nd, uint32_t bin_path)
+{
+    const bool part = nd.role == GF_ROLE_TRANSFORM && nd.op >= GF_CLS_WEIGHT_ROUTE && nd.op < GF_XF_SCALE_WEIGHT;
+    if (!part) return false;
+    if (bin_path == GF_BIN_PATH_DENSE && nd.bins > kDenseBinLimit)
+        fail(GF_BAD_ARGUMENT, "graphFlow::Config: the dense bin path supports at most " + std::to_string(kDenseBinLimit) +
+                                  " bins; select the hash path");
+    return bin_path == GF_BIN_PATH_HASH || (bin_path == GF_BIN_PATH_AUTO && nd.bins > kDenseBinLimit);
+}
+
+// Slots of a hash-path tile's bin table (HashBinTable, BinTable.hpp:69-82): nextPow2 of the
+// configured entries, or of two per tile item; 0 on the dense path.
+uint32_t hash_entries_of(const gf_node_desc& nd, uint32_t bin_path, uint32_t configured, uint32_t tile_items)
+{
+    if (!on_hash_path(nd, bin_path)) return 0u;
+    return next_pow2(configured ? configured : 2u * tile_items);
 }
 
 uint32_t payload_bytes(uint32_t payload)
@@ -103,20 +140,22 @@ std::vector<uint32_t> column_bytes(uint32_t payload)
 // ==============================================================================================
 template <class P, int R>
 __global__ __launch_bounds__(kThreads) void k_count(gf_node_desc nd, const Batch* bp, Cols cols, uint32_t C,
-                                                    uint32_t* status)
+                                                    uint32_t* status, uint32_t hash_entries)
 {
     extern __shared__ char smem[];
     const Batch b = *bp;
-    count_tile<P, R>(nd, b, blockIdx.x, cols, C, status, smem);
+    if (hash_entries) count_tile_hash<P, R>(nd, b, blockIdx.x, cols, C, status, smem, hash_entries);
+    else count_tile<P, R>(nd, b, blockIdx.x, cols, C, status, smem);
 }
 
 template <class P, int R, bool Stage>
 __global__ __launch_bounds__(kThreads) void k_scatter(gf_node_desc nd, const Batch* bp, Cols cols, uint32_t C,
-                                                      uint32_t* status)
+                                                      uint32_t* status, uint32_t hash_entries)
 {
     extern __shared__ char smem[];
     const Batch b = *bp;
-    scatter_tile<P, R, Stage>(nd, b, blockIdx.x, cols, C, status, smem);
+    if (hash_entries) scatter_tile_hash<P, R>(nd, b, blockIdx.x, cols, C, status, smem, hash_entries);
+    else scatter_tile<P, R, Stage>(nd, b, blockIdx.x, cols, C, status, smem);
 }
 
 template <class P, int R>
@@ -173,7 +212,9 @@ __global__ __launch_bounds__(kThreads) void k_engine_ka(Engine e)
         const uint32_t k = find_node(e.ka_prefix, e.n_nodes, t);
         const NodeSt s = e.st[k];
         const Batch b = make_batch(e, k, s);
-        count_tile<P, R>(e.nodes[k], b, t - e.ka_prefix[k], e.cols, e.C, &e.ctl[CTL_STATUS], smem);
+        const uint32_t E = e.arr[k].hash_entries;
+        if (E) count_tile_hash<P, R>(e.nodes[k], b, t - e.ka_prefix[k], e.cols, e.C, &e.ctl[CTL_STATUS], smem, E);
+        else count_tile<P, R>(e.nodes[k], b, t - e.ka_prefix[k], e.cols, e.C, &e.ctl[CTL_STATUS], smem);
     }
 }
 
@@ -222,7 +263,10 @@ __global__ __launch_bounds__(kThreads) void k_engine_kb(Engine e)
             sink_tile<P>(nd, b, lt, e.cols, e.C, &e.ctl[CTL_STATUS], smem);
         } else if (kind == KIND_PARTITION) {
             const int32_t* fz = e.fuse + nd.route_offset;
-            if (e.arr[k].flags & NODE_STAGE)
+            if (e.arr[k].hash_entries)
+                scatter_tile_hash<P, R>(nd, b, lt, e.cols, e.C, &e.ctl[CTL_STATUS], smem, e.arr[k].hash_entries,
+                                        e.real_nodes, fz, &e.rp, b.totals + nd.bins);
+            else if (e.arr[k].flags & NODE_STAGE)
                 scatter_tile<P, R, true>(nd, b, lt, e.cols, e.C, &e.ctl[CTL_STATUS], smem, e.real_nodes, fz, &e.rp, b.totals + nd.bins);
             else
                 scatter_tile<P, R, false>(nd, b, lt, e.cols, e.C, &e.ctl[CTL_STATUS], smem, e.real_nodes, fz, &e.rp, b.totals + nd.bins);
@@ -284,7 +328,9 @@ __global__ __launch_bounds__(1024) void k_micro(Engine e, uint32_t max_steps, ui
             const Batch b = make_batch(e, k, s);
             for (uint32_t t0 = 0; t0 < s.n_tiles; t0 += G) {
                 const uint32_t t = t0 + g;
-                count_tile<P, R>(e.nodes[k], b, t < s.n_tiles ? t : kNoTile, e.cols, e.C, &e.ctl[CTL_STATUS], gsm);
+                const uint32_t E = e.arr[k].hash_entries;
+                if (E) count_tile_hash<P, R>(e.nodes[k], b, t < s.n_tiles ? t : kNoTile, e.cols, e.C, &e.ctl[CTL_STATUS], gsm, E);
+                else count_tile<P, R>(e.nodes[k], b, t < s.n_tiles ? t : kNoTile, e.cols, e.C, &e.ctl[CTL_STATUS], gsm);
             }
         }
         __syncthreads();
@@ -308,7 +354,10 @@ __global__ __launch_bounds__(1024) void k_micro(Engine e, uint32_t max_steps, ui
                     sink_tile<P>(nd, b, t, e.cols, e.C, &e.ctl[CTL_STATUS], gsm);
                 } else if (kind == KIND_PARTITION) {
                     const int32_t* fz = e.fuse + nd.route_offset;
-                    if (stage)
+                    if (e.arr[k].hash_entries)
+                        scatter_tile_hash<P, R>(nd, b, t, e.cols, e.C, &e.ctl[CTL_STATUS], gsm, e.arr[k].hash_entries,
+                                                e.real_nodes, fz, &e.rp, b.totals + nd.bins);
+                    else if (stage)
                         scatter_tile<P, R, true>(nd, b, t, e.cols, e.C, &e.ctl[CTL_STATUS], gsm, e.real_nodes, fz, &e.rp, b.totals + nd.bins);
                     else
                         scatter_tile<P, R, false>(nd, b, t, e.cols, e.C, &e.ctl[CTL_STATUS], gsm, e.real_nodes, fz, &e.rp, b.totals + nd.bins);
@@ -602,7 +651,8 @@ struct gf_stage {
     uint64_t first_id = 0;
     bool failed = false, launched = false;
     uint64_t launches = 0;
-    uint32_t bin_path = GF_BIN_PATH_AUTO, hash_entries = 0, stall_ms = 0;
+    uint32_t bin_path = GF_BIN_PATH_AUTO, hash_entries = 0, stall_ms = 0, hash_E = 0;
+    size_t lds_count = 0, lds_scatter = 0;
     std::string err;
 
     ~gf_stage()
@@ -630,8 +680,21 @@ struct gf_stage {
         key = k;
         seed = sd;
         Q = q ? q : 1u;
-        r = effective_r(p->payload, pick_tile_r(ti ? ti : 1024u));
+        uint32_t want = ti ? ti : 1024u;
+        if (on_hash_path(d, bin_path)) want = std::min(want, kHashTileLimit);
+        r = effective_r(p->payload, pick_tile_r(want));
         tile_items = 256u * r;
+        hash_E = hash_entries_of(d, bin_path, hash_entries, tile_items);
+        const uint32_t rec = payload_bytes(p->payload);
+        if (kind() == 1) {
+            const size_t lc = hash_E ? count_hash_smem(hash_E) : (size_t)std::max<uint32_t>(d.bins, 1u) * 4u;
+            const size_t ls = hash_E ? scatter_hash_smem(hash_E)
+                                     : scatter_smem(d.bins, tile_items, rec, stage_fits(d.bins, tile_items, rec));
+            if (lc > 160u * 1024u || ls > 160u * 1024u)
+                fail(GF_BAD_ARGUMENT, "graphFlow: partition width or hash table exceeds the workgroup LDS budget");
+            lds_count = lc;
+            lds_scatter = ls;
+        }
         bins = nd.role == GF_ROLE_SINK ? 1u : std::max<uint32_t>(nd.bins, 1u);
         if (kind() == 1 && nd.bins == 0) fail(GF_BAD_ARGUMENT, "graphFlow: a partition needs at least one bin");
         cap_in = nd.role == GF_ROLE_SOURCE ? 1u : Q;
@@ -710,6 +773,10 @@ struct gf_stage {
     void check_status(const char* what)
     {
         uint32_t s = *h_status;
+        if (s == kStatusTableFull) {
+            failed = true;
+            fail(GF_INVALID, std::string(what) + ": hash bin table full");
+        }
         if (s != 0u) {
             failed = true;
             const char* name = s == GF_INVALID ? "invalid" : s == GF_OVERFLOW ? "overflow" : "invalid";
@@ -808,8 +875,9 @@ struct gf_stage {
             upload_batch(st, 0);
             with_payload(pool->payload, r, [&](auto pt, auto rc) {
                 using P = typename decltype(pt)::type;
-                hipLaunchKernelGGL((k_count<P, decltype(rc)::value>), dim3(n_tiles), dim3(kThreads),
-                                   std::max<uint32_t>(bins, 1u) * 4u, st, nd, d_batch, cols, C, d_status);
+                set_lds(k_count<P, decltype(rc)::value>, lds_count);
+                hipLaunchKernelGGL((k_count<P, decltype(rc)::value>), dim3(n_tiles), dim3(kThreads), lds_count, st, nd,
+                                   d_batch, cols, C, d_status, hash_E);
             });
             HIPX(hipMemcpyAsync(h_counts, h_batch->counts, sizeof(uint32_t) * (size_t)n_tiles * bins,
                                 hipMemcpyDeviceToHost, st));
@@ -909,17 +977,20 @@ struct gf_stage {
         upload_batch(st, h_batch->n_out);
         reset_status(st);
         Cols cols = pool->dcols();
-        bool stage = stage_fits(B, tile_items, payload_bytes(pool->payload));
-        size_t lds = scatter_smem(B, tile_items, payload_bytes(pool->payload), stage);
+        const bool stage = !hash_E && stage_fits(B, tile_items, payload_bytes(pool->payload));
+        const size_t lds = lds_scatter;
         with_payload(pool->payload, r, [&](auto pt, auto rc) {
             using P = typename decltype(pt)::type;
             constexpr int RR = decltype(rc)::value;
-            if (stage)
+            if (stage) {
+                set_lds(k_scatter<P, RR, true>, lds);
                 hipLaunchKernelGGL((k_scatter<P, RR, true>), dim3(n_tiles), dim3(kThreads), lds, st, nd, d_batch, cols, C,
-                                   d_status);
-            else
+                                   d_status, hash_E);
+            } else {
+                set_lds(k_scatter<P, RR, false>, lds);
                 hipLaunchKernelGGL((k_scatter<P, RR, false>), dim3(n_tiles), dim3(kThreads), lds, st, nd, d_batch, cols,
-                                   C, d_status);
+                                   C, d_status, hash_E);
+            }
         });
         HIPX(hipGetLastError());
         HIPX(hipMemcpyAsync(h_status, d_status, sizeof(uint32_t), hipMemcpyDeviceToHost, st));
@@ -1033,6 +1104,8 @@ RunShape resolve(const gf_flow& f, uint64_t items)
     // autotuned tile: a workgroup tile must amortise its [tile][bin] metadata for wide partitions
     uint32_t ti = f.tuning.tile_items;
     if (!ti) ti = maxb >= 64u ? 4096u : 1024u;
+    for (const auto& n : f.nodes)
+        if (on_hash_path(n.nd, f.tuning.bin_path)) ti = std::min(ti, kHashTileLimit);
     s.tile_items = 256u * effective_r(f.payload, pick_tile_r(ti));
     return s;
 }
@@ -1319,7 +1392,9 @@ std::string shape_key(const gf_flow& f, const RunShape& s, uint64_t N)
                     std::to_string(f.tuning.trace) + "/" +
                     std::to_string(f.tuning.steps_per_launch) + "/" + std::to_string(f.tuning.fuse_transforms) + "/" +
                     std::to_string(f.tuning.max_workgroups) + "/" + std::to_string(f.tuning.device_engine) + "/" +
-                    std::to_string(f.tuning.use_graph) + "/" + std::to_string(f.tuning.team_xcd);
+                    std::to_string(f.tuning.use_graph) + "/" + std::to_string(f.tuning.team_xcd) + "/" +
+                    std::to_string(f.tuning.inflight_batches) + "/" + std::to_string(f.tuning.bin_path) + "/" +
+                    std::to_string(f.tuning.hash_table_entries);
     {
         const PhotonParams ph = photon_params_of(f.tuning);
         char buf[160];
@@ -1339,11 +1414,6 @@ std::string shape_key(const gf_flow& f, const RunShape& s, uint64_t N)
     return k;
 }
 
-template <class K>
-void set_lds(K kernel, size_t bytes)
-{
-    if (bytes > 64u * 1024u) HIPX(hipFuncSetAttribute((const void*)kernel, hipFuncAttributeMaxDynamicSharedMemorySize, (int)bytes));
-}
 
 // Stream-ordered host<->device copy on the flow's stream, then wait for it: never touches the
 // legacy stream, so it is safe while another thread captures a graph on its own stream.
@@ -1449,11 +1519,18 @@ void build_engine(gf_flow& f, uint64_t N, const RunShape& sh, uint32_t src)
             a.purpose = 2u * k;
             if (wave_sink[k]) a.flags |= NODE_WAVE_SINK;
             if (part) {
-                lds_a = std::max(lds_a, (size_t)B * 4u);
-                bool stg = stage_fits(B, sh.tile_items, rec);
-                if (stg) a.flags |= NODE_STAGE;
-                lds_b = std::max(lds_b, scatter_smem(B, sh.tile_items, rec, stg));
-                group_lds = std::max(group_lds, std::max((size_t)B * 4u, scatter_smem(B, sh.tile_items, rec, stg)));
+                a.hash_entries = hash_entries_of(nd, t.bin_path, t.hash_table_entries, sh.tile_items);
+                if (a.hash_entries) {
+                    lds_a = std::max(lds_a, count_hash_smem(a.hash_entries));
+                    lds_b = std::max(lds_b, scatter_hash_smem(a.hash_entries));
+                    group_lds = std::max(group_lds, std::max(count_hash_smem(a.hash_entries), scatter_hash_smem(a.hash_entries)));
+                } else {
+                    lds_a = std::max(lds_a, (size_t)B * 4u);
+                    bool stg = stage_fits(B, sh.tile_items, rec);
+                    if (stg) a.flags |= NODE_STAGE;
+                    lds_b = std::max(lds_b, scatter_smem(B, sh.tile_items, rec, stg));
+                    group_lds = std::max(group_lds, std::max((size_t)B * 4u, scatter_smem(B, sh.tile_items, rec, stg)));
+                }
                 grid_tiles += cap_tiles;
             } else if (nd.role != GF_ROLE_SINK) {
                 grid_tiles += cap_tiles;
@@ -1813,6 +1890,7 @@ void finish_device(gf_flow& f, EngineBuffers& eb, gf_stats& st, const uint32_t* 
     st.kernel_launches = launches;
     f.part_items = ctr[CNT_PART_ITEMS];
     f.xf_items = ctr[CNT_XF_ITEMS];
+    if (ctl[CTL_STATUS] == kStatusTableFull) fail(GF_INVALID, "partition classify: hash bin table full");
     if (ctl[CTL_STATUS]) {
         gf_status code = (gf_status)ctl[CTL_STATUS];
         const char* what = code == GF_INVALID ? "partition: device status reports an invalid classifier or updater result"
@@ -1971,6 +2049,37 @@ gf_status gf_pool_counts(gf_pool* pool, uint32_t counts[6])
     return GF_OK;
 }
 
+gf_status gf_pool_lease_count(gf_pool* pool, gf_chunk_handle h, uint32_t* count)
+{
+    if (!pool || !count) return GF_BAD_ARGUMENT;
+    return guard(&pool->ctx->err, [&] {
+        if (h.index >= pool->S) fail(GF_BAD_ARGUMENT, "graphFlow::ChunkPool: chunk index out of range");
+        *count = pool->lease[h.index];
+    });
+}
+
+gf_status gf_pool_is_live(gf_pool* pool, gf_chunk_handle h, int* live)
+{
+    if (!pool || !live) return GF_BAD_ARGUMENT;
+    *live = h.index < pool->S && pool->gen[h.index] == h.generation ? 1 : 0;
+    return GF_OK;
+}
+
+gf_status gf_pool_generation(gf_pool* pool, uint32_t index, uint32_t* generation)
+{
+    if (!pool || !generation) return GF_BAD_ARGUMENT;
+    return guard(&pool->ctx->err, [&] {
+        if (index >= pool->S) fail(GF_BAD_ARGUMENT, "graphFlow::ChunkPool: chunk index out of range");
+        *generation = pool->gen[index];
+    });
+}
+
+gf_status gf_pool_validate_live(gf_pool* pool, gf_chunk_handle h)
+{
+    if (!pool) return GF_BAD_ARGUMENT;
+    return guard(&pool->ctx->err, [&] { pool->validate(h); });
+}
+
 gf_status gf_pool_base(gf_pool* pool, gf_chunk_handle h, uint64_t* slot)
 {
     if (!pool || !slot) return GF_BAD_ARGUMENT;
@@ -2005,6 +2114,19 @@ gf_status gf_pool_read(gf_pool* pool, uint32_t column, uint64_t first, void* hos
     });
 }
 
+gf_status gf_stage_create_tuned(gf_pool* pool, const gf_node_desc* node, uint32_t node_key, const gf_tuning* tuning,
+                                gf_stage** out)
+{
+    if (!pool || !node || !tuning || !out) return GF_BAD_ARGUMENT;
+    return guard(&pool->ctx->err, [&] {
+        if (tuning->bin_path > GF_BIN_PATH_HASH) fail(GF_BAD_ARGUMENT, "graphFlow::Tuning: unknown bin path");
+        auto s = std::make_unique<gf_stage>();
+        s->init(pool, *node, node_key, tuning->seed, tuning->batch_chunks, tuning->tile_items, tuning);
+        s->rp.ph = photon_params_of(*tuning);
+        *out = s.release();
+    });
+}
+
 gf_status gf_stage_create(gf_pool* pool, const gf_node_desc* node, uint32_t node_key, uint64_t seed,
                           uint32_t batch_chunks, uint32_t tile_items, gf_stage** out)
 {
@@ -2030,6 +2152,13 @@ gf_status gf_stage_launch(gf_stage* stage, const gf_chunk_desc* chunks, uint32_t
     return guard(&stage->err, [&] {
         stage->launch(chunks, n, first_id, stream ? (hipStream_t)stream : stage->pool->ctx->stream);
     });
+}
+
+gf_status gf_stage_last_error(gf_stage* stage, char* buffer, uint32_t size)
+{
+    if (!stage || !buffer || !size) return GF_BAD_ARGUMENT;
+    std::snprintf(buffer, size, "%s", stage->err.c_str());
+    return GF_OK;
 }
 
 gf_status gf_stage_advance(gf_stage* stage, void* stream, uint32_t* allocated, int* more_work)

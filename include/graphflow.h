@@ -6,7 +6,8 @@
  * docs/architecture.md:89-92); device work is stream ordered.
  *
  * Reference interfaces replaced (paths relative to the reference repository):
- *   gf_pool_*        ChunkStorage concept      include/graphFlow/core/ChunkStorage.hpp:35-54,
+ *   gf_pool_*        ChunkStorage concept      include/graphFlow/core/ChunkStorage.hpp:35-54
+ *                    (leaseCount/isLive/generation/validateLive: core/ChunkLedger.hpp:152-187),
  *                    ChunkPool / ChunkLedger   include/graphFlow/core/ChunkPool.hpp:155-269,
  *                                              include/graphFlow/core/ChunkLedger.hpp:83-188
  *   gf_stage_*       StageBase hooks           include/graphFlow/flow/Execution.hpp:1537-1561
@@ -167,6 +168,14 @@ gf_status gf_pool_release_lease(gf_pool* pool, gf_chunk_handle handle);
 /* counts[0..5] = numChunks, chunkCapacity, freeCount, liveCount, peakLive, peakLeases */
 gf_status gf_pool_counts(gf_pool* pool, uint32_t counts[6]);
 gf_status gf_pool_base(gf_pool* pool, gf_chunk_handle handle, uint64_t* slot);
+/* leaseCount(handle): reader leases of the chunk; GF_BAD_ARGUMENT for an index out of range */
+gf_status gf_pool_lease_count(gf_pool* pool, gf_chunk_handle handle, uint32_t* count);
+/* isLive(handle): 1 when the index is in range and the generation matches, else 0 */
+gf_status gf_pool_is_live(gf_pool* pool, gf_chunk_handle handle, int* live);
+/* generation(chunkIndex): the chunk's current generation; GF_BAD_ARGUMENT out of range */
+gf_status gf_pool_generation(gf_pool* pool, uint32_t index, uint32_t* generation);
+/* validateLive(handle): GF_OK, or GF_BAD_ARGUMENT ("chunk index out of range" / "stale chunk handle") */
+gf_status gf_pool_validate_live(gf_pool* pool, gf_chunk_handle handle);
 gf_status gf_pool_column(gf_pool* pool, uint32_t column, void** device_ptr, uint32_t* elem_bytes);
 gf_status gf_pool_write(gf_pool* pool, uint32_t column, uint64_t first_slot, const void* host, uint64_t count);
 gf_status gf_pool_read(gf_pool* pool, uint32_t column, uint64_t first_slot, void* host, uint64_t count);
@@ -180,11 +189,18 @@ gf_status gf_pool_read(gf_pool* pool, uint32_t column, uint64_t first_slot, void
  * abort:   release unpublished outputs; the stage is single-use afterwards. */
 gf_status gf_stage_create(gf_pool* pool, const gf_node_desc* node, uint32_t node_key, uint64_t seed,
                           uint32_t batch_chunks, uint32_t tile_items, gf_stage** out);
+/* The same with the run's Tuning: seed, batch_chunks (Q), tile_items, bin_path, hash_table_entries,
+ * stall_timeout_ms and the photon slab (a reference stage is built from the resolved Config,
+ * flow/Flow.hpp:328-412). */
+gf_status gf_stage_create_tuned(gf_pool* pool, const gf_node_desc* node, uint32_t node_key, const gf_tuning* tuning,
+                                gf_stage** out);
 gf_status gf_stage_destroy(gf_stage* stage);
 gf_status gf_stage_launch(gf_stage* stage, const gf_chunk_desc* chunks, uint32_t n, uint64_t first_id, void* stream);
 gf_status gf_stage_advance(gf_stage* stage, void* stream, uint32_t* allocated, int* more_work);
 gf_status gf_stage_finish(gf_stage* stage, gf_chunk_desc* outputs, uint32_t cap, uint32_t* n_out);
 gf_status gf_stage_abort(gf_stage* stage);
+/* message of the stage's last failed call */
+gf_status gf_stage_last_error(gf_stage* stage, char* buffer, uint32_t size);
 /* per-bin totals of the last classify (bins entries) */
 gf_status gf_stage_counts(gf_stage* stage, uint32_t* counts, uint32_t bins);
 /* tile descriptors of the last launch, flattened on the device (TileDescriptor list) */

@@ -746,10 +746,11 @@ static uint32_t reserve_bound_of(const or_node_desc* nd, uint32_t C, uint32_t Q)
 
 static int is_partition(const or_node_desc* nd) { return nd->op >= OR_CLS_WEIGHT_ROUTE && nd->op < OR_XF_SCALE_WEIGHT; }
 
-/* Bin path of a partition (BinPath.hpp): hash above 1024 bins unless the run fixes one. */
+/* Bin path of a partition (BinPath.hpp): hash above 4096 bins unless the run fixes one (the
+ * build's dense limit: 24 bytes of LDS per bin in a scatter tile of MI355X's 160 KB). */
 static int uses_hash_path(const or_run_cfg* cfg, const or_node_desc* nd)
 {
-    return cfg->bin_path == 2u || (cfg->bin_path == 0u && nd->bins > 1024u);
+    return cfg->bin_path == 2u || (cfg->bin_path == 0u && nd->bins > 4096u);
 }
 
 /* Hash-path classify check (Execution.hpp:1207-1307): every tile of hash_tile items of every input

@@ -127,7 +127,7 @@ typedef struct or_run_cfg {
     uint32_t tally_bins;
     double layer_mfp, mu_t, albedo, hg_g, roulette_w, roulette_p;
     uint32_t inflight;     /* Tuning::inflightBatches: batches in flight per node (0 -> 1)          */
-    uint32_t bin_path;     /* BinPath: 0 automatic (hash above 1024 bins), 1 dense, 2 hash           */
+    uint32_t bin_path;     /* BinPath: 0 automatic (hash above 4096 bins), 1 dense, 2 hash           */
     uint32_t hash_tile;    /* hash path: items per classify tile (the device's tile size)            */
     uint32_t hash_entries; /* hash path: slots of a tile's bin table (0 -> nextPow2(2 * tile), never full) */
 } or_run_cfg;
