@@ -95,7 +95,7 @@ class TuningC(C.Structure):
                 ("photon_layer_mfp", C.c_double), ("photon_mu_t", C.c_double), ("photon_albedo", C.c_double),
                 ("photon_hg_g", C.c_double), ("photon_roulette_w", C.c_double), ("photon_roulette_p", C.c_double),
                 ("team_xcd", C.c_uint32), ("inflight_batches", C.c_uint32), ("bin_path", C.c_uint32),
-                ("hash_table_entries", C.c_uint32), ("stall_timeout_ms", C.c_uint32), ("reserved1", C.c_uint32)]
+                ("hash_table_entries", C.c_uint32), ("stall_timeout_ms", C.c_uint32), ("fused_batches", C.c_uint32)]
 
 
 class Stats(C.Structure):
@@ -132,6 +132,7 @@ SIGNATURES = {
     "gf_stage_create": (C.c_int, [_V, _P(NodeDesc), C.c_uint32, C.c_uint64, C.c_uint32, C.c_uint32, _P(_V)]),
     "gf_stage_create_tuned": (C.c_int, [_V, _P(NodeDesc), C.c_uint32, _P(TuningC), _P(_V)]),
     "gf_stage_last_error": (C.c_int, [_V, C.c_char_p, C.c_uint32]),
+    "gf_transform_batch": (C.c_int, [_V, _P(ChunkDesc), C.c_uint32, C.c_uint64, _V]),
     "gf_pool_lease_count": (C.c_int, [_V, ChunkHandle, _P(C.c_uint32)]),
     "gf_pool_is_live": (C.c_int, [_V, ChunkHandle, _P(C.c_int)]),
     "gf_pool_generation": (C.c_int, [_V, C.c_uint32, _P(C.c_uint32)]),
@@ -351,6 +352,11 @@ class Stage:
         arr = (ChunkDesc * len(chunks))(*[ChunkDesc(*c) for c in chunks])
         self._check(lib().gf_stage_launch(self.handle, arr, len(chunks), first_id, None))
 
+    def transform_batch(self, chunks: Sequence[tuple], first_id: int = 0):
+        """One launch for the whole batch (gf_transform_batch); then one advance() and finish()."""
+        arr = (ChunkDesc * len(chunks))(*[ChunkDesc(*c) for c in chunks])
+        self._check(lib().gf_transform_batch(self.handle, arr, len(chunks), first_id, None))
+
     def advance(self):
         a, m = C.c_uint32(), C.c_int()
         self._check(lib().gf_stage_advance(self.handle, None, C.byref(a), C.byref(m)))
@@ -445,9 +451,12 @@ def IdMod(bins: int, distribution: int = 0, update: str = "NONE") -> Op:
 
 
 def Lut(bins: int, table_ptr: int, nid: int, ngen: int, offset: int = 0, update: str = "NONE", add_bin: int = 0,
-        add: float = 0.0) -> Op:
+        add: float = 0.0, table_len: Optional[int] = None) -> Op:
+    """bin = table[offset + min(id, nid-1) * ngen + min(gen, ngen-1)]; table_len: u32 entries of the
+    table (default offset + nid * ngen, the entries this classifier reads)."""
+    n = offset + nid * ngen if table_len is None else table_len
     return Op(ROLE_TRANSFORM, "CLS_LUT", bins=bins, upd=update, u=[nid, ngen, add_bin, offset], f=[0.0, add],
-              ptr=[table_ptr])
+              ptr=[table_ptr], size=[n])
 
 
 def PhotonAlive() -> Op:
@@ -458,8 +467,9 @@ def ScaleWeight(m: float) -> Op:
     return Op(ROLE_TRANSFORM, "XF_SCALE_WEIGHT", f=[m])
 
 
-def PhotonStep(tally_ptr: int) -> Op:
-    return Op(ROLE_TRANSFORM, "XF_PHOTON_STEP", ptr=[tally_ptr])
+def PhotonStep(tally_ptr: int, tally_bins: int = 1024) -> Op:
+    """tally_bins: f64 entries of the tally buffer (at least the run's photon tally bins)."""
+    return Op(ROLE_TRANSFORM, "XF_PHOTON_STEP", ptr=[tally_ptr], size=[tally_bins])
 
 
 def WeightHistogram(counts_ptr: int, lo: int, n: int, field_: int = 0) -> Op:
@@ -513,6 +523,7 @@ class Tuning:
     bin_path: int = 0  # BIN_PATH_AUTO (hash above 1024 bins), BIN_PATH_DENSE, BIN_PATH_HASH
     hash_table_entries: Optional[int] = None  # hash path: slots of a tile's bin table (None: 2 * tile items)
     stall_timeout_ms: Optional[int] = None  # plain schedule: longest wait for a device stage (Tuning::stallTimeout)
+    fused_batches: bool = False  # plain schedule: one launch per partition batch (gf_transform_batch)
 
     def validate(self):
         for name in ("batch_chunks", "chunk_capacity", "live_particles", "num_chunks", "tile_items",
@@ -530,7 +541,8 @@ class Tuning:
                     max_workgroups=self.max_workgroups or 0, device_engine=self.device_engine,
                     team_xcd=0 if self.team_xcd is None else self.team_xcd + 1,
                     inflight_batches=self.inflight_batches or 0, bin_path=self.bin_path,
-                    hash_table_entries=self.hash_table_entries or 0, stall_timeout_ms=self.stall_timeout_ms or 0)
+                    hash_table_entries=self.hash_table_entries or 0, stall_timeout_ms=self.stall_timeout_ms or 0,
+                    fused_batches=1 if self.fused_batches else 0)
         ph = self.photon or {}
         t.photon_layers = ph.get("layers", 10)
         t.photon_tally_bins = ph.get("tally_bins", 1024)

@@ -21,6 +21,7 @@
 #include <vector>
 
 #include "gf_team.hpp"
+#include "gf_batch.hpp"
 
 using namespace gfd;
 
@@ -121,6 +122,27 @@ uint32_t hash_entries_of(const gf_node_desc& nd, uint32_t bin_path, uint32_t con
     return next_pow2(configured ? configured : 2u * tile_items);
 }
 
+// bytes of the record struct a kernel holds per item (LDS staging), not the payload's HBM bytes
+uint32_t record_bytes(uint32_t payload)
+{
+    return payload == GF_PAYLOAD_PARTICLE ? (uint32_t)sizeof(gfd::Particle)
+         : payload == GF_PAYLOAD_KEYED ? (uint32_t)sizeof(gfd::Keyed) : (uint32_t)sizeof(gfd::Photon);
+}
+
+// Device buffers a node's functor reads or writes must hold what it indexes: a kernel never checks
+// these bounds itself.
+void validate_node(const gf_node_desc& nd, uint32_t tally_bins)
+{
+    if (nd.op == GF_CLS_LUT) {
+        if (nd.u[0] == 0u || nd.u[1] == 0u) fail(GF_BAD_ARGUMENT, "graphFlow: a table classifier needs ids and generations");
+        if (!nd.ptr[0]) fail(GF_BAD_ARGUMENT, "graphFlow: a table classifier needs its table");
+        if ((uint64_t)nd.u[3] + (uint64_t)nd.u[0] * nd.u[1] > nd.size[0])
+            fail(GF_BAD_ARGUMENT, "graphFlow: the table classifier reads past its table (size[0])");
+    }
+    if (nd.op == GF_XF_PHOTON_STEP && (!nd.ptr[0] || nd.size[0] < tally_bins))
+        fail(GF_BAD_ARGUMENT, "graphFlow: the photon step's tally buffer (size[0]) holds fewer than the run's tally bins");
+}
+
 uint32_t payload_bytes(uint32_t payload)
 {
     return payload == GF_PAYLOAD_PARTICLE ? 12u : payload == GF_PAYLOAD_KEYED ? 12u : 36u;
@@ -181,6 +203,14 @@ __global__ __launch_bounds__(kThreads) void k_sink(gf_node_desc nd, const Batch*
     extern __shared__ char smem[];
     const Batch b = *bp;
     sink_tile<P>(nd, b, blockIdx.x, cols, C, status, smem);
+}
+
+// One partition batch in one launch (gf_batch.hpp).
+template <class P, int R>
+__global__ __launch_bounds__(1024) void k_batch_fused(FusedArgs a, Cols cols)
+{
+    extern __shared__ char smem[];
+    fused_partition_batch<P, R>(a, cols, smem);
 }
 
 // Tile descriptors of a batch as the device computes them (TileDescriptor list).
@@ -521,6 +551,15 @@ struct gf_pool {
         gen[h.index] += 1u;
         free_list.push_back(h.index);
     }
+    // Return a reserved, never published chunk: back on the free list without a generation bump,
+    // so a reservation of k chunks of which the first r were used leaves the ledger as if exactly
+    // those r had been acquired (call in reverse acquisition order).
+    void cancel(gf_chunk_handle h)
+    {
+        validate(h);
+        if (lease[h.index]) fail(GF_BAD_ARGUMENT, "graphFlow::ChunkPool: cancel while a reader lease is outstanding");
+        free_list.push_back(h.index);
+    }
     void reset_ledger()
     {
         free_list.resize(S);
@@ -608,9 +647,11 @@ uint32_t effective_r(uint32_t payload, uint32_t r)
     return r >= 16u ? 16u : r >= 4u ? 4u : 1u;
 }
 
+// LDS-staged scatter: records of 4-byte fields only (the staged array sits at a 4-byte offset),
+// sized by the record struct (Particle 12 B, Photon 36 B)
 bool stage_fits(uint32_t bins, uint32_t tile_items, uint32_t rec_bytes)
 {
-    return bins <= 64u && scatter_smem(bins, tile_items, rec_bytes, true) <= 64u * 1024u;
+    return rec_bytes % 8u != 0u && bins <= 64u && scatter_smem(bins, tile_items, rec_bytes, true) <= 64u * 1024u;
 }
 
 std::vector<double> zipf_table(uint32_t values, double s)
@@ -625,6 +666,21 @@ std::vector<double> zipf_table(uint32_t values, double s)
 }
 
 }  // namespace
+
+// One fused partition batch (gf_batch.hpp) on `st`: one workgroup of as many 256-thread tile groups
+// (at most 4) as the LDS holds.
+void launch_fused_kernel(uint32_t payload, uint32_t r, const FusedArgs& a, const Cols& cols, hipStream_t st)
+{
+    const uint32_t G = std::max<uint32_t>(1u, std::min<uint32_t>(4u, (uint32_t)((160u * 1024u) / a.group_lds)));
+    const size_t lds = (size_t)G * a.group_lds;
+    with_payload(payload, r, [&](auto pt, auto rc) {
+        using P = typename decltype(pt)::type;
+        constexpr int RR = decltype(rc)::value;
+        set_lds(k_batch_fused<P, RR>, lds);
+        hipLaunchKernelGGL((k_batch_fused<P, RR>), dim3(1), dim3(kThreads * G), lds, st, a, cols);
+    });
+    HIPX(hipGetLastError());
+}
 
 // ---- stage: one node's batch protocol (StageBase, Execution.hpp:1537-1561) ----------------------
 struct gf_stage {
@@ -653,6 +709,15 @@ struct gf_stage {
     uint64_t launches = 0;
     uint32_t bin_path = GF_BIN_PATH_AUTO, hash_entries = 0, stall_ms = 0, hash_E = 0;
     size_t lds_count = 0, lds_scatter = 0;
+    // fused single-launch form (gf_transform_batch)
+    gf_chunk_handle* d_res = nullptr;
+    gf_chunk_handle* h_res = nullptr;
+    uint32_t* d_result = nullptr;
+    uint32_t* h_result = nullptr;
+    uint32_t* d_fscratch = nullptr;
+    uint32_t* h_totals = nullptr;
+    std::vector<gf_chunk_handle> reserved;
+    bool fused = false, collected = false;
     std::string err;
 
     ~gf_stage()
@@ -685,7 +750,7 @@ struct gf_stage {
         r = effective_r(p->payload, pick_tile_r(want));
         tile_items = 256u * r;
         hash_E = hash_entries_of(d, bin_path, hash_entries, tile_items);
-        const uint32_t rec = payload_bytes(p->payload);
+        const uint32_t rec = record_bytes(p->payload);
         if (kind() == 1) {
             const size_t lc = hash_E ? count_hash_smem(hash_E) : (size_t)std::max<uint32_t>(d.bins, 1u) * 4u;
             const size_t ls = hash_E ? scatter_hash_smem(hash_E)
@@ -695,6 +760,7 @@ struct gf_stage {
             lds_count = lc;
             lds_scatter = ls;
         }
+        validate_node(nd, t && t->photon_tally_bins ? t->photon_tally_bins : 1024u);
         bins = nd.role == GF_ROLE_SINK ? 1u : std::max<uint32_t>(nd.bins, 1u);
         if (kind() == 1 && nd.bins == 0) fail(GF_BAD_ARGUMENT, "graphFlow: a partition needs at least one bin");
         cap_in = nd.role == GF_ROLE_SOURCE ? 1u : Q;
@@ -718,6 +784,14 @@ struct gf_stage {
         h_in = mem.h<gf_chunk_desc>(cap_in);
         h_out = mem.h<gf_chunk_desc>(cap_out);
         h_prefix = mem.h<uint32_t>(2u * (cap_in + 1u));
+        if (kind() == 1) {
+            d_res = mem.d<gf_chunk_handle>(cap_out);
+            h_res = mem.h<gf_chunk_handle>(cap_out);
+            d_result = mem.d<uint32_t>(2);
+            h_result = mem.h<uint32_t>(2);
+            d_fscratch = mem.d<uint32_t>(fused_scratch_words(cap_in, bins, cap_tiles));
+            h_totals = mem.h<uint32_t>(bins);
+        }
         HIPX(hipEventCreateWithFlags(&ev, hipEventDisableTiming));
         rp.seed = seed;
         rp.C = p->C;
@@ -793,6 +867,111 @@ struct gf_stage {
             }
         }
         pending.clear();
+    }
+
+    void cancel_reserved()
+    {
+        for (size_t i = reserved.size(); i-- > 0;) {
+            try {
+                pool->cancel(reserved[i]);
+            } catch (...) {
+            }
+        }
+        reserved.clear();
+    }
+
+    // ---- fused form: one launch per partition batch (gf_batch.hpp); other stages as launch()
+    void launch_fused(const gf_chunk_desc* chunks, uint32_t n, uint64_t fid, hipStream_t st)
+    {
+        if (kind() != 1) {
+            launch(chunks, n, fid, st);
+            return;
+        }
+        throw_if_failed();
+        if (n == 0) fail(GF_PLAN_MISMATCH, "graphFlow: empty batch");
+        if (n > cap_in) fail(GF_PLAN_MISMATCH, "graphFlow: batch exceeds batchChunks");
+        const uint32_t C = pool->C;
+        in.assign(chunks, chunks + n);
+        first_id = fid;
+        pending.clear();
+        particles = 0;
+        n_tiles = 0;
+        for (uint32_t c = 0; c < n; ++c) {
+            if (chunks[c].fill > C) fail(GF_PLAN_MISMATCH, "graphFlow: chunk fill exceeds the chunk capacity");
+            h_in[c] = chunks[c];
+            n_tiles += (uint32_t)ceil_div(chunks[c].fill, tile_items);
+            particles += chunks[c].fill;
+        }
+        if (particles == 0) fail(GF_PLAN_MISMATCH, "partition: empty batch");
+        // the batch's reservation: outputChunkBound handles, popped in LIFO order
+        const uint64_t bound = output_chunk_bound(particles, C, bins);
+        if (bound > reserve_bound) fail(GF_PLAN_MISMATCH, "partition plan: reservation exceeds the stage's bound");
+        const uint32_t peak0 = pool->peak_live;
+        reserved.clear();
+        try {
+            for (uint64_t r = 0; r < bound; ++r) {
+                reserved.push_back(pool->acquire());
+                h_res[r] = reserved.back();
+            }
+        } catch (...) {
+            cancel_reserved();
+            pool->peak_live = peak0;
+            throw;
+        }
+        pool->peak_live = peak0;
+        HIPX(hipMemcpyAsync(h_batch->in, h_in, sizeof(gf_chunk_desc) * n, hipMemcpyHostToDevice, st));
+        HIPX(hipMemcpyAsync(d_res, h_res, sizeof(gf_chunk_handle) * bound, hipMemcpyHostToDevice, st));
+        FusedArgs a{};
+        a.nd = nd;
+        a.in = h_batch->in;
+        a.n = n;
+        a.tile_items = tile_items;
+        a.C = C;
+        a.hash_entries = hash_E;
+        a.reserved = d_res;
+        a.n_reserved = (uint32_t)bound;
+        a.purpose = 2u * key;
+        a.out = h_batch->out;
+        a.scratch = d_fscratch;
+        a.result = d_result;
+        a.group_lds = (uint32_t)((std::max(lds_count, lds_scatter) + 15u) & ~(size_t)15u);
+        a.tiles_cap = cap_tiles;
+        launch_fused_kernel(pool->payload, r, a, pool->dcols(), st);
+        HIPX(hipMemcpyAsync(h_result, d_result, 2u * sizeof(uint32_t), hipMemcpyDeviceToHost, st));
+        HIPX(hipMemcpyAsync(h_out, h_batch->out, sizeof(gf_chunk_desc) * bound, hipMemcpyDeviceToHost, st));
+        HIPX(hipMemcpyAsync(h_totals, d_fscratch + 2u * (n + 1u), sizeof(uint32_t) * bins, hipMemcpyDeviceToHost, st));
+        HIPX(hipEventRecord(ev, st));
+        fused = true;
+        launched = true;
+        ++launches;
+    }
+
+    // Completion of a fused batch: the outputs it used, the unused reservation returned.
+    void fused_collect(uint32_t& allocated)
+    {
+        wait_stage();
+        fused = false;
+        const uint32_t status = h_result[1], R = h_result[0];
+        if (status) {
+            cancel_reserved();
+            failed = true;
+            if (status == kStatusTableFull) fail(GF_INVALID, "partition classify: hash bin table full");
+            if (status == GF_OVERFLOW) fail(GF_OVERFLOW, "partition scatter: device status reports overflow");
+            if (status == GF_PLAN_MISMATCH)
+                fail(GF_PLAN_MISMATCH, "partition plan: classified items or output chunks do not match the batch");
+            fail(GF_INVALID, "partition: device status reports an invalid classifier or updater result");
+        }
+        if (R > reserved.size()) {
+            cancel_reserved();
+            failed = true;
+            fail(GF_PLAN_MISMATCH, "partition plan: exact chunk count exceeds the reservation bound");
+        }
+        totals.assign(h_totals, h_totals + bins);
+        pending.assign(h_out, h_out + R);
+        for (size_t i = reserved.size(); i-- > R;) pool->cancel(reserved[i]);
+        reserved.clear();
+        pool->peak_live = std::max<uint32_t>(pool->peak_live, pool->S - (uint32_t)pool->free_list.size());
+        allocated = R;
     }
 
     // ---- launch (Execution.hpp:2363-2430 / 1805-1901 / 1615-1637 / 2809-2828)
@@ -894,6 +1073,16 @@ struct gf_stage {
     {
         throw_if_failed();
         allocated = 0;
+        if (fused) {
+            fused_collect(allocated);
+            return false;
+        }
+        if (collected) {
+            // the fused batch was collected at dispatch (host runner): report its outputs now
+            collected = false;
+            allocated = (uint32_t)pending.size();
+            return false;
+        }
         wait_stage();
         const int kd = kind();
         if (kd == 0) {
@@ -977,7 +1166,7 @@ struct gf_stage {
         upload_batch(st, h_batch->n_out);
         reset_status(st);
         Cols cols = pool->dcols();
-        const bool stage = !hash_E && stage_fits(B, tile_items, payload_bytes(pool->payload));
+        const bool stage = !hash_E && stage_fits(B, tile_items, record_bytes(pool->payload));
         const size_t lds = lds_scatter;
         with_payload(pool->payload, r, [&](auto pt, auto rc) {
             using P = typename decltype(pt)::type;
@@ -1007,6 +1196,12 @@ struct gf_stage {
 
     void abort()
     {
+        if (fused) {
+            (void)hipEventSynchronize(ev);
+            fused = false;
+        }
+        cancel_reserved();
+        collected = false;
         if (kind() != 0) release_pending();
         pending.clear();
         failed = true;
@@ -1060,6 +1255,7 @@ namespace {
 
 void check_graph(const gf_flow& f, uint32_t source)
 {
+    for (const auto& n : f.nodes) validate_node(n.nd, f.tuning.photon_tally_bins ? f.tuning.photon_tally_bins : 1024u);
     if (source >= f.nodes.size() || f.nodes[source].nd.role != GF_ROLE_SOURCE)
         fail(GF_BAD_ARGUMENT, "graphFlow: run() requires the registered source node");
     uint32_t sources = 0;
@@ -1197,6 +1393,7 @@ void run_host(gf_flow& f, uint32_t src, uint64_t N, const RunShape& sh, gf_stats
     // lanes (Tuning::inflightBatches, GraphRunner.hpp:766-771): each lane owns its stage, batch,
     // leases and output credit; lane j of node k is lanes[k * D + j]
     const uint32_t D = f.tuning.inflight_batches ? f.tuning.inflight_batches : 1u;
+    const bool fused = f.tuning.fused_batches != 0u;
     std::vector<std::unique_ptr<gf_stage>> stages((size_t)n * D);
     for (uint32_t k = 0; k < n; ++k)
         for (uint32_t j = 0; j < D; ++j) {
@@ -1345,7 +1542,17 @@ void run_host(gf_flow& f, uint32_t src, uint64_t N, const RunShape& sh, gf_stats
                         for (auto& d : L.batch) L.particles += d.fill;
                         L.armed = true;
                         gf_stage& stg = *stages[(size_t)k * D + (uint32_t)j];
-                        stg.launch(L.batch.data(), (uint32_t)L.batch.size(), 0, s);
+                        if (fused && stg.kind() == 1) {
+                            // one launch classifies, plans, acquires and scatters; the batch is
+                            // complete before the next iteration like every other batch here
+                            stg.launch_fused(L.batch.data(), (uint32_t)L.batch.size(), 0, s);
+                            uint32_t got = 0;
+                            stg.fused_collect(got);
+                            stg.collected = true;
+                            trace_host(f.trace, tr, k, iter, L.batch, nd.bins, stg.totals.data(), stg.pending);
+                        } else {
+                            stg.launch(L.batch.data(), (uint32_t)L.batch.size(), 0, s);
+                        }
                         if (nd.role == GF_ROLE_SINK) trace_host(f.trace, tr, k, iter, L.batch, 0, nullptr, {});
                         if (stg.kind() == 2) {
                             uint32_t tot = (uint32_t)L.particles;
@@ -1458,7 +1665,7 @@ void build_engine(gf_flow& f, uint64_t N, const RunShape& sh, uint32_t src)
     std::vector<int32_t> routes, fuse;
     std::vector<uint32_t> ring_of(nv);
     const uint32_t r = effective_r(f.payload, pick_tile_r(sh.tile_items));
-    const uint32_t rec = payload_bytes(f.payload);
+    const uint32_t rec = record_bytes(f.payload);
     // KA and KB hold only their tiles' LDS (the plan, retire and select run in KP and KR)
     size_t lds_a = 256, lds_b = 256, group_lds = 256;
     uint32_t grid_tiles = 0;
@@ -2151,6 +2358,14 @@ gf_status gf_stage_launch(gf_stage* stage, const gf_chunk_desc* chunks, uint32_t
     if (!stage || !chunks) return GF_BAD_ARGUMENT;
     return guard(&stage->err, [&] {
         stage->launch(chunks, n, first_id, stream ? (hipStream_t)stream : stage->pool->ctx->stream);
+    });
+}
+
+gf_status gf_transform_batch(gf_stage* stage, const gf_chunk_desc* chunks, uint32_t n, uint64_t first_id, void* stream)
+{
+    if (!stage || !chunks) return GF_BAD_ARGUMENT;
+    return guard(&stage->err, [&] {
+        stage->launch_fused(chunks, n, first_id, stream ? (hipStream_t)stream : stage->pool->ctx->stream);
     });
 }
 

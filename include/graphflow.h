@@ -126,8 +126,9 @@ typedef struct gf_tuning {
     uint32_t hash_table_entries; /* hash path: slots of a tile's bin table (0 -> nextPow2(2 * tile items), which
                                     never fills); a tile with more distinct bins fails its batch (GF_INVALID) */
     uint32_t stall_timeout_ms; /* Tuning::stallTimeout: plain schedule, longest wait for a device stage
-                                  before the run fails with GF_STALL (0 -> 120000)                         */
-    uint32_t reserved1;
+                                  before the run fails with GF_STALL (0 -> no limit)                       */
+    uint32_t fused_batches;  /* plain schedule: each partition batch as one launch (gf_transform_batch)
+                                instead of classify -> host plan -> scatter                                */
 } gf_tuning;
 
 enum { GF_BIN_PATH_AUTO = 0, GF_BIN_PATH_DENSE = 1, GF_BIN_PATH_HASH = 2 };
@@ -199,6 +200,14 @@ gf_status gf_stage_launch(gf_stage* stage, const gf_chunk_desc* chunks, uint32_t
 gf_status gf_stage_advance(gf_stage* stage, void* stream, uint32_t* allocated, int* more_work);
 gf_status gf_stage_finish(gf_stage* stage, gf_chunk_desc* outputs, uint32_t cap, uint32_t* n_out);
 gf_status gf_stage_abort(gf_stage* stage);
+/* One partition batch in ONE launch (no counterpart kernel in the reference, which plans on the host
+ * between its classify and scatter kernels, flow/Execution.hpp:2363-2725): classify, device plan,
+ * device output acquisition and stable scatter of a ragged list of n <= Q chunks. The stage
+ * reserves outputChunkBound(items, C, bins) chunks from its pool before the launch; the kernel
+ * uses the first R in bin-major order. gf_stage_advance (one call, more_work = 0) returns the
+ * unused reservation and reports allocated = R; gf_stage_finish copies the R outputs. Plain
+ * transforms, sources and sinks run their one kernel as with gf_stage_launch. */
+gf_status gf_transform_batch(gf_stage* stage, const gf_chunk_desc* chunks, uint32_t n, uint64_t first_id, void* stream);
 /* message of the stage's last failed call */
 gf_status gf_stage_last_error(gf_stage* stage, char* buffer, uint32_t size);
 /* per-bin totals of the last classify (bins entries) */

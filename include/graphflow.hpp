@@ -165,6 +165,7 @@ struct Tuning {
     BinPath binPath = BinPath::automatic;                   // Tuning::binPath
     std::optional<std::uint32_t> hashTableEntries;          // hash path: slots of a tile's bin table
     std::optional<std::chrono::milliseconds> stallTimeout;  // Tuning::stallTimeout
+    bool fusedBatches = false;                              // plain schedule: one launch per partition batch
     Schedule schedule = Schedule::device;
     bool trace = false;
     bool fuseTransforms = false;
@@ -192,6 +193,7 @@ struct Tuning {
         t.bin_path = (std::uint32_t)binPath;
         t.hash_table_entries = hashTableEntries.value_or(0);
         t.stall_timeout_ms = stallTimeout ? (std::uint32_t)stallTimeout->count() : 0u;
+        t.fused_batches = fusedBatches ? 1u : 0u;
         t.photon_layers = 10;
         t.photon_tally_bins = 1024;
         t.photon_layer_mfp = 1.0;

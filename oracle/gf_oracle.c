@@ -858,7 +858,9 @@ int or_run(const or_node_desc* nodes, uint32_t n_nodes, const int32_t* routes, c
                     uint32_t purpose = 2u * k;
                     int more = 0;
                     uint32_t allocated = 0;
-                    if (nd->role == OR_ROLE_TRANSFORM && is_partition(nd)) {
+                    if (nd->role == OR_ROLE_TRANSFORM && is_partition(nd) && b->stage == 2) {
+                        allocated = b->n_out;  /* fused batch: planned and scattered at dispatch */
+                    } else if (nd->role == OR_ROLE_TRANSFORM && is_partition(nd)) {
                         if (b->stage == 0) {
                             int r2 = partition_plan_scatter(&x, nd, purpose + 1u, &pool, b, reserve_bound_of(nd, C, Q));
                             if (r2 == OR_OVERFLOW) FAIL_RUN(OR_OVERFLOW, "partition scatter: checked updater overflow");
@@ -969,6 +971,14 @@ int or_run(const or_node_desc* nodes, uint32_t n_nodes, const int32_t* routes, c
                             int r2 = partition_classify(&x, nd, &pool, b);
                             if (r2) FAIL_RUN(r2, "partition classify: device status reports an invalid classifier result");
                             if (!hash_tables_fit(cfg, nd, b)) FAIL_RUN(OR_INVALID, "partition classify: hash bin table full");
+                            if (cfg->fused_stages) {
+                                r2 = partition_plan_scatter(&x, nd, purpose + 1u, &pool, b, reserve_bound_of(nd, C, Q));
+                                if (r2 == OR_OVERFLOW) FAIL_RUN(OR_OVERFLOW, "partition scatter: checked updater overflow");
+                                if (r2 == OR_INVALID) FAIL_RUN(OR_INVALID, "partition scatter: invalid updater result");
+                                if (r2) FAIL_RUN(r2, "partition plan failed");
+                                trace_batch(out, k, iter, b, nd->bins);
+                                b->stage = 2;
+                            }
                         } else {
                             int r2 = transform_batch(&x, nd, purpose, &pool, b, reserve_bound_of(nd, C, Q));
                             if (r2) FAIL_RUN(r2, "transform batch failed");

@@ -130,6 +130,8 @@ typedef struct or_run_cfg {
     uint32_t bin_path;     /* BinPath: 0 automatic (hash above 4096 bins), 1 dense, 2 hash           */
     uint32_t hash_tile;    /* hash path: items per classify tile (the device's tile size)            */
     uint32_t hash_entries; /* hash path: slots of a tile's bin table (0 -> nextPow2(2 * tile), never full) */
+    uint32_t fused_stages; /* runner mode: a partition batch classifies, plans, acquires and scatters at
+                              dispatch (one launch) and retires at the next iteration             */
 } or_run_cfg;
 
 typedef struct or_run_out {

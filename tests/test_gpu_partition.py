@@ -256,3 +256,37 @@ def test_registration_errors():
     flow.connect(src, f)
     with pytest.raises(ValueError):
         flow.run(src, 10)  # missing routes of the filter
+
+
+def test_descriptor_buffers_are_validated():
+    """A table classifier whose table is shorter than what it indexes, and a photon step whose tally
+    buffer holds fewer than the run's tally bins, are rejected before any kernel runs."""
+    torch = torch_cuda()
+    lut = torch.zeros(4, dtype=torch.int32, device="cuda")
+    flow = g.Flow(g.PAYLOAD_PARTICLE)
+    src = flow.source("generate", g.UniformParticles())
+    f = flow.partition("filter", g.Lut(2, lut.data_ptr(), 4, 3, 0, table_len=4))
+    rec = torch.zeros(16, dtype=torch.int32, device="cuda")
+    sink = flow.sink("sink", g.RecordWeight(rec.data_ptr(), 16))
+    flow.connect(src, f)
+    flow.connect(f.out(0), sink)
+    flow.connect(f.out(1), sink)
+    flow.configure(g.Tuning(seed=SEED, chunk_capacity=16, schedule=g.SCHEDULE_DEVICE))
+    with pytest.raises(ValueError, match="table"):
+        flow.run(src, 16)
+    pool = g.Pool(g.PAYLOAD_PARTICLE, 4, 16)
+    with pytest.raises(ValueError):
+        g.Stage(pool, g.Lut(2, lut.data_ptr(), 4, 3, 0, table_len=4).desc(), 1, SEED, 1)
+    tally = torch.zeros(16, dtype=torch.float64, device="cuda")
+    pf = g.Flow(g.PAYLOAD_PHOTON)
+    psrc = pf.source("generate", g.PhotonSource())
+    step = pf.transform("step", g.PhotonStep(tally.data_ptr(), 16))
+    alive = pf.partition("alive", g.PhotonAlive())
+    done = pf.sink("done", g.PhotonDone())
+    pf.connect(psrc, step)
+    pf.connect(step, alive)
+    pf.connect(alive.out(0), step)
+    pf.connect(alive.out(1), done)
+    pf.configure(g.Tuning(seed=SEED, chunk_capacity=64, schedule=g.SCHEDULE_DEVICE))  # 1024 tally bins
+    with pytest.raises(ValueError, match="tally"):
+        pf.run(psrc, 64)
