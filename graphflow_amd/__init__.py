@@ -133,6 +133,8 @@ SIGNATURES = {
     "gf_stage_create_tuned": (C.c_int, [_V, _P(NodeDesc), C.c_uint32, _P(TuningC), _P(_V)]),
     "gf_stage_last_error": (C.c_int, [_V, C.c_char_p, C.c_uint32]),
     "gf_transform_batch": (C.c_int, [_V, _P(ChunkDesc), C.c_uint32, C.c_uint64, _V]),
+    "gf_transform_batch_noalloc": (C.c_int, [_V, _P(NodeDesc), C.c_uint32, _P(TuningC), _V, C.c_uint32, _V,
+                                             C.c_uint32, _V, _V, _V]),
     "gf_pool_lease_count": (C.c_int, [_V, ChunkHandle, _P(C.c_uint32)]),
     "gf_pool_is_live": (C.c_int, [_V, ChunkHandle, _P(C.c_int)]),
     "gf_pool_generation": (C.c_int, [_V, C.c_uint32, _P(C.c_uint32)]),

@@ -15,6 +15,7 @@
 #include <cstring>
 #include <deque>
 #include <memory>
+#include <mutex>
 #include <string>
 #include <thread>
 #include <type_traits>
@@ -680,6 +681,33 @@ void launch_fused_kernel(uint32_t payload, uint32_t r, const FusedArgs& a, const
         hipLaunchKernelGGL((k_batch_fused<P, RR>), dim3(1), dim3(kThreads * G), lds, st, a, cols);
     });
     HIPX(hipGetLastError());
+}
+
+// ---- no-alloc fused batch ----------------------------------------------------------------------
+// The fused kernel's scratch for gf_transform_batch_noalloc: one array per device, held until the
+// process ends. Its name is a host stand-in; kernels get the device address that
+// hipGetSymbolAddress returns for the current device.
+constexpr uint64_t kNoAllocScratchWords = 1ull << 22;  // 16 MiB
+__device__ __align__(256) uint32_t scratch[kNoAllocScratchWords];
+
+std::mutex g_scratch_mu;
+uint32_t* g_scratch_ptr[64] = {};
+
+uint32_t* scratch_of_device(int device)
+{
+    if (device < 0 || device >= 64) fail(GF_BAD_ARGUMENT, "graphFlow: device index out of range");
+    std::lock_guard<std::mutex> lk(g_scratch_mu);
+    if (!g_scratch_ptr[device]) {
+        int cur = -1;
+        HIPX(hipGetDevice(&cur));
+        if (cur != device) HIPX(hipSetDevice(device));
+        void* p = nullptr;
+        const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+        if (cur != device) (void)hipSetDevice(cur);
+        HIPX(e);
+        g_scratch_ptr[device] = (uint32_t*)p;
+    }
+    return g_scratch_ptr[device];
 }
 
 // ---- stage: one node's batch protocol (StageBase, Execution.hpp:1537-1561) ----------------------
@@ -2366,6 +2394,52 @@ gf_status gf_transform_batch(gf_stage* stage, const gf_chunk_desc* chunks, uint3
     if (!stage || !chunks) return GF_BAD_ARGUMENT;
     return guard(&stage->err, [&] {
         stage->launch_fused(chunks, n, first_id, stream ? (hipStream_t)stream : stage->pool->ctx->stream);
+    });
+}
+
+gf_status gf_transform_batch_noalloc(gf_pool* pool, const gf_node_desc* node, uint32_t node_key, const gf_tuning* tuning,
+                                     const gf_chunk_desc* chunks, uint32_t n, const gf_chunk_handle* reserved,
+                                     uint32_t n_reserved, gf_chunk_desc* outputs, uint32_t* result, void* stream)
+{
+    if (!pool || !node || !tuning || !chunks || !reserved || !outputs || !result) return GF_BAD_ARGUMENT;
+    return guard(&pool->ctx->err, [&] {
+        const gf_node_desc& nd = *node;
+        if (!(nd.role == GF_ROLE_TRANSFORM && is_partition(nd)) || nd.bins == 0u)
+            fail(GF_BAD_ARGUMENT, "graphFlow: the no-alloc batch runs partition nodes");
+        if (tuning->bin_path > GF_BIN_PATH_HASH) fail(GF_BAD_ARGUMENT, "graphFlow::Tuning: unknown bin path");
+        if (n == 0u) fail(GF_PLAN_MISMATCH, "graphFlow: empty batch");
+        validate_node(nd, tuning->photon_tally_bins ? tuning->photon_tally_bins : 1024u);
+        const uint32_t C = pool->C, B = nd.bins;
+        uint32_t want = tuning->tile_items ? tuning->tile_items : 1024u;
+        if (on_hash_path(nd, tuning->bin_path)) want = std::min(want, kHashTileLimit);
+        const uint32_t r = effective_r(pool->payload, pick_tile_r(want)), T = 256u * r;
+        const uint32_t E = hash_entries_of(nd, tuning->bin_path, tuning->hash_table_entries, T);
+        const uint32_t rec = record_bytes(pool->payload);
+        const size_t lc = E ? count_hash_smem(E) : (size_t)B * 4u;
+        const size_t ls = E ? scatter_hash_smem(E) : scatter_smem(B, T, rec, stage_fits(B, T, rec));
+        if (lc > 160u * 1024u || ls > 160u * 1024u)
+            fail(GF_BAD_ARGUMENT, "graphFlow: partition width or hash table exceeds the workgroup LDS budget");
+        // worst case: every input chunk full
+        const uint64_t tiles = (uint64_t)n * ceil_div(C, T);
+        if (fused_scratch_words(n, B, (uint32_t)std::min<uint64_t>(tiles, UINT32_MAX)) > kNoAllocScratchWords)
+            fail(GF_BAD_ARGUMENT, "graphFlow: the batch's tile counts exceed the no-alloc scratch (" +
+                                      std::to_string(kNoAllocScratchWords) + " words)");
+        FusedArgs a{};
+        a.nd = nd;
+        a.in = chunks;
+        a.n = n;
+        a.tile_items = T;
+        a.C = C;
+        a.hash_entries = E;
+        a.reserved = reserved;
+        a.n_reserved = n_reserved;
+        a.purpose = 2u * node_key;
+        a.out = outputs;
+        a.scratch = scratch_of_device(pool->ctx->device);
+        a.result = result;
+        a.group_lds = (uint32_t)((std::max(lc, ls) + 15u) & ~(size_t)15u);
+        a.tiles_cap = (uint32_t)((kNoAllocScratchWords - 2ull * (n + 1u) - 2ull * B) / B);
+        launch_fused_kernel(pool->payload, r, a, pool->dcols(), stream ? (hipStream_t)stream : pool->ctx->stream);
     });
 }
 

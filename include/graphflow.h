@@ -208,6 +208,19 @@ gf_status gf_stage_abort(gf_stage* stage);
  * unused reservation and reports allocated = R; gf_stage_finish copies the R outputs. Plain
  * transforms, sources and sinks run their one kernel as with gf_stage_launch. */
 gf_status gf_transform_batch(gf_stage* stage, const gf_chunk_desc* chunks, uint32_t n, uint64_t first_id, void* stream);
+/* gf_transform_batch without allocation. No counterpart in the reference: for callers that cannot
+ * allocate device memory (code under hipGraph capture, tight serving loops). Every argument but
+ * pool, node and tuning is device memory the caller owns: chunks (n input descriptors), reserved
+ * (n_reserved output handles, normally outputChunkBound(items, C, bins) popped from the pool's free
+ * list), outputs (n_reserved descriptors; the first R are written), result ({R, status}; status 0,
+ * GF_INVALID, GF_OVERFLOW or GF_PLAN_MISMATCH; 17 = a hash bin table was full). The kernel's
+ * scratch is a static device array of 4 Mi words per device (16 MiB, held until the process
+ * ends): GF_BAD_ARGUMENT when n full chunks' tile counts would not fit it. Calls on one device
+ * share that array, so calls that overlap on one device race; the caller returns the unused
+ * reservation (handles R .. n_reserved-1). Stream ordered, asynchronous, no host synchronisation. */
+gf_status gf_transform_batch_noalloc(gf_pool* pool, const gf_node_desc* node, uint32_t node_key, const gf_tuning* tuning,
+                                     const gf_chunk_desc* chunks, uint32_t n, const gf_chunk_handle* reserved,
+                                     uint32_t n_reserved, gf_chunk_desc* outputs, uint32_t* result, void* stream);
 /* message of the stage's last failed call */
 gf_status gf_stage_last_error(gf_stage* stage, char* buffer, uint32_t size);
 /* per-bin totals of the last classify (bins entries) */
