@@ -89,10 +89,25 @@ namespace {
 
 uint64_t pool_bytes_of(uint32_t payload, uint32_t S, uint32_t C) { return (uint64_t)S * C * payload_bytes(payload); }
 
+void destroy_slot_flow(SlotFlow& sf);
+void build_slot_flow(SlotFlow& sf, gf_server& sv, gf_context* ctx, uint32_t topo, uint32_t slot);
+
 // Build the topology's flow for a slot (caller holds no lock; the budget was reserved by the caller).
+// A failed build frees what it had created.
 SlotFlow make_slot_flow(gf_server& sv, gf_context* ctx, uint32_t topo, uint32_t slot)
 {
     SlotFlow sf;
+    try {
+        build_slot_flow(sf, sv, ctx, topo, slot);
+    } catch (...) {
+        destroy_slot_flow(sf);
+        throw;
+    }
+    return sf;
+}
+
+void build_slot_flow(SlotFlow& sf, gf_server& sv, gf_context* ctx, uint32_t topo, uint32_t slot)
+{
     const uint32_t payload = topo == GF_JOB_FEEDBACK ? GF_PAYLOAD_PARTICLE : topo == GF_JOB_HASH ? GF_PAYLOAD_KEYED : GF_PAYLOAD_PHOTON;
     auto ck = [](gf_status s, const char* what) {
         if (s != GF_OK) fail(s, std::string("serving: ") + what);
@@ -191,7 +206,6 @@ SlotFlow make_slot_flow(gf_server& sv, gf_context* ctx, uint32_t topo, uint32_t 
     t.num_chunks = sh.S;
     ck(gf_flow_configure(f, &t), "configure");
     sf.bytes = pool_bytes_of(payload, sh.S, kServeCap) + sizeof(uint32_t) * sf.rec_cap + (64ull << 20);
-    return sf;
 }
 
 void destroy_slot_flow(SlotFlow& sf)
@@ -259,8 +273,12 @@ void server_worker(gf_server* sv, uint32_t slot)
 {
     gf_context* ctx = nullptr;
     if (gf_context_create(sv->ctx->device, &ctx) != GF_OK) {
-        std::lock_guard<std::mutex> lk(sv->mu);
-        sv->err = "serving: slot context creation failed";
+        {
+            std::lock_guard<std::mutex> lk(sv->mu);
+            sv->err = "serving: slot context creation failed";
+            sv->warm++;
+        }
+        sv->cv.notify_all();
         return;
     }
     SlotFlow flows[3];
@@ -282,6 +300,7 @@ void server_worker(gf_server* sv, uint32_t slot)
                 }) != GF_OK) {
                 std::lock_guard<std::mutex> lk(sv->mu);
                 sv->err = msg;
+                if (!flows[tp].flow) sv->mem_in_use -= sv->cfg.topology_bytes[tp];  // never built: refund
             }
         }
         std::lock_guard<std::mutex> lk(sv->mu);
@@ -294,7 +313,10 @@ void server_worker(gf_server* sv, uint32_t slot)
         {
             std::unique_lock<std::mutex> lk(sv->mu);
             // first come, first served: the oldest job runs in the next free slot once its
-            // topology's flow is resident here or fits the free budget
+            // topology's flow is resident here or fits the free budget. A slot waiting for memory
+            // first frees its own resident flows of other topologies (it is idle, so they are too):
+            // every slot that waits does the same, so a head job whose topology fits the budget
+            // alone is always admitted eventually.
             for (;;) {
                 if (sv->stop && sv->queue.empty()) break;
                 if (!sv->queue.empty()) {
@@ -302,6 +324,17 @@ void server_worker(gf_server* sv, uint32_t slot)
                     const uint32_t tp = h.d.topology;
                     need = flows[tp].flow ? 0u : sv->cfg.topology_bytes[tp];
                     if (sv->mem_in_use + need <= sv->cfg.memory_budget) break;
+                    bool evicted = false;
+                    for (uint32_t o = 0; o < 3u; ++o)
+                        if (o != tp && flows[o].flow) {
+                            destroy_slot_flow(flows[o]);
+                            sv->mem_in_use -= sv->cfg.topology_bytes[o];
+                            evicted = true;
+                        }
+                    if (evicted) {
+                        sv->cv.notify_all();
+                        continue;
+                    }
                 }
                 sv->cv.wait(lk);
             }
@@ -320,7 +353,19 @@ void server_worker(gf_server* sv, uint32_t slot)
         std::string msg;
         const gf_status rc = guard(&msg, [&] {
             SlotFlow& sf = flows[j.d.topology];
-            if (!sf.flow) sf = make_slot_flow(*sv, ctx, j.d.topology, slot);
+            if (!sf.flow) {
+                try {
+                    sf = make_slot_flow(*sv, ctx, j.d.topology, slot);
+                } catch (...) {
+                    destroy_slot_flow(sf);
+                    {
+                        std::lock_guard<std::mutex> lk(sv->mu);
+                        sv->mem_in_use -= need;  // the flow was never built: refund its charge
+                    }
+                    sv->cv.notify_all();
+                    throw;
+                }
+            }
             run_job(*sv, ctx, sf, j, r, &tally);
         });
         if (rc != GF_OK) r.status = rc;

@@ -75,3 +75,25 @@ def test_latency_accounting_and_slots():
         assert r["engine"] == 3  # the team engine, on the slot's XCD
     assert {r["slot"] for r in res} <= {0, 1, 2, 3}
     assert len({r["digest"] for r in res}) == 16  # different seeds, different results
+
+
+def test_small_budget_mixed_topologies_drain():
+    """A memory budget that holds one topology's flow at a time: slots free their idle flows of other
+    topologies for the queue head, so every job finishes (no admission deadlock) with the result it
+    has alone."""
+    torch_cuda()
+    jobs = [(g.JOB_FEEDBACK, 5000, 1), (g.JOB_PHOTON, 3000, 2), (g.JOB_HASH, 9000, 3),
+            (g.JOB_FEEDBACK, 7000, 4), (g.JOB_HASH, 2000, 5), (g.JOB_PHOTON, 4000, 6)]
+    sv = g.Server(slots=2, max_items=200000, memory_budget=int(2.6 * 2**30))
+    ids = [sv.submit(t, n, s) for t, n, s in jobs]
+    res = {r["job"]: r for r in sv.poll(wait_all=True)}
+    sv.close()
+    assert len(res) == len(jobs)
+    alone = serve(jobs, slots=1)
+    for jid, a in zip(ids, alone):
+        r = res[jid]
+        assert r["status"] == 0
+        if r["topology"] != g.JOB_PHOTON:
+            assert r["digest"] == a["digest"]
+        else:
+            assert abs(r["tally_sum"] - a["tally_sum"]) <= 1e-9 * abs(a["tally_sum"])
