@@ -1,20 +1,28 @@
 """Benchmark of the transform-batch hot path on one MI355X.
 
-Workload: the fp32 particle feedback network Generate -> Filter -> {A (x0.5), B (x0.75)} -> Filter,
-Filter.sink -> Sink at N=1e8, C=8192, P=16384, Q=8, seed 42 (synthetic inputs generated on the
-device by the source node). One "step" of this benchmark is one complete run of the network over
-all N particles; `value` is N / seconds per run, measured around `Flow.run` with the pool, the
+Headline workload: the fp32 particle feedback network Generate -> Filter -> {A (x0.5), B (x0.75)}
+-> Filter, Filter.sink -> Sink at N=1e8, C=8192, P=16384, Q=8, seed 42 (synthetic inputs generated
+on the device by the source node). One "step" of this benchmark is one complete run of the network
+over all N particles; `value` is N / seconds per run, measured around `Flow.run` with the pool, the
 sink histogram and the engine buffers already resident in HBM.
 
-Beside the headline (batched form: device step engine replayed from a hipGraph) it reports the
-plain one-batch-at-a-time form, the roofline of the dominant engine kernel (timed with HIP events
-on the engine's stream) and the CPU oracle's restatement of the reference path on a bounded sample.
+Beside the headline it reports (each at the size its workload states):
+  - the roofline of the dominant kernel, with its launch durations taken from the timed runs
+    (HIP events on the engine's stream);
+  - the other routing distributions of the feedback network at N=1e8;
+  - the keyed hash pipeline at 1e8 keys, Q in {1, 4, 8, 16}, with in-flight batch lanes;
+  - photon transport at 1e8 photons;
+  - the served mixed stream of 4096 jobs (N log-uniform 1e3..1e7, 64 GB budget);
+  - the plain one-batch-at-a-time forms;
+and, in the CPU leg, the oracle's restatement of the reference CPU path (runner schedule, one thread
+and all host threads) plus the CPU results every GPU form is checked against.
 """
 from __future__ import annotations
 
 import argparse
 import json
 import os
+import subprocess
 import sys
 import time
 
@@ -23,6 +31,8 @@ sys.path.insert(0, ROOT)
 
 HBM_PEAK_GBS = 8000.0
 METRIC = "particles/s at N=1e8 C=8192 P=16384 Q=8; % HBM peak; served p99 job latency"
+GOLDEN = os.path.join(ROOT, "tests", "golden", "feedback_N1e8.json")
+TRAFFIC_FILE = os.path.join(ROOT, "profiles", "k_team_feedback_N1e8_hbm_traffic.txt")
 
 
 def log(*a):
@@ -34,29 +44,20 @@ ENGINES = {0: "plain host-planned", 1: "wide (hipGraph of step kernels)", 2: "mi
            4: "team handing over to wide"}
 
 
-def build_feedback(g, torch, N, C, P, Q, schedule, seed=42, use_graph=1, fuse=False, engine=0, tile_items=None,
-                   team=0):
+def build_feedback(g, torch, C, P, Q, schedule, seed=42, use_graph=1, fuse=False, engine=0, team=0,
+                   distribution="parity", destinations=2, **kw):
     counts = torch.zeros(g.FEEDBACK_HIST_N, dtype=torch.int64, device="cuda")
     flow = g.Flow(g.PAYLOAD_PARTICLE)
-    src, *_ = g.feedback_network(flow, g.WeightHistogram(counts.data_ptr(), g.FEEDBACK_HIST_LO, g.FEEDBACK_HIST_N))
+    src, *_ = g.feedback_network(flow, g.WeightHistogram(counts.data_ptr(), g.FEEDBACK_HIST_LO, g.FEEDBACK_HIST_N),
+                                 1e-3, distribution, destinations)
     flow.configure(g.Tuning(seed=seed, batch_chunks=Q, chunk_capacity=C, live_particles=P, schedule=schedule,
-                            use_graph=use_graph, tile_items=tile_items, fuse_transforms=fuse, device_engine=engine,
-                            max_workgroups=team or None))
+                            use_graph=use_graph, fuse_transforms=fuse, device_engine=engine,
+                            max_workgroups=team or None, **kw))
     return flow, src, counts
 
 
-def tuning_of(g, N, C, P, Q, fuse, args, tuned, use_graph):
-    """The timed configuration again (the autotuner's pick when it ran), with another use_graph."""
-    eng = tuned["device_engine"] if tuned else args.engine
-    wg = tuned["max_workgroups"] if tuned else (args.team or None)
-    tile = tuned["tile_items"] if tuned else None
-    return g.Tuning(seed=42, batch_chunks=Q, chunk_capacity=C, live_particles=P, schedule=g.SCHEDULE_DEVICE,
-                    use_graph=use_graph, fuse_transforms=fuse, device_engine=eng, max_workgroups=wg or None,
-                    tile_items=tile or None)
-
-
-def run_once(flow, src, counts, N, torch):
-    counts.zero_()
+def run_once(flow, src, buf, N, torch):
+    buf.zero_()
     torch.cuda.synchronize()
     t0 = time.perf_counter()
     st = flow.run(src, N)
@@ -64,7 +65,11 @@ def run_once(flow, src, counts, N, torch):
     return time.perf_counter() - t0, st
 
 
-TRAFFIC_FILE = os.path.join(os.path.dirname(os.path.abspath(__file__)), "profiles", "k_team_feedback_N1e8_hbm_traffic.txt")
+def golden_cases():
+    try:
+        return json.load(open(GOLDEN))
+    except OSError:
+        return None
 
 
 def pmc_summary(fetch_csv, write_csv, workload, out):
@@ -85,8 +90,7 @@ def pmc_summary(fetch_csv, write_csv, workload, out):
 
     lines = ["# HBM traffic of the team engine kernel (k_team), from two rocprofv3 --pmc passes:",
              "#   cd /tmp && rocprofv3 --pmc FETCH_SIZE --output-format csv -d $OUT/pmc_fetch -o run -- "
-             "python $R/bench.py --no-cpu --no-plain --photons 0 --serve-jobs 0 --hash-items 0 --no-autotune --no-roofline "
-             "--steps 1 --warmup 0 --items 1e8 --copy-calib 1024",
+             "python $R/bench.py --only-headline --no-cpu --no-autotune --steps 1 --warmup 0 --copy-calib 1024",
              "#   (the same with --pmc WRITE_SIZE into $OUT/pmc_write; R = the repository root, OUT = an output directory)",
              "#   python bench.py --pmc-summary $OUT/pmc_fetch/run_counter_collection.csv "
              "$OUT/pmc_write/run_counter_collection.csv <this file>",
@@ -112,7 +116,7 @@ def pmc_summary(fetch_csv, write_csv, workload, out):
     print("\n".join(lines))
 
 
-def traffic_for(workload, launches):
+def traffic_for(workload):
     """HBM bytes per k_team launch from the committed PMC summary of this workload, or None."""
     import re
 
@@ -120,18 +124,89 @@ def traffic_for(workload, launches):
         kv = dict(m.groups() for m in (re.match(r"^([a-z_]+)=(.*)$", l.strip()) for l in open(TRAFFIC_FILE)) if m)
     except OSError:
         return None
-    if kv.get("workload") != workload or int(kv.get("launches", -1)) != launches:
+    if kv.get("workload") != workload:
         return None
-    return float(kv["traffic_bytes_per_launch"])
+    return float(kv["traffic_bytes_per_launch"]), int(kv["launches"])
 
 
-def photon_forms(g, torch, N, Np, C, P, Q, verify):
-    """Photon transport (10 layers of 1 mfp, albedo 0.9, HG g=0.9, roulette 1e-4 / 0.1, 1024 depth
-    bins): the batched persistent form (team engine) and the plain host-planned one-batch form,
-    photons/s and photon steps/s; with verify, the worst per-bin relative difference of the batched
-    tallies from the FP64 CPU oracle on the same Philox words."""
-    cfg = dict(g.PHOTON_DEFAULTS)
+# ---- the other routing distributions of config 2 -------------------------------------------------
+def distribution_forms(g, torch, N, C, P, Q, golden):
     out = {}
+    for name, dest in (("skew_90_10", 2), ("all_one", 2), ("uniform", 16)):
+        flow, src, counts = build_feedback(g, torch, C, P, Q, g.SCHEDULE_DEVICE, fuse=True, distribution=name,
+                                           destinations=dest)
+        run_once(flow, src, counts, min(N, 1_000_000), torch)  # engine build
+        dt, st = run_once(flow, src, counts, N, torch)
+        rows = g.hist_rows(counts.data_ptr(), g.FEEDBACK_HIST_N, g.FEEDBACK_HIST_LO)
+        sha = g.rows_sha256(rows)
+        rec = {"value": N / dt, "unit": "particles/s", "items": N, "engine": int(st["engine"]),
+               "histogram_rows": int(len(rows)), "histogram_sha256": sha}
+        if golden and N == golden["items"]:
+            rec["histogram_equals_golden"] = sha == golden["cases"][name]["sha256"]
+        out[f"feedback_{name}_{dest}way"] = rec
+        log(f"[bench] feedback {name} ({dest} destinations) N={N}: {N / dt:.4g} particles/s, rows {len(rows)}, "
+            f"equals golden: {rec.get('histogram_equals_golden')}")
+        flow.close()
+    return out
+
+
+# ---- config 3: keyed hash pipeline ---------------------------------------------------------------
+def hash_flow(g, ms, B=4096):
+    f = g.Flow(g.PAYLOAD_KEYED)
+    src = f.source("keys", g.ZipfKeys(1 << 20, 1.1))
+    part = f.partition("bucket", g.KeyBucket(B))
+    snk = f.sink("digest", g.ChunkDigest(0, 0, ms.data_ptr(), B))
+    f.connect(src, part)
+    for b in range(B):
+        f.connect(part, snk, b)
+    return f, src
+
+
+def hash_forms(g, torch, N, qs, lanes, live):
+    """Keyed hash pipeline (Zipf(1.1) keys over 2^20 values, murmur3 fmix32 into 4096 buckets,
+    C=8192) at each Q: keys/s of the batched wide engine with `lanes` batches in flight per node and
+    `live` keys in flight. Per-bucket (hash sum, count) pairs are kept for the CPU leg's check.
+    A second run with HIP events around every step kernel gives the kernels' time."""
+    out, multisets = {}, {}
+    B, C = 4096, 8192
+    ms = torch.zeros(2 * B, dtype=torch.int64, device="cuda")
+    for Q in qs:
+        f, src = hash_flow(g, ms, B)
+        S = min(524000, live // C + 2 * lanes * (Q + B) + 64)
+        tun = dict(seed=42, batch_chunks=Q, chunk_capacity=C, live_particles=live, num_chunks=S,
+                   schedule=g.SCHEDULE_DEVICE, device_engine=1, max_workgroups=1024, tile_items=4096,
+                   inflight_batches=lanes)
+        f.configure(g.Tuning(**tun))
+        run_once(f, src, ms, min(N, 2_000_000), torch)  # engine build, hipGraph capture
+        dt, st = run_once(f, src, ms, N, torch)
+        multisets[Q] = ms.cpu().numpy().view("uint64").reshape(-1, 2).copy()
+        rec = {"value": N / dt, "unit": "keys/s", "items": N, "Q": Q, "inflight_batches": lanes, "live": live,
+               "num_chunks": S, "engine_steps": int(st["steps"]), "engine": int(st["engine"])}
+        # kernel time per step kernel (direct launches timed with HIP events, not the graph replay)
+        f.configure(g.Tuning(**{**tun, "use_graph": 2}))
+        run_once(f, src, ms, min(N, 2_000_000), torch)
+        run_once(f, src, ms, N, torch)
+        kms, kn = f.kernel_times()
+        part_items, _ = f.item_passes()
+        kab = kms[0] + kms[2]
+        rec["kernel_ms"] = {"classify": kms[0], "plan": kms[1], "scatter_source_sink_retire_select": kms[2]}
+        # algorithmic bytes of the classify+scatter(+source, sink) kernels: 12 B source write, 24 B
+        # partition pass (read + write), 12 B sink read per key
+        rec["classify_scatter_gbs"] = 48.0 * N / (kab * 1e-3) / 1e9 if kab > 0 else None
+        rec["classify_scatter_hbm_frac"] = rec["classify_scatter_gbs"] / HBM_PEAK_GBS if kab > 0 else None
+        out[f"hash_pipeline_B4096_Q{Q}"] = rec
+        log(f"[bench] hash pipeline B=4096 Q={Q} lanes={lanes} N={N}: {N / dt:.4g} keys/s ({int(st['steps'])} steps); "
+            f"kernels KA {kms[0]:.1f} KP {kms[1]:.1f} KB+KR {kms[2]:.1f} ms")
+        f.close()
+    return out, multisets
+
+
+# ---- config 4: photon transport ------------------------------------------------------------------
+def photon_forms(g, torch, N, Np, C, P, Q):
+    """Photon transport (10 layers of 1 mfp, albedo 0.9, HG g=0.9, roulette 1e-4 / 0.1, 1024 depth
+    bins): the batched persistent form (team engine) and the plain host-planned one-batch form."""
+    cfg = dict(g.PHOTON_DEFAULTS)
+    out, tallies = {}, {}
     for name, n, kw in (("photon_batched_team", N, dict(schedule=g.SCHEDULE_DEVICE, batch_chunks=Q)),
                         ("photon_plain_host_planned_Q1", Np, dict(schedule=g.SCHEDULE_HOST, batch_chunks=1))):
         tally = torch.zeros(cfg["tally_bins"], dtype=torch.float64, device="cuda")
@@ -139,75 +214,40 @@ def photon_forms(g, torch, N, Np, C, P, Q, verify):
         src, *_ = g.photon_network(flow, tally.data_ptr())
         flow.configure(g.Tuning(seed=42, chunk_capacity=C, live_particles=P, photon=cfg, **kw))
         flow.run(src, min(n, 100000))  # warm-up: engine build, pool
-        tally.zero_()
         dt, st = run_once(flow, src, tally, n, torch)
         part_items, xf_items = flow.item_passes()
         photon_steps = part_items  # one alive-partition pass per photon step (the step is fused into it)
-        rec = {"value": n / dt, "unit": "photons/s", "items": n, "photon_steps_per_s": photon_steps / dt,
-               "photon_steps": int(photon_steps), "engine_steps": int(st["steps"])}
-        if verify and name == "photon_batched_team":
-            import numpy as np
-            import oracle as O
-            ref, ref_steps = O.photon_scalar(42, n, cfg, threads=min(os.cpu_count() or 1, 16))
-            t = tally.cpu().numpy()
-            rec["fp64_oracle_max_bin_rel_err"] = float((np.abs(t - ref) / ref).max())
-            rec["fp64_oracle_total_rel_err"] = float(abs(t.sum() - ref.sum()) / ref.sum())
-            rec["fp64_oracle_photon_steps"] = int(ref_steps)
-        out[name] = rec
-        log(f"[bench] {name} N={n}: {n / dt:.4g} photons/s, {photon_steps / dt:.4g} photon steps/s "
-            + (f"(max bin rel err vs FP64 {rec['fp64_oracle_max_bin_rel_err']:.2e})" if "fp64_oracle_max_bin_rel_err" in rec else ""))
-    return out
+        out[name] = {"value": n / dt, "unit": "photons/s", "items": n, "photon_steps_per_s": photon_steps / dt,
+                     "photon_steps": int(photon_steps), "engine_steps": int(st["steps"])}
+        tallies[name] = tally.cpu().numpy()
+        log(f"[bench] {name} N={n}: {n / dt:.4g} photons/s, {photon_steps / dt:.4g} photon steps/s")
+        flow.close()
+    return out, tallies
 
 
-def hash_form(g, torch, N, qs=(8, 16), autotune=True):
-    """Keyed hash pipeline (Zipf(1.1) keys over 2^20 values, murmur3 finalizer into 4096 buckets,
-    C=8192) at each Q of `qs`: keys/s on the device engine, one partition pass per key (24 B of
-    algorithmic traffic per key). Without autotune: the wide engine, 1024 workgroups, 1024-item
-    tiles."""
-    out = {}
-    bins = 4096
-    ms = torch.zeros(2 * bins, dtype=torch.int64, device="cuda")
-    for Q in qs:
-        f = g.Flow(g.PAYLOAD_KEYED)
-        src = f.source("keys", g.ZipfKeys(1 << 20, 1.1))
-        part = f.partition("bucket", g.KeyBucket(bins))
-        snk = f.sink("digest", g.ChunkDigest(0, 0, ms.data_ptr(), bins))
-        f.connect(src, part)
-        for b in range(bins):
-            f.connect(part, snk, b)
-        if autotune:
-            f.configure(g.Tuning(seed=42, batch_chunks=Q, chunk_capacity=8192, schedule=g.SCHEDULE_DEVICE))
-            tuned = f.autotune(src, probe_items=min(N, 1_000_000))
-        else:
-            tuned = {"device_engine": 1, "max_workgroups": 1024, "batch_chunks": Q, "tile_items": 1024}
-            f.configure(g.Tuning(seed=42, batch_chunks=Q, chunk_capacity=8192, schedule=g.SCHEDULE_DEVICE,
-                                 device_engine=1, max_workgroups=1024, tile_items=1024))
-        dt, st = run_once(f, src, ms, N, torch)
-        out[f"hash_pipeline_B4096_Q{Q}"] = {"value": N / dt, "unit": "keys/s", "items": N, "engine_steps": int(st["steps"]),
-                                           "engine": int(st["engine"]), "autotuned": tuned}
-        log(f"[bench] hash pipeline B=4096 Q={Q} N={N}: {N / dt:.4g} keys/s ({int(st['steps'])} steps, {tuned})")
-    return out
-
-
-def served_form(g, torch, jobs, rate, max_items, seed=42, plain=True):
-    """Served mixed stream: `jobs` independent flow jobs, Poisson arrivals at `rate` per second,
-    N ~ log-uniform[1e3, max_items], topology uniform over {particle feedback, keyed hash pipeline,
-    photon transport}; aggregate items/s and p50/p99 job latency (submit to finish), next to the
-    same jobs run one at a time with the plain host-planned form (Q=1)."""
-    import threading
+# ---- config 5: served mixed stream ---------------------------------------------------------------
+def job_stream(jobs, max_items, seed=42):
     import numpy as np
 
     rng = np.random.default_rng(seed)
     topo = rng.integers(0, 3, jobs)
     items = np.exp(rng.uniform(np.log(1e3), np.log(max_items), jobs)).astype(np.int64)
-    arrive = np.cumsum(rng.exponential(1.0 / rate, jobs))
+    gaps = rng.exponential(1.0, jobs)
     seeds = 1000 + np.arange(jobs)
-    sv = g.Server(slots=8, max_items=int(max_items) + 1, prewarm=True)  # every slot builds its flows first
-    ids = [None] * jobs
+    return topo, items, gaps, seeds
+
+
+def serve_stream(g, topo, items, arrive, seeds, slots, max_items):
+    import threading
+
+    import numpy as np
+
+    sv = g.Server(slots=slots, max_items=int(max_items) + 1, prewarm=True)
+    ids = [None] * len(topo)
 
     def submitter():
         t0 = time.perf_counter()
-        for i in range(jobs):
+        for i in range(len(topo)):
             d = arrive[i] - (time.perf_counter() - t0)
             if d > 0:
                 time.sleep(d)
@@ -218,42 +258,53 @@ def served_form(g, torch, jobs, rate, max_items, seed=42, plain=True):
     th.join()
     res = sv.poll(wait_all=True)
     sv.close()
-    bad = [r for r in res if r["status"] != 0]
     lat = np.array([r["end_ms"] - r["submit_ms"] for r in res])
-    t_first = min(r["submit_ms"] for r in res)
-    t_last = max(r["end_ms"] for r in res)
+    span = (max(r["end_ms"] for r in res) - min(r["submit_ms"] for r in res)) * 1e-3
+    return res, lat, span
+
+
+def served_forms(g, torch, jobs, max_items, rate, slots=8):
+    """Config 5: `jobs` independent flow jobs, Poisson arrivals at `rate` per second, N log-uniform over
+    [1e3, max_items], topology uniform over {particle feedback, keyed hash pipeline, photon transport},
+    64 GB budget. Aggregate items/s and p50/p99 job latency (submit to finish), next to the same
+    stream's jobs run one at a time on the batched engine (every 16th job, one slot) and with the
+    plain host-planned form (the stream's 16 smallest jobs)."""
+    import numpy as np
+
+    topo, items, gaps, seeds = job_stream(jobs, max_items)
+    arrive = np.cumsum(gaps / rate)
+    res, lat, span = serve_stream(g, topo, items, arrive, seeds, slots, max_items)
     total = int(items.sum())
-    served = total / ((t_last - t_first) * 1e-3)
-    per_topo = {}
-    for t in range(3):
-        rs = [r for r in res if r["topology"] == t]
-        if rs:
-            per_topo[g.JOB_NAMES[t]] = {
-                "jobs": len(rs), "items": int(sum(r["items"] for r in rs)),
-                "mean_service_ms": float(np.mean([r["end_ms"] - r["start_ms"] for r in rs])),
-                "service_items_per_s": float(sum(r["items"] for r in rs) / max(1e-9, 1e-3 * sum(r["end_ms"] - r["start_ms"] for r in rs)))}
-    # plain form: every 4th job of the same stream one at a time, host-planned one batch at a time
-    sample = list(range(0, jobs, 4)) if plain else []
-    plain_t = 0.0
+    bad = [r for r in res if r["status"] != 0]
+    out = {"value": total / span, "unit": "items/s", "jobs": jobs, "items": total, "arrival_rate_per_s": rate,
+           "n_range": [1000, int(max_items)], "slots": slots, "memory_budget_bytes": 64 << 30,
+           "failed_jobs": len(bad), "p50_latency_ms": float(np.percentile(lat, 50)),
+           "p99_latency_ms": float(np.percentile(lat, 99)), "makespan_s": span,
+           "jobs_per_topology": {g.JOB_NAMES[t]: int((topo == t).sum()) for t in range(3)}}
+    # one at a time on the batched engine: every 16th job through one slot, submitted together
+    sample = np.arange(0, jobs, 16)
+    r1, _, span1 = serve_stream(g, topo[sample], items[sample], np.zeros(len(sample)), seeds[sample], 1, max_items)
+    one = int(items[sample].sum()) / span1
+    out["batched_one_at_a_time_items_per_s"] = one
+    out["speedup_vs_batched_one_at_a_time"] = out["value"] / one
+    # plain host-planned form, one batch at a time (Q=1): the 16 smallest jobs of the stream
+    small = np.argsort(items)[:16]
     tally = torch.zeros(1024, dtype=torch.float64, device="cuda")
     rec = torch.zeros(int(max_items) + 1, dtype=torch.int32, device="cuda")
     ms = torch.zeros(2 * 4096, dtype=torch.int64, device="cuda")
     flows = {}
     for t in range(3):
-        f = g.Flow([g.PAYLOAD_PARTICLE, g.PAYLOAD_KEYED, g.PAYLOAD_PHOTON][t])
-        if t == 0:
-            src, *_ = g.feedback_network(f, g.RecordWeight(rec.data_ptr(), int(max_items) + 1))
-        elif t == 1:
-            src = f.source("keys", g.ZipfKeys(1 << 20, 1.1))
-            part = f.partition("bucket", g.KeyBucket(4096))
-            snk = f.sink("digest", g.ChunkDigest(0, 0, ms.data_ptr(), 4096))
-            f.connect(src, part)
-            for b in range(4096):
-                f.connect(part, snk, b)
+        if t == 1:
+            f, src = hash_flow(g, ms)
         else:
-            src, *_ = g.photon_network(f, tally.data_ptr())
+            f = g.Flow(g.PAYLOAD_PARTICLE if t == 0 else g.PAYLOAD_PHOTON)
+            if t == 0:
+                src, *_ = g.feedback_network(f, g.RecordWeight(rec.data_ptr(), int(max_items) + 1))
+            else:
+                src, *_ = g.photon_network(f, tally.data_ptr())
         flows[t] = (f, src)
-    for i in sample:
+    plain_t = 0.0
+    for i in small:
         f, src = flows[int(topo[i])]
         f.configure(g.Tuning(seed=int(seeds[i]), batch_chunks=1, chunk_capacity=8192, live_particles=16384,
                              schedule=g.SCHEDULE_HOST, photon=dict(g.PHOTON_DEFAULTS)))
@@ -262,17 +313,74 @@ def served_form(g, torch, jobs, rate, max_items, seed=42, plain=True):
         f.run(src, int(items[i]))
         torch.cuda.synchronize()
         plain_t += time.perf_counter() - t0
-    plain = int(items[sample].sum()) / plain_t if sample else None
-    out = {"value": served, "unit": "items/s", "jobs": jobs, "items": total, "arrival_rate_per_s": rate,
-           "per_topology": per_topo, "plain_sample_jobs": len(sample),
-           "n_range": [1000, int(max_items)], "slots": 8, "failed_jobs": len(bad),
-           "p50_latency_ms": float(np.percentile(lat, 50)), "p99_latency_ms": float(np.percentile(lat, 99)),
-           "makespan_s": (t_last - t_first) * 1e-3, "plain_one_at_a_time_items_per_s": plain,
-           "speedup_vs_plain": served / plain if plain else None,
-           "jobs_per_topology": {g.JOB_NAMES[t]: int((topo == t).sum()) for t in range(3)}}
-    log(f"[bench] served mixed stream at {rate:.0f} jobs/s: {jobs} jobs, {total} items, {served:.4g} items/s, "
-        f"p50 {out['p50_latency_ms']:.2f} ms, p99 {out['p99_latency_ms']:.2f} ms"
-        + (f"; plain one at a time {plain:.4g} items/s ({served / plain:.1f}x)" if plain else ""))
+    plain = int(items[small].sum()) / plain_t
+    out["plain_one_at_a_time_items_per_s"] = plain
+    out["plain_sample_jobs"] = len(small)
+    out["speedup_vs_plain"] = out["value"] / plain
+    log(f"[bench] served {jobs} jobs at {rate:.0f}/s: {out['value']:.4g} items/s, p50 {out['p50_latency_ms']:.1f} ms, "
+        f"p99 {out['p99_latency_ms']:.1f} ms; one at a time batched {one:.4g}, plain {plain:.4g} items/s")
+    for f, _ in flows.values():
+        f.close()
+    return out
+
+
+# ---- the CPU leg: reference CPU path (oracle restatement) and every form's CPU check ---------------
+def cpu_model():
+    try:
+        for line in subprocess.run(["lscpu"], capture_output=True, text=True).stdout.splitlines():
+            if line.startswith("Model name"):
+                return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return "unknown"
+
+
+def cpu_leg(g, flow, N, C, P, Q, rows, Nc, forms, hash_sets, photon_tallies, photons):
+    import numpy as np
+
+    import oracle as O
+
+    threads = min(os.cpu_count() or 1, 16)
+    graph = flow.neutral_graph()
+    base = {}
+    for th in (1, threads):
+        t0 = time.perf_counter()
+        r = O.run(graph, seed=42, items=Nc, capacity=C, live=P, batch=Q, mode=O.MODE_RUNNER, threads=th,
+                  hist_size=g.FEEDBACK_HIST_N)
+        base[th] = (Nc / (time.perf_counter() - t0), r["status"])
+        log(f"[bench] cpu: oracle runner schedule N={Nc} threads={th}: {base[th][0]:.4g} particles/s")
+    out = {"value": base[1][0], "unit": "particles/s", "cores": 1, "kind": "port",
+           "sample": f"oracle restatement of the reference CPU path (GraphRunner schedule, classify / plan / "
+                     f"scatter per batch), same network, N={Nc:.0e} C={C} P={P} Q={Q} seed 42, one thread, "
+                     f"status={base[1][1]}",
+           "all_cores": {"value": base[threads][0], "threads": threads, "status": base[threads][1]},
+           "nproc": os.cpu_count(), "cpu_model": cpu_model()}
+    checks = {}
+    # the headline histogram against the schedule-free model at the full N
+    t0 = time.perf_counter()
+    w, _ = O.feedback_scalar(42, N, threads=threads)
+    v, c = np.unique(w, return_counts=True)
+    checks["headline_histogram_equals_oracle"] = bool(len(v) == len(rows) and (rows[:, 0] == v).all()
+                                                      and (rows[:, 1] == c).all())
+    checks["headline_oracle_seconds"] = time.perf_counter() - t0
+    del w
+    # config 3: every Q's per-bucket multisets against the schedule-free model
+    if hash_sets:
+        n = next(iter(forms[k]["items"] for k in forms if k.startswith("hash_pipeline")))
+        msum, mcnt = O.keyed_scalar(42, n, threads=threads)
+        for q, got in hash_sets.items():
+            checks[f"hash_Q{q}_buckets_equal_oracle"] = bool((got[:, 0] == msum).all() and (got[:, 1] == mcnt).all())
+    # config 4: the batched photon tallies against the FP64 oracle on the same Philox words
+    if photon_tallies is not None:
+        t0 = time.perf_counter()
+        ref, steps = O.photon_scalar(42, photons, dict(g.PHOTON_DEFAULTS), threads=threads)
+        t = photon_tallies
+        checks["photon_fp64_max_bin_rel_err"] = float((np.abs(t - ref) / ref).max())
+        checks["photon_fp64_total_rel_err"] = float(abs(t.sum() - ref.sum()) / ref.sum())
+        checks["photon_fp64_oracle_steps"] = int(steps)
+        checks["photon_oracle_seconds"] = time.perf_counter() - t0
+    out["checks"] = checks
+    log(f"[bench] cpu checks: {checks}")
     return out
 
 
@@ -286,24 +394,25 @@ def main():
     ap.add_argument("--live", type=int, default=16384)
     ap.add_argument("--batch", type=int, default=8)
     ap.add_argument("--plain-items", type=float, default=2e6)
-    ap.add_argument("--cpu-items", type=float, default=2e7)
+    ap.add_argument("--cpu-items", type=float, default=2e7, help="CPU baseline sample (oracle runner schedule)")
     ap.add_argument("--no-cpu", action="store_true")
-    ap.add_argument("--no-plain", action="store_true")
-    ap.add_argument("--no-roofline", action="store_true")
-    ap.add_argument("--verify", action="store_true", help="compare the sorted histogram with the oracle")
+    ap.add_argument("--only-headline", action="store_true", help="the headline runs and roofline only")
     ap.add_argument("--engine", type=int, default=0, help="device engine: 0 autotuned, 1 wide, 2 micro, 3 team")
     ap.add_argument("--team", type=int, default=0, help="team engine: workgroups in the team (0 = default)")
     ap.add_argument("--no-autotune", action="store_true", help="skip the autotuner's probe runs")
     ap.add_argument("--phases", action="store_true", help="device engines: one more run with the per-phase profile")
     ap.add_argument("--members", action="store_true", help="team engine: with --phases, per-member phase times")
     ap.add_argument("--no-fuse", action="store_true", help="run A/B as separate transform batches")
-    ap.add_argument("--photons", type=float, default=1e7, help="photon transport form: photons (0 = skip)")
+    ap.add_argument("--photons", type=float, default=1e8, help="photon transport form: photons (0 = skip)")
     ap.add_argument("--plain-photons", type=float, default=1e6)
-    ap.add_argument("--hash-items", type=float, default=4e6, help="keyed hash pipeline form: keys (0 = skip)")
-    ap.add_argument("--hash-q", default="8,16", help="keyed hash pipeline form: batch sizes Q, comma separated")
-    ap.add_argument("--serve-jobs", type=int, default=256, help="served mixed stream: jobs (0 = skip)")
-    ap.add_argument("--serve-rate", type=float, default=1000.0, help="served mixed stream: Poisson arrivals per second")
-    ap.add_argument("--serve-max-items", type=float, default=1e6, help="served mixed stream: N ~ log-uniform[1e3, this]")
+    ap.add_argument("--hash-items", type=float, default=1e8, help="keyed hash pipeline form: keys (0 = skip)")
+    ap.add_argument("--hash-q", default="1,4,8,16", help="keyed hash pipeline form: batch sizes Q, comma separated")
+    ap.add_argument("--hash-lanes", type=int, default=32, help="keyed hash pipeline: inflight batches per node")
+    ap.add_argument("--hash-live", type=int, default=1 << 21, help="keyed hash pipeline: keys in flight (P)")
+    ap.add_argument("--serve-jobs", type=int, default=4096, help="served mixed stream: jobs (0 = skip)")
+    ap.add_argument("--serve-rate", type=float, default=80.0, help="served mixed stream: Poisson arrivals per second")
+    ap.add_argument("--serve-max-items", type=float, default=1e7, help="served mixed stream: N ~ log-uniform[1e3, this]")
+    ap.add_argument("--no-distributions", action="store_true")
     ap.add_argument("--copy-calib", type=float, default=0.0,
                     help="after the runs, copy this many MiB device to device once (the known-size kernel that "
                          "scales rocprofv3 FETCH_SIZE / WRITE_SIZE readings to bytes)")
@@ -327,20 +436,33 @@ def main():
     import torch
     import graphflow_amd as g
 
+    golden = golden_cases()
     log(f"[bench] batched form N={N} C={C} P={P} Q={Q}")
-    flow, src, counts = build_feedback(g, torch, N, C, P, Q, g.SCHEDULE_DEVICE, fuse=fuse, engine=args.engine,
+    flow, src, counts = build_feedback(g, torch, C, P, Q, g.SCHEDULE_DEVICE, fuse=fuse, engine=args.engine,
                                        team=args.team)
     tuned = None
     if not args.no_autotune and args.engine == 0 and not args.team:
         tuned = flow.autotune(src, probe_items=min(N, 10_000_000))
         log(f"[bench] autotuner picked {tuned}")
+    eng = tuned["device_engine"] if tuned else args.engine
+    wg = tuned["max_workgroups"] if tuned else (args.team or None)
+    tile = tuned["tile_items"] if tuned else None
+    # the team engine's launches are timed with HIP events on its stream in the timed runs themselves
+    # (a handful of launches per run); the wide engine keeps its hipGraph replay
+    timed_graph = 2 if eng in (0, 3) else 1
+    flow.configure(g.Tuning(seed=42, batch_chunks=Q, chunk_capacity=C, live_particles=P, schedule=g.SCHEDULE_DEVICE,
+                            use_graph=timed_graph, fuse_transforms=fuse, device_engine=eng, max_workgroups=wg or None,
+                            tile_items=tile or None))
     for i in range(args.warmup):
         dt, st = run_once(flow, src, counts, N, torch)
         log(f"[bench] warmup {i}: {dt:.3f} s, {N / dt:.4g} particles/s, steps={st['steps']}")
-    times = []
+    times, k_ms, k_n = [], 0.0, 0
     for i in range(args.steps):
         dt, st = run_once(flow, src, counts, N, torch)
         times.append(dt)
+        ms, n = flow.kernel_times()
+        k_ms += ms[3]
+        k_n += n[3]
         log(f"[bench] step {i}: {dt:.3f} s, {N / dt:.4g} particles/s, steps={st['steps']}")
     tmean = sum(times) / len(times)
     value = N / tmean
@@ -362,53 +484,39 @@ def main():
                    "histogram_rows": int(len(rows)), "histogram_sum": row_sum, "histogram_sha256": sha,
                    "autotuned": tuned},
     }
+    if golden and N == golden["items"]:
+        result["config"]["histogram_equals_golden"] = sha == golden["cases"]["parity"]["sha256"]
+        log(f"[bench] histogram equals the oracle fixture: {result['config']['histogram_equals_golden']}")
 
-    if args.verify:
-        import numpy as np
-        import oracle as O
-        w, _ = O.feedback_scalar(42, N, threads=min(os.cpu_count() or 1, 16))
-        v, c = np.unique(w, return_counts=True)
-        ok = len(v) == len(rows) and bool((rows[:, 0] == v).all()) and bool((rows[:, 1] == c).all())
-        result["config"]["histogram_equals_oracle"] = ok
-        log(f"[bench] histogram equals oracle: {ok}")
-
-    if not args.no_roofline:
-        # one extra run with HIP events around every engine kernel, on the engine's own stream
-        flow.configure(tuning_of(g, N, C, P, Q, fuse, args, tuned, use_graph=2))
-        counts.zero_()
-        tst = flow.run(src, N)
-        ms, n = flow.kernel_times()
-        part_items, xf_items = flow.item_passes()
-        # algorithmic bytes: 24 B per partition or transform item pass (one 12 B read and one 12 B
-        # write of the payload), 12 B per generated particle (write) and per sunk particle (read)
-        alg_bytes = 24 * part_items + 24 * xf_items + 12 * N + 12 * N
-        traffic = None
-        if n[3] and int(tst["engine"]) == 3:  # team engine: one persistent launch runs many steps
-            k_ms, k_n, kname = ms[3], n[3], "k_team (whole steps, team of workgroups on one XCD)"
-            others = {"steps_per_launch": int(tst["steps"]) / max(n[3], 1),
-                      "avg_step_us": ms[3] * 1e3 / max(int(tst["steps"]), 1)}
-            traffic = traffic_for(wkey, int(k_n))
-            if traffic is not None:
-                others["traffic_source"] = os.path.relpath(TRAFFIC_FILE, os.path.dirname(os.path.abspath(__file__)))
-        elif n[3]:  # micro engine: one persistent kernel does every phase
-            k_ms, k_n, kname = ms[3], n[3], "k_micro (whole steps in one workgroup)"
-            others = {}
-        else:
-            k_ms, k_n, kname = ms[2], n[2], "k_engine_kb (scatter+transform+source+sink of one step)"
-            others = {"ka_avg_launch_us": ms[0] * 1e3 / max(n[0], 1), "kp_avg_launch_us": ms[1] * 1e3 / max(n[1], 1)}
-        achieved = alg_bytes / (k_ms * 1e-3) / 1e9 if k_ms > 0 else 0.0
+    # roofline of the dominant kernel. The team engine's k_team runs whole steps: its bound is the
+    # step latency (P=16384 live particles, ~10.5K items a step, a chain of ~152K dependent steps),
+    # not HBM bandwidth, so the HBM fraction is reported against the 8 TB/s peak but bound = latency.
+    alg_bytes = 24 * part_items + 24 * xf_items + 12 * N + 12 * N  # partition/transform passes, source, sink
+    if k_n and int(st["engine"]) == 3:
+        per_run = k_n / args.steps
+        alg_launch = alg_bytes / per_run
+        avg_us = k_ms * 1e3 / k_n
+        achieved = alg_launch / (avg_us * 1e-6) / 1e9
+        tr = traffic_for(wkey)
         result["roofline"] = {
-            "bound": "hbm", "kernel": kname, "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-            "frac": achieved / HBM_PEAK_GBS, "traffic": traffic,
-            "algorithmic_bytes_per_launch": alg_bytes / max(k_n, 1), "avg_launch_us": k_ms * 1e3 / max(k_n, 1),
-            "launches": k_n, **others,
+            "bound": "latency", "kernel": "k_team (whole steps, team of workgroups on one XCD)",
+            "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS,
+            "traffic": tr[0] if tr else None,
+            "traffic_source": (os.path.relpath(TRAFFIC_FILE, ROOT) + " (committed rocprofv3 --pmc summary of this "
+                               "workload, not measured in this run)") if tr else None,
+            "algorithmic_bytes_per_launch": alg_launch, "avg_launch_us": avg_us, "launches": k_n,
+            "launch_times_from": f"HIP events on the engine stream around every k_team launch of the {args.steps} timed runs",
+            "steps_per_launch": int(st["steps"]) / per_run, "avg_step_us": k_ms * 1e3 / (int(st["steps"]) * args.steps),
+            "step_floor_note": "a step is two dependent single-wave chains plus one team barrier (dependent L2 "
+                               "load ~220 cycles, one-XCD barrier of 32 workgroups 0.5-0.8 us)",
         }
-        log(f"[bench] roofline: {kname} avg {k_ms * 1e3 / max(k_n, 1):.2f} us over {k_n} launches, "
-            f"achieved {achieved:.1f} GB/s ({achieved / HBM_PEAK_GBS:.2e} of peak), traffic/launch {traffic}")
+        log(f"[bench] roofline: k_team avg {avg_us:.1f} us over {k_n} launches, {achieved:.1f} GB/s "
+            f"({achieved / HBM_PEAK_GBS:.2e} of peak), {result['roofline']['avg_step_us']:.2f} us per step")
 
     if args.phases:
-        # per-phase device profile (slows the profiled member; not used for any reported rate)
-        flow.configure(tuning_of(g, N, C, P, Q, fuse, args, tuned, use_graph=3))
+        flow.configure(g.Tuning(seed=42, batch_chunks=Q, chunk_capacity=C, live_particles=P,
+                                schedule=g.SCHEDULE_DEVICE, use_graph=3, fuse_transforms=fuse, device_engine=eng,
+                                max_workgroups=wg or None, tile_items=tile or None))
         counts.zero_()
         pst = flow.run(src, N)
         phases, psteps = flow.phase_profile()
@@ -419,44 +527,35 @@ def main():
                     f"sched {r[3]:.2f} us, segments {r[4]:.2f} sinks {r[5]:.2f} per step, wave work "
                     + " ".join(f"{x:.2f}" for x in r[6:]))
 
-    forms = {}
-    if not args.no_plain:
+    forms, hash_sets, photon_tally = {}, {}, None
+    if not args.only_headline:
         Np = int(args.plain_items)
-        pflow, psrc, pcounts = build_feedback(g, torch, Np, C, P, 1, g.SCHEDULE_HOST)
-        dt, _ = run_once(pflow, psrc, pcounts, Np, torch)
-        forms["plain_host_planned_Q1"] = {"value": Np / dt, "unit": "particles/s", "items": Np}
-        log(f"[bench] plain form (host planned, Q=1) N={Np}: {Np / dt:.4g} particles/s")
-    if args.photons > 0:
-        forms.update(photon_forms(g, torch, int(args.photons), int(args.plain_photons), C, P, Q,
-                                  verify=not args.no_cpu))
-    if args.hash_items > 0:
-        forms.update(hash_form(g, torch, int(args.hash_items), tuple(int(q) for q in args.hash_q.split(",")),
-                               autotune=not args.no_autotune))
-    if args.serve_jobs > 0:
-        sat = served_form(g, torch, args.serve_jobs, args.serve_rate, args.serve_max_items)
-        forms["served_mixed_stream"] = sat
-        # the same stream offered at 70 % of the capacity the saturated run measured: latency
-        # without a standing queue
-        mean_items = sat["items"] / sat["jobs"]
-        rate70 = 0.7 * sat["value"] / mean_items
-        forms["served_mixed_stream_70pct_load"] = served_form(g, torch, args.serve_jobs, rate70, args.serve_max_items,
-                                                             plain=False)
+        for name, kw in (("plain_host_planned_Q1", {}), ("plain_fused_batches_Q1", {"fused_batches": True})):
+            pflow, psrc, pcounts = build_feedback(g, torch, C, P, 1, g.SCHEDULE_HOST, **kw)
+            dt, _ = run_once(pflow, psrc, pcounts, Np, torch)
+            forms[name] = {"value": Np / dt, "unit": "particles/s", "items": Np}
+            log(f"[bench] {name} N={Np}: {Np / dt:.4g} particles/s")
+            pflow.close()
+        if not args.no_distributions:
+            forms.update(distribution_forms(g, torch, N, C, P, Q, golden))
+        if args.hash_items > 0:
+            hf, hash_sets = hash_forms(g, torch, int(args.hash_items), tuple(int(q) for q in args.hash_q.split(",")),
+                                       args.hash_lanes, args.hash_live)
+            forms.update(hf)
+        if args.photons > 0:
+            pf, pt = photon_forms(g, torch, int(args.photons), int(args.plain_photons), C, P, Q)
+            forms.update(pf)
+            photon_tally = pt["photon_batched_team"]
+        if args.serve_jobs > 0:
+            forms["served_mixed_stream"] = served_forms(g, torch, args.serve_jobs, args.serve_max_items,
+                                                        args.serve_rate)
     result["forms"] = forms
 
     if not args.no_cpu:
-        import oracle as O
-        Nc = int(args.cpu_items)
-        graph = flow.neutral_graph()
-        t0 = time.perf_counter()
-        r = O.run(graph, seed=42, items=Nc, capacity=C, live=P, batch=Q, mode=O.MODE_STEP,
-                  hist_size=g.FEEDBACK_HIST_N)
-        dt = time.perf_counter() - t0
-        result["cpu_baseline"] = {
-            "value": Nc / dt, "unit": "particles/s", "cores": 1, "kind": "port",
-            "sample": f"oracle restatement of the reference CPU path, same network, N={Nc:.0e} "
-                      f"C={C} P={P} Q={Q} seed 42, single thread, status={r['status']}",
-        }
-        log(f"[bench] cpu baseline N={Nc}: {Nc / dt:.4g} particles/s ({dt:.1f} s)")
+        result["cpu_baseline"] = cpu_leg(g, flow, N, C, P, Q, rows, int(args.cpu_items), forms, hash_sets,
+                                         photon_tally, int(args.photons))
+        ck = result["cpu_baseline"]["checks"]
+        result["config"]["histogram_equals_oracle"] = ck["headline_histogram_equals_oracle"]
 
     if args.copy_calib > 0:
         nbytes = int(args.copy_calib * (1 << 20)) // 4 * 4

@@ -106,6 +106,8 @@ def lib():
                                             C.POINTER(C.c_float), u32p, u32p, u32p, u32p, u32p, u32p]
         _lib.or_feedback_scalar.argtypes = [C.c_uint64, C.c_uint64, C.c_float, C.c_uint32, C.c_uint32,
                                             C.POINTER(C.c_float), u32p, u32p, C.c_uint32]
+        _lib.or_keyed_scalar.argtypes = [C.c_uint64, C.c_uint64, C.c_uint32, C.c_double, C.c_uint32, u64p, u64p,
+                                         C.c_uint32]
         _lib.or_photon_scalar.argtypes = [C.c_uint64, C.c_uint64, C.POINTER(RunCfg),
                                           C.POINTER(C.c_double), u64p, C.c_uint32]
         f64p = C.POINTER(C.c_double)
@@ -250,6 +252,16 @@ def feedback_scalar(seed, items, threshold=1e-3, distribution=0, ndest=2, multip
     lib().or_feedback_scalar(seed, items, threshold, distribution, ndest, _p(m, C.c_float),
                              _p(w, C.c_uint32), _p(g, C.c_uint32), threads)
     return w, g
+
+
+def keyed_scalar(seed, items, values=1 << 20, s=1.1, bins=4096, threads=1):
+    ms = np.zeros(bins, np.uint64)
+    ct = np.zeros(bins, np.uint64)
+    # the source node carries the exponent as fp32 (gf_node_desc.f), so the table uses that value
+    rc = lib().or_keyed_scalar(seed, items, values, float(np.float32(s)), bins, _p(ms, C.c_uint64),
+                               _p(ct, C.c_uint64), threads)
+    assert rc == OK
+    return ms, ct
 
 
 def photon_scalar(seed, photons, photon_cfg, threads=1):

@@ -25,6 +25,8 @@
  *                                        (tests/integration/test_batching.cpp:577-602),
  *                                        credit-shrunk prefix (test_batching.cpp:550-575)
  *   or_zipf_table/or_zipf_sample         closed-form Zipf probabilities (top-1 / top-10 mass)
+ *   or_keyed_scalar                      equals or_run's per-bucket multisets and counts (step and
+ *                                        runner schedules) on the same seed
  *   or_hg_cos_f64                        closed-form HG moments <ct> = g, <ct^2> = (1 + 2 g^2) / 3
  *   or_hg_rotate_f64                     unit length and <u, u'> = ct for any u, both frames
  *   or_photon_scalar                     albedo 0 -> Beer-Lambert absorption profile N (e^-mu z0 - e^-mu z1)
@@ -543,6 +545,14 @@ static void fail(or_run_out* o, int status, const char* msg)
     }
 }
 
+/* OpenMP threads for a batch loop of n items (the reference's CPU backend runs a batch's blocks on
+ * the host cores): every result is independent of the thread count. */
+static int batch_threads(const or_ctx* x, uint64_t n)
+{
+    int t = x->cfg && x->cfg->threads ? (int)x->cfg->threads : 1;
+    return n >= 4096u ? t : 1;
+}
+
 /* gather the batch's items in batch order: chunk order, then slot order */
 static void gather(const or_pool* p, batch* b)
 {
@@ -560,12 +570,16 @@ static int partition_classify(const or_ctx* x, const or_node_desc* nd, const or_
 {
     gather(p, b);
     memset(b->counts, 0, sizeof(uint32_t) * nd->bins);
-    for (uint64_t i = 0; i < b->particles; ++i) {
+    const int64_t n = (int64_t)b->particles;
+    int bad = 0;
+#pragma omp parallel for schedule(static) num_threads(batch_threads(x, b->particles)) reduction(| : bad)
+    for (int64_t i = 0; i < n; ++i) {
         uint32_t bin = or_classify(x, nd, &b->items[i]);
-        if (bin >= nd->bins) return OR_INVALID;
+        if (bin >= nd->bins) bad = 1;
         b->bins[i] = bin;
-        b->counts[bin]++;
     }
+    if (bad) return OR_INVALID;
+    for (int64_t i = 0; i < n; ++i) b->counts[b->bins[i]]++;
     return OR_OK;
 }
 
@@ -610,22 +624,50 @@ static int partition_plan_scatter(const or_ctx* x, const or_node_desc* nd, uint3
         }
     }
     b->n_out = R;
-    /* stable scatter with the updater (purpose + 1) */
-    uint32_t* cursor = (uint32_t*)calloc(nd->bins, sizeof(uint32_t));
-    int urc = OR_OK;
+    /* stable scatter with the updater (purpose + 1): T contiguous item ranges, each range's start
+     * per bin is the count of the bin in the earlier ranges, so every item gets its sequential rank */
     (void)purpose;
-    for (uint64_t i = 0; i < b->particles; ++i) {
-        uint32_t bin = b->bins[i];
-        uint32_t j = cursor[bin]++;
-        or_item it = b->items[i];
-        int u = or_update(nd, bin, &it);
-        if (u != OR_OK && urc == OR_OK) urc = u;
-        int fz = fused_of(x, nd, bin);
-        if (fz >= 0) or_transform_item(x, &x->nodes[fz], 2u * (uint32_t)fz, &it);
-        uint32_t chunk = b->out[base[bin] + j / p->C].index;
-        p->slots[(size_t)chunk * p->C + j % p->C] = it;
+    const int T = batch_threads(x, b->particles);
+    const uint64_t n = b->particles, B = nd->bins;
+    uint32_t* cursor = (uint32_t*)calloc((size_t)T * B, sizeof(uint32_t));
+    int* urc_t = (int*)calloc((size_t)T, sizeof(int));
+#pragma omp parallel num_threads(T)
+    {
+#ifdef _OPENMP
+        const int t = omp_get_thread_num(), nt = omp_get_num_threads();
+#else
+        const int t = 0, nt = 1;
+#endif
+        const uint64_t lo = n * (uint64_t)t / (uint64_t)nt, hi = n * (uint64_t)(t + 1) / (uint64_t)nt;
+        uint32_t* cur = cursor + (size_t)t * B;
+        for (uint64_t i = lo; i < hi; ++i) cur[b->bins[i]]++;
+#pragma omp barrier
+#pragma omp single
+        {
+            for (uint64_t k = 0; k < B; ++k) {
+                uint32_t acc = 0;
+                for (int tt = 0; tt < nt; ++tt) {
+                    uint32_t c = cursor[(size_t)tt * B + k];
+                    cursor[(size_t)tt * B + k] = acc;
+                    acc += c;
+                }
+            }
+        }
+        for (uint64_t i = lo; i < hi; ++i) {
+            uint32_t bin = b->bins[i];
+            uint32_t j = cur[bin]++;
+            or_item it = b->items[i];
+            int u = or_update(nd, bin, &it);
+            if (u != OR_OK && urc_t[t] == OR_OK) urc_t[t] = u;
+            int fz = fused_of(x, nd, bin);
+            if (fz >= 0) or_transform_item(x, &x->nodes[fz], 2u * (uint32_t)fz, &it);
+            uint32_t chunk = b->out[base[bin] + j / p->C].index;
+            p->slots[(size_t)chunk * p->C + j % p->C] = it;
+        }
     }
-    free(cursor); free(base); free(fills);
+    int urc = OR_OK;
+    for (int t = 0; t < T && urc == OR_OK; ++t) urc = urc_t[t];
+    free(urc_t); free(cursor); free(base); free(fills);
     return urc;
 }
 
@@ -651,7 +693,9 @@ static int transform_batch(const or_ctx* x, const or_node_desc* nd, uint32_t pur
     }
     b->n_out = (uint32_t)R;
     b->counts[0] = (uint32_t)b->particles;
-    for (uint64_t i = 0; i < b->particles; ++i) {
+    const int64_t n = (int64_t)b->particles;
+#pragma omp parallel for schedule(static) num_threads(batch_threads(x, b->particles))
+    for (int64_t i = 0; i < n; ++i) {
         or_item it = b->items[i];
         or_transform_item(x, nd, purpose, &it);
         p->slots[(size_t)b->out[i / p->C].index * p->C + i % p->C] = it;
@@ -662,6 +706,27 @@ static int transform_batch(const or_ctx* x, const or_node_desc* nd, uint32_t pur
 static void sink_batch(const or_ctx* x, const or_node_desc* nd, const or_pool* p, batch* b)
 {
     or_run_out* o = x->out;
+    if (nd->op == OR_SNK_HIST && batch_threads(x, b->particles) > 1) {
+        /* histogram increments commute: the chunks' items in parallel */
+        int bad = 0;
+        for (uint32_t c = 0; c < b->n_in; ++c) {
+            const or_item* src = p->slots + (size_t)b->in[c].index * p->C;
+            const int64_t n = b->in[c].fill;
+#pragma omp parallel for schedule(static) num_threads(batch_threads(x, b->particles)) reduction(| : bad)
+            for (int64_t i = 0; i < n; ++i) {
+                uint32_t v = nd->u[0] == 0u ? 0u : src[i].gen;
+                if (nd->u[0] == 0u) memcpy(&v, &src[i].w, 4);
+                uint32_t rel = v - nd->u[1];
+                if (v < nd->u[1] || rel >= nd->u[2]) bad = 1;
+                else if (o->hist) {
+#pragma omp atomic
+                    o->hist[rel]++;
+                }
+            }
+        }
+        if (bad) fail(o, OR_INVALID, "sink: histogram value out of range");
+        return;
+    }
     for (uint32_t c = 0; c < b->n_in; ++c) {
         const or_item* src = p->slots + (size_t)b->in[c].index * p->C;
         uint64_t h0 = 0;
@@ -708,7 +773,9 @@ static void source_batch(const or_ctx* x, const or_node_desc* nd, uint32_t purpo
                          uint64_t first_id)
 {
     or_item* dst = p->slots + (size_t)b->out[0].index * p->C;
-    for (uint32_t i = 0; i < b->out[0].fill; ++i) or_source_item(x, nd, purpose, first_id + i, &dst[i]);
+    const int64_t n = b->out[0].fill;
+#pragma omp parallel for schedule(static) num_threads(batch_threads(x, (uint64_t)n))
+    for (int64_t i = 0; i < n; ++i) or_source_item(x, nd, purpose, first_id + (uint64_t)i, &dst[i]);
 }
 
 /* ------------------------------------------------------------------------------------------
@@ -1174,6 +1241,48 @@ int or_partition_items(const or_node_desc* node, const uint32_t* lut, uint64_t s
 }
 
 /* Schedule-free feedback model (tests/reference/ScalarModel.hpp:84-133 pattern, feedback-network functors). */
+/* Schedule-free model of the keyed hash pipeline: key i = Zipf(s) rank from the source's stream
+ * (seed, i, 0, purpose 0), value i, bucket fmix32(key) % bins; per bucket the sum of the item hashes
+ * and the item count. Every schedule delivers each key to its bucket exactly once, so these equal a
+ * run's per-bucket multiset digests whatever C, P, Q and lanes. */
+int or_keyed_scalar(uint64_t seed, uint64_t items, uint32_t values, double s, uint32_t bins, uint64_t* multiset,
+                    uint64_t* counts, uint32_t threads)
+{
+    if (!values || !bins) return OR_BAD_ARGUMENT;
+    double* cdf = (double*)malloc(sizeof(double) * values);
+    if (!cdf) return OR_BAD_ARGUMENT;
+    or_zipf_table(values, s, cdf);
+    memset(multiset, 0, sizeof(uint64_t) * bins);
+    memset(counts, 0, sizeof(uint64_t) * bins);
+    const int64_t n = (int64_t)items;
+#pragma omp parallel num_threads(threads ? threads : 1)
+    {
+        uint64_t* ms = (uint64_t*)calloc(bins, sizeof(uint64_t));
+        uint64_t* ct = (uint64_t*)calloc(bins, sizeof(uint64_t));
+#pragma omp for schedule(static)
+        for (int64_t i = 0; i < n; ++i) {
+            uint32_t r[4];
+            or_stream_words(seed, (uint64_t)i, 0, 0u, r);
+            or_item it;
+            memset(&it, 0, sizeof(it));
+            it.key = or_zipf_sample(cdf, values, r[0], r[1]);
+            it.val = (uint64_t)i;
+            const uint32_t b = or_fmix32(it.key) % bins;
+            ms[b] += item_hash(&it);
+            ct[b] += 1u;
+        }
+#pragma omp critical
+        for (uint32_t b = 0; b < bins; ++b) {
+            multiset[b] += ms[b];
+            counts[b] += ct[b];
+        }
+        free(ms);
+        free(ct);
+    }
+    free(cdf);
+    return OR_OK;
+}
+
 int or_feedback_scalar(uint64_t seed, uint64_t items, float threshold, uint32_t distribution, uint32_t ndest,
                        const float* multipliers, uint32_t* weight_bits, uint32_t* gens, uint32_t threads)
 {

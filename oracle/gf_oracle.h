@@ -176,6 +176,10 @@ int or_partition_items(const or_node_desc* node, const uint32_t* lut, uint64_t s
 int or_feedback_scalar(uint64_t seed, uint64_t items, float threshold, uint32_t distribution, uint32_t ndest,
                        const float* multipliers, uint32_t* weight_bits, uint32_t* gens, uint32_t threads);
 
+/* Schedule-free model of the keyed hash pipeline: per-bucket item-hash sums and counts. */
+int or_keyed_scalar(uint64_t seed, uint64_t items, uint32_t values, double s, uint32_t bins, uint64_t* multiset,
+                    uint64_t* counts, uint32_t threads);
+
 /* FP64 photon histories, schedule free, same RNG words as the device (ScalarTransport.hpp:87-211 pattern). */
 int or_hg_cos_f64(double g, const double* xi, double* ct, uint64_t n);
 int or_hg_rotate_f64(const double* u, const double* ct, const double* phi, double* out, uint64_t n);

@@ -62,3 +62,29 @@ def test_team_engine_rejects_lanes():
     flow, src, _ = feedback_flow(1000)
     with pytest.raises(ValueError):
         flow.configure(g.Tuning(seed=SEED, schedule=g.SCHEDULE_DEVICE, device_engine=3, inflight_batches=2))
+
+
+@pytest.mark.parametrize("Q", [1, 16])
+def test_config3_full_size_bucket_multisets(Q):
+    """Config 3 at full size: 1e8 Zipf(1.1) keys over 2^20 values into 4096 murmur3 buckets, C=8192,
+    32 lanes per node; every bucket's item count and item-hash sum equal the oracle's schedule-free
+    model on the same seed."""
+    torch = torch_cuda()
+    N, B, C, D, P = 100_000_000, 4096, 8192, 32, 1 << 21
+    mset = torch.zeros(2 * B, dtype=torch.int64, device="cuda")
+    flow = g.Flow(g.PAYLOAD_KEYED)
+    src = flow.source("keys", g.ZipfKeys(1 << 20, 1.1))
+    part = flow.partition("bucket", g.KeyBucket(B))
+    sink = flow.sink("digest", g.ChunkDigest(0, 0, mset.data_ptr(), B))
+    flow.connect(src, part)
+    for b in range(B):
+        flow.connect(part, sink, b)
+    S = min(524000, P // C + 2 * D * (Q + B) + 64)
+    flow.configure(g.Tuning(seed=SEED, batch_chunks=Q, chunk_capacity=C, live_particles=P, num_chunks=S,
+                            schedule=g.SCHEDULE_DEVICE, device_engine=1, max_workgroups=1024, tile_items=4096,
+                            inflight_batches=D))
+    st = flow.run(src, N)
+    assert st["sunk"] == N
+    ms, ct = O.keyed_scalar(SEED, N, threads=16)
+    got = mset.cpu().numpy().view(np.uint64).reshape(-1, 2)
+    assert (got[:, 1] == ct).all() and (got[:, 0] == ms).all()

@@ -86,3 +86,24 @@ def test_hash_table_exhaustion_aborts_the_batch(mode):
     # the dense path has no table
     r = O.run(HASH, hash_entries=8, **{**kw, "bin_path": 1})
     assert r["status"] == O.OK
+
+
+@pytest.mark.parametrize("mode,depth", [(O.MODE_RUNNER, 1), (O.MODE_STEP, 2)])
+def test_keyed_scalar_model_equals_the_scheduled_pipeline(mode, depth):
+    g = [dict(role=0, op="SRC_KEYED_ZIPF", bins=1, u=[1 << 20], f=[1.1], routes=[1]),
+         dict(role=1, op="CLS_KEY_BUCKET", bins=4096, routes=[2] * 4096),
+         dict(role=2, op="SNK_CHUNK_DIGEST", bins=0)]
+    n = 50000
+    r = O.run(g, seed=9, items=n, capacity=8192, live=32768, batch=4, mode=mode, multiset_bins=4096, inflight=depth)
+    assert r["status"] == O.OK, r["message"]
+    ms, ct = O.keyed_scalar(9, n, threads=4)
+    assert (ms == r["multiset"]).all() and (ct == r["multiset_items"]).all() and int(ct.sum()) == n
+
+
+def test_batch_loops_do_not_depend_on_threads():
+    kw = dict(seed=42, items=60000, capacity=8192, live=16384, batch=8, mode=O.MODE_RUNNER, trace_cap=1 << 23,
+              record=True)
+    a = O.run(FEEDBACK, threads=1, **kw)
+    b = O.run(FEEDBACK, threads=4, **kw)
+    assert a["status"] == b["status"] == O.OK
+    assert (a["trace"] == b["trace"]).all() and (a["record"] == b["record"]).all()
