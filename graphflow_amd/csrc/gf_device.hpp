@@ -203,11 +203,17 @@ struct PhotonParams {
     uint32_t tally_bins;
 };
 
+constexpr uint32_t kZipfGuide = 1u << 16;  // guide table intervals of the Zipf source
+
 struct RunParams {
     uint64_t seed;
     uint32_t C;
     const double* zipf;
     uint32_t zipf_values;
+    // Zipf guide table: for u in [j, j+1) / kZipfGuide the rank lies in [guide[2j], guide[2j+1]]
+    // (bounds taken with a 1e-12 relative margin, so the search below returns the rank of the
+    // full-range search); null: search the whole table
+    const uint32_t* zipf_guide;
     PhotonParams ph;
 };
 
@@ -390,8 +396,14 @@ __device__ inline void source(const gf_node_desc&, const RunParams& rp, uint32_t
 {
     uint32_t r[4];
     stream_words(rp.seed, id, 0, purpose, r);
-    double target = uniform_f64(r[0], r[1]) * rp.zipf[rp.zipf_values - 1u];
+    const double u = uniform_f64(r[0], r[1]);
+    double target = u * rp.zipf[rp.zipf_values - 1u];
     uint32_t lo = 0, hi = rp.zipf_values - 1u;
+    if (rp.zipf_guide) {
+        const uint32_t j = (uint32_t)(u * (double)kZipfGuide);  // u < 1: j < kZipfGuide
+        lo = rp.zipf_guide[2u * j];
+        hi = rp.zipf_guide[2u * j + 1u];
+    }
     while (lo < hi) {
         uint32_t mid = lo + (hi - lo) / 2u;
         if (rp.zipf[mid] > target) hi = mid;

@@ -655,6 +655,26 @@ bool stage_fits(uint32_t bins, uint32_t tile_items, uint32_t rec_bytes)
     return rec_bytes % 8u != 0u && bins <= 64u && scatter_smem(bins, tile_items, rec_bytes, true) <= 64u * 1024u;
 }
 
+// Guide of the Zipf source's inverse-CDF search (2 words per interval, see RunParams): interval j
+// of [0, 1) holds targets in [j, j + 1) / G of the total mass; its search range runs from the first
+// rank whose CDF exceeds the lower end less a 1e-12 relative margin to the first rank whose CDF
+// exceeds the upper end plus that margin, so the answer of the full search is always inside and the
+// range's last rank always exceeds the target (or is the last rank).
+std::vector<uint32_t> zipf_guide(const std::vector<double>& cdf)
+{
+    const uint32_t G = kZipfGuide, n = (uint32_t)cdf.size();
+    const double total = cdf[n - 1u];
+    std::vector<uint32_t> g(2u * G);
+    auto first_above = [&](double v) -> uint32_t {
+        return (uint32_t)std::min<size_t>(std::upper_bound(cdf.begin(), cdf.end(), v) - cdf.begin(), n - 1u);
+    };
+    for (uint32_t j = 0; j < G; ++j) {
+        g[2u * j] = first_above((double)j / G * total * (1.0 - 1e-12));
+        g[2u * j + 1u] = first_above((double)(j + 1u) / G * total * (1.0 + 1e-12));
+    }
+    return g;
+}
+
 std::vector<double> zipf_table(uint32_t values, double s)
 {
     std::vector<double> t(values);
@@ -828,8 +848,12 @@ struct gf_stage {
             auto t = zipf_table(nd.u[0], (double)nd.f[0]);
             d_zipf = mem.d<double>(t.size());
             HIPX(hipMemcpy(d_zipf, t.data(), t.size() * sizeof(double), hipMemcpyHostToDevice));
+            auto gd = zipf_guide(t);
+            uint32_t* d_guide = mem.d<uint32_t>(gd.size());
+            HIPX(hipMemcpy(d_guide, gd.data(), gd.size() * sizeof(uint32_t), hipMemcpyHostToDevice));
             rp.zipf = d_zipf;
             rp.zipf_values = nd.u[0];
+            rp.zipf_guide = d_guide;
         }
     }
 
@@ -1743,14 +1767,17 @@ void build_engine(gf_flow& f, uint64_t N, const RunShape& sh, uint32_t src)
             a.real = k;
             a.lanes = j == 0 ? lanes_of[k] : 0u;
             a.ring = ring;
-            uint32_t cap_tiles = (uint32_t)(ceil_div(sh.C, sh.tile_items) * (nd.role == GF_ROLE_SINK ? 1u : a.cap_in));
+            // a source's tiles hold one item per thread: each item's generation (Philox, a Zipf
+            // search) is a chain of dependent steps, so the chunk is spread over more workgroups
+            const uint32_t ti = nd.role == GF_ROLE_SOURCE ? (uint32_t)kThreads : sh.tile_items;
+            uint32_t cap_tiles = (uint32_t)(ceil_div(sh.C, ti) * (nd.role == GF_ROLE_SINK ? 1u : a.cap_in));
             a.in = m.d<gf_chunk_desc>(a.cap_in);
             a.tile_prefix = m.d<uint32_t>(a.cap_in + 1u);
             a.item_prefix = m.d<uint32_t>(a.cap_in + 1u);
             a.out = m.d<gf_chunk_desc>(a.cap_out);
             a.counts = part ? m.d<uint32_t>((size_t)cap_tiles * B) : nullptr;
             a.totals = m.d<uint32_t>(2u * B);
-            a.tile_items = sh.tile_items;
+            a.tile_items = ti;
             a.purpose = 2u * k;
             if (wave_sink[k]) a.flags |= NODE_WAVE_SINK;
             if (part) {
@@ -1775,8 +1802,12 @@ void build_engine(gf_flow& f, uint64_t N, const RunShape& sh, uint32_t src)
             auto z = zipf_table(nd.u[0], (double)nd.f[0]);
             double* dz = m.d<double>(z.size());
             copy_sync(dz, z.data(), z.size() * sizeof(double), hipMemcpyHostToDevice, f.ctx->stream);
+            auto gd = zipf_guide(z);
+            uint32_t* dg = m.d<uint32_t>(gd.size());
+            copy_sync(dg, gd.data(), gd.size() * sizeof(uint32_t), hipMemcpyHostToDevice, f.ctx->stream);
             e.rp.zipf = dz;
             e.rp.zipf_values = nd.u[0];
+            e.rp.zipf_guide = dg;
         }
     }
     if (lds_b > 160u * 1024u || lds_a > 160u * 1024u)
