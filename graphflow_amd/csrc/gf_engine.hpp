@@ -32,7 +32,7 @@ constexpr uint32_t kPlanWords = 8192; // [tile][bin] words of one partition plan
 // Dynamic per-node state (64 bytes), rewritten by the last workgroups.
 struct NodeSt {
     uint32_t active, n_in, n_tiles, particles;
-    uint32_t n_out, pad0, pad1, pad2;
+    uint32_t n_out, in_slot, pad1, pad2;  // in_slot: ring slot of the lane's first input
     unsigned long long first_id, digest_base, head, tail;
 };
 
@@ -79,12 +79,16 @@ struct Engine {
     unsigned long long trace_cap;
     unsigned long long* prof;  // optional per-phase wall-clock totals (micro engine), PROF_*
     unsigned long long* lb;    // look-back words of the grid plan, one per lane
+    unsigned long long* rlb;   // look-back words of the grid routing, [unit][lane]
+    unsigned long long* llb;   // look-back words of the grid input lists, [unit][2] (tiles, items)
+    uint32_t route_units_cap, list_units_cap;
 };
 
 enum { PROF_COUNT = 0, PROF_PLAN = 1, PROF_SOURCE = 2, PROF_PART = 3, PROF_XF = 4, PROF_SINK = 5, PROF_RETIRE = 6,
        PROF_SELECT = 7, PROF_STEPS = 8, PROF_WORDS = 32 };
 
-enum { CTL_STATUS = 0, CTL_DONE = 1, CTL_TOP = 2, CTL_TICKET_A = 3, CTL_TICKET_B = 4, CTL_HANDOFF = 5, CTL_WORDS = 8 };
+enum { CTL_STATUS = 0, CTL_DONE = 1, CTL_TOP = 2, CTL_TICKET_A = 3, CTL_TICKET_B = 4, CTL_HANDOFF = 5,
+       CTL_TICKET_C = 6, CTL_LIST_DONE = 7, CTL_WORDS = 8 };
 enum {
     CNT_GENERATED = 0, CNT_SUNK = 1, CNT_STEP = 2, CNT_BATCHES = 3, CNT_PART_ITEMS = 4, CNT_XF_ITEMS = 5,
     CNT_SRC_ITEMS = 6, CNT_SINK_ITEMS = 7, CNT_TRACE_LEN = 8, CNT_PEAK_LIVE = 9, CNT_FUSED_ITEMS = 10,
@@ -625,7 +629,10 @@ __device__ inline void engine_retire(const Engine& e, SelLds& L)
     __syncthreads();
 }
 
-__device__ inline void engine_select(const Engine& e, SelLds& L)
+// Select of the next step. With lists = false the input lists, tile / item prefixes and work
+// prefixes are left to the grid list kernel (engine_lists_grid), which reads each lane's inputs
+// from its ring at in_slot.
+__device__ inline void engine_select(const Engine& e, SelLds& L, bool lists = true)
 {
     const uint32_t tid = threadIdx.x, n = e.n_nodes, C = e.C, P = e.P, Q = e.Q;
     // stage the candidate ready descriptors of every transform's ring in LDS: <= Q per lane (the
@@ -695,6 +702,8 @@ __device__ inline void engine_select(const Engine& e, SelLds& L)
                 s.n_in = L.wanted[k];
                 s.active = s.n_in ? 1u : 0u;
                 L.in_at[k] = 0u;
+                s.in_slot = L.ring_at[k];
+                L.head[k] += s.n_in;
             } else {
                 // this lane's candidates follow the chunks the node's earlier lanes took
                 const uint32_t l = e.ring_of[k], at = L.taken[l];
@@ -719,13 +728,17 @@ __device__ inline void engine_select(const Engine& e, SelLds& L)
                 s.active = wanted ? 1u : 0u;
                 L.in_at[k] = at;
                 L.taken[l] = at + wanted;
+                uint32_t slot = L.ring_at[l] + at;  // at < ring occupancy <= ring_cap
+                if (slot >= e.ring_cap) slot -= e.ring_cap;
+                s.in_slot = slot;
+                L.head[l] += wanted;
             }
             L.any |= s.active;
         }
     }
     __syncthreads();
     // input lists and tile / item prefixes of every active non-source node
-    for (uint32_t k = 0; k < n; ++k) {
+    for (uint32_t k = 0; k < n && lists; ++k) {
         NodeSt& s = L.st[k];
         if (!s.active || e.arr[k].kind == KIND_SOURCE) continue;
         const NodeArr& a = e.arr[k];
@@ -771,7 +784,6 @@ __device__ inline void engine_select(const Engine& e, SelLds& L)
             a.item_prefix[m] = carry_i;
             s.n_tiles = carry_t;
             s.particles = carry_i;
-            L.head[l] += m;
         }
         __syncthreads();
     }
@@ -779,8 +791,10 @@ __device__ inline void engine_select(const Engine& e, SelLds& L)
     if (tid == 0) {
         uint32_t ka = 0, kb = 0;
         for (uint32_t k = 0; k < n; ++k) {
-            e.ka_prefix[k] = ka;
-            e.kb_prefix[k] = kb;
+            if (lists) {
+                e.ka_prefix[k] = ka;
+                e.kb_prefix[k] = kb;
+            }
             NodeSt& s = L.st[k];
             s.head = L.head[k];
             s.tail = L.tail[k];
@@ -789,10 +803,13 @@ __device__ inline void engine_select(const Engine& e, SelLds& L)
                 kb += s.n_tiles;
             }
         }
-        e.ka_prefix[n] = ka;
-        e.kb_prefix[n] = kb;
+        if (lists) {
+            e.ka_prefix[n] = ka;
+            e.kb_prefix[n] = kb;
+        }
         e.ctl[CTL_TOP] = L.top;
         e.ctl[CTL_TICKET_A] = 0u;  // the next step's grid plan takes its lanes from ticket 0
+        e.ctl[CTL_TICKET_B] = 0u;  // and its grid routing its units
         e.ctr[CNT_GENERATED] = L.generated;
         e.ctr[CNT_SUNK] = L.sunk;
         e.ctr[CNT_STEP] = L.step;
@@ -1030,6 +1047,318 @@ __device__ inline void engine_trace_step(const Engine& e, unsigned long long* le
         if (kind == KIND_PARTITION) trace_record(e, k, b, e.nodes[k].bins, e.arr[k].totals, len_sh, step);
         else if (kind == KIND_TRANSFORM) trace_record(e, k, b, 1u, e.arr[k].totals, len_sh, step);
         else trace_record(e, k, b, 0u, nullptr, len_sh, step);
+    }
+}
+
+// ------------------------------------------------------------------------------------------
+// KR over the grid: routing + release (engine_route_grid), the step's counters and the select in
+// one workgroup (engine_retire_counters + engine_select without lists), input lists over the grid
+// (engine_lists_grid). Same results as engine_retire + engine_select.
+// ------------------------------------------------------------------------------------------
+constexpr uint32_t kGridBlock = 4096;  // outputs (routing) or inputs (lists) per unit: 1024 threads x 4
+
+// Lane owning flat item j: the k with prefix[k] <= j < prefix[k+1] (prefix has n+1 entries).
+__device__ inline uint32_t lane_of(const uint32_t* prefix, uint32_t n, uint32_t j)
+{
+    uint32_t lo = 0, hi = n;  // first index in [0, n] with prefix > j, minus one
+    while (lo < hi) {
+        const uint32_t mid = (lo + hi) >> 1;
+        if (prefix[mid] <= j) lo = mid + 1u;
+        else hi = mid;
+    }
+    return lo - 1u;
+}
+
+// Exclusive prefix of the published values before unit u over words[stride * j + col] (thread-local).
+__device__ inline unsigned long long lb_lookback_col(unsigned long long* words, uint32_t u, uint32_t stride,
+                                                     uint32_t col, uint32_t first, uint32_t epoch)
+{
+    unsigned long long acc = 0;
+    for (int32_t j = (int32_t)u - 1; j >= (int32_t)first; --j) {
+        unsigned long long w;
+        for (;;) {
+            w = __hip_atomic_load(&words[(size_t)j * stride + col], __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT);
+            if ((uint32_t)(w >> 36) == (epoch & 0xFFFFFFFu) && ((w >> 34) & 3ull) != 0ull) break;
+            __builtin_amdgcn_s_sleep(1);
+        }
+        acc += w & ((1ull << 34) - 1ull);
+        if (((w >> 34) & 3ull) == LB_INC) break;
+    }
+    return acc;
+}
+
+struct RouteLds {
+    uint32_t off[kMaxNodes + 1];   // outputs before lane k (node order; active non-sink lanes)
+    uint32_t ioff[kMaxNodes + 1];  // inputs before lane k (active non-source lanes)
+    uint32_t cnt[kMaxWaves][kMaxNodes];
+    uint32_t run[kMaxNodes];       // per destination ring: this unit's outputs so far
+    uint32_t base[kMaxNodes];      // per ring: outputs of the earlier units
+    uint32_t ring_at[kMaxNodes];   // per ring: slot of its tail at the step's start
+    uint32_t unit, units, total, total_in, top;
+};
+
+// Routing of the step's outputs to the destination rings, in order, and the release of its inputs.
+// Units of kGridBlock outputs are taken by ticket; a unit ranks its outputs per destination ring,
+// and the ring offsets of the earlier units come from one decoupled look-back per ring.
+__device__ inline void engine_route_grid(const Engine& e, RouteLds& L)
+{
+    const uint32_t tid = threadIdx.x, n = e.n_nodes, w = tid >> 6;
+    if (tid == 0) {
+        uint32_t acc = 0, ai = 0;
+        for (uint32_t k = 0; k < n; ++k) {
+            const NodeSt& st = e.st[k];
+            const uint32_t kind = e.arr[k].kind;
+            L.off[k] = acc;
+            L.ioff[k] = ai;
+            if (st.active && kind != KIND_SINK) acc += st.n_out;
+            if (st.active && kind != KIND_SOURCE) ai += st.n_in;
+        }
+        L.off[n] = acc;
+        L.ioff[n] = ai;
+        L.total = acc;
+        L.total_in = ai;
+        L.units = (acc + kGridBlock - 1u) / kGridBlock;
+        L.top = e.ctl[CTL_TOP];
+    }
+    for (uint32_t k = tid; k < n; k += bsz()) L.ring_at[k] = (uint32_t)(e.st[k].tail % e.ring_cap);
+    __syncthreads();
+    const uint32_t epoch = (uint32_t)e.ctr[CNT_STEP];
+    // every consumed input back on the free stack (generation + 1), above the step's pops
+    for (uint32_t j = blockIdx.x * bsz() + tid; j < L.total_in; j += gridDim.x * bsz()) {
+        const uint32_t k = lane_of(L.ioff, n, j);
+        const gf_chunk_desc d = e.arr[k].in[j - L.ioff[k]];
+        e.free_stack[L.top + j] = (unsigned long long)d.index | ((unsigned long long)(d.generation + 1u) << 32);
+    }
+    for (;;) {
+        if (tid == 0) L.unit = atomicAdd(&e.ctl[CTL_TICKET_B], 1u);
+        for (uint32_t k = tid; k < n; k += bsz()) L.run[k] = 0u;
+        __syncthreads();
+        const uint32_t u = L.unit;
+        if (u >= L.units) return;
+        gf_chunk_desc dd[kAhead];
+        uint32_t ds[kAhead], rk[kAhead];
+#pragma unroll
+        for (uint32_t r = 0; r < kAhead; ++r) {
+            const uint32_t j = u * kGridBlock + r * bsz() + tid;
+            const bool v = j < L.total;
+            dd[r] = gf_chunk_desc{0, 0, 0, 0};
+            ds[r] = 0xFFFFFFFFu;
+            if (v) {
+                const uint32_t k = lane_of(L.off, n, j);
+                dd[r] = e.arr[k].out[j - L.off[k]];
+                ds[r] = (uint32_t)e.routes[e.nodes[k].route_offset + dd[r].bin];
+            }
+            for (uint32_t i = tid; i < kMaxWaves * n; i += bsz()) (&L.cnt[0][0])[(i / n) * kMaxNodes + i % n] = 0u;
+            __syncthreads();
+            const uint64_t peers = peer_mask(v, ds[r]);
+            if (v && (peers & lanemask_lt()) == 0ull) L.cnt[w][ds[r]] = (uint32_t)__popcll(peers);
+            __syncthreads();
+            if (v) {
+                uint32_t pre = L.run[ds[r]];
+                for (uint32_t ww = 0; ww < w; ++ww) pre += L.cnt[ww][ds[r]];
+                rk[r] = pre + (uint32_t)__popcll(peers & lanemask_lt());
+            }
+            __syncthreads();
+            for (uint32_t k = tid; k < n; k += bsz()) {
+                uint32_t sum = 0;
+                for (uint32_t ww = 0; ww < bwaves(); ++ww) sum += L.cnt[ww][k];
+                L.run[k] += sum;
+            }
+            __syncthreads();
+        }
+        // per ring: publish this unit's count, look back for the earlier units', publish the prefix
+        if (tid < n) {
+            unsigned long long* word = &e.rlb[(size_t)u * n + tid];
+            lb_publish(word, lb_word(epoch, u == 0u ? LB_INC : LB_AGG, L.run[tid]));
+            const unsigned long long b = lb_lookback_col(e.rlb, u, n, tid, 0u, epoch);
+            if (u != 0u) lb_publish(word, lb_word(epoch, LB_INC, b + L.run[tid]));
+            L.base[tid] = (uint32_t)b;
+        }
+        __syncthreads();
+#pragma unroll
+        for (uint32_t r = 0; r < kAhead; ++r) {
+            if (ds[r] == 0xFFFFFFFFu) continue;
+            // a ring never holds more than ring_cap (= S) chunks: one subtraction wraps the slot
+            uint32_t slot = L.ring_at[ds[r]] + L.base[ds[r]] + rk[r];
+            if (slot >= e.ring_cap) slot -= e.ring_cap;
+            e.rings[(size_t)e.arr[ds[r]].ring * e.ring_cap + slot] = dd[r];
+        }
+        if (u + 1u == L.units)
+            for (uint32_t k = tid; k < n; k += bsz()) e.st[k].tail += L.base[k] + L.run[k];
+        __syncthreads();
+    }
+}
+
+// The rest of the retire (one workgroup, after the grid routing): counters, the inputs' release
+// offset on the free stack, every lane inactive; then the caller runs engine_select(e, L, false).
+__device__ inline void engine_retire_counters(const Engine& e, SelLds& L)
+{
+    const uint32_t tid = threadIdx.x, n = e.n_nodes;
+    load_states(e, L.st);
+    __syncthreads();
+    for (uint32_t k = tid; k < n; k += bsz()) {
+        L.tail[k] = L.st[k].tail;
+        L.head[k] = L.st[k].head;
+    }
+    if (tid == 0) {
+        L.top = e.ctl[CTL_TOP];
+        L.generated = e.ctr[CNT_GENERATED];
+        L.sunk = e.ctr[CNT_SUNK];
+        L.step = e.ctr[CNT_STEP];
+        L.batches = e.ctr[CNT_BATCHES];
+        L.part_items = e.ctr[CNT_PART_ITEMS];
+        L.xf_items = e.ctr[CNT_XF_ITEMS];
+        L.src_items = e.ctr[CNT_SRC_ITEMS];
+        L.sink_items = e.ctr[CNT_SINK_ITEMS];
+        uint32_t released = 0;
+        for (uint32_t k = 0; k < n; ++k) {
+            if (!L.st[k].active) continue;
+            const uint32_t kind = e.arr[k].kind;
+            if (kind != KIND_SOURCE) released += L.st[k].n_in;
+            if (kind == KIND_SINK) {
+                L.sunk += L.st[k].particles;
+                L.sink_items += L.st[k].particles;
+            } else if (kind == KIND_SOURCE) {
+                L.src_items += L.st[k].particles;
+            } else {
+                L.batches += 1u;
+                if (kind == KIND_PARTITION) L.part_items += L.st[k].particles;
+                else L.xf_items += L.st[k].particles;
+            }
+            L.st[k].active = 0u;
+        }
+        L.top += released;
+        L.step += 1u;
+    }
+    __syncthreads();
+}
+
+struct ListLds {
+    uint32_t uoff[kMaxNodes + 1];  // list units before lane k (active non-source lanes)
+    uint32_t scan[kMaxWaves];
+    uint32_t unit, units, k, first, last, ex_t, ex_i, tot_t, tot_i;
+};
+
+// Input lists and tile / item prefixes of every active non-source lane: units of kGridBlock inputs
+// in (lane, position) order by ticket; a lane's prefixes continue across its units through a
+// decoupled look-back over the lane's earlier units. The workgroup that completes the last unit
+// writes the step's work prefixes.
+__device__ inline void engine_lists_grid(const Engine& e, ListLds& L)
+{
+    const uint32_t tid = threadIdx.x, n = e.n_nodes;
+    if (tid == 0) {
+        uint32_t acc = 0;
+        for (uint32_t k = 0; k < n; ++k) {
+            L.uoff[k] = acc;
+            const NodeSt& st = e.st[k];
+            if (st.active && e.arr[k].kind != KIND_SOURCE) acc += (st.n_in + kGridBlock - 1u) / kGridBlock;
+        }
+        L.uoff[n] = acc;
+        L.units = acc;
+    }
+    __syncthreads();
+    const uint32_t epoch = (uint32_t)e.ctr[CNT_STEP];
+    bool finisher = false;
+    if (L.units == 0u) {
+        finisher = blockIdx.x == 0;
+    } else {
+        for (;;) {
+            if (tid == 0) L.unit = atomicAdd(&e.ctl[CTL_TICKET_C], 1u);
+            __syncthreads();
+            const uint32_t u = L.unit;
+            if (u >= L.units) break;
+            if (tid == 0) {
+                L.k = lane_of(L.uoff, n, u);
+                L.first = L.uoff[L.k];
+                L.last = L.uoff[L.k + 1u] - 1u;
+            }
+            __syncthreads();
+            const uint32_t k = L.k;
+            const NodeSt st = e.st[k];
+            const NodeArr& a = e.arr[k];
+            const uint32_t l = e.ring_of[k], T = a.tile_items;
+            const uint32_t i0 = (u - L.first) * kGridBlock;
+            gf_chunk_desc dd[kAhead];
+            uint32_t tl[kAhead], sum_t = 0, sum_i = 0;
+#pragma unroll
+            for (uint32_t r = 0; r < kAhead; ++r) {
+                const uint32_t i = i0 + tid * kAhead + r;
+                dd[r] = gf_chunk_desc{0, 0, 0, 0};
+                if (i < st.n_in) {
+                    uint32_t slot = st.in_slot + i;
+                    if (slot >= e.ring_cap) slot -= e.ring_cap;
+                    dd[r] = e.rings[(size_t)e.arr[l].ring * e.ring_cap + slot];
+                }
+                tl[r] = (dd[r].fill + T - 1u) / T;
+                sum_t += tl[r];
+                sum_i += dd[r].fill;
+            }
+            uint32_t tt, ti;
+            uint32_t pt = block_exscan(sum_t, L.scan, &tt);
+            uint32_t pi = block_exscan(sum_i, L.scan, &ti);
+            if (tid == 0) {
+                unsigned long long* wt = &e.llb[2ull * u];
+                const bool head = u == L.first;
+                lb_publish(wt, lb_word(epoch, head ? LB_INC : LB_AGG, tt));
+                lb_publish(wt + 1, lb_word(epoch, head ? LB_INC : LB_AGG, ti));
+                const unsigned long long bt = head ? 0ull : lb_lookback_col(e.llb, u, 2u, 0u, L.first, epoch);
+                const unsigned long long bi = head ? 0ull : lb_lookback_col(e.llb, u, 2u, 1u, L.first, epoch);
+                if (!head) {
+                    lb_publish(wt, lb_word(epoch, LB_INC, bt + tt));
+                    lb_publish(wt + 1, lb_word(epoch, LB_INC, bi + ti));
+                }
+                L.ex_t = (uint32_t)bt;
+                L.ex_i = (uint32_t)bi;
+                L.tot_t = (uint32_t)bt + tt;
+                L.tot_i = (uint32_t)bi + ti;
+            }
+            __syncthreads();
+            pt += L.ex_t;
+            pi += L.ex_i;
+#pragma unroll
+            for (uint32_t r = 0; r < kAhead; ++r) {
+                const uint32_t i = i0 + tid * kAhead + r;
+                if (i < st.n_in) {
+                    a.in[i] = dd[r];
+                    a.tile_prefix[i] = pt;
+                    a.item_prefix[i] = pi;
+                }
+                pt += tl[r];
+                pi += dd[r].fill;
+            }
+            if (tid == 0 && u == L.last) {
+                a.tile_prefix[st.n_in] = L.tot_t;
+                a.item_prefix[st.n_in] = L.tot_i;
+                e.st[k].n_tiles = L.tot_t;
+                e.st[k].particles = L.tot_i;
+            }
+            __syncthreads();
+            if (tid == 0) {
+                // the unit's stores are ordered before its completion count; the finisher reads
+                // every lane's n_tiles after an acquire
+                const uint32_t done = __hip_atomic_fetch_add(&e.ctl[CTL_LIST_DONE], 1u, __ATOMIC_ACQ_REL,
+                                                             __HIP_MEMORY_SCOPE_AGENT);
+                L.k = done + 1u == L.units ? 1u : 0u;
+            }
+            __syncthreads();
+            if (L.k) finisher = true;
+        }
+    }
+    if (finisher && tid == 0) {
+        uint32_t ka = 0, kb = 0;
+        for (uint32_t k = 0; k < n; ++k) {
+            e.ka_prefix[k] = ka;
+            e.kb_prefix[k] = kb;
+            const NodeSt& st = e.st[k];
+            if (st.active) {
+                if (e.arr[k].kind == KIND_PARTITION) ka += st.n_tiles;
+                kb += st.n_tiles;
+            }
+        }
+        e.ka_prefix[n] = ka;
+        e.kb_prefix[n] = kb;
+        e.ctl[CTL_LIST_DONE] = 0u;
+        e.ctl[CTL_TICKET_C] = 0u;
     }
 }
 

@@ -316,16 +316,34 @@ __global__ __launch_bounds__(kThreads) void k_engine_kb(Engine e)
     }
 }
 
-// Retire + select of the step (one workgroup of 1024 threads, after every tile of KB): outputs to
-// the destination rings, inputs back on the free stack, the next step's batches.
+// Retire of the step over the grid (after every tile of KB): outputs to the destination rings in
+// order, inputs back on the free stack.
+__global__ __launch_bounds__(1024) void k_engine_route(Engine e)
+{
+    __shared__ RouteLds L;
+    __shared__ uint32_t flag;
+    if (engine_stopped(e, &flag)) return;
+    engine_route_grid(e, L);
+}
+
+// The step's counters and the select of the next step's batches (one workgroup of 1024 threads).
 __global__ __launch_bounds__(1024) void k_engine_kr(Engine e)
 {
     extern __shared__ char smem[];
     __shared__ uint32_t flag;
     if (engine_stopped(e, &flag)) return;
     SelLds& L = *reinterpret_cast<SelLds*>(smem);
-    engine_retire(e, L);
-    engine_select(e, L);
+    engine_retire_counters(e, L);
+    engine_select(e, L, false);
+}
+
+// The selected lanes' input lists, tile / item prefixes and the work prefixes, over the grid.
+__global__ __launch_bounds__(1024) void k_engine_lists(Engine e)
+{
+    __shared__ ListLds L;
+    __shared__ uint32_t flag;
+    if (engine_stopped(e, &flag)) return;
+    engine_lists_grid(e, L);
 }
 
 // Micro engine: one persistent workgroup of G tile groups (256 threads each) runs whole steps with
@@ -418,6 +436,8 @@ __global__ __launch_bounds__(kThreads) void k_engine_setup(Engine e, uint32_t do
         e.free_stack[i] = (unsigned long long)(e.S - 1u - i);  // index 0 first (ChunkLedger.hpp:43-46), generation 0
     for (uint32_t i = tid; i < CNT_WORDS; i += kThreads) e.ctr[i] = 0ull;
     for (uint32_t i = tid; i < CTL_WORDS; i += kThreads) e.ctl[i] = 0u;
+    for (uint64_t i = tid; i < (uint64_t)e.route_units_cap * e.n_nodes; i += kThreads) e.rlb[i] = 0ull;
+    for (uint64_t i = tid; i < 2ull * e.list_units_cap; i += kThreads) e.llb[i] = 0ull;
     if (e.prof)
         for (uint32_t i = tid; i < PROF_WORDS; i += kThreads) e.prof[i] = 0ull;
     for (uint32_t k = tid; k < e.n_nodes; k += kThreads) {
@@ -1271,7 +1291,7 @@ struct EngineBuffers {
     DevMem mem;
     std::string key;
     Engine e{};
-    uint32_t stage_mask = 0, wave_sinks = 0, r = 4, grid = 256;
+    uint32_t stage_mask = 0, wave_sinks = 0, r = 4, grid = 256, route_grid = 1;
     size_t lds_a = 0, lds_b = 0;
     bool micro = false;
     uint32_t group_lds = 0;
@@ -1844,6 +1864,12 @@ void build_engine(gf_flow& f, uint64_t N, const RunShape& sh, uint32_t src)
     e.trace = t.trace ? m.d<uint32_t>(e.trace_cap) : nullptr;
     e.prof = t.use_graph == 3u ? m.d<unsigned long long>(PROF_WORDS) : nullptr;
     e.lb = m.d<unsigned long long>(nv);
+    // grid retire: a step routes at most S outputs; its input lists hold at most S chunks
+    e.route_units_cap = (uint32_t)ceil_div(sh.S, kGridBlock) + 1u;
+    e.list_units_cap = (uint32_t)ceil_div(sh.S, kGridBlock) + nv + 1u;
+    e.rlb = m.d<unsigned long long>((size_t)e.route_units_cap * nv);
+    e.llb = m.d<unsigned long long>(2ull * e.list_units_cap);
+    eb->route_grid = std::min<uint32_t>(64u, e.route_units_cap + nv);
     const uint32_t gmax = t.max_workgroups ? t.max_workgroups : 64u;
     eb->grid = std::min<uint32_t>(std::max<uint32_t>(grid_tiles, 1u), gmax);
     eb->r = r;
@@ -1913,8 +1939,9 @@ void build_engine(gf_flow& f, uint64_t N, const RunShape& sh, uint32_t src)
     f.eng = std::move(eb);
 }
 
-// One step: KA (classify/count tiles), KP (plan, one workgroup), KB (scatter/transform/source/sink
-// tiles), KR (retire + select, one workgroup). With `ev`, events are recorded around KA, KP and
+// One step: KA (classify/count tiles), KP (plan, a workgroup per active lane), KB (scatter /
+// transform / source / sink tiles), KR (routing and release over the grid, counters and select in
+// one workgroup, input lists over the grid). With `ev`, events are recorded around KA, KP and
 // KB + KR.
 void launch_step(const EngineBuffers& eb, uint32_t payload, hipStream_t s, hipEvent_t* ev = nullptr)
 {
@@ -1928,7 +1955,9 @@ void launch_step(const EngineBuffers& eb, uint32_t payload, hipStream_t s, hipEv
         if (eb.e.trace_on) hipLaunchKernelGGL(k_engine_trace, dim3(1), dim3(1024), 0, s, eb.e);
         if (ev) HIPX(hipEventRecord(ev[2], s));
         hipLaunchKernelGGL((k_engine_kb<P, RR>), dim3(eb.grid), dim3(kThreads), eb.lds_b, s, eb.e);
+        hipLaunchKernelGGL(k_engine_route, dim3(eb.route_grid), dim3(1024), 0, s, eb.e);
         hipLaunchKernelGGL(k_engine_kr, dim3(1), dim3(1024), sizeof(SelLds), s, eb.e);
+        hipLaunchKernelGGL(k_engine_lists, dim3(eb.route_grid), dim3(1024), 0, s, eb.e);
         if (ev) HIPX(hipEventRecord(ev[3], s));
     });
 }
