@@ -531,6 +531,8 @@ def main():
     if not args.only_headline:
         Np = int(args.plain_items)
         for name, kw in (("plain_host_planned_Q1", {}), ("plain_fused_batches_Q1", {"fused_batches": True})):
+            if Np <= 0:
+                break
             pflow, psrc, pcounts = build_feedback(g, torch, C, P, 1, g.SCHEDULE_HOST, **kw)
             dt, _ = run_once(pflow, psrc, pcounts, Np, torch)
             forms[name] = {"value": Np / dt, "unit": "particles/s", "items": Np}
