@@ -708,12 +708,12 @@ __device__ void scatter_tile_hash(const gf_node_desc& nd, const Batch& b, uint32
     __syncthreads();
 }
 
-// LDS bytes of scatter_tile.
+// LDS bytes of scatter_tile: the per-wave running counts; with staging also the tile totals, their
+// prefix and the staged records.
 __host__ __device__ inline size_t scatter_smem(uint32_t bins, uint32_t tile_items, uint32_t rec_bytes, bool stage)
 {
-    size_t s = (size_t)(kWaves + 2) * bins * 4u;
-    if (stage) s += (size_t)tile_items * (rec_bytes + 4u);
-    return s;
+    if (!stage) return (size_t)kWaves * bins * 4u;
+    return (size_t)(kWaves + 2) * bins * 4u + (size_t)tile_items * (rec_bytes + 4u);
 }
 
 // Stable scatter of one tile (Execution.hpp:1096-1187 / :1327-1502 semantics with the canonical
@@ -731,8 +731,8 @@ __device__ void scatter_tile(const gf_node_desc& nd, const Batch& b, uint32_t t,
 {
     const uint32_t B = nd.bins, tid = ltid(), w = tid >> 6, lane = lane_id();
     uint32_t* wcnt = (uint32_t*)smem;      // [kWaves][B] running, then exclusive wave prefix
-    uint32_t* ttot = wcnt + kWaves * B;     // [B] tile totals
-    uint32_t* tstart = ttot + B;            // [B] exclusive prefix of ttot over bins
+    uint32_t* ttot = wcnt + kWaves * B;     // Stage: [B] tile totals
+    uint32_t* tstart = ttot + B;            // Stage: [B] exclusive prefix of ttot over bins
     P* stage = (P*)(tstart + B);            // [T]
     uint32_t* flat = (uint32_t*)(stage + (Stage ? b.tile_items : 0u));
     for (uint32_t i = tid; i < kWaves * B; i += kThreads) wcnt[i] = 0u;
@@ -768,7 +768,7 @@ __device__ void scatter_tile(const gf_node_desc& nd, const Batch& b, uint32_t t,
             wcnt[ww * B + k] = acc;
             acc += x;
         }
-        ttot[k] = acc;
+        if constexpr (Stage) ttot[k] = acc;
     }
     __syncthreads();
     const uint32_t* toff = b.counts + (size_t)(t == kNoTile ? 0u : t) * B;

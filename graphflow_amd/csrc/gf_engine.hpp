@@ -473,17 +473,32 @@ struct SelLds {
     uint32_t wanted[kMaxNodes];
     uint32_t taken[kMaxNodes];  // first lane of a node: ready chunks its lanes took this step
     uint32_t in_at[kMaxNodes];  // lane: position of its first input in its node's ready chunks
+    uint32_t kind[kMaxNodes], ring_of[kMaxNodes], lanes[kMaxNodes], bins[kMaxNodes];  // load_lane_tab
     uint32_t wcnt[kMaxWaves][kMaxNodes];
     uint32_t scan[kMaxWaves];
     uint32_t top, total, any, staged, overflow;
     unsigned long long generated, sunk, step, batches, part_items, xf_items, src_items, sink_items;
     gf_chunk_desc lst[kListCap];
+    uint32_t fpre[kListCap + 1];  // exclusive prefix of the staged descriptors' fills
 };
+
+// Lane owning flat item j: the k with prefix[k] <= j < prefix[k+1] (prefix has n+1 entries).
+__device__ inline uint32_t lane_of(const uint32_t* prefix, uint32_t n, uint32_t j)
+{
+    uint32_t lo = 0, hi = n;  // first index in [0, n] with prefix > j, minus one
+    while (lo < hi) {
+        const uint32_t mid = (lo + hi) >> 1;
+        if (prefix[mid] <= j) lo = mid + 1u;
+        else hi = mid;
+    }
+    return lo - 1u;
+}
 
 __device__ inline uint64_t bound_of(uint64_t n, uint64_t C, uint64_t B)
 {
     if (n == 0u) return 0u;
-    uint64_t per = (n + C - 1u) / C;
+    // 32-bit division whenever the operands fit (a batch's items, the chunk capacity)
+    uint64_t per = n + C - 1u <= 0xFFFFFFFFull ? (uint64_t)((uint32_t)(n + C - 1u) / (uint32_t)C) : (n + C - 1u) / C;
     uint64_t bins = B < n ? B : n;
     uint64_t r = per + bins - 1u;
     return r < n ? r : n;
@@ -491,10 +506,24 @@ __device__ inline uint64_t bound_of(uint64_t n, uint64_t C, uint64_t B)
 
 constexpr uint32_t kAhead = 4;  // block-sized rounds of descriptor loads issued together
 
+// The static per-lane fields the one-workgroup phases consult, staged in LDS by one parallel load
+// (a thread per lane): their walks over the lanes in node order then read LDS only, instead of
+// waiting on one global round trip per lane.
+__device__ inline void load_lane_tab(const Engine& e, SelLds& L)
+{
+    for (uint32_t k = threadIdx.x; k < e.n_nodes; k += bsz()) {
+        L.kind[k] = e.arr[k].kind;
+        L.ring_of[k] = e.ring_of[k];
+        L.lanes[k] = e.arr[k].lanes;
+        L.bins[k] = e.nodes[k].bins;
+    }
+}
+
 __device__ inline void engine_retire(const Engine& e, SelLds& L)
 {
     const uint32_t tid = threadIdx.x, n = e.n_nodes, w = tid >> 6;
     load_states(e, L.st);
+    load_lane_tab(e, L);
     if (tid == 0) {
         L.top = e.ctl[CTL_TOP];
         L.generated = e.ctr[CNT_GENERATED];
@@ -517,7 +546,7 @@ __device__ inline void engine_retire(const Engine& e, SelLds& L)
         uint32_t acc = 0;
         for (uint32_t k = 0; k < n; ++k) {
             L.off[k] = acc;
-            if (L.st[k].active && e.arr[k].kind != KIND_SINK) acc += L.st[k].n_out;
+            if (L.st[k].active && L.kind[k] != KIND_SINK) acc += L.st[k].n_out;
         }
         L.off[n] = acc;
         L.total = acc;
@@ -581,7 +610,7 @@ __device__ inline void engine_retire(const Engine& e, SelLds& L)
         uint32_t acc = 0;
         for (uint32_t k = 0; k < n; ++k) {
             L.off[k] = acc;
-            if (L.st[k].active && e.arr[k].kind != KIND_SOURCE) acc += L.st[k].n_in;
+            if (L.st[k].active && L.kind[k] != KIND_SOURCE) acc += L.st[k].n_in;
         }
         L.off[n] = acc;
         L.total = acc;
@@ -611,7 +640,7 @@ __device__ inline void engine_retire(const Engine& e, SelLds& L)
         L.top += L.total;
         for (uint32_t k = 0; k < n; ++k) {
             if (!L.st[k].active) continue;
-            const uint32_t kind = e.arr[k].kind;
+            const uint32_t kind = L.kind[k];
             if (kind == KIND_SINK) {
                 L.sunk += L.st[k].particles;
                 L.sink_items += L.st[k].particles;
@@ -635,6 +664,8 @@ __device__ inline void engine_retire(const Engine& e, SelLds& L)
 __device__ inline void engine_select(const Engine& e, SelLds& L, bool lists = true)
 {
     const uint32_t tid = threadIdx.x, n = e.n_nodes, C = e.C, P = e.P, Q = e.Q;
+    load_lane_tab(e, L);
+    __syncthreads();
     // stage the candidate ready descriptors of every transform's ring in LDS: <= Q per lane (the
     // first lane of a node stages for all its lanes). A sink takes all its ready chunks, which its
     // input list reads straight from the ring.
@@ -642,10 +673,10 @@ __device__ inline void engine_select(const Engine& e, SelLds& L, bool lists = tr
         uint32_t acc = 0;
         L.overflow = 0u;
         for (uint32_t k = 0; k < n; ++k) {
-            const uint32_t kind = e.arr[k].kind;
+            const uint32_t kind = L.kind[k];
             unsigned long long avail = L.tail[k] - L.head[k];
-            const unsigned long long cap = (unsigned long long)Q * e.arr[k].lanes;
-            uint32_t want = kind == KIND_SOURCE || e.ring_of[k] != k ? 0u : kind == KIND_SINK ? (uint32_t)avail
+            const unsigned long long cap = (unsigned long long)Q * L.lanes[k];
+            uint32_t want = kind == KIND_SOURCE || L.ring_of[k] != k ? 0u : kind == KIND_SINK ? (uint32_t)avail
                             : (avail < cap ? (uint32_t)avail : (uint32_t)cap);
             L.lst_off[k] = acc;
             L.wanted[k] = want;
@@ -666,10 +697,15 @@ __device__ inline void engine_select(const Engine& e, SelLds& L, bool lists = tr
     };
     if (!L.overflow) {
         for (uint32_t j = tid; j < L.staged; j += bsz()) {
-            uint32_t k = 0;
-            while (k + 1u < n && L.lst_off[k + 1u] <= j) ++k;
+            const uint32_t k = lane_of(L.lst_off, n, j);
             L.lst[j] = ring_entry(k, j - L.lst_off[k]);
+            L.fpre[j] = L.lst[j].fill;
         }
+        __syncthreads();
+        // the items of any run of a node's candidates are a difference of two prefix entries
+        // (staged <= kListCap descriptors of <= C items: 32-bit sums)
+        const uint32_t sum = block_exscan_array(L.fpre, L.staged, L.scan);
+        if (tid == 0) L.fpre[L.staged] = sum;
     }
     __syncthreads();
     // decisions in node order (one thread, all operands in LDS)
@@ -678,7 +714,7 @@ __device__ inline void engine_select(const Engine& e, SelLds& L, bool lists = tr
         L.any = 0u;
         for (uint32_t k = 0; k < n; ++k) {
             NodeSt& s = L.st[k];
-            const uint32_t kind = e.arr[k].kind;
+            const uint32_t kind = L.kind[k];
             s.active = 0u;
             s.n_in = 0u;
             s.n_out = 0u;
@@ -706,14 +742,15 @@ __device__ inline void engine_select(const Engine& e, SelLds& L, bool lists = tr
                 L.head[k] += s.n_in;
             } else {
                 // this lane's candidates follow the chunks the node's earlier lanes took
-                const uint32_t l = e.ring_of[k], at = L.taken[l];
+                const uint32_t l = L.ring_of[k], at = L.taken[l];
                 uint32_t wanted = L.wanted[l] - at < Q ? L.wanted[l] - at : Q;
                 unsigned long long parts = 0;
-                for (uint32_t i = 0; i < wanted; ++i) {
-                    gf_chunk_desc d = L.overflow ? ring_entry(l, at + i) : L.lst[L.lst_off[l] + at + i];
-                    parts += d.fill;
+                if (L.overflow) {
+                    for (uint32_t i = 0; i < wanted; ++i) parts += ring_entry(l, at + i).fill;
+                } else {
+                    parts = L.fpre[L.lst_off[l] + at + wanted] - L.fpre[L.lst_off[l] + at];
                 }
-                const uint32_t B = kind == KIND_PARTITION ? e.nodes[k].bins : 1u;
+                const uint32_t B = kind == KIND_PARTITION ? L.bins[k] : 1u;
                 while (wanted) {
                     uint64_t bound = bound_of(parts, C, B);
                     if ((uint64_t)(L.top - reserved) >= bound) {
@@ -721,8 +758,7 @@ __device__ inline void engine_select(const Engine& e, SelLds& L, bool lists = tr
                         break;
                     }
                     --wanted;
-                    gf_chunk_desc d = L.overflow ? ring_entry(l, at + wanted) : L.lst[L.lst_off[l] + at + wanted];
-                    parts -= d.fill;
+                    parts -= L.overflow ? ring_entry(l, at + wanted).fill : L.lst[L.lst_off[l] + at + wanted].fill;
                 }
                 s.n_in = wanted;
                 s.active = wanted ? 1u : 0u;
@@ -740,9 +776,9 @@ __device__ inline void engine_select(const Engine& e, SelLds& L, bool lists = tr
     // input lists and tile / item prefixes of every active non-source node
     for (uint32_t k = 0; k < n && lists; ++k) {
         NodeSt& s = L.st[k];
-        if (!s.active || e.arr[k].kind == KIND_SOURCE) continue;
+        if (!s.active || L.kind[k] == KIND_SOURCE) continue;
         const NodeArr& a = e.arr[k];
-        const uint32_t m = s.n_in, T = a.tile_items, l = e.ring_of[k], at = L.in_at[k];
+        const uint32_t m = s.n_in, T = a.tile_items, l = L.ring_of[k], at = L.in_at[k];
         uint32_t carry_t = 0, carry_i = 0;
         for (uint32_t g0 = 0; g0 < m; g0 += kAhead * bsz()) {
             // kAhead consecutive inputs per thread: their loads together, one pair of block scans
@@ -754,7 +790,7 @@ __device__ inline void engine_select(const Engine& e, SelLds& L, bool lists = tr
                 const uint32_t i = g0 + tid * kAhead + u;
                 dd[u] = gf_chunk_desc{0, 0, 0, 0};
                 if (i < m)
-                    dd[u] = L.overflow || a.kind == KIND_SINK ? ring_entry(l, at + i) : L.lst[L.lst_off[l] + at + i];
+                    dd[u] = L.overflow || L.kind[k] == KIND_SINK ? ring_entry(l, at + i) : L.lst[L.lst_off[l] + at + i];
             }
 #pragma unroll
             for (uint32_t u = 0; u < kAhead; ++u) {
@@ -799,7 +835,7 @@ __device__ inline void engine_select(const Engine& e, SelLds& L, bool lists = tr
             s.head = L.head[k];
             s.tail = L.tail[k];
             if (s.active) {
-                if (e.arr[k].kind == KIND_PARTITION) ka += s.n_tiles;
+                if (L.kind[k] == KIND_PARTITION) ka += s.n_tiles;
                 kb += s.n_tiles;
             }
         }
@@ -856,6 +892,9 @@ __device__ inline uint32_t find_node(const uint32_t* prefix, uint32_t n, uint32_
 // steps earlier is never read), so no word is ever cleared. A
 // workgroup publishes its R before it waits, and waits only on lanes with earlier tickets, which
 // are held by running workgroups: every wait ends.
+// A word carries its value itself and no other data is published through it, so its stores and
+// loads are relaxed agent-scope atomics: an acquire / release here would write back and invalidate
+// the XCD's L2 (buffer_wbl2 / buffer_inv on gfx950) at every publish and every spin.
 enum { LB_AGG = 1ull, LB_INC = 2ull };
 
 __device__ inline unsigned long long lb_word(uint32_t epoch, unsigned long long flag, unsigned long long v)
@@ -865,7 +904,7 @@ __device__ inline unsigned long long lb_word(uint32_t epoch, unsigned long long 
 
 __device__ inline void lb_publish(unsigned long long* w, unsigned long long v)
 {
-    __hip_atomic_store(w, v, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
+    __hip_atomic_store(w, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
 }
 
 // Exclusive prefix of the published values before unit u (thread 0 only).
@@ -875,7 +914,7 @@ __device__ inline unsigned long long lb_lookback(unsigned long long* words, uint
     for (int32_t j = (int32_t)u - 1; j >= 0; --j) {
         unsigned long long w;
         for (;;) {
-            w = __hip_atomic_load(&words[j], __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT);
+            w = __hip_atomic_load(&words[j], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
             if ((uint32_t)(w >> 36) == (epoch & 0xFFFFFFFu) && ((w >> 34) & 3ull) != 0ull) break;
             __builtin_amdgcn_s_sleep(1);
         }
@@ -930,13 +969,13 @@ __device__ inline uint32_t plan_lane_counts(const Engine& e, uint32_t k, uint32_
 __device__ inline void engine_plan_grid(const Engine& e, GridPlanLds& L)
 {
     const uint32_t tid = threadIdx.x, n = e.n_nodes, C = e.C;
-    if (tid == 0) {
-        L.status = load_ctl(&e.ctl[CTL_STATUS]);
-        uint32_t u = 0;
-        for (uint32_t k = 0; k < n; ++k)
-            if (e.st[k].active) L.act[u++] = k;
-        L.units = u;
-    }
+    // the active lanes in node order: a thread per lane (n <= kMaxNodes <= the block), one scan
+    const uint32_t on = tid < n && e.st[tid].active ? 1u : 0u;
+    if (tid == 0) L.status = load_ctl(&e.ctl[CTL_STATUS]);
+    uint32_t units;
+    const uint32_t pos = block_exscan(on, L.scan, &units);
+    if (on) L.act[pos] = tid;
+    if (tid == 0) L.units = units;
     __syncthreads();
     if (L.status) return;
     const uint32_t epoch = (uint32_t)e.ctr[CNT_STEP];
@@ -1057,17 +1096,6 @@ __device__ inline void engine_trace_step(const Engine& e, unsigned long long* le
 // ------------------------------------------------------------------------------------------
 constexpr uint32_t kGridBlock = 4096;  // outputs (routing) or inputs (lists) per unit: 1024 threads x 4
 
-// Lane owning flat item j: the k with prefix[k] <= j < prefix[k+1] (prefix has n+1 entries).
-__device__ inline uint32_t lane_of(const uint32_t* prefix, uint32_t n, uint32_t j)
-{
-    uint32_t lo = 0, hi = n;  // first index in [0, n] with prefix > j, minus one
-    while (lo < hi) {
-        const uint32_t mid = (lo + hi) >> 1;
-        if (prefix[mid] <= j) lo = mid + 1u;
-        else hi = mid;
-    }
-    return lo - 1u;
-}
 
 // Exclusive prefix of the published values before unit u over words[stride * j + col] (thread-local).
 __device__ inline unsigned long long lb_lookback_col(unsigned long long* words, uint32_t u, uint32_t stride,
@@ -1077,7 +1105,7 @@ __device__ inline unsigned long long lb_lookback_col(unsigned long long* words, 
     for (int32_t j = (int32_t)u - 1; j >= (int32_t)first; --j) {
         unsigned long long w;
         for (;;) {
-            w = __hip_atomic_load(&words[(size_t)j * stride + col], __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT);
+            w = __hip_atomic_load(&words[(size_t)j * stride + col], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
             if ((uint32_t)(w >> 36) == (epoch & 0xFFFFFFFu) && ((w >> 34) & 3ull) != 0ull) break;
             __builtin_amdgcn_s_sleep(1);
         }
@@ -1090,6 +1118,10 @@ __device__ inline unsigned long long lb_lookback_col(unsigned long long* words, 
 struct RouteLds {
     uint32_t off[kMaxNodes + 1];   // outputs before lane k (node order; active non-sink lanes)
     uint32_t ioff[kMaxNodes + 1];  // inputs before lane k (active non-source lanes)
+    const gf_chunk_desc* out[kMaxNodes];  // per lane: its output / input descriptors, route offset,
+    const gf_chunk_desc* in[kMaxNodes];   // ready ring
+    uint32_t route_off[kMaxNodes], ring[kMaxNodes];
+    uint32_t scan[kMaxWaves];
     uint32_t cnt[kMaxWaves][kMaxNodes];
     uint32_t run[kMaxNodes];       // per destination ring: this unit's outputs so far
     uint32_t base[kMaxNodes];      // per ring: outputs of the earlier units
@@ -1103,30 +1135,42 @@ struct RouteLds {
 __device__ inline void engine_route_grid(const Engine& e, RouteLds& L)
 {
     const uint32_t tid = threadIdx.x, n = e.n_nodes, w = tid >> 6;
-    if (tid == 0) {
-        uint32_t acc = 0, ai = 0;
-        for (uint32_t k = 0; k < n; ++k) {
-            const NodeSt& st = e.st[k];
-            const uint32_t kind = e.arr[k].kind;
-            L.off[k] = acc;
-            L.ioff[k] = ai;
-            if (st.active && kind != KIND_SINK) acc += st.n_out;
-            if (st.active && kind != KIND_SOURCE) ai += st.n_in;
-        }
-        L.off[n] = acc;
-        L.ioff[n] = ai;
-        L.total = acc;
-        L.total_in = ai;
-        L.units = (acc + kGridBlock - 1u) / kGridBlock;
-        L.top = e.ctl[CTL_TOP];
+    // a thread per lane (n <= kMaxNodes <= the block): the lane's fields into LDS, its output and
+    // input counts through two block scans
+    uint32_t no = 0u, ni = 0u;
+    if (tid < n) {
+        const NodeSt& st = e.st[tid];
+        const NodeArr& a = e.arr[tid];
+        const uint32_t kind = a.kind;
+        if (st.active && kind != KIND_SINK) no = st.n_out;
+        if (st.active && kind != KIND_SOURCE) ni = st.n_in;
+        L.out[tid] = a.out;
+        L.in[tid] = a.in;
+        L.ring[tid] = a.ring;
+        L.route_off[tid] = e.nodes[tid].route_offset;
+        L.ring_at[tid] = (uint32_t)(st.tail % e.ring_cap);
     }
-    for (uint32_t k = tid; k < n; k += bsz()) L.ring_at[k] = (uint32_t)(e.st[k].tail % e.ring_cap);
+    if (tid == 0) L.top = e.ctl[CTL_TOP];
+    uint32_t to, ti;
+    const uint32_t po = block_exscan(no, L.scan, &to);
+    const uint32_t pi = block_exscan(ni, L.scan, &ti);
+    if (tid < n) {
+        L.off[tid] = po;
+        L.ioff[tid] = pi;
+    }
+    if (tid == 0) {
+        L.off[n] = to;
+        L.ioff[n] = ti;
+        L.total = to;
+        L.total_in = ti;
+        L.units = (to + kGridBlock - 1u) / kGridBlock;
+    }
     __syncthreads();
     const uint32_t epoch = (uint32_t)e.ctr[CNT_STEP];
     // every consumed input back on the free stack (generation + 1), above the step's pops
     for (uint32_t j = blockIdx.x * bsz() + tid; j < L.total_in; j += gridDim.x * bsz()) {
         const uint32_t k = lane_of(L.ioff, n, j);
-        const gf_chunk_desc d = e.arr[k].in[j - L.ioff[k]];
+        const gf_chunk_desc d = L.in[k][j - L.ioff[k]];
         e.free_stack[L.top + j] = (unsigned long long)d.index | ((unsigned long long)(d.generation + 1u) << 32);
     }
     for (;;) {
@@ -1136,18 +1180,25 @@ __device__ inline void engine_route_grid(const Engine& e, RouteLds& L)
         const uint32_t u = L.unit;
         if (u >= L.units) return;
         gf_chunk_desc dd[kAhead];
-        uint32_t ds[kAhead], rk[kAhead];
+        uint32_t ds[kAhead], rk[kAhead], dk[kAhead];
+        // the kAhead rounds' descriptors, then their destinations, each as one batch of loads
 #pragma unroll
         for (uint32_t r = 0; r < kAhead; ++r) {
             const uint32_t j = u * kGridBlock + r * bsz() + tid;
-            const bool v = j < L.total;
             dd[r] = gf_chunk_desc{0, 0, 0, 0};
-            ds[r] = 0xFFFFFFFFu;
-            if (v) {
-                const uint32_t k = lane_of(L.off, n, j);
-                dd[r] = e.arr[k].out[j - L.off[k]];
-                ds[r] = (uint32_t)e.routes[e.nodes[k].route_offset + dd[r].bin];
+            dk[r] = 0u;
+            if (j < L.total) {
+                dk[r] = lane_of(L.off, n, j);
+                dd[r] = L.out[dk[r]][j - L.off[dk[r]]];
             }
+        }
+#pragma unroll
+        for (uint32_t r = 0; r < kAhead; ++r)
+            ds[r] = u * kGridBlock + r * bsz() + tid < L.total ? (uint32_t)e.routes[L.route_off[dk[r]] + dd[r].bin]
+                                                               : 0xFFFFFFFFu;
+#pragma unroll
+        for (uint32_t r = 0; r < kAhead; ++r) {
+            const bool v = u * kGridBlock + r * bsz() + tid < L.total;
             for (uint32_t i = tid; i < kMaxWaves * n; i += bsz()) (&L.cnt[0][0])[(i / n) * kMaxNodes + i % n] = 0u;
             __syncthreads();
             const uint64_t peers = peer_mask(v, ds[r]);
@@ -1181,7 +1232,7 @@ __device__ inline void engine_route_grid(const Engine& e, RouteLds& L)
             // a ring never holds more than ring_cap (= S) chunks: one subtraction wraps the slot
             uint32_t slot = L.ring_at[ds[r]] + L.base[ds[r]] + rk[r];
             if (slot >= e.ring_cap) slot -= e.ring_cap;
-            e.rings[(size_t)e.arr[ds[r]].ring * e.ring_cap + slot] = dd[r];
+            e.rings[(size_t)L.ring[ds[r]] * e.ring_cap + slot] = dd[r];
         }
         if (u + 1u == L.units)
             for (uint32_t k = tid; k < n; k += bsz()) e.st[k].tail += L.base[k] + L.run[k];
@@ -1195,6 +1246,7 @@ __device__ inline void engine_retire_counters(const Engine& e, SelLds& L)
 {
     const uint32_t tid = threadIdx.x, n = e.n_nodes;
     load_states(e, L.st);
+    load_lane_tab(e, L);
     __syncthreads();
     for (uint32_t k = tid; k < n; k += bsz()) {
         L.tail[k] = L.st[k].tail;
@@ -1213,7 +1265,7 @@ __device__ inline void engine_retire_counters(const Engine& e, SelLds& L)
         uint32_t released = 0;
         for (uint32_t k = 0; k < n; ++k) {
             if (!L.st[k].active) continue;
-            const uint32_t kind = e.arr[k].kind;
+            const uint32_t kind = L.kind[k];
             if (kind != KIND_SOURCE) released += L.st[k].n_in;
             if (kind == KIND_SINK) {
                 L.sunk += L.st[k].particles;
@@ -1246,15 +1298,18 @@ struct ListLds {
 __device__ inline void engine_lists_grid(const Engine& e, ListLds& L)
 {
     const uint32_t tid = threadIdx.x, n = e.n_nodes;
+    // list units of every lane: a thread per lane (n <= kMaxNodes <= the block), one block scan
+    uint32_t nu = 0u;
+    if (tid < n) {
+        const NodeSt& st = e.st[tid];
+        if (st.active && e.arr[tid].kind != KIND_SOURCE) nu = (st.n_in + kGridBlock - 1u) / kGridBlock;
+    }
+    uint32_t units;
+    const uint32_t pu = block_exscan(nu, L.scan, &units);
+    if (tid < n) L.uoff[tid] = pu;
     if (tid == 0) {
-        uint32_t acc = 0;
-        for (uint32_t k = 0; k < n; ++k) {
-            L.uoff[k] = acc;
-            const NodeSt& st = e.st[k];
-            if (st.active && e.arr[k].kind != KIND_SOURCE) acc += (st.n_in + kGridBlock - 1u) / kGridBlock;
-        }
-        L.uoff[n] = acc;
-        L.units = acc;
+        L.uoff[n] = units;
+        L.units = units;
     }
     __syncthreads();
     const uint32_t epoch = (uint32_t)e.ctr[CNT_STEP];
@@ -1344,21 +1399,32 @@ __device__ inline void engine_lists_grid(const Engine& e, ListLds& L)
             if (L.k) finisher = true;
         }
     }
-    if (finisher && tid == 0) {
-        uint32_t ka = 0, kb = 0;
-        for (uint32_t k = 0; k < n; ++k) {
-            e.ka_prefix[k] = ka;
-            e.kb_prefix[k] = kb;
-            const NodeSt& st = e.st[k];
+    // the finisher (one workgroup; `finisher` is uniform in it) writes the work prefixes: a thread
+    // per lane, two block scans. Its acquire (thread 0, on the done count) ordered the other
+    // workgroups' n_tiles before this fence; each thread's fence orders its own loads after it.
+    if (finisher) {
+        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
+        uint32_t ta = 0u, tb = 0u;
+        if (tid < n) {
+            const NodeSt& st = e.st[tid];
             if (st.active) {
-                if (e.arr[k].kind == KIND_PARTITION) ka += st.n_tiles;
-                kb += st.n_tiles;
+                tb = st.n_tiles;
+                if (e.arr[tid].kind == KIND_PARTITION) ta = st.n_tiles;
             }
         }
-        e.ka_prefix[n] = ka;
-        e.kb_prefix[n] = kb;
-        e.ctl[CTL_LIST_DONE] = 0u;
-        e.ctl[CTL_TICKET_C] = 0u;
+        uint32_t sa, sb;
+        const uint32_t pa = block_exscan(ta, L.scan, &sa);
+        const uint32_t pb = block_exscan(tb, L.scan, &sb);
+        if (tid < n) {
+            e.ka_prefix[tid] = pa;
+            e.kb_prefix[tid] = pb;
+        }
+        if (tid == 0) {
+            e.ka_prefix[n] = sa;
+            e.kb_prefix[n] = sb;
+            e.ctl[CTL_LIST_DONE] = 0u;
+            e.ctl[CTL_TICKET_C] = 0u;
+        }
     }
 }
 

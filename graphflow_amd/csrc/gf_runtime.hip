@@ -92,9 +92,10 @@ uint32_t reserve_bound_of(const gf_node_desc& nd, uint32_t C, uint32_t Q)
     return (uint32_t)b;
 }
 
-// Bin path of a partition (core/BinPath.hpp). The dense path keeps 24 bytes of LDS per bin in a
-// scatter tile, which on MI355X's 160 KB LDS holds up to 4096 bins; automatic selects the hash
-// path above that (the reference's limit, Config.hpp:174, is derived from its GPUs' shared memory).
+// Bin path of a partition (core/BinPath.hpp). The dense path keeps 16 bytes of LDS per bin in a
+// scatter tile (per-wave running counts): up to 4096 bins that is 64 KB, two scatter workgroups
+// per CU of MI355X's 160 KB LDS; automatic selects the hash path above that (the reference's
+// limit, Config.hpp:174, is derived from its GPUs' shared memory).
 constexpr uint32_t kDenseBinLimit = 4096u;
 constexpr uint32_t kHashTileLimit = 1024u;  // hash path tiles: a tile's table of 2 * 1024 slots fits LDS
 
@@ -232,20 +233,28 @@ __device__ inline bool engine_stopped(const Engine& e, uint32_t* flag)
     return *flag != 0u;
 }
 
+// The step's work prefix (n + 1 words, n <= kMaxNodes) into LDS: a tile's lane is then a binary
+// search in LDS instead of a walk over global memory.
+__device__ inline void load_prefix(const uint32_t* prefix, uint32_t n, uint32_t* sh)
+{
+    for (uint32_t k = threadIdx.x; k <= n; k += blockDim.x) sh[k] = prefix[k];
+}
+
 template <class P, int R>
 __global__ __launch_bounds__(kThreads) void k_engine_ka(Engine e)
 {
     extern __shared__ char smem[];
-    __shared__ uint32_t flag;
-    if (engine_stopped(e, &flag)) return;
-    const uint32_t total = e.ka_prefix[e.n_nodes];
+    __shared__ uint32_t flag, pre[kMaxNodes + 1];
+    load_prefix(e.ka_prefix, e.n_nodes, pre);
+    if (engine_stopped(e, &flag)) return;  // its barrier also publishes `pre`
+    const uint32_t total = pre[e.n_nodes];
     for (uint32_t t = blockIdx.x; t < total; t += gridDim.x) {
-        const uint32_t k = find_node(e.ka_prefix, e.n_nodes, t);
+        const uint32_t k = lane_of(pre, e.n_nodes, t);
         const NodeSt s = e.st[k];
         const Batch b = make_batch(e, k, s);
         const uint32_t E = e.arr[k].hash_entries;
-        if (E) count_tile_hash<P, R>(e.nodes[k], b, t - e.ka_prefix[k], e.cols, e.C, &e.ctl[CTL_STATUS], smem, E);
-        else count_tile<P, R>(e.nodes[k], b, t - e.ka_prefix[k], e.cols, e.C, &e.ctl[CTL_STATUS], smem);
+        if (E) count_tile_hash<P, R>(e.nodes[k], b, t - pre[k], e.cols, e.C, &e.ctl[CTL_STATUS], smem, E);
+        else count_tile<P, R>(e.nodes[k], b, t - pre[k], e.cols, e.C, &e.ctl[CTL_STATUS], smem);
     }
 }
 
@@ -272,21 +281,24 @@ template <class P, int R>
 __global__ __launch_bounds__(kThreads) void k_engine_kb(Engine e)
 {
     extern __shared__ char smem[];
-    __shared__ uint32_t flag;
-    if (engine_stopped(e, &flag)) return;
-    const uint32_t total = e.kb_prefix[e.n_nodes], G = gridDim.x;
+    __shared__ uint32_t flag, pre[kMaxNodes + 1], wsink[kMaxNodes];
+    load_prefix(e.kb_prefix, e.n_nodes, pre);
+    for (uint32_t k = threadIdx.x; k < e.n_nodes; k += kThreads)
+        wsink[k] = (e.arr[k].flags & NODE_WAVE_SINK) && e.st[k].active ? 1u : 0u;
+    if (engine_stopped(e, &flag)) return;  // its barrier also publishes `pre` and `wsink`
+    const uint32_t total = pre[e.n_nodes], G = gridDim.x;
     for (uint32_t t = blockIdx.x; t < total; t += G) {
-        const uint32_t k = find_node(e.kb_prefix, e.n_nodes, t);
-        if (e.arr[k].flags & NODE_WAVE_SINK) {
+        const uint32_t k = lane_of(pre, e.n_nodes, t);
+        if (wsink[k]) {
             // skip the node's tiles: the first tile of this workgroup's round-robin at or after them
-            const uint32_t end = e.kb_prefix[k + 1u];
+            const uint32_t end = pre[k + 1u];
             t += (end - t + G - 1u) / G * G - G;
             continue;
         }
         const gf_node_desc& nd = e.nodes[k];
         const NodeSt s = e.st[k];
         const Batch b = make_batch(e, k, s);
-        const uint32_t lt = t - e.kb_prefix[k];
+        const uint32_t lt = t - pre[k];
         const uint32_t kind = e.arr[k].kind;
         if (kind == KIND_SOURCE) {
             source_tile<P>(nd, b, lt, e.cols, e.C, e.rp);
@@ -307,8 +319,8 @@ __global__ __launch_bounds__(kThreads) void k_engine_kb(Engine e)
     }
     const uint32_t w = threadIdx.x >> 6, W = G * (uint32_t)kWaves;
     for (uint32_t k = 0; k < e.n_nodes; ++k) {
-        if (!(e.arr[k].flags & NODE_WAVE_SINK) || !e.st[k].active) continue;
-        const uint32_t t0 = e.kb_prefix[k], t1 = e.kb_prefix[k + 1u];
+        if (!wsink[k]) continue;
+        const uint32_t t0 = pre[k], t1 = pre[k + 1u];
         const gf_node_desc& nd = e.nodes[k];
         const Batch b = make_batch(e, k, e.st[k]);
         for (uint32_t t = t0 + blockIdx.x * (uint32_t)kWaves + w; t < t1; t += W)
@@ -1934,7 +1946,10 @@ void build_engine(gf_flow& f, uint64_t N, const RunShape& sh, uint32_t src)
         set_lds(k_engine_kb<P, RR>, lds_b);
         if (micro_fits) set_lds(k_micro<P, RR>, eb->micro_lds);
     });
+    static_assert(sizeof(SelLds) <= 160u * 1024u, "the select's LDS state exceeds a workgroup's LDS");
     set_lds(k_engine_kr, sizeof(SelLds));
+    set_lds(k_engine_setup, sizeof(SelLds));
+    set_lds(k_engine_resume, sizeof(SelLds));
     (void)src;
     f.eng = std::move(eb);
 }
