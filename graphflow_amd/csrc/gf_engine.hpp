@@ -480,7 +480,36 @@ struct SelLds {
     unsigned long long generated, sunk, step, batches, part_items, xf_items, src_items, sink_items;
     gf_chunk_desc lst[kListCap];
     uint32_t fpre[kListCap + 1];  // exclusive prefix of the staged descriptors' fills
+    uint32_t dfill[kMaxNodes], dtiles[kMaxNodes];  // source lanes: admitted items and tiles, first id
+    unsigned long long dfirst[kMaxNodes];
+    unsigned long long red[kMaxWaves * 8];  // block_sum64
 };
+// the one-workgroup phases run a thread per lane: every block has at least kMaxNodes threads
+static_assert(kMaxNodes <= (uint32_t)kThreads, "a block must hold a thread per lane");
+
+// Block-wide sums of V 64-bit values per thread; every thread gets the sums (sh: kMaxWaves * V words).
+template <int V>
+__device__ inline void block_sum64(unsigned long long (&v)[V], unsigned long long* sh)
+{
+    const uint32_t lane = lane_id(), w = threadIdx.x >> 6;
+#pragma unroll
+    for (int i = 0; i < V; ++i) {
+        for (int o = 32; o > 0; o >>= 1) {
+            const uint32_t lo = (uint32_t)__shfl_xor((int)(uint32_t)v[i], o);
+            const uint32_t hi = (uint32_t)__shfl_xor((int)(uint32_t)(v[i] >> 32), o);
+            v[i] += ((unsigned long long)hi << 32) | lo;
+        }
+        if (lane == 0) sh[w * V + i] = v[i];
+    }
+    __syncthreads();
+#pragma unroll
+    for (int i = 0; i < V; ++i) {
+        unsigned long long t = 0ull;
+        for (uint32_t ww = 0; ww < bwaves(); ++ww) t += sh[ww * V + i];
+        v[i] = t;
+    }
+    __syncthreads();
+}
 
 // Lane owning flat item j: the k with prefix[k] <= j < prefix[k+1] (prefix has n+1 entries).
 __device__ inline uint32_t lane_of(const uint32_t* prefix, uint32_t n, uint32_t j)
@@ -668,26 +697,29 @@ __device__ inline void engine_select(const Engine& e, SelLds& L, bool lists = tr
     __syncthreads();
     // stage the candidate ready descriptors of every transform's ring in LDS: <= Q per lane (the
     // first lane of a node stages for all its lanes). A sink takes all its ready chunks, which its
-    // input list reads straight from the ring.
-    if (tid == 0) {
-        uint32_t acc = 0;
-        L.overflow = 0u;
-        for (uint32_t k = 0; k < n; ++k) {
-            const uint32_t kind = L.kind[k];
-            unsigned long long avail = L.tail[k] - L.head[k];
-            const unsigned long long cap = (unsigned long long)Q * L.lanes[k];
-            uint32_t want = kind == KIND_SOURCE || L.ring_of[k] != k ? 0u : kind == KIND_SINK ? (uint32_t)avail
-                            : (avail < cap ? (uint32_t)avail : (uint32_t)cap);
-            L.lst_off[k] = acc;
-            L.wanted[k] = want;
-            L.taken[k] = 0u;
-            if (kind != KIND_SINK) acc += want;
+    // input list reads straight from the ring. A thread per lane, one block scan for the offsets.
+    {
+        uint32_t staged_w = 0u;
+        if (tid < n) {
+            const uint32_t kind = L.kind[tid];
+            const unsigned long long avail = L.tail[tid] - L.head[tid];
+            const unsigned long long cap = (unsigned long long)Q * L.lanes[tid];
+            const uint32_t want = kind == KIND_SOURCE || L.ring_of[tid] != tid ? 0u
+                                : kind == KIND_SINK ? (uint32_t)avail : (avail < cap ? (uint32_t)avail : (uint32_t)cap);
+            L.wanted[tid] = want;
+            L.taken[tid] = 0u;
+            L.ring_at[tid] = (uint32_t)(L.head[tid] % e.ring_cap);
+            if (kind != KIND_SINK) staged_w = want;
         }
-        L.lst_off[n] = acc;
-        L.staged = acc;
-        if (acc > kListCap) L.overflow = 1u;
+        uint32_t staged;
+        const uint32_t off = block_exscan(staged_w, L.scan, &staged);
+        if (tid < n) L.lst_off[tid] = off;
+        if (tid == 0) {
+            L.lst_off[n] = staged;
+            L.staged = staged;
+            L.overflow = staged > kListCap ? 1u : 0u;
+        }
     }
-    for (uint32_t k = tid; k < n; k += bsz()) L.ring_at[k] = (uint32_t)(L.head[k] % e.ring_cap);
     __syncthreads();
     // descriptor i of node k's ready chunks (i < the node's ring occupancy <= ring_cap)
     auto ring_entry = [&](uint32_t k, uint32_t i) -> const gf_chunk_desc& {
@@ -708,8 +740,93 @@ __device__ inline void engine_select(const Engine& e, SelLds& L, bool lists = tr
         if (tid == 0) L.fpre[L.staged] = sum;
     }
     __syncthreads();
-    // decisions in node order (one thread, all operands in LDS)
-    if (tid == 0) {
+    // decisions (GraphRunner's order: node order, every lane after the earlier ones). A lane depends
+    // on the earlier lanes only through the credits they reserved, the candidates its node's earlier
+    // lanes took and, at the source, the items admitted before it. When the step's whole demand fits
+    // the free stack no lane shrinks: lane i of a node takes min(Q, wanted - i Q) candidates and the
+    // source's lanes admit in turn as if alone. A thread per lane decides that case (one block sum
+    // checks the demand against the free stack); otherwise one thread walks the lanes in node order.
+    bool serial = L.overflow != 0u;
+    if (!serial) {
+        if (tid < n && L.kind[tid] == KIND_SOURCE && L.ring_of[tid] == tid) {
+            unsigned long long gen = L.generated;
+            const uint32_t T = e.arr[tid].tile_items;
+            for (uint32_t j = 0; j < L.lanes[tid]; ++j) {
+                const unsigned long long live = gen - L.sunk, remaining = e.N - gen;
+                unsigned long long min_fill = C < P ? C : P;
+                if (remaining < min_fill) min_fill = remaining;
+                uint32_t fill = 0u;
+                if (gen < e.N && live < P && (P - live) >= min_fill) {
+                    unsigned long long f = remaining;
+                    if (f > P - live) f = P - live;
+                    if (f > C) f = C;
+                    fill = (uint32_t)f;
+                }
+                L.dfill[tid + j] = fill;
+                L.dtiles[tid + j] = (fill + T - 1u) / T;
+                L.dfirst[tid + j] = gen;
+                gen += fill;
+            }
+        }
+        __syncthreads();
+        unsigned long long dem[1] = {0ull};
+        if (tid < n) {
+            const uint32_t kind = L.kind[tid];
+            if (kind == KIND_SOURCE) {
+                dem[0] = L.dfill[tid] ? 1ull : 0ull;
+            } else if (kind != KIND_SINK) {
+                const uint32_t l = L.ring_of[tid], w = L.wanted[l], i = tid - l;
+                const uint32_t at = i * Q < w ? i * Q : w, m = w - at < Q ? w - at : Q;
+                const unsigned long long parts = L.fpre[L.lst_off[l] + at + m] - L.fpre[L.lst_off[l] + at];
+                dem[0] = bound_of(parts, C, kind == KIND_PARTITION ? L.bins[tid] : 1u);
+            }
+        }
+        block_sum64(dem, L.red);
+        serial = dem[0] > (unsigned long long)L.top;
+    }
+    if (!serial) {
+        unsigned long long act[1] = {0ull};
+        if (tid < n) {
+            NodeSt& s = L.st[tid];
+            const uint32_t kind = L.kind[tid];
+            s.active = 0u;
+            s.n_in = 0u;
+            s.n_out = 0u;
+            s.n_tiles = 0u;
+            if (kind == KIND_SOURCE) {
+                if (L.dfill[tid]) {
+                    s.particles = L.dfill[tid];
+                    s.first_id = L.dfirst[tid];
+                    s.n_tiles = L.dtiles[tid];
+                    s.active = 1u;
+                }
+                if (L.ring_of[tid] == tid) {
+                    const uint32_t last = tid + L.lanes[tid] - 1u;
+                    L.generated = L.dfirst[last] + L.dfill[last];
+                }
+            } else if (kind == KIND_SINK) {
+                s.n_in = L.wanted[tid];
+                s.active = s.n_in ? 1u : 0u;
+                L.in_at[tid] = 0u;
+                s.in_slot = L.ring_at[tid];
+            } else {
+                const uint32_t l = L.ring_of[tid], w = L.wanted[l], i = tid - l;
+                const uint32_t at = i * Q < w ? i * Q : w, m = w - at < Q ? w - at : Q;
+                s.n_in = m;
+                s.active = m ? 1u : 0u;
+                L.in_at[tid] = at;
+                uint32_t slot = L.ring_at[l] + at;  // at < ring occupancy <= ring_cap
+                if (slot >= e.ring_cap) slot -= e.ring_cap;
+                s.in_slot = slot;
+            }
+            // a node's lanes take all its candidates together: wanted <= Q * lanes
+            if (kind != KIND_SOURCE && L.ring_of[tid] == tid) L.head[tid] += L.wanted[tid];
+            act[0] = s.active;
+        }
+        block_sum64(act, L.red);
+        if (tid == 0) L.any = act[0] ? 1u : 0u;
+    }
+    if (serial && tid == 0) {
         uint32_t reserved = 0;
         L.any = 0u;
         for (uint32_t k = 0; k < n; ++k) {
@@ -823,26 +940,37 @@ __device__ inline void engine_select(const Engine& e, SelLds& L, bool lists = tr
         }
         __syncthreads();
     }
-    // work prefixes, ring cursors, termination
-    if (tid == 0) {
-        uint32_t ka = 0, kb = 0;
-        for (uint32_t k = 0; k < n; ++k) {
-            if (lists) {
-                e.ka_prefix[k] = ka;
-                e.kb_prefix[k] = kb;
-            }
-            NodeSt& s = L.st[k];
-            s.head = L.head[k];
-            s.tail = L.tail[k];
+    // work prefixes, ring cursors, termination: a thread per lane
+    __syncthreads();
+    unsigned long long busy[1] = {0ull};
+    {
+        uint32_t ta = 0u, tb = 0u;
+        if (tid < n) {
+            NodeSt& s = L.st[tid];
+            s.head = L.head[tid];
+            s.tail = L.tail[tid];
             if (s.active) {
-                if (L.kind[k] == KIND_PARTITION) ka += s.n_tiles;
-                kb += s.n_tiles;
+                tb = s.n_tiles;
+                if (L.kind[tid] == KIND_PARTITION) ta = s.n_tiles;
             }
+            busy[0] = L.tail[tid] != L.head[tid] ? 1ull : 0ull;
         }
         if (lists) {
-            e.ka_prefix[n] = ka;
-            e.kb_prefix[n] = kb;
+            uint32_t sa, sb;
+            const uint32_t pa = block_exscan(ta, L.scan, &sa);
+            const uint32_t pb = block_exscan(tb, L.scan, &sb);
+            if (tid < n) {
+                e.ka_prefix[tid] = pa;
+                e.kb_prefix[tid] = pb;
+            }
+            if (tid == 0) {
+                e.ka_prefix[n] = sa;
+                e.kb_prefix[n] = sb;
+            }
         }
+        block_sum64(busy, L.red);
+    }
+    if (tid == 0) {
         e.ctl[CTL_TOP] = L.top;
         e.ctl[CTL_TICKET_A] = 0u;  // the next step's grid plan takes its lanes from ticket 0
         e.ctl[CTL_TICKET_B] = 0u;  // and its grid routing its units
@@ -855,9 +983,7 @@ __device__ inline void engine_select(const Engine& e, SelLds& L, bool lists = tr
         e.ctr[CNT_SRC_ITEMS] = L.src_items;
         e.ctr[CNT_SINK_ITEMS] = L.sink_items;
         if (!L.any) {
-            bool drained = true;
-            for (uint32_t k = 0; k < n; ++k)
-                if (L.tail[k] != L.head[k]) drained = false;
+            const bool drained = busy[0] == 0ull;
             if (drained && L.generated >= e.N) {
                 e.ctl[CTL_DONE] = 1u;
                 if (L.top != e.S) set_status(&e.ctl[CTL_STATUS], GF_BAD_ARGUMENT);
@@ -1248,10 +1374,29 @@ __device__ inline void engine_retire_counters(const Engine& e, SelLds& L)
     load_states(e, L.st);
     load_lane_tab(e, L);
     __syncthreads();
-    for (uint32_t k = tid; k < n; k += bsz()) {
-        L.tail[k] = L.st[k].tail;
-        L.head[k] = L.st[k].head;
+    // the retired lanes' counts: a thread per lane, one block sum (released inputs, sunk items,
+    // admitted items, transform batches, partition items, transform items)
+    unsigned long long v[6] = {0ull, 0ull, 0ull, 0ull, 0ull, 0ull};
+    if (tid < n) {
+        const NodeSt& s = L.st[tid];
+        L.tail[tid] = s.tail;
+        L.head[tid] = s.head;
+        if (s.active) {
+            const uint32_t kind = L.kind[tid];
+            if (kind != KIND_SOURCE) v[0] = s.n_in;
+            if (kind == KIND_SINK) {
+                v[1] = s.particles;
+            } else if (kind == KIND_SOURCE) {
+                v[2] = s.particles;
+            } else {
+                v[3] = 1ull;
+                if (kind == KIND_PARTITION) v[4] = s.particles;
+                else v[5] = s.particles;
+            }
+        }
     }
+    block_sum64(v, L.red);
+    if (tid < n) L.st[tid].active = 0u;
     if (tid == 0) {
         L.top = e.ctl[CTL_TOP];
         L.generated = e.ctr[CNT_GENERATED];
@@ -1261,25 +1406,13 @@ __device__ inline void engine_retire_counters(const Engine& e, SelLds& L)
         L.part_items = e.ctr[CNT_PART_ITEMS];
         L.xf_items = e.ctr[CNT_XF_ITEMS];
         L.src_items = e.ctr[CNT_SRC_ITEMS];
-        L.sink_items = e.ctr[CNT_SINK_ITEMS];
-        uint32_t released = 0;
-        for (uint32_t k = 0; k < n; ++k) {
-            if (!L.st[k].active) continue;
-            const uint32_t kind = L.kind[k];
-            if (kind != KIND_SOURCE) released += L.st[k].n_in;
-            if (kind == KIND_SINK) {
-                L.sunk += L.st[k].particles;
-                L.sink_items += L.st[k].particles;
-            } else if (kind == KIND_SOURCE) {
-                L.src_items += L.st[k].particles;
-            } else {
-                L.batches += 1u;
-                if (kind == KIND_PARTITION) L.part_items += L.st[k].particles;
-                else L.xf_items += L.st[k].particles;
-            }
-            L.st[k].active = 0u;
-        }
-        L.top += released;
+        L.sink_items = e.ctr[CNT_SINK_ITEMS] + v[1];
+        L.top += (uint32_t)v[0];
+        L.sunk += v[1];
+        L.src_items += v[2];
+        L.batches += v[3];
+        L.part_items += v[4];
+        L.xf_items += v[5];
         L.step += 1u;
     }
     __syncthreads();

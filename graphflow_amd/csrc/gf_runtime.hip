@@ -166,7 +166,7 @@ template <class P, int R>
 __global__ __launch_bounds__(kThreads) void k_count(gf_node_desc nd, const Batch* bp, Cols cols, uint32_t C,
                                                     uint32_t* status, uint32_t hash_entries)
 {
-    extern __shared__ char smem[];
+    extern __shared__ __align__(16) char smem[];
     const Batch b = *bp;
     if (hash_entries) count_tile_hash<P, R>(nd, b, blockIdx.x, cols, C, status, smem, hash_entries);
     else count_tile<P, R>(nd, b, blockIdx.x, cols, C, status, smem);
@@ -176,7 +176,7 @@ template <class P, int R, bool Stage>
 __global__ __launch_bounds__(kThreads) void k_scatter(gf_node_desc nd, const Batch* bp, Cols cols, uint32_t C,
                                                       uint32_t* status, uint32_t hash_entries)
 {
-    extern __shared__ char smem[];
+    extern __shared__ __align__(16) char smem[];
     const Batch b = *bp;
     if (hash_entries) scatter_tile_hash<P, R>(nd, b, blockIdx.x, cols, C, status, smem, hash_entries);
     else scatter_tile<P, R, Stage>(nd, b, blockIdx.x, cols, C, status, smem);
@@ -202,7 +202,7 @@ template <class P>
 __global__ __launch_bounds__(kThreads) void k_sink(gf_node_desc nd, const Batch* bp, Cols cols, uint32_t C,
                                                    uint32_t* status)
 {
-    extern __shared__ char smem[];
+    extern __shared__ __align__(16) char smem[];
     const Batch b = *bp;
     sink_tile<P>(nd, b, blockIdx.x, cols, C, status, smem);
 }
@@ -211,7 +211,7 @@ __global__ __launch_bounds__(kThreads) void k_sink(gf_node_desc nd, const Batch*
 template <class P, int R>
 __global__ __launch_bounds__(1024) void k_batch_fused(FusedArgs a, Cols cols)
 {
-    extern __shared__ char smem[];
+    extern __shared__ __align__(16) char smem[];
     fused_partition_batch<P, R>(a, cols, smem);
 }
 
@@ -243,7 +243,7 @@ __device__ inline void load_prefix(const uint32_t* prefix, uint32_t n, uint32_t*
 template <class P, int R>
 __global__ __launch_bounds__(kThreads) void k_engine_ka(Engine e)
 {
-    extern __shared__ char smem[];
+    extern __shared__ __align__(16) char smem[];
     __shared__ uint32_t flag, pre[kMaxNodes + 1];
     load_prefix(e.ka_prefix, e.n_nodes, pre);
     if (engine_stopped(e, &flag)) return;  // its barrier also publishes `pre`
@@ -280,7 +280,7 @@ __global__ __launch_bounds__(1024) void k_engine_trace(Engine e)
 template <class P, int R>
 __global__ __launch_bounds__(kThreads) void k_engine_kb(Engine e)
 {
-    extern __shared__ char smem[];
+    extern __shared__ __align__(16) char smem[];
     __shared__ uint32_t flag, pre[kMaxNodes + 1], wsink[kMaxNodes];
     load_prefix(e.kb_prefix, e.n_nodes, pre);
     for (uint32_t k = threadIdx.x; k < e.n_nodes; k += kThreads)
@@ -341,7 +341,7 @@ __global__ __launch_bounds__(1024) void k_engine_route(Engine e)
 // The step's counters and the select of the next step's batches (one workgroup of 1024 threads).
 __global__ __launch_bounds__(1024) void k_engine_kr(Engine e)
 {
-    extern __shared__ char smem[];
+    extern __shared__ __align__(16) char smem[];
     __shared__ uint32_t flag;
     if (engine_stopped(e, &flag)) return;
     SelLds& L = *reinterpret_cast<SelLds*>(smem);
@@ -365,7 +365,7 @@ __global__ __launch_bounds__(1024) void k_engine_lists(Engine e)
 template <class P, int R>
 __global__ __launch_bounds__(1024) void k_micro(Engine e, uint32_t max_steps, uint32_t group_lds)
 {
-    extern __shared__ char smem[];
+    extern __shared__ __align__(16) char smem[];
     __shared__ uint32_t flag;
     const uint32_t G = blockDim.x / kThreads, g = threadIdx.x / kThreads;
     char* gsm = smem + (size_t)g * group_lds;
@@ -441,7 +441,7 @@ __global__ __launch_bounds__(1024) void k_micro(Engine e, uint32_t max_steps, ui
 
 __global__ __launch_bounds__(kThreads) void k_engine_setup(Engine e, uint32_t do_select)
 {
-    extern __shared__ char smem[];
+    extern __shared__ __align__(16) char smem[];
     SelLds& L = *reinterpret_cast<SelLds*>(smem);
     const uint32_t tid = threadIdx.x;
     for (uint32_t i = tid; i < e.S; i += kThreads)
@@ -477,7 +477,7 @@ __global__ __launch_bounds__(kThreads) void k_engine_setup(Engine e, uint32_t do
 // Continue a run on the wide engine from the pre-select state another engine left behind.
 __global__ __launch_bounds__(kThreads) void k_engine_resume(Engine e)
 {
-    extern __shared__ char smem[];
+    extern __shared__ __align__(16) char smem[];
     SelLds& L = *reinterpret_cast<SelLds*>(smem);
     const uint32_t tid = threadIdx.x;
     load_states(e, L.st);

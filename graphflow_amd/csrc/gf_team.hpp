@@ -1526,7 +1526,7 @@ template <class P>
 __global__ __launch_bounds__(kTeamThreads) void k_team(const TeamArgs* __restrict__ ga)
 {
     const TeamArgs& a = *ga;
-    extern __shared__ char smem[];
+    extern __shared__ __align__(16) char smem[];
     TeamLds& L = *reinterpret_cast<TeamLds*>(smem);
     __shared__ uint32_t sh[4];
     team_join(a, sh);
