@@ -469,10 +469,20 @@ __device__ inline uint64_t peer_mask_bits(bool valid, uint32_t bin, uint32_t nbi
 // Chunk of tile t: largest c with prefix[c] <= t (prefix has n+1 entries).
 __device__ inline uint32_t find_chunk(const uint32_t* prefix, uint32_t n, uint32_t t)
 {
-    // common case, two independent loads: every chunk before t holds exactly one tile (chunks of at
-    // most tile_items items, e.g. the many small per-bucket chunks a sink takes)
-    if (t < n && prefix[t] == t && prefix[t + 1u] > t) return t;
-    uint32_t lo = 0, hi = n;  // invariant: prefix[lo] <= t < prefix[hi]
+    // Chunk c holds tile t iff prefix[c] <= t < prefix[c + 1]. Every chunk of an input list holds at
+    // least one item, so at least one tile: the chunks before c hold c tiles plus at most
+    // X = prefix[n] - n more, hence t - X <= c <= t. Common case (X tiles all after t, e.g. the many
+    // small per-bucket chunks a sink takes): c = t, from loads issued together with prefix[n].
+    const uint32_t tot = prefix[n];
+    const bool in = t < n;
+    const uint32_t pt = in ? prefix[t] : 0u, pt1 = in ? prefix[t + 1u] : 0u;
+    if (in && pt == t && pt1 > t) return t;
+    const uint32_t X = tot - n;
+    uint32_t lo = t > X ? t - X : 0u, hi = t + 1u < n ? t + 1u : n;  // invariant: prefix[lo] <= t < prefix[hi]
+    if (lo >= n || prefix[lo] > t || (hi < n && prefix[hi] <= t)) {
+        lo = 0u;  // the window's ends do not hold the invariant: search the whole list
+        hi = n;
+    }
     while (hi - lo > 1u) {
         uint32_t mid = (lo + hi) >> 1;
         if (prefix[mid] <= t) lo = mid;
@@ -708,12 +718,14 @@ __device__ void scatter_tile_hash(const gf_node_desc& nd, const Batch& b, uint32
     __syncthreads();
 }
 
-// LDS bytes of scatter_tile: the per-wave running counts; with staging also the tile totals, their
+// LDS bytes of scatter_tile: the per-wave running counts (16 bits: a wave ranks at most 64 R <= 1024
+// items of a tile of at most 4096), padded to 16 bytes; with staging also the tile totals, their
 // prefix and the staged records.
+__host__ __device__ inline size_t scatter_wcnt_bytes(uint32_t bins) { return ((size_t)kWaves * bins * 2u + 15u) & ~(size_t)15u; }
 __host__ __device__ inline size_t scatter_smem(uint32_t bins, uint32_t tile_items, uint32_t rec_bytes, bool stage)
 {
-    if (!stage) return (size_t)kWaves * bins * 4u;
-    return (size_t)(kWaves + 2) * bins * 4u + (size_t)tile_items * (rec_bytes + 4u);
+    if (!stage) return scatter_wcnt_bytes(bins);
+    return scatter_wcnt_bytes(bins) + (size_t)bins * 8u + (size_t)tile_items * (rec_bytes + 4u);
 }
 
 // Stable scatter of one tile (Execution.hpp:1096-1187 / :1327-1502 semantics with the canonical
@@ -730,12 +742,12 @@ __device__ void scatter_tile(const gf_node_desc& nd, const Batch& b, uint32_t t,
                              const uint32_t* bin_base = nullptr)
 {
     const uint32_t B = nd.bins, tid = ltid(), w = tid >> 6, lane = lane_id();
-    uint32_t* wcnt = (uint32_t*)smem;      // [kWaves][B] running, then exclusive wave prefix
-    uint32_t* ttot = wcnt + kWaves * B;     // Stage: [B] tile totals
-    uint32_t* tstart = ttot + B;            // Stage: [B] exclusive prefix of ttot over bins
-    P* stage = (P*)(tstart + B);            // [T]
+    uint16_t* wcnt = (uint16_t*)smem;                                    // [kWaves][B] running, then exclusive wave prefix
+    uint32_t* ttot = (uint32_t*)(smem + scatter_wcnt_bytes(B));          // Stage: [B] tile totals
+    uint32_t* tstart = ttot + B;                                         // Stage: [B] exclusive prefix of ttot over bins
+    P* stage = (P*)(tstart + B);                                         // [T]
     uint32_t* flat = (uint32_t*)(stage + (Stage ? b.tile_items : 0u));
-    for (uint32_t i = tid; i < kWaves * B; i += kThreads) wcnt[i] = 0u;
+    for (uint32_t i = tid; i < (uint32_t)(scatter_wcnt_bytes(B) / 4u); i += kThreads) ((uint32_t*)smem)[i] = 0u;
     __syncthreads();
     TileSpan s = tile_span(b, b.in, b.n_in, t, C);
     P rec[R];
@@ -755,17 +767,17 @@ __device__ void scatter_tile(const gf_node_desc& nd, const Batch& b, uint32_t t,
             }
         }
         uint64_t peers = B > 64u ? peer_mask_bits(v, bin[r], 32u - (uint32_t)__clz((int)(B - 1u))) : peer_mask(v, bin[r]);
-        uint32_t cur = v ? wcnt[w * B + bin[r]] : 0u;
+        uint32_t cur = v ? (uint32_t)wcnt[w * B + bin[r]] : 0u;
         rank[r] = cur + (uint32_t)__popcll(peers & lanemask_lt());
         // every lane of the wave has read `cur` before a leader stores the new running count
-        if (v && (peers & lanemask_lt()) == 0ull) wcnt[w * B + bin[r]] = cur + (uint32_t)__popcll(peers);
+        if (v && (peers & lanemask_lt()) == 0ull) wcnt[w * B + bin[r]] = (uint16_t)(cur + (uint32_t)__popcll(peers));
     }
     __syncthreads();
     for (uint32_t k = tid; k < B; k += kThreads) {
         uint32_t acc = 0;
         for (int ww = 0; ww < kWaves; ++ww) {
             uint32_t x = wcnt[ww * B + k];
-            wcnt[ww * B + k] = acc;
+            wcnt[ww * B + k] = (uint16_t)acc;
             acc += x;
         }
         if constexpr (Stage) ttot[k] = acc;

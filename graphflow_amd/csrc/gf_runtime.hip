@@ -281,24 +281,43 @@ template <class P, int R>
 __global__ __launch_bounds__(kThreads) void k_engine_kb(Engine e)
 {
     extern __shared__ __align__(16) char smem[];
-    __shared__ uint32_t flag, pre[kMaxNodes + 1], wsink[kMaxNodes];
-    load_prefix(e.kb_prefix, e.n_nodes, pre);
-    for (uint32_t k = threadIdx.x; k < e.n_nodes; k += kThreads)
-        wsink[k] = (e.arr[k].flags & NODE_WAVE_SINK) && e.st[k].active ? 1u : 0u;
-    if (engine_stopped(e, &flag)) return;  // its barrier also publishes `pre` and `wsink`
-    const uint32_t total = pre[e.n_nodes], G = gridDim.x;
+    __shared__ uint32_t flag, pre[kMaxNodes + 1], order[kMaxNodes], wsink[kMaxNodes], scan[kWaves];
+    const uint32_t n = e.n_nodes, tid = threadIdx.x;
+    // work order: the partitions' scatter tiles first (the longest tiles of a step), then the other
+    // lanes' tiles, each group in node order; pre = tiles before position i, order[i] = its lane
+    // (a thread per lane, n <= kMaxNodes <= the block)
+    uint32_t tiles = 0u, part = 0u, other = 0u;
+    if (tid < n) {
+        tiles = e.kb_prefix[tid + 1u] - e.kb_prefix[tid];
+        part = e.arr[tid].kind == KIND_PARTITION ? 1u : 0u;
+        other = 1u - part;
+        wsink[tid] = (e.arr[tid].flags & NODE_WAVE_SINK) && e.st[tid].active ? 1u : 0u;
+    }
+    uint32_t n_part, t_part, t_other, n_other;
+    const uint32_t i_part = block_exscan(part, scan, &n_part);
+    const uint32_t i_other = block_exscan(other, scan, &n_other);
+    const uint32_t p_part = block_exscan(part * tiles, scan, &t_part);
+    const uint32_t p_other = block_exscan(other * tiles, scan, &t_other);
+    if (tid < n) {
+        const uint32_t i = part ? i_part : n_part + i_other;
+        order[i] = tid;
+        pre[i] = part ? p_part : t_part + p_other;
+    }
+    if (tid == 0) pre[n] = t_part + t_other;
+    if (engine_stopped(e, &flag)) return;  // its barrier also publishes `pre`, `order` and `wsink`
+    const uint32_t total = pre[n], G = gridDim.x;
     for (uint32_t t = blockIdx.x; t < total; t += G) {
-        const uint32_t k = lane_of(pre, e.n_nodes, t);
+        const uint32_t i = lane_of(pre, n, t), k = order[i];
         if (wsink[k]) {
-            // skip the node's tiles: the first tile of this workgroup's round-robin at or after them
-            const uint32_t end = pre[k + 1u];
+            // skip the lane's tiles: the first tile of this workgroup's round-robin at or after them
+            const uint32_t end = pre[i + 1u];
             t += (end - t + G - 1u) / G * G - G;
             continue;
         }
         const gf_node_desc& nd = e.nodes[k];
         const NodeSt s = e.st[k];
         const Batch b = make_batch(e, k, s);
-        const uint32_t lt = t - pre[k];
+        const uint32_t lt = t - pre[i];
         const uint32_t kind = e.arr[k].kind;
         if (kind == KIND_SOURCE) {
             source_tile<P>(nd, b, lt, e.cols, e.C, e.rp);
@@ -318,9 +337,10 @@ __global__ __launch_bounds__(kThreads) void k_engine_kb(Engine e)
         }
     }
     const uint32_t w = threadIdx.x >> 6, W = G * (uint32_t)kWaves;
-    for (uint32_t k = 0; k < e.n_nodes; ++k) {
+    for (uint32_t i = n_part; i < n; ++i) {
+        const uint32_t k = order[i];
         if (!wsink[k]) continue;
-        const uint32_t t0 = pre[k], t1 = pre[k + 1u];
+        const uint32_t t0 = pre[i], t1 = pre[i + 1u];
         const gf_node_desc& nd = e.nodes[k];
         const Batch b = make_batch(e, k, e.st[k]);
         for (uint32_t t = t0 + blockIdx.x * (uint32_t)kWaves + w; t < t1; t += W)
