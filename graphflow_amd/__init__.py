@@ -115,6 +115,7 @@ SIGNATURES = {
     "gf_status_name": (C.c_char_p, [C.c_int]),
     "gf_context_create": (C.c_int, [C.c_int, _P(_V)]),
     "gf_context_destroy": (C.c_int, [_V]),
+    "gf_context_last_error": (C.c_int, [_V, C.c_char_p, C.c_uint32]),
     "gf_output_chunk_bound": (C.c_int, [C.c_uint64, C.c_uint64, C.c_uint64, _P(C.c_uint64)]),
     "gf_plan_dense_bins": (C.c_int, [_P(C.c_uint32), C.c_uint32, C.c_uint32, _P(C.c_uint32), _P(C.c_uint32),
                                      C.c_uint32, _P(C.c_uint32)]),
@@ -779,7 +780,11 @@ class Server:
         self.cfg = ServerConfigC(slots=slots, team_size=team_size, memory_budget=memory_budget, max_items=max_items,
                                  prewarm=1 if prewarm else 0)
         h = _V()
-        _raise(lib().gf_server_create(self.ctx.handle, C.byref(self.cfg), C.byref(h)))
+        rc = lib().gf_server_create(self.ctx.handle, C.byref(self.cfg), C.byref(h))
+        if rc != OK:
+            buf = C.create_string_buffer(512)
+            lib().gf_context_last_error(self.ctx.handle, buf, 512)
+            _raise(rc, buf.value.decode(errors="replace"))
         self.handle = h
 
     def submit(self, topology: int, items: int, seed: int = 42) -> int:

@@ -712,7 +712,7 @@ __device__ void scatter_tile_hash(const gf_node_desc& nd, const Batch& b, uint32
         P p = rec[r];
         const uint32_t u = update(nd, k, p);
         if (u) set_status(status, u);
-        if (fuse && fuse[k] >= 0) transform(all[fuse[k]], *rp, 2u * (uint32_t)fuse[k], p);
+        if (fuse && fuse[k] >= 0) transform(all[fuse[k] & 0xFFFF], *rp, (uint32_t)fuse[k] >> 16, p);
         p.store(cols, (size_t)b.out[f / C].index * C + f % C);
     }
     __syncthreads();
@@ -799,7 +799,7 @@ __device__ void scatter_tile(const gf_node_desc& nd, const Batch& b, uint32_t t,
             uint32_t k = bin[r], within = wcnt[w * B + k] + rank[r];
             uint32_t u = update(nd, k, rec[r]);
             if (u) set_status(status, u);
-            if (fuse && fuse[k] >= 0) transform(all[fuse[k]], *rp, 2u * (uint32_t)fuse[k], rec[r]);
+            if (fuse && fuse[k] >= 0) transform(all[fuse[k] & 0xFFFF], *rp, (uint32_t)fuse[k] >> 16, rec[r]);
             uint32_t p = tstart[k] + within;
             stage[p] = rec[r];
             flat[p] = toff[k] + (bin_base ? bin_base[k] * C : 0u) + within;
@@ -820,7 +820,7 @@ __device__ void scatter_tile(const gf_node_desc& nd, const Batch& b, uint32_t t,
             P p = rec[r];
             uint32_t u = update(nd, k, p);
             if (u) set_status(status, u);
-            if (fuse && fuse[k] >= 0) transform(all[fuse[k]], *rp, 2u * (uint32_t)fuse[k], p);
+            if (fuse && fuse[k] >= 0) transform(all[fuse[k] & 0xFFFF], *rp, (uint32_t)fuse[k] >> 16, p);
             p.store(cols, (size_t)b.out[f / C].index * C + f % C);
         }
         __syncthreads();

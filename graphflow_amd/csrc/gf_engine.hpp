@@ -50,15 +50,24 @@ struct NodeArr {
     uint32_t lanes;  // first lane of a node: the node's lanes (it stages candidates for all of them)
     uint32_t ring;   // first lane of a node: index of the node's ready ring in `rings`
     uint32_t hash_entries;  // partitions on the hash bin path: slots of a tile's LDS bin table; 0: dense path
-    uint32_t pad;
+    uint32_t job;           // served engines: the job slot whose copy of the graph the lane belongs to
 };
 
 enum { NODE_STAGE = 1u, NODE_WAVE_SINK = 2u };
 
 enum { KIND_SOURCE = 0, KIND_PARTITION = 1, KIND_TRANSFORM = 2, KIND_SINK = 3 };
 
+// One job slot of a served engine: the engine runs the union of M copies of a topology's graph, one
+// per slot, and the host admits a job into a free slot between launches (items, live bound, seed;
+// admitted and sunk items start at 0). The select keeps admitted and sunk up to date; a job is done
+// when sunk == N.
+struct JobSlot {
+    unsigned long long N, P, seed, gen, sunk;
+};
+constexpr uint32_t kMaxJobs = 64;
+
 struct Engine {
-    uint32_t n_nodes, C, P, Q, S, ring_cap, trace_on, pad;
+    uint32_t n_nodes, C, P, Q, S, ring_cap, trace_on, n_jobs;  // n_jobs: slots of a served engine, else 0
     uint64_t N;
     RunParams rp;
     Cols cols;
@@ -82,6 +91,7 @@ struct Engine {
     unsigned long long* rlb;   // look-back words of the grid routing, [unit][lane]
     unsigned long long* llb;   // look-back words of the grid input lists, [unit][2] (tiles, items)
     uint32_t route_units_cap, list_units_cap;
+    JobSlot* jobs;             // served engines: n_jobs slots; null otherwise (one job: N, P, the run's seed)
 };
 
 enum { PROF_COUNT = 0, PROF_PLAN = 1, PROF_SOURCE = 2, PROF_PART = 3, PROF_XF = 4, PROF_SINK = 5, PROF_RETIRE = 6,
@@ -473,7 +483,9 @@ struct SelLds {
     uint32_t wanted[kMaxNodes];
     uint32_t taken[kMaxNodes];  // first lane of a node: ready chunks its lanes took this step
     uint32_t in_at[kMaxNodes];  // lane: position of its first input in its node's ready chunks
-    uint32_t kind[kMaxNodes], ring_of[kMaxNodes], lanes[kMaxNodes], bins[kMaxNodes];  // load_lane_tab
+    uint32_t kind[kMaxNodes], ring_of[kMaxNodes], lanes[kMaxNodes], bins[kMaxNodes], job[kMaxNodes];  // load_lane_tab
+    unsigned long long jN[kMaxJobs], jP[kMaxJobs], jgen[kMaxJobs], jsunk[kMaxJobs];  // load_job_tab
+    uint32_t njobs;
     uint32_t wcnt[kMaxWaves][kMaxNodes];
     uint32_t scan[kMaxWaves];
     uint32_t top, total, any, staged, overflow;
@@ -545,6 +557,31 @@ __device__ inline void load_lane_tab(const Engine& e, SelLds& L)
         L.ring_of[k] = e.ring_of[k];
         L.lanes[k] = e.arr[k].lanes;
         L.bins[k] = e.nodes[k].bins;
+        L.job[k] = e.arr[k].job;
+    }
+}
+
+// The jobs' items, live bounds and admitted / sunk counts into LDS (a thread per job). An engine
+// that is not served runs one job: the run's N and P, the engine's admitted and sunk totals
+// (L.generated and L.sunk must be current).
+__device__ inline void load_job_tab(const Engine& e, SelLds& L)
+{
+    const uint32_t tid = threadIdx.x;
+    if (e.jobs) {
+        if (tid < e.n_jobs) {
+            const JobSlot& j = e.jobs[tid];
+            L.jN[tid] = j.N;
+            L.jP[tid] = j.P;
+            L.jgen[tid] = j.gen;
+            L.jsunk[tid] = j.sunk;
+        }
+        if (tid == 0) L.njobs = e.n_jobs;
+    } else if (tid == 0) {
+        L.jN[0] = e.N;
+        L.jP[0] = e.P;
+        L.jgen[0] = L.generated;
+        L.jsunk[0] = L.sunk;
+        L.njobs = 1u;
     }
 }
 
@@ -685,6 +722,8 @@ __device__ inline void engine_retire(const Engine& e, SelLds& L)
         L.step += 1u;
     }
     __syncthreads();
+    load_job_tab(e, L);
+    __syncthreads();
 }
 
 // Select of the next step. With lists = false the input lists, tile / item prefixes and work
@@ -692,7 +731,7 @@ __device__ inline void engine_retire(const Engine& e, SelLds& L)
 // from its ring at in_slot.
 __device__ inline void engine_select(const Engine& e, SelLds& L, bool lists = true)
 {
-    const uint32_t tid = threadIdx.x, n = e.n_nodes, C = e.C, P = e.P, Q = e.Q;
+    const uint32_t tid = threadIdx.x, n = e.n_nodes, C = e.C, Q = e.Q;
     load_lane_tab(e, L);
     __syncthreads();
     // stage the candidate ready descriptors of every transform's ring in LDS: <= Q per lane (the
@@ -749,16 +788,19 @@ __device__ inline void engine_select(const Engine& e, SelLds& L, bool lists = tr
     bool serial = L.overflow != 0u;
     if (!serial) {
         if (tid < n && L.kind[tid] == KIND_SOURCE && L.ring_of[tid] == tid) {
-            unsigned long long gen = L.generated;
+            // the source of one job: its lanes admit in turn against the job's items and live bound
+            const uint32_t jb = L.job[tid];
+            const unsigned long long N = L.jN[jb], Pj = L.jP[jb], sunk = L.jsunk[jb];
+            unsigned long long gen = L.jgen[jb];
             const uint32_t T = e.arr[tid].tile_items;
             for (uint32_t j = 0; j < L.lanes[tid]; ++j) {
-                const unsigned long long live = gen - L.sunk, remaining = e.N - gen;
-                unsigned long long min_fill = C < P ? C : P;
+                const unsigned long long live = gen - sunk, remaining = N - gen;
+                unsigned long long min_fill = C < Pj ? C : Pj;
                 if (remaining < min_fill) min_fill = remaining;
                 uint32_t fill = 0u;
-                if (gen < e.N && live < P && (P - live) >= min_fill) {
+                if (gen < N && live < Pj && (Pj - live) >= min_fill) {
                     unsigned long long f = remaining;
-                    if (f > P - live) f = P - live;
+                    if (f > Pj - live) f = Pj - live;
                     if (f > C) f = C;
                     fill = (uint32_t)f;
                 }
@@ -785,7 +827,7 @@ __device__ inline void engine_select(const Engine& e, SelLds& L, bool lists = tr
         serial = dem[0] > (unsigned long long)L.top;
     }
     if (!serial) {
-        unsigned long long act[1] = {0ull};
+        unsigned long long act[2] = {0ull, 0ull};  // active lanes, admitted items
         if (tid < n) {
             NodeSt& s = L.st[tid];
             const uint32_t kind = L.kind[tid];
@@ -799,10 +841,11 @@ __device__ inline void engine_select(const Engine& e, SelLds& L, bool lists = tr
                     s.first_id = L.dfirst[tid];
                     s.n_tiles = L.dtiles[tid];
                     s.active = 1u;
+                    act[1] = L.dfill[tid];
                 }
                 if (L.ring_of[tid] == tid) {
                     const uint32_t last = tid + L.lanes[tid] - 1u;
-                    L.generated = L.dfirst[last] + L.dfill[last];
+                    L.jgen[L.job[tid]] = L.dfirst[last] + L.dfill[last];
                 }
             } else if (kind == KIND_SINK) {
                 s.n_in = L.wanted[tid];
@@ -824,7 +867,10 @@ __device__ inline void engine_select(const Engine& e, SelLds& L, bool lists = tr
             act[0] = s.active;
         }
         block_sum64(act, L.red);
-        if (tid == 0) L.any = act[0] ? 1u : 0u;
+        if (tid == 0) {
+            L.any = act[0] ? 1u : 0u;
+            L.generated += act[1];
+        }
     }
     if (serial && tid == 0) {
         uint32_t reserved = 0;
@@ -837,16 +883,19 @@ __device__ inline void engine_select(const Engine& e, SelLds& L, bool lists = tr
             s.n_out = 0u;
             s.n_tiles = 0u;
             if (kind == KIND_SOURCE) {
-                unsigned long long live = L.generated - L.sunk, remaining = e.N - L.generated;
-                unsigned long long min_fill = C < P ? C : P;
+                const uint32_t jb = L.job[k];
+                const unsigned long long gen = L.jgen[jb], N = L.jN[jb], Pj = L.jP[jb];
+                unsigned long long live = gen - L.jsunk[jb], remaining = N - gen;
+                unsigned long long min_fill = C < Pj ? C : Pj;
                 if (remaining < min_fill) min_fill = remaining;
-                if (L.generated < e.N && live < P && (P - live) >= min_fill && L.top - reserved >= 1u) {
+                if (gen < N && live < Pj && (Pj - live) >= min_fill && L.top - reserved >= 1u) {
                     unsigned long long fill = remaining;
-                    if (fill > P - live) fill = P - live;
+                    if (fill > Pj - live) fill = Pj - live;
                     if (fill > C) fill = C;
                     s.particles = (uint32_t)fill;
-                    s.first_id = L.generated;
+                    s.first_id = gen;
                     s.n_tiles = (uint32_t)((fill + e.arr[k].tile_items - 1u) / e.arr[k].tile_items);
+                    L.jgen[jb] = gen + fill;
                     L.generated += fill;
                     reserved += 1u;
                     s.active = 1u;
@@ -942,7 +991,7 @@ __device__ inline void engine_select(const Engine& e, SelLds& L, bool lists = tr
     }
     // work prefixes, ring cursors, termination: a thread per lane
     __syncthreads();
-    unsigned long long busy[1] = {0ull};
+    unsigned long long busy[2] = {0ull, 0ull};  // lanes with ready chunks, jobs not fully admitted
     {
         uint32_t ta = 0u, tb = 0u;
         if (tid < n) {
@@ -954,6 +1003,12 @@ __device__ inline void engine_select(const Engine& e, SelLds& L, bool lists = tr
                 if (L.kind[tid] == KIND_PARTITION) ta = s.n_tiles;
             }
             busy[0] = L.tail[tid] != L.head[tid] ? 1ull : 0ull;
+        }
+        if (tid < L.njobs) busy[1] = L.jgen[tid] < L.jN[tid] ? 1ull : 0ull;
+        // a served engine keeps its jobs' counters in the job table
+        if (e.jobs && tid < L.njobs) {
+            e.jobs[tid].gen = L.jgen[tid];
+            e.jobs[tid].sunk = L.jsunk[tid];
         }
         if (lists) {
             uint32_t sa, sb;
@@ -969,6 +1024,14 @@ __device__ inline void engine_select(const Engine& e, SelLds& L, bool lists = tr
             }
         }
         block_sum64(busy, L.red);
+        // done (every lane idle, nothing in flight, every job admitted): no work prefix may keep an
+        // earlier step's tiles, since the grid list kernel that rewrites them stops with the engine
+        // and a served engine is woken for its next jobs without a setup
+        if (!L.any && busy[0] == 0ull && busy[1] == 0ull)
+            for (uint32_t k = tid; k <= n; k += bsz()) {
+                e.ka_prefix[k] = 0u;
+                e.kb_prefix[k] = 0u;
+            }
     }
     if (tid == 0) {
         e.ctl[CTL_TOP] = L.top;
@@ -983,8 +1046,10 @@ __device__ inline void engine_select(const Engine& e, SelLds& L, bool lists = tr
         e.ctr[CNT_SRC_ITEMS] = L.src_items;
         e.ctr[CNT_SINK_ITEMS] = L.sink_items;
         if (!L.any) {
+            // every job admitted and nothing in flight: done (a served engine: idle until the host
+            // admits more jobs)
             const bool drained = busy[0] == 0ull;
-            if (drained && L.generated >= e.N) {
+            if (drained && busy[1] == 0ull) {
                 e.ctl[CTL_DONE] = 1u;
                 if (L.top != e.S) set_status(&e.ctl[CTL_STATUS], GF_BAD_ARGUMENT);
             } else {
@@ -1395,6 +1460,7 @@ __device__ inline void engine_retire_counters(const Engine& e, SelLds& L)
             }
         }
     }
+    const unsigned long long sunk_here = v[1];  // this lane's sunk items (a sink lane)
     block_sum64(v, L.red);
     if (tid < n) L.st[tid].active = 0u;
     if (tid == 0) {
@@ -1415,6 +1481,12 @@ __device__ inline void engine_retire_counters(const Engine& e, SelLds& L)
         L.xf_items += v[5];
         L.step += 1u;
     }
+    __syncthreads();
+    load_job_tab(e, L);
+    __syncthreads();
+    // a served engine: each sink lane's items to its job (an engine that is not served has them in
+    // L.sunk already)
+    if (e.jobs && sunk_here) atomicAdd(&L.jsunk[L.job[tid]], sunk_here);
     __syncthreads();
 }
 

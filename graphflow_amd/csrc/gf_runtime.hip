@@ -319,21 +319,23 @@ __global__ __launch_bounds__(kThreads) void k_engine_kb(Engine e)
         const Batch b = make_batch(e, k, s);
         const uint32_t lt = t - pre[i];
         const uint32_t kind = e.arr[k].kind;
+        RunParams rp = e.rp;
+        if (e.jobs) rp.seed = e.jobs[e.arr[k].job].seed;  // a served engine: the lane's job seed
         if (kind == KIND_SOURCE) {
-            source_tile<P>(nd, b, lt, e.cols, e.C, e.rp);
+            source_tile<P>(nd, b, lt, e.cols, e.C, rp);
         } else if (kind == KIND_SINK) {
             sink_tile<P>(nd, b, lt, e.cols, e.C, &e.ctl[CTL_STATUS], smem);
         } else if (kind == KIND_PARTITION) {
             const int32_t* fz = e.fuse + nd.route_offset;
             if (e.arr[k].hash_entries)
                 scatter_tile_hash<P, R>(nd, b, lt, e.cols, e.C, &e.ctl[CTL_STATUS], smem, e.arr[k].hash_entries,
-                                        e.real_nodes, fz, &e.rp, b.totals + nd.bins);
+                                        e.real_nodes, fz, &rp, b.totals + nd.bins);
             else if (e.arr[k].flags & NODE_STAGE)
-                scatter_tile<P, R, true>(nd, b, lt, e.cols, e.C, &e.ctl[CTL_STATUS], smem, e.real_nodes, fz, &e.rp, b.totals + nd.bins);
+                scatter_tile<P, R, true>(nd, b, lt, e.cols, e.C, &e.ctl[CTL_STATUS], smem, e.real_nodes, fz, &rp, b.totals + nd.bins);
             else
-                scatter_tile<P, R, false>(nd, b, lt, e.cols, e.C, &e.ctl[CTL_STATUS], smem, e.real_nodes, fz, &e.rp, b.totals + nd.bins);
+                scatter_tile<P, R, false>(nd, b, lt, e.cols, e.C, &e.ctl[CTL_STATUS], smem, e.real_nodes, fz, &rp, b.totals + nd.bins);
         } else {
-            transform_tile<P, R>(nd, b, lt, e.cols, e.C, e.rp);
+            transform_tile<P, R>(nd, b, lt, e.cols, e.C, rp);
         }
     }
     const uint32_t w = threadIdx.x >> 6, W = G * (uint32_t)kWaves;
@@ -485,6 +487,9 @@ __global__ __launch_bounds__(kThreads) void k_engine_setup(Engine e, uint32_t do
         L.part_items = L.xf_items = L.src_items = L.sink_items = 0ull;
     }
     __syncthreads();
+    load_lane_tab(e, L);
+    load_job_tab(e, L);
+    __syncthreads();
     if (do_select) {
         engine_select(e, L);
         return;
@@ -519,6 +524,9 @@ __global__ __launch_bounds__(kThreads) void k_engine_resume(Engine e)
         L.head[k] = L.st[k].head;
         L.st[k].active = 0u;
     }
+    __syncthreads();
+    load_lane_tab(e, L);
+    load_job_tab(e, L);
     __syncthreads();
     engine_select(e, L);
 }
@@ -1317,6 +1325,7 @@ struct NodeRec {
     std::string name;
     gf_node_desc nd{};
     std::vector<int32_t> routes;
+    uint32_t slot = 0, local = 0;  // served flows: the job slot whose graph copy holds the node, its id there
 };
 
 struct EngineBuffers {
@@ -1344,6 +1353,7 @@ struct EngineBuffers {
 struct gf_flow {
     gf_context* ctx = nullptr;
     uint32_t payload = 0;
+    uint32_t slots = 0;  // served flows: copies of one topology's graph, one per job slot (0: a plain flow)
     std::vector<NodeRec> nodes;
     gf_tuning tuning{};
     std::string err;
@@ -1374,7 +1384,20 @@ void check_graph(const gf_flow& f, uint32_t source)
             if ((size_t)d >= f.nodes.size()) fail(GF_BAD_ARGUMENT, "graphFlow: route destination out of range");
         }
     }
-    if (sources != 1) fail(GF_BAD_ARGUMENT, "graphFlow: a graph needs exactly one source node");
+    if (!f.slots && sources != 1) fail(GF_BAD_ARGUMENT, "graphFlow: a graph needs exactly one source node");
+    if (f.slots) {
+        // a served flow: one source per job slot, and no route leaves its slot's copy of the graph
+        std::vector<uint32_t> per(f.slots, 0u);
+        for (const auto& n : f.nodes) {
+            if (n.slot >= f.slots) fail(GF_BAD_ARGUMENT, "graphFlow: node outside the served flow's slots");
+            if (n.nd.role == GF_ROLE_SOURCE) ++per[n.slot];
+            for (int32_t d : n.routes)
+                if (f.nodes[(size_t)d].slot != n.slot) fail(GF_BAD_ARGUMENT, "graphFlow: a route crosses job slots");
+        }
+        for (uint32_t c : per)
+            if (c != 1u) fail(GF_BAD_ARGUMENT, "graphFlow: every job slot needs exactly one source node");
+        if (f.slots > gfd::kMaxJobs) fail(GF_BAD_ARGUMENT, "graphFlow: too many job slots");
+    }
 }
 
 struct RunShape {
@@ -1705,7 +1728,7 @@ std::string shape_key(const gf_flow& f, const RunShape& s, uint64_t N)
                     std::to_string(f.tuning.max_workgroups) + "/" + std::to_string(f.tuning.device_engine) + "/" +
                     std::to_string(f.tuning.use_graph) + "/" + std::to_string(f.tuning.team_xcd) + "/" +
                     std::to_string(f.tuning.inflight_batches) + "/" + std::to_string(f.tuning.bin_path) + "/" +
-                    std::to_string(f.tuning.hash_table_entries);
+                    std::to_string(f.tuning.hash_table_entries) + "/" + std::to_string(f.slots);
     {
         const PhotonParams ph = photon_params_of(f.tuning);
         char buf[160];
@@ -1774,18 +1797,23 @@ void build_engine(gf_flow& f, uint64_t N, const RunShape& sh, uint32_t src)
     size_t lds_a = 256, lds_b = 256, group_lds = 256;
     uint32_t grid_tiles = 0;
     std::vector<NodeArr> arr(nv);
+    float zipf_s = 0.0f;
     auto is_plain_xf = [&](int32_t k) {
         return k >= 0 && f.nodes[k].nd.role == GF_ROLE_TRANSFORM && !is_partition(f.nodes[k].nd) && f.nodes[k].nd.bins == 1u;
     };
+    // stream purpose of a node: 2 x its id in its job's graph (a served flow's slot copy keeps the
+    // purposes of the job run alone)
+    auto purpose_of = [&](uint32_t k) { return 2u * (f.slots ? f.nodes[k].local : k); };
     for (uint32_t k = 0; k < n; ++k) {
         real[k] = f.nodes[k].nd;
         real[k].route_offset = (uint32_t)routes.size();
         const bool part = real[k].role == GF_ROLE_TRANSFORM && is_partition(real[k]);
         for (int32_t dst : f.nodes[k].routes) {
             // a plain transform fed by a partition bin runs inside that partition's scatter; routes
-            // name the destination's first lane, which owns the destination's ready ring
+            // name the destination's first lane, which owns the destination's ready ring. A fused
+            // entry packs the transform's node (low 16 bits) and its stream purpose (high 16 bits).
             if (part && t.fuse_transforms && is_plain_xf(dst)) {
-                fuse.push_back(dst);
+                fuse.push_back((int32_t)((uint32_t)dst | purpose_of((uint32_t)dst) << 16));
                 routes.push_back((int32_t)first_lane[f.nodes[dst].routes[0]]);
             } else {
                 fuse.push_back(-1);
@@ -1830,7 +1858,8 @@ void build_engine(gf_flow& f, uint64_t N, const RunShape& sh, uint32_t src)
             a.counts = part ? m.d<uint32_t>((size_t)cap_tiles * B) : nullptr;
             a.totals = m.d<uint32_t>(2u * B);
             a.tile_items = ti;
-            a.purpose = 2u * k;
+            a.purpose = purpose_of(k);
+            a.job = f.nodes[k].slot;
             if (wave_sink[k]) a.flags |= NODE_WAVE_SINK;
             if (part) {
                 a.hash_entries = hash_entries_of(nd, t.bin_path, t.hash_table_entries, sh.tile_items);
@@ -1850,7 +1879,8 @@ void build_engine(gf_flow& f, uint64_t N, const RunShape& sh, uint32_t src)
                 grid_tiles += cap_tiles;
             }
         }
-        if (nd.op == GF_SRC_KEYED_ZIPF) {
+        if (nd.op == GF_SRC_KEYED_ZIPF && !(e.rp.zipf && e.rp.zipf_values == nd.u[0] && zipf_s == nd.f[0])) {
+            zipf_s = nd.f[0];  // a served flow's slot copies share one table
             auto z = zipf_table(nd.u[0], (double)nd.f[0]);
             double* dz = m.d<double>(z.size());
             copy_sync(dz, z.data(), z.size() * sizeof(double), hipMemcpyHostToDevice, f.ctx->stream);
@@ -1901,6 +1931,15 @@ void build_engine(gf_flow& f, uint64_t N, const RunShape& sh, uint32_t src)
     e.list_units_cap = (uint32_t)ceil_div(sh.S, kGridBlock) + nv + 1u;
     e.rlb = m.d<unsigned long long>((size_t)e.route_units_cap * nv);
     e.llb = m.d<unsigned long long>(2ull * e.list_units_cap);
+    e.n_jobs = f.slots;
+    e.jobs = nullptr;
+    if (f.slots) {
+        // every slot empty (N = 0: done); the host admits jobs between launches
+        std::vector<JobSlot> z(f.slots);
+        std::memset(z.data(), 0, sizeof(JobSlot) * z.size());
+        e.jobs = m.d<JobSlot>(f.slots);
+        copy_sync(e.jobs, z.data(), sizeof(JobSlot) * z.size(), hipMemcpyHostToDevice, f.ctx->stream);
+    }
     eb->route_grid = std::min<uint32_t>(64u, e.route_units_cap + nv);
     const uint32_t gmax = t.max_workgroups ? t.max_workgroups : 64u;
     eb->grid = std::min<uint32_t>(std::max<uint32_t>(grid_tiles, 1u), gmax);
@@ -1918,7 +1957,8 @@ void build_engine(gf_flow& f, uint64_t N, const RunShape& sh, uint32_t src)
     // team engine: every partition narrow enough for a per-segment count row in LDS; its replicated
     // scheduler runs one lane per node
     uint32_t bb = 1;
-    bool team_ok = D == 1u && n <= kTeamNodes && routes.size() <= kTeamRoutes && ceil_div(sh.C, kSeg) <= (1u << 17);
+    bool team_ok = D == 1u && f.slots <= kTeamJobs && n <= kTeamNodes && routes.size() <= kTeamRoutes &&
+                   ceil_div(sh.C, kSeg) <= (1u << 17);
     for (uint32_t k = 0; k < n; ++k)
         if (is_partition(real[k]) && real[k].role == GF_ROLE_TRANSFORM) bb = std::max(bb, real[k].bins + 1u);
     if (bb > kTeamBins) team_ok = false;
@@ -1957,7 +1997,8 @@ void build_engine(gf_flow& f, uint64_t N, const RunShape& sh, uint32_t src)
         HIPX(hipFuncSetAttribute((const void*)k_team<Keyed>, hipFuncAttributeMaxDynamicSharedMemorySize, (int)sizeof(TeamLds)));
         HIPX(hipFuncSetAttribute((const void*)k_team<Photon>, hipFuncAttributeMaxDynamicSharedMemorySize, (int)sizeof(TeamLds)));
     }
-    eb->micro = !eb->team && (t.device_engine == 2u ||
+    if (f.slots && t.device_engine == 2u) fail(GF_BAD_ARGUMENT, "graphFlow: a served flow runs on the wide or the team engine");
+    eb->micro = !eb->team && !f.slots && (t.device_engine == 2u ||
                               (t.device_engine == 0u && micro_fits && (uint64_t)sh.P * rec <= (4ull << 20)));
     with_payload(f.payload, r, [&](auto pt, auto rc) {
         using P = typename decltype(pt)::type;
@@ -2278,6 +2319,13 @@ gf_status gf_context_destroy(gf_context* ctx)
     if (!ctx) return GF_BAD_ARGUMENT;
     if (ctx->stream) (void)hipStreamDestroy(ctx->stream);
     delete ctx;
+    return GF_OK;
+}
+
+gf_status gf_context_last_error(gf_context* ctx, char* buffer, uint32_t size)
+{
+    if (!ctx || !buffer || size == 0) return GF_BAD_ARGUMENT;
+    std::snprintf(buffer, size, "%s", ctx->err.c_str());
     return GF_OK;
 }
 
@@ -2624,7 +2672,7 @@ gf_status gf_flow_add_node(gf_flow* flow, const char* name, const gf_node_desc* 
     if (!flow || !node || !key) return GF_BAD_ARGUMENT;
     return guard(&flow->err, [&] {
         if (node->role > GF_ROLE_SINK) fail(GF_BAD_ARGUMENT, "graphFlow: unknown node role");
-        if (node->role == GF_ROLE_SOURCE)
+        if (node->role == GF_ROLE_SOURCE && !flow->slots)  // a served flow: one source per job slot
             for (auto& n : flow->nodes)
                 if (n.nd.role == GF_ROLE_SOURCE) fail(GF_BAD_ARGUMENT, "graphFlow: a Flow has exactly one source node");
         if (node->role == GF_ROLE_SINK && node->bins != 0) fail(GF_BAD_ARGUMENT, "graphFlow: a sink node has no bins and no routes");

@@ -41,6 +41,7 @@
 namespace gfd {
 
 constexpr uint32_t kTeamNodes = 32;
+constexpr uint32_t kTeamJobs = 8;    // job slots of a served engine on the team engine
 constexpr uint32_t kTeamBins = 32;  // count row stride: bins of the widest partition + invalid slot
 constexpr uint32_t kTeamRoutes = 512;
 constexpr uint32_t kSegShift = 7;
@@ -156,6 +157,12 @@ struct TeamLds {
     uint32_t zero[kTeamZero];  // count rows to clear in the next step: chunk | (generation & 1) << 31
     unsigned long long generated, sunk, step, batches, part_items, xf_items, src_items, sink_items, peak_live, trace_len;
     unsigned long long digest[kTeamNodes];
+    // jobs: the node's job slot and seed; per job its items, live bound, admitted and sunk items (an
+    // engine that is not served: one job, the run's N, P and seed)
+    unsigned long long seed[kTeamNodes];
+    uint32_t job[kTeamNodes];
+    unsigned long long jN[kTeamJobs], jP[kTeamJobs], jgen[kTeamJobs], jsunk[kTeamJobs];
+    uint32_t njobs;
     uint32_t fs_base, top, zero_n, status, exit_code, total_out, err_prev;
     uint32_t seg_err;      // a segment of this member reported a device status this step
     uint32_t prof_member;  // this member records sub-phase timings
@@ -310,7 +317,7 @@ __device__ inline void team_load(const TeamArgs& a, TeamLds& L)
         ((uint32_t*)L.nd)[i] = ((const uint32_t*)e.nodes)[i];
     for (uint32_t i = tid; i < a.n_routes; i += kTeamThreads) {
         L.route[i] = e.routes[i];
-        L.fuse[i] = e.fuse[i];
+        L.fuse[i] = e.fuse[i] < 0 ? -1 : (e.fuse[i] & 0xFFFF);  // the fused node; its stream purpose is 2 x node here
     }
     if (tid < n) {
         L.kind[tid] = e.arr[tid].kind;
@@ -322,6 +329,21 @@ __device__ inline void team_load(const TeamArgs& a, TeamLds& L)
         L.digest[tid] = e.ctr[CNT_DIGEST0 + tid];
         L.sb[0].active[tid] = 0u;
         L.sb[1].active[tid] = 0u;
+        L.job[tid] = e.arr[tid].job;
+        L.seed[tid] = e.jobs ? e.jobs[e.arr[tid].job].seed : e.rp.seed;
+    }
+    if (e.jobs) {
+        if (tid < e.n_jobs) {
+            L.jN[tid] = e.jobs[tid].N;
+            L.jP[tid] = e.jobs[tid].P;
+            L.jgen[tid] = e.jobs[tid].gen;
+            L.jsunk[tid] = e.jobs[tid].sunk;
+        }
+    } else if (tid == 0) {
+        L.jN[0] = e.N;
+        L.jP[0] = e.P;
+        L.jgen[0] = e.ctr[CNT_GENERATED];
+        L.jsunk[0] = e.ctr[CNT_SUNK];
     }
     if (tid == 0) {
         L.top = e.ctl[CTL_TOP];
@@ -336,6 +358,7 @@ __device__ inline void team_load(const TeamArgs& a, TeamLds& L)
         L.sink_items = e.ctr[CNT_SINK_ITEMS];
         L.trace_len = e.ctr[CNT_TRACE_LEN];
         L.peak_live = e.ctr[CNT_PEAK_LIVE];
+        L.njobs = e.jobs ? e.n_jobs : 1u;
         L.zero_n = a.zero_g[0];
         L.status = 0u;
         L.err_prev = 0u;
@@ -379,6 +402,10 @@ __device__ inline void team_flush(const TeamArgs& a, TeamLds& L)
     }
     for (uint32_t i = tid; i < L.top - L.fs_base; i += kTeamThreads) e.free_stack[L.fs_base + i] = L.fs[i];
     for (uint32_t i = tid; i < L.zero_n; i += kTeamThreads) a.zero_g[1 + i] = L.zero[i];
+    if (e.jobs && tid < L.njobs) {
+        e.jobs[tid].gen = L.jgen[tid];
+        e.jobs[tid].sunk = L.jsunk[tid];
+    }
     if (tid == 0) {
         a.zero_g[0] = L.zero_n;
         e.ctl[CTL_TOP] = L.top;
@@ -494,17 +521,19 @@ __device__ inline void team_select(const TeamArgs& a, TeamLds& L, StepBuf& Bn, b
     // credits in node order, over the nodes that may take a batch
     const uint32_t top = L.top;
     uint32_t credits = top;
-    const unsigned long long gen0 = L.generated, sunk = L.sunk;
-    unsigned long long gen = gen0;
-    uint32_t src_fill = 0, src_node = 0xFFFFFFFFu;
-    // the source's admission (live < P, room for min(C, P, remaining)) does not depend on credits
-    // beyond the one chunk it needs; the source lane evaluates it up front
+    unsigned long long gen = L.generated;
+    uint32_t src_fill = 0;
+    // a source's admission (live < P, room for min(C, P, remaining); one source per job, against its
+    // job's counters) does not depend on credits beyond the one chunk it needs; each source lane
+    // evaluates it up front
     uint32_t adm = 0;
+    const uint32_t jb = own ? L.job[k] : 0u;
+    const unsigned long long gen0 = kind == KIND_SOURCE ? L.jgen[jb] : 0ull;
     if (kind == KIND_SOURCE) {
-        const unsigned long long P = e.P, live = gen0 - sunk, remaining = e.N - gen0;
+        const unsigned long long P = L.jP[jb], N = L.jN[jb], live = gen0 - L.jsunk[jb], remaining = N - gen0;
         unsigned long long min_fill = C < P ? C : P;
         if (remaining < min_fill) min_fill = remaining;
-        if (gen0 < e.N && live < P && P - live >= min_fill) {
+        if (gen0 < N && live < P && P - live >= min_fill) {
             unsigned long long fill = remaining;
             if (fill > P - live) fill = P - live;
             if (fill > C) fill = C;
@@ -517,9 +546,8 @@ __device__ inline void team_select(const TeamArgs& a, TeamLds& L, StepBuf& Bn, b
         cand &= cand - 1u;
         if (rl(kind, j) == KIND_SOURCE) {
             if (credits >= 1u) {
-                src_fill = rl(adm, j);
-                src_node = j;
-                gen += src_fill;
+                if (lane == j) src_fill = adm;
+                gen += rl(adm, j);
                 credits -= 1u;
             }
         } else {
@@ -545,8 +573,8 @@ __device__ inline void team_select(const TeamArgs& a, TeamLds& L, StepBuf& Bn, b
     tk(PROF_C_SEL_CREDIT);
     uint32_t nin = 0, act = 0;
     if (kind == KIND_SOURCE) {
-        act = lane == src_node ? 1u : 0u;
-        parts = act ? src_fill : 0u;
+        act = src_fill ? 1u : 0u;
+        parts = src_fill;
         nseg = (parts + kSeg - 1u) >> kSegShift;
     } else if (own && want) {
         nin = want;
@@ -571,8 +599,10 @@ __device__ inline void team_select(const TeamArgs& a, TeamLds& L, StepBuf& Bn, b
     const uint32_t reserved = top - credits;
     const uint64_t actm = __ballot(act != 0u), pend = __ballot(own && tail != head);
     uint32_t code = EXIT_NONE;
+    // every job fully admitted (lane j: job j)
+    const bool admitted = __ballot(lane < L.njobs && L.jgen[lane < kTeamJobs ? lane : 0u] < L.jN[lane < kTeamJobs ? lane : 0u]) == 0ull;
     if (!actm) {
-        code = (!pend && gen0 >= e.N) ? EXIT_DONE : EXIT_STALL;
+        code = (!pend && admitted) ? EXIT_DONE : EXIT_STALL;
     } else if (at_limit) {
         code = EXIT_LIMIT;
     } else {
@@ -600,7 +630,10 @@ __device__ inline void team_select(const TeamArgs& a, TeamLds& L, StepBuf& Bn, b
             Bn.digest_base[k] = g.digest;
             g.digest += nin;
         }
-        if (kind == KIND_SOURCE && act) Bn.first_id[k] = gen0;
+        if (kind == KIND_SOURCE && act) {
+            Bn.first_id[k] = gen0;
+            L.jgen[jb] = gen0 + parts;  // one source per job
+        }
         if (act && kind != KIND_SOURCE) {
             g.head = head + nin;
             g.rsum -= parts;
@@ -1023,6 +1056,9 @@ __device__ inline void team_retire(const TeamArgs& a, TeamLds& L, StepBuf& Bc, S
     tk(PROF_C_RET_OUT);
     // the consumed inputs go back on the free stack in team_push, once the scatter waves no longer
     // read the popped entries; the top moves now, for the next step's credits
+    // sunk items per job (lane = node; an engine that is not served: job 0 = L.sunk)
+    if (lane < e.n_nodes && g.kind == KIND_SINK && Bc.active[lane])
+        atomicAdd(&L.jsunk[L.job[lane]], (unsigned long long)Bc.particles[lane]);
     if (lane == 0) {
         L.top = top + in_tot;
         L.sunk += Bc.sum_sink;
@@ -1321,7 +1357,11 @@ __device__ void team_partition_seg(const TeamArgs& a, const TeamLds& L, const St
         const uint32_t u = update(nd, bin[r], p);
         if (u) seg_fail(a, L, u);
         const int32_t fz = (int32_t)((w0 >> 8) & 0xFFu) - 1;
-        if (!transform_p((w0 >> 16) & 0xFFu, xf0, p) && fz >= 0) transform(L.nd[fz], e.rp, 2u * (uint32_t)fz, p);
+        if (!transform_p((w0 >> 16) & 0xFFu, xf0, p) && fz >= 0) {
+            RunParams rp = e.rp;
+            rp.seed = L.seed[fz];
+            transform(L.nd[fz], rp, L.purpose[fz], p);
+        }
         p.store(e.cols, (size_t)od.index * C + off);
         const uint32_t dop = w0 >> 24;
         if (dop) {
@@ -1349,6 +1389,8 @@ __device__ void team_transform_seg(const TeamArgs& a, const TeamLds& L, const St
     const uint32_t pos0 = Bc.in_items[Bc.in_off[k] + i] + off0;
     const uint32_t dst = (uint32_t)L.route[L.route_off[k]], dkind = L.kind[dst], purpose = L.purpose[k];
     const gf_node_desc dnd = L.nd[dst];
+    RunParams rp = e.rp;
+    rp.seed = L.seed[k];  // the node's job seed
     uint32_t key[kSegRounds];
 #pragma unroll
     for (uint32_t r = 0; r < kSegRounds; ++r) {
@@ -1359,7 +1401,7 @@ __device__ void team_transform_seg(const TeamArgs& a, const TeamLds& L, const St
             gf_chunk_desc od;
             if (q < pl.R && pop_desc(L, pl, pl.base + q, od)) {
                 P p = R.rec[r];
-                transform(nd, e.rp, purpose, p);
+                transform(nd, rp, purpose, p);
                 p.store(e.cols, (size_t)od.index * C + off);
                 key[r] = next_key(a, dkind, dnd, od, off, p);
             } else {
@@ -1385,6 +1427,8 @@ __device__ void team_source_seg(const TeamArgs& a, const TeamLds& L, const StepB
     const uint32_t dst = (uint32_t)L.route[L.route_off[k]], dkind = L.kind[dst], purpose = L.purpose[k];
     const gf_node_desc dnd = L.nd[dst];
     const unsigned long long id0 = Bc.first_id[k] + off0;
+    RunParams rp = e.rp;
+    rp.seed = L.seed[k];  // the node's job seed
     uint32_t key[kSegRounds];
 #pragma unroll
     for (uint32_t r = 0; r < kSegRounds; ++r) {
@@ -1392,7 +1436,7 @@ __device__ void team_source_seg(const TeamArgs& a, const TeamLds& L, const StepB
         key[r] = ~0u;
         if (it < cnt) {
             P p;
-            source(nd, e.rp, purpose, id0 + it, p);
+            source(nd, rp, purpose, id0 + it, p);
             p.store(e.cols, (size_t)od.index * C + off0 + it);
             key[r] = next_key(a, dkind, dnd, od, off0 + it, p);
         }

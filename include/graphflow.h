@@ -151,6 +151,8 @@ typedef struct gf_flow gf_flow;
 /* ---- context ------------------------------------------------------------------------------ */
 gf_status gf_context_create(int device, gf_context** out);
 gf_status gf_context_destroy(gf_context* ctx);
+/* message of the last failed call that reported through the context (gf_server_create) */
+gf_status gf_context_last_error(gf_context* ctx, char* buffer, uint32_t size);
 const char* gf_status_name(gf_status s);
 uint32_t gf_abi_version(void);
 
@@ -258,20 +260,24 @@ gf_status gf_flow_item_passes(gf_flow* flow, uint64_t* partition_items, uint64_t
  * No counterpart in the reference (its runner executes one flow per run). Jobs are one of three
  * topologies: the particle feedback network (result: digest of the per-id weight bits), the keyed
  * murmur3 hash pipeline with 4096 buckets (result: digest of the per-bucket multisets) and photon
- * transport (result: the 1024-bin absorbed-weight tally). Each slot owns a stream and keeps one
- * resident flow per topology; a job is admitted first come, first served into a free slot when its
- * topology's flow is resident there or fits the free memory budget. Submission, polling and
- * destruction may be called from one host thread while the slots run. */
+ * transport (result: the 1024-bin absorbed-weight tally). A topology's `slots` job slots live in
+ * its device engines, each over copies of the topology's graph (one per slot) whose every step
+ * runs the chunks of all its admitted jobs together: team engines of up to 6 (feedback) or 8
+ * (photon) slots on distinct XCDs, one wide engine for the hash pipeline. One driver thread admits
+ * jobs first come, first served within a topology into a free slot whose engine's unreserved
+ * chunks hold the job's pool bound (live bound + the graph's output bounds), launches the engines
+ * side by side and finishes jobs. Submission, polling and destruction may be called from one host
+ * thread while the engines run. */
 enum { GF_JOB_FEEDBACK = 0, GF_JOB_HASH = 1, GF_JOB_PHOTON = 2 };
 
 typedef struct gf_server_config {
-    uint32_t slots;             /* jobs running side by side (0 -> 8, one per XCD)                  */
-    uint32_t team_size;         /* team engine workgroups per job (0 -> CUs of one XCD minus 4)     */
-    uint64_t memory_budget;     /* device bytes the slots' flows may hold (0 -> 64 GiB)             */
-    uint64_t max_items;         /* largest feedback job (0 -> 1e7; sizes its result buffer)         */
-    uint32_t prewarm;           /* 1: every slot builds its three flows before create returns       */
+    uint32_t slots;             /* job slots per topology (0 -> 16, at most 42)                     */
+    uint32_t team_size;         /* team engine workgroups (0 -> CUs of one XCD minus 4)             */
+    uint64_t memory_budget;     /* device bytes of the engines' pools and result buffers (0 -> 64 GiB) */
+    uint64_t max_items;         /* largest job (0 -> 1e7; sizes the feedback result buffers)        */
+    uint32_t prewarm;           /* unused: the engines are built before create returns              */
     uint32_t reserved;
-    uint64_t topology_bytes[3]; /* out: device bytes a slot holds per topology                      */
+    uint64_t topology_bytes[3]; /* out: device bytes each topology's engine holds                   */
 } gf_server_config;
 
 typedef struct gf_job_desc {
