@@ -407,59 +407,8 @@ def cpu_leg(g, flow, N, C, P, Q, rows, Nc, forms, hash_sets, photon_tallies, pho
     return out
 
 
-def main():
-    ap = argparse.ArgumentParser()
-    ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=3)
-    ap.add_argument("--warmup", type=int, default=1)
-    ap.add_argument("--items", type=float, default=1e8)
-    ap.add_argument("--capacity", type=int, default=8192)
-    ap.add_argument("--live", type=int, default=16384)
-    ap.add_argument("--batch", type=int, default=8)
-    ap.add_argument("--plain-items", type=float, default=2e6)
-    ap.add_argument("--cpu-items", type=float, default=2e7, help="CPU baseline sample (oracle runner schedule)")
-    ap.add_argument("--no-cpu", action="store_true")
-    ap.add_argument("--only-headline", action="store_true", help="the headline runs and roofline only")
-    ap.add_argument("--engine", type=int, default=0, help="device engine: 0 autotuned, 1 wide, 2 micro, 3 team")
-    ap.add_argument("--team", type=int, default=0, help="team engine: workgroups in the team (0 = default)")
-    ap.add_argument("--no-autotune", action="store_true", help="skip the autotuner's probe runs")
-    ap.add_argument("--phases", action="store_true", help="device engines: one more run with the per-phase profile")
-    ap.add_argument("--members", action="store_true", help="team engine: with --phases, per-member phase times")
-    ap.add_argument("--no-fuse", action="store_true", help="run A/B as separate transform batches")
-    ap.add_argument("--photons", type=float, default=1e8, help="photon transport form: photons (0 = skip)")
-    ap.add_argument("--plain-photons", type=float, default=1e6)
-    ap.add_argument("--hash-items", type=float, default=1e8, help="keyed hash pipeline form: keys (0 = skip)")
-    ap.add_argument("--hash-q", default="1,4,8,16", help="keyed hash pipeline form: batch sizes Q, comma separated")
-    ap.add_argument("--hash-lanes", type=int, default=63, help="keyed hash pipeline: inflight batches per node")
-    ap.add_argument("--hash-live", type=int, default=1 << 22, help="keyed hash pipeline: keys in flight (P)")
-    ap.add_argument("--serve-jobs", type=int, default=4096, help="served mixed stream: jobs (0 = skip)")
-    ap.add_argument("--serve-rate", type=float, default=80.0, help="served mixed stream: Poisson arrivals per second")
-    ap.add_argument("--serve-max-items", type=float, default=1e7, help="served mixed stream: N ~ log-uniform[1e3, this]")
-    ap.add_argument("--no-distributions", action="store_true")
-    ap.add_argument("--copy-calib", type=float, default=0.0,
-                    help="after the runs, copy this many MiB device to device once (the known-size kernel that "
-                         "scales rocprofv3 FETCH_SIZE / WRITE_SIZE readings to bytes)")
-    ap.add_argument("--pmc-summary", nargs=3, metavar=("FETCH_CSV", "WRITE_CSV", "OUT"),
-                    help="write the HBM traffic summary of two rocprofv3 --pmc runs and exit")
-    args = ap.parse_args()
-
-    N, C, P, Q = int(args.items), args.capacity, args.live, args.batch
-    fuse = not args.no_fuse
-    # the HBM traffic of a launch depends on the workload, not on the team size
-    wkey = f"particle_feedback N={N:.0e} C={C} P={P} Q={Q} seed=42 fuse={int(fuse)}"
-    if args.pmc_summary:
-        pmc_summary(*args.pmc_summary[:2], wkey, args.pmc_summary[2])
-        return
-
-    if args.gpus != 1:
-        print(json.dumps({"error": "this build runs on one GPU; --gpus above 1 is not supported "
-                                   "(the path would shard by particle-id range, one flow per GPU)"}))
-        sys.exit(2)
-
-    import torch
-    import graphflow_amd as g
-
-    golden = golden_cases()
+def headline(args, g, torch, golden, N, C, P, Q, fuse, wkey):
+    """The headline runs: autotune, warmup, the timed runs, the histogram checks and the roofline."""
     log(f"[bench] batched form N={N} C={C} P={P} Q={Q}")
     flow, src, counts = build_feedback(g, torch, C, P, Q, g.SCHEDULE_DEVICE, fuse=fuse, engine=args.engine,
                                        team=args.team)
@@ -549,7 +498,67 @@ def main():
                 log(f"[bench] member {m:2d}: wait {r[0]:.2f} start {r[1]:.2f} scatter {r[2]:.2f} "
                     f"sched {r[3]:.2f} us, segments {r[4]:.2f} sinks {r[5]:.2f} per step, wave work "
                     + " ".join(f"{x:.2f}" for x in r[6:]))
+    return result, flow, rows
 
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=3)
+    ap.add_argument("--warmup", type=int, default=1)
+    ap.add_argument("--items", type=float, default=1e8)
+    ap.add_argument("--capacity", type=int, default=8192)
+    ap.add_argument("--live", type=int, default=16384)
+    ap.add_argument("--batch", type=int, default=8)
+    ap.add_argument("--plain-items", type=float, default=2e6)
+    ap.add_argument("--cpu-items", type=float, default=2e7, help="CPU baseline sample (oracle runner schedule)")
+    ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--only-headline", action="store_true", help="the headline runs and roofline only")
+    ap.add_argument("--skip-headline", action="store_true",
+                    help="run only the selected forms (for profiling one form; implies --no-cpu)")
+    ap.add_argument("--engine", type=int, default=0, help="device engine: 0 autotuned, 1 wide, 2 micro, 3 team")
+    ap.add_argument("--team", type=int, default=0, help="team engine: workgroups in the team (0 = default)")
+    ap.add_argument("--no-autotune", action="store_true", help="skip the autotuner's probe runs")
+    ap.add_argument("--phases", action="store_true", help="device engines: one more run with the per-phase profile")
+    ap.add_argument("--members", action="store_true", help="team engine: with --phases, per-member phase times")
+    ap.add_argument("--no-fuse", action="store_true", help="run A/B as separate transform batches")
+    ap.add_argument("--photons", type=float, default=1e8, help="photon transport form: photons (0 = skip)")
+    ap.add_argument("--plain-photons", type=float, default=1e6)
+    ap.add_argument("--hash-items", type=float, default=1e8, help="keyed hash pipeline form: keys (0 = skip)")
+    ap.add_argument("--hash-q", default="1,4,8,16", help="keyed hash pipeline form: batch sizes Q, comma separated")
+    ap.add_argument("--hash-lanes", type=int, default=63, help="keyed hash pipeline: inflight batches per node")
+    ap.add_argument("--hash-live", type=int, default=1 << 22, help="keyed hash pipeline: keys in flight (P)")
+    ap.add_argument("--serve-jobs", type=int, default=4096, help="served mixed stream: jobs (0 = skip)")
+    ap.add_argument("--serve-rate", type=float, default=80.0, help="served mixed stream: Poisson arrivals per second")
+    ap.add_argument("--serve-max-items", type=float, default=1e7, help="served mixed stream: N ~ log-uniform[1e3, this]")
+    ap.add_argument("--no-distributions", action="store_true")
+    ap.add_argument("--copy-calib", type=float, default=0.0,
+                    help="after the runs, copy this many MiB device to device once (the known-size kernel that "
+                         "scales rocprofv3 FETCH_SIZE / WRITE_SIZE readings to bytes)")
+    ap.add_argument("--pmc-summary", nargs=3, metavar=("FETCH_CSV", "WRITE_CSV", "OUT"),
+                    help="write the HBM traffic summary of two rocprofv3 --pmc runs and exit")
+    args = ap.parse_args()
+
+    N, C, P, Q = int(args.items), args.capacity, args.live, args.batch
+    fuse = not args.no_fuse
+    # the HBM traffic of a launch depends on the workload, not on the team size
+    wkey = f"particle_feedback N={N:.0e} C={C} P={P} Q={Q} seed=42 fuse={int(fuse)}"
+    if args.pmc_summary:
+        pmc_summary(*args.pmc_summary[:2], wkey, args.pmc_summary[2])
+        return
+
+    if args.gpus != 1:
+        print(json.dumps({"error": "this build runs on one GPU; --gpus above 1 is not supported "
+                                   "(the path would shard by particle-id range, one flow per GPU)"}))
+        sys.exit(2)
+
+    import torch
+    import graphflow_amd as g
+
+    golden = golden_cases()
+    result, flow, rows = {"metric": METRIC, "n_gpus": 1, "config": {}}, None, None
+    if not args.skip_headline:
+        result, flow, rows = headline(args, g, torch, golden, N, C, P, Q, fuse, wkey)
     forms, hash_sets, photon_tally = {}, {}, None
     if not args.only_headline:
         Np = int(args.plain_items)
@@ -576,7 +585,7 @@ def main():
                                                         args.serve_rate)
     result["forms"] = forms
 
-    if not args.no_cpu:
+    if not args.no_cpu and not args.skip_headline:
         result["cpu_baseline"] = cpu_leg(g, flow, N, C, P, Q, rows, int(args.cpu_items), forms, hash_sets,
                                          photon_tally, int(args.photons))
         ck = result["cpu_baseline"]["checks"]

@@ -95,7 +95,8 @@ class TuningC(C.Structure):
                 ("photon_layer_mfp", C.c_double), ("photon_mu_t", C.c_double), ("photon_albedo", C.c_double),
                 ("photon_hg_g", C.c_double), ("photon_roulette_w", C.c_double), ("photon_roulette_p", C.c_double),
                 ("team_xcd", C.c_uint32), ("inflight_batches", C.c_uint32), ("bin_path", C.c_uint32),
-                ("hash_table_entries", C.c_uint32), ("stall_timeout_ms", C.c_uint32), ("fused_batches", C.c_uint32)]
+                ("hash_table_entries", C.c_uint32), ("stall_timeout_ms", C.c_uint32), ("fused_batches", C.c_uint32),
+                ("source_chunks", C.c_uint32)]
 
 
 class Stats(C.Structure):
@@ -523,14 +524,16 @@ class Tuning:
     photon: Optional[dict] = None
     team_xcd: Optional[int] = None  # team engine: run the team on this XCD (None: where it first starts)
     inflight_batches: Optional[int] = None  # batches in flight per node (Tuning::inflightBatches)
-    bin_path: int = 0  # BIN_PATH_AUTO (hash above 1024 bins), BIN_PATH_DENSE, BIN_PATH_HASH
+    bin_path: int = 0  # BIN_PATH_AUTO (hash above 4096 bins), BIN_PATH_DENSE, BIN_PATH_HASH
     hash_table_entries: Optional[int] = None  # hash path: slots of a tile's bin table (None: 2 * tile items)
     stall_timeout_ms: Optional[int] = None  # plain schedule: longest wait for a device stage (Tuning::stallTimeout)
     fused_batches: bool = False  # plain schedule: one launch per partition batch (gf_transform_batch)
+    source_chunks: Optional[int] = None  # device schedule: chunks a source lane admits per step (None: 1)
 
     def validate(self):
         for name in ("batch_chunks", "chunk_capacity", "live_particles", "num_chunks", "tile_items",
-                     "steps_per_launch", "inflight_batches", "hash_table_entries", "stall_timeout_ms"):
+                     "steps_per_launch", "inflight_batches", "hash_table_entries", "stall_timeout_ms",
+                     "source_chunks"):
             v = getattr(self, name)
             if v is not None and v == 0:
                 raise ValueError(f"graphFlow::Tuning: {name} must be non-zero")
@@ -545,7 +548,7 @@ class Tuning:
                     team_xcd=0 if self.team_xcd is None else self.team_xcd + 1,
                     inflight_batches=self.inflight_batches or 0, bin_path=self.bin_path,
                     hash_table_entries=self.hash_table_entries or 0, stall_timeout_ms=self.stall_timeout_ms or 0,
-                    fused_batches=1 if self.fused_batches else 0)
+                    fused_batches=1 if self.fused_batches else 0, source_chunks=self.source_chunks or 0)
         ph = self.photon or {}
         t.photon_layers = ph.get("layers", 10)
         t.photon_tally_bins = ph.get("tally_bins", 1024)

@@ -214,6 +214,10 @@ struct RunParams {
     // (bounds taken with a 1e-12 relative margin, so the search below returns the rank of the
     // full-range search); null: search the whole table
     const uint32_t* zipf_guide;
+    // rank estimate inside a wide guide interval: the partial sums H(m) = sum_{i<=m} i^-s follow
+    // H(m) ~ A + m^(1-s) / (1-s) + m^-s / 2 (Euler-Maclaurin; s = 1: A + ln m + 1 / (2m)), with A
+    // fitted to the table's last entry. Only a starting point: the search verifies it.
+    double zipf_a, zipf_s;
     PhotonParams ph;
 };
 
@@ -383,6 +387,53 @@ __device__ inline void transform(const gf_node_desc& nd, const RunParams& rp, ui
     if (nd.op == GF_XF_PHOTON_STEP) photon_step(rp, purpose, p, (double*)nd.ptr[0]);
 }
 
+// Zipf rank of u: the smallest r with cdf[r] > u * cdf[n - 1] (n - 1 if none), the full-table
+// binary search's answer. The guide interval [lo, hi] holds it; a wide interval (the tail, where
+// a guide slot spans many ranks) is first probed at the analytic estimate with one window of four
+// consecutive entries, which settles the rank in one round trip when the estimate is within one
+// rank, and narrows the interval otherwise.
+__device__ inline uint32_t zipf_rank(const RunParams& rp, double u)
+{
+    const double* cdf = rp.zipf;
+    const double target = u * cdf[rp.zipf_values - 1u];
+    uint32_t lo = 0, hi = rp.zipf_values - 1u;
+    if (rp.zipf_guide) {
+        const uint32_t j = (uint32_t)(u * (double)kZipfGuide);  // u < 1: j < kZipfGuide
+        lo = rp.zipf_guide[2u * j];
+        hi = rp.zipf_guide[2u * j + 1u];
+    }
+    if (hi - lo >= 4u) {
+        // continuous solution m* of H(m) = target (two fixed-point steps); rank floor(m*) (0-based)
+        const double s = rp.zipf_s;
+        double m;
+        if (s == 1.0) {
+            m = exp(target - rp.zipf_a);
+            m = exp(target - rp.zipf_a - 0.5 / m);
+        } else {
+            const double e = 1.0 / (1.0 - s);
+            m = pow((1.0 - s) * (target - rp.zipf_a), e);
+            m = pow((1.0 - s) * (target - rp.zipf_a - 0.5 * pow(m, -s)), e);
+        }
+        // window [g - 1, g + 2] inside [lo, hi]: g in [lo + 1, hi - 2] (a NaN estimate compares false)
+        double gd = floor(m);
+        if (!(gd >= (double)(lo + 1u))) gd = (double)(lo + 1u);
+        if (!(gd <= (double)(hi - 2u))) gd = (double)(hi - 2u);
+        const uint32_t g = (uint32_t)gd;
+        const double c0 = cdf[g - 1u], c1 = cdf[g], c2 = cdf[g + 1u], c3 = cdf[g + 2u];
+        if (c0 > target) hi = g - 1u;  // rank <= g - 1
+        else if (c1 > target) return g;
+        else if (c2 > target) return g + 1u;
+        else if (c3 > target) return g + 2u;
+        else lo = g + 3u;  // rank >= g + 3
+    }
+    while (lo < hi) {
+        uint32_t mid = lo + (hi - lo) / 2u;
+        if (cdf[mid] > target) hi = mid;
+        else lo = mid + 1u;
+    }
+    return lo;
+}
+
 // Source functors.
 __device__ inline void source(const gf_node_desc&, const RunParams& rp, uint32_t purpose, uint64_t id, Particle& p)
 {
@@ -397,19 +448,7 @@ __device__ inline void source(const gf_node_desc&, const RunParams& rp, uint32_t
     uint32_t r[4];
     stream_words(rp.seed, id, 0, purpose, r);
     const double u = uniform_f64(r[0], r[1]);
-    double target = u * rp.zipf[rp.zipf_values - 1u];
-    uint32_t lo = 0, hi = rp.zipf_values - 1u;
-    if (rp.zipf_guide) {
-        const uint32_t j = (uint32_t)(u * (double)kZipfGuide);  // u < 1: j < kZipfGuide
-        lo = rp.zipf_guide[2u * j];
-        hi = rp.zipf_guide[2u * j + 1u];
-    }
-    while (lo < hi) {
-        uint32_t mid = lo + (hi - lo) / 2u;
-        if (rp.zipf[mid] > target) hi = mid;
-        else lo = mid + 1u;
-    }
-    p.key = lo;
+    p.key = zipf_rank(rp, u);
     p.val = id;
 }
 __device__ inline void source(const gf_node_desc&, const RunParams&, uint32_t, uint64_t id, Photon& p)
@@ -704,16 +743,25 @@ __device__ void scatter_tile_hash(const gf_node_desc& nd, const Batch& b, uint32
     }
     __syncthreads();
     const uint32_t* toff = b.counts + (size_t)(t == kNoTile ? 0u : t) * B;
+    // flat positions, then output chunks, then the stores: every round's loads are issued before the
+    // first store (the compiler cannot move a load of the plan past a store to the pool)
+    uint32_t f[R], ci[R];
+#pragma unroll
+    for (int r = 0; r < R; ++r) {
+        const uint32_t k = bin[r] == 0xFFFFFFFFu ? 0u : bin[r];
+        f[r] = toff[k] + (bin_base ? bin_base[k] * C : 0u) + (bin[r] == 0xFFFFFFFFu ? 0u : wcnt[w * E + slot[r]]) + rank[r];
+    }
+#pragma unroll
+    for (int r = 0; r < R; ++r) ci[r] = bin[r] == 0xFFFFFFFFu ? 0u : b.out[f[r] / C].index;
 #pragma unroll
     for (int r = 0; r < R; ++r) {
         if (bin[r] == 0xFFFFFFFFu) continue;
         const uint32_t k = bin[r];
-        const uint32_t f = toff[k] + (bin_base ? bin_base[k] * C : 0u) + wcnt[w * E + slot[r]] + rank[r];
         P p = rec[r];
         const uint32_t u = update(nd, k, p);
         if (u) set_status(status, u);
         if (fuse && fuse[k] >= 0) transform(all[fuse[k] & 0xFFFF], *rp, (uint32_t)fuse[k] >> 16, p);
-        p.store(cols, (size_t)b.out[f / C].index * C + f % C);
+        p.store(cols, (size_t)ci[r] * C + f[r] % C);
     }
     __syncthreads();
 }
@@ -812,16 +860,24 @@ __device__ void scatter_tile(const gf_node_desc& nd, const Batch& b, uint32_t t,
         }
         __syncthreads();
     } else {
+        // flat positions, then output chunks, then the stores (all loads before the first store)
+        uint32_t f[R], ci[R];
+#pragma unroll
+        for (int r = 0; r < R; ++r) {
+            const uint32_t k = bin[r] == 0xFFFFFFFFu ? 0u : bin[r];
+            f[r] = toff[k] + (bin_base ? bin_base[k] * C : 0u) + (uint32_t)wcnt[w * B + k] + rank[r];
+        }
+#pragma unroll
+        for (int r = 0; r < R; ++r) ci[r] = bin[r] == 0xFFFFFFFFu ? 0u : b.out[f[r] / C].index;
 #pragma unroll
         for (int r = 0; r < R; ++r) {
             if (bin[r] == 0xFFFFFFFFu) continue;
             uint32_t k = bin[r];
-            uint32_t f = toff[k] + (bin_base ? bin_base[k] * C : 0u) + wcnt[w * B + k] + rank[r];
             P p = rec[r];
             uint32_t u = update(nd, k, p);
             if (u) set_status(status, u);
             if (fuse && fuse[k] >= 0) transform(all[fuse[k] & 0xFFFF], *rp, (uint32_t)fuse[k] >> 16, p);
-            p.store(cols, (size_t)b.out[f / C].index * C + f % C);
+            p.store(cols, (size_t)ci[r] * C + f[r] % C);
         }
         __syncthreads();
     }
@@ -843,7 +899,8 @@ __device__ void transform_tile(const gf_node_desc& nd, const Batch& b, uint32_t 
     }
 }
 
-// Source of one tile of the admitted chunk (SourceKernel, Execution.hpp:664-692).
+// Source of one tile of the admitted chunks (SourceKernel, Execution.hpp:664-692): item g of the
+// lane's batch (chunks of C items but the last) goes to chunk out[g / C], slot g % C.
 template <class P>
 __device__ void source_tile(const gf_node_desc& nd, const Batch& b, uint32_t t, const Cols& cols, uint32_t C,
                             const RunParams& rp)
@@ -851,11 +908,11 @@ __device__ void source_tile(const gf_node_desc& nd, const Batch& b, uint32_t t, 
     if (t == kNoTile) return;
     uint32_t off = t * b.tile_items;
     uint32_t count = b.particles - off < b.tile_items ? b.particles - off : b.tile_items;
-    size_t base = (size_t)b.out[0].index * C + off;
     for (uint32_t i = ltid(); i < count; i += kThreads) {
         P p;
-        source(nd, rp, b.purpose, b.first_id + off + i, p);
-        p.store(cols, base + i);
+        const uint32_t g = off + i;
+        source(nd, rp, b.purpose, b.first_id + g, p);
+        p.store(cols, (size_t)b.out[g / C].index * C + g % C);
     }
 }
 
@@ -924,17 +981,20 @@ __device__ void sink_tile(const gf_node_desc& nd, const Batch& b, uint32_t t, co
     }
 }
 
-// Sink of one tile by one wave (same results as sink_tile): for sinks that take many small chunks
-// per step, the waves of a workgroup each run a tile, so their latency chains overlap.
-template <class P>
-__device__ void sink_tile_wave(const gf_node_desc& nd, const Batch& b, uint32_t t, const Cols& cols, uint32_t C,
-                               uint32_t* status)
+// Sink of NT consecutive tiles t .. t + nt - 1 (nt <= NT) by one wave (same results as sink_tile):
+// for sinks that take many small chunks per step. The tiles' spans, their first 64 items and the
+// digests' reductions are computed for all NT tiles together, so one wave keeps NT tiles' loads in
+// flight instead of one tile's dependent chain at a time; larger tiles continue item by item.
+template <class P, uint32_t NT>
+__device__ void sink_tiles_wave(const gf_node_desc& nd, const Batch& b, uint32_t t, uint32_t nt, const Cols& cols,
+                                uint32_t C, uint32_t* status)
 {
     const uint32_t lane = lane_id();
-    TileSpan s = tile_span(b, b.in, b.n_in, t, C);
+    TileSpan sp[NT];
+#pragma unroll
+    for (uint32_t q = 0; q < NT; ++q) sp[q] = tile_span(b, b.in, b.n_in, q < nt ? t + q : kNoTile, C);
     if constexpr (std::is_same_v<P, Particle>) {
-        for (uint32_t i = lane; i < s.count; i += 64u) {
-            Particle p = Particle::load(cols, s.base + i);
+        auto item = [&](const Particle& p) {
             if (nd.op == GF_SNK_HIST) {
                 uint32_t v = nd.u[0] == 0u ? __float_as_uint(p.w) : p.gen;
                 uint32_t rel = v - nd.u[1];
@@ -944,35 +1004,62 @@ __device__ void sink_tile_wave(const gf_node_desc& nd, const Batch& b, uint32_t 
                 if ((uint64_t)p.id < nd.size[0]) ((uint32_t*)nd.ptr[0])[p.id] = __float_as_uint(p.w);
                 else set_status(status, GF_INVALID);
             }
-        }
+        };
+        Particle pk[NT];
+#pragma unroll
+        for (uint32_t q = 0; q < NT; ++q)
+            if (lane < sp[q].count) pk[q] = Particle::load(cols, sp[q].base + lane);
+#pragma unroll
+        for (uint32_t q = 0; q < NT; ++q)
+            if (lane < sp[q].count) item(pk[q]);
+#pragma unroll
+        for (uint32_t q = 0; q < NT; ++q)
+            for (uint32_t i = lane + 64u; i < sp[q].count; i += 64u) item(Particle::load(cols, sp[q].base + i));
     } else if constexpr (std::is_same_v<P, Keyed>) {
         if (nd.op != GF_SNK_CHUNK_DIGEST) return;
-        const gf_chunk_desc d = s.count ? b.in[s.chunk] : gf_chunk_desc{0, 0, 0, 0};
-        uint64_t acc = 0, msum = 0;
-        for (uint32_t i = lane; i < s.count; i += 64u) {
-            Keyed p = Keyed::load(cols, s.base + i);
-            uint64_t h = keyed_hash(p);
-            acc += mix64(h ^ ((uint64_t)(s.offset + i + 1u) * 0xD6E8FEB86659FD93ull));
-            msum += h;
+        uint64_t acc[NT], msum[NT];
+        Keyed pk[NT];
+#pragma unroll
+        for (uint32_t q = 0; q < NT; ++q) {
+            acc[q] = msum[q] = 0ull;
+            if (lane < sp[q].count) pk[q] = Keyed::load(cols, sp[q].base + lane);
         }
-        for (int o = 32; o > 0; o >>= 1) {
-            acc += __shfl_down(acc, o);
-            msum += __shfl_down(msum, o);
-        }
-        if (lane == 0 && s.count) {
-            uint64_t entry = b.digest_base + s.chunk;
-            if (nd.ptr[0] && entry < nd.size[0]) {
-                uint32_t* log = (uint32_t*)nd.ptr[0] + 4 * entry;
-                if (s.offset == 0) {
-                    log[0] = d.bin;
-                    log[1] = d.fill;
-                }
-                atomicAdd((unsigned long long*)(log + 2), (unsigned long long)acc);
+        auto item = [&](uint32_t q, const Keyed& p, uint32_t i) {
+            const uint64_t h = keyed_hash(p);
+            acc[q] += mix64(h ^ ((uint64_t)(sp[q].offset + i + 1u) * 0xD6E8FEB86659FD93ull));
+            msum[q] += h;
+        };
+#pragma unroll
+        for (uint32_t q = 0; q < NT; ++q)
+            if (lane < sp[q].count) item(q, pk[q], lane);
+#pragma unroll
+        for (uint32_t q = 0; q < NT; ++q)
+            for (uint32_t i = lane + 64u; i < sp[q].count; i += 64u) item(q, Keyed::load(cols, sp[q].base + i), i);
+#pragma unroll
+        for (uint32_t q = 0; q < NT; ++q)
+            for (int o = 32; o > 0; o >>= 1) {
+                acc[q] += __shfl_down(acc[q], o);
+                msum[q] += __shfl_down(msum[q], o);
             }
-            if (nd.ptr[1] && d.bin < nd.size[1]) {
-                unsigned long long* ms = (unsigned long long*)nd.ptr[1] + 2 * d.bin;
-                atomicAdd(ms, (unsigned long long)msum);
-                atomicAdd(ms + 1, (unsigned long long)s.count);
+        if (lane == 0) {
+#pragma unroll
+            for (uint32_t q = 0; q < NT; ++q) {
+                if (!sp[q].count) continue;
+                const gf_chunk_desc d = b.in[sp[q].chunk];
+                const uint64_t entry = b.digest_base + sp[q].chunk;
+                if (nd.ptr[0] && entry < nd.size[0]) {
+                    uint32_t* log = (uint32_t*)nd.ptr[0] + 4 * entry;
+                    if (sp[q].offset == 0) {
+                        log[0] = d.bin;
+                        log[1] = d.fill;
+                    }
+                    atomicAdd((unsigned long long*)(log + 2), (unsigned long long)acc[q]);
+                }
+                if (nd.ptr[1] && d.bin < nd.size[1]) {
+                    unsigned long long* ms = (unsigned long long*)nd.ptr[1] + 2 * d.bin;
+                    atomicAdd(ms, (unsigned long long)msum[q]);
+                    atomicAdd(ms + 1, (unsigned long long)sp[q].count);
+                }
             }
         }
     }

@@ -68,6 +68,7 @@ constexpr uint32_t kMaxJobs = 64;
 
 struct Engine {
     uint32_t n_nodes, C, P, Q, S, ring_cap, trace_on, n_jobs;  // n_jobs: slots of a served engine, else 0
+    uint32_t src_chunks, pad0;  // chunks a source lane admits per step
     uint64_t N;
     RunParams rp;
     Cols cols;
@@ -95,7 +96,11 @@ struct Engine {
 };
 
 enum { PROF_COUNT = 0, PROF_PLAN = 1, PROF_SOURCE = 2, PROF_PART = 3, PROF_XF = 4, PROF_SINK = 5, PROF_RETIRE = 6,
-       PROF_SELECT = 7, PROF_STEPS = 8, PROF_WORDS = 32 };
+       PROF_SELECT = 7, PROF_STEPS = 8,
+       // wide engine scatter kernel: workgroup wall-clock ticks and tiles per KIND_*, then the
+       // one-wave sink tiles (ticks summed over waves)
+       PROF_KB_TICKS = 16, PROF_KB_TILES = 22, PROF_KB_WAVE_TICKS = 28, PROF_KB_WAVE_TILES = 29,
+       PROF_WORDS = 32 };
 
 enum { CTL_STATUS = 0, CTL_DONE = 1, CTL_TOP = 2, CTL_TICKET_A = 3, CTL_TICKET_B = 4, CTL_HANDOFF = 5,
        CTL_TICKET_C = 6, CTL_LIST_DONE = 7, CTL_WORDS = 8 };
@@ -370,7 +375,7 @@ __device__ inline void engine_plan(const Engine& e, PlanLds& L)
             plan_partition_counts(e, k, L);
         } else if (tid == 0) {
             if (kind == KIND_SOURCE) {
-                L.R[k] = 1u;
+                L.R[k] = (L.st[k].particles + C - 1u) / C;  // the admitted chunks
             } else if (kind == KIND_SINK) {
                 L.R[k] = 0u;
                 L.st[k].digest_base = L.digest[k];
@@ -431,12 +436,8 @@ __device__ inline void engine_plan(const Engine& e, PlanLds& L)
                 gf_chunk_desc d;
                 d.index = (uint32_t)ent;
                 d.generation = (uint32_t)(ent >> 32);
-                if (kind == KIND_SOURCE) {
-                    d.fill = L.st[k].particles;
-                } else {
-                    const uint32_t rem = L.st[k].particles - r * C;
-                    d.fill = rem < C ? rem : C;
-                }
+                const uint32_t rem = L.st[k].particles - r * C;  // source and transform: C, ..., C, rest
+                d.fill = rem < C ? rem : C;
                 d.bin = 0u;
                 out[r] = d;
             }
@@ -794,20 +795,23 @@ __device__ inline void engine_select(const Engine& e, SelLds& L, bool lists = tr
             unsigned long long gen = L.jgen[jb];
             const uint32_t T = e.arr[tid].tile_items;
             for (uint32_t j = 0; j < L.lanes[tid]; ++j) {
-                const unsigned long long live = gen - sunk, remaining = N - gen;
-                unsigned long long min_fill = C < Pj ? C : Pj;
-                if (remaining < min_fill) min_fill = remaining;
-                uint32_t fill = 0u;
-                if (gen < N && live < Pj && (Pj - live) >= min_fill) {
+                // up to src_chunks chunks, each by the admission rule; only the last can be partial,
+                // so the lane's items are a dense concatenation of its chunks
+                const unsigned long long first = gen;
+                for (uint32_t c = 0; c < e.src_chunks; ++c) {
+                    const unsigned long long live = gen - sunk, remaining = N - gen;
+                    unsigned long long min_fill = C < Pj ? C : Pj;
+                    if (remaining < min_fill) min_fill = remaining;
+                    if (!(gen < N && live < Pj && (Pj - live) >= min_fill)) break;
                     unsigned long long f = remaining;
                     if (f > Pj - live) f = Pj - live;
                     if (f > C) f = C;
-                    fill = (uint32_t)f;
+                    gen += f;
                 }
+                const uint32_t fill = (uint32_t)(gen - first);
                 L.dfill[tid + j] = fill;
                 L.dtiles[tid + j] = (fill + T - 1u) / T;
-                L.dfirst[tid + j] = gen;
-                gen += fill;
+                L.dfirst[tid + j] = first;
             }
         }
         __syncthreads();
@@ -815,7 +819,7 @@ __device__ inline void engine_select(const Engine& e, SelLds& L, bool lists = tr
         if (tid < n) {
             const uint32_t kind = L.kind[tid];
             if (kind == KIND_SOURCE) {
-                dem[0] = L.dfill[tid] ? 1ull : 0ull;
+                dem[0] = (L.dfill[tid] + C - 1u) / C;  // chunks admitted
             } else if (kind != KIND_SINK) {
                 const uint32_t l = L.ring_of[tid], w = L.wanted[l], i = tid - l;
                 const uint32_t at = i * Q < w ? i * Q : w, m = w - at < Q ? w - at : Q;
@@ -884,20 +888,26 @@ __device__ inline void engine_select(const Engine& e, SelLds& L, bool lists = tr
             s.n_tiles = 0u;
             if (kind == KIND_SOURCE) {
                 const uint32_t jb = L.job[k];
-                const unsigned long long gen = L.jgen[jb], N = L.jN[jb], Pj = L.jP[jb];
-                unsigned long long live = gen - L.jsunk[jb], remaining = N - gen;
-                unsigned long long min_fill = C < Pj ? C : Pj;
-                if (remaining < min_fill) min_fill = remaining;
-                if (gen < N && live < Pj && (Pj - live) >= min_fill && L.top - reserved >= 1u) {
+                const unsigned long long first = L.jgen[jb], N = L.jN[jb], Pj = L.jP[jb];
+                unsigned long long gen = first;
+                for (uint32_t c = 0; c < e.src_chunks; ++c) {
+                    const unsigned long long live = gen - L.jsunk[jb], remaining = N - gen;
+                    unsigned long long min_fill = C < Pj ? C : Pj;
+                    if (remaining < min_fill) min_fill = remaining;
+                    if (!(gen < N && live < Pj && (Pj - live) >= min_fill && L.top - reserved >= 1u)) break;
                     unsigned long long fill = remaining;
                     if (fill > Pj - live) fill = Pj - live;
                     if (fill > C) fill = C;
-                    s.particles = (uint32_t)fill;
-                    s.first_id = gen;
-                    s.n_tiles = (uint32_t)((fill + e.arr[k].tile_items - 1u) / e.arr[k].tile_items);
-                    L.jgen[jb] = gen + fill;
-                    L.generated += fill;
+                    gen += fill;
                     reserved += 1u;
+                }
+                if (gen > first) {
+                    const uint32_t fill = (uint32_t)(gen - first);
+                    s.particles = fill;
+                    s.first_id = first;
+                    s.n_tiles = (fill + e.arr[k].tile_items - 1u) / e.arr[k].tile_items;
+                    L.jgen[jb] = gen;
+                    L.generated += fill;
                     s.active = 1u;
                 }
             } else if (kind == KIND_SINK) {
@@ -1192,7 +1202,7 @@ __device__ inline void engine_plan_grid(const Engine& e, GridPlanLds& L)
                 L.fail = 1u;
             }
         } else if (kind == KIND_SOURCE) {
-            R = 1u;
+            R = (s.particles + C - 1u) / C;  // the admitted chunks
         } else if (kind == KIND_TRANSFORM) {
             R = (s.particles + C - 1u) / C;
             if (tid == 0 && (s.particles == 0u || R > a.reserve_bound)) {
@@ -1250,12 +1260,8 @@ __device__ inline void engine_plan_grid(const Engine& e, GridPlanLds& L)
                 gf_chunk_desc d;
                 d.index = (uint32_t)ent;
                 d.generation = (uint32_t)(ent >> 32);
-                if (kind == KIND_SOURCE) {
-                    d.fill = s.particles;
-                } else {
-                    const uint32_t rem = s.particles - r * C;
-                    d.fill = rem < C ? rem : C;
-                }
+                const uint32_t rem = s.particles - r * C;  // source and transform: C, ..., C, rest
+                d.fill = rem < C ? rem : C;
                 d.bin = 0u;
                 out[r] = d;
             }

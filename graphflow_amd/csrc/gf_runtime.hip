@@ -276,7 +276,7 @@ __global__ __launch_bounds__(1024) void k_engine_trace(Engine e)
 }
 
 // Tiles round-robin over the workgroups; the tiles of the sinks flagged NODE_WAVE_SINK (sinks fed by
-// wide partitions, whose chunks are small) round-robin over the waves instead.
+// wide partitions, whose chunks are small) round-robin over the waves instead, four at a time.
 template <class P, int R>
 __global__ __launch_bounds__(kThreads) void k_engine_kb(Engine e)
 {
@@ -306,6 +306,18 @@ __global__ __launch_bounds__(kThreads) void k_engine_kb(Engine e)
     if (tid == 0) pre[n] = t_part + t_other;
     if (engine_stopped(e, &flag)) return;  // its barrier also publishes `pre`, `order` and `wsink`
     const uint32_t total = pre[n], G = gridDim.x;
+    // optional profile (use_graph = 3): wall-clock ticks and tiles per tile kind, summed over workgroups
+    unsigned long long tprof = e.prof ? wall_clock64() : 0ull;
+    auto prof_tile = [&](uint32_t kind) {
+        if (!e.prof) return;
+        __syncthreads();
+        if (tid == 0) {
+            const unsigned long long now = wall_clock64();
+            atomicAdd(&e.prof[PROF_KB_TICKS + kind], now - tprof);
+            atomicAdd(&e.prof[PROF_KB_TILES + kind], 1ull);
+            tprof = now;
+        }
+    };
     for (uint32_t t = blockIdx.x; t < total; t += G) {
         const uint32_t i = lane_of(pre, n, t), k = order[i];
         if (wsink[k]) {
@@ -337,16 +349,25 @@ __global__ __launch_bounds__(kThreads) void k_engine_kb(Engine e)
         } else {
             transform_tile<P, R>(nd, b, lt, e.cols, e.C, rp);
         }
+        prof_tile(kind);
     }
     const uint32_t w = threadIdx.x >> 6, W = G * (uint32_t)kWaves;
+    unsigned long long wprof = e.prof ? wall_clock64() : 0ull, wtiles = 0ull;
     for (uint32_t i = n_part; i < n; ++i) {
         const uint32_t k = order[i];
         if (!wsink[k]) continue;
         const uint32_t t0 = pre[i], t1 = pre[i + 1u];
         const gf_node_desc& nd = e.nodes[k];
         const Batch b = make_batch(e, k, e.st[k]);
-        for (uint32_t t = t0 + blockIdx.x * (uint32_t)kWaves + w; t < t1; t += W)
-            sink_tile_wave<P>(nd, b, t - t0, e.cols, e.C, &e.ctl[CTL_STATUS]);
+        // each wave takes four consecutive tiles at a time
+        for (uint32_t t = t0 + (blockIdx.x * (uint32_t)kWaves + w) * 4u; t < t1; t += 4u * W) {
+            sink_tiles_wave<P, 4>(nd, b, t - t0, t1 - t < 4u ? t1 - t : 4u, e.cols, e.C, &e.ctl[CTL_STATUS]);
+            wtiles += t1 - t < 4u ? t1 - t : 4u;
+        }
+    }
+    if (e.prof && lane_id() == 0u && wtiles) {
+        atomicAdd(&e.prof[PROF_KB_WAVE_TICKS], wall_clock64() - wprof);
+        atomicAdd(&e.prof[PROF_KB_WAVE_TILES], wtiles);
     }
 }
 
@@ -746,6 +767,15 @@ std::vector<double> zipf_table(uint32_t values, double s)
     return t;
 }
 
+// Constant A of H(m) ~ A + m^(1-s) / (1-s) + m^-s / 2 (s = 1: A + ln m + 1 / (2m)) fitted to the
+// table's last entry, for the device's rank estimate.
+double zipf_fit_a(const std::vector<double>& cdf, double s)
+{
+    const double n = (double)cdf.size();
+    if (s == 1.0) return cdf.back() - std::log(n) - 0.5 / n;
+    return cdf.back() - std::pow(n, 1.0 - s) / (1.0 - s) - 0.5 * std::pow(n, -s);
+}
+
 }  // namespace
 
 // One fused partition batch (gf_batch.hpp) on `st`: one workgroup of as many 256-thread tile groups
@@ -914,6 +944,8 @@ struct gf_stage {
             rp.zipf = d_zipf;
             rp.zipf_values = nd.u[0];
             rp.zipf_guide = d_guide;
+            rp.zipf_s = (double)nd.f[0];
+            rp.zipf_a = zipf_fit_a(t, rp.zipf_s);
         }
     }
 
@@ -1728,7 +1760,8 @@ std::string shape_key(const gf_flow& f, const RunShape& s, uint64_t N)
                     std::to_string(f.tuning.max_workgroups) + "/" + std::to_string(f.tuning.device_engine) + "/" +
                     std::to_string(f.tuning.use_graph) + "/" + std::to_string(f.tuning.team_xcd) + "/" +
                     std::to_string(f.tuning.inflight_batches) + "/" + std::to_string(f.tuning.bin_path) + "/" +
-                    std::to_string(f.tuning.hash_table_entries) + "/" + std::to_string(f.slots);
+                    std::to_string(f.tuning.hash_table_entries) + "/" + std::to_string(f.slots) + "/" +
+                    std::to_string(f.tuning.source_chunks);
     {
         const PhotonParams ph = photon_params_of(f.tuning);
         char buf[160];
@@ -1788,6 +1821,7 @@ void build_engine(gf_flow& f, uint64_t N, const RunShape& sh, uint32_t src)
     e.rp.seed = t.seed;
     e.rp.C = sh.C;
     e.rp.ph = photon_params_of(t);
+    e.src_chunks = t.source_chunks ? t.source_chunks : 1u;
     std::vector<gf_node_desc> real(n), nds(nv);
     std::vector<int32_t> routes, fuse;
     std::vector<uint32_t> ring_of(nv);
@@ -1842,7 +1876,7 @@ void build_engine(gf_flow& f, uint64_t N, const RunShape& sh, uint32_t src)
             std::memset(&a, 0, sizeof(a));
             a.kind = nd.role == GF_ROLE_SOURCE ? KIND_SOURCE : nd.role == GF_ROLE_SINK ? KIND_SINK : part ? KIND_PARTITION : KIND_TRANSFORM;
             a.cap_in = nd.role == GF_ROLE_SOURCE ? 1u : nd.role == GF_ROLE_SINK ? sh.S : sh.Q;
-            a.cap_out = nd.role == GF_ROLE_SOURCE ? 1u : nd.role == GF_ROLE_SINK ? 1u : reserve_bound_of(nd, sh.C, sh.Q);
+            a.cap_out = nd.role == GF_ROLE_SOURCE ? e.src_chunks : nd.role == GF_ROLE_SINK ? 1u : reserve_bound_of(nd, sh.C, sh.Q);
             a.reserve_bound = a.cap_out;
             a.real = k;
             a.lanes = j == 0 ? lanes_of[k] : 0u;
@@ -1850,7 +1884,8 @@ void build_engine(gf_flow& f, uint64_t N, const RunShape& sh, uint32_t src)
             // a source's tiles hold one item per thread: each item's generation (Philox, a Zipf
             // search) is a chain of dependent steps, so the chunk is spread over more workgroups
             const uint32_t ti = nd.role == GF_ROLE_SOURCE ? (uint32_t)kThreads : sh.tile_items;
-            uint32_t cap_tiles = (uint32_t)(ceil_div(sh.C, ti) * (nd.role == GF_ROLE_SINK ? 1u : a.cap_in));
+            uint32_t cap_tiles = (uint32_t)(nd.role == GF_ROLE_SOURCE ? ceil_div((uint64_t)sh.C * e.src_chunks, ti)
+                                                                        : ceil_div(sh.C, ti) * (nd.role == GF_ROLE_SINK ? 1u : a.cap_in));
             a.in = m.d<gf_chunk_desc>(a.cap_in);
             a.tile_prefix = m.d<uint32_t>(a.cap_in + 1u);
             a.item_prefix = m.d<uint32_t>(a.cap_in + 1u);
@@ -1890,6 +1925,8 @@ void build_engine(gf_flow& f, uint64_t N, const RunShape& sh, uint32_t src)
             e.rp.zipf = dz;
             e.rp.zipf_values = nd.u[0];
             e.rp.zipf_guide = dg;
+            e.rp.zipf_s = (double)nd.f[0];
+            e.rp.zipf_a = zipf_fit_a(z, e.rp.zipf_s);
         }
     }
     if (lds_b > 160u * 1024u || lds_a > 160u * 1024u)
@@ -1957,8 +1994,8 @@ void build_engine(gf_flow& f, uint64_t N, const RunShape& sh, uint32_t src)
     // team engine: every partition narrow enough for a per-segment count row in LDS; its replicated
     // scheduler runs one lane per node
     uint32_t bb = 1;
-    bool team_ok = D == 1u && f.slots <= kTeamJobs && n <= kTeamNodes && routes.size() <= kTeamRoutes &&
-                   ceil_div(sh.C, kSeg) <= (1u << 17);
+    bool team_ok = D == 1u && e.src_chunks == 1u && f.slots <= kTeamJobs && n <= kTeamNodes &&
+                   routes.size() <= kTeamRoutes && ceil_div(sh.C, kSeg) <= (1u << 17);
     for (uint32_t k = 0; k < n; ++k)
         if (is_partition(real[k]) && real[k].role == GF_ROLE_TRANSFORM) bb = std::max(bb, real[k].bins + 1u);
     if (bb > kTeamBins) team_ok = false;
@@ -2712,6 +2749,9 @@ gf_status gf_flow_configure(gf_flow* flow, const gf_tuning* t)
         if (t->inflight_batches > kMaxNodes) fail(GF_BAD_ARGUMENT, "graphFlow::Tuning: inflightBatches exceeds the engine's lanes");
         if (t->inflight_batches > 1u && t->schedule == GF_SCHEDULE_DEVICE && t->device_engine == 3u)
             fail(GF_BAD_ARGUMENT, "graphFlow::Tuning: the team engine runs one batch per node and step (inflightBatches 1)");
+        if (t->source_chunks > 64u) fail(GF_BAD_ARGUMENT, "graphFlow::Tuning: sourceChunks is at most 64");
+        if (t->source_chunks > 1u && t->schedule == GF_SCHEDULE_DEVICE && t->device_engine == 3u)
+            fail(GF_BAD_ARGUMENT, "graphFlow::Tuning: the team engine admits one source chunk per step (sourceChunks 1)");
         flow->tuning = *t;
     });
 }

@@ -122,13 +122,16 @@ typedef struct gf_tuning {
     uint32_t inflight_batches; /* Tuning::inflightBatches: batches in flight per source and transform node
                                   (0 -> 1); plain schedule: lanes of the host runner; device schedule: a
                                   node runs up to this many batches per step (wide and micro engines) */
-    uint32_t bin_path;       /* Tuning::binPath: GF_BIN_PATH_* (hash above 1024 bins when automatic)      */
+    uint32_t bin_path;       /* Tuning::binPath: GF_BIN_PATH_* (hash above 4096 bins when automatic)      */
     uint32_t hash_table_entries; /* hash path: slots of a tile's bin table (0 -> nextPow2(2 * tile items), which
                                     never fills); a tile with more distinct bins fails its batch (GF_INVALID) */
     uint32_t stall_timeout_ms; /* Tuning::stallTimeout: plain schedule, longest wait for a device stage
                                   before the run fails with GF_STALL (0 -> no limit)                       */
     uint32_t fused_batches;  /* plain schedule: each partition batch as one launch (gf_transform_batch)
                                 instead of classify -> host plan -> scatter                                */
+    uint32_t source_chunks;  /* device schedule (wide and micro engines): chunks a source lane admits per
+                                step, each by the admission rule with the earlier ones counted (0 -> 1,
+                                at most 64); the plain schedule admits one chunk per dispatch            */
 } gf_tuning;
 
 enum { GF_BIN_PATH_AUTO = 0, GF_BIN_PATH_DENSE = 1, GF_BIN_PATH_HASH = 2 };

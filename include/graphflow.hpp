@@ -155,7 +155,7 @@ inline Op PhotonDone() { return op(GF_ROLE_SINK, GF_SNK_PHOTON_TALLY, 0); }
 
 // ---- tuning and stats (flow/Tuning.hpp:27-145, sched/GraphRunner.hpp:154-210) --------------------
 enum class Schedule : std::uint32_t { plain = GF_SCHEDULE_HOST, device = GF_SCHEDULE_DEVICE };
-// BinPath (core/BinPath.hpp): dense LDS counters or per-tile hash tables; automatic = hash above 1024 bins
+// BinPath (core/BinPath.hpp): dense LDS counters or per-tile hash tables; automatic = hash above 4096 bins
 enum class BinPath : std::uint32_t { automatic = GF_BIN_PATH_AUTO, dense = GF_BIN_PATH_DENSE, hash = GF_BIN_PATH_HASH };
 
 struct Tuning {
@@ -166,6 +166,7 @@ struct Tuning {
     std::optional<std::uint32_t> hashTableEntries;          // hash path: slots of a tile's bin table
     std::optional<std::chrono::milliseconds> stallTimeout;  // Tuning::stallTimeout
     bool fusedBatches = false;                              // plain schedule: one launch per partition batch
+    std::optional<std::uint32_t> sourceChunks;              // device schedule: chunks a source lane admits per step
     Schedule schedule = Schedule::device;
     bool trace = false;
     bool fuseTransforms = false;
@@ -194,6 +195,7 @@ struct Tuning {
         t.hash_table_entries = hashTableEntries.value_or(0);
         t.stall_timeout_ms = stallTimeout ? (std::uint32_t)stallTimeout->count() : 0u;
         t.fused_batches = fusedBatches ? 1u : 0u;
+        t.source_chunks = sourceChunks.value_or(0);
         t.photon_layers = 10;
         t.photon_tally_bins = 1024;
         t.photon_layer_mfp = 1.0;

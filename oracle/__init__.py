@@ -40,7 +40,8 @@ class RunCfg(C.Structure):
                 ("tally_bins", C.c_uint32), ("layer_mfp", C.c_double), ("mu_t", C.c_double),
                 ("albedo", C.c_double), ("hg_g", C.c_double), ("roulette_w", C.c_double),
                 ("roulette_p", C.c_double), ("inflight", C.c_uint32), ("bin_path", C.c_uint32),
-                ("hash_tile", C.c_uint32), ("hash_entries", C.c_uint32), ("fused_stages", C.c_uint32)]
+                ("hash_tile", C.c_uint32), ("hash_entries", C.c_uint32), ("fused_stages", C.c_uint32),
+                ("source_chunks", C.c_uint32)]
 
 
 class RunOut(C.Structure):
@@ -192,12 +193,12 @@ def make_nodes(graph):
 
 def run(graph, *, seed, items, capacity, live=0, batch=1, num_chunks=0, mode=MODE_RUNNER, threads=1, fuse=False,
         trace_cap=0, hist_size=0, record=False, digest_cap=0, multiset_bins=0, lut=None, photon=None,
-        inflight=1, bin_path=0, hash_tile=0, hash_entries=0, fused_stages=False):
+        inflight=1, bin_path=0, hash_tile=0, hash_entries=0, fused_stages=False, source_chunks=1):
     nodes, routes = make_nodes(graph)
     cfg = RunCfg(seed=seed, items=items, capacity=capacity, live=live, batch=batch,
                  num_chunks=num_chunks, mode=mode, threads=threads, fuse=1 if fuse else 0,
                  inflight=inflight, bin_path=bin_path, hash_tile=hash_tile, hash_entries=hash_entries,
-                 fused_stages=1 if fused_stages else 0)
+                 fused_stages=1 if fused_stages else 0, source_chunks=source_chunks)
     if photon:
         for k, v in photon.items():
             setattr(cfg, k, v)

@@ -769,11 +769,11 @@ static void sink_batch(const or_ctx* x, const or_node_desc* nd, const or_pool* p
     }
 }
 
-static void source_batch(const or_ctx* x, const or_node_desc* nd, uint32_t purpose, or_pool* p, batch* b,
+static void source_batch(const or_ctx* x, const or_node_desc* nd, uint32_t purpose, or_pool* p, const desc* chunk,
                          uint64_t first_id)
 {
-    or_item* dst = p->slots + (size_t)b->out[0].index * p->C;
-    const int64_t n = b->out[0].fill;
+    or_item* dst = p->slots + (size_t)chunk->index * p->C;
+    const int64_t n = chunk->fill;
 #pragma omp parallel for schedule(static) num_threads(batch_threads(x, (uint64_t)n))
     for (int64_t i = 0; i < n; ++i) or_source_item(x, nd, purpose, first_id + (uint64_t)i, &dst[i]);
 }
@@ -889,13 +889,15 @@ int or_run(const or_node_desc* nodes, uint32_t n_nodes, const int32_t* routes, c
     fifo* ready = (fifo*)calloc(n_nodes, sizeof(fifo));
     /* lane j of node k is lane[k * D + j]; every lane owns its batch, leases and credit */
     batch* lane = (batch*)calloc((size_t)n_nodes * D, sizeof(batch));
+    /* step mode: a source lane admits up to SC chunks per step (runner mode: one per dispatch) */
+    const uint32_t SC = cfg->mode == OR_MODE_STEP && cfg->source_chunks ? cfg->source_chunks : 1u;
     uint64_t item_cap = (uint64_t)Q * C;
     for (uint32_t k = 0; k < n_nodes; ++k)
         for (uint32_t j = 0; j < D; ++j) {
             batch* b = &lane[k * D + j];
             uint32_t cap_in = nodes[k].role == OR_ROLE_SINK ? S : Q;
             b->in = (desc*)malloc(sizeof(desc) * (cap_in + 1u));
-            uint32_t cap_out = nodes[k].role == OR_ROLE_SOURCE ? 1u : reserve_bound_of(&nodes[k], C, Q) + 1u;
+            uint32_t cap_out = nodes[k].role == OR_ROLE_SOURCE ? SC : reserve_bound_of(&nodes[k], C, Q) + 1u;
             b->out = (desc*)malloc(sizeof(desc) * cap_out);
             b->counts = (uint32_t*)malloc(sizeof(uint32_t) * (nodes[k].bins + 1u));
             if (nodes[k].role == OR_ROLE_TRANSFORM) {
@@ -985,7 +987,7 @@ int or_run(const or_node_desc* nodes, uint32_t n_nodes, const int32_t* routes, c
                     b->out[0].index = idx; b->out[0].gen = pool.gen[idx];
                     b->out[0].fill = (uint32_t)fill; b->out[0].bin = 0;
                     b->particles = fill;
-                    source_batch(&x, &nodes[src], 2u * (uint32_t)src, &pool, b, generated);
+                    source_batch(&x, &nodes[src], 2u * (uint32_t)src, &pool, &b->out[0], generated);
                     trace_batch(out, (uint32_t)src, iter, b, 0);
                     generated += fill;
                     b->armed = 1;
@@ -1088,21 +1090,31 @@ int or_run(const or_node_desc* nodes, uint32_t n_nodes, const int32_t* routes, c
                     b->n_out = 0;
                     if (stop) continue;
                     if (nd->role == OR_ROLE_SOURCE) {
-                        uint64_t live = generated - sunk, remaining = N - generated;
-                        uint64_t min_fill = C < P ? C : P;
-                        if (remaining < min_fill) min_fill = remaining;
-                        if (generated < N && live < P && (P - live) >= min_fill && pool.top - res >= 1u) {
+                        /* up to SC chunks, each admitted by GraphRunner's rule with the earlier
+                         * admissions counted; a refusal holds for the rest of the step, so the
+                         * node's later lanes stop too. Only a batch's last chunk can be partial
+                         * (a fill below C needs the tail of N or a live bound below C, and either
+                         * refuses the next chunk), so its items are a dense concatenation. */
+                        b->particles = 0;
+                        b->first_id = generated;
+                        for (uint32_t c = 0; c < SC; ++c) {
+                            uint64_t live = generated - sunk, remaining = N - generated;
+                            uint64_t min_fill = C < P ? C : P;
+                            if (remaining < min_fill) min_fill = remaining;
+                            if (!(generated < N && live < P && (P - live) >= min_fill && pool.top - res >= 1u)) {
+                                stop = 1;
+                                break;
+                            }
                             uint64_t fill = remaining;
                             if (fill > P - live) fill = P - live;
                             if (fill > C) fill = C;
-                            b->particles = fill;
-                            b->first_id = generated;
+                            b->out[c].fill = (uint32_t)fill;
+                            b->n_out = c + 1u;
+                            b->particles += fill;
                             generated += fill;
                             res += 1;
-                            b->armed = 1;
-                        } else {
-                            stop = 1;
                         }
+                        b->armed = b->n_out != 0;
                     } else if (nd->role == OR_ROLE_TRANSFORM) {
                         uint64_t wanted = fifo_size(&ready[k]) < Q ? fifo_size(&ready[k]) : Q, parts = 0;
                         for (uint64_t i = 0; i < wanted; ++i) parts += fifo_at(&ready[k], i).fill;
@@ -1151,12 +1163,14 @@ int or_run(const or_node_desc* nodes, uint32_t n_nodes, const int32_t* routes, c
                     if (!b->armed) continue;
                     uint32_t purpose = 2u * k;
                     if (nd->role == OR_ROLE_SOURCE) {
-                        uint32_t idx = 0;
-                        pool_acquire(&pool, &idx);
-                        b->n_out = 1;
-                        b->out[0].index = idx; b->out[0].gen = pool.gen[idx];
-                        b->out[0].fill = (uint32_t)b->particles; b->out[0].bin = 0;
-                        source_batch(&x, nd, purpose, &pool, b, b->first_id);
+                        uint64_t first = b->first_id;
+                        for (uint32_t c = 0; c < b->n_out; ++c) {
+                            uint32_t idx = 0;
+                            pool_acquire(&pool, &idx);
+                            b->out[c].index = idx; b->out[c].gen = pool.gen[idx]; b->out[c].bin = 0;
+                            source_batch(&x, nd, purpose, &pool, &b->out[c], first);
+                            first += b->out[c].fill;
+                        }
                         trace_batch(out, k, iter, b, 0);
                     } else if (nd->role == OR_ROLE_SINK) {
                         sink_batch(&x, nd, &pool, b);

@@ -132,6 +132,8 @@ typedef struct or_run_cfg {
     uint32_t hash_entries; /* hash path: slots of a tile's bin table (0 -> nextPow2(2 * tile), never full) */
     uint32_t fused_stages; /* runner mode: a partition batch classifies, plans, acquires and scatters at
                               dispatch (one launch) and retires at the next iteration             */
+    uint32_t source_chunks;/* step mode: chunks a source lane admits per step (0 -> 1), each by the
+                              admission rule with the step's earlier admissions counted          */
 } or_run_cfg;
 
 typedef struct or_run_out {

@@ -30,9 +30,10 @@ def test_feedback_lanes_trace(depth, schedule, engine, fuse):
     assert (rec.cpu().numpy().view(np.uint32) == w).all()
 
 
-@pytest.mark.parametrize("depth", [2, 4])
-def test_hash_pipeline_lanes_trace(depth):
-    """Config 3 shape with several batches of the 4096-bucket partition per step."""
+@pytest.mark.parametrize("depth,chunks", [(2, 1), (4, 1), (2, 4)])
+def test_hash_pipeline_lanes_trace(depth, chunks):
+    """Config 3 shape with several batches of the 4096-bucket partition per step (and source lanes
+    of several chunks)."""
     torch = torch_cuda()
     N, B, C, Q = 300000, 4096, 8192, 4
     cap = 1 << 17
@@ -47,10 +48,10 @@ def test_hash_pipeline_lanes_trace(depth):
         flow.connect(part, sink, b)
     P = 1 << 17
     flow.configure(g.Tuning(seed=SEED, batch_chunks=Q, chunk_capacity=C, live_particles=P, schedule=g.SCHEDULE_DEVICE,
-                            trace=True, inflight_batches=depth, device_engine=1))
+                            trace=True, inflight_batches=depth, device_engine=1, source_chunks=chunks))
     st = flow.run(src, N)
     ref = run_oracle(flow, N, C=C, P=P, Q=Q, schedule=g.SCHEDULE_DEVICE, digest_cap=cap, multiset_bins=B,
-                     inflight=depth)
+                     inflight=depth, source_chunks=chunks)
     assert ref["status"] == O.OK, ref["message"]
     assert_same_trace(flow.trace(), ref["trace"])
     torch.cuda.synchronize()
@@ -58,10 +59,32 @@ def test_hash_pipeline_lanes_trace(depth):
     assert (ms[:, 0] == ref["multiset"]).all() and int(ms[:, 1].sum()) == N == st["sunk"]
 
 
+@pytest.mark.parametrize("chunks,depth", [(2, 1), (8, 2)])
+@pytest.mark.parametrize("engine", [1, 2])
+def test_source_chunks_trace(chunks, depth, engine):
+    """Source lanes admitting several chunks per step (wide and one-workgroup engines): every batch
+    record and every final weight equals the oracle's step schedule with the same setting."""
+    torch = torch_cuda()
+    N, C, P, Q = 30011, 512, 4096, 4
+    flow, src, rec = feedback_flow(N)
+    stats, trace = run_flow(flow, src, N, C=C, P=P, Q=Q, schedule=g.SCHEDULE_DEVICE, fuse=True, engine=engine,
+                            inflight_batches=depth, source_chunks=chunks)
+    ref = run_oracle(flow, N, C=C, P=P, Q=Q, schedule=g.SCHEDULE_DEVICE, fuse=True, record=True, inflight=depth,
+                     source_chunks=chunks)
+    assert ref["status"] == O.OK, ref["message"]
+    assert_same_trace(trace, ref["trace"])
+    assert stats["steps"] == ref["steps"]
+    w, _ = O.feedback_scalar(SEED, N)
+    torch.cuda.synchronize()
+    assert (rec.cpu().numpy().view(np.uint32) == w).all()
+
+
 def test_team_engine_rejects_lanes():
     flow, src, _ = feedback_flow(1000)
     with pytest.raises(ValueError):
         flow.configure(g.Tuning(seed=SEED, schedule=g.SCHEDULE_DEVICE, device_engine=3, inflight_batches=2))
+    with pytest.raises(ValueError):
+        flow.configure(g.Tuning(seed=SEED, schedule=g.SCHEDULE_DEVICE, device_engine=3, source_chunks=2))
 
 
 @pytest.mark.parametrize("Q", [1, 16])

@@ -107,3 +107,44 @@ def test_batch_loops_do_not_depend_on_threads():
     b = O.run(FEEDBACK, threads=4, **kw)
     assert a["status"] == b["status"] == O.OK
     assert (a["trace"] == b["trace"]).all() and (a["record"] == b["record"]).all()
+
+
+@pytest.mark.parametrize("chunks,depth", [(2, 1), (4, 1), (3, 2), (8, 4)])
+def test_source_lanes_admit_several_chunks(chunks, depth):
+    # step mode: a source lane admits up to `chunks` chunks per step, each by GraphRunner's admission
+    # rule (GraphRunner.hpp:902-983) with the earlier ones counted; every trajectory is unchanged
+    n, C, P = 30011, 512, 4096
+    kw = dict(seed=42, items=n, capacity=C, live=P, batch=4, mode=O.MODE_STEP, record=True, fuse=True,
+              trace_cap=1 << 23, inflight=depth)
+    ref = O.run(FEEDBACK, **kw)
+    r = O.run(FEEDBACK, source_chunks=chunks, **kw)
+    assert r["status"] == ref["status"] == O.OK, r["message"]
+    w, _ = O.feedback_scalar(42, n)
+    assert (r["record"] == w).all() and r["sunk"] == n
+    src = [(step, outs) for node, step, n_in, counts, outs in decode_outputs(r["trace"]) if node == 0]
+    per_step = {}
+    for step, outs in src:
+        per_step[step] = per_step.get(step, 0) + len(outs)
+        # a lane's chunks: C items each but the last, ids consecutive over the lane's chunks
+        assert 1 <= len(outs) <= chunks
+        assert all(f == C for f in outs[:-1, 2]) and 0 < outs[-1, 2] <= C
+    assert max(per_step.values()) == min(chunks * depth, P // C)
+    assert sum(int(o[:, 2].sum()) for _, o in src) == n
+
+
+def test_source_chunks_do_not_change_the_runner_schedule():
+    kw = dict(seed=42, items=20000, capacity=512, live=4096, batch=4, mode=O.MODE_RUNNER, trace_cap=1 << 23)
+    a = O.run(FEEDBACK, **kw)
+    b = O.run(FEEDBACK, source_chunks=8, **kw)
+    assert a["status"] == b["status"] == O.OK and (a["trace"] == b["trace"]).all()
+
+
+def decode_outputs(tr):
+    i = 0
+    while i < len(tr):
+        node, step, n_in = int(tr[i]) & 0xFFFF, int(tr[i + 1]), int(tr[i + 2])
+        nb = int(tr[i + 3 + 3 * n_in])
+        R = int(tr[i + 4 + 3 * n_in + nb])
+        o0 = i + 5 + 3 * n_in + nb
+        yield node, step, n_in, tr[i + 4 + 3 * n_in:o0 - 1], tr[o0:o0 + 4 * R].reshape(-1, 4)
+        i = o0 + 4 * R
