@@ -162,11 +162,12 @@ def hash_flow(g, ms, B=4096):
     return f, src
 
 
-def hash_forms(g, torch, N, qs, lanes, live):
+def hash_forms(g, torch, N, qs, lanes, live, src_chunks=8, tile=1024):
     """Keyed hash pipeline (Zipf(1.1) keys over 2^20 values, murmur3 fmix32 into 4096 buckets,
-    C=8192) at each Q: keys/s of the batched wide engine with `lanes` batches in flight per node and
-    `live` keys in flight. Per-bucket (hash sum, count) pairs are kept for the CPU leg's check.
-    A second run with HIP events around every step kernel gives the kernels' time."""
+    C=8192) at each Q: keys/s of the batched wide engine with `lanes` batches in flight per node,
+    source lanes of `src_chunks` chunks per step and `live` keys in flight. Per-bucket (hash sum,
+    count) pairs are kept for the CPU leg's check. A second run with HIP events around every step
+    kernel gives the kernels' time."""
     out, multisets = {}, {}
     B, C = 4096, 8192
     ms = torch.zeros(2 * B, dtype=torch.int64, device="cuda")
@@ -174,13 +175,14 @@ def hash_forms(g, torch, N, qs, lanes, live):
         f, src = hash_flow(g, ms, B)
         S = min(524000, live // C + 2 * lanes * (Q + B) + 64)
         tun = dict(seed=42, batch_chunks=Q, chunk_capacity=C, live_particles=live, num_chunks=S,
-                   schedule=g.SCHEDULE_DEVICE, device_engine=1, max_workgroups=1024, tile_items=4096,
-                   inflight_batches=lanes)
+                   schedule=g.SCHEDULE_DEVICE, device_engine=1, max_workgroups=1024, tile_items=tile,
+                   inflight_batches=lanes, source_chunks=src_chunks)
         f.configure(g.Tuning(**tun))
         run_once(f, src, ms, min(N, 2_000_000), torch)  # engine build, hipGraph capture
         dt, st = run_once(f, src, ms, N, torch)
         multisets[Q] = ms.cpu().numpy().view("uint64").reshape(-1, 2).copy()
         rec = {"value": N / dt, "unit": "keys/s", "items": N, "Q": Q, "inflight_batches": lanes, "live": live,
+               "source_chunks": src_chunks, "tile_items": tile,
                "num_chunks": S, "engine_steps": int(st["steps"]), "engine": int(st["engine"])}
         # kernel time per step kernel (direct launches timed with HIP events, not the graph replay)
         f.configure(g.Tuning(**{**tun, "use_graph": 2}))
@@ -188,15 +190,17 @@ def hash_forms(g, torch, N, qs, lanes, live):
         run_once(f, src, ms, N, torch)
         kms, kn = f.kernel_times()
         part_items, _ = f.item_passes()
-        kab = kms[0] + kms[2]
-        rec["kernel_ms"] = {"classify": kms[0], "plan": kms[1], "scatter_source_sink_retire_select": kms[2]}
-        # algorithmic bytes of the classify+scatter(+source, sink) kernels: 12 B source write, 24 B
-        # partition pass (read + write), 12 B sink read per key
-        rec["classify_scatter_gbs"] = 48.0 * N / (kab * 1e-3) / 1e9 if kab > 0 else None
+        rec["kernel_ms"] = {"classify": kms[0], "plan": kms[1], "partition_scatter": kms[4], "source": kms[5],
+                            "sink": kms[6], "retire_select": kms[7]}
+        # the partition's classify + scatter kernels: 24 B per partition pass (12 B read, 12 B written)
+        kab = kms[0] + kms[4]
+        rec["classify_scatter_gbs"] = 24.0 * part_items / (kab * 1e-3) / 1e9 if kab > 0 else None
         rec["classify_scatter_hbm_frac"] = rec["classify_scatter_gbs"] / HBM_PEAK_GBS if kab > 0 else None
+        rec["steps_timed"] = int(kn[0])
         out[f"hash_pipeline_B4096_Q{Q}"] = rec
         log(f"[bench] hash pipeline B=4096 Q={Q} lanes={lanes} N={N}: {N / dt:.4g} keys/s ({int(st['steps'])} steps); "
-            f"kernels KA {kms[0]:.1f} KP {kms[1]:.1f} KB+KR {kms[2]:.1f} ms")
+            f"kernels (ms) classify {kms[0]:.1f} plan {kms[1]:.1f} partition scatter {kms[4]:.1f} source {kms[5]:.1f} "
+            f"sink {kms[6]:.1f} retire/select {kms[7]:.1f}; classify+scatter {rec['classify_scatter_gbs']:.0f} GB/s")
         f.close()
     return out, multisets
 
@@ -527,7 +531,10 @@ def main():
     ap.add_argument("--hash-items", type=float, default=1e8, help="keyed hash pipeline form: keys (0 = skip)")
     ap.add_argument("--hash-q", default="1,4,8,16", help="keyed hash pipeline form: batch sizes Q, comma separated")
     ap.add_argument("--hash-lanes", type=int, default=63, help="keyed hash pipeline: inflight batches per node")
-    ap.add_argument("--hash-live", type=int, default=1 << 22, help="keyed hash pipeline: keys in flight (P)")
+    ap.add_argument("--hash-live", type=int, default=1 << 24, help="keyed hash pipeline: keys in flight (P)")
+    ap.add_argument("--hash-source-chunks", type=int, default=16,
+                    help="keyed hash pipeline: chunks a source lane admits per step")
+    ap.add_argument("--hash-tile", type=int, default=1024, help="keyed hash pipeline: items per workgroup tile")
     ap.add_argument("--serve-jobs", type=int, default=4096, help="served mixed stream: jobs (0 = skip)")
     ap.add_argument("--serve-rate", type=float, default=80.0, help="served mixed stream: Poisson arrivals per second")
     ap.add_argument("--serve-max-items", type=float, default=1e7, help="served mixed stream: N ~ log-uniform[1e3, this]")
@@ -574,7 +581,7 @@ def main():
             forms.update(distribution_forms(g, torch, N, C, P, Q, golden))
         if args.hash_items > 0:
             hf, hash_sets = hash_forms(g, torch, int(args.hash_items), tuple(int(q) for q in args.hash_q.split(",")),
-                                       args.hash_lanes, args.hash_live)
+                                       args.hash_lanes, args.hash_live, args.hash_source_chunks, args.hash_tile)
             forms.update(hf)
         if args.photons > 0:
             pf, pt = photon_forms(g, torch, int(args.photons), int(args.plain_photons), C, P, Q)

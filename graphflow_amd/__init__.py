@@ -654,9 +654,12 @@ class Flow:
         return a[:n.value]
 
     def kernel_times(self):
-        ms = (C.c_double * 4)()
-        n = (C.c_uint64 * 4)()
-        lib().gf_flow_kernel_times(self.handle, ms, n, 4)
+        """Kernel ms and launches per kind of the last timed run (use_graph 2 or 3): classify, plan,
+        scatter + retire/select, persistent engine launches, then the wide engine's partition, source
+        and sink scatter kernels and its retire/select kernels."""
+        ms = (C.c_double * 8)()
+        n = (C.c_uint64 * 8)()
+        lib().gf_flow_kernel_times(self.handle, ms, n, 8)
         return list(ms), list(n)
 
     def phase_profile(self):

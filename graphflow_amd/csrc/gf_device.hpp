@@ -434,6 +434,81 @@ __device__ inline uint32_t zipf_rank(const RunParams& rp, double u)
     return lo;
 }
 
+// Zipf ranks of NQ uniforms with their loads batched: the NQ guide intervals in one round trip,
+// then one window of four consecutive entries per item in a second (at the analytic estimate in a
+// wide interval, at the interval itself in a narrow one), and the rare miss finishes with a binary
+// search. Same answers as zipf_rank.
+template <int NQ>
+__device__ inline void zipf_ranks(const RunParams& rp, const double (&u)[NQ], uint32_t (&rank)[NQ])
+{
+    const double* cdf = rp.zipf;
+    const uint32_t n = rp.zipf_values;
+    if (n < 4u || !rp.zipf_guide) {
+#pragma unroll
+        for (int q = 0; q < NQ; ++q) rank[q] = zipf_rank(rp, u[q]);
+        return;
+    }
+    const double total = cdf[n - 1u];
+    double target[NQ];
+    uint32_t lo[NQ], hi[NQ];
+#pragma unroll
+    for (int q = 0; q < NQ; ++q) {
+        target[q] = u[q] * total;
+        const uint32_t j = (uint32_t)(u[q] * (double)kZipfGuide);  // u < 1: j < kZipfGuide
+        lo[q] = rp.zipf_guide[2u * j];
+        hi[q] = rp.zipf_guide[2u * j + 1u];
+    }
+    uint32_t base[NQ];
+#pragma unroll
+    for (int q = 0; q < NQ; ++q) {
+        if (hi[q] - lo[q] >= 4u) {
+            const double s = rp.zipf_s;
+            double m;
+            if (s == 1.0) {
+                m = exp(target[q] - rp.zipf_a);
+                m = exp(target[q] - rp.zipf_a - 0.5 / m);
+            } else {
+                const double e = 1.0 / (1.0 - s);
+                m = pow((1.0 - s) * (target[q] - rp.zipf_a), e);
+                m = pow((1.0 - s) * (target[q] - rp.zipf_a - 0.5 * pow(m, -s)), e);
+            }
+            double gd = floor(m);
+            if (!(gd >= (double)(lo[q] + 1u))) gd = (double)(lo[q] + 1u);
+            if (!(gd <= (double)(hi[q] - 2u))) gd = (double)(hi[q] - 2u);
+            base[q] = (uint32_t)gd - 1u;  // window [g - 1, g + 2] inside [lo, hi]
+        } else {
+            base[q] = lo[q] < n - 4u ? lo[q] : n - 4u;  // window holds [lo, hi]
+        }
+    }
+    double c[NQ][4];
+#pragma unroll
+    for (int q = 0; q < NQ; ++q)
+#pragma unroll
+        for (int x = 0; x < 4; ++x) c[q][x] = cdf[base[q] + (uint32_t)x];
+#pragma unroll
+    for (int q = 0; q < NQ; ++q) {
+        // every entry before the rank is <= target and the rank's entry (but the last) is above it
+        uint32_t k = 0;
+#pragma unroll
+        for (int x = 0; x < 4; ++x) k += c[q][x] <= target[q] ? 1u : 0u;
+        uint32_t a = lo[q], h = hi[q];
+        if (k > 0u && k < 4u) {
+            a = h = base[q] + k;
+        } else if (k == 0u) {
+            if (base[q] <= a) h = a;  // the rank is at most base <= lo
+            else h = base[q];
+        } else if (base[q] + 4u > a) {
+            a = base[q] + 4u < h ? base[q] + 4u : h;
+        }
+        while (a < h) {
+            const uint32_t mid = a + (h - a) / 2u;
+            if (cdf[mid] > target[q]) h = mid;
+            else a = mid + 1u;
+        }
+        rank[q] = a;
+    }
+}
+
 // Source functors.
 __device__ inline void source(const gf_node_desc&, const RunParams& rp, uint32_t purpose, uint64_t id, Particle& p)
 {
@@ -783,12 +858,22 @@ __host__ __device__ inline size_t scatter_smem(uint32_t bins, uint32_t tile_item
 // bin_base[bin] * C when bin_base is given (the device engines keep each bin's first output chunk
 // beside the bin totals instead of adding it to every tile's entry); flat position f lives at
 // slot out[f / C].index * C + f % C.
+template <class P, int R>
+__device__ void scatter_tile_2p(const gf_node_desc& nd, const Batch& b, uint32_t t, const Cols& cols, uint32_t C,
+                                uint32_t* status, char* smem, const gf_node_desc* all, const int32_t* fuse,
+                                const RunParams* rp, const uint32_t* bin_base);
+
 template <class P, int R, bool Stage>
 __device__ void scatter_tile(const gf_node_desc& nd, const Batch& b, uint32_t t, const Cols& cols, uint32_t C,
                              uint32_t* status, char* smem, const gf_node_desc* all = nullptr,
                              const int32_t* fuse = nullptr, const RunParams* rp = nullptr,
                              const uint32_t* bin_base = nullptr)
 {
+    if constexpr (R > 4) {
+        // tiles of more than 1024 items: never staged (the staged records would exceed 64 KB of LDS)
+        scatter_tile_2p<P, R>(nd, b, t, cols, C, status, smem, all, fuse, rp, bin_base);
+        return;
+    }
     const uint32_t B = nd.bins, tid = ltid(), w = tid >> 6, lane = lane_id();
     uint16_t* wcnt = (uint16_t*)smem;                                    // [kWaves][B] running, then exclusive wave prefix
     uint32_t* ttot = (uint32_t*)(smem + scatter_wcnt_bytes(B));          // Stage: [B] tile totals
@@ -883,6 +968,93 @@ __device__ void scatter_tile(const gf_node_desc& nd, const Batch& b, uint32_t t,
     }
 }
 
+// scatter_tile for tiles of more than 4 rounds per thread (unstaged), in two passes over the tile:
+// the ranking pass keeps each item's bin and in-wave rank in one register (16 bits each: dense
+// partitions have at most 4096 bins, a wave ranks at most 64 R items), the placing pass loads the
+// records again (L2 hits) four rounds at a time. Registers do not grow with the records of R rounds.
+template <class P, int R>
+__device__ void scatter_tile_2p(const gf_node_desc& nd, const Batch& b, uint32_t t, const Cols& cols, uint32_t C,
+                                uint32_t* status, char* smem, const gf_node_desc* all, const int32_t* fuse,
+                                const RunParams* rp, const uint32_t* bin_base)
+{
+    static_assert(R % 4 == 0, "rounds in groups of four");
+    uint32_t tid = ltid();
+    // the round offsets derive from the thread index: made opaque here, so they are recomputed per
+    // tile instead of being hoisted out of the caller's tile loop into R live 64-bit registers
+    asm volatile("" : "+v"(tid));
+    const uint32_t B = nd.bins, w = tid >> 6, lane = tid & 63u;
+    uint16_t* wcnt = (uint16_t*)smem;  // [kWaves][B] running, then exclusive wave prefix
+    for (uint32_t i = tid; i < (uint32_t)(scatter_wcnt_bytes(B) / 4u); i += kThreads) ((uint32_t*)smem)[i] = 0u;
+    __syncthreads();
+    TileSpan s = tile_span(b, b.in, b.n_in, t, C);
+    uint32_t br[R];  // bin | rank << 16; 0xFFFFFFFF: no item
+    const uint32_t nbits = 32u - (uint32_t)__clz((int)(B > 1u ? B - 1u : 1u));
+#pragma unroll
+    for (int r = 0; r < R; ++r) {
+        const uint32_t i = w * 64u * R + (uint32_t)r * 64u + lane;
+        bool v = i < s.count;
+        uint32_t bin = 0u;
+        if (v) {
+            bin = classify(nd, P::load(cols, s.base + i));
+            if (bin >= B) {
+                set_status(status, GF_INVALID);
+                v = false;
+            }
+        }
+        const uint64_t peers = B > 64u ? peer_mask_bits(v, bin, nbits) : peer_mask(v, bin);
+        const uint32_t cur = v ? (uint32_t)wcnt[w * B + bin] : 0u;
+        const uint32_t rank = cur + (uint32_t)__popcll(peers & lanemask_lt());
+        // every lane of the wave has read `cur` before a leader stores the new running count
+        if (v && (peers & lanemask_lt()) == 0ull) wcnt[w * B + bin] = (uint16_t)(cur + (uint32_t)__popcll(peers));
+        br[r] = v ? (bin | rank << 16) : 0xFFFFFFFFu;
+    }
+    __syncthreads();
+    for (uint32_t k = tid; k < B; k += kThreads) {
+        uint32_t acc = 0;
+        for (int ww = 0; ww < kWaves; ++ww) {
+            const uint32_t x = wcnt[ww * B + k];
+            wcnt[ww * B + k] = (uint16_t)acc;
+            acc += x;
+        }
+    }
+    __syncthreads();
+    const uint32_t* toff = b.counts + (size_t)(t == kNoTile ? 0u : t) * B;
+    // the placing pass with and without the bins' first output chunks (a uniform choice made once,
+    // so no per-item branch waits on a load)
+    auto place = [&](auto has_base) {
+        constexpr bool HB = decltype(has_base)::value;
+#pragma unroll
+        for (int r0 = 0; r0 < R; r0 += 4) {
+            uint32_t f[4], ci[4];
+            P rec[4];
+#pragma unroll
+            for (int u = 0; u < 4; ++u) {
+                const uint32_t x = br[r0 + u], k = x == 0xFFFFFFFFu ? 0u : x & 0xFFFFu;
+                f[u] = toff[k] + (uint32_t)wcnt[w * B + k] + (x >> 16);
+                if constexpr (HB) f[u] += bin_base[k] * C;
+                if (x != 0xFFFFFFFFu) rec[u] = P::load(cols, s.base + w * 64u * R + (uint32_t)(r0 + u) * 64u + lane);
+            }
+#pragma unroll
+            for (int u = 0; u < 4; ++u) ci[u] = br[r0 + u] == 0xFFFFFFFFu ? 0u : b.out[f[u] / C].index;
+#pragma unroll
+            for (int u = 0; u < 4; ++u) {
+                const uint32_t x = br[r0 + u];
+                if (x == 0xFFFFFFFFu) continue;
+                const uint32_t k = x & 0xFFFFu;
+                P p = rec[u];
+                const uint32_t st = update(nd, k, p);
+                if (st) set_status(status, st);
+                if (fuse && fuse[k] >= 0) transform(all[fuse[k] & 0xFFFF], *rp, (uint32_t)fuse[k] >> 16, p);
+                p.store(cols, (size_t)ci[u] * C + f[u] % C);
+            }
+            __builtin_amdgcn_sched_barrier(0);  // one group's records live at a time
+        }
+    };
+    if (bin_base) place(std::true_type{});
+    else place(std::false_type{});
+    __syncthreads();
+}
+
 // Out-of-place transform of one tile (TransformKernel, Execution.hpp:708-776): dense
 // concatenation, global position = items before the chunk + offset.
 template <class P, int R>
@@ -908,11 +1080,37 @@ __device__ void source_tile(const gf_node_desc& nd, const Batch& b, uint32_t t, 
     if (t == kNoTile) return;
     uint32_t off = t * b.tile_items;
     uint32_t count = b.particles - off < b.tile_items ? b.particles - off : b.tile_items;
-    for (uint32_t i = ltid(); i < count; i += kThreads) {
-        P p;
-        const uint32_t g = off + i;
-        source(nd, rp, b.purpose, b.first_id + g, p);
-        p.store(cols, (size_t)b.out[g / C].index * C + g % C);
+    if constexpr (std::is_same_v<P, Keyed>) {
+        // four items per thread: their Zipf searches share two round trips
+        for (uint32_t i0 = ltid(); i0 < count; i0 += 4u * kThreads) {
+            double u[4];
+            uint32_t key[4];
+#pragma unroll
+            for (int q = 0; q < 4; ++q) {
+                const uint32_t g = off + i0 + (uint32_t)q * kThreads;
+                uint32_t r[4];
+                stream_words(rp.seed, b.first_id + g, 0, b.purpose, r);
+                u[q] = i0 + (uint32_t)q * kThreads < count ? uniform_f64(r[0], r[1]) : 0.0;
+            }
+            zipf_ranks<4>(rp, u, key);
+#pragma unroll
+            for (int q = 0; q < 4; ++q) {
+                const uint32_t i = i0 + (uint32_t)q * kThreads;
+                if (i >= count) continue;
+                const uint32_t g = off + i;
+                Keyed p;
+                p.key = key[q];
+                p.val = b.first_id + g;
+                p.store(cols, (size_t)b.out[g / C].index * C + g % C);
+            }
+        }
+    } else {
+        for (uint32_t i = ltid(); i < count; i += kThreads) {
+            P p;
+            const uint32_t g = off + i;
+            source(nd, rp, b.purpose, b.first_id + g, p);
+            p.store(cols, (size_t)b.out[g / C].index * C + g % C);
+        }
     }
 }
 
@@ -1062,6 +1260,70 @@ __device__ void sink_tiles_wave(const gf_node_desc& nd, const Batch& b, uint32_t
                 }
             }
         }
+    }
+}
+
+// Sink of four consecutive tiles t .. t + nt - 1 (nt <= 4) by one wave, one tile per 16-lane row
+// (same results as sink_tile): the small per-bucket chunks a sink of a wide partition takes need at
+// most four row iterations each, their loads are in flight together, and a row's digest sums are
+// reduced within the row. Tiles of more than 64 items go to the whole wave afterwards, one at a time.
+template <class P>
+__device__ void sink_tiles_rows(const gf_node_desc& nd, const Batch& b, uint32_t t, uint32_t nt, const Cols& cols,
+                                uint32_t C, uint32_t* status)
+{
+    const uint32_t lane = lane_id(), row = lane >> 4, j = lane & 15u;
+    const TileSpan sp = tile_span(b, b.in, b.n_in, row < nt ? t + row : kNoTile, C);
+    const bool small = sp.count <= 64u;
+    if constexpr (std::is_same_v<P, Particle>) {
+        if (small)
+            for (uint32_t i = j; i < sp.count; i += 16u) {
+                const Particle p = Particle::load(cols, sp.base + i);
+                if (nd.op == GF_SNK_HIST) {
+                    uint32_t v = nd.u[0] == 0u ? __float_as_uint(p.w) : p.gen;
+                    uint32_t rel = v - nd.u[1];
+                    if (v < nd.u[1] || rel >= nd.u[2]) set_status(status, GF_INVALID);
+                    else atomicAdd((unsigned long long*)nd.ptr[0] + rel, 1ull);
+                } else if (nd.op == GF_SNK_RECORD_WEIGHT) {
+                    if ((uint64_t)p.id < nd.size[0]) ((uint32_t*)nd.ptr[0])[p.id] = __float_as_uint(p.w);
+                    else set_status(status, GF_INVALID);
+                }
+            }
+    } else if constexpr (std::is_same_v<P, Keyed>) {
+        if (nd.op != GF_SNK_CHUNK_DIGEST) return;
+        uint64_t acc = 0ull, msum = 0ull;
+        if (small)
+            for (uint32_t i = j; i < sp.count; i += 16u) {
+                const Keyed p = Keyed::load(cols, sp.base + i);
+                const uint64_t h = keyed_hash(p);
+                acc += mix64(h ^ ((uint64_t)(sp.offset + i + 1u) * 0xD6E8FEB86659FD93ull));
+                msum += h;
+            }
+        for (int o = 8; o > 0; o >>= 1) {
+            acc += __shfl_down(acc, o, 16);
+            msum += __shfl_down(msum, o, 16);
+        }
+        if (j == 0u && small && sp.count) {
+            const gf_chunk_desc d = b.in[sp.chunk];
+            const uint64_t entry = b.digest_base + sp.chunk;
+            if (nd.ptr[0] && entry < nd.size[0]) {
+                uint32_t* log = (uint32_t*)nd.ptr[0] + 4 * entry;
+                if (sp.offset == 0) {
+                    log[0] = d.bin;
+                    log[1] = d.fill;
+                }
+                atomicAdd((unsigned long long*)(log + 2), (unsigned long long)acc);
+            }
+            if (nd.ptr[1] && d.bin < nd.size[1]) {
+                unsigned long long* ms = (unsigned long long*)nd.ptr[1] + 2 * d.bin;
+                atomicAdd(ms, (unsigned long long)msum);
+                atomicAdd(ms + 1, (unsigned long long)sp.count);
+            }
+        }
+    }
+    // the larger tiles, each by the whole wave
+    for (uint64_t big = __ballot(j == 0u && !small); big; big &= big - 1ull) {
+        const uint32_t r = (uint32_t)__ffsll((unsigned long long)big) - 1u;
+        sink_tiles_wave<P, 1>(nd, b, t + (r >> 4), 1u, cols, C, status);
     }
 }
 

@@ -97,7 +97,8 @@ uint32_t reserve_bound_of(const gf_node_desc& nd, uint32_t C, uint32_t Q)
 // per CU of MI355X's 160 KB LDS; automatic selects the hash path above that (the reference's
 // limit, Config.hpp:174, is derived from its GPUs' shared memory).
 constexpr uint32_t kDenseBinLimit = 4096u;
-constexpr uint32_t kHashTileLimit = 1024u;  // hash path tiles: a tile's table of 2 * 1024 slots fits LDS
+constexpr uint32_t kHashTileLimit = 1024u;
+constexpr uint32_t kKernelKinds = 8;  // kernel-time kinds of gf_flow_kernel_times  // hash path tiles: a tile's table of 2 * 1024 slots fits LDS
 
 uint32_t next_pow2(uint32_t v)
 {
@@ -275,37 +276,51 @@ __global__ __launch_bounds__(1024) void k_engine_trace(Engine e)
     engine_trace_step(e, &len);
 }
 
+// Tile kinds one scatter-kernel launch runs. A step launches three instantiations (partition and
+// transform tiles, source tiles, sink tiles: disjoint lanes, disjoint chunks), so each holds only its
+// own code's registers and occupancy does not follow the largest tile function.
+enum { KB_PART = 1u, KB_SRC = 2u, KB_SINK = 4u };
+
+__device__ inline uint32_t kb_group(uint32_t kind)
+{
+    return kind == KIND_SOURCE ? KB_SRC : kind == KIND_SINK ? KB_SINK : KB_PART;
+}
+
 // Tiles round-robin over the workgroups; the tiles of the sinks flagged NODE_WAVE_SINK (sinks fed by
-// wide partitions, whose chunks are small) round-robin over the waves instead, four at a time.
-template <class P, int R>
+// wide partitions, whose chunks are small) round-robin over the waves instead, four at a time (one per
+// 16-lane row).
+template <class P, int R, uint32_t Kinds>
 __global__ __launch_bounds__(kThreads) void k_engine_kb(Engine e)
 {
     extern __shared__ __align__(16) char smem[];
     __shared__ uint32_t flag, pre[kMaxNodes + 1], order[kMaxNodes], wsink[kMaxNodes], scan[kWaves];
     const uint32_t n = e.n_nodes, tid = threadIdx.x;
-    // work order: the partitions' scatter tiles first (the longest tiles of a step), then the other
-    // lanes' tiles, each group in node order; pre = tiles before position i, order[i] = its lane
-    // (a thread per lane, n <= kMaxNodes <= the block)
+    // this launch's lanes in node order (partitions first in the partition launch: their tiles are
+    // the step's longest); pre = tiles before position i, order[i] = its lane (a thread per lane,
+    // n <= kMaxNodes <= the block)
     uint32_t tiles = 0u, part = 0u, other = 0u;
     if (tid < n) {
-        tiles = e.kb_prefix[tid + 1u] - e.kb_prefix[tid];
-        part = e.arr[tid].kind == KIND_PARTITION ? 1u : 0u;
-        other = 1u - part;
-        wsink[tid] = (e.arr[tid].flags & NODE_WAVE_SINK) && e.st[tid].active ? 1u : 0u;
+        const uint32_t kind = e.arr[tid].kind;
+        const bool mine = (kb_group(kind) & Kinds) != 0u;
+        tiles = mine ? e.kb_prefix[tid + 1u] - e.kb_prefix[tid] : 0u;
+        part = mine && kind == KIND_PARTITION ? 1u : 0u;
+        other = mine && !part ? 1u : 0u;
+        wsink[tid] = mine && (e.arr[tid].flags & NODE_WAVE_SINK) && e.st[tid].active ? 1u : 0u;
     }
     uint32_t n_part, t_part, t_other, n_other;
     const uint32_t i_part = block_exscan(part, scan, &n_part);
     const uint32_t i_other = block_exscan(other, scan, &n_other);
     const uint32_t p_part = block_exscan(part * tiles, scan, &t_part);
     const uint32_t p_other = block_exscan(other * tiles, scan, &t_other);
-    if (tid < n) {
+    const uint32_t nm = n_part + n_other;
+    if (tid < n && (part || other)) {
         const uint32_t i = part ? i_part : n_part + i_other;
         order[i] = tid;
         pre[i] = part ? p_part : t_part + p_other;
     }
-    if (tid == 0) pre[n] = t_part + t_other;
+    if (tid == 0) pre[nm] = t_part + t_other;
     if (engine_stopped(e, &flag)) return;  // its barrier also publishes `pre`, `order` and `wsink`
-    const uint32_t total = pre[n], G = gridDim.x;
+    const uint32_t total = pre[nm], G = gridDim.x;
     // optional profile (use_graph = 3): wall-clock ticks and tiles per tile kind, summed over workgroups
     unsigned long long tprof = e.prof ? wall_clock64() : 0ull;
     auto prof_tile = [&](uint32_t kind) {
@@ -319,7 +334,7 @@ __global__ __launch_bounds__(kThreads) void k_engine_kb(Engine e)
         }
     };
     for (uint32_t t = blockIdx.x; t < total; t += G) {
-        const uint32_t i = lane_of(pre, n, t), k = order[i];
+        const uint32_t i = lane_of(pre, nm, t), k = order[i];
         if (wsink[k]) {
             // skip the lane's tiles: the first tile of this workgroup's round-robin at or after them
             const uint32_t end = pre[i + 1u];
@@ -333,41 +348,53 @@ __global__ __launch_bounds__(kThreads) void k_engine_kb(Engine e)
         const uint32_t kind = e.arr[k].kind;
         RunParams rp = e.rp;
         if (e.jobs) rp.seed = e.jobs[e.arr[k].job].seed;  // a served engine: the lane's job seed
-        if (kind == KIND_SOURCE) {
-            source_tile<P>(nd, b, lt, e.cols, e.C, rp);
-        } else if (kind == KIND_SINK) {
-            sink_tile<P>(nd, b, lt, e.cols, e.C, &e.ctl[CTL_STATUS], smem);
-        } else if (kind == KIND_PARTITION) {
-            const int32_t* fz = e.fuse + nd.route_offset;
-            if (e.arr[k].hash_entries)
-                scatter_tile_hash<P, R>(nd, b, lt, e.cols, e.C, &e.ctl[CTL_STATUS], smem, e.arr[k].hash_entries,
-                                        e.real_nodes, fz, &rp, b.totals + nd.bins);
-            else if (e.arr[k].flags & NODE_STAGE)
-                scatter_tile<P, R, true>(nd, b, lt, e.cols, e.C, &e.ctl[CTL_STATUS], smem, e.real_nodes, fz, &rp, b.totals + nd.bins);
-            else
-                scatter_tile<P, R, false>(nd, b, lt, e.cols, e.C, &e.ctl[CTL_STATUS], smem, e.real_nodes, fz, &rp, b.totals + nd.bins);
-        } else {
-            transform_tile<P, R>(nd, b, lt, e.cols, e.C, rp);
+        if constexpr ((Kinds & KB_SRC) != 0u) {
+            if (kind == KIND_SOURCE) source_tile<P>(nd, b, lt, e.cols, e.C, rp);
+        }
+        if constexpr ((Kinds & KB_SINK) != 0u) {
+            if (kind == KIND_SINK) sink_tile<P>(nd, b, lt, e.cols, e.C, &e.ctl[CTL_STATUS], smem);
+        }
+        if constexpr ((Kinds & KB_PART) != 0u) {
+            if (kind == KIND_PARTITION) {
+                const int32_t* fz = e.fuse + nd.route_offset;
+                if constexpr (R <= 4) {
+                    // hash-path tiles hold at most 1024 items
+                    if (e.arr[k].hash_entries) {
+                        scatter_tile_hash<P, R>(nd, b, lt, e.cols, e.C, &e.ctl[CTL_STATUS], smem, e.arr[k].hash_entries,
+                                                e.real_nodes, fz, &rp, b.totals + nd.bins);
+                        prof_tile(kind);
+                        continue;
+                    }
+                }
+                if (e.arr[k].flags & NODE_STAGE)
+                    scatter_tile<P, R, true>(nd, b, lt, e.cols, e.C, &e.ctl[CTL_STATUS], smem, e.real_nodes, fz, &rp, b.totals + nd.bins);
+                else
+                    scatter_tile<P, R, false>(nd, b, lt, e.cols, e.C, &e.ctl[CTL_STATUS], smem, e.real_nodes, fz, &rp, b.totals + nd.bins);
+            } else if (kind == KIND_TRANSFORM) {
+                transform_tile<P, R>(nd, b, lt, e.cols, e.C, rp);
+            }
         }
         prof_tile(kind);
     }
-    const uint32_t w = threadIdx.x >> 6, W = G * (uint32_t)kWaves;
-    unsigned long long wprof = e.prof ? wall_clock64() : 0ull, wtiles = 0ull;
-    for (uint32_t i = n_part; i < n; ++i) {
-        const uint32_t k = order[i];
-        if (!wsink[k]) continue;
-        const uint32_t t0 = pre[i], t1 = pre[i + 1u];
-        const gf_node_desc& nd = e.nodes[k];
-        const Batch b = make_batch(e, k, e.st[k]);
-        // each wave takes four consecutive tiles at a time
-        for (uint32_t t = t0 + (blockIdx.x * (uint32_t)kWaves + w) * 4u; t < t1; t += 4u * W) {
-            sink_tiles_wave<P, 4>(nd, b, t - t0, t1 - t < 4u ? t1 - t : 4u, e.cols, e.C, &e.ctl[CTL_STATUS]);
-            wtiles += t1 - t < 4u ? t1 - t : 4u;
+    if constexpr ((Kinds & KB_SINK) != 0u) {
+        const uint32_t w = threadIdx.x >> 6, W = G * (uint32_t)kWaves;
+        unsigned long long wprof = e.prof ? wall_clock64() : 0ull, wtiles = 0ull;
+        for (uint32_t i = n_part; i < nm; ++i) {
+            const uint32_t k = order[i];
+            if (!wsink[k]) continue;
+            const uint32_t t0 = pre[i], t1 = pre[i + 1u];
+            const gf_node_desc& nd = e.nodes[k];
+            const Batch b = make_batch(e, k, e.st[k]);
+            // each wave takes four consecutive tiles at a time
+            for (uint32_t t = t0 + (blockIdx.x * (uint32_t)kWaves + w) * 4u; t < t1; t += 4u * W) {
+                sink_tiles_rows<P>(nd, b, t - t0, t1 - t < 4u ? t1 - t : 4u, e.cols, e.C, &e.ctl[CTL_STATUS]);
+                wtiles += t1 - t < 4u ? t1 - t : 4u;
+            }
         }
-    }
-    if (e.prof && lane_id() == 0u && wtiles) {
-        atomicAdd(&e.prof[PROF_KB_WAVE_TICKS], wall_clock64() - wprof);
-        atomicAdd(&e.prof[PROF_KB_WAVE_TILES], wtiles);
+        if (e.prof && lane_id() == 0u && wtiles) {
+            atomicAdd(&e.prof[PROF_KB_WAVE_TICKS], wall_clock64() - wprof);
+            atomicAdd(&e.prof[PROF_KB_WAVE_TILES], wtiles);
+        }
     }
 }
 
@@ -1392,8 +1419,10 @@ struct gf_flow {
     std::vector<uint32_t> trace;
     std::unique_ptr<gf_pool> pool;
     std::unique_ptr<EngineBuffers> eng;
-    double kt_ms[4] = {0, 0, 0, 0};
-    uint64_t kt_n[4] = {0, 0, 0, 0};
+    // kernel time per kind (gf_flow_kernel_times): classify, plan, scatter + retire/select, persistent
+    // engines, then the wide engine's partition, source and sink scatter kernels and its retire/select
+    double kt_ms[kKernelKinds] = {};
+    uint64_t kt_n[kKernelKinds] = {};
     uint64_t part_items = 0, xf_items = 0;
 };
 
@@ -1881,9 +1910,9 @@ void build_engine(gf_flow& f, uint64_t N, const RunShape& sh, uint32_t src)
             a.real = k;
             a.lanes = j == 0 ? lanes_of[k] : 0u;
             a.ring = ring;
-            // a source's tiles hold one item per thread: each item's generation (Philox, a Zipf
-            // search) is a chain of dependent steps, so the chunk is spread over more workgroups
-            const uint32_t ti = nd.role == GF_ROLE_SOURCE ? (uint32_t)kThreads : sh.tile_items;
+            // a source's tiles hold four items per thread: each item's generation (Philox, a Zipf
+            // search) is a chain of dependent steps, and a thread runs its four chains together
+            const uint32_t ti = nd.role == GF_ROLE_SOURCE ? 4u * (uint32_t)kThreads : sh.tile_items;
             uint32_t cap_tiles = (uint32_t)(nd.role == GF_ROLE_SOURCE ? ceil_div((uint64_t)sh.C * e.src_chunks, ti)
                                                                         : ceil_div(sh.C, ti) * (nd.role == GF_ROLE_SINK ? 1u : a.cap_in));
             a.in = m.d<gf_chunk_desc>(a.cap_in);
@@ -2041,7 +2070,7 @@ void build_engine(gf_flow& f, uint64_t N, const RunShape& sh, uint32_t src)
         using P = typename decltype(pt)::type;
         constexpr int RR = decltype(rc)::value;
         set_lds(k_engine_ka<P, RR>, lds_a);
-        set_lds(k_engine_kb<P, RR>, lds_b);
+        set_lds(k_engine_kb<P, RR, KB_PART>, lds_b);
         if (micro_fits) set_lds(k_micro<P, RR>, eb->micro_lds);
     });
     static_assert(sizeof(SelLds) <= 160u * 1024u, "the select's LDS state exceeds a workgroup's LDS");
@@ -2056,6 +2085,8 @@ void build_engine(gf_flow& f, uint64_t N, const RunShape& sh, uint32_t src)
 // transform / source / sink tiles), KR (routing and release over the grid, counters and select in
 // one workgroup, input lists over the grid). With `ev`, events are recorded around KA, KP and
 // KB + KR.
+// With events (timed runs): ev[0..6] around classify, plan, the three scatter kernels and the
+// retire/select kernels.
 void launch_step(const EngineBuffers& eb, uint32_t payload, hipStream_t s, hipEvent_t* ev = nullptr)
 {
     with_payload(payload, eb.r, [&](auto pt, auto rc) {
@@ -2067,11 +2098,18 @@ void launch_step(const EngineBuffers& eb, uint32_t payload, hipStream_t s, hipEv
         hipLaunchKernelGGL(k_engine_plan, dim3(eb.e.n_nodes), dim3(1024), 0, s, eb.e);
         if (eb.e.trace_on) hipLaunchKernelGGL(k_engine_trace, dim3(1), dim3(1024), 0, s, eb.e);
         if (ev) HIPX(hipEventRecord(ev[2], s));
-        hipLaunchKernelGGL((k_engine_kb<P, RR>), dim3(eb.grid), dim3(kThreads), eb.lds_b, s, eb.e);
+        // partition / transform tiles, then source tiles, then sink tiles (disjoint lanes and chunks;
+        // running them on forked streams measured no faster than back to back)
+        hipLaunchKernelGGL((k_engine_kb<P, RR, KB_PART>), dim3(eb.grid), dim3(kThreads), eb.lds_b, s, eb.e);
+        if (ev) HIPX(hipEventRecord(ev[3], s));
+        hipLaunchKernelGGL((k_engine_kb<P, RR, KB_SRC>), dim3(eb.grid), dim3(kThreads), 256, s, eb.e);
+        if (ev) HIPX(hipEventRecord(ev[4], s));
+        hipLaunchKernelGGL((k_engine_kb<P, RR, KB_SINK>), dim3(eb.grid), dim3(kThreads), 256, s, eb.e);
+        if (ev) HIPX(hipEventRecord(ev[5], s));
         hipLaunchKernelGGL(k_engine_route, dim3(eb.route_grid), dim3(1024), 0, s, eb.e);
         hipLaunchKernelGGL(k_engine_kr, dim3(1), dim3(1024), sizeof(SelLds), s, eb.e);
         hipLaunchKernelGGL(k_engine_lists, dim3(eb.route_grid), dim3(1024), 0, s, eb.e);
-        if (ev) HIPX(hipEventRecord(ev[3], s));
+        if (ev) HIPX(hipEventRecord(ev[6], s));
     });
 }
 
@@ -2088,7 +2126,7 @@ void run_micro(gf_flow& f, EngineBuffers& eb, uint64_t N, gf_stats& st, bool tim
     hipEvent_t ev[2] = {nullptr, nullptr};
     if (timed)
         for (auto& x : ev) HIPX(hipEventCreate(&x));
-    for (int i = 0; i < 4; ++i) {
+    for (uint32_t i = 0; i < kKernelKinds; ++i) {
         f.kt_ms[i] = 0.0;
         f.kt_n[i] = 0;
     }
@@ -2137,7 +2175,7 @@ void run_team(gf_flow& f, EngineBuffers& eb, uint64_t N, gf_stats& st, bool time
     hipEvent_t ev[2] = {nullptr, nullptr};
     if (timed)
         for (auto& x : ev) HIPX(hipEventCreate(&x));
-    for (int i = 0; i < 4; ++i) {
+    for (uint32_t i = 0; i < kKernelKinds; ++i) {
         f.kt_ms[i] = 0.0;
         f.kt_n[i] = 0;
     }
@@ -2237,10 +2275,10 @@ void run_wide(gf_flow& f, EngineBuffers& eb, uint64_t N, gf_stats& st, bool time
     }
     uint32_t ctl[CTL_WORDS];
     uint64_t launches = launches0;
-    hipEvent_t ev[4] = {nullptr, nullptr, nullptr, nullptr};
+    hipEvent_t ev[7] = {};
     if (timed)
         for (auto& x : ev) HIPX(hipEventCreate(&x));
-    for (int i = 0; i < 4; ++i) {
+    for (uint32_t i = 0; i < kKernelKinds; ++i) {
         f.kt_ms[i] = 0.0;
         f.kt_n[i] = 0;
     }
@@ -2256,12 +2294,14 @@ void run_wide(gf_flow& f, EngineBuffers& eb, uint64_t N, gf_stats& st, bool time
             for (uint32_t i = 0; i < K; ++i) {
                 if (timed) {
                     launch_step(eb, f.payload, s, ev);
-                    HIPX(hipEventSynchronize(ev[3]));
-                    for (int q = 0; q < 3; ++q) {
+                    HIPX(hipEventSynchronize(ev[6]));
+                    // kinds 0, 1, 2 (= 4 + 5 + 6 + 7), 4: partition, 5: source, 6: sink, 7: retire/select
+                    const int from[7] = {0, 1, 2, 2, 3, 4, 5}, to[7] = {1, 2, 6, 3, 4, 5, 6}, kind[7] = {0, 1, 2, 4, 5, 6, 7};
+                    for (int q = 0; q < 7; ++q) {
                         float x = 0;
-                        HIPX(hipEventElapsedTime(&x, ev[q], ev[q + 1]));
-                        f.kt_ms[q] += x;
-                        f.kt_n[q]++;
+                        HIPX(hipEventElapsedTime(&x, ev[from[q]], ev[to[q]]));
+                        f.kt_ms[kind[q]] += x;
+                        f.kt_n[kind[q]]++;
                     }
                 } else {
                     launch_step(eb, f.payload, s);
@@ -2877,9 +2917,9 @@ gf_status gf_flow_trace(gf_flow* flow, uint32_t* host, uint64_t cap, uint64_t* l
 gf_status gf_flow_kernel_times(gf_flow* flow, double* ms, uint64_t* launches, uint32_t kinds)
 {
     if (!flow || !ms || !launches) return GF_BAD_ARGUMENT;
-    for (uint32_t i = 0; i < kinds && i < 4; ++i) {
-        ms[i] = flow->kt_ms[i];
-        launches[i] = flow->kt_n[i];
+    for (uint32_t i = 0; i < kinds; ++i) {
+        ms[i] = i < kKernelKinds ? flow->kt_ms[i] : 0.0;
+        launches[i] = i < kKernelKinds ? flow->kt_n[i] : 0u;
     }
     return GF_OK;
 }

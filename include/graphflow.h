@@ -248,7 +248,9 @@ gf_status gf_flow_autotune(gf_flow* flow, uint32_t source, uint64_t probe_items,
 gf_status gf_flow_last_error(gf_flow* flow, char* buffer, uint32_t size);
 gf_status gf_flow_trace(gf_flow* flow, uint32_t* host, uint64_t cap, uint64_t* len);
 /* kernel timing of the last device-engine run with use_graph 2 or 3: total ms and launches per kind
- * (0: classify, 1: plan, 2: scatter + retire/select of the wide engine, 3: team engine launches) */
+ * (0: classify, 1: plan, 2: scatter + retire/select of the wide engine, 3: team / one-workgroup engine
+ * launches, 4: partition and transform scatter, 5: source tiles, 6: sink tiles, 7: retire/select;
+ * kinds beyond 7 read 0) */
 gf_status gf_flow_kernel_times(gf_flow* flow, double* ms, uint64_t* launches, uint32_t kinds);
 /* micro or team engine run with use_graph = 3: wall-clock ticks (100 MHz) per step phase and the
  * number of steps (micro: count, plan, source, partition scatter, transform, sink, retire, select,
