@@ -12,11 +12,12 @@
 // that does not fit the team's limits hands the engine to the wide engine until it is idle. The
 // hash pipeline's 4096-bucket partition runs on one wide engine.
 //
-// One host thread (the driver) runs the loop: it admits queued jobs into free slots, writes their
-// items, live bounds and seeds into the job tables between launches, launches every busy engine on
-// its own stream (a team launch of up to kServeTeamSteps steps, or a hipGraph of kServeSteps wide
-// steps; the engines run side by side on the device), then reads the job tables and finishes the
-// jobs whose items are all sunk.
+// One host thread (the driver) runs the loop: it admits queued jobs into free slots of engines with no
+// launch in flight, writes their items, live bounds and seeds into the job tables between launches,
+// and keeps every busy engine launching on its own stream (a team launch of up to kServeTeamSteps
+// steps, or a hipGraph of kServeSteps wide steps; the engines run side by side on the device, each
+// at its own pace). A launch is followed by copies of the engine's job table into pinned host memory
+// and an event; when the event completes the driver finishes the jobs whose items are all sunk.
 //
 // Admission (first come, first served within a topology): a job needs a free slot on one of its
 // topology's engines whose unreserved chunks hold the job's pool bound S_j = P_j + the graph's
@@ -97,7 +98,13 @@ struct ServeEngine {
     uint32_t busy = 0;
     bool team = false;   // a team engine (runs on the wide engine while `wide` is set)
     bool wide = false;
-    bool launched = false;
+    // a launch in flight: its job table, control words and step counter are copied into pinned host
+    // memory behind it, and `done` records its completion (the driver polls it)
+    bool inflight = false;
+    hipEvent_t done = nullptr;
+    gfd::JobSlot* jt_h = nullptr;
+    uint32_t* ctl_h = nullptr;
+    unsigned long long* step_h = nullptr;
 };
 
 }  // namespace
@@ -205,6 +212,11 @@ uint32_t add_slot_graph(gf_flow* f, uint32_t topo, uint32_t slot, ServeSlot& ss,
 
 void destroy_engine(ServeEngine& te)
 {
+    if (te.ctx && te.inflight) (void)hipStreamSynchronize(te.ctx->stream);
+    if (te.done) (void)hipEventDestroy(te.done);
+    if (te.jt_h) (void)hipHostFree(te.jt_h);
+    if (te.ctl_h) (void)hipHostFree(te.ctl_h);
+    if (te.step_h) (void)hipHostFree(te.step_h);
     if (te.flow) gf_flow_destroy(te.flow);
     for (auto& ss : te.slots) {
         if (ss.rec) (void)hipFree(ss.rec);
@@ -297,10 +309,16 @@ void build_serve_engine(gf_server& sv, ServeEngine& te, uint32_t topo, uint32_t 
     eb.graph_steps = kServeSteps;
     reset_engine(te);
     te.table.assign(M, gfd::JobSlot{0, 0, 0, 0, 0});
+    HIPX(hipEventCreateWithFlags(&te.done, hipEventDisableTiming));
+    HIPX(hipHostMalloc((void**)&te.jt_h, sizeof(gfd::JobSlot) * M, hipHostMallocDefault));
+    HIPX(hipHostMalloc((void**)&te.ctl_h, sizeof(uint32_t) * CTL_WORDS, hipHostMallocDefault));
+    HIPX(hipHostMalloc((void**)&te.step_h, sizeof(unsigned long long), hipHostMallocDefault));
     te.bytes = pool * chunk_bytes + (uint64_t)M * (topo == GF_JOB_FEEDBACK ? 4u * sv.cfg.max_items : 65536u) + (64ull << 20);
 }
 
-// One launch on an engine's stream: a team launch, or the wide engine's K-step graph.
+// One launch on an engine's stream (a team launch, or the wide engine's K-step graph), followed by
+// the copies of its job table, control words and step counter into pinned host memory and the
+// completion event the driver polls.
 void launch_engine(ServeEngine& te)
 {
     EngineBuffers& eb = *te.flow->eng;
@@ -316,6 +334,11 @@ void launch_engine(ServeEngine& te)
     } else {
         HIPX(hipGraphLaunch(eb.exec, st));
     }
+    HIPX(hipMemcpyAsync(te.jt_h, eb.e.jobs, sizeof(gfd::JobSlot) * te.table.size(), hipMemcpyDeviceToHost, st));
+    HIPX(hipMemcpyAsync(te.ctl_h, eb.e.ctl, sizeof(uint32_t) * CTL_WORDS, hipMemcpyDeviceToHost, st));
+    HIPX(hipMemcpyAsync(te.step_h, eb.e.ctr + CNT_STEP, sizeof(unsigned long long), hipMemcpyDeviceToHost, st));
+    HIPX(hipEventRecord(te.done, st));
+    te.inflight = true;
 }
 
 // The job's pool bound and live bound on an engine (P_j = min(N, P), lowered to fit the pool).
@@ -433,27 +456,111 @@ void fail_engine(gf_server& sv, ServeEngine& te, uint32_t status, std::vector<gf
     reset_engine(te);
 }
 
+// Fail the admitted jobs of every engine after an error outside the engines' own reports.
+void fail_all_jobs(gf_server& sv, gf_status rc)
+{
+    for (ServeEngine& te : sv.eng) {
+        if (te.inflight) {
+            (void)hipStreamSynchronize(te.ctx->stream);
+            te.inflight = false;
+        }
+        for (uint32_t s = 0; s < (uint32_t)te.slots.size(); ++s) {
+            ServeSlot& ss = te.slots[s];
+            if (!ss.busy) continue;
+            gf_job_result r{};
+            r.job = ss.job;
+            r.topology = te.topo;
+            r.slot = s;
+            r.status = rc;
+            r.items = ss.items;
+            r.submit_ms = ss.submit_ms;
+            r.start_ms = ss.start_ms;
+            r.end_ms = sv.now_ms();
+            sv.done.push_back(r);
+            sv.finished++;
+            ss.busy = false;
+            te.reserved -= ss.bound;
+            te.busy--;
+        }
+    }
+}
+
 void server_driver(gf_server* svp)
 {
     gf_server& sv = *svp;
     for (;;) {
-        // 1. admission: first come, first served within each topology, into the first engine of the
-        //    job's topology with a free slot and room for the job's pool bound
+        bool progressed = false;
+        // 1. finished launches: finish the jobs whose items are all sunk
+        {
+            std::vector<gf_job_result> results;
+            std::vector<std::pair<uint64_t, std::vector<double>>> tallies;
+            std::string msg;
+            const gf_status rc = guard(&msg, [&] {
+                for (ServeEngine& te : sv.eng) {
+                    if (!te.inflight) continue;
+                    const hipError_t q = hipEventQuery(te.done);
+                    if (q == hipErrorNotReady) continue;
+                    HIPX(q);
+                    te.inflight = false;
+                    progressed = true;
+                    EngineBuffers& eb = *te.flow->eng;
+                    hipStream_t st = te.ctx->stream;
+                    const uint32_t* ctl = te.ctl_h;
+                    const unsigned long long step = *te.step_h;
+                    te.step = step;
+                    if (ctl[CTL_STATUS]) {
+                        const uint32_t code = ctl[CTL_STATUS];
+                        fail_engine(sv, te, code, results);
+                        fail((gf_status)code, "serving: an engine of topology " + std::to_string(te.topo) +
+                                                  " reported an error");
+                    }
+                    for (uint32_t s = 0; s < (uint32_t)te.slots.size(); ++s) {
+                        ServeSlot& ss = te.slots[s];
+                        if (!ss.busy || te.jt_h[s].sunk < ss.items) continue;
+                        gf_job_result r{};
+                        std::vector<double> tally;
+                        finish_job(sv, te, s, step, r, tally);
+                        if (te.topo == GF_JOB_PHOTON) tallies.emplace_back(r.job, std::move(tally));
+                        results.push_back(r);
+                    }
+                    if (te.team && !te.wide && ctl[CTL_HANDOFF]) {
+                        // a team step that does not fit the team's limits: the engine continues on the
+                        // wide engine from the team's pre-select state until it is idle
+                        hipLaunchKernelGGL(k_engine_resume, dim3(1), dim3(kThreads), sizeof(SelLds), st, eb.e);
+                        HIPX(hipGetLastError());
+                        te.wide = true;
+                    } else if (te.team && te.wide && ctl[CTL_DONE] && !te.busy) {
+                        reset_engine(te);  // idle: back to the team engine
+                    }
+                }
+            });
+            std::lock_guard<std::mutex> lk(sv.mu);
+            if (rc != GF_OK) sv.err = msg;
+            for (auto& tl : tallies) sv.tallies[tl.first] = std::move(tl.second);
+            for (const auto& r : results) sv.done.push_back(r);
+            sv.finished += results.size();
+            if (rc != GF_OK && results.empty()) fail_all_jobs(sv, rc);
+            if (!results.empty() || rc != GF_OK) sv.cv.notify_all();
+        }
+        // 2. admission: first come, first served within each topology, into the first engine of the
+        //    job's topology with no launch in flight, a free slot and room for the job's pool bound
         std::vector<std::pair<uint32_t, ServerJob>> admit;  // (engine, job)
         {
             std::unique_lock<std::mutex> lk(sv.mu);
-            auto idle = [&] {
+            auto quiet = [&] {  // nothing running and nothing admitted
                 for (const ServeEngine& te : sv.eng)
-                    if (te.busy) return false;
+                    if (te.busy || te.inflight) return false;
                 return true;
             };
-            sv.cv.wait(lk, [&] { return sv.stop || !sv.queue.empty() || !idle(); });
-            if (sv.stop && sv.queue.empty() && idle()) break;
+            // with launches in flight the driver polls their events; otherwise it sleeps until a
+            // submission or the server's end
+            if (quiet()) sv.cv.wait(lk, [&] { return sv.stop || !sv.queue.empty(); });
+            if (sv.stop && sv.queue.empty() && quiet()) break;
             bool blocked[3] = {false, false, false};
             std::vector<uint32_t> free_slots(sv.eng.size());
             std::vector<uint64_t> room(sv.eng.size());
             for (size_t i = 0; i < sv.eng.size(); ++i) {
-                free_slots[i] = (uint32_t)sv.eng[i].slots.size() - sv.eng[i].busy;
+                free_slots[i] = sv.eng[i].inflight ? 0u : (uint32_t)sv.eng[i].slots.size() - sv.eng[i].busy;
                 room[i] = sv.eng[i].pool - sv.eng[i].reserved;
             }
             for (auto it = sv.queue.begin(); it != sv.queue.end();) {
@@ -474,94 +581,38 @@ void server_driver(gf_server* svp)
                     admit.emplace_back((uint32_t)pick, *it);
                     it = sv.queue.erase(it);
                 } else {
-                    blocked[t] = true;
+                    blocked[t] = true;  // first come, first served within the topology
                     ++it;
                 }
             }
         }
-        std::vector<gf_job_result> results;
-        std::vector<std::pair<uint64_t, std::vector<double>>> tallies;
-        std::string msg;
-        const gf_status rc = guard(&msg, [&] {
-            for (const auto& aj : admit) {
-                ServeEngine& te = sv.eng[aj.first];
-                uint32_t s = 0;
-                while (te.slots[s].busy) ++s;
-                start_job(sv, te, s, aj.second);
-            }
-            // 2. one launch on every busy engine (side by side, one stream each)
-            for (ServeEngine& te : sv.eng) {
-                te.launched = te.busy != 0u;
-                if (te.launched) launch_engine(te);
-            }
-            // 3. read the job tables; finish the jobs whose items are all sunk
-            for (ServeEngine& te : sv.eng) {
-                if (!te.launched) continue;
-                EngineBuffers& eb = *te.flow->eng;
-                hipStream_t st = te.ctx->stream;
-                std::vector<gfd::JobSlot> jt(te.table.size());
-                uint32_t ctl[CTL_WORDS];
-                unsigned long long step = 0;
-                HIPX(hipMemcpyAsync(jt.data(), eb.e.jobs, sizeof(gfd::JobSlot) * jt.size(), hipMemcpyDeviceToHost, st));
-                HIPX(hipMemcpyAsync(ctl, eb.e.ctl, sizeof(ctl), hipMemcpyDeviceToHost, st));
-                HIPX(hipMemcpyAsync(&step, eb.e.ctr + CNT_STEP, sizeof(step), hipMemcpyDeviceToHost, st));
-                HIPX(hipStreamSynchronize(st));
-                te.step = step;
-                if (ctl[CTL_STATUS]) {
-                    fail_engine(sv, te, ctl[CTL_STATUS], results);
-                    fail((gf_status)ctl[CTL_STATUS], "serving: an engine of topology " + std::to_string(te.topo) +
-                                                         " reported an error");
-                }
-                for (uint32_t s = 0; s < (uint32_t)te.slots.size(); ++s) {
-                    ServeSlot& ss = te.slots[s];
-                    if (!ss.busy || jt[s].sunk < ss.items) continue;
-                    gf_job_result r{};
-                    std::vector<double> tally;
-                    finish_job(sv, te, s, step, r, tally);
-                    if (te.topo == GF_JOB_PHOTON) tallies.emplace_back(r.job, std::move(tally));
-                    results.push_back(r);
-                }
-                if (te.team && !te.wide && ctl[CTL_HANDOFF]) {
-                    // a team step that does not fit the team's limits: the engine continues on the
-                    // wide engine from the team's pre-select state until it is idle
-                    hipLaunchKernelGGL(k_engine_resume, dim3(1), dim3(kThreads), sizeof(SelLds), st, eb.e);
-                    HIPX(hipGetLastError());
-                    te.wide = true;
-                } else if (te.team && te.wide && ctl[CTL_DONE] && !te.busy) {
-                    reset_engine(te);  // idle: back to the team engine
-                }
-            }
-        });
+        // 3. start the admitted jobs; the next launch of every busy engine with none in flight
+        //    (engines advance independently, side by side on their own streams)
         {
-            std::lock_guard<std::mutex> lk(sv.mu);
-            if (rc != GF_OK) sv.err = msg;
-            for (auto& tl : tallies) sv.tallies[tl.first] = std::move(tl.second);
-            for (const auto& r : results) sv.done.push_back(r);
-            sv.finished += results.size();
-            if (rc != GF_OK && results.empty()) {
-                // an error outside any engine's report: fail the admitted jobs still waiting
+            std::string msg;
+            const gf_status rc = guard(&msg, [&] {
+                for (const auto& aj : admit) {
+                    ServeEngine& te = sv.eng[aj.first];
+                    uint32_t s = 0;
+                    while (te.slots[s].busy) ++s;
+                    start_job(sv, te, s, aj.second);
+                    progressed = true;
+                }
                 for (ServeEngine& te : sv.eng)
-                    for (uint32_t s = 0; s < (uint32_t)te.slots.size(); ++s) {
-                        ServeSlot& ss = te.slots[s];
-                        if (!ss.busy) continue;
-                        gf_job_result r{};
-                        r.job = ss.job;
-                        r.topology = te.topo;
-                        r.slot = s;
-                        r.status = rc;
-                        r.items = ss.items;
-                        r.submit_ms = ss.submit_ms;
-                        r.start_ms = ss.start_ms;
-                        r.end_ms = sv.now_ms();
-                        sv.done.push_back(r);
-                        sv.finished++;
-                        ss.busy = false;
-                        te.reserved -= ss.bound;
-                        te.busy--;
+                    if (te.busy && !te.inflight) {
+                        launch_engine(te);
+                        progressed = true;
                     }
+            });
+            if (rc != GF_OK) {
+                std::lock_guard<std::mutex> lk(sv.mu);
+                sv.err = msg;
+                fail_all_jobs(sv, rc);
+                sv.cv.notify_all();
             }
         }
-        sv.cv.notify_all();
+        // launches in flight and nothing else to do: poll again shortly
+        if (!progressed) std::this_thread::sleep_for(std::chrono::microseconds(50));
     }
 }
 
