@@ -553,33 +553,56 @@ static int batch_threads(const or_ctx* x, uint64_t n)
     return n >= 4096u ? t : 1;
 }
 
-/* gather the batch's items in batch order: chunk order, then slot order */
-static void gather(const or_pool* p, batch* b)
+/* gather the batch's items in batch order: chunk order, then slot order (the chunks in parallel) */
+static void gather(const or_ctx* x, const or_pool* p, batch* b)
 {
     uint64_t k = 0;
-    for (uint32_t c = 0; c < b->n_in; ++c) {
-        const or_item* src = p->slots + (size_t)b->in[c].index * p->C;
-        memcpy(b->items + k, src, sizeof(or_item) * b->in[c].fill);
-        k += b->in[c].fill;
-    }
+    for (uint32_t c = 0; c < b->n_in; ++c) k += b->in[c].fill;
     b->particles = k;
+    uint64_t* at = (uint64_t*)malloc(sizeof(uint64_t) * (b->n_in + 1u));
+    at[0] = 0;
+    for (uint32_t c = 0; c < b->n_in; ++c) at[c + 1] = at[c] + b->in[c].fill;
+    const int64_t nc = b->n_in;
+#pragma omp parallel for schedule(dynamic, 1) num_threads(batch_threads(x, k))
+    for (int64_t c = 0; c < nc; ++c) {
+        const or_item* src = p->slots + (size_t)b->in[c].index * p->C;
+        memcpy(b->items + at[c], src, sizeof(or_item) * b->in[c].fill);
+    }
+    free(at);
 }
 
 /* Partition classify (Execution.hpp:869-954 semantics): bins per item, totals per bin. */
 static int partition_classify(const or_ctx* x, const or_node_desc* nd, const or_pool* p, batch* b)
 {
-    gather(p, b);
+    gather(x, p, b);
     memset(b->counts, 0, sizeof(uint32_t) * nd->bins);
     const int64_t n = (int64_t)b->particles;
+    const int T = batch_threads(x, b->particles);
+    const uint32_t B = nd->bins;
+    /* per-thread counts over contiguous item ranges, summed afterwards (integer sums: the totals
+     * do not depend on the thread count) */
+    uint32_t* tc = (uint32_t*)calloc((size_t)T * B, sizeof(uint32_t));
     int bad = 0;
-#pragma omp parallel for schedule(static) num_threads(batch_threads(x, b->particles)) reduction(| : bad)
-    for (int64_t i = 0; i < n; ++i) {
-        uint32_t bin = or_classify(x, nd, &b->items[i]);
-        if (bin >= nd->bins) bad = 1;
-        b->bins[i] = bin;
+#pragma omp parallel num_threads(T) reduction(| : bad)
+    {
+#ifdef _OPENMP
+        const int t = omp_get_thread_num(), nt = omp_get_num_threads();
+#else
+        const int t = 0, nt = 1;
+#endif
+        const int64_t lo = n * t / nt, hi = n * (t + 1) / nt;
+        uint32_t* c = tc + (size_t)t * B;
+        for (int64_t i = lo; i < hi; ++i) {
+            uint32_t bin = or_classify(x, nd, &b->items[i]);
+            if (bin >= B) { bad = 1; bin = 0; b->bins[i] = 0xFFFFFFFFu; continue; }
+            b->bins[i] = bin;
+            c[bin]++;
+        }
     }
+    for (int t = 0; t < T; ++t)
+        for (uint32_t k = 0; k < B; ++k) b->counts[k] += tc[(size_t)t * B + k];
+    free(tc);
     if (bad) return OR_INVALID;
-    for (int64_t i = 0; i < n; ++i) b->counts[b->bins[i]]++;
     return OR_OK;
 }
 
@@ -676,7 +699,7 @@ static int partition_plan_scatter(const or_ctx* x, const or_node_desc* nd, uint3
 static int transform_batch(const or_ctx* x, const or_node_desc* nd, uint32_t purpose, or_pool* p, batch* b,
                            uint32_t reserve_bound)
 {
-    gather(p, b);
+    gather(x, p, b);
     uint64_t R;
     or_ceil_div(b->particles, p->C, &R);
     if (b->particles == 0) return OR_PLAN_MISMATCH;
