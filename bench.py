@@ -72,11 +72,11 @@ def golden_cases():
         return None
 
 
-def pmc_summary(fetch_csv, write_csv, workload, out):
+def pmc_summary(fetch_csv, write_csv, workload, out, kernel="k_team", command=None):
     """Turn two rocprofv3 --pmc runs (FETCH_SIZE, then WRITE_SIZE; counter_collection.csv, KiB per
-    dispatch) of `bench.py --copy-calib 1024` into HBM bytes per k_team launch. Each counter is
-    scaled by what it reads for the calibration copy of known size (the last device copy of the
-    run: 1 GiB read and 1 GiB written)."""
+    dispatch) of a bench.py command with `--copy-calib 1024` into HBM bytes per launch of the
+    kernels whose name contains `kernel`. Each counter is scaled by what it reads for the
+    calibration copy of known size (the last device copy of the run: 1 GiB read and 1 GiB written)."""
     import csv
 
     def per_dispatch(path, counter):
@@ -88,12 +88,14 @@ def pmc_summary(fetch_csv, write_csv, workload, out):
             d[k] = d.get(k, 0.0) + float(r["Counter_Value"])
         return d
 
-    lines = ["# HBM traffic of the team engine kernel (k_team), from two rocprofv3 --pmc passes:",
+    cmd = command or "--only-headline --no-cpu --no-autotune --steps 1 --warmup 0"
+    lines = [f"# HBM traffic of {kernel}, from two rocprofv3 --pmc passes:",
              "#   cd /tmp && rocprofv3 --pmc FETCH_SIZE --output-format csv -d $OUT/pmc_fetch -o run -- "
-             "python $R/bench.py --only-headline --no-cpu --no-autotune --steps 1 --warmup 0 --copy-calib 1024",
+             f"python $R/bench.py {cmd} --copy-calib 1024",
              "#   (the same with --pmc WRITE_SIZE into $OUT/pmc_write; R = the repository root, OUT = an output directory)",
              "#   python bench.py --pmc-summary $OUT/pmc_fetch/run_counter_collection.csv "
-             "$OUT/pmc_write/run_counter_collection.csv <this file>",
+             f"$OUT/pmc_write/run_counter_collection.csv <this file> --pmc-kernel '{kernel}'"
+             + (f" --pmc-command '{command}'" if command else ""),
              f"workload={workload}"]
     per = {}
     for counter, path in (("FETCH_SIZE", fetch_csv), ("WRITE_SIZE", write_csv)):
@@ -101,10 +103,11 @@ def pmc_summary(fetch_csv, write_csv, workload, out):
         copies = [(k, v) for k, v in d.items() if "copyBuffer" in k[1]]
         cal_k, cal_v = max(copies, key=lambda kv: kv[0][0])
         factor = float(1 << 20) / cal_v  # KiB the 1 GiB copy moved / KiB the counter read
-        team = [v for k, v in sorted(d.items()) if "k_team" in k[1]]
-        per[counter] = [v * factor * 1024.0 for v in team]
+        mine = [v for k, v in sorted(d.items()) if kernel in k[1]]
+        per[counter] = [v * factor * 1024.0 for v in mine]
+        shown = ", ".join(f"{v:.1f}" for v in mine) if len(mine) <= 16 else f"{len(mine)} dispatches, {sum(mine):.1f} in total"
         lines.append(f"{counter}: calibration copy dispatch {cal_k[0]} read {cal_v:.1f} KiB for 1048576 KiB "
-                     f"-> factor {factor:.4f}; k_team dispatches (KiB): {', '.join(f'{v:.1f}' for v in team)}")
+                     f"-> factor {factor:.4f}; {kernel} dispatches (KiB): {shown}")
     launches = len(per["FETCH_SIZE"])
     total = sum(per["FETCH_SIZE"]) + sum(per["WRITE_SIZE"])
     lines.append(f"launches={launches}")
@@ -544,6 +547,9 @@ def main():
                          "scales rocprofv3 FETCH_SIZE / WRITE_SIZE readings to bytes)")
     ap.add_argument("--pmc-summary", nargs=3, metavar=("FETCH_CSV", "WRITE_CSV", "OUT"),
                     help="write the HBM traffic summary of two rocprofv3 --pmc runs and exit")
+    ap.add_argument("--pmc-kernel", default="k_team", help="--pmc-summary: kernels whose name contains this")
+    ap.add_argument("--pmc-command", default=None, help="--pmc-summary: the profiled bench.py arguments")
+    ap.add_argument("--pmc-workload", default=None, help="--pmc-summary: workload label (default: the headline's)")
     args = ap.parse_args()
 
     N, C, P, Q = int(args.items), args.capacity, args.live, args.batch
@@ -551,7 +557,8 @@ def main():
     # the HBM traffic of a launch depends on the workload, not on the team size
     wkey = f"particle_feedback N={N:.0e} C={C} P={P} Q={Q} seed=42 fuse={int(fuse)}"
     if args.pmc_summary:
-        pmc_summary(*args.pmc_summary[:2], wkey, args.pmc_summary[2])
+        pmc_summary(*args.pmc_summary[:2], args.pmc_workload or wkey, args.pmc_summary[2], args.pmc_kernel,
+                    args.pmc_command)
         return
 
     if args.gpus != 1:
