@@ -10,6 +10,7 @@ Beside the headline it reports (each at the size its workload states):
   - the roofline of the dominant kernel, with its launch durations taken from the timed runs
     (HIP events on the engine's stream);
   - the other routing distributions of the feedback network at N=1e8;
+  - the same network with 16M particles in flight on the wide engine (kernels at throughput);
   - the keyed hash pipeline at 1e8 keys, Q in {1, 4, 8, 16}, with in-flight batch lanes;
   - photon transport at 1e8 photons;
   - the served mixed stream of 4096 jobs (N log-uniform 1e3..1e7, 64 GB budget);
@@ -151,6 +152,49 @@ def distribution_forms(g, torch, N, C, P, Q, golden):
             f"equals golden: {rec.get('histogram_equals_golden')}")
         flow.close()
     return out
+
+
+# ---- the feedback network with millions of particles in flight -----------------------------------
+def wide_feedback_form(g, torch, N, C, live, golden, lanes=31, src_chunks=16, wg=2048, tile=1024):
+    """The headline network and seed with a live bound of `live` particles instead of 16384 (wide
+    engine, `lanes` batches in flight per node, source lanes of `src_chunks` chunks, Q=16): every
+    particle's trajectory is the headline's, so the histogram must equal the headline's fixture.
+    A step carries millions of partition passes, so the classify and partition-scatter kernels run
+    at their throughput instead of the team engine's step latency; their HIP-event times give
+    their bandwidth."""
+    Q = 16
+    S = live // C * 3 + 2 * lanes * (Q + 4) + 64
+    flow, src, counts = build_feedback(g, torch, C, live, Q, g.SCHEDULE_DEVICE, fuse=True, engine=1, team=wg,
+                                       tile_items=tile, inflight_batches=lanes, source_chunks=src_chunks,
+                                       num_chunks=S)
+    run_once(flow, src, counts, 2_000_000, torch)  # engine build, hipGraph capture
+    dt, st = run_once(flow, src, counts, N, torch)
+    rows = g.hist_rows(counts.data_ptr(), g.FEEDBACK_HIST_N, g.FEEDBACK_HIST_LO)
+    sha = g.rows_sha256(rows)
+    rec = {"value": N / dt, "unit": "particles/s", "items": N, "live": live, "Q": Q, "inflight_batches": lanes,
+           "source_chunks": src_chunks, "workgroups": wg, "engine_steps": int(st["steps"]), "engine": int(st["engine"]),
+           "histogram_sha256": sha}
+    if golden and N == golden["items"]:
+        rec["histogram_equals_golden"] = sha == golden["cases"]["parity"]["sha256"]
+    # kernel times (direct launches timed with HIP events)
+    flow.configure(g.Tuning(seed=42, batch_chunks=Q, chunk_capacity=C, live_particles=live, schedule=g.SCHEDULE_DEVICE,
+                            use_graph=2, fuse_transforms=True, device_engine=1, max_workgroups=wg, tile_items=tile,
+                            inflight_batches=lanes, source_chunks=src_chunks, num_chunks=S))
+    run_once(flow, src, counts, 2_000_000, torch)
+    run_once(flow, src, counts, N, torch)
+    kms, kn = flow.kernel_times()
+    part_items, _ = flow.item_passes()
+    rec["kernel_ms"] = {"classify": kms[0], "plan": kms[1], "partition_scatter": kms[4], "source": kms[5],
+                        "sink": kms[6], "retire_select": kms[7]}
+    rec["partition_passes"] = int(part_items)
+    rec["partition_scatter_gbs"] = 24.0 * part_items / (kms[4] * 1e-3) / 1e9
+    rec["classify_scatter_gbs"] = 24.0 * part_items / ((kms[0] + kms[4]) * 1e-3) / 1e9
+    rec["classify_scatter_hbm_frac"] = rec["classify_scatter_gbs"] / HBM_PEAK_GBS
+    log(f"[bench] feedback network, {live} particles in flight (wide engine) N={N}: {N / dt:.4g} particles/s, "
+        f"{int(st['steps'])} steps; partition scatter {rec['partition_scatter_gbs']:.0f} GB/s, classify+scatter "
+        f"{rec['classify_scatter_gbs']:.0f} GB/s; histogram equals golden: {rec.get('histogram_equals_golden')}")
+    flow.close()
+    return rec
 
 
 # ---- config 3: keyed hash pipeline ---------------------------------------------------------------
@@ -542,6 +586,8 @@ def main():
     ap.add_argument("--serve-rate", type=float, default=80.0, help="served mixed stream: Poisson arrivals per second")
     ap.add_argument("--serve-max-items", type=float, default=1e7, help="served mixed stream: N ~ log-uniform[1e3, this]")
     ap.add_argument("--no-distributions", action="store_true")
+    ap.add_argument("--wide-live", type=int, default=1 << 24,
+                    help="feedback network on the wide engine with this many particles in flight (0 = skip)")
     ap.add_argument("--copy-calib", type=float, default=0.0,
                     help="after the runs, copy this many MiB device to device once (the known-size kernel that "
                          "scales rocprofv3 FETCH_SIZE / WRITE_SIZE readings to bytes)")
@@ -586,6 +632,9 @@ def main():
             pflow.close()
         if not args.no_distributions:
             forms.update(distribution_forms(g, torch, N, C, P, Q, golden))
+        if args.wide_live > 0:
+            forms[f"feedback_wide_engine_live_{args.wide_live}"] = wide_feedback_form(g, torch, N, C, args.wide_live,
+                                                                                     golden)
         if args.hash_items > 0:
             hf, hash_sets = hash_forms(g, torch, int(args.hash_items), tuple(int(q) for q in args.hash_q.split(",")),
                                        args.hash_lanes, args.hash_live, args.hash_source_chunks, args.hash_tile)

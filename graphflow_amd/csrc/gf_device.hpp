@@ -551,6 +551,12 @@ __device__ inline uint32_t ltid() { return threadIdx.x & (uint32_t)(kThreads - 1
 constexpr uint32_t kNoTile = 0xFFFFFFFFu;
 __device__ inline uint64_t lanemask_lt() { return (1ull << lane_id()) - 1ull; }
 
+// Flat output position -> (chunk, slot) with a shift and a mask when the chunk capacity C is a power
+// of two (every workload's C), a division otherwise; `sh` = cap_shift(C), uniform per launch.
+__host__ __device__ inline uint32_t cap_shift(uint32_t C) { return (C & (C - 1u)) == 0u ? (uint32_t)__builtin_ctz(C) : 32u; }
+__device__ inline uint32_t cdiv(uint32_t f, uint32_t C, uint32_t sh) { return sh < 32u ? f >> sh : f / C; }
+__device__ inline uint32_t cmod(uint32_t f, uint32_t C, uint32_t sh) { return sh < 32u ? f & (C - 1u) : f % C; }
+
 // Lanes with the same bin (the "peers" of each lane); iterations = distinct bins in the wave.
 __device__ inline uint64_t peer_mask(bool valid, uint32_t bin)
 {
@@ -769,6 +775,7 @@ __device__ void scatter_tile_hash(const gf_node_desc& nd, const Batch& b, uint32
                                   const int32_t* fuse = nullptr, const RunParams* rp = nullptr,
                                   const uint32_t* bin_base = nullptr)
 {
+    const uint32_t csh = cap_shift(C);
     const uint32_t B = nd.bins, tid = ltid(), w = tid >> 6, lane = lane_id(), mask = E - 1u;
     uint32_t* keys = (uint32_t*)smem;
     uint32_t* wcnt = keys + E;            // [kWaves][E] running, then exclusive wave prefix
@@ -827,7 +834,7 @@ __device__ void scatter_tile_hash(const gf_node_desc& nd, const Batch& b, uint32
         f[r] = toff[k] + (bin_base ? bin_base[k] * C : 0u) + (bin[r] == 0xFFFFFFFFu ? 0u : wcnt[w * E + slot[r]]) + rank[r];
     }
 #pragma unroll
-    for (int r = 0; r < R; ++r) ci[r] = bin[r] == 0xFFFFFFFFu ? 0u : b.out[f[r] / C].index;
+    for (int r = 0; r < R; ++r) ci[r] = bin[r] == 0xFFFFFFFFu ? 0u : b.out[cdiv(f[r], C, csh)].index;
 #pragma unroll
     for (int r = 0; r < R; ++r) {
         if (bin[r] == 0xFFFFFFFFu) continue;
@@ -836,7 +843,7 @@ __device__ void scatter_tile_hash(const gf_node_desc& nd, const Batch& b, uint32
         const uint32_t u = update(nd, k, p);
         if (u) set_status(status, u);
         if (fuse && fuse[k] >= 0) transform(all[fuse[k] & 0xFFFF], *rp, (uint32_t)fuse[k] >> 16, p);
-        p.store(cols, (size_t)ci[r] * C + f[r] % C);
+        p.store(cols, (size_t)ci[r] * C + cmod(f[r], C, csh));
     }
     __syncthreads();
 }
@@ -848,7 +855,7 @@ __host__ __device__ inline size_t scatter_wcnt_bytes(uint32_t bins) { return ((s
 __host__ __device__ inline size_t scatter_smem(uint32_t bins, uint32_t tile_items, uint32_t rec_bytes, bool stage)
 {
     if (!stage) return scatter_wcnt_bytes(bins);
-    return scatter_wcnt_bytes(bins) + (size_t)bins * 8u + (size_t)tile_items * (rec_bytes + 4u);
+    return scatter_wcnt_bytes(bins) + (size_t)bins * 20u + (size_t)tile_items * (rec_bytes + 4u);
 }
 
 // Stable scatter of one tile (Execution.hpp:1096-1187 / :1327-1502 semantics with the canonical
@@ -869,6 +876,7 @@ __device__ void scatter_tile(const gf_node_desc& nd, const Batch& b, uint32_t t,
                              const int32_t* fuse = nullptr, const RunParams* rp = nullptr,
                              const uint32_t* bin_base = nullptr)
 {
+    const uint32_t csh = cap_shift(C);
     if constexpr (R > 4) {
         // tiles of more than 1024 items: never staged (the staged records would exceed 64 KB of LDS)
         scatter_tile_2p<P, R>(nd, b, t, cols, C, status, smem, all, fuse, rp, bin_base);
@@ -918,6 +926,12 @@ __device__ void scatter_tile(const gf_node_desc& nd, const Batch& b, uint32_t t,
     __syncthreads();
     const uint32_t* toff = b.counts + (size_t)(t == kNoTile ? 0u : t) * B;
     if constexpr (Stage) {
+        // With C >= the tile, a bin's run of the tile covers at most two output chunks: their
+        // indices (and the run's flat start) are loaded once per bin into LDS, and each staged item
+        // keeps its absolute slot, so no item waits on a load of the plan or of a descriptor.
+        uint32_t* tb = flat + b.tile_items;  // [B] flat start of the tile's run of each bin
+        uint32_t* cx = tb + B;               // [B][2] output chunk indices the run covers
+        const bool direct = C >= b.tile_items;
         if (tid == 0) {
             uint32_t acc = 0;
             for (uint32_t k = 0; k < B; ++k) {
@@ -925,6 +939,14 @@ __device__ void scatter_tile(const gf_node_desc& nd, const Batch& b, uint32_t t,
                 acc += ttot[k];
             }
         }
+        if (direct)
+            for (uint32_t k = tid; k < B; k += kThreads) {
+                if (!ttot[k]) continue;
+                const uint32_t f0 = toff[k] + (bin_base ? bin_base[k] * C : 0u), c0 = cdiv(f0, C, csh);
+                tb[k] = f0;
+                cx[2u * k] = b.out[c0].index;
+                cx[2u * k + 1u] = cdiv(f0 + ttot[k] - 1u, C, csh) > c0 ? b.out[c0 + 1u].index : 0u;
+            }
         __syncthreads();
 #pragma unroll
         for (int r = 0; r < R; ++r) {
@@ -935,13 +957,22 @@ __device__ void scatter_tile(const gf_node_desc& nd, const Batch& b, uint32_t t,
             if (fuse && fuse[k] >= 0) transform(all[fuse[k] & 0xFFFF], *rp, (uint32_t)fuse[k] >> 16, rec[r]);
             uint32_t p = tstart[k] + within;
             stage[p] = rec[r];
-            flat[p] = toff[k] + (bin_base ? bin_base[k] * C : 0u) + within;
+            if (direct) {
+                const uint32_t f = tb[k] + within;
+                flat[p] = cx[2u * k + (cdiv(f, C, csh) - cdiv(tb[k], C, csh))] * C + cmod(f, C, csh);  // slots stay below 2^32 (S * C)
+            } else {
+                flat[p] = toff[k] + (bin_base ? bin_base[k] * C : 0u) + within;
+            }
         }
         __syncthreads();
-        for (uint32_t j = tid; j < s.count; j += kThreads) {
-            uint32_t f = flat[j];
-            size_t slot = (size_t)b.out[f / C].index * C + f % C;
-            stage[j].store(cols, slot);
+        if (direct) {
+            for (uint32_t j = tid; j < s.count; j += kThreads) stage[j].store(cols, flat[j]);
+        } else {
+            for (uint32_t j = tid; j < s.count; j += kThreads) {
+                uint32_t f = flat[j];
+                size_t slot = (size_t)b.out[cdiv(f, C, csh)].index * C + cmod(f, C, csh);
+                stage[j].store(cols, slot);
+            }
         }
         __syncthreads();
     } else {
@@ -953,7 +984,7 @@ __device__ void scatter_tile(const gf_node_desc& nd, const Batch& b, uint32_t t,
             f[r] = toff[k] + (bin_base ? bin_base[k] * C : 0u) + (uint32_t)wcnt[w * B + k] + rank[r];
         }
 #pragma unroll
-        for (int r = 0; r < R; ++r) ci[r] = bin[r] == 0xFFFFFFFFu ? 0u : b.out[f[r] / C].index;
+        for (int r = 0; r < R; ++r) ci[r] = bin[r] == 0xFFFFFFFFu ? 0u : b.out[cdiv(f[r], C, csh)].index;
 #pragma unroll
         for (int r = 0; r < R; ++r) {
             if (bin[r] == 0xFFFFFFFFu) continue;
@@ -962,7 +993,7 @@ __device__ void scatter_tile(const gf_node_desc& nd, const Batch& b, uint32_t t,
             uint32_t u = update(nd, k, p);
             if (u) set_status(status, u);
             if (fuse && fuse[k] >= 0) transform(all[fuse[k] & 0xFFFF], *rp, (uint32_t)fuse[k] >> 16, p);
-            p.store(cols, (size_t)ci[r] * C + f[r] % C);
+            p.store(cols, (size_t)ci[r] * C + cmod(f[r], C, csh));
         }
         __syncthreads();
     }
@@ -977,6 +1008,7 @@ __device__ void scatter_tile_2p(const gf_node_desc& nd, const Batch& b, uint32_t
                                 uint32_t* status, char* smem, const gf_node_desc* all, const int32_t* fuse,
                                 const RunParams* rp, const uint32_t* bin_base)
 {
+    const uint32_t csh = cap_shift(C);
     static_assert(R % 4 == 0, "rounds in groups of four");
     uint32_t tid = ltid();
     // the round offsets derive from the thread index: made opaque here, so they are recomputed per
@@ -1035,7 +1067,7 @@ __device__ void scatter_tile_2p(const gf_node_desc& nd, const Batch& b, uint32_t
                 if (x != 0xFFFFFFFFu) rec[u] = P::load(cols, s.base + w * 64u * R + (uint32_t)(r0 + u) * 64u + lane);
             }
 #pragma unroll
-            for (int u = 0; u < 4; ++u) ci[u] = br[r0 + u] == 0xFFFFFFFFu ? 0u : b.out[f[u] / C].index;
+            for (int u = 0; u < 4; ++u) ci[u] = br[r0 + u] == 0xFFFFFFFFu ? 0u : b.out[cdiv(f[u], C, csh)].index;
 #pragma unroll
             for (int u = 0; u < 4; ++u) {
                 const uint32_t x = br[r0 + u];
@@ -1045,7 +1077,7 @@ __device__ void scatter_tile_2p(const gf_node_desc& nd, const Batch& b, uint32_t
                 const uint32_t st = update(nd, k, p);
                 if (st) set_status(status, st);
                 if (fuse && fuse[k] >= 0) transform(all[fuse[k] & 0xFFFF], *rp, (uint32_t)fuse[k] >> 16, p);
-                p.store(cols, (size_t)ci[u] * C + f[u] % C);
+                p.store(cols, (size_t)ci[u] * C + cmod(f[u], C, csh));
             }
             __builtin_amdgcn_sched_barrier(0);  // one group's records live at a time
         }
@@ -1061,13 +1093,14 @@ template <class P, int R>
 __device__ void transform_tile(const gf_node_desc& nd, const Batch& b, uint32_t t, const Cols& cols, uint32_t C,
                                const RunParams& rp)
 {
+    const uint32_t csh = cap_shift(C);
     TileSpan s = tile_span(b, b.in, b.n_in, t, C);
     const uint32_t pos0 = s.count ? b.item_prefix[s.chunk] + s.offset : 0u;
     for (uint32_t i = ltid(); i < s.count; i += kThreads) {
         P p = P::load(cols, s.base + i);
         transform(nd, rp, b.purpose, p);
         uint32_t g = pos0 + i;
-        p.store(cols, (size_t)b.out[g / C].index * C + g % C);
+        p.store(cols, (size_t)b.out[cdiv(g, C, csh)].index * C + cmod(g, C, csh));
     }
 }
 
@@ -1077,6 +1110,7 @@ template <class P>
 __device__ void source_tile(const gf_node_desc& nd, const Batch& b, uint32_t t, const Cols& cols, uint32_t C,
                             const RunParams& rp)
 {
+    const uint32_t csh = cap_shift(C);
     if (t == kNoTile) return;
     uint32_t off = t * b.tile_items;
     uint32_t count = b.particles - off < b.tile_items ? b.particles - off : b.tile_items;
@@ -1101,7 +1135,7 @@ __device__ void source_tile(const gf_node_desc& nd, const Batch& b, uint32_t t, 
                 Keyed p;
                 p.key = key[q];
                 p.val = b.first_id + g;
-                p.store(cols, (size_t)b.out[g / C].index * C + g % C);
+                p.store(cols, (size_t)b.out[cdiv(g, C, csh)].index * C + cmod(g, C, csh));
             }
         }
     } else {
@@ -1109,7 +1143,7 @@ __device__ void source_tile(const gf_node_desc& nd, const Batch& b, uint32_t t, 
             P p;
             const uint32_t g = off + i;
             source(nd, rp, b.purpose, b.first_id + g, p);
-            p.store(cols, (size_t)b.out[g / C].index * C + g % C);
+            p.store(cols, (size_t)b.out[cdiv(g, C, csh)].index * C + cmod(g, C, csh));
         }
     }
 }
