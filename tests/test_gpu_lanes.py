@@ -59,13 +59,15 @@ def test_hash_pipeline_lanes_trace(depth, chunks):
     assert (ms[:, 0] == ref["multiset"]).all() and int(ms[:, 1].sum()) == N == st["sunk"]
 
 
-@pytest.mark.parametrize("chunks,depth", [(2, 1), (8, 2)])
+@pytest.mark.parametrize("chunks,depth,N,P", [(2, 1, 30011, 4096), (8, 2, 30011, 4096), (4, 2, 5000, 300),
+                                              (8, 2, 4097, 8192)])
 @pytest.mark.parametrize("engine", [1, 2])
-def test_source_chunks_trace(chunks, depth, engine):
+def test_source_chunks_trace(chunks, depth, N, P, engine):
     """Source lanes admitting several chunks per step (wide and one-workgroup engines): every batch
-    record and every final weight equals the oracle's step schedule with the same setting."""
+    record and every final weight equals the oracle's step schedule with the same setting, also
+    with a live bound below C and a partial tail chunk."""
     torch = torch_cuda()
-    N, C, P, Q = 30011, 512, 4096, 4
+    C, Q = 512, 4
     flow, src, rec = feedback_flow(N)
     stats, trace = run_flow(flow, src, N, C=C, P=P, Q=Q, schedule=g.SCHEDULE_DEVICE, fuse=True, engine=engine,
                             inflight_batches=depth, source_chunks=chunks)

@@ -148,3 +148,22 @@ def decode_outputs(tr):
         o0 = i + 5 + 3 * n_in + nb
         yield node, step, n_in, tr[i + 4 + 3 * n_in:o0 - 1], tr[o0:o0 + 4 * R].reshape(-1, 4)
         i = o0 + 4 * R
+
+
+@pytest.mark.parametrize("n,C,P,chunks", [(5000, 512, 300, 4), (4097, 1024, 8192, 8), (1, 512, 4096, 4)])
+def test_source_chunks_edges(n, C, P, chunks):
+    # a live bound below C (one partial chunk per admission), N not a multiple of C (a partial tail
+    # chunk), a one-item run: trajectories unchanged, every lane's chunks full but the last
+    kw = dict(seed=42, items=n, capacity=C, live=P, batch=4, mode=O.MODE_STEP, record=True, fuse=True,
+              trace_cap=1 << 23, inflight=2)
+    r = O.run(FEEDBACK, source_chunks=chunks, **kw)
+    assert r["status"] == O.OK, r["message"]
+    w, _ = O.feedback_scalar(42, n)
+    assert (r["record"] == w).all() and r["sunk"] == n
+    admitted = 0
+    for node, step, n_in, counts, outs in decode_outputs(r["trace"]):
+        if node != 0:
+            continue
+        assert all(f == C for f in outs[:-1, 2])
+        admitted += int(outs[:, 2].sum())
+    assert admitted == n
