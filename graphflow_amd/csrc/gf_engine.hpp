@@ -97,6 +97,9 @@ struct Engine {
 
 enum { PROF_COUNT = 0, PROF_PLAN = 1, PROF_SOURCE = 2, PROF_PART = 3, PROF_XF = 4, PROF_SINK = 5, PROF_RETIRE = 6,
        PROF_SELECT = 7, PROF_STEPS = 8,
+       // the select's phases (staging, decisions, input lists, prefixes and termination) and the
+       // retire counters before it
+       PROF_SEL_STAGE = 9, PROF_SEL_DECIDE = 10, PROF_SEL_LISTS = 11, PROF_SEL_END = 12, PROF_SEL_RETIRE = 13,
        // wide engine scatter kernel: workgroup wall-clock ticks and tiles per KIND_*, then the
        // one-wave sink tiles (ticks summed over waves)
        PROF_KB_TICKS = 16, PROF_KB_TILES = 22, PROF_KB_WAVE_TICKS = 28, PROF_KB_WAVE_TILES = 29,
@@ -733,6 +736,14 @@ __device__ inline void engine_retire(const Engine& e, SelLds& L)
 __device__ inline void engine_select(const Engine& e, SelLds& L, bool lists = true)
 {
     const uint32_t tid = threadIdx.x, n = e.n_nodes, C = e.C, Q = e.Q;
+    // optional profile (use_graph = 3): wall-clock ticks of the select's phases
+    unsigned long long pt = e.prof && tid == 0u ? wall_clock64() : 0ull;
+    auto pmark = [&](uint32_t slot) {
+        if (!e.prof || tid != 0u) return;
+        const unsigned long long now = wall_clock64();
+        e.prof[slot] += now - pt;
+        pt = now;
+    };
     load_lane_tab(e, L);
     __syncthreads();
     // stage the candidate ready descriptors of every transform's ring in LDS: <= Q per lane (the
@@ -780,6 +791,7 @@ __device__ inline void engine_select(const Engine& e, SelLds& L, bool lists = tr
         if (tid == 0) L.fpre[L.staged] = sum;
     }
     __syncthreads();
+    pmark(PROF_SEL_STAGE);
     // decisions (GraphRunner's order: node order, every lane after the earlier ones). A lane depends
     // on the earlier lanes only through the credits they reserved, the candidates its node's earlier
     // lanes took and, at the source, the items admitted before it. When the step's whole demand fits
@@ -788,31 +800,44 @@ __device__ inline void engine_select(const Engine& e, SelLds& L, bool lists = tr
     // checks the demand against the free stack); otherwise one thread walks the lanes in node order.
     bool serial = L.overflow != 0u;
     if (!serial) {
-        if (tid < n && L.kind[tid] == KIND_SOURCE && L.ring_of[tid] == tid) {
-            // the source of one job: its lanes admit in turn against the job's items and live bound
-            const uint32_t jb = L.job[tid];
-            const unsigned long long N = L.jN[jb], Pj = L.jP[jb], sunk = L.jsunk[jb];
-            unsigned long long gen = L.jgen[jb];
-            const uint32_t T = e.arr[tid].tile_items;
-            for (uint32_t j = 0; j < L.lanes[tid]; ++j) {
-                // up to src_chunks chunks, each by the admission rule; only the last can be partial,
-                // so the lane's items are a dense concatenation of its chunks
-                const unsigned long long first = gen;
-                for (uint32_t c = 0; c < e.src_chunks; ++c) {
-                    const unsigned long long live = gen - sunk, remaining = N - gen;
-                    unsigned long long min_fill = C < Pj ? C : Pj;
-                    if (remaining < min_fill) min_fill = remaining;
-                    if (!(gen < N && live < Pj && (Pj - live) >= min_fill)) break;
-                    unsigned long long f = remaining;
-                    if (f > Pj - live) f = Pj - live;
-                    if (f > C) f = C;
-                    gen += f;
+        if (tid < n && L.kind[tid] == KIND_SOURCE) {
+            // the source of one job: its lanes admit in turn, up to src_chunks chunks each, against
+            // the job's items and live bound. The node's admissions in closed form: while at least
+            // C items remain and the live bound leaves room for C, every chunk is full; then at most
+            // one partial chunk (the tail of N, or a live bound below C). Lane j takes chunks
+            // [j K, (j + 1) K) of that sequence; a thread per lane.
+            const uint32_t l = L.ring_of[tid], j = tid - l, jb = L.job[tid];
+            const unsigned long long N = L.jN[jb], Pj = L.jP[jb], gen = L.jgen[jb];
+            const unsigned long long live = gen - L.jsunk[jb], rem = N - gen;
+            const unsigned long long cap = (unsigned long long)L.lanes[l] * e.src_chunks;
+            unsigned long long full = 0ull, tail = 0ull;  // full chunks, then the items of a partial one
+            if (gen < N && live < Pj) {
+                if (Pj >= C) {
+                    full = (Pj - live) / C;
+                    if (rem / C < full) full = rem / C;
+                    if (cap < full) full = cap;
+                    const unsigned long long rem1 = rem - full * C, live1 = live + full * C;
+                    if (full < cap && rem1 > 0ull && rem1 < C && live1 < Pj && Pj - live1 >= rem1) tail = rem1;
+                } else if (rem >= Pj) {
+                    if (live == 0ull) tail = Pj;  // one chunk of P items
+                } else if (Pj - live >= rem) {
+                    tail = rem;
                 }
-                const uint32_t fill = (uint32_t)(gen - first);
-                L.dfill[tid + j] = fill;
-                L.dtiles[tid + j] = (fill + T - 1u) / T;
-                L.dfirst[tid + j] = first;
             }
+            const unsigned long long total = full + (tail ? 1ull : 0ull);
+            const unsigned long long c0 = (unsigned long long)j * e.src_chunks;
+            unsigned long long c1 = c0 + e.src_chunks;
+            if (c1 > total) c1 = total;
+            unsigned long long fill = 0ull;
+            if (c1 > c0) {
+                const unsigned long long fc = (c1 < full ? c1 : full) - (c0 < full ? c0 : full);  // full chunks of the lane
+                fill = fc * C + (tail && c1 == total ? tail : 0ull);
+            }
+            const uint32_t T = e.arr[tid].tile_items;
+            L.dfill[tid] = (uint32_t)fill;
+            L.dtiles[tid] = (uint32_t)((fill + T - 1u) / T);
+            // an empty lane's first id is the end of the node's admissions (the job's next id)
+            L.dfirst[tid] = c1 > c0 ? gen + (c0 < full ? c0 : full) * C : gen + full * C + tail;
         }
         __syncthreads();
         unsigned long long dem[1] = {0ull};
@@ -949,6 +974,7 @@ __device__ inline void engine_select(const Engine& e, SelLds& L, bool lists = tr
         }
     }
     __syncthreads();
+    pmark(PROF_SEL_DECIDE);
     // input lists and tile / item prefixes of every active non-source node
     for (uint32_t k = 0; k < n && lists; ++k) {
         NodeSt& s = L.st[k];
@@ -999,6 +1025,7 @@ __device__ inline void engine_select(const Engine& e, SelLds& L, bool lists = tr
         }
         __syncthreads();
     }
+    pmark(PROF_SEL_LISTS);
     // work prefixes, ring cursors, termination: a thread per lane
     __syncthreads();
     unsigned long long busy[2] = {0ull, 0ull};  // lanes with ready chunks, jobs not fully admitted
@@ -1070,6 +1097,7 @@ __device__ inline void engine_select(const Engine& e, SelLds& L, bool lists = tr
     __syncthreads();
     store_states(e, L.st);
     __syncthreads();
+    pmark(PROF_SEL_END);
 }
 
 // Node owning work item t: largest k < n with prefix[k] <= t (prefix has n+1 entries).

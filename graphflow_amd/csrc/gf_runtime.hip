@@ -415,7 +415,9 @@ __global__ __launch_bounds__(1024) void k_engine_kr(Engine e)
     __shared__ uint32_t flag;
     if (engine_stopped(e, &flag)) return;
     SelLds& L = *reinterpret_cast<SelLds*>(smem);
+    const unsigned long long t0 = e.prof && threadIdx.x == 0u ? wall_clock64() : 0ull;
     engine_retire_counters(e, L);
+    if (e.prof && threadIdx.x == 0u) e.prof[PROF_SEL_RETIRE] += wall_clock64() - t0;
     engine_select(e, L, false);
 }
 
