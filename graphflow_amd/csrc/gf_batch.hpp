@@ -29,6 +29,8 @@ __host__ __device__ inline uint64_t fused_scratch_words(uint32_t n, uint32_t bin
     return 2ull * (n + 1u) + 2ull * bins + (uint64_t)tiles * bins;
 }
 
+constexpr uint32_t kArgRes = 64u;
+
 struct FusedArgs {
     gf_node_desc nd;
     const gf_chunk_desc* in;          // n input chunks (device)
@@ -39,6 +41,11 @@ struct FusedArgs {
     uint32_t* scratch;                // fused_scratch_words(n, bins, tiles) words (device)
     uint32_t* result;                 // [0] R, [1] status
     uint32_t group_lds, tiles_cap;    // LDS bytes per tile group; most tiles the scratch holds
+    // inl = 1: the inputs and the reservation travel in the launch itself (n <= kArgIn,
+    // n_reserved <= kArgRes), read in the kernel-argument segment like a plain stage's batch
+    uint32_t inl, pad;
+    gf_chunk_desc in_arg[kArgIn];
+    gf_chunk_handle res_arg[kArgRes];
 };
 
 template <class P, int R>
@@ -52,13 +59,15 @@ __device__ void fused_partition_batch(const FusedArgs& a, const Cols& cols, char
     uint32_t* totals = item_prefix + (n + 1u);  // [B] totals, then [B] bin bases
     uint32_t* counts = totals + 2u * B;         // [tile][bin]
     char* gsm = smem + (size_t)g * a.group_lds;
+    const gf_chunk_desc* in = a.inl ? a.in_arg : a.in;
+    const gf_chunk_handle* reserved = a.inl ? a.res_arg : a.reserved;
     // 1. prefixes (n <= Q: one thread)
     if (tid == 0) {
         uint32_t t = 0, p = 0, bad = 0;
         for (uint32_t c = 0; c < n; ++c) {
             tile_prefix[c] = t;
             item_prefix[c] = p;
-            const uint32_t f = a.in[c].fill;
+            const uint32_t f = in[c].fill;
             if (f > C) bad = 1u;
             t += (f + a.tile_items - 1u) / a.tile_items;
             p += f;
@@ -87,7 +96,7 @@ __device__ void fused_partition_batch(const FusedArgs& a, const Cols& cols, char
     b.purpose = a.purpose;
     b.first_id = 0ull;
     b.digest_base = 0ull;
-    b.in = const_cast<gf_chunk_desc*>(a.in);
+    b.in = const_cast<gf_chunk_desc*>(in);
     b.tile_prefix = tile_prefix;
     b.item_prefix = item_prefix;
     b.out = a.out;
@@ -137,7 +146,7 @@ __device__ void fused_partition_batch(const FusedArgs& a, const Cols& cols, char
     for (uint32_t bin = tid; bin < B; bin += blockDim.x) {
         const uint32_t b0 = totals[B + bin];
         for (uint32_t q = 0, rem = totals[bin]; rem; ++q) {
-            const gf_chunk_handle h = a.reserved[b0 + q];
+            const gf_chunk_handle h = reserved[b0 + q];
             gf_chunk_desc d;
             d.index = h.index;
             d.generation = h.generation;

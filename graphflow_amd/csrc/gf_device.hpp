@@ -235,6 +235,41 @@ struct Batch {
     uint32_t* totals;      // [bins]
 };
 
+// A plain stage's batch as a kernel argument. A small batch travels in the launch itself: the
+// record and copies of the arrays the kernel only reads (inputs and prefixes, output descriptors,
+// planned offsets), so the launch needs no upload. Otherwise `block` names the uploaded record.
+constexpr uint32_t kArgIn = 16u, kArgOut = 32u, kArgCounts = 128u;
+enum : uint32_t { ARG_IN = 1u, ARG_OUT = 2u, ARG_COUNTS = 4u };
+
+struct BatchArg {
+    const Batch* block;  // the record in device memory, or null: `b` and the arrays flagged in `inl`
+    Batch b;
+    uint32_t inl, pad[3];
+    gf_chunk_desc in[kArgIn];
+    gf_chunk_desc out[kArgOut];
+    uint32_t prefix[2u * (kArgIn + 1u)];  // tile prefix, then item prefix
+    uint32_t counts[kArgCounts];
+};
+
+// A plain-stage kernel takes its BatchArg as the first parameter, which the code-object ABI places at
+// offset 0 of the kernel-argument segment; the batch is read there, so that the arrays it carries are
+// addressed in the argument segment itself (a pointer to the parameter object would have the compiler
+// copy the whole argument into per-lane scratch memory).
+__device__ inline Batch batch_arg()
+{
+    const BatchArg& a = *(const BatchArg*)__builtin_amdgcn_kernarg_segment_ptr();
+    if (a.block) return *a.block;
+    Batch b = a.b;
+    if (a.inl & ARG_IN) {
+        b.in = const_cast<gf_chunk_desc*>(a.in);
+        b.tile_prefix = const_cast<uint32_t*>(a.prefix);
+        b.item_prefix = b.tile_prefix + (kArgIn + 1u);
+    }
+    if (a.inl & ARG_OUT) b.out = const_cast<gf_chunk_desc*>(a.out);
+    if (a.inl & ARG_COUNTS) b.counts = const_cast<uint32_t*>(a.counts);
+    return b;
+}
+
 __device__ inline void set_status(uint32_t* status, uint32_t code) { atomicCAS(status, 0u, code); }
 
 __device__ inline uint32_t route_of(uint32_t dist, uint32_t ndest, uint32_t h)

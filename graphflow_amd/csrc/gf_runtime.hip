@@ -164,56 +164,57 @@ std::vector<uint32_t> column_bytes(uint32_t payload)
 // Kernels
 // ==============================================================================================
 template <class P, int R>
-__global__ __launch_bounds__(kThreads) void k_count(gf_node_desc nd, const Batch* bp, Cols cols, uint32_t C,
+__global__ __launch_bounds__(kThreads) void k_count(const BatchArg, gf_node_desc nd, Cols cols, uint32_t C,
                                                     uint32_t* status, uint32_t hash_entries)
 {
     extern __shared__ __align__(16) char smem[];
-    const Batch b = *bp;
+    const Batch b = batch_arg();
     if (hash_entries) count_tile_hash<P, R>(nd, b, blockIdx.x, cols, C, status, smem, hash_entries);
     else count_tile<P, R>(nd, b, blockIdx.x, cols, C, status, smem);
 }
 
 template <class P, int R, bool Stage>
-__global__ __launch_bounds__(kThreads) void k_scatter(gf_node_desc nd, const Batch* bp, Cols cols, uint32_t C,
+__global__ __launch_bounds__(kThreads) void k_scatter(const BatchArg, gf_node_desc nd, Cols cols, uint32_t C,
                                                       uint32_t* status, uint32_t hash_entries)
 {
     extern __shared__ __align__(16) char smem[];
-    const Batch b = *bp;
+    const Batch b = batch_arg();
     if (hash_entries) scatter_tile_hash<P, R>(nd, b, blockIdx.x, cols, C, status, smem, hash_entries);
     else scatter_tile<P, R, Stage>(nd, b, blockIdx.x, cols, C, status, smem);
 }
 
 template <class P, int R>
-__global__ __launch_bounds__(kThreads) void k_transform(gf_node_desc nd, const Batch* bp, Cols cols, uint32_t C,
+__global__ __launch_bounds__(kThreads) void k_transform(const BatchArg, gf_node_desc nd, Cols cols, uint32_t C,
                                                         RunParams rp)
 {
-    const Batch b = *bp;
+    const Batch b = batch_arg();
     transform_tile<P, R>(nd, b, blockIdx.x, cols, C, rp);
 }
 
 template <class P>
-__global__ __launch_bounds__(kThreads) void k_source(gf_node_desc nd, const Batch* bp, Cols cols, uint32_t C,
+__global__ __launch_bounds__(kThreads) void k_source(const BatchArg, gf_node_desc nd, Cols cols, uint32_t C,
                                                      RunParams rp)
 {
-    const Batch b = *bp;
+    const Batch b = batch_arg();
     source_tile<P>(nd, b, blockIdx.x, cols, C, rp);
 }
 
 template <class P>
-__global__ __launch_bounds__(kThreads) void k_sink(gf_node_desc nd, const Batch* bp, Cols cols, uint32_t C,
+__global__ __launch_bounds__(kThreads) void k_sink(const BatchArg, gf_node_desc nd, Cols cols, uint32_t C,
                                                    uint32_t* status)
 {
     extern __shared__ __align__(16) char smem[];
-    const Batch b = *bp;
+    const Batch b = batch_arg();
     sink_tile<P>(nd, b, blockIdx.x, cols, C, status, smem);
 }
 
-// One partition batch in one launch (gf_batch.hpp).
+// One partition batch in one launch (gf_batch.hpp). FusedArgs is the first parameter: read in the
+// kernel-argument segment (offset 0), where the inputs and reservation of a small batch travel.
 template <class P, int R>
-__global__ __launch_bounds__(1024) void k_batch_fused(FusedArgs a, Cols cols)
+__global__ __launch_bounds__(1024) void k_batch_fused(FusedArgs, Cols cols)
 {
     extern __shared__ __align__(16) char smem[];
-    fused_partition_batch<P, R>(a, cols, smem);
+    fused_partition_batch<P, R>(*(const FusedArgs*)__builtin_amdgcn_kernarg_segment_ptr(), cols, smem);
 }
 
 // Tile descriptors of a batch as the device computes them (TileDescriptor list).
@@ -858,6 +859,13 @@ struct gf_stage {
     uint64_t seed = 0;
     RunParams rp{};
     DevMem mem;
+    // One device block per stage with a pinned host mirror of the same layout, so that a launch
+    // uploads its batch in one copy and reads its status (and classify counts) back in one:
+    //   [Batch | in (cap_in), reserve handles (fused) | tile prefix | item prefix | out (cap_out)
+    //    | status (16 B) | counts (cap_tiles x bins)]
+    uint8_t* d_blk = nullptr;
+    uint8_t* h_blk = nullptr;
+    size_t o_in = 0, o_tp = 0, o_out = 0, o_status = 0, o_counts = 0;
     Batch* d_batch = nullptr;
     Batch* h_batch = nullptr;
     uint32_t* d_status = nullptr;
@@ -867,7 +875,22 @@ struct gf_stage {
     gf_chunk_desc* h_out = nullptr;
     uint32_t* h_prefix = nullptr;
     double* d_zipf = nullptr;
-    hipEvent_t ev = nullptr;
+    hipEvent_t ev = nullptr;  // completion of the launch that uses the current mirror
+    // The host runner's form (lazy): completion is awaited only where the host needs device results
+    // (a classify's counts) or reuses a pinned mirror. Source, transform, sink and scatter launches
+    // run ahead on the runner's one stream, where the order of launches keeps every chunk's writer
+    // ahead of its readers; sink and scatter failures are read from the stage's status word at the
+    // end of the run. Two mirrors alternate, each with its completion event.
+    struct Mirror {
+        uint8_t* h = nullptr;
+        hipEvent_t ev = nullptr;
+        bool inflight = false;
+    };
+    Mirror mir[2];
+    uint32_t cur = 0;
+    bool lazy = false;
+    BatchArg ba{};  // the current launch's batch argument
+    bool uploaded = false;
     std::vector<gf_chunk_desc> in, pending;
     std::vector<uint32_t> totals;
     uint32_t stage_no = 0, n_tiles = 0, particles = 0;
@@ -876,20 +899,24 @@ struct gf_stage {
     uint64_t launches = 0;
     uint32_t bin_path = GF_BIN_PATH_AUTO, hash_entries = 0, stall_ms = 0, hash_E = 0;
     size_t lds_count = 0, lds_scatter = 0;
-    // fused single-launch form (gf_transform_batch)
-    gf_chunk_handle* d_res = nullptr;
-    gf_chunk_handle* h_res = nullptr;
+    // fused single-launch form (gf_transform_batch): outputs end where the result words begin, and
+    // the scratch (prefixes, totals, bin bases, rows) follows them, so that one copy reads back the
+    // used outputs, {R, status}, the prefixes and the bin totals:
+    //   [out (cap_out) | result (16 B) | scratch]
+    uint8_t* d_fblk = nullptr;
+    uint8_t* h_fblk = nullptr;
+    size_t o_fres = 0;
     uint32_t* d_result = nullptr;
-    uint32_t* h_result = nullptr;
     uint32_t* d_fscratch = nullptr;
-    uint32_t* h_totals = nullptr;
+    uint32_t f_n = 0, f_bound = 0;
     std::vector<gf_chunk_handle> reserved;
     bool fused = false, collected = false;
     std::string err;
 
     ~gf_stage()
     {
-        if (ev) (void)hipEventDestroy(ev);
+        for (auto& m : mir)
+            if (m.ev) (void)hipEventDestroy(m.ev);
     }
 
     int kind() const
@@ -934,32 +961,41 @@ struct gf_stage {
         cap_out = nd.role == GF_ROLE_SOURCE ? 1u : nd.role == GF_ROLE_SINK ? 1u : reserve_bound_of(nd, p->C, Q);
         reserve_bound = cap_out;
         cap_tiles = (uint32_t)(ceil_div(p->C, tile_items) * cap_in);
-        d_batch = mem.d<Batch>(1);
-        h_batch = mem.h<Batch>(1);
-        std::memset(h_batch, 0, sizeof(Batch));
-        h_batch->in = mem.d<gf_chunk_desc>(cap_in);
-        h_batch->tile_prefix = mem.d<uint32_t>(cap_in + 1u);
-        h_batch->item_prefix = mem.d<uint32_t>(cap_in + 1u);
-        h_batch->out = mem.d<gf_chunk_desc>(cap_out);
-        h_batch->counts = mem.d<uint32_t>((size_t)cap_tiles * bins);
+        auto al = [](size_t x) { return (x + 15u) & ~(size_t)15u; };
+        const size_t res_bytes = kind() == 1 ? sizeof(gf_chunk_handle) * cap_out : 0u;
+        o_in = al(sizeof(Batch));
+        o_tp = al(o_in + sizeof(gf_chunk_desc) * cap_in + res_bytes);
+        o_out = al(o_tp + 2u * sizeof(uint32_t) * (cap_in + 1u));
+        o_status = al(o_out + sizeof(gf_chunk_desc) * cap_out);
+        o_counts = o_status + 16u;
+        const size_t blk = al(o_counts + sizeof(uint32_t) * (size_t)cap_tiles * bins);
+        d_blk = mem.d<uint8_t>(blk);
+        for (auto& m : mir) {
+            m.h = mem.h<uint8_t>(blk);
+            std::memset(m.h, 0, blk);
+            HIPX(hipEventCreateWithFlags(&m.ev, hipEventDisableTiming));
+        }
+        d_batch = (Batch*)d_blk;
+        h_batch = (Batch*)mir[0].h;
+        h_batch->in = (gf_chunk_desc*)(d_blk + o_in);
+        h_batch->tile_prefix = (uint32_t*)(d_blk + o_tp);
+        h_batch->item_prefix = h_batch->tile_prefix + (cap_in + 1u);
+        h_batch->out = (gf_chunk_desc*)(d_blk + o_out);
+        h_batch->counts = (uint32_t*)(d_blk + o_counts);
         h_batch->totals = mem.d<uint32_t>(2u * bins);
         h_batch->tile_items = tile_items;
         h_batch->purpose = 2u * key;
-        d_status = mem.d<uint32_t>(1);
-        h_status = mem.h<uint32_t>(1);
-        h_counts = mem.h<uint32_t>((size_t)cap_tiles * bins);
-        h_in = mem.h<gf_chunk_desc>(cap_in);
-        h_out = mem.h<gf_chunk_desc>(cap_out);
-        h_prefix = mem.h<uint32_t>(2u * (cap_in + 1u));
+        std::memcpy(mir[1].h, mir[0].h, sizeof(Batch));
+        d_status = (uint32_t*)(d_blk + o_status);
+        HIPX(hipMemset(d_status, 0, 16u));
+        use_mirror(0);
         if (kind() == 1) {
-            d_res = mem.d<gf_chunk_handle>(cap_out);
-            h_res = mem.h<gf_chunk_handle>(cap_out);
-            d_result = mem.d<uint32_t>(2);
-            h_result = mem.h<uint32_t>(2);
-            d_fscratch = mem.d<uint32_t>(fused_scratch_words(cap_in, bins, cap_tiles));
-            h_totals = mem.h<uint32_t>(bins);
+            o_fres = al(sizeof(gf_chunk_desc) * cap_out);
+            d_fblk = mem.d<uint8_t>(o_fres + 16u + sizeof(uint32_t) * fused_scratch_words(cap_in, bins, cap_tiles));
+            h_fblk = mem.h<uint8_t>(o_fres + 16u + sizeof(uint32_t) * (2u * (cap_in + 1u) + bins));
+            d_result = (uint32_t*)(d_fblk + o_fres);
+            d_fscratch = (uint32_t*)(d_fblk + o_fres + 16u);
         }
-        HIPX(hipEventCreateWithFlags(&ev, hipEventDisableTiming));
         rp.seed = seed;
         rp.C = p->C;
         if (nd.op == GF_SRC_KEYED_ZIPF) {
@@ -978,17 +1014,43 @@ struct gf_stage {
         }
     }
 
+    void use_mirror(uint32_t i)
+    {
+        cur = i;
+        h_blk = mir[i].h;
+        ev = mir[i].ev;
+        h_batch = (Batch*)h_blk;
+        h_status = (uint32_t*)(h_blk + o_status);
+        h_counts = (uint32_t*)(h_blk + o_counts);
+        h_in = (gf_chunk_desc*)(h_blk + o_in);
+        h_out = (gf_chunk_desc*)(h_blk + o_out);
+        h_prefix = (uint32_t*)(h_blk + o_tp);
+    }
+
+    // Mark the end of the current launch on the stream.
+    void mark(hipStream_t st)
+    {
+        HIPX(hipEventRecord(ev, st));
+        mir[cur].inflight = true;
+    }
+
     // Wait for the launched device stage. With a stall timeout (Tuning::stallTimeout,
     // GraphRunner.hpp:1742) a stage that has not completed in time fails the run with GF_STALL.
     void wait_stage()
     {
+        wait_event(ev);
+        mir[cur].inflight = false;
+    }
+
+    void wait_event(hipEvent_t e)
+    {
         if (!stall_ms) {
-            HIPX(hipEventSynchronize(ev));
+            HIPX(hipEventSynchronize(e));
             return;
         }
         const auto t0 = std::chrono::steady_clock::now();
         for (;;) {
-            const hipError_t q = hipEventQuery(ev);
+            const hipError_t q = hipEventQuery(e);
             if (q == hipSuccess) return;
             if (q != hipErrorNotReady) HIPX(q);
             if (std::chrono::steady_clock::now() - t0 > std::chrono::milliseconds(stall_ms))
@@ -1002,24 +1064,77 @@ struct gf_stage {
         if (failed) fail(GF_STAGE_FAILED, "graphFlow: stage is single-use after a failed batch");
     }
 
-    void upload_batch(hipStream_t st, uint32_t n_out)
+    // Complete mirror i's launch.
+    void settle(uint32_t i)
     {
-        HIPX(hipMemcpyAsync(h_batch->in, h_in, sizeof(gf_chunk_desc) * std::max<uint32_t>(h_batch->n_in, 1u),
-                            hipMemcpyHostToDevice, st));
-        HIPX(hipMemcpyAsync(h_batch->tile_prefix, h_prefix, sizeof(uint32_t) * (h_batch->n_in + 1u),
-                            hipMemcpyHostToDevice, st));
-        HIPX(hipMemcpyAsync(h_batch->item_prefix, h_prefix + cap_in + 1u, sizeof(uint32_t) * (h_batch->n_in + 1u),
-                            hipMemcpyHostToDevice, st));
-        if (n_out)
-            HIPX(hipMemcpyAsync(h_batch->out, h_out, sizeof(gf_chunk_desc) * n_out, hipMemcpyHostToDevice, st));
-        HIPX(hipMemcpyAsync(d_batch, h_batch, sizeof(Batch), hipMemcpyHostToDevice, st));
+        Mirror& m = mir[i];
+        if (!m.inflight) return;
+        wait_event(m.ev);
+        m.inflight = false;
     }
 
-    void reset_status(hipStream_t st) { HIPX(hipMemsetAsync(d_status, 0, sizeof(uint32_t), st)); }
-
-    void check_status(const char* what)
+    // Lazy form: before a launch writes a mirror, the launch that used it last is complete.
+    void prepare_launch()
     {
-        uint32_t s = *h_status;
+        if (!lazy) return;
+        settle(cur ^ 1u);
+        use_mirror(cur ^ 1u);
+    }
+
+    // Lazy form, end of a run: every launch complete, and the status of sink and scatter launches
+    // (a classify's is checked when its counts are read) checked.
+    void settle_all(hipStream_t st)
+    {
+        settle(cur ^ 1u);
+        settle(cur);
+        if (lazy && (kind() == 1 || kind() == 3)) {
+            HIPX(hipMemcpyAsync(h_status, d_status, sizeof(uint32_t), hipMemcpyDeviceToHost, st));
+            HIPX(hipStreamSynchronize(st));
+            check_status(kind() == 3 ? "sink" : "partition scatter");
+        }
+    }
+
+    // The launch's batch: in the kernel arguments when the host runner launches a batch that fits,
+    // else uploaded (record, inputs, prefixes, the first n_out output descriptors, the first
+    // count_words planned offsets). The status word is never uploaded: cleared once, it stays zero
+    // until a kernel reports a failure, after which the stage takes no further batch. `uploaded`
+    // tells whether the launch reads the current mirror.
+    void stage_batch(hipStream_t st, uint32_t n_out, size_t count_words = 0)
+    {
+        uploaded = false;
+        const uint32_t n = h_batch->n_in;
+        if (lazy && n <= kArgIn && n_out <= kArgOut && count_words <= kArgCounts) {
+            ba.block = nullptr;
+            ba.b = *h_batch;
+            ba.inl = ARG_IN | (n_out ? ARG_OUT : 0u) | (count_words ? ARG_COUNTS : 0u);
+            std::memcpy(ba.in, h_in, sizeof(gf_chunk_desc) * n);
+            std::memcpy(ba.prefix, h_prefix, sizeof(uint32_t) * (n + 1u));
+            std::memcpy(ba.prefix + kArgIn + 1u, h_prefix + cap_in + 1u, sizeof(uint32_t) * (n + 1u));
+            if (n_out) std::memcpy(ba.out, h_out, sizeof(gf_chunk_desc) * n_out);
+            if (count_words) std::memcpy(ba.counts, h_counts, sizeof(uint32_t) * count_words);
+            return;
+        }
+        ba.block = d_batch;
+        ba.inl = 0u;
+        uploaded = true;
+        HIPX(hipMemcpyAsync(d_blk, h_blk, o_out + sizeof(gf_chunk_desc) * n_out, hipMemcpyHostToDevice, st));
+        if (count_words)
+            HIPX(hipMemcpyAsync(d_blk + o_counts, h_blk + o_counts, sizeof(uint32_t) * count_words,
+                                hipMemcpyHostToDevice, st));
+    }
+
+    // Read back the status word and the first count_words counts, then mark the stage's event.
+    void download_status(hipStream_t st, size_t count_words = 0)
+    {
+        HIPX(hipMemcpyAsync(h_blk + o_status, d_blk + o_status, 16u + sizeof(uint32_t) * count_words,
+                            hipMemcpyDeviceToHost, st));
+        mark(st);
+    }
+
+    void check_status(const char* what) { check_code(*h_status, what); }
+
+    void check_code(uint32_t s, const char* what)
+    {
         if (s == kStatusTableFull) {
             failed = true;
             fail(GF_INVALID, std::string(what) + ": hash bin table full");
@@ -1063,6 +1178,7 @@ struct gf_stage {
         throw_if_failed();
         if (n == 0) fail(GF_PLAN_MISMATCH, "graphFlow: empty batch");
         if (n > cap_in) fail(GF_PLAN_MISMATCH, "graphFlow: batch exceeds batchChunks");
+        prepare_launch();
         const uint32_t C = pool->C;
         in.assign(chunks, chunks + n);
         first_id = fid;
@@ -1084,7 +1200,7 @@ struct gf_stage {
         try {
             for (uint64_t r = 0; r < bound; ++r) {
                 reserved.push_back(pool->acquire());
-                h_res[r] = reserved.back();
+                ((gf_chunk_handle*)(h_in + n))[r] = reserved.back();
             }
         } catch (...) {
             cancel_reserved();
@@ -1092,28 +1208,40 @@ struct gf_stage {
             throw;
         }
         pool->peak_live = peak0;
-        HIPX(hipMemcpyAsync(h_batch->in, h_in, sizeof(gf_chunk_desc) * n, hipMemcpyHostToDevice, st));
-        HIPX(hipMemcpyAsync(d_res, h_res, sizeof(gf_chunk_handle) * bound, hipMemcpyHostToDevice, st));
         FusedArgs a{};
+        if (n <= kArgIn && bound <= kArgRes) {
+            // inputs and reservation in the launch's arguments
+            a.inl = 1u;
+            std::memcpy(a.in_arg, h_in, sizeof(gf_chunk_desc) * n);
+            std::memcpy(a.res_arg, h_in + n, sizeof(gf_chunk_handle) * bound);
+        } else {
+            // inputs and the reservation's handles right after them: one upload
+            HIPX(hipMemcpyAsync(d_blk + o_in, h_blk + o_in,
+                                sizeof(gf_chunk_desc) * n + sizeof(gf_chunk_handle) * bound, hipMemcpyHostToDevice, st));
+        }
         a.nd = nd;
         a.in = h_batch->in;
         a.n = n;
         a.tile_items = tile_items;
         a.C = C;
         a.hash_entries = hash_E;
-        a.reserved = d_res;
+        a.reserved = (const gf_chunk_handle*)(h_batch->in + n);
         a.n_reserved = (uint32_t)bound;
         a.purpose = 2u * key;
-        a.out = h_batch->out;
+        a.out = (gf_chunk_desc*)(d_fblk + o_fres) - bound;
         a.scratch = d_fscratch;
         a.result = d_result;
         a.group_lds = (uint32_t)((std::max(lds_count, lds_scatter) + 15u) & ~(size_t)15u);
         a.tiles_cap = cap_tiles;
         launch_fused_kernel(pool->payload, r, a, pool->dcols(), st);
-        HIPX(hipMemcpyAsync(h_result, d_result, 2u * sizeof(uint32_t), hipMemcpyDeviceToHost, st));
-        HIPX(hipMemcpyAsync(h_out, h_batch->out, sizeof(gf_chunk_desc) * bound, hipMemcpyDeviceToHost, st));
-        HIPX(hipMemcpyAsync(h_totals, d_fscratch + 2u * (n + 1u), sizeof(uint32_t) * bins, hipMemcpyDeviceToHost, st));
-        HIPX(hipEventRecord(ev, st));
+        // the used outputs, {R, status}, prefixes and bin totals: one readback
+        const size_t o0 = o_fres - sizeof(gf_chunk_desc) * bound;
+        HIPX(hipMemcpyAsync(h_fblk + o0, d_fblk + o0,
+                            sizeof(gf_chunk_desc) * bound + 16u + sizeof(uint32_t) * (2u * (n + 1u) + bins),
+                            hipMemcpyDeviceToHost, st));
+        mark(st);
+        f_n = n;
+        f_bound = (uint32_t)bound;
         fused = true;
         launched = true;
         ++launches;
@@ -1124,7 +1252,9 @@ struct gf_stage {
     {
         wait_stage();
         fused = false;
-        const uint32_t status = h_result[1], R = h_result[0];
+        const uint32_t* res = (const uint32_t*)(h_fblk + o_fres);
+        const gf_chunk_desc* outs = (const gf_chunk_desc*)(h_fblk + o_fres) - f_bound;
+        const uint32_t status = res[1], R = res[0];
         if (status) {
             cancel_reserved();
             failed = true;
@@ -1139,8 +1269,9 @@ struct gf_stage {
             failed = true;
             fail(GF_PLAN_MISMATCH, "partition plan: exact chunk count exceeds the reservation bound");
         }
-        totals.assign(h_totals, h_totals + bins);
-        pending.assign(h_out, h_out + R);
+        const uint32_t* tot = res + 4u + 2u * (f_n + 1u);
+        totals.assign(tot, tot + bins);
+        pending.assign(outs, outs + R);
         for (size_t i = reserved.size(); i-- > R;) pool->cancel(reserved[i]);
         reserved.clear();
         pool->peak_live = std::max<uint32_t>(pool->peak_live, pool->S - (uint32_t)pool->free_list.size());
@@ -1153,6 +1284,7 @@ struct gf_stage {
         throw_if_failed();
         if (n == 0) fail(GF_PLAN_MISMATCH, "graphFlow: empty batch");
         if (n > cap_in) fail(GF_PLAN_MISMATCH, "graphFlow: batch exceeds batchChunks");
+        prepare_launch();
         in.assign(chunks, chunks + n);
         first_id = fid;
         stage_no = 0;
@@ -1175,27 +1307,26 @@ struct gf_stage {
         h_batch->digest_base = 0;
         Cols cols = pool->dcols();
         const uint32_t C = pool->C;
-        reset_status(st);
         const int kd = kind();
         if (kd == 0) {
             // the source fills the admitted chunk it is handed (SourceStage::launch)
             h_out[0] = chunks[0];
             h_batch->n_out = 1;
-            upload_batch(st, 1);
+            stage_batch(st, 1);
             uint32_t tiles = (uint32_t)ceil_div(chunks[0].fill, tile_items);
             if (tiles)
                 with_payload(pool->payload, r, [&](auto pt, auto) {
                     using P = typename decltype(pt)::type;
-                    hipLaunchKernelGGL(k_source<P>, dim3(tiles), dim3(kThreads), 0, st, nd, d_batch, cols, C, rp);
+                    hipLaunchKernelGGL(k_source<P>, dim3(tiles), dim3(kThreads), 0, st, ba, nd, cols, C, rp);
                 });
             pending.assign(chunks, chunks + 1);
         } else if (kd == 3) {
             h_batch->n_out = 0;
-            upload_batch(st, 0);
+            stage_batch(st, 0);
             if (n_tiles)
                 with_payload(pool->payload, r, [&](auto pt, auto) {
                     using P = typename decltype(pt)::type;
-                    hipLaunchKernelGGL(k_sink<P>, dim3(n_tiles), dim3(kThreads), 64, st, nd, d_batch, cols, C, d_status);
+                    hipLaunchKernelGGL(k_sink<P>, dim3(n_tiles), dim3(kThreads), 64, st, ba, nd, cols, C, d_status);
                 });
         } else if (kd == 2) {
             uint64_t R = ceil_div(particles, C);
@@ -1216,27 +1347,27 @@ struct gf_stage {
                 h_out[k] = d;
             }
             h_batch->n_out = (uint32_t)R;
-            upload_batch(st, (uint32_t)R);
+            stage_batch(st, (uint32_t)R);
             with_payload(pool->payload, r, [&](auto pt, auto rc) {
                 using P = typename decltype(pt)::type;
-                hipLaunchKernelGGL((k_transform<P, decltype(rc)::value>), dim3(n_tiles), dim3(kThreads), 0, st, nd,
-                                   d_batch, cols, C, rp);
+                hipLaunchKernelGGL((k_transform<P, decltype(rc)::value>), dim3(n_tiles), dim3(kThreads), 0, st, ba,
+                                   nd, cols, C, rp);
             });
         } else {
             h_batch->n_out = 0;
-            upload_batch(st, 0);
+            stage_batch(st, 0);
             with_payload(pool->payload, r, [&](auto pt, auto rc) {
                 using P = typename decltype(pt)::type;
                 set_lds(k_count<P, decltype(rc)::value>, lds_count);
-                hipLaunchKernelGGL((k_count<P, decltype(rc)::value>), dim3(n_tiles), dim3(kThreads), lds_count, st, nd,
-                                   d_batch, cols, C, d_status, hash_E);
+                hipLaunchKernelGGL((k_count<P, decltype(rc)::value>), dim3(n_tiles), dim3(kThreads), lds_count, st, ba,
+                                   nd, cols, C, d_status, hash_E);
             });
-            HIPX(hipMemcpyAsync(h_counts, h_batch->counts, sizeof(uint32_t) * (size_t)n_tiles * bins,
-                                hipMemcpyDeviceToHost, st));
         }
         HIPX(hipGetLastError());
-        HIPX(hipMemcpyAsync(h_status, d_status, sizeof(uint32_t), hipMemcpyDeviceToHost, st));
-        HIPX(hipEventRecord(ev, st));
+        // the status word, and for a classify the [tile][bin] counts right after it; source and
+        // transform kernels write no status, so the cleared host word stands
+        if (kd == 1 || (kd == 3 && !lazy)) download_status(st, kd == 1 ? (size_t)n_tiles * bins : 0u);
+        else if (!lazy || uploaded) mark(st);  // lazy: an event only where the launch reads the mirror
         launched = true;
         ++launches;
     }
@@ -1256,8 +1387,14 @@ struct gf_stage {
             allocated = (uint32_t)pending.size();
             return false;
         }
-        wait_stage();
         const int kd = kind();
+        if (lazy && !(kd == 1 && stage_no == 0)) {
+            // outputs known on the host: complete without waiting (failures are read at the end)
+            if (kd == 2) allocated = (uint32_t)pending.size();
+            if (kd == 1) stage_no = 0;
+            return false;
+        }
+        wait_stage();
         if (kd == 0) {
             check_status("source");
             return false;
@@ -1335,9 +1472,7 @@ struct gf_stage {
                 run[b] += x;
             }
         h_batch->n_out = (uint32_t)pending.size();
-        HIPX(hipMemcpyAsync(h_batch->counts, h_counts, sizeof(uint32_t) * (size_t)n_tiles * B, hipMemcpyHostToDevice, st));
-        upload_batch(st, h_batch->n_out);
-        reset_status(st);
+        stage_batch(st, h_batch->n_out, (size_t)n_tiles * B);
         Cols cols = pool->dcols();
         const bool stage = !hash_E && stage_fits(B, tile_items, record_bytes(pool->payload));
         const size_t lds = lds_scatter;
@@ -1346,17 +1481,17 @@ struct gf_stage {
             constexpr int RR = decltype(rc)::value;
             if (stage) {
                 set_lds(k_scatter<P, RR, true>, lds);
-                hipLaunchKernelGGL((k_scatter<P, RR, true>), dim3(n_tiles), dim3(kThreads), lds, st, nd, d_batch, cols, C,
+                hipLaunchKernelGGL((k_scatter<P, RR, true>), dim3(n_tiles), dim3(kThreads), lds, st, ba, nd, cols, C,
                                    d_status, hash_E);
             } else {
                 set_lds(k_scatter<P, RR, false>, lds);
-                hipLaunchKernelGGL((k_scatter<P, RR, false>), dim3(n_tiles), dim3(kThreads), lds, st, nd, d_batch, cols,
+                hipLaunchKernelGGL((k_scatter<P, RR, false>), dim3(n_tiles), dim3(kThreads), lds, st, ba, nd, cols,
                                    C, d_status, hash_E);
             }
         });
         HIPX(hipGetLastError());
-        HIPX(hipMemcpyAsync(h_status, d_status, sizeof(uint32_t), hipMemcpyDeviceToHost, st));
-        HIPX(hipEventRecord(ev, st));
+        if (!lazy) download_status(st);
+        else if (uploaded) mark(st);
         allocated = (uint32_t)pending.size();
         ++launches;
     }
@@ -1591,6 +1726,7 @@ void run_host(gf_flow& f, uint32_t src, uint64_t N, const RunShape& sh, gf_stats
             sg = std::make_unique<gf_stage>();
             sg->init(&pool, f.nodes[k].nd, k, f.tuning.seed, Q, sh.tile_items, &f.tuning);
             sg->rp.ph = photon_params_of(f.tuning);
+            sg->lazy = true;
         }
     hipStream_t s = f.ctx->stream;
     struct Lane {
@@ -1762,6 +1898,18 @@ void run_host(gf_flow& f, uint32_t src, uint64_t N, const RunShape& sh, gf_stats
             if (generated >= N && drained && reserved == 0) break;
             if (!progress) fail(GF_STALL, "graphFlow: scheduler stall, no progress possible");
         }
+        // every launch complete (within the stall timeout), then the status of sinks and scatters
+        struct EventGuard {
+            hipEvent_t e = nullptr;
+            ~EventGuard()
+            {
+                if (e) (void)hipEventDestroy(e);
+            }
+        } done;
+        HIPX(hipEventCreateWithFlags(&done.e, hipEventDisableTiming));
+        HIPX(hipEventRecord(done.e, s));
+        stages[0]->wait_event(done.e);
+        for (auto& sg : stages) sg->settle_all(s);
     } catch (...) {
         teardown();
         throw;
