@@ -204,6 +204,32 @@ def test_invalid_classifier_bin_is_a_batch_error(schedule, engine):
     assert flow.run(src, 2)["sunk"] == 2
 
 
+@pytest.mark.parametrize("schedule,engine,fused", [(g.SCHEDULE_HOST, 0, False), (g.SCHEDULE_HOST, 0, True),
+                                                   (g.SCHEDULE_DEVICE, 1, False), (g.SCHEDULE_DEVICE, 2, False),
+                                                   (g.SCHEDULE_DEVICE, 3, False)])
+def test_sink_record_out_of_range_is_a_batch_error(schedule, engine, fused):
+    """A sink record outside its buffer fails the run (BatchError), also where the host runner
+    completes sink batches without waiting for them; the pool is whole again and the same flow
+    runs again (test_failure_paths.cpp:228-366: a failed batch fails the run and restores the pool)."""
+    torch = torch_cuda()
+    N = 6000
+    rec = torch.full((N // 2,), -1, dtype=torch.int32, device="cuda")
+    flow = g.Flow(g.PAYLOAD_PARTICLE)
+    src = flow.source("generate", g.UniformParticles())
+    part = flow.partition("split", g.IdMod(2, 0))
+    sink = flow.sink("record", g.RecordWeight(rec.data_ptr(), N // 2))
+    flow.connect(src, part)
+    flow.connect(part, sink, 0)
+    flow.connect(part, sink, 1)
+    flow.configure(g.Tuning(seed=SEED, batch_chunks=2, chunk_capacity=512, live_particles=2048, schedule=schedule,
+                            device_engine=engine, fused_batches=fused))
+    with pytest.raises(g.BatchError):
+        flow.run(src, N)
+    st = flow.run(src, N // 2)
+    torch.cuda.synchronize()
+    assert st["sunk"] == N // 2 and (rec.cpu().numpy() != -1).all()
+
+
 @pytest.mark.parametrize("schedule", [g.SCHEDULE_HOST, g.SCHEDULE_DEVICE])
 def test_updater_overflow_is_a_batch_error(schedule):
     """test_failure_paths.cpp:1061-1085: a checked updater overflow fails the batch."""
