@@ -217,6 +217,26 @@ __global__ __launch_bounds__(1024) void k_batch_fused(FusedArgs, Cols cols)
     fused_partition_batch<P, R>(*(const FusedArgs*)__builtin_amdgcn_kernarg_segment_ptr(), cols, smem);
 }
 
+// The grid form of the same batch (gf_batch.hpp): count, plan, scatter.
+template <class P, int R>
+__global__ __launch_bounds__(kThreads) void k_fused_count(FusedArgs, Cols cols)
+{
+    extern __shared__ __align__(16) char smem[];
+    fused_grid_count<P, R>(*(const FusedArgs*)__builtin_amdgcn_kernarg_segment_ptr(), cols, smem);
+}
+
+__global__ __launch_bounds__(1024) void k_fused_plan(FusedArgs)
+{
+    fused_grid_plan(*(const FusedArgs*)__builtin_amdgcn_kernarg_segment_ptr());
+}
+
+template <class P, int R>
+__global__ __launch_bounds__(kThreads) void k_fused_scatter(FusedArgs, Cols cols)
+{
+    extern __shared__ __align__(16) char smem[];
+    fused_grid_scatter<P, R>(*(const FusedArgs*)__builtin_amdgcn_kernarg_segment_ptr(), cols, smem);
+}
+
 // Tile descriptors of a batch as the device computes them (TileDescriptor list).
 __global__ void k_tiles(const Batch* bp, uint32_t C, gf_tile_desc* out)
 {
@@ -810,15 +830,29 @@ double zipf_fit_a(const std::vector<double>& cdf, double s)
 
 // One fused partition batch (gf_batch.hpp) on `st`: one workgroup of as many 256-thread tile groups
 // (at most 4) as the LDS holds.
-void launch_fused_kernel(uint32_t payload, uint32_t r, const FusedArgs& a, const Cols& cols, hipStream_t st)
+// One launch on one workgroup of G tile groups; with `tiles` > 0 (the batch's tile count, known on the
+// host) and more than two rounds of G tiles, the grid form instead (three launches, a tile per
+// workgroup). Both run the same tile functions and plan: equal results.
+void launch_fused_kernel(uint32_t payload, uint32_t r, const FusedArgs& a, const Cols& cols, hipStream_t st,
+                         uint32_t tiles = 0)
 {
     const uint32_t G = std::max<uint32_t>(1u, std::min<uint32_t>(4u, (uint32_t)((160u * 1024u) / a.group_lds)));
-    const size_t lds = (size_t)G * a.group_lds;
+    const bool grid = tiles > 2u * G && a.n <= kGridIn;
     with_payload(payload, r, [&](auto pt, auto rc) {
         using P = typename decltype(pt)::type;
         constexpr int RR = decltype(rc)::value;
-        set_lds(k_batch_fused<P, RR>, lds);
-        hipLaunchKernelGGL((k_batch_fused<P, RR>), dim3(1), dim3(kThreads * G), lds, st, a, cols);
+        if (grid) {
+            const size_t lds = a.group_lds;
+            set_lds(k_fused_count<P, RR>, lds);
+            set_lds(k_fused_scatter<P, RR>, lds);
+            hipLaunchKernelGGL((k_fused_count<P, RR>), dim3(tiles), dim3(kThreads), lds, st, a, cols);
+            hipLaunchKernelGGL(k_fused_plan, dim3(1), dim3(1024), 0, st, a);
+            hipLaunchKernelGGL((k_fused_scatter<P, RR>), dim3(tiles), dim3(kThreads), lds, st, a, cols);
+        } else {
+            const size_t lds = (size_t)G * a.group_lds;
+            set_lds(k_batch_fused<P, RR>, lds);
+            hipLaunchKernelGGL((k_batch_fused<P, RR>), dim3(1), dim3(kThreads * G), lds, st, a, cols);
+        }
     });
     HIPX(hipGetLastError());
 }
@@ -995,6 +1029,7 @@ struct gf_stage {
             h_fblk = mem.h<uint8_t>(o_fres + 16u + sizeof(uint32_t) * (2u * (cap_in + 1u) + bins));
             d_result = (uint32_t*)(d_fblk + o_fres);
             d_fscratch = (uint32_t*)(d_fblk + o_fres + 16u);
+            HIPX(hipMemset(d_result, 0, 16u));  // word 2 of the grid form starts at zero
         }
         rp.seed = seed;
         rp.C = p->C;
@@ -1233,7 +1268,7 @@ struct gf_stage {
         a.result = d_result;
         a.group_lds = (uint32_t)((std::max(lds_count, lds_scatter) + 15u) & ~(size_t)15u);
         a.tiles_cap = cap_tiles;
-        launch_fused_kernel(pool->payload, r, a, pool->dcols(), st);
+        launch_fused_kernel(pool->payload, r, a, pool->dcols(), st, n_tiles);
         // the used outputs, {R, status}, prefixes and bin totals: one readback
         const size_t o0 = o_fres - sizeof(gf_chunk_desc) * bound;
         HIPX(hipMemcpyAsync(h_fblk + o0, d_fblk + o0,

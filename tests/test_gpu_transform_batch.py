@@ -40,20 +40,26 @@ def _read_outputs(pool, outs, C):
     return cols
 
 
-@pytest.mark.parametrize("B,T", [(7, 1024), (3, 256), (300, 1024)])
-def test_one_ragged_batch_equals_three_step_protocol_and_oracle(B, T):
-    C, fills = 1000, [1000, 37, 613]
+LONG = [1000, 37, 613, 999, 1000, 512, 1, 800, 1000, 250]
+
+
+@pytest.mark.parametrize("B,T,fills", [(7, 1024, [1000, 37, 613]), (3, 256, [1000, 37, 613]),
+                                       (300, 1024, [1000, 37, 613]),
+                                       # 28 tiles: the grid form (count, plan, scatter launches)
+                                       (7, 256, LONG), (3, 256, LONG), (300, 256, LONG)])
+def test_one_ragged_batch_equals_three_step_protocol_and_oracle(B, T, fills):
+    C = 1000
     torch_cuda()
     pa, ha, ids_all = _filled_pool(C, fills)
     pb, hb, _ = _filled_pool(C, fills)
     node = g.IdMod(B, 0).desc()
     # reference protocol: launch (classify) -> advance (host plan + scatter) -> advance -> finish
-    sa = g.Stage(pa, node, node_key=1, seed=SEED, batch_chunks=4, tile_items=T)
+    sa = g.Stage(pa, node, node_key=1, seed=SEED, batch_chunks=16, tile_items=T)
     sa.launch([(h.index, h.generation, f, 0) for h, f in zip(ha, fills)])
     assert sa.advance()[0] and not sa.advance()[0]
     outs_a = sa.finish()
     # one launch -> one advance -> finish
-    sb = g.Stage(pb, node, node_key=1, seed=SEED, batch_chunks=4, tile_items=T)
+    sb = g.Stage(pb, node, node_key=1, seed=SEED, batch_chunks=16, tile_items=T)
     sb.transform_batch([(h.index, h.generation, f, 0) for h, f in zip(hb, fills)])
     more, allocated = sb.advance()
     assert not more and allocated == len(outs_a)
@@ -70,7 +76,7 @@ def test_one_ragged_batch_equals_three_step_protocol_and_oracle(B, T):
     pos = np.array([flat[base[b] + j // C] + j % C for b, j in zip(bins.tolist(), rank.tolist())])
     assert (cb[1][pos] == ids_all).all()
     # the unused reservation went back: both pools hold the same live chunks
-    assert pa.counts()["free"] == pb.counts()["free"] == 512 - 3 - R
+    assert pa.counts()["free"] == pb.counts()["free"] == 512 - len(fills) - R
     for h in hb:
         pb.release(h)
     for o in outs_b:
@@ -78,15 +84,16 @@ def test_one_ragged_batch_equals_three_step_protocol_and_oracle(B, T):
     assert pb.counts()["free"] == 512
 
 
-@pytest.mark.parametrize("depth", [1, 2])
-def test_runner_with_fused_batches_trace(depth):
-    """The host GraphRunner driving every partition batch as one launch: every batch record equals
-    the oracle's runner with fused stages."""
+@pytest.mark.parametrize("depth,Q,T", [(1, 4, None), (2, 4, None), (1, 8, 256), (2, 8, 256)])
+def test_runner_with_fused_batches_trace(depth, Q, T):
+    """The host GraphRunner driving every partition batch as one fused batch (one launch, or the grid
+    form's three for batches of many tiles): every batch record equals the oracle's runner with
+    fused stages."""
     torch = torch_cuda()
-    N, C, P, Q = 20000, 1024, 2048, 4
+    N, C, P = 20000, 1024, 2048 if Q == 4 else 8192
     flow, src, rec = feedback_flow(N)
     stats, trace = run_flow(flow, src, N, C=C, P=P, Q=Q, schedule=g.SCHEDULE_HOST, fused_batches=True,
-                            inflight_batches=depth)
+                            inflight_batches=depth, tile_items=T)
     ref = run_oracle(flow, N, C=C, P=P, Q=Q, schedule=g.SCHEDULE_HOST, record=True, fused_stages=True,
                      inflight=depth)
     assert ref["status"] == O.OK, ref["message"]
@@ -130,3 +137,32 @@ def test_fused_batch_invalid_bin_returns_the_reservation():
         stage.advance()
     assert pool.counts()["free"] == 15
     assert [pool.generation(i) for i in range(16)] == [0] * 16
+
+
+def test_fused_grid_batch_invalid_bin_returns_the_reservation():
+    """The same error in the grid form (12 tiles of 256 items: a count launch over the grid reports
+    it, the plan publishes nothing, the scatter launch does nothing); a new stage on the same pool
+    then runs a valid batch of the same shape."""
+    torch = torch_cuda()
+    lut = torch.tensor([0, 1, 5, 1], dtype=torch.int32, device="cuda")  # id 2 -> bin 5 of 2
+    ok = torch.tensor([0, 1, 1, 0], dtype=torch.int32, device="cuda")
+    C, n = 256, 12
+    pool = g.Pool(g.PAYLOAD_PARTICLE, 64, C)
+    hs = [pool.acquire() for _ in range(n)]
+    for h in hs:
+        pool.write(0, h.index * C, np.ones(C, np.float32))
+        pool.write(1, h.index * C, (np.arange(C) % 4).astype(np.uint32))
+        pool.write(2, h.index * C, np.zeros(C, np.uint32))
+    chunks = [(h.index, h.generation, C, 0) for h in hs]
+    stage = g.Stage(pool, g.Lut(2, lut.data_ptr(), 4, 1, 0).desc(), 1, SEED, 16, tile_items=256)
+    stage.transform_batch(chunks)
+    with pytest.raises(g.BatchError):
+        stage.advance()
+    assert pool.counts()["free"] == 64 - n
+    assert [pool.generation(i) for i in range(64)] == [0] * 64
+    good = g.Stage(pool, g.Lut(2, ok.data_ptr(), 4, 1, 0).desc(), 1, SEED, 16, tile_items=256)
+    good.transform_batch(chunks)
+    more, allocated = good.advance()
+    outs = good.finish()
+    assert not more and allocated == len(outs) == 12 and [o[3] for o in outs] == [0] * 6 + [1] * 6
+    assert all(o[2] == C for o in outs)
