@@ -417,9 +417,23 @@ __device__ inline void transform(const gf_node_desc& nd, const RunParams&, uint3
     transform_p(nd.op, nd.f[0], p);
 }
 __device__ inline void transform(const gf_node_desc&, const RunParams&, uint32_t, Keyed&) {}
+// The tally a photon step adds into. A flow's runs spread it over kTallyCopies private copies
+// (ptr[1], tally_bins doubles each, folded into ptr[0] when the run ends): the 1024 double bins
+// are 64 cache lines, and every step's atomic add would otherwise queue behind the others on
+// them. Waves take the copies round robin. Without copies (a stage made through the C ABI),
+// the step adds into ptr[0] itself.
+constexpr uint32_t kTallyCopies = 128u;
+
+__device__ inline double* photon_tally(const gf_node_desc& nd, uint32_t bins)
+{
+    if (!nd.ptr[1]) return (double*)nd.ptr[0];
+    const uint32_t w = blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6);
+    return (double*)nd.ptr[1] + (size_t)(w & (kTallyCopies - 1u)) * bins;
+}
+
 __device__ inline void transform(const gf_node_desc& nd, const RunParams& rp, uint32_t purpose, Photon& p)
 {
-    if (nd.op == GF_XF_PHOTON_STEP) photon_step(rp, purpose, p, (double*)nd.ptr[0]);
+    if (nd.op == GF_XF_PHOTON_STEP) photon_step(rp, purpose, p, photon_tally(nd, rp.ph.tally_bins));
 }
 
 // Zipf rank of u: the smallest r with cdf[r] > u * cdf[n - 1] (n - 1 if none), the full-table

@@ -237,6 +237,20 @@ __global__ __launch_bounds__(kThreads) void k_fused_scatter(FusedArgs, Cols cols
     fused_grid_scatter<P, R>(*(const FusedArgs*)__builtin_amdgcn_kernarg_segment_ptr(), cols, smem);
 }
 
+// A photon step's private tally copies summed into its tally, copy by copy in a fixed order, and
+// cleared for the next run.
+__global__ void k_tally_fold(double* copies, uint32_t bins, double* tally)
+{
+    for (uint32_t b = blockIdx.x * blockDim.x + threadIdx.x; b < bins; b += gridDim.x * blockDim.x) {
+        double sum = 0.0;
+        for (uint32_t k = 0; k < kTallyCopies; ++k) {
+            sum += copies[(size_t)k * bins + b];
+            copies[(size_t)k * bins + b] = 0.0;
+        }
+        tally[b] += sum;
+    }
+}
+
 // Tile descriptors of a batch as the device computes them (TileDescriptor list).
 __global__ void k_tiles(const Batch* bp, uint32_t C, gf_tile_desc* out)
 {
@@ -1557,6 +1571,8 @@ struct NodeRec {
     gf_node_desc nd{};
     std::vector<int32_t> routes;
     uint32_t slot = 0, local = 0;  // served flows: the job slot whose graph copy holds the node, its id there
+    double* tally_copies = nullptr;  // photon steps: kTallyCopies private tallies of tally_copy_bins
+    uint32_t tally_copy_bins = 0;
 };
 
 struct EngineBuffers {
@@ -1746,7 +1762,7 @@ gf_tuning default_photon_tuning()
     return t;
 }
 
-void run_host(gf_flow& f, uint32_t src, uint64_t N, const RunShape& sh, gf_stats& st)
+void run_host_form(gf_flow& f, uint32_t src, uint64_t N, const RunShape& sh, gf_stats& st)
 {
     gf_pool& pool = *f.pool;
     const uint32_t n = (uint32_t)f.nodes.size(), C = sh.C, P = sh.P, Q = sh.Q;
@@ -2406,7 +2422,7 @@ void run_team(gf_flow& f, EngineBuffers& eb, uint64_t N, gf_stats& st, bool time
     finish_device(f, eb, st, ctl, launches);
 }
 
-void run_device(gf_flow& f, uint32_t src, uint64_t N, const RunShape& sh, gf_stats& st, bool timed)
+void run_device_form(gf_flow& f, uint32_t src, uint64_t N, const RunShape& sh, gf_stats& st, bool timed)
 {
     const std::string key = shape_key(f, sh, N);
     if (!f.eng || f.eng->key != key) build_engine(f, N, sh, src);
@@ -2438,6 +2454,71 @@ void run_device(gf_flow& f, uint32_t src, uint64_t N, const RunShape& sh, gf_sta
     }
     st.engine = 1;
     run_wide(f, eb, N, st, timed, 0);
+}
+
+// A run's private tally copies (photon_tally): attached to the flow's photon steps before anything is
+// built or launched, folded into each step's tally when the run ends, also when it fails (the
+// caller's tally then holds what the launched batches added, as with direct atomic adds).
+struct TallyScope {
+    gf_flow& f;
+    bool folded = false;
+
+    explicit TallyScope(gf_flow& fl) : f(fl)
+    {
+        const uint32_t tb = f.tuning.photon_tally_bins ? f.tuning.photon_tally_bins : 1024u;
+        for (auto& n : f.nodes) {
+            if (n.nd.role != GF_ROLE_TRANSFORM || n.nd.op != GF_XF_PHOTON_STEP) continue;
+            if (n.tally_copies && n.tally_copy_bins != tb) {
+                f.eng.reset();  // its node array names the old copies
+                (void)hipFree(n.tally_copies);
+                n.tally_copies = nullptr;
+            }
+            if (!n.tally_copies) {
+                n.tally_copies = dalloc<double>((size_t)kTallyCopies * tb);
+                HIPX(hipMemset(n.tally_copies, 0, sizeof(double) * kTallyCopies * tb));
+                n.tally_copy_bins = tb;
+            }
+            n.nd.ptr[1] = (uint64_t)(uintptr_t)n.tally_copies;
+        }
+    }
+
+    void fold()
+    {
+        folded = true;
+        hipStream_t s = f.ctx->stream;
+        for (auto& n : f.nodes) {
+            if (!n.tally_copies || n.nd.op != GF_XF_PHOTON_STEP) continue;
+            const uint32_t tb = n.tally_copy_bins;
+            hipLaunchKernelGGL(k_tally_fold, dim3((tb + 255u) / 256u), dim3(256), 0, s, n.tally_copies, tb,
+                               (double*)n.nd.ptr[0]);
+            HIPX(hipGetLastError());
+        }
+        HIPX(hipStreamSynchronize(s));
+    }
+
+    ~TallyScope()
+    {
+        if (folded) return;
+        try {
+            (void)hipStreamSynchronize(f.ctx->stream);
+            fold();
+        } catch (...) {
+        }
+    }
+};
+
+void run_host(gf_flow& f, uint32_t src, uint64_t N, const RunShape& sh, gf_stats& st)
+{
+    TallyScope ts(f);
+    run_host_form(f, src, N, sh, st);
+    ts.fold();
+}
+
+void run_device(gf_flow& f, uint32_t src, uint64_t N, const RunShape& sh, gf_stats& st, bool timed)
+{
+    TallyScope ts(f);
+    run_device_form(f, src, N, sh, st, timed);
+    ts.fold();
 }
 
 void run_wide(gf_flow& f, EngineBuffers& eb, uint64_t N, gf_stats& st, bool timed, uint64_t launches0)
@@ -2925,6 +3006,8 @@ gf_status gf_flow_destroy(gf_flow* flow)
     flow->eng.reset();
     if (flow->pool)
         for (void* p : flow->pool->cols) (void)hipFree(p);
+    for (auto& n : flow->nodes)
+        if (n.tally_copies) (void)hipFree(n.tally_copies);
     delete flow;
     return GF_OK;
 }
