@@ -12,7 +12,7 @@ Beside the headline it reports (each at the size its workload states):
   - the other routing distributions of the feedback network at N=1e8;
   - the same network with 16M particles in flight on the wide engine (kernels at throughput);
   - the keyed hash pipeline at 1e8 keys, Q in {1, 4, 8, 16}, with in-flight batch lanes;
-  - photon transport at 1e8 photons;
+  - photon transport at 1e8 photons (team engine with P photons in flight; wide engine with 16M);
   - the served mixed stream of 4096 jobs (N log-uniform 1e3..1e7, 64 GB budget);
   - the plain one-batch-at-a-time forms;
 and, in the CPU leg, the oracle's restatement of the reference CPU path (runner schedule, one thread
@@ -253,23 +253,33 @@ def hash_forms(g, torch, N, qs, lanes, live, src_chunks=8, tile=1024):
 
 
 # ---- config 4: photon transport ------------------------------------------------------------------
-def photon_forms(g, torch, N, Np, C, P, Q):
+def photon_forms(g, torch, N, Np, C, P, Q, wide_live=1 << 24, lanes=31, src_chunks=16, wg=2048, tile=1024):
     """Photon transport (10 layers of 1 mfp, albedo 0.9, HG g=0.9, roulette 1e-4 / 0.1, 1024 depth
-    bins): the batched persistent form (team engine) and the plain host-planned one-batch form."""
+    bins): the batched persistent form (team engine, P photons in flight), the batched hipGraph form
+    with `wide_live` photons in flight (wide engine, `lanes` batches per node, Q=16; 0 = skip) and the
+    plain host-planned one-batch form."""
     cfg = dict(g.PHOTON_DEFAULTS)
     out, tallies = {}, {}
-    for name, n, kw in (("photon_batched_team", N, dict(schedule=g.SCHEDULE_DEVICE, batch_chunks=Q)),
-                        ("photon_plain_host_planned_Q1", Np, dict(schedule=g.SCHEDULE_HOST, batch_chunks=1))):
+    Sw = wide_live // C * 3 + 2 * lanes * (16 + 4) + 64
+    forms = [("photon_batched_team", N, P, dict(schedule=g.SCHEDULE_DEVICE, batch_chunks=Q))]
+    if wide_live:
+        forms.append((f"photon_batched_wide_live_{wide_live}", N, wide_live,
+                      dict(schedule=g.SCHEDULE_DEVICE, batch_chunks=16, fuse_transforms=True, device_engine=1,
+                           max_workgroups=wg, tile_items=tile, inflight_batches=lanes, source_chunks=src_chunks,
+                           num_chunks=Sw)))
+    forms.append(("photon_plain_host_planned_Q1", Np, P, dict(schedule=g.SCHEDULE_HOST, batch_chunks=1)))
+    for name, n, live, kw in forms:
         tally = torch.zeros(cfg["tally_bins"], dtype=torch.float64, device="cuda")
         flow = g.Flow(g.PAYLOAD_PHOTON)
         src, *_ = g.photon_network(flow, tally.data_ptr())
-        flow.configure(g.Tuning(seed=42, chunk_capacity=C, live_particles=P, photon=cfg, **kw))
+        flow.configure(g.Tuning(seed=42, chunk_capacity=C, live_particles=live, photon=cfg, **kw))
         flow.run(src, min(n, 100000))  # warm-up: engine build, pool
         dt, st = run_once(flow, src, tally, n, torch)
         part_items, xf_items = flow.item_passes()
         photon_steps = part_items  # one alive-partition pass per photon step (the step is fused into it)
-        out[name] = {"value": n / dt, "unit": "photons/s", "items": n, "photon_steps_per_s": photon_steps / dt,
-                     "photon_steps": int(photon_steps), "engine_steps": int(st["steps"])}
+        out[name] = {"value": n / dt, "unit": "photons/s", "items": n, "live": live,
+                     "photon_steps_per_s": photon_steps / dt, "photon_steps": int(photon_steps),
+                     "engine_steps": int(st["steps"])}
         tallies[name] = tally.cpu().numpy()
         log(f"[bench] {name} N={n}: {n / dt:.4g} photons/s, {photon_steps / dt:.4g} photon steps/s")
         flow.close()
@@ -445,12 +455,13 @@ def cpu_leg(g, flow, N, C, P, Q, rows, Nc, forms, hash_sets, photon_tallies, pho
         for q, got in hash_sets.items():
             checks[f"hash_Q{q}_buckets_equal_oracle"] = bool((got[:, 0] == msum).all() and (got[:, 1] == mcnt).all())
     # config 4: the batched photon tallies against the FP64 oracle on the same Philox words
-    if photon_tallies is not None:
+    if photon_tallies:
         t0 = time.perf_counter()
         ref, steps = O.photon_scalar(42, photons, dict(g.PHOTON_DEFAULTS), threads=threads)
-        t = photon_tallies
-        checks["photon_fp64_max_bin_rel_err"] = float((np.abs(t - ref) / ref).max())
-        checks["photon_fp64_total_rel_err"] = float(abs(t.sum() - ref.sum()) / ref.sum())
+        for name, t in photon_tallies.items():
+            pre = "photon_fp64" if name == "photon_batched_team" else "photon_wide_fp64"
+            checks[f"{pre}_max_bin_rel_err"] = float((np.abs(t - ref) / ref).max())
+            checks[f"{pre}_total_rel_err"] = float(abs(t.sum() - ref.sum()) / ref.sum())
         checks["photon_fp64_oracle_steps"] = int(steps)
         checks["photon_oracle_seconds"] = time.perf_counter() - t0
     out["checks"] = checks
@@ -575,6 +586,8 @@ def main():
     ap.add_argument("--no-fuse", action="store_true", help="run A/B as separate transform batches")
     ap.add_argument("--photons", type=float, default=1e8, help="photon transport form: photons (0 = skip)")
     ap.add_argument("--plain-photons", type=float, default=1e6)
+    ap.add_argument("--photon-live", type=int, default=1 << 24,
+                    help="photon transport on the wide engine with this many photons in flight (0 = skip)")
     ap.add_argument("--hash-items", type=float, default=1e8, help="keyed hash pipeline form: keys (0 = skip)")
     ap.add_argument("--hash-q", default="1,4,8,16", help="keyed hash pipeline form: batch sizes Q, comma separated")
     ap.add_argument("--hash-lanes", type=int, default=63, help="keyed hash pipeline: inflight batches per node")
@@ -640,9 +653,9 @@ def main():
                                        args.hash_lanes, args.hash_live, args.hash_source_chunks, args.hash_tile)
             forms.update(hf)
         if args.photons > 0:
-            pf, pt = photon_forms(g, torch, int(args.photons), int(args.plain_photons), C, P, Q)
+            pf, pt = photon_forms(g, torch, int(args.photons), int(args.plain_photons), C, P, Q, args.photon_live)
             forms.update(pf)
-            photon_tally = pt["photon_batched_team"]
+            photon_tally = {k: v for k, v in pt.items() if k.startswith("photon_batched")}
         if args.serve_jobs > 0:
             forms["served_mixed_stream"] = served_forms(g, torch, args.serve_jobs, args.serve_max_items,
                                                         args.serve_rate)
