@@ -59,7 +59,7 @@ __global__ void k_zero_u64(unsigned long long* p, uint64_t n)
 }
 
 constexpr uint32_t kServeLive = 16384;      // live bound of a served job (the headline P)
-constexpr uint32_t kServePhotonLive = 8192; // photons in flight per served photon job
+constexpr uint32_t kServePhotonLive = 32768; // photons in flight per served photon job (fewer dependent steps)
 constexpr uint32_t kServeCap = 8192;        // chunk capacity
 constexpr uint32_t kServeBatch = 8;         // Q
 constexpr uint32_t kServeBuckets = 4096;
