@@ -352,6 +352,13 @@ def served_forms(g, torch, jobs, max_items, rate, slots=16):
            "failed_jobs": len(bad), "p50_latency_ms": float(np.percentile(lat, 50)),
            "p99_latency_ms": float(np.percentile(lat, 99)), "makespan_s": span,
            "jobs_per_topology": {g.JOB_NAMES[t]: int((topo == t).sum()) for t in range(3)}}
+    by_topo = {}
+    for t in range(3):
+        lt = np.array([r["end_ms"] - r["submit_ms"] for r in res if r["topology"] == t])
+        if len(lt):
+            by_topo[g.JOB_NAMES[t]] = {"p50_latency_ms": float(np.percentile(lt, 50)),
+                                       "p99_latency_ms": float(np.percentile(lt, 99))}
+    out["latency_per_topology"] = by_topo
     # one job at a time: every 16th job as a single flow on the batched device engine (engine picked
     # per topology as for a run alone), then the plain host-planned form, one batch at a time (Q=1),
     # on the stream's 16 smallest jobs
@@ -375,7 +382,7 @@ def served_forms(g, torch, jobs, max_items, rate, slots=16):
     for i in sample:
         f, src = flows[int(topo[i])]
         f.configure(g.Tuning(seed=int(seeds[i]), batch_chunks=8, chunk_capacity=8192,
-                             live_particles=8192 if int(topo[i]) == 2 else 16384, schedule=g.SCHEDULE_DEVICE,
+                             live_particles=16384 if int(topo[i]) == 0 else 32768, schedule=g.SCHEDULE_DEVICE,
                              photon=dict(g.PHOTON_DEFAULTS)))
         f.run(src, min(int(items[i]), 16384))  # engine build (same pool shape) outside the timed run
         torch.cuda.synchronize()
@@ -402,7 +409,8 @@ def served_forms(g, torch, jobs, max_items, rate, slots=16):
     out["plain_sample_jobs"] = len(small)
     out["speedup_vs_plain"] = out["value"] / plain
     log(f"[bench] served {jobs} jobs at {rate:.0f}/s: {out['value']:.4g} items/s, p50 {out['p50_latency_ms']:.1f} ms, "
-        f"p99 {out['p99_latency_ms']:.1f} ms; one at a time batched {one:.4g}, plain {plain:.4g} items/s")
+        f"p99 {out['p99_latency_ms']:.1f} ms; one at a time batched {one:.4g}, plain {plain:.4g} items/s; p50/p99 per "
+        f"topology (ms): " + ", ".join(f"{k} {v['p50_latency_ms']:.1f}/{v['p99_latency_ms']:.1f}" for k, v in by_topo.items()))
     for f, _ in flows.values():
         f.close()
     return out

@@ -58,7 +58,8 @@ __global__ void k_zero_u64(unsigned long long* p, uint64_t n)
     for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (uint64_t)gridDim.x * blockDim.x) p[i] = 0ull;
 }
 
-constexpr uint32_t kServeLive = 16384;      // live bound of a served job (the headline P)
+constexpr uint32_t kServeLive = 16384;      // live bound of a served feedback job (the headline P)
+constexpr uint32_t kServeHashLive = 32768;  // keys in flight per served hash-pipeline job
 constexpr uint32_t kServePhotonLive = 32768; // photons in flight per served photon job (fewer dependent steps)
 constexpr uint32_t kServeCap = 8192;        // chunk capacity
 constexpr uint32_t kServeBatch = 8;         // Q
@@ -272,7 +273,7 @@ void build_serve_engine(gf_server& sv, ServeEngine& te, uint32_t topo, uint32_t 
         if (f.nodes[k].slot == 0 && f.nodes[k].nd.role == GF_ROLE_TRANSFORM)
             reserve += output_chunk_bound(kServeCap, kServeCap, f.nodes[k].nd.bins);
     te.reserve_graph = reserve;
-    te.live_max = topo == GF_JOB_PHOTON ? kServePhotonLive : kServeLive;
+    te.live_max = topo == GF_JOB_PHOTON ? kServePhotonLive : topo == GF_JOB_HASH ? kServeHashLive : kServeLive;
     const uint64_t chunk_bytes = (uint64_t)kServeCap * payload_bytes(payload);
     uint64_t pool = pool_bytes / chunk_bytes;
     pool = std::min<uint64_t>(pool, (uint64_t)M * (te.live_max + reserve));  // more than M full jobs is never used
