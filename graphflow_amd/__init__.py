@@ -527,7 +527,7 @@ class Tuning:
     bin_path: int = 0  # BIN_PATH_AUTO (hash above 4096 bins), BIN_PATH_DENSE, BIN_PATH_HASH
     hash_table_entries: Optional[int] = None  # hash path: slots of a tile's bin table (None: 2 * tile items)
     stall_timeout_ms: Optional[int] = None  # plain schedule: longest wait for a device stage (Tuning::stallTimeout)
-    fused_batches: bool = False  # plain schedule: one launch per partition batch (gf_transform_batch)
+    fused_batches: bool = False  # plain schedule: gf_transform_batch per partition batch (no host round trip)
     source_chunks: Optional[int] = None  # device schedule: chunks a source lane admits per step (None: 1)
 
     def validate(self):

@@ -127,7 +127,7 @@ typedef struct gf_tuning {
                                     never fills); a tile with more distinct bins fails its batch (GF_INVALID) */
     uint32_t stall_timeout_ms; /* Tuning::stallTimeout: plain schedule, longest wait for a device stage
                                   before the run fails with GF_STALL (0 -> no limit)                       */
-    uint32_t fused_batches;  /* plain schedule: each partition batch as one launch (gf_transform_batch)
+    uint32_t fused_batches;  /* plain schedule: each partition batch as one fused batch (gf_transform_batch)
                                 instead of classify -> host plan -> scatter                                */
     uint32_t source_chunks;  /* device schedule (wide and micro engines): chunks a source lane admits per
                                 step, each by the admission rule with the earlier ones counted (0 -> 1,
@@ -205,9 +205,11 @@ gf_status gf_stage_launch(gf_stage* stage, const gf_chunk_desc* chunks, uint32_t
 gf_status gf_stage_advance(gf_stage* stage, void* stream, uint32_t* allocated, int* more_work);
 gf_status gf_stage_finish(gf_stage* stage, gf_chunk_desc* outputs, uint32_t cap, uint32_t* n_out);
 gf_status gf_stage_abort(gf_stage* stage);
-/* One partition batch in ONE launch (no counterpart kernel in the reference, which plans on the host
- * between its classify and scatter kernels, flow/Execution.hpp:2363-2725): classify, device plan,
- * device output acquisition and stable scatter of a ragged list of n <= Q chunks. The stage
+/* One partition batch with no host round trip (no counterpart kernel in the reference, which plans on
+ * the host between its classify and scatter kernels, flow/Execution.hpp:2363-2725): classify, device
+ * plan, device output acquisition and stable scatter of a ragged list of n <= Q chunks, in one launch
+ * on one workgroup, or for batches of more than two rounds of that workgroup's tiles (and n <= 256)
+ * in three stream-ordered launches over the grid (count, plan, scatter), same result. The stage
  * reserves outputChunkBound(items, C, bins) chunks from its pool before the launch; the kernel
  * uses the first R in bin-major order. gf_stage_advance (one call, more_work = 0) returns the
  * unused reservation and reports allocated = R; gf_stage_finish copies the R outputs. Plain

@@ -165,7 +165,7 @@ struct Tuning {
     BinPath binPath = BinPath::automatic;                   // Tuning::binPath
     std::optional<std::uint32_t> hashTableEntries;          // hash path: slots of a tile's bin table
     std::optional<std::chrono::milliseconds> stallTimeout;  // Tuning::stallTimeout
-    bool fusedBatches = false;                              // plain schedule: one launch per partition batch
+    bool fusedBatches = false;                              // plain schedule: gf_transform_batch per partition batch
     std::optional<std::uint32_t> sourceChunks;              // device schedule: chunks a source lane admits per step
     Schedule schedule = Schedule::device;
     bool trace = false;
