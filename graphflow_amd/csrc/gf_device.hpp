@@ -1346,20 +1346,22 @@ __device__ void sink_tiles_wave(const gf_node_desc& nd, const Batch& b, uint32_t
     }
 }
 
-// Sink of four consecutive tiles t .. t + nt - 1 (nt <= 4) by one wave, one tile per 16-lane row
-// (same results as sink_tile): the small per-bucket chunks a sink of a wide partition takes need at
-// most four row iterations each, their loads are in flight together, and a row's digest sums are
-// reduced within the row. Tiles of more than 64 items go to the whole wave afterwards, one at a time.
-template <class P>
+// Sink of 64 / RL consecutive tiles t .. t + nt - 1 by one wave, one tile per row of RL lanes (same
+// results as sink_tile): the small per-bucket chunks a sink of a wide partition takes need a few row
+// iterations each, the rows' loads are in flight together, and a row's digest sums are reduced
+// within the row. Tiles of more than 64 items go to the whole wave afterwards, one at a time. The
+// caller picks RL from the batch's items per tile (narrow rows for the tiny chunks of a batch of
+// few keys per bucket: more chunks per round trip).
+template <class P, uint32_t RL>
 __device__ void sink_tiles_rows(const gf_node_desc& nd, const Batch& b, uint32_t t, uint32_t nt, const Cols& cols,
                                 uint32_t C, uint32_t* status)
 {
-    const uint32_t lane = lane_id(), row = lane >> 4, j = lane & 15u;
+    const uint32_t lane = lane_id(), row = lane / RL, j = lane % RL;
     const TileSpan sp = tile_span(b, b.in, b.n_in, row < nt ? t + row : kNoTile, C);
     const bool small = sp.count <= 64u;
     if constexpr (std::is_same_v<P, Particle>) {
         if (small)
-            for (uint32_t i = j; i < sp.count; i += 16u) {
+            for (uint32_t i = j; i < sp.count; i += RL) {
                 const Particle p = Particle::load(cols, sp.base + i);
                 if (nd.op == GF_SNK_HIST) {
                     uint32_t v = nd.u[0] == 0u ? __float_as_uint(p.w) : p.gen;
@@ -1375,15 +1377,15 @@ __device__ void sink_tiles_rows(const gf_node_desc& nd, const Batch& b, uint32_t
         if (nd.op != GF_SNK_CHUNK_DIGEST) return;
         uint64_t acc = 0ull, msum = 0ull;
         if (small)
-            for (uint32_t i = j; i < sp.count; i += 16u) {
+            for (uint32_t i = j; i < sp.count; i += RL) {
                 const Keyed p = Keyed::load(cols, sp.base + i);
                 const uint64_t h = keyed_hash(p);
                 acc += mix64(h ^ ((uint64_t)(sp.offset + i + 1u) * 0xD6E8FEB86659FD93ull));
                 msum += h;
             }
-        for (int o = 8; o > 0; o >>= 1) {
-            acc += __shfl_down(acc, o, 16);
-            msum += __shfl_down(msum, o, 16);
+        for (int o = (int)RL / 2; o > 0; o >>= 1) {
+            acc += __shfl_down(acc, o, (int)RL);
+            msum += __shfl_down(msum, o, (int)RL);
         }
         if (j == 0u && small && sp.count) {
             const gf_chunk_desc d = b.in[sp.chunk];
@@ -1406,7 +1408,7 @@ __device__ void sink_tiles_rows(const gf_node_desc& nd, const Batch& b, uint32_t
     // the larger tiles, each by the whole wave
     for (uint64_t big = __ballot(j == 0u && !small); big; big &= big - 1ull) {
         const uint32_t r = (uint32_t)__ffsll((unsigned long long)big) - 1u;
-        sink_tiles_wave<P, 1>(nd, b, t + (r >> 4), 1u, cols, C, status);
+        sink_tiles_wave<P, 1>(nd, b, t + r / RL, 1u, cols, C, status);
     }
 }
 

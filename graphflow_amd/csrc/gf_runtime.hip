@@ -322,8 +322,8 @@ __device__ inline uint32_t kb_group(uint32_t kind)
 }
 
 // Tiles round-robin over the workgroups; the tiles of the sinks flagged NODE_WAVE_SINK (sinks fed by
-// wide partitions, whose chunks are small) round-robin over the waves instead, four at a time (one per
-// 16-lane row).
+// wide partitions, whose chunks are small) round-robin over the waves instead, several at a time (one
+// per row of 4, 8 or 16 lanes).
 template <class P, int R, uint32_t Kinds>
 __global__ __launch_bounds__(kThreads) void k_engine_kb(Engine e)
 {
@@ -420,11 +420,20 @@ __global__ __launch_bounds__(kThreads) void k_engine_kb(Engine e)
             const uint32_t t0 = pre[i], t1 = pre[i + 1u];
             const gf_node_desc& nd = e.nodes[k];
             const Batch b = make_batch(e, k, e.st[k]);
-            // each wave takes four consecutive tiles at a time
-            for (uint32_t t = t0 + (blockIdx.x * (uint32_t)kWaves + w) * 4u; t < t1; t += 4u * W) {
-                sink_tiles_rows<P>(nd, b, t - t0, t1 - t < 4u ? t1 - t : 4u, e.cols, e.C, &e.ctl[CTL_STATUS]);
-                wtiles += t1 - t < 4u ? t1 - t : 4u;
-            }
+            // each wave takes 64 / RL consecutive tiles at a time, one per row of RL lanes; rows of 4
+            // lanes up to 12 items per tile on average, of 8 up to 24, else of 16
+            auto rows = [&](auto rl) {
+                constexpr uint32_t NR = 64u / decltype(rl)::value;
+                for (uint32_t t = t0 + (blockIdx.x * (uint32_t)kWaves + w) * NR; t < t1; t += NR * W) {
+                    sink_tiles_rows<P, decltype(rl)::value>(nd, b, t - t0, t1 - t < NR ? t1 - t : NR, e.cols, e.C,
+                                                            &e.ctl[CTL_STATUS]);
+                    wtiles += t1 - t < NR ? t1 - t : NR;
+                }
+            };
+            const uint32_t per = t1 > t0 ? b.particles / (t1 - t0) : 0u;
+            if (per <= 12u) rows(std::integral_constant<uint32_t, 4u>{});
+            else if (per <= 24u) rows(std::integral_constant<uint32_t, 8u>{});
+            else rows(std::integral_constant<uint32_t, 16u>{});
         }
         if (e.prof && lane_id() == 0u && wtiles) {
             atomicAdd(&e.prof[PROF_KB_WAVE_TICKS], wall_clock64() - wprof);
