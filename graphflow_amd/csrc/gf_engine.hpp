@@ -732,7 +732,8 @@ __device__ inline void engine_retire(const Engine& e, SelLds& L)
 
 // Select of the next step. With lists = false the input lists, tile / item prefixes and work
 // prefixes are left to the grid list kernel (engine_lists_grid), which reads each lane's inputs
-// from its ring at in_slot.
+// from its ring at in_slot. The caller has loaded the lane table, the rings' head and tail and the
+// job table into L.
 __device__ inline void engine_select(const Engine& e, SelLds& L, bool lists = true)
 {
     const uint32_t tid = threadIdx.x, n = e.n_nodes, C = e.C, Q = e.Q;
@@ -744,7 +745,7 @@ __device__ inline void engine_select(const Engine& e, SelLds& L, bool lists = tr
         e.prof[slot] += now - pt;
         pt = now;
     };
-    load_lane_tab(e, L);
+    // the lane table (load_lane_tab) and the rings' head / tail are in LDS: every caller loads them
     __syncthreads();
     // stage the candidate ready descriptors of every transform's ring in LDS: <= Q per lane (the
     // first lane of a node stages for all its lanes). A sink takes all its ready chunks, which its
