@@ -14,7 +14,7 @@ Beside the headline it reports (each at the size its workload states):
   - the keyed hash pipeline at 1e8 keys, Q in {1, 4, 8, 16}, with in-flight batch lanes;
   - photon transport at 1e8 photons (team engine with P photons in flight; wide engine with 16M);
   - the served mixed stream of 4096 jobs (N log-uniform 1e3..1e7, 64 GB budget);
-  - the plain one-batch-at-a-time forms;
+  - the plain one-batch-at-a-time forms (Q=1 at 2e6 particles; the headline's N=1e8, Q=8);
 and, in the CPU leg, the oracle's restatement of the reference CPU path (runner schedule, one thread
 and all host threads) plus the CPU results every GPU form is checked against.
 """
@@ -581,6 +581,8 @@ def main():
     ap.add_argument("--live", type=int, default=16384)
     ap.add_argument("--batch", type=int, default=8)
     ap.add_argument("--plain-items", type=float, default=2e6)
+    ap.add_argument("--plain-full-items", type=float, default=1e8,
+                    help="the plain forms at the headline's Q (0 = skip)")
     ap.add_argument("--cpu-items", type=float, default=2e7, help="CPU baseline sample (oracle runner schedule)")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--only-headline", action="store_true", help="the headline runs and roofline only")
@@ -650,6 +652,23 @@ def main():
             dt, _ = run_once(pflow, psrc, pcounts, Np, torch)
             forms[name] = {"value": Np / dt, "unit": "particles/s", "items": Np}
             log(f"[bench] {name} N={Np}: {Np / dt:.4g} particles/s")
+            pflow.close()
+        # the headline's own configuration (N=1e8, Q=8) in the reference's protocol: the host runner,
+        # host-planned batches or one fused batch per partition batch; its histogram must be the
+        # headline's
+        Nf = int(args.plain_full_items)
+        for name, kw in (("plain_host_planned_Q8", {}), ("plain_fused_batches_Q8", {"fused_batches": True})):
+            if Nf <= 0:
+                break
+            pflow, psrc, pcounts = build_feedback(g, torch, C, P, Q, g.SCHEDULE_HOST, **kw)
+            dt, _ = run_once(pflow, psrc, pcounts, Nf, torch)
+            prow = g.hist_rows(pcounts.data_ptr(), g.FEEDBACK_HIST_N, g.FEEDBACK_HIST_LO)
+            psha = g.rows_sha256(prow)
+            forms[name] = {"value": Nf / dt, "unit": "particles/s", "items": Nf, "Q": Q, "histogram_sha256": psha}
+            if golden and Nf == golden["items"]:
+                forms[name]["histogram_equals_golden"] = psha == golden["cases"]["parity"]["sha256"]
+            log(f"[bench] {name} N={Nf}: {Nf / dt:.4g} particles/s; histogram equals golden: "
+                f"{forms[name].get('histogram_equals_golden')}")
             pflow.close()
         if not args.no_distributions:
             forms.update(distribution_forms(g, torch, N, C, P, Q, golden))
