@@ -267,7 +267,8 @@ def photon_forms(g, torch, N, Np, C, P, Q, wide_live=1 << 24, lanes=31, src_chun
                       dict(schedule=g.SCHEDULE_DEVICE, batch_chunks=16, fuse_transforms=True, device_engine=1,
                            max_workgroups=wg, tile_items=tile, inflight_batches=lanes, source_chunks=src_chunks,
                            num_chunks=Sw)))
-    forms.append(("photon_plain_host_planned_Q1", Np, P, dict(schedule=g.SCHEDULE_HOST, batch_chunks=1)))
+    if Np > 0:
+        forms.append(("photon_plain_host_planned_Q1", Np, P, dict(schedule=g.SCHEDULE_HOST, batch_chunks=1)))
     for name, n, live, kw in forms:
         tally = torch.zeros(cfg["tally_bins"], dtype=torch.float64, device="cuda")
         flow = g.Flow(g.PAYLOAD_PHOTON)

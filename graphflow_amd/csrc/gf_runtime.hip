@@ -99,6 +99,7 @@ uint32_t reserve_bound_of(const gf_node_desc& nd, uint32_t C, uint32_t Q)
 constexpr uint32_t kDenseBinLimit = 4096u;
 constexpr uint32_t kHashTileLimit = 1024u;
 constexpr uint32_t kKernelKinds = 8;  // kernel-time kinds of gf_flow_kernel_times  // hash path tiles: a tile's table of 2 * 1024 slots fits LDS
+constexpr uint32_t kXcds = 8;  // XCDs of an MI355X
 
 uint32_t next_pow2(uint32_t v)
 {
@@ -368,7 +369,20 @@ __global__ __launch_bounds__(kThreads) void k_engine_kb(Engine e)
             tprof = now;
         }
     };
-    for (uint32_t t = blockIdx.x; t < total; t += G) {
+    // Partition tiles by XCD: workgroups are dispatched to the 8 XCDs round robin (XCD =
+    // blockIdx.x % 8), and each XCD takes a contiguous eighth of the tiles, so the tiles of one batch
+    // (whose stores meet in the same output chunks) mostly share one L2, which merges their partial
+    // lines before writing them back. Any other launch: the tiles round robin over the workgroups.
+    uint32_t t_first = blockIdx.x, t_end = total, t_step = G;
+    if constexpr (Kinds == KB_PART) {
+        if (G % kXcds == 0u) {
+            const uint32_t x = blockIdx.x % kXcds;
+            t_first = (uint32_t)((uint64_t)total * x / kXcds) + blockIdx.x / kXcds;
+            t_end = (uint32_t)((uint64_t)total * (x + 1u) / kXcds);
+            t_step = G / kXcds;
+        }
+    }
+    for (uint32_t t = t_first; t < t_end; t += t_step) {
         const uint32_t i = lane_of(pre, nm, t), k = order[i];
         if (wsink[k]) {
             // skip the lane's tiles: the first tile of this workgroup's round-robin at or after them
