@@ -582,8 +582,8 @@ def main():
     ap.add_argument("--live", type=int, default=16384)
     ap.add_argument("--batch", type=int, default=8)
     ap.add_argument("--plain-items", type=float, default=2e6)
-    ap.add_argument("--plain-full-items", type=float, default=1e8,
-                    help="the plain forms at the headline's Q (0 = skip)")
+    ap.add_argument("--plain-full-items", type=float, default=None,
+                    help="the plain forms at the headline's Q (default 1e8, 0 with --skip-headline; 0 = skip)")
     ap.add_argument("--cpu-items", type=float, default=2e7, help="CPU baseline sample (oracle runner schedule)")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--only-headline", action="store_true", help="the headline runs and roofline only")
@@ -657,7 +657,7 @@ def main():
         # the headline's own configuration (N=1e8, Q=8) in the reference's protocol: the host runner,
         # host-planned batches or one fused batch per partition batch; its histogram must be the
         # headline's
-        Nf = int(args.plain_full_items)
+        Nf = int(args.plain_full_items if args.plain_full_items is not None else 0 if args.skip_headline else N)
         for name, kw in (("plain_host_planned_Q8", {}), ("plain_fused_batches_Q8", {"fused_batches": True})):
             if Nf <= 0:
                 break
