@@ -63,6 +63,7 @@ constexpr uint32_t kServeHashLive = 32768;  // keys in flight per served hash-pi
 constexpr uint32_t kServePhotonLive = 32768; // photons in flight per served photon job (fewer dependent steps)
 constexpr uint32_t kServeCap = 8192;        // chunk capacity
 constexpr uint32_t kServeBatch = 8;         // Q
+constexpr uint32_t kServeSourceChunks = 4;  // chunks a wide engine's source lane admits per step
 constexpr uint32_t kServeBuckets = 4096;
 constexpr uint32_t kServeTallyBins = 1024;
 constexpr uint32_t kServeSlots = 16;        // job slots per topology (default)
@@ -290,6 +291,7 @@ void build_serve_engine(gf_server& sv, ServeEngine& te, uint32_t topo, uint32_t 
     t.use_graph = 1;
     t.device_engine = team ? 0u : 1u;  // 0: the team engine when the union fits it
     t.inflight_batches = 1;
+    t.source_chunks = team ? 1u : kServeSourceChunks;  // the team engine admits one chunk per lane and step
     t.steps_per_launch = team ? kServeTeamSteps : kServeSteps;
     t.max_workgroups = team ? sv.cfg.team_size : 512u;
     t.team_xcd = team ? xcd : 0u;
