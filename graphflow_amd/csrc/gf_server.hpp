@@ -58,7 +58,7 @@ __global__ void k_zero_u64(unsigned long long* p, uint64_t n)
     for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (uint64_t)gridDim.x * blockDim.x) p[i] = 0ull;
 }
 
-constexpr uint32_t kServeLive = 16384;      // live bound of a served feedback job (the headline P)
+constexpr uint32_t kServeLive = 65536;      // live bound of a served feedback job
 constexpr uint32_t kServeHashLive = 32768;  // keys in flight per served hash-pipeline job
 constexpr uint32_t kServePhotonLive = 32768; // photons in flight per served photon job (fewer dependent steps)
 constexpr uint32_t kServeCap = 8192;        // chunk capacity
