@@ -1,4 +1,4 @@
-/* CPU oracle of graphFlow's transform-batch path — TEST INFRASTRUCTURE, the checker only.
+/* CPU oracle of graphFlow's transform-batch path — TEST INFRASTRUCTURE only: it checks results.
  *
  * Nothing in the product (graphflow_amd) links or loads this file. Only tests/, the smoke check
  * and the CPU-baseline leg of bench.py use it, always as the reference the GPU results are

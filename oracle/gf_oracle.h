@@ -1,6 +1,6 @@
 /* CPU oracle of graphFlow's transform-batch path.
  *
- * TEST INFRASTRUCTURE ONLY. This library is the checker: only tests/, the smoke check and the
+ * TEST INFRASTRUCTURE ONLY. This library checks results: only tests/, the smoke check and the
  * CPU-baseline leg of bench.py load it. The product library (graphflow_amd) never links,
  * loads or calls anything here, and nothing here is ever the thing being measured as GPU
  * throughput.
