@@ -1166,16 +1166,15 @@ struct gf_stage {
         }
     }
 
-    // The launch's batch: in the kernel arguments when the host runner launches a batch that fits,
-    // else uploaded (record, inputs, prefixes, the first n_out output descriptors, the first
-    // count_words planned offsets). The status word is never uploaded: cleared once, it stays zero
+    // The launch's batch: in the kernel arguments when it fits, else uploaded (record, inputs,
+    // prefixes, the first n_out output descriptors, the first count_words planned offsets). The status word is never uploaded: cleared once, it stays zero
     // until a kernel reports a failure, after which the stage takes no further batch. `uploaded`
     // tells whether the launch reads the current mirror.
     void stage_batch(hipStream_t st, uint32_t n_out, size_t count_words = 0)
     {
         uploaded = false;
         const uint32_t n = h_batch->n_in;
-        if (lazy && n <= kArgIn && n_out <= kArgOut && count_words <= kArgCounts) {
+        if (n <= kArgIn && n_out <= kArgOut && count_words <= kArgCounts) {
             ba.block = nullptr;
             ba.b = *h_batch;
             ba.inl = ARG_IN | (n_out ? ARG_OUT : 0u) | (count_words ? ARG_COUNTS : 0u);
@@ -3003,6 +3002,9 @@ gf_status gf_stage_tiles(gf_stage* stage, gf_tile_desc* tiles, uint32_t cap, uin
         if (!stage->n_tiles) return;
         gf_tile_desc* d = dalloc<gf_tile_desc>(stage->n_tiles);
         hipStream_t s = stage->pool->ctx->stream;
+        // a batch that travelled in its kernel arguments: its record, inputs and prefixes to the block
+        if (!stage->ba.block)
+            HIPX(hipMemcpyAsync(stage->d_blk, stage->h_blk, stage->o_out, hipMemcpyHostToDevice, s));
         hipLaunchKernelGGL(k_tiles, dim3(std::min<uint32_t>((stage->n_tiles + 255u) / 256u, 1024u)), dim3(256), 0, s,
                            stage->d_batch, stage->pool->C, d);
         HIPX(hipGetLastError());
