@@ -868,13 +868,13 @@ double zipf_fit_a(const std::vector<double>& cdf, double s)
 // One fused partition batch (gf_batch.hpp) on `st`: one workgroup of as many 256-thread tile groups
 // (at most 4) as the LDS holds.
 // One launch on one workgroup of G tile groups; with `tiles` > 0 (the batch's tile count, known on the
-// host) and more than two rounds of G tiles, the grid form instead (three launches, a tile per
+// host) and more than one round of G tiles, the grid form instead (three launches, a tile per
 // workgroup). Both run the same tile functions and plan: equal results.
 void launch_fused_kernel(uint32_t payload, uint32_t r, const FusedArgs& a, const Cols& cols, hipStream_t st,
                          uint32_t tiles = 0)
 {
     const uint32_t G = std::max<uint32_t>(1u, std::min<uint32_t>(4u, (uint32_t)((160u * 1024u) / a.group_lds)));
-    const bool grid = tiles > 2u * G && a.n <= kGridIn;
+    const bool grid = tiles > G && a.n <= kGridIn;
     with_payload(payload, r, [&](auto pt, auto rc) {
         using P = typename decltype(pt)::type;
         constexpr int RR = decltype(rc)::value;

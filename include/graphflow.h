@@ -208,7 +208,7 @@ gf_status gf_stage_abort(gf_stage* stage);
 /* One partition batch with no host round trip (no counterpart kernel in the reference, which plans on
  * the host between its classify and scatter kernels, flow/Execution.hpp:2363-2725): classify, device
  * plan, device output acquisition and stable scatter of a ragged list of n <= Q chunks, in one launch
- * on one workgroup, or for batches of more than two rounds of that workgroup's tiles (and n <= 256)
+ * on one workgroup, or for batches of more than one round of that workgroup's tiles (and n <= 256)
  * in three stream-ordered launches over the grid (count, plan, scatter), same result. The stage
  * reserves outputChunkBound(items, C, bins) chunks from its pool before the launch; the kernel
  * uses the first R in bin-major order. gf_stage_advance (one call, more_work = 0) returns the
