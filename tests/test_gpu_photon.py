@@ -87,3 +87,29 @@ def test_slab_variants(cfg):
     # sparse deep bins: besides 5e-3 relative, allow the weight of one history (at most 1) moved
     # to the neighbouring bin by fp32 rounding of its depth, twice
     assert (np.abs(t - ref) <= 5e-3 * ref + 2.0).all()
+
+
+@pytest.mark.parametrize("form", ["plain", "wide", "team"])
+def test_tally_copies_fold_once_per_run(form):
+    """A flow's runs add into private copies of the tally, folded into the caller's tally when the run
+    ends and cleared: a second run of the same flow adds the same tally again, and a run after the
+    caller cleared its tally gives the first run's tally (and the FP64 oracle's)."""
+    torch = torch_cuda()
+    N = 100_000
+    cfg = dict(g.PHOTON_DEFAULTS)
+    tally = torch.zeros(cfg["tally_bins"], dtype=torch.float64, device="cuda")
+    flow = g.Flow(g.PAYLOAD_PHOTON)
+    src, *_ = g.photon_network(flow, tally.data_ptr())
+    flow.configure(g.Tuning(seed=SEED, chunk_capacity=8192, live_particles=16384, photon=cfg, **FORMS[form]))
+    flow.run(src, N)
+    t1 = tally.cpu().numpy().copy()
+    flow.run(src, N)
+    t2 = tally.cpu().numpy().copy()
+    tally.zero_()
+    torch.cuda.synchronize()
+    flow.run(src, N)
+    t3 = tally.cpu().numpy()
+    ref, _ = fp64_reference(N, cfg)
+    assert (np.abs(t1 - ref) / ref).max() <= 5e-3
+    np.testing.assert_allclose(t2, 2.0 * t1, rtol=1e-10)
+    np.testing.assert_allclose(t3, t1, rtol=1e-10)
