@@ -383,7 +383,7 @@ def served_forms(g, torch, jobs, max_items, rate, slots=16):
     for i in sample:
         f, src = flows[int(topo[i])]
         f.configure(g.Tuning(seed=int(seeds[i]), batch_chunks=8, chunk_capacity=8192,
-                             live_particles=65536 if int(topo[i]) == 0 else 32768, schedule=g.SCHEDULE_DEVICE,
+                             live_particles=32768 if int(topo[i]) == 2 else 65536, schedule=g.SCHEDULE_DEVICE,
                              photon=dict(g.PHOTON_DEFAULTS)))
         f.run(src, min(int(items[i]), 16384))  # engine build (same pool shape) outside the timed run
         torch.cuda.synchronize()

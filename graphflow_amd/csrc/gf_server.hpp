@@ -59,11 +59,11 @@ __global__ void k_zero_u64(unsigned long long* p, uint64_t n)
 }
 
 constexpr uint32_t kServeLive = 65536;      // live bound of a served feedback job
-constexpr uint32_t kServeHashLive = 32768;  // keys in flight per served hash-pipeline job
+constexpr uint32_t kServeHashLive = 65536;  // keys in flight per served hash-pipeline job
 constexpr uint32_t kServePhotonLive = 32768; // photons in flight per served photon job (fewer dependent steps)
 constexpr uint32_t kServeCap = 8192;        // chunk capacity
 constexpr uint32_t kServeBatch = 8;         // Q
-constexpr uint32_t kServeSourceChunks = 4;  // chunks a wide engine's source lane admits per step
+constexpr uint32_t kServeSourceChunks = 8;  // chunks a wide engine's source lane admits per step
 constexpr uint32_t kServeBuckets = 4096;
 constexpr uint32_t kServeTallyBins = 1024;
 constexpr uint32_t kServeSlots = 16;        // job slots per topology (default)
